@@ -1,0 +1,239 @@
+"""Write the golden fixtures under tests/golden/ from the reference's own code.
+
+Runs only in a container that holds the reference sources: the driver
+oracle/ref_driver.f90 is linked against the reference's Fortran modules by
+oracle/ref.mk (``make -f oracle/ref.mk``).  The fixtures it writes are plain
+numbers (inputs and the reference's outputs) and are what the tests read; the
+reference itself never travels with them.
+
+    python tests/golden/make_golden.py            # all cases
+    python tests/golden/make_golden.py ev8_trc_fd # one case
+
+Inputs are seeded (numpy PCG64): harmonic-oscillator products (b = 1.7 fm)
+whose per-axis parity follows each spinor component's reflection sign, plus
+1e-3 uniform noise, and smooth Woods-Saxon mean fields (R = 4 fm, a = 0.6 fm).
+Meshes are deliberately non-cubic so that axis mix-ups show.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "mocca_ref_driver")
+RUNDIR = os.path.join(ROOT, "oracle", "_ref", "run")
+FORCES = "/root/reference/forces.param"
+OUTDIR = os.path.dirname(os.path.abspath(__file__))
+
+# Per-component reflection signs (EV8 rules, Derivatives.f90:348-439 and
+# OptimizedDerivatives.f90:533-542) for a state with parity P, signature S,
+# time simplex TS = +1.
+def comp_signs(P, S, TS=1):
+    sx = [S * TS, -S * TS, -S * TS, S * TS]
+    sy = [TS, -TS, TS, -TS]
+    sz = [S * P, S * P, -S * P, -S * P]
+    return sx, sy, sz
+
+
+def hermite(n, t):
+    h0, h1 = np.ones_like(t), 2 * t
+    if n == 0:
+        return h0
+    for k in range(1, n):
+        h0, h1 = h1, 2 * t * h1 - 2 * k * h0
+    return h1
+
+
+def mesh(nx, ny, nz, dx):
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dx
+    z = (np.arange(nz) + 0.5) * dx
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return X, Y, Z
+
+
+def seeded_states(rng, nx, ny, nz, dx, qns, b=1.7, noise=1e-3):
+    X, Y, Z = mesh(nx, ny, nz, dx)
+    env = np.exp(-(X**2 + Y**2 + Z**2) / (2 * b * b))
+    out = np.zeros((len(qns), 4, nz, ny, nx))
+    for i, (iso, P, S, TS) in enumerate(qns):
+        sx, sy, sz = comp_signs(P, S, TS)
+        for c in range(4):
+            f = np.zeros_like(X)
+            for _ in range(2):
+                ex = [n for n in range(4) if (1 if n % 2 == 0 else -1) == sx[c]]
+                ey = [n for n in range(4) if (1 if n % 2 == 0 else -1) == sy[c]]
+                ez = [n for n in range(4) if (1 if n % 2 == 0 else -1) == sz[c]]
+                a = rng.uniform(-1, 1)
+                f += a * hermite(rng.choice(ex), X / b) * hermite(rng.choice(ey), Y / b) \
+                    * hermite(rng.choice(ez), Z / b)
+            out[i, c] = f * env + noise * rng.uniform(-1, 1, size=f.shape)
+    return out
+
+
+def synthetic_fields(rng, nx, ny, nz, dx, R=4.0, a=0.6):
+    X, Y, Z = mesh(nx, ny, nz, dx)
+    r = np.sqrt(X**2 + Y**2 + Z**2)
+    f = 1.0 / (1.0 + np.exp((r - R) / a))
+    df = -f * (1 - f) / a                      # d f / d r
+    grad = [df * X / r, df * Y / r, df * Z / r]
+    eps = np.zeros((3, 3, 3))
+    eps[0, 1, 2] = eps[1, 2, 0] = eps[2, 0, 1] = 1
+    eps[0, 2, 1] = eps[2, 1, 0] = eps[1, 0, 2] = -1
+    sh = (nz, ny, nx)
+    U = np.stack([-50 * f, -45 * f]) + 0.1 * rng.uniform(-1, 1, (2,) + sh)
+    B = np.stack([10 * f, 11 * f])
+    NablaB = np.stack([[10 * g for g in grad], [11 * g for g in grad]])
+    w = 60 * 0.16
+    W = np.zeros((2, 3, 3) + sh)               # [it][nu][mu]
+    for it in range(2):
+        for mu in range(3):
+            for nu in range(3):
+                W[it, nu, mu] = -sum(eps[k, mu, nu] * w * (1 + 0.1 * it) * grad[k]
+                                     for k in range(3))
+    DivW = np.stack([np.stack(grad) * 3.0, np.stack(grad) * 2.5]) + \
+        0.05 * rng.uniform(-1, 1, (2, 3) + sh)
+    S = np.zeros((2, 3) + sh)
+    S[:, 2] = 2 * f
+    S[:, 0] = 0.3 * f * X
+    S[:, 1] = -0.2 * f * Y
+    A = np.stack([np.stack(grad) * 0.7, np.stack(grad) * -0.4])
+    DivA = np.stack([0.5 * f, -0.3 * f])
+    return dict(U=U, B=B, NablaB=NablaB, W=W, DivW=DivW, S=S, A=A, DivA=DivA)
+
+
+CASES = {
+    # name: (nx, ny, nz, dx, trc, lagrange)
+    "ev8_trc_fd": (8, 7, 6, 0.8, True, False),
+    "ev8_trc_lag": (8, 7, 6, 0.8, True, True),
+    "ev8_todd_fd": (7, 8, 6, 0.75, False, False),
+    "ev8_todd_lag": (7, 6, 8, 0.75, False, True),
+}
+
+
+def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom):
+    fd = "-1" if lag else "3"
+    fdl = "-1" if lag else "4"
+    return f"""&GenInfo
+TimeReversal={1 if trc else 0}, neutrons={neutrons}, protons={protons}
+nx={nx}, ny={ny}, nz={nz}, dx={dx}
+MaxIter=2, ParameterEstimation=0, dt={dt}, momentum={mom}
+/
+&SpwfStorage
+nwt={nwt}
+/
+&Densit
+MixingScheme=0
+/
+&Pairing
+Type="HF"
+/
+&Derivatives
+MaxFDOrder={fd}, MaxFDLapOrder={fdl}, BStack=.false.
+/
+&MomentParam
+/
+&InAndOutput
+/
+&Force
+afor="SLy4"
+/
+&Coulomb
+/
+&Cranking
+/
+"""
+
+
+def run_case(name):
+    nx, ny, nz, dx, trc, lag = CASES[name]
+    seed = sum(map(ord, name))
+    rng = np.random.default_rng(seed)
+    if trc:
+        qns = [(-1, 1, 1, 1), (-1, -1, 1, 1), (-1, 1, 1, 1), (-1, -1, 1, 1),
+               (1, 1, 1, 1), (1, -1, 1, 1), (1, 1, 1, 1), (1, -1, 1, 1)]
+        occ = [2.0, 2.0, 1.3, 0.4, 2.0, 1.7, 0.6, 0.0]
+        neutrons = protons = 4
+    else:
+        qns = [(-1, 1, 1, 1), (-1, 1, -1, 1), (-1, -1, 1, 1), (-1, 1, 1, 1),
+               (1, 1, 1, 1), (1, 1, -1, 1), (1, -1, -1, 1), (1, 1, 1, 1)]
+        occ = [1.0, 1.0, 0.8, 0.3, 1.0, 0.9, 0.5, 0.2]
+        neutrons = protons = 3
+    nwt = len(qns)
+    psi = seeded_states(rng, nx, ny, nz, dx, qns)
+    fields = synthetic_fields(rng, nx, ny, nz, dx)
+    dt, mom = 0.015, 0.2
+
+    wd = os.path.join(RUNDIR, name)
+    os.makedirs(wd, exist_ok=True)
+    if not os.path.exists(os.path.join(wd, "forces.param")):
+        os.symlink(FORCES, os.path.join(wd, "forces.param"))
+    with open(os.path.join(wd, "input.bin"), "wb") as fh:
+        np.array([nwt], "<i4").tofile(fh)
+        np.array(qns, "<i4").tofile(fh)
+        np.array(occ, "<f8").tofile(fh)
+        psi.astype("<f8").tofile(fh)
+        for k in ("U", "B", "NablaB", "W", "DivW", "S", "A", "DivA"):
+            fields[k].astype("<f8").tofile(fh)
+    nml = namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom)
+    with open(os.path.join(wd, "input.nml"), "w") as fh:
+        fh.write(nml)
+    with open(os.path.join(wd, "input.nml")) as fin:
+        subprocess.run([DRIVER, "input.bin", "out", "force,derive,hpsi,dens,step"],
+                       cwd=wd, stdin=fin, check=True, stdout=subprocess.DEVNULL)
+
+    def rd(fn, count=-1, offset=0):
+        return np.fromfile(os.path.join(wd, fn), "<f8", count=count, offset=offset)
+
+    sp = (4, nz, ny, nx)
+    n = nx * ny * nz
+    d = rd("out_derive.bin").reshape(nwt, 4, *sp)
+    h = rd("out_hpsi.bin").reshape(nwt, *sp)
+    dens = rd("out_dens.bin")
+    k = 0
+
+    def take(shape):
+        nonlocal k
+        cnt = int(np.prod(shape))
+        v = dens[k:k + cnt].reshape(shape)
+        k += cnt
+        return v
+    res = dict(rho=take((2, nz, ny, nx)), tau=take((2, nz, ny, nx)),
+               nablaJ=take((2, nz, ny, nx)), derrho=take((2, 3, nz, ny, nx)),
+               laprho=take((2, nz, ny, nx)))
+    if not trc:
+        res.update(vecj=take((2, 3, nz, ny, nx)), vecs=take((2, 3, nz, ny, nx)),
+                   rotvecj=take((2, 3, nz, ny, nx)), rots=take((2, 3, nz, ny, nx)),
+                   ders=take((2, 3, 3, nz, ny, nx)))
+    assert k == dens.size
+    st = rd("out_step.bin")
+    per = nwt * 4 * n + 2 * nwt
+    steps = []
+    for it in range(2):
+        blk = st[it * per:(it + 1) * per]
+        steps.append((blk[:nwt * 4 * n].reshape(nwt, *sp),
+                      blk[nwt * 4 * n:].reshape(nwt, 2)))
+    fb = open(os.path.join(wd, "out_force.bin"), "rb").read()
+    coef = np.frombuffer(fb[:28 * 8], "<f8")
+    rest = np.frombuffer(fb[28 * 8:37 * 8], "<f8")
+    com1 = int(np.frombuffer(fb[37 * 8:37 * 8 + 4], "<i4")[0])
+    meta = dict(nx=nx, ny=ny, nz=nz, dx=dx, trc=trc, lagrange=lag, nwt=nwt,
+                dt=dt, momentum=mom, neutrons=neutrons, protons=protons,
+                com1body=com1, seed=seed,
+                source="reference MOCCa modules via oracle/ref_driver.f90 "
+                       "(amdflang -O2)")
+    np.savez_compressed(
+        os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
+        qn=np.array(qns, np.int32), occ=np.array(occ), psi=psi, **fields,
+        der=d[:, :3], lap=d[:, 3], hpsi=h, **res,
+        step1_psi=steps[0][0], step1_ed=steps[0][1],
+        step2_psi=steps[1][0], step2_ed=steps[1][1],
+        coef=coef, consts=rest)
+    print(name, "written")
+
+
+if __name__ == "__main__":
+    for c in (sys.argv[1:] or CASES):
+        run_case(c)
