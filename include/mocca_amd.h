@@ -1,0 +1,143 @@
+/*
+ * mocca_amd -- MI355X (gfx950) library for the imaginary-time iteration hot
+ * path of the MOCCa Skyrme-HF(B) mesh solver.  C ABI only: plain pointers,
+ * sizes and status codes, so Fortran (iso_c_binding), C and Python (ctypes)
+ * bind it directly.
+ *
+ * Each entry point replaces one module procedure of the reference (cited as
+ * path:line in the reference tree).  A solve handle owns all device memory:
+ * the states of HFBasis, the heavy-ball memory, mean-field potentials and
+ * densities.  Host arrays stay owned by the caller.
+ *
+ * Layouts (x fastest, identical to the reference's Fortran arrays):
+ *   field     f[(z*ny + y)*nx + x],  N = nx*ny*nz stored points
+ *   spinor    grid[c][N], c = 0..3 = (Re up, Im up, Re down, Im down)
+ *             (Spinor.f90:6-19: Grid(nx,ny,nz,4,1))
+ *   states    grid[i][c][N] for i = first .. first+count-1
+ *   qn        qn[i][4] = (isospin -1/+1, parity, signature, time simplex)
+ *   U, B      [2][N]        (isospin 0 = neutron, 1 = proton)
+ *   NablaB    [2][3][N]     DivW [2][3][N]    S, A [2][3][N]   DivA [2][N]
+ *   W         [2][nu][mu][N]  (WPot(:,:,:,mu,nu,it), MeanFields.f90:80-95)
+ *
+ * Errors: every call returns 0 on success or a negative MOCCA_ERR_* code;
+ * mocca_last_error() then describes it.  The reference aborts through stp
+ * (GenInfo.f90:91-177); a Fortran shim maps a non-zero status to
+ * stp(mocca_last_error(h)).  No call returns partial results silently.
+ *
+ * Threading: a handle is not thread-safe; its work is ordered on one HIP
+ * stream that the handle creates.  Calls that return host data synchronise.
+ */
+#ifndef MOCCA_AMD_H
+#define MOCCA_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOCCA_OK 0
+#define MOCCA_ERR_ARG -1      /* invalid argument or size */
+#define MOCCA_ERR_UNSUPPORTED -2
+#define MOCCA_ERR_HIP -3      /* HIP runtime error (message has details) */
+#define MOCCA_ERR_STATE -4    /* call order: something was not set yet */
+#define MOCCA_ERR_NUMERIC -5  /* non-positive norm in Gram-Schmidt, NaN */
+
+#define MOCCA_FP64 0          /* everything in FP64 */
+#define MOCCA_MIXED 1         /* FP32 states/stencils/hPsi/densities,
+                                 FP64 Gram-Schmidt and energy sums */
+
+typedef struct mocca_solve mocca_solve;
+
+/* Mesh and symmetry set-up: GenInfo (nx, ny, nz, dx, dv after
+ * AssignSymmetries, GenInfo.f90:220-297), the derivative scheme of
+ * AssignFDCoefs (Derivatives.f90:206-346): lagrange = 1 for
+ * MaxFDOrder = MaxFDLapOrder = -1 (Opt_LAG*_EV8, Lapla_LAG_EV8), 0 for the
+ * EV8 finite differences (MaxFDOrder = 3, MaxFDLapOrder = 4).  Parity,
+ * signature and time simplex must be conserved (the EV8 octant). */
+typedef struct mocca_mesh {
+    int nx, ny, nz;
+    double dx;
+    double dv;
+    int time_reversal;   /* 1 = TRC conserved (no time-odd fields) */
+    int lagrange;
+} mocca_mesh;
+
+/* Mean-field potentials of one iteration (MeanFields.f90:80-95), host
+ * pointers in the layouts above.  S, A, DivA are read only when
+ * time_reversal == 0.  kin[it] = hbm(it)/2 * reduced-mass factor(it)
+ * (ActionOfBNew, MeanFields.f90:826-838). */
+typedef struct mocca_fields {
+    const double *U, *B, *NablaB, *W, *DivW, *S, *A, *DivA;
+    double kin[2];
+} mocca_fields;
+
+/* Densities (DensityVector, Densities.f90:39-94), host pointers; any may be
+ * NULL.  rho, tau, nablaJ, laprho [2][N]; derrho, vecj, vecs, rotvecj,
+ * rots [2][3][N]; ders [2][3 comp][3 dir][N].  Time-odd members are only
+ * written when time_reversal == 0. */
+typedef struct mocca_density {
+    double *rho, *tau, *nablaJ, *derrho, *laprho;
+    double *vecj, *vecs, *rotvecj, *rots, *ders;
+} mocca_density;
+
+/* Create / destroy a solve on the current HIP device.  nwt = number of
+ * states (SpwfStorage nwt).  No reference counterpart beyond the
+ * allocations of ReadSpwfStorageInfo/NewSpinor (SpwfStorage.f90:77-97). */
+int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
+                       mocca_solve **out);
+int mocca_solve_destroy(mocca_solve *h);
+const char *mocca_last_error(const mocca_solve *h);
+int mocca_synchronize(mocca_solve *h);
+
+/* Quantum numbers and occupations of HFBasis(i) (Spwf.f90:14-194,
+ * SetOcc :624).  Must be called before any compute call. */
+int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ);
+int mocca_set_occupations(mocca_solve *h, const double *occ);
+/* HFBasis(i)%Energy as used by the Gram-Schmidt ordering (SetEnergy). */
+int mocca_set_energies(mocca_solve *h, const double *energy);
+int mocca_get_energies(mocca_solve *h, double *energy, double *disp);
+
+/* HFBasis(first+k)%Value%Grid <-> host (SetGrid / GetValue). */
+int mocca_upload_states(mocca_solve *h, int first, int count,
+                        const double *grid);
+int mocca_download_states(mocca_solve *h, int first, int count, double *grid);
+/* Heavy-ball memory `updates` of GradDesc (ImaginaryTime.f90:24,456-461). */
+int mocca_reset_updates(mocca_solve *h);
+
+int mocca_set_potentials(mocca_solve *h, const mocca_fields *f);
+
+/* DeriveAll (SpwfStorage.f90:586-604) -> CompDer of every state: first
+ * derivatives and Laplacian kept on the device; download one state's
+ * der[3][4][N] and lap[4][N]. */
+int mocca_derive_all(mocca_solve *h);
+int mocca_download_derivatives(mocca_solve *h, int state, double *der,
+                               double *lap);
+
+/* hPsi(HFBasis(state)) (ImaginaryTime.f90:213-304) into host out[4][N]. */
+int mocca_hpsi(mocca_solve *h, int state, double *out);
+
+/* GradDesc (ImaginaryTime.f90:439-500) with given dt/hbar and momentum:
+ * hPsi of every state, energies and dispersions, heavy-ball step, then
+ * Gramschmidt and the update of the heavy-ball memory.  Derivatives are
+ * formed on the fly from the current states. */
+int mocca_grad_desc(mocca_solve *h, double dt_over_hbar, double momentum);
+
+/* Gramschmidt (SpwfStorage.f90:372-480): ascending-energy orthonormalisation
+ * per (isospin, parity, signature) block, done as Cholesky QR in FP64. */
+int mocca_gram_schmidt(mocca_solve *h);
+
+/* ComputeDensity(Density, .false.) (Densities.f90:431-1187, SLy4-type
+ * functional: no tensor or N2LO densities). */
+int mocca_compute_density(mocca_solve *h, mocca_density *out);
+
+/* Kernel timing over the handle's stream with HIP events (for benchmarks):
+ * enable = 1 starts recording each launch; mocca_profile_get returns the
+ * summed milliseconds and launch count for a kernel family name
+ * ("hpsi_step", "gram", "cholesky", "gs_apply", "density",
+ * "density_reduce", "field_jobs", "derive_all", "reduce_ed"). */
+int mocca_profile_enable(mocca_solve *h, int enable);
+int mocca_profile_get(mocca_solve *h, const char *name, double *ms,
+                      long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""Python view of the mocca_amd C ABI (include/mocca_amd.h).
+
+The HIP library ``libmocca_amd.so`` is built in-tree by ``mocca_amd.build``.
+Nothing here computes: every call goes to the HIP library, and importing
+the bindings without the built library raises at once (no CPU fallback).
+
+The class :class:`Solve` mirrors the reference's module procedures for the
+imaginary-time path (SpwfStorage, ImaginaryTime, Densities), one method per
+C entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmocca_amd.so")
+
+FP64 = 0
+MIXED = 1
+
+# Every symbol declared in include/mocca_amd.h.
+EXPORTS = (
+    "mocca_solve_create", "mocca_solve_destroy", "mocca_last_error",
+    "mocca_synchronize", "mocca_set_states_meta", "mocca_set_occupations",
+    "mocca_set_energies", "mocca_get_energies", "mocca_upload_states",
+    "mocca_download_states", "mocca_reset_updates", "mocca_set_potentials",
+    "mocca_derive_all", "mocca_download_derivatives", "mocca_hpsi",
+    "mocca_grad_desc", "mocca_gram_schmidt", "mocca_compute_density",
+    "mocca_profile_enable", "mocca_profile_get",
+)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class MeshC(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dv", ctypes.c_double),
+                ("time_reversal", ctypes.c_int), ("lagrange", ctypes.c_int)]
+
+
+class FieldsC(ctypes.Structure):
+    _fields_ = [(k, _dp) for k in ("U", "B", "NablaB", "W", "DivW", "S", "A", "DivA")] + \
+        [("kin", ctypes.c_double * 2)]
+
+
+class DensityC(ctypes.Structure):
+    _fields_ = [(k, _dp) for k in ("rho", "tau", "nablaJ", "derrho", "laprho",
+                                   "vecj", "vecs", "rotvecj", "rots", "ders")]
+
+
+class MoccaError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise MoccaError(f"{LIBPATH} is missing: build it with `python -m mocca_amd.build`")
+    lib = ctypes.CDLL(LIBPATH)
+    lib.mocca_last_error.restype = ctypes.c_char_p
+    lib.mocca_last_error.argtypes = [ctypes.c_void_p]
+    lib.mocca_solve_create.argtypes = [ctypes.POINTER(MeshC), ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+    for name in EXPORTS:
+        f = getattr(lib, name)
+        if name not in ("mocca_last_error", "mocca_solve_create"):
+            f.restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solve:
+    """A device-resident set of nwt states on one mesh (one HIP stream)."""
+
+    def __init__(self, nx, ny, nz, dx, dv, nwt, time_reversal=True, lagrange=False,
+                 precision=FP64):
+        self.lib = load()
+        self.nx, self.ny, self.nz, self.N, self.M = nx, ny, nz, nx * ny * nz, nwt
+        self.trc = bool(time_reversal)
+        mesh = MeshC(nx, ny, nz, dx, dv, int(bool(time_reversal)), int(bool(lagrange)))
+        h = ctypes.c_void_p()
+        self._chk(self.lib.mocca_solve_create(ctypes.byref(mesh), nwt, precision,
+                                              ctypes.byref(h)), None)
+        self.h = h
+        self._keep = []
+
+    def _chk(self, rc, h=-1):
+        if rc != 0:
+            msg = self.lib.mocca_last_error(self.h if h == -1 else h)
+            raise MoccaError(f"mocca error {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mocca_solve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # SpwfStorage --------------------------------------------------------
+    def set_states_meta(self, qn, occ):
+        qn = np.ascontiguousarray(qn, dtype=np.int32).reshape(self.M, 4)
+        occ = _f64(occ)
+        self._chk(self.lib.mocca_set_states_meta(self.h, _p(qn), _p(occ)))
+
+    def set_occupations(self, occ):
+        self._chk(self.lib.mocca_set_occupations(self.h, _p(_f64(occ))))
+
+    def set_energies(self, e):
+        self._chk(self.lib.mocca_set_energies(self.h, _p(_f64(e))))
+
+    def energies(self):
+        e, d = np.zeros(self.M), np.zeros(self.M)
+        self._chk(self.lib.mocca_get_energies(self.h, _p(e), _p(d)))
+        return e, d
+
+    def upload_states(self, psi, first=0):
+        psi = _f64(psi).reshape(-1, 4 * self.N)
+        self._chk(self.lib.mocca_upload_states(self.h, first, psi.shape[0], _p(psi)))
+
+    def download_states(self, first=0, count=None):
+        count = self.M - first if count is None else count
+        out = np.zeros((count, 4, self.N))
+        self._chk(self.lib.mocca_download_states(self.h, first, count, _p(out)))
+        return out
+
+    def reset_updates(self):
+        self._chk(self.lib.mocca_reset_updates(self.h))
+
+    def derive_all(self):
+        self._chk(self.lib.mocca_derive_all(self.h))
+
+    def derivatives(self, state):
+        der, lap = np.zeros((3, 4, self.N)), np.zeros((4, self.N))
+        self._chk(self.lib.mocca_download_derivatives(self.h, state, _p(der), _p(lap)))
+        return der, lap
+
+    def gram_schmidt(self):
+        self._chk(self.lib.mocca_gram_schmidt(self.h))
+
+    # MeanFields / ImaginaryTime -------------------------------------------
+    def set_potentials(self, U, B, NablaB, W, DivW, kin, S=None, A=None, DivA=None):
+        arrs = [_f64(x) for x in (U, B, NablaB, W, DivW, S, A, DivA)]
+        self._keep = arrs
+        f = FieldsC(*[_p(a) if a is not None else None for a in arrs],
+                    (ctypes.c_double * 2)(*kin))
+        self._chk(self.lib.mocca_set_potentials(self.h, ctypes.byref(f)))
+
+    def hpsi(self, state):
+        out = np.zeros((4, self.N))
+        self._chk(self.lib.mocca_hpsi(self.h, state, _p(out)))
+        return out
+
+    def grad_desc(self, dt_over_hbar, momentum):
+        self._chk(self.lib.mocca_grad_desc(self.h, ctypes.c_double(dt_over_hbar),
+                                           ctypes.c_double(momentum)))
+
+    # Densities -----------------------------------------------------------
+    def compute_density(self, fetch=True):
+        N = self.N
+        if not fetch:
+            self._chk(self.lib.mocca_compute_density(self.h, None))
+            return None
+        out = dict(rho=np.zeros((2, N)), tau=np.zeros((2, N)), nablaJ=np.zeros((2, N)),
+                   derrho=np.zeros((2, 3, N)), laprho=np.zeros((2, N)))
+        if not self.trc:
+            out.update(vecj=np.zeros((2, 3, N)), vecs=np.zeros((2, 3, N)),
+                       rotvecj=np.zeros((2, 3, N)), rots=np.zeros((2, 3, N)),
+                       ders=np.zeros((2, 3, 3, N)))
+        keys = ("rho", "tau", "nablaJ", "derrho", "laprho", "vecj", "vecs", "rotvecj",
+                "rots", "ders")
+        d = DensityC(*[_p(out[k]) if k in out else None for k in keys])
+        self._chk(self.lib.mocca_compute_density(self.h, ctypes.byref(d)))
+        return out
+
+    def synchronize(self):
+        self._chk(self.lib.mocca_synchronize(self.h))
+
+    def profile_enable(self, on=True):
+        self._chk(self.lib.mocca_profile_enable(self.h, int(on)))
+
+    def profile_get(self, name):
+        ms, cnt = ctypes.c_double(), ctypes.c_long()
+        self._chk(self.lib.mocca_profile_get(self.h, name.encode(), ctypes.byref(ms),
+                                             ctypes.byref(cnt)))
+        return ms.value, cnt.value

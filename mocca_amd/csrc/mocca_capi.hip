@@ -1,0 +1,866 @@
+// C ABI of the mocca_amd library: solve handles, device memory, launches.
+// The entry points and their reference counterparts are documented in
+// include/mocca_amd.h.
+#include "../../include/mocca_amd.h"
+#include "mocca_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace mocca;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Pending {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct Buffers {
+    void *psi = nullptr, *psi2 = nullptr, *upd = nullptr, *hbuf = nullptr;
+    void *fields = nullptr;        // T [48][N]
+    void *dpart = nullptr;         // T [nchunks][9][N]
+    void *lagT = nullptr;          // T lagrange matrices
+    double *lagD = nullptr;        // double lagrange matrices
+    double *der = nullptr, *lap = nullptr;
+    double *dens = nullptr;        // [2][9][N]
+    double *derived = nullptr;     // derrho[2][3] laprho[2] rotvecj[2][3] rots[2][3] ders[2][3][3]
+    int *qn = nullptr;
+    double *occ = nullptr, *E = nullptr, *D = nullptr;
+    double *eparts = nullptr;
+    int *order = nullptr;
+    Chunk *chunks = nullptr;
+    int *all_states = nullptr;
+    int *blk_states = nullptr, *blk_m = nullptr, *perm = nullptr;
+    double *G = nullptr, *Tm = nullptr, *work = nullptr, *gpart = nullptr;
+    int *status = nullptr;
+    FieldJob *jobs = nullptr;
+    Chunk *one_chunk = nullptr;
+    int *one_order = nullptr;
+};
+
+}  // namespace
+
+struct mocca_solve {
+    mocca_mesh mesh{};
+    int M = 0, N = 0, prec = MOCCA_FP64, tsize = 8;
+    hipStream_t st = nullptr;
+    std::string err;
+    Buffers b;
+    MeshK<double> md{};
+    MeshK<float> mf{};
+    bool meta_set = false, fields_set = false;
+    double kin[2] = {0, 0};
+    std::vector<int> qn;
+    std::vector<double> occ;
+    int nchunks = 0, ntiles = 0;
+    int nblk = 0, max_m = 0, nsplit = 1;
+    int njobs = 0;
+    // profiling
+    bool prof = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, long>> acc;
+};
+
+namespace {
+
+int fail(mocca_solve *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                           \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return fail(h, MOCCA_ERR_HIP, "%s failed: %s", #call,                 \
+                        hipGetErrorString(e_));                                   \
+    } while (0)
+
+template <typename P>
+int dalloc(mocca_solve *h, P **p, size_t bytes) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes ? bytes : 8);
+    if (e != hipSuccess)
+        return fail(h, MOCCA_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes,
+                    hipGetErrorString(e));
+    *p = static_cast<P *>(q);
+    return 0;
+}
+
+hipEvent_t take_event(mocca_solve *h) {
+    if (!h->pool.empty()) {
+        hipEvent_t e = h->pool.back();
+        h->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+template <typename F>
+int run(mocca_solve *h, const char *name, F &&f) {
+    if (h->prof) {
+        hipEvent_t a = take_event(h), b = take_event(h);
+        (void)hipEventRecord(a, h->st);
+        f();
+        (void)hipEventRecord(b, h->st);
+        h->pending.push_back({name, a, b});
+    } else {
+        f();
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(h, MOCCA_ERR_HIP, "launch of %s failed: %s", name,
+                    hipGetErrorString(e));
+    return 0;
+}
+
+int drain_profile(mocca_solve *h) {
+    if (h->pending.empty()) return 0;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    for (auto &p : h->pending) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        auto &slot = h->acc[p.name];
+        slot.first += ms;
+        slot.second += 1;
+        h->pool.push_back(p.a);
+        h->pool.push_back(p.b);
+    }
+    h->pending.clear();
+    return 0;
+}
+
+// Lagrange-mesh matrices on a line of n stored points whose full line has
+// nfull points (IniLag + reduceLag, Derivatives.f90:899-1078): the
+// Baye-Heenen first-derivative coefficients pi/(N dx) (-1)^k / sin(pi k/N)
+// and second-derivative coefficients, folded onto the stored half line for
+// even (+) and odd (-) reflection.  Output order: first(-), first(+),
+// second(-), second(+), each n x n row-major.
+void lagrange_axis(int n, int nfull, double dx, double *out) {
+    const double pi = 4.0 * std::atan2(1.0, 1.0);
+    std::vector<double> c(2 * nfull + 2, 0.0), l(2 * nfull + 2, 0.0);
+    const double f = pi / nfull, df = f / dx;
+    const double diag = -pi * pi / (3.0 * dx * dx) * (1.0 - 1.0 / double(nfull * nfull));
+    for (int k = 1; k < nfull; ++k) {
+        const double s = std::sin(f * k), co = std::cos(f * k);
+        const double sg = (k & 1) ? -1.0 : 1.0;
+        c[k] = df * sg / s;
+        l[k] = -sg * 2.0 * df * df * co / (s * s);
+    }
+    double *f1m = out, *f1p = out + n * n, *f2m = out + 2 * n * n, *f2p = out + 3 * n * n;
+    for (int i = 1; i <= n; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const int r = (i - 1) * n + (j - 1);
+            const int a = std::abs(i - j), e = i + j - 1;
+            if (i == j) {
+                f1m[r] = -c[e];
+                f1p[r] = c[e];
+                f2m[r] = diag - l[e];
+                f2p[r] = diag + l[e];
+            } else {
+                const double ca = (j < i) ? c[a] : -c[a];
+                f1p[r] = ca + c[e];
+                f1m[r] = ca - c[e];
+                f2p[r] = l[a] + l[e];
+                f2m[r] = l[a] - l[e];
+            }
+        }
+}
+
+template <typename T>
+void fill_mesh(mocca_solve *h, MeshK<T> &m, const T *lag) {
+    const mocca_mesh &g = h->mesh;
+    m.nx = g.nx; m.ny = g.ny; m.nz = g.nz; m.N = h->N; m.lag = g.lagrange;
+    m.inv60dx = T(1.0 / (60.0 * g.dx));
+    const double xf = -8064.0;
+    double xd = 5040.0 / xf * (g.dx * g.dx);
+    xd = 1.0 / xd;
+    m.l2m = T(43050.0 / (xf * 3));
+    m.l2c2 = T(1008.0 / xf);
+    m.l2c3 = T(-128.0 / xf);
+    m.l2c4 = T(9.0 / xf);
+    m.l2scale = T(-xd);
+    const int ns[3] = {g.nx, g.ny, g.nz};
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        const size_t nn = (size_t)ns[a] * ns[a];
+        m.lag1[a][0] = lag + off;
+        m.lag1[a][1] = lag + off + nn;
+        m.lag2[a][0] = lag + off + 2 * nn;
+        m.lag2[a][1] = lag + off + 3 * nn;
+        off += 4 * nn;
+    }
+}
+
+int sign_first_h(int axis, int P, int S, int TS, int c, int lag) {
+    const int cx = (c == 0 || c == 3) ? 1 : -1;
+    const int cy = (c == 0 || c == 2) ? 1 : -1;
+    const int cz = (c < 2) ? 1 : -1;
+    if (axis == 0) return cx * (lag ? S * TS : S);
+    if (axis == 1) return cy * TS;
+    return cz * S * P;
+}
+
+int sign_lap_h(int axis, int P, int S, int TS, int c) {
+    const int cx = (c == 0 || c == 3) ? 1 : -1;
+    const int cy = (c == 0 || c == 2) ? 1 : -1;
+    const int cz = (c < 2) ? 1 : -1;
+    if (axis == 0) return cx * S;
+    if (axis == 1) return cy * TS;
+    return cz * S * P;
+}
+
+// Jobs for the derived density fields (Densities.f90:541-550,1003-1101).
+// Field "quantum numbers" (P, S, TS, component) follow the reference calls.
+void build_density_jobs(mocca_solve *h, std::vector<FieldJob> &jobs) {
+    const long N = h->N;
+    const int lag = h->mesh.lagrange, trc = h->mesh.time_reversal;
+    const int nf = 9;
+    double *dens = h->b.dens;
+    double *derrho = h->b.derived, *laprho = derrho + 6 * N, *rotj = laprho + 2 * N,
+           *rots = rotj + 6 * N, *ders = rots + 6 * N;
+    const int P = 1, S = 1, TS = 1;
+    auto first = [&](FieldJob &j, int t, const double *src, int axis, int p, int s,
+                     int ts, int comp, double coef) {
+        j.src[t] = src;
+        j.axis[t] = axis;
+        j.sign[t][0] = j.sign[t][1] = j.sign[t][2] = sign_first_h(axis, p, s, ts, comp - 1, lag);
+        j.coef[t] = coef;
+        j.nterms = t + 1;
+    };
+    for (int it = 0; it < 2; ++it) {
+        const double *rho = dens + ((long)it * nf + 0) * N;
+        for (int a = 0; a < 3; ++a) {
+            FieldJob j{};
+            first(j, 0, rho, a, P, S, TS, 1, 1.0);
+            j.dst = derrho + (it * 3 + a) * N;
+            jobs.push_back(j);
+        }
+        FieldJob j{};
+        j.src[0] = rho;
+        j.axis[0] = 3;
+        for (int a = 0; a < 3; ++a) j.sign[0][a] = sign_lap_h(a, P, S, TS, 0);
+        j.coef[0] = 1.0;
+        j.nterms = 1;
+        j.dst = laprho + it * N;
+        jobs.push_back(j);
+        if (trc) continue;
+        const double *jx = dens + ((long)it * nf + 3) * N, *jy = jx + N, *jz = jy + N;
+        const double *sx = dens + ((long)it * nf + 6) * N, *sy = sx + N, *sz = sy + N;
+        // rot j: (Densities.f90:1008-1025)
+        FieldJob r{};
+        first(r, 0, jz, 1, -P, S, TS, 2, 1.0);
+        first(r, 1, jy, 2, -P, -S, TS, 1, -1.0);
+        r.dst = rotj + (it * 3 + 0) * N;
+        jobs.push_back(r);
+        r = FieldJob{};
+        first(r, 0, jx, 2, -P, -S, TS, 2, 1.0);
+        first(r, 1, jz, 0, -P, S, TS, 2, -1.0);
+        r.dst = rotj + (it * 3 + 1) * N;
+        jobs.push_back(r);
+        r = FieldJob{};
+        first(r, 0, jy, 0, -P, -S, TS, 1, 1.0);
+        first(r, 1, jx, 1, -P, -S, TS, 2, -1.0);
+        r.dst = rotj + (it * 3 + 2) * N;
+        jobs.push_back(r);
+        // DerS and rot s (Densities.f90:1058-1101)
+        const double *sc[3] = {sx, sy, sz};
+        const int sS[3] = {-S, -S, S}, sC[3] = {1, 2, 1};
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) {
+                FieldJob d{};
+                first(d, 0, sc[c], a, P, sS[c], TS, sC[c], 1.0);
+                d.dst = ders + ((it * 3 + c) * 3 + a) * N;
+                jobs.push_back(d);
+            }
+        // rot s_x = d_y s_z - d_z s_y ; rot s_y = d_z s_x - d_x s_z ;
+        // rot s_z = d_x s_y - d_y s_x
+        const int rc[3][4] = {{2, 1, 1, 2}, {0, 2, 2, 0}, {1, 0, 0, 1}};
+        for (int q = 0; q < 3; ++q) {
+            FieldJob d{};
+            first(d, 0, sc[rc[q][0]], rc[q][1], P, sS[rc[q][0]], TS, sC[rc[q][0]], 1.0);
+            first(d, 1, sc[rc[q][2]], rc[q][3], P, sS[rc[q][2]], TS, sC[rc[q][2]], -1.0);
+            d.dst = rots + (it * 3 + q) * N;
+            jobs.push_back(d);
+        }
+    }
+}
+
+template <typename T>
+const MeshK<T> &mesh_of(mocca_solve *h);
+template <>
+const MeshK<double> &mesh_of<double>(mocca_solve *h) { return h->md; }
+template <>
+const MeshK<float> &mesh_of<float>(mocca_solve *h) { return h->mf; }
+
+int check_ready(mocca_solve *h) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    if (!h->meta_set) return fail(h, MOCCA_ERR_STATE, "mocca_set_states_meta not called");
+    return 0;
+}
+
+int check_status(mocca_solve *h) {
+    int s = 0;
+    HIPCHK(h, hipMemcpyAsync(&s, h->b.status, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (s) {
+        HIPCHK(h, hipMemsetAsync(h->b.status, 0, sizeof(int), h->st));
+        return fail(h, MOCCA_ERR_NUMERIC,
+                    "Gram-Schmidt met a non-positive norm (linearly dependent or NaN states)");
+    }
+    return 0;
+}
+
+template <typename T>
+int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_mode) {
+    const long K4 = 4L * h->N;
+    int rc = run(h, "gram", [&] {
+        launch_gram<T>(src, K4, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m,
+                       h->b.gpart, h->nsplit, h->st);
+    });
+    if (rc) return rc;
+    rc = run(h, "gram_reduce", [&] {
+        launch_gram_reduce(h->b.gpart, h->b.blk_m, h->nblk, h->max_m, h->nsplit,
+                           h->mesh.dv, h->b.G, h->st);
+    });
+    if (rc) return rc;
+    rc = run(h, "cholesky", [&] {
+        launch_cholesky_ld(h->b.G, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m,
+                           h->b.E, h->b.Tm, h->b.perm, h->b.work, h->b.status, h->st);
+    });
+    if (rc) return rc;
+    return run(h, "gs_apply", [&] {
+        launch_gs_apply<T>(src, dst, upd, K4, h->b.perm, h->b.blk_m, h->nblk, h->max_m,
+                           h->b.Tm, update_mode, h->st);
+    });
+}
+
+
+template <typename T>
+int grad_desc_impl(mocca_solve *h, double dth, double mu) {
+    const MeshK<T> &m = mesh_of<T>(h);
+    T *psi = (T *)h->b.psi, *psi2 = (T *)h->b.psi2, *upd = (T *)h->b.upd;
+    int rc = run(h, "hpsi_step", [&] {
+        launch_hpsi_step<T>(m, psi, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
+                            h->b.order, h->b.chunks, h->nchunks, h->mesh.time_reversal,
+                            T(h->kin[0]), T(h->kin[1]), T(dth), T(mu), kStep,
+                            h->b.eparts, h->ntiles, h->st);
+    });
+    if (rc) return rc;
+    rc = run(h, "reduce_ed", [&] {
+        launch_reduce_ed(h->b.eparts, h->ntiles, h->b.all_states, h->M, h->mesh.dv,
+                         h->b.E, h->b.D, h->st);
+    });
+    if (rc) return rc;
+    // Gram-Schmidt of the stepped states psi2 -> psi, with u = psi_new - u.
+    return gram_schmidt_impl<T>(h, psi2, psi, upd, 1);
+}
+
+template <typename T>
+int hpsi_one_impl(mocca_solve *h, int state) {
+    const MeshK<T> &m = mesh_of<T>(h);
+    Chunk c{0, 1, (h->qn[4 * state] + 3) / 2 - 1, 0};
+    HIPCHK(h, hipMemcpy(h->b.one_chunk, &c, sizeof c, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->b.one_order, &state, sizeof(int), hipMemcpyHostToDevice));
+    return run(h, "hpsi", [&] {
+        launch_hpsi_step<T>(m, (const T *)h->b.psi, nullptr, nullptr, (T *)h->b.hbuf,
+                            (const T *)h->b.fields, h->b.qn, h->b.one_order,
+                            h->b.one_chunk, 1, h->mesh.time_reversal, T(h->kin[0]),
+                            T(h->kin[1]), T(0), T(0), kWriteH, nullptr, h->ntiles, h->st);
+    });
+}
+
+template <typename T>
+int density_impl(mocca_solve *h) {
+    const MeshK<T> &m = mesh_of<T>(h);
+    const int nf = h->mesh.time_reversal ? 3 : 9;
+    int rc = run(h, "density", [&] {
+        launch_density<T>(m, (const T *)h->b.psi, h->b.qn, h->b.occ, h->b.order,
+                          h->b.chunks, h->nchunks, h->mesh.time_reversal,
+                          (T *)h->b.dpart, h->st);
+    });
+    if (rc) return rc;
+    rc = run(h, "density_reduce", [&] {
+        launch_density_reduce<T>((const T *)h->b.dpart, h->b.chunks, h->nchunks, nf, h->N,
+                                 h->b.dens, h->st);
+    });
+    if (rc) return rc;
+    return run(h, "field_jobs", [&] {
+        launch_field_jobs(h->md, h->b.jobs, h->njobs, h->st);
+    });
+}
+
+template <typename T>
+int upload_convert(mocca_solve *h, T *dst, const double *src, size_t n) {
+    if (sizeof(T) == sizeof(double)) {
+        HIPCHK(h, hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    }
+    std::vector<T> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+    HIPCHK(h, hipMemcpy(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+int download_convert(mocca_solve *h, double *dst, const T *src, size_t n) {
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (sizeof(T) == sizeof(double)) {
+        HIPCHK(h, hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<T> tmp(n);
+    HIPCHK(h, hipMemcpy(tmp.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) dst[i] = (double)tmp[i];
+    return 0;
+}
+
+int up(mocca_solve *h, void *dst, const double *src, size_t n) {
+    return h->prec == MOCCA_FP64 ? upload_convert(h, (double *)dst, src, n)
+                                 : upload_convert(h, (float *)dst, src, n);
+}
+
+int down(mocca_solve *h, double *dst, const void *src, size_t n) {
+    return h->prec == MOCCA_FP64 ? download_convert(h, dst, (const double *)src, n)
+                                 : download_convert(h, dst, (const float *)src, n);
+}
+
+void free_all(mocca_solve *h) {
+    Buffers &b = h->b;
+    void *ptrs[] = {b.psi, b.psi2, b.upd, b.hbuf, b.fields, b.dpart, b.lagT, b.lagD,
+                    b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.eparts,
+                    b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
+                    b.G, b.Tm, b.work, b.gpart, b.status, b.jobs, b.one_chunk,
+                    b.one_order};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &p : h->pending) { h->pool.push_back(p.a); h->pool.push_back(p.b); }
+    for (auto e : h->pool) (void)hipEventDestroy(e);
+    if (h->st) (void)hipStreamDestroy(h->st);
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *mocca_last_error(const mocca_solve *h) {
+    return h ? h->err.c_str() : g_err.c_str();
+}
+
+int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
+                       mocca_solve **out) {
+    if (!mesh || !out) return fail(nullptr, MOCCA_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (mesh->nx < 1 || mesh->ny < 1 || mesh->nz < 1 || !(mesh->dx > 0) || !(mesh->dv > 0))
+        return fail(nullptr, MOCCA_ERR_ARG, "invalid mesh (nx=%d ny=%d nz=%d dx=%g dv=%g)",
+                    mesh->nx, mesh->ny, mesh->nz, mesh->dx, mesh->dv);
+    if (!mesh->lagrange && (mesh->nx < 6 || mesh->ny < 6 || mesh->nz < 6))
+        return fail(nullptr, MOCCA_ERR_ARG,
+                    "EV8 finite differences need at least 6 points per axis");
+    if (nwt < 1) return fail(nullptr, MOCCA_ERR_ARG, "nwt must be positive (got %d)", nwt);
+    if (precision != MOCCA_FP64 && precision != MOCCA_MIXED)
+        return fail(nullptr, MOCCA_ERR_ARG, "unknown precision %d", precision);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, MOCCA_ERR_HIP, "no HIP device available");
+    mocca_solve *h = new mocca_solve;
+    h->mesh = *mesh;
+    h->M = nwt;
+    h->N = mesh->nx * mesh->ny * mesh->nz;
+    h->prec = precision;
+    h->tsize = precision == MOCCA_FP64 ? 8 : 4;
+    int rc = 0;
+#define TRY(x)                     \
+    do {                           \
+        if ((rc = (x)) != 0) {     \
+            g_err = h->err;        \
+            free_all(h);           \
+            delete h;              \
+            return rc;             \
+        }                          \
+    } while (0)
+    if (hipStreamCreate(&h->st) != hipSuccess) {
+        delete h;
+        return fail(nullptr, MOCCA_ERR_HIP, "hipStreamCreate failed");
+    }
+    const size_t N = h->N, M = h->M, ts = h->tsize;
+    Buffers &b = h->b;
+    TRY(dalloc(h, &b.psi, M * 4 * N * ts));
+    TRY(dalloc(h, &b.psi2, M * 4 * N * ts));
+    TRY(dalloc(h, &b.upd, M * 4 * N * ts));
+    TRY(dalloc(h, &b.hbuf, 4 * N * ts));
+    TRY(dalloc(h, &b.fields, 48 * N * ts));
+    TRY(dalloc(h, &b.dens, 2 * 9 * N * sizeof(double)));
+    TRY(dalloc(h, &b.derived, 38 * N * sizeof(double)));
+    TRY(dalloc(h, &b.qn, M * 4 * sizeof(int)));
+    TRY(dalloc(h, &b.occ, M * sizeof(double)));
+    TRY(dalloc(h, &b.E, M * sizeof(double)));
+    TRY(dalloc(h, &b.D, M * sizeof(double)));
+    TRY(dalloc(h, &b.status, sizeof(int)));
+    TRY(dalloc(h, &b.one_chunk, sizeof(Chunk)));
+    TRY(dalloc(h, &b.one_order, sizeof(int)));
+    h->ntiles = (int)((N + kBlock - 1) / kBlock);
+    TRY(dalloc(h, &b.eparts, M * (size_t)h->ntiles * 2 * sizeof(double)));
+    HIPCHK(h, hipMemset(b.upd, 0, M * 4 * N * ts));
+    HIPCHK(h, hipMemset(b.fields, 0, 48 * N * ts));
+    HIPCHK(h, hipMemset(b.dens, 0, 2 * 9 * N * sizeof(double)));
+    HIPCHK(h, hipMemset(b.derived, 0, 38 * N * sizeof(double)));
+    HIPCHK(h, hipMemset(b.E, 0, M * sizeof(double)));
+    HIPCHK(h, hipMemset(b.D, 0, M * sizeof(double)));
+    HIPCHK(h, hipMemset(b.status, 0, sizeof(int)));
+    // Lagrange matrices (EV8: the full line doubles every stored axis).
+    const int ns[3] = {mesh->nx, mesh->ny, mesh->nz};
+    size_t nl = 0;
+    for (int a = 0; a < 3; ++a) nl += 4 * (size_t)ns[a] * ns[a];
+    std::vector<double> lag(nl, 0.0);
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        lagrange_axis(ns[a], 2 * ns[a], mesh->dx, lag.data() + off);
+        off += 4 * (size_t)ns[a] * ns[a];
+    }
+    TRY(dalloc(h, &b.lagD, nl * sizeof(double)));
+    HIPCHK(h, hipMemcpy(b.lagD, lag.data(), nl * sizeof(double), hipMemcpyHostToDevice));
+    TRY(dalloc(h, &b.lagT, nl * ts));
+    TRY(up(h, b.lagT, lag.data(), nl));
+    fill_mesh<double>(h, h->md, b.lagD);
+    fill_mesh<float>(h, h->mf, (const float *)b.lagT);
+    if (precision == MOCCA_FP64) fill_mesh<double>(h, h->md, (const double *)b.lagT);
+    // Derived-density jobs.
+    std::vector<FieldJob> jobs;
+    build_density_jobs(h, jobs);
+    h->njobs = (int)jobs.size();
+    TRY(dalloc(h, &b.jobs, jobs.size() * sizeof(FieldJob)));
+    HIPCHK(h, hipMemcpy(b.jobs, jobs.data(), jobs.size() * sizeof(FieldJob),
+                        hipMemcpyHostToDevice));
+#undef TRY
+    *out = h;
+    return 0;
+}
+
+int mocca_solve_destroy(mocca_solve *h) {
+    if (!h) return 0;
+    free_all(h);
+    delete h;
+    return 0;
+}
+
+int mocca_synchronize(mocca_solve *h) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return check_status(h);
+}
+
+int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
+    if (!h || !qn || !occ) return fail(h, MOCCA_ERR_ARG, "null argument");
+    const int M = h->M;
+    for (int i = 0; i < M; ++i) {
+        const int *q = qn + 4 * i;
+        if ((q[0] != -1 && q[0] != 1) || (q[1] != -1 && q[1] != 1) ||
+            (q[2] != -1 && q[2] != 1) || (q[3] != -1 && q[3] != 1))
+            return fail(h, MOCCA_ERR_ARG, "state %d: quantum numbers must be +-1", i);
+        if (q[3] != 1)
+            return fail(h, MOCCA_ERR_UNSUPPORTED,
+                        "state %d: time simplex -1 is not stored in the EV8 octant", i);
+    }
+    h->qn.assign(qn, qn + 4 * M);
+    h->occ.assign(occ, occ + M);
+    Buffers &b = h->b;
+    HIPCHK(h, hipMemcpy(b.qn, qn, M * 4 * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(b.occ, occ, M * sizeof(double), hipMemcpyHostToDevice));
+    // Chunks: states grouped by isospin, kChunk per chunk.
+    std::vector<int> order;
+    std::vector<Chunk> chunks;
+    for (int it = 0; it < 2; ++it) {
+        std::vector<int> st;
+        for (int i = 0; i < M; ++i)
+            if ((qn[4 * i] + 3) / 2 - 1 == it) st.push_back(i);
+        for (size_t k = 0; k < st.size(); k += kChunk) {
+            Chunk c{(int)order.size(), (int)std::min<size_t>(kChunk, st.size() - k), it, 0};
+            for (int q = 0; q < c.count; ++q) order.push_back(st[k + q]);
+            chunks.push_back(c);
+        }
+    }
+    // Gram-Schmidt blocks: (isospin, parity, signature) (Spwf.f90:842-866).
+    std::vector<std::vector<int>> blocks;
+    for (int iso = -1; iso <= 1; iso += 2)
+        for (int P = -1; P <= 1; P += 2)
+            for (int S = -1; S <= 1; S += 2) {
+                std::vector<int> st;
+                for (int i = 0; i < M; ++i)
+                    if (qn[4 * i] == iso && qn[4 * i + 1] == P && qn[4 * i + 2] == S)
+                        st.push_back(i);
+                if (!st.empty()) blocks.push_back(st);
+            }
+    int mmax = 0;
+    for (auto &bl : blocks) mmax = std::max<int>(mmax, (int)bl.size());
+    if (mmax > kMaxBlock)
+        return fail(h, MOCCA_ERR_UNSUPPORTED,
+                    "a (isospin, parity, signature) block holds %d states (max %d)", mmax,
+                    kMaxBlock);
+    const int max_m = (mmax + kGramTile - 1) / kGramTile * kGramTile;
+    const int nblk = (int)blocks.size();
+    std::vector<int> bs((size_t)nblk * max_m, 0), bm(nblk);
+    for (int k = 0; k < nblk; ++k) {
+        bm[k] = (int)blocks[k].size();
+        for (int q = 0; q < bm[k]; ++q) bs[(size_t)k * max_m + q] = blocks[k][q];
+    }
+    std::vector<int> all(M);
+    for (int i = 0; i < M; ++i) all[i] = i;
+    // (Re)allocate index and GS buffers.
+    void *old[] = {b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
+                   b.G, b.Tm, b.work, b.gpart, b.dpart};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    b.order = nullptr; b.chunks = nullptr; b.all_states = nullptr;
+    b.blk_states = nullptr; b.blk_m = nullptr; b.perm = nullptr;
+    b.G = b.Tm = b.work = b.gpart = nullptr; b.dpart = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &b.order, order.size() * sizeof(int)))) return rc;
+    if ((rc = dalloc(h, &b.chunks, chunks.size() * sizeof(Chunk)))) return rc;
+    if ((rc = dalloc(h, &b.all_states, M * sizeof(int)))) return rc;
+    if ((rc = dalloc(h, &b.blk_states, bs.size() * sizeof(int)))) return rc;
+    if ((rc = dalloc(h, &b.blk_m, nblk * sizeof(int)))) return rc;
+    if ((rc = dalloc(h, &b.perm, bs.size() * sizeof(int)))) return rc;
+    const size_t gsz = (size_t)nblk * max_m * max_m * sizeof(double);
+    if ((rc = dalloc(h, &b.G, gsz))) return rc;
+    if ((rc = dalloc(h, &b.Tm, gsz))) return rc;
+    if ((rc = dalloc(h, &b.work, gsz))) return rc;
+    const int nt = max_m / kGramTile, ntile_max = nt * (nt + 1) / 2;
+    const long K4 = 4L * h->N;
+    int nsplit = std::max(1, 2048 / std::max(1, nblk * ntile_max));
+    nsplit = (int)std::min<long>(nsplit, std::max<long>(1, K4 / (kGramKC * 16)));
+    h->nsplit = nsplit;
+    if ((rc = dalloc(h, &b.gpart, (size_t)nblk * ntile_max * nsplit * kGramTile * kGramTile *
+                                      sizeof(double))))
+        return rc;
+    if ((rc = dalloc(h, &b.dpart, chunks.size() * 9 * (size_t)h->N * h->tsize))) return rc;
+    HIPCHK(h, hipMemcpy(b.order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(b.chunks, chunks.data(), chunks.size() * sizeof(Chunk),
+                        hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(b.all_states, all.data(), M * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(b.blk_states, bs.data(), bs.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(b.blk_m, bm.data(), nblk * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(b.G, 0, gsz));
+    HIPCHK(h, hipMemset(b.Tm, 0, gsz));
+    HIPCHK(h, hipMemset(b.work, 0, gsz));
+    h->nchunks = (int)chunks.size();
+    h->nblk = nblk;
+    h->max_m = max_m;
+    h->meta_set = true;
+    return 0;
+}
+
+int mocca_set_occupations(mocca_solve *h, const double *occ) {
+    if (int rc = check_ready(h)) return rc;
+    if (!occ) return fail(h, MOCCA_ERR_ARG, "null occupations");
+    h->occ.assign(occ, occ + h->M);
+    HIPCHK(h, hipMemcpy(h->b.occ, occ, h->M * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int mocca_set_energies(mocca_solve *h, const double *energy) {
+    if (!h || !energy) return fail(h, MOCCA_ERR_ARG, "null argument");
+    HIPCHK(h, hipMemcpy(h->b.E, energy, h->M * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int mocca_get_energies(mocca_solve *h, double *energy, double *disp) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (energy)
+        HIPCHK(h, hipMemcpy(energy, h->b.E, h->M * sizeof(double), hipMemcpyDeviceToHost));
+    if (disp)
+        HIPCHK(h, hipMemcpy(disp, h->b.D, h->M * sizeof(double), hipMemcpyDeviceToHost));
+    return check_status(h);
+}
+
+int mocca_upload_states(mocca_solve *h, int first, int count, const double *grid) {
+    if (!h || !grid) return fail(h, MOCCA_ERR_ARG, "null argument");
+    if (first < 0 || count < 0 || first + count > h->M)
+        return fail(h, MOCCA_ERR_ARG, "states [%d, %d) out of range (nwt=%d)", first,
+                    first + count, h->M);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    const size_t n4 = 4 * (size_t)h->N;
+    return up(h, (char *)h->b.psi + first * n4 * h->tsize, grid, count * n4);
+}
+
+int mocca_download_states(mocca_solve *h, int first, int count, double *grid) {
+    if (!h || !grid) return fail(h, MOCCA_ERR_ARG, "null argument");
+    if (first < 0 || count < 0 || first + count > h->M)
+        return fail(h, MOCCA_ERR_ARG, "states [%d, %d) out of range (nwt=%d)", first,
+                    first + count, h->M);
+    const size_t n4 = 4 * (size_t)h->N;
+    if (int rc = down(h, grid, (char *)h->b.psi + first * n4 * h->tsize, count * n4))
+        return rc;
+    return check_status(h);
+}
+
+int mocca_reset_updates(mocca_solve *h) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    HIPCHK(h, hipMemsetAsync(h->b.upd, 0, (size_t)h->M * 4 * h->N * h->tsize, h->st));
+    return 0;
+}
+
+int mocca_set_potentials(mocca_solve *h, const mocca_fields *f) {
+    if (!h || !f) return fail(h, MOCCA_ERR_ARG, "null argument");
+    if (!f->U || !f->B || !f->NablaB || !f->W || !f->DivW)
+        return fail(h, MOCCA_ERR_ARG, "U, B, NablaB, W and DivW are required");
+    if (!h->mesh.time_reversal && (!f->S || !f->A || !f->DivA))
+        return fail(h, MOCCA_ERR_ARG, "S, A and DivA are required when time reversal is broken");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    const size_t N = h->N, ts = h->tsize;
+    char *F = (char *)h->b.fields;
+    struct { const double *p; int off, n; } parts[] = {
+        {f->U, 0, 2}, {f->B, 2, 2}, {f->NablaB, 4, 6}, {f->W, 10, 18}, {f->DivW, 28, 6},
+        {f->S, 34, 6}, {f->A, 40, 6}, {f->DivA, 46, 2}};
+    for (auto &q : parts) {
+        if (!q.p) {
+            HIPCHK(h, hipMemset(F + q.off * N * ts, 0, q.n * N * ts));
+            continue;
+        }
+        if (int rc = up(h, F + q.off * N * ts, q.p, q.n * N)) return rc;
+    }
+    h->kin[0] = f->kin[0];
+    h->kin[1] = f->kin[1];
+    h->fields_set = true;
+    return 0;
+}
+
+int mocca_derive_all(mocca_solve *h) {
+    if (int rc = check_ready(h)) return rc;
+    const size_t N = h->N, M = h->M;
+    if (!h->b.der) {
+        if (int rc = dalloc(h, &h->b.der, M * 12 * N * sizeof(double))) return rc;
+        if (int rc = dalloc(h, &h->b.lap, M * 4 * N * sizeof(double))) return rc;
+    }
+    return run(h, "derive_all", [&] {
+        if (h->prec == MOCCA_FP64)
+            launch_derive_all<double>(h->md, (const double *)h->b.psi, h->b.qn, h->M,
+                                      h->b.der, h->b.lap, h->st);
+        else
+            launch_derive_all<float>(h->mf, (const float *)h->b.psi, h->b.qn, h->M,
+                                     h->b.der, h->b.lap, h->st);
+    });
+}
+
+int mocca_download_derivatives(mocca_solve *h, int state, double *der, double *lap) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    if (!h->b.der) return fail(h, MOCCA_ERR_STATE, "mocca_derive_all not called");
+    if (state < 0 || state >= h->M) return fail(h, MOCCA_ERR_ARG, "state %d out of range", state);
+    const size_t N = h->N;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (der)
+        HIPCHK(h, hipMemcpy(der, h->b.der + state * 12 * N, 12 * N * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    if (lap)
+        HIPCHK(h, hipMemcpy(lap, h->b.lap + state * 4 * N, 4 * N * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mocca_hpsi(mocca_solve *h, int state, double *out) {
+    if (int rc = check_ready(h)) return rc;
+    if (!h->fields_set) return fail(h, MOCCA_ERR_STATE, "mocca_set_potentials not called");
+    if (state < 0 || state >= h->M || !out)
+        return fail(h, MOCCA_ERR_ARG, "invalid state %d or null output", state);
+    int rc = h->prec == MOCCA_FP64 ? hpsi_one_impl<double>(h, state)
+                                   : hpsi_one_impl<float>(h, state);
+    if (rc) return rc;
+    return down(h, out, h->b.hbuf, 4 * (size_t)h->N);
+}
+
+int mocca_grad_desc(mocca_solve *h, double dt_over_hbar, double momentum) {
+    if (int rc = check_ready(h)) return rc;
+    if (!h->fields_set) return fail(h, MOCCA_ERR_STATE, "mocca_set_potentials not called");
+    return h->prec == MOCCA_FP64 ? grad_desc_impl<double>(h, dt_over_hbar, momentum)
+                                 : grad_desc_impl<float>(h, dt_over_hbar, momentum);
+}
+
+int mocca_gram_schmidt(mocca_solve *h) {
+    if (int rc = check_ready(h)) return rc;
+    int rc = h->prec == MOCCA_FP64
+                 ? gram_schmidt_impl<double>(h, (const double *)h->b.psi,
+                                             (double *)h->b.psi2, nullptr, 0)
+                 : gram_schmidt_impl<float>(h, (const float *)h->b.psi,
+                                            (float *)h->b.psi2, nullptr, 0);
+    if (rc) return rc;
+    std::swap(h->b.psi, h->b.psi2);
+    return 0;
+}
+
+int mocca_compute_density(mocca_solve *h, mocca_density *out) {
+    if (int rc = check_ready(h)) return rc;
+    int rc = h->prec == MOCCA_FP64 ? density_impl<double>(h) : density_impl<float>(h);
+    if (rc) return rc;
+    if (!out) return 0;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    const size_t N = h->N;
+    auto cp = [&](double *dst, const double *src, size_t n) -> int {
+        if (!dst) return 0;
+        HIPCHK(h, hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    };
+    // dens is [2][9][N] (slots beyond nf unused when time reversal holds)
+    std::vector<double> d(2 * 9 * N);
+    HIPCHK(h, hipMemcpy(d.data(), h->b.dens, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    auto scatter = [&](double *dst, int f, int ncomp) {
+        if (!dst) return;
+        for (int it = 0; it < 2; ++it)
+            for (int c = 0; c < ncomp; ++c)
+                std::memcpy(dst + ((size_t)it * ncomp + c) * N,
+                            d.data() + ((size_t)it * 9 + f + c) * N, N * sizeof(double));
+    };
+    scatter(out->rho, 0, 1);
+    scatter(out->tau, 1, 1);
+    scatter(out->nablaJ, 2, 1);
+    const double *der = h->b.derived;
+    int rc2 = 0;
+    if ((rc2 = cp(out->derrho, der, 6 * N))) return rc2;
+    if ((rc2 = cp(out->laprho, der + 6 * N, 2 * N))) return rc2;
+    if (!h->mesh.time_reversal) {
+        scatter(out->vecj, 3, 3);
+        scatter(out->vecs, 6, 3);
+        if ((rc2 = cp(out->rotvecj, der + 8 * N, 6 * N))) return rc2;
+        if ((rc2 = cp(out->rots, der + 14 * N, 6 * N))) return rc2;
+        if ((rc2 = cp(out->ders, der + 20 * N, 18 * N))) return rc2;
+    }
+    return 0;
+}
+
+int mocca_profile_enable(mocca_solve *h, int enable) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    if (int rc = drain_profile(h)) return rc;
+    h->prof = enable != 0;
+    if (enable) h->acc.clear();
+    return 0;
+}
+
+int mocca_profile_get(mocca_solve *h, const char *name, double *ms, long *count) {
+    if (!h || !name) return fail(h, MOCCA_ERR_ARG, "null argument");
+    if (int rc = drain_profile(h)) return rc;
+    auto it = h->acc.find(name);
+    if (ms) *ms = it == h->acc.end() ? 0.0 : it->second.first;
+    if (count) *count = it == h->acc.end() ? 0 : it->second.second;
+    return 0;
+}
+
+}  // extern "C"

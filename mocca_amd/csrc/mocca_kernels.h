@@ -1,0 +1,99 @@
+// Device kernels of the mocca_amd library (gfx950, wave64).
+//
+// Storage: states psi[i][c][N] (x fastest), fields [it][...][N].  All kernels
+// are templated on the storage/stencil type T (double for the FP64 path,
+// float for the mixed path); reductions that feed Gram-Schmidt and energies
+// accumulate in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mocca {
+
+constexpr int kBlock = 256;      // threads per workgroup for point kernels
+constexpr int kChunk = 8;        // states per chunk (one isospin per chunk)
+constexpr int kGramTile = 64;    // Gram tile edge (16x16 threads x 4x4)
+constexpr int kGramKC = 16;      // K sub-chunk staged in LDS by the Gram kernel
+constexpr int kApplyL = 32;      // output states per workgroup in the apply
+constexpr int kMaxBlock = 512;   // largest (isospin, parity, signature) block
+
+// Mesh description passed by value to kernels.
+template <typename T>
+struct MeshK {
+    int nx, ny, nz, N;
+    int lag;                  // 1 = Lagrange matrices, 0 = EV8 finite differences
+    T inv60dx;                // 1/(60 dx) of the 7-point first derivative
+    T l2m, l2c2, l2c3, l2c4;  // 9-point second-derivative weights
+    T l2scale;                // -1/(5040/(-8064) dx^2)
+    // Lagrange matrices (row-major n x n): first[axis][sign(0:-,1:+)],
+    // second[axis][sign].
+    const T *lag1[3][2];
+    const T *lag2[3][2];
+};
+
+// Chunk of states of one isospin: states order[start .. start+count-1].
+struct Chunk {
+    int start, count, it, pad;
+};
+
+// One job of the field-derivative kernel: dst = sum_t coef[t] * D_t(src[t]),
+// with D_t the first derivative along axis[t] (0..2) or the Laplacian
+// (axis 3) with per-axis ghost signs sign[t][0..2].
+struct FieldJob {
+    const double *src[3];
+    double *dst;
+    int axis[3];
+    int sign[3][3];
+    double coef[3];
+    int nterms;
+    int pad;
+};
+
+enum HpsiMode { kWriteH = 1, kStep = 2 };
+
+template <typename T>
+void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
+                       double *der, double *lap, hipStream_t st);
+
+template <typename T>
+void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
+                      T *hout, const T *fields, const int *qn, const int *order,
+                      const Chunk *chunks, int nchunks, int trc, T kin0, T kin1,
+                      T dth, T mu, int mode, double *eparts, int ntiles,
+                      hipStream_t st);
+
+void launch_reduce_ed(const double *eparts, int ntiles, const int *states,
+                      int nstates, double dv, double *E, double *D,
+                      hipStream_t st);
+
+template <typename T>
+void launch_density(const MeshK<T> &m, const T *psi, const int *qn,
+                    const double *occ, const int *order, const Chunk *chunks,
+                    int nchunks, int trc, T *part, hipStream_t st);
+
+template <typename T>
+void launch_density_reduce(const T *part, const Chunk *chunks, int nchunks,
+                           int nf, int N, double *dens, hipStream_t st);
+
+void launch_field_jobs(const MeshK<double> &m, const FieldJob *jobs, int njobs,
+                       hipStream_t st);
+
+template <typename T>
+void launch_gram(const T *psi, long K4, const int *blk_states, const int *blk_m,
+                 int nblk, int max_m, double *partial, int nsplit,
+                 hipStream_t st);
+
+void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
+                        int max_m, int nsplit, double dv, double *G,
+                        hipStream_t st);
+
+void launch_cholesky_ld(const double *G, const int *blk_states,
+                        const int *blk_m, int nblk, int max_m, const double *E,
+                        double *Tm, int *perm, double *work, int *status,
+                        hipStream_t st);
+
+template <typename T>
+void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
+                     const int *blk_m, int nblk, int max_m, const double *Tm,
+                     int update_mode, hipStream_t st);
+
+}  // namespace mocca

@@ -1,0 +1,722 @@
+// HIP kernels of the mocca_amd library (gfx950 / CDNA4, wave64).
+//
+// Reference routines each kernel replaces are cited as path:line in the
+// reference MOCCa tree.
+#include "mocca_kernels.h"
+
+namespace mocca {
+
+// Field buffer offsets, in units of N (one contiguous buffer per handle):
+// U[2] B[2] NablaB[2][3] W[2][nu][mu] DivW[2][3] S[2][3] A[2][3] DivA[2]
+constexpr int kOffU = 0, kOffB = 2, kOffNB = 4, kOffW = 10, kOffDW = 28,
+              kOffS = 34, kOffA = 40, kOffDA = 46, kNumFieldSlots = 48;
+
+// ---------------------------------------------------------------------------
+// Sign tables of the EV8 octant (OptimizedDerivatives.f90:24-44,115-132,
+// 208-225,340-357 for first derivatives; :522-571 for the Laplacian).
+__device__ __forceinline__ int sign_first(int axis, int P, int S, int TS,
+                                          int c, int lag) {
+    const int cx = (c == 0 || c == 3) ? 1 : -1;
+    const int cy = (c == 0 || c == 2) ? 1 : -1;
+    const int cz = (c < 2) ? 1 : -1;
+    if (axis == 0) return cx * (lag ? S * TS : S);
+    if (axis == 1) return cy * TS;
+    return cz * S * P;
+}
+
+__device__ __forceinline__ int sign_lap(int axis, int P, int S, int TS, int c) {
+    const int cx = (c == 0 || c == 3) ? 1 : -1;
+    const int cy = (c == 0 || c == 2) ? 1 : -1;
+    const int cz = (c < 2) ? 1 : -1;
+    if (axis == 0) return cx * S;
+    if (axis == 1) return cy * TS;
+    return cz * S * P;
+}
+
+// Line value with the reflection ghost below the symmetry plane and zero
+// beyond the far box edge (derx/lapla_X_ev8 boundary rows).
+template <typename T>
+__device__ __forceinline__ T line_at(const T *__restrict__ line, int j, int n,
+                                     int stride, T s) {
+    if (j < 0) return s * line[(-1 - j) * stride];
+    if (j >= n) return T(0);
+    return line[j * stride];
+}
+
+// First derivative d1 (sign s1) and second derivative d2 (sign s2) along one
+// axis at the point (x, y, z) of field f.
+template <typename T>
+__device__ __forceinline__ void axis_derivs(const MeshK<T> &m,
+                                            const T *__restrict__ f, int x,
+                                            int y, int z, int axis, int s1,
+                                            int s2, T &d1, T &d2, bool need2) {
+    int n, i, stride;
+    long base;
+    if (axis == 0) {
+        n = m.nx; i = x; stride = 1; base = ((long)z * m.ny + y) * m.nx;
+    } else if (axis == 1) {
+        n = m.ny; i = y; stride = m.nx; base = (long)z * m.ny * m.nx + x;
+    } else {
+        n = m.nz; i = z; stride = m.nx * m.ny; base = (long)y * m.nx + x;
+    }
+    const T *line = f + base;
+    if (!m.lag) {
+        const T s = s2 > 0 ? T(1) : T(-1);
+        T v[9];
+#pragma unroll
+        for (int k = -4; k <= 4; ++k) v[k + 4] = line_at(line, i + k, n, stride, s);
+        if (need2)
+            d2 = m.l2scale * (m.l2m * v[4] + (v[5] + v[3]) + m.l2c2 * (v[6] + v[2]) +
+                              m.l2c3 * (v[7] + v[1]) + m.l2c4 * (v[8] + v[0]));
+        if (s1 != s2) {
+            const T s1t = s1 > 0 ? T(1) : T(-1);
+#pragma unroll
+            for (int k = -3; k <= 3; ++k) v[k + 4] = line_at(line, i + k, n, stride, s1t);
+        }
+        d1 = (T(45) * (v[5] - v[3]) - T(9) * (v[6] - v[2]) + (v[7] - v[1])) * m.inv60dx;
+    } else {
+        const T *M1 = m.lag1[axis][s1 > 0 ? 1 : 0] + (long)i * n;
+        const T *M2 = m.lag2[axis][s2 > 0 ? 1 : 0] + (long)i * n;
+        T a1 = 0, a2 = 0;
+        for (int j = 0; j < n; ++j) {
+            const T w = line[j * stride];
+            a1 += M1[j] * w;
+            if (need2) a2 += M2[j] * w;
+        }
+        d1 = a1;
+        d2 = a2;
+    }
+}
+
+// Value, gradient and Laplacian of all four components of state `s` at a
+// point (CompDer, Spwf.f90:778-792).
+template <typename T>
+__device__ __forceinline__ void state_derivs(const MeshK<T> &m,
+                                             const T *__restrict__ ps, int P,
+                                             int S, int TS, int x, int y, int z,
+                                             long p, T v[4], T d[3][4], T l[4],
+                                             bool need_lap) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const T *f = ps + (long)c * m.N;
+        v[c] = f[p];
+        T lap = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T d1, d2 = 0;
+            axis_derivs(m, f, x, y, z, a, sign_first(a, P, S, TS, c, m.lag),
+                        sign_lap(a, P, S, TS, c), d1, d2, need_lap);
+            d[a][c] = d1;
+            lap += d2;
+        }
+        l[c] = lap;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DeriveAll (SpwfStorage.f90:586-604): materialised derivatives.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_derive_all(MeshK<T> m,
+                                                       const T *__restrict__ psi,
+                                                       const int *__restrict__ qn,
+                                                       double *__restrict__ der,
+                                                       double *__restrict__ lap) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int s = blockIdx.y;
+    if (p >= m.N) return;
+    const int x = p % m.nx, y = (p / m.nx) % m.ny, z = p / ((long)m.nx * m.ny);
+    T v[4], d[3][4], l[4];
+    state_derivs(m, psi + (long)s * 4 * m.N, qn[4 * s + 1], qn[4 * s + 2],
+                 qn[4 * s + 3], x, y, z, p, v, d, l, true);
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 4; ++c)
+            der[(((long)s * 3 + a) * 4 + c) * m.N + p] = (double)d[a][c];
+    for (int c = 0; c < 4; ++c) lap[((long)s * 4 + c) * m.N + p] = (double)l[c];
+}
+
+template <typename T>
+void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
+                       double *der, double *lap, hipStream_t st) {
+    dim3 g((m.N + kBlock - 1) / kBlock, M);
+    hipLaunchKernelGGL(k_derive_all<T>, g, dim3(kBlock), 0, st, m, psi, qn, der, lap);
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide sum of two values (wave64 shuffles, then LDS over the waves).
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { sh[2 * w] = a; sh[2 * w + 1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int k = 0; k < kBlock / 64; ++k) { sa += sh[2 * k]; sb += sh[2 * k + 1]; }
+        a = sa; b = sb;
+    }
+    __syncthreads();
+}
+
+// hPsi (ImaginaryTime.f90:213-304) fused with the heavy-ball step of GradDesc
+// (ImaginaryTime.f90:467-491).  One thread per mesh point, looping over the
+// states of one chunk (one isospin), so the isospin's potentials are read
+// once per point and reused.  Partial sums of <psi|h psi> and <h psi|h psi>
+// go to eparts[state][tile].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hpsi_step(
+    MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
+    T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
+    const int *__restrict__ qn, const int *__restrict__ order,
+    const Chunk *__restrict__ chunks, int trc, T kin0, T kin1, T dth, T mu,
+    int mode, double *__restrict__ eparts, int ntiles) {
+    __shared__ double sh[2 * kBlock / 64];
+    const long N = m.N;
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const Chunk ch = chunks[blockIdx.y];
+    const bool valid = p < N;
+    const long pp = valid ? p : 0;
+    const int x = pp % m.nx, y = (pp / m.nx) % m.ny, z = pp / ((long)m.nx * m.ny);
+    const int it = ch.it;
+    const T kin = it == 0 ? kin0 : kin1;
+    // Potentials of this isospin at the point (MeanFields.f90:80-95).
+    const T u = F[(kOffU + it) * N + pp];
+    const T b = F[(kOffB + it) * N + pp];
+    T nb[3], w[3][3], dw[3], sv[3] = {0, 0, 0}, av[3] = {0, 0, 0}, da = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        nb[k] = F[(kOffNB + it * 3 + k) * N + pp];
+        dw[k] = F[(kOffDW + it * 3 + k) * N + pp];
+#pragma unroll
+        for (int mu_ = 0; mu_ < 3; ++mu_) w[k][mu_] = F[(kOffW + it * 9 + k * 3 + mu_) * N + pp];
+    }
+    if (!trc) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sv[k] = F[(kOffS + it * 3 + k) * N + pp];
+            av[k] = F[(kOffA + it * 3 + k) * N + pp];
+        }
+        da = F[(kOffDA + it) * N + pp];
+    }
+    for (int q = 0; q < ch.count; ++q) {
+        const int s = order[ch.start + q];
+        const T *ps = psi + (long)s * 4 * N;
+        T v[4] = {0, 0, 0, 0}, d[3][4] = {{0}}, l[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0};
+        if (valid) {
+            state_derivs(m, ps, qn[4 * s + 1], qn[4 * s + 2], qn[4 * s + 3], x, y, z,
+                         p, v, d, l, true);
+            // ActionOfBNew (MeanFields.f90:811-851)
+            // ActionOfU    (MeanFields.f90:792-809)
+            // ActionOfW    (MeanFields.f90:1029-1065): -i[W_mn s_n d_m + 1/2 divW_n s_n] psi
+            T aw[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int mu_ = 0; mu_ < 3; ++mu_) {
+                const T *e = d[mu_];
+                const T wx = w[0][mu_], wy = w[1][mu_], wz = w[2][mu_];
+                aw[0] += wx * e[2] + wy * e[3] + wz * e[0];
+                aw[1] += wx * e[3] - wy * e[2] + wz * e[1];
+                aw[2] += wx * e[0] - wy * e[1] - wz * e[2];
+                aw[3] += wx * e[1] + wy * e[0] - wz * e[3];
+            }
+            {
+                const T hx = T(0.5) * dw[0], hy = T(0.5) * dw[1], hz = T(0.5) * dw[2];
+                aw[0] += hx * v[2] + hy * v[3] + hz * v[0];
+                aw[1] += hx * v[3] - hy * v[2] + hz * v[1];
+                aw[2] += hx * v[0] - hy * v[1] - hz * v[2];
+                aw[3] += hx * v[1] + hy * v[0] - hz * v[3];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const T ab = -b * l[c] - kin * l[c] -
+                             (nb[0] * d[0][c] + nb[1] * d[1][c] + nb[2] * d[2][c]);
+                h[c] = ab + u * v[c];
+            }
+            h[0] += aw[1]; h[1] -= aw[0]; h[2] += aw[3]; h[3] -= aw[2];
+            if (!trc) {
+                // ActionOfS (MeanFields.f90:1131-1147): S . sigma psi
+                h[0] += sv[0] * v[2] + sv[1] * v[3] + sv[2] * v[0];
+                h[1] += sv[0] * v[3] - sv[1] * v[2] + sv[2] * v[1];
+                h[2] += sv[0] * v[0] - sv[1] * v[1] - sv[2] * v[2];
+                h[3] += sv[0] * v[1] + sv[1] * v[0] - sv[2] * v[3];
+                // ActionOfA (MeanFields.f90:1107-1129): -i[A.grad + 1/2 divA] psi
+                T aa[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    aa[c] = av[0] * d[0][c] + av[1] * d[1][c] + av[2] * d[2][c] +
+                            T(0.5) * da * v[c];
+                h[0] += aa[1]; h[1] -= aa[0]; h[2] += aa[3]; h[3] -= aa[2];
+            }
+        }
+        double e = 0, dd = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            e += (double)v[c] * (double)h[c];
+            dd += (double)h[c] * (double)h[c];
+        }
+        if (valid) {
+            if (mode & kWriteH) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) hout[(long)c * N + p] = h[c];
+            }
+            if (mode & kStep) {
+                T *us = upd + (long)s * 4 * N;
+                T *po = psi_out + (long)s * 4 * N;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const T uo = us[(long)c * N + p];
+                    po[(long)c * N + p] = v[c] + (-(dth * h[c]) + mu * uo);
+                    us[(long)c * N + p] = v[c];
+                }
+            }
+        }
+        if (eparts) {
+            block_sum2(e, dd, sh);
+            if (threadIdx.x == 0) {
+                eparts[((long)s * ntiles + blockIdx.x) * 2] = e;
+                eparts[((long)s * ntiles + blockIdx.x) * 2 + 1] = dd;
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
+                      T *hout, const T *fields, const int *qn, const int *order,
+                      const Chunk *chunks, int nchunks, int trc, T kin0, T kin1,
+                      T dth, T mu, int mode, double *eparts, int ntiles,
+                      hipStream_t st) {
+    dim3 g(ntiles, nchunks);
+    hipLaunchKernelGGL(k_hpsi_step<T>, g, dim3(kBlock), 0, st, m, psi, psi_out, upd,
+                       hout, fields, qn, order, chunks, trc, kin0, kin1, dth, mu,
+                       mode, eparts, ntiles);
+}
+
+// Energies and dispersions (ImaginaryTime.f90:470-471): E = dv sum psi.h,
+// disp = dv sum h.h - E^2.
+__global__ __launch_bounds__(kBlock) void k_reduce_ed(const double *__restrict__ eparts,
+                                                      int ntiles,
+                                                      const int *__restrict__ states,
+                                                      double dv, double *E, double *D) {
+    __shared__ double sh[2 * kBlock / 64];
+    const int s = states[blockIdx.x];
+    double a = 0, b = 0;
+    for (int t = threadIdx.x; t < ntiles; t += kBlock) {
+        a += eparts[((long)s * ntiles + t) * 2];
+        b += eparts[((long)s * ntiles + t) * 2 + 1];
+    }
+    block_sum2(a, b, sh);
+    if (threadIdx.x == 0) {
+        const double e = a * dv;
+        E[s] = e;
+        D[s] = b * dv - e * e;
+    }
+}
+
+void launch_reduce_ed(const double *eparts, int ntiles, const int *states,
+                      int nstates, double dv, double *E, double *D,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_ed, dim3(nstates), dim3(kBlock), 0, st, eparts,
+                       ntiles, states, dv, E, D);
+}
+
+// ---------------------------------------------------------------------------
+// Density accumulation (ComputeDensity, Densities.f90:464-534; getters
+// Spwf.f90:536,1897,2034,2071,2206): per chunk partial fields
+// part[chunk][f][N], f = rho, tau, nablaJ, [j_x, j_y, j_z, s_x, s_y, s_z].
+template <typename T>
+__device__ __forceinline__ T im_dot(const T a[4], const T b[4]) {
+    return a[0] * b[1] - a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+}
+
+template <typename T>
+__device__ __forceinline__ void pauli(const T a[4], int dir, T o[4]) {
+    if (dir == 0) { o[0] = a[2]; o[1] = a[3]; o[2] = a[0]; o[3] = a[1]; }
+    else if (dir == 1) { o[0] = a[3]; o[1] = -a[2]; o[2] = -a[1]; o[3] = a[0]; }
+    else { o[0] = a[0]; o[1] = a[1]; o[2] = -a[2]; o[3] = -a[3]; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_density(
+    MeshK<T> m, const T *__restrict__ psi, const int *__restrict__ qn,
+    const double *__restrict__ occ, const int *__restrict__ order,
+    const Chunk *__restrict__ chunks, int trc, T *__restrict__ part) {
+    const long N = m.N;
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= N) return;
+    const Chunk ch = chunks[blockIdx.y];
+    const int x = p % m.nx, y = (p / m.nx) % m.ny, z = p / ((long)m.nx * m.ny);
+    T acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < ch.count; ++q) {
+        const int s = order[ch.start + q];
+        const T o = (T)occ[s];
+        if (occ[s] == 0.0) continue;
+        T v[4], d[3][4], l[4];
+        state_derivs(m, psi + (long)s * 4 * N, qn[4 * s + 1], qn[4 * s + 2],
+                     qn[4 * s + 3], x, y, z, p, v, d, l, false);
+        T t[4];
+        acc[0] += o * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+        T ta = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            ta += d[a][0] * d[a][0] + d[a][1] * d[a][1] + d[a][2] * d[a][2] + d[a][3] * d[a][3];
+        acc[1] += o * ta;
+        // nabla.J = sum eps_ijk Im(d_i psi^+ sigma_k d_j psi) (Spwf.f90:2206-2243)
+        T nj = 0;
+        pauli(d[1], 2, t); nj += im_dot(d[0], t);
+        pauli(d[2], 1, t); nj -= im_dot(d[0], t);
+        pauli(d[1], 0, t); nj -= im_dot(d[2], t);
+        pauli(d[0], 1, t); nj += im_dot(d[2], t);
+        pauli(d[0], 2, t); nj -= im_dot(d[1], t);
+        pauli(d[2], 0, t); nj += im_dot(d[1], t);
+        acc[2] += o * nj;
+        if (!trc) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) acc[3 + a] += o * im_dot(v, d[a]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                pauli(v, a, t);
+                acc[6 + a] += o * (v[0] * t[0] + v[1] * t[1] + v[2] * t[2] + v[3] * t[3]);
+            }
+        }
+    }
+    const int nf = trc ? 3 : 9;
+    for (int f = 0; f < nf; ++f) part[((long)blockIdx.y * nf + f) * N + p] = acc[f];
+}
+
+template <typename T>
+void launch_density(const MeshK<T> &m, const T *psi, const int *qn,
+                    const double *occ, const int *order, const Chunk *chunks,
+                    int nchunks, int trc, T *part, hipStream_t st) {
+    dim3 g((m.N + kBlock - 1) / kBlock, nchunks);
+    hipLaunchKernelGGL(k_density<T>, g, dim3(kBlock), 0, st, m, psi, qn, occ, order,
+                       chunks, trc, part);
+}
+
+// Sum of chunk partials per isospin in chunk order: dens[it][f][N] (double,
+// nine field slots per isospin).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_density_reduce(const T *__restrict__ part,
+                                                           const Chunk *__restrict__ chunks,
+                                                           int nchunks, int nf, long N,
+                                                           double *__restrict__ dens) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int f = blockIdx.y;
+    if (p >= N) return;
+    double a0 = 0, a1 = 0;
+    for (int c = 0; c < nchunks; ++c) {
+        const double v = (double)part[((long)c * nf + f) * N + p];
+        if (chunks[c].it == 0) a0 += v; else a1 += v;
+    }
+    dens[((long)0 * 9 + f) * N + p] = a0;
+    dens[((long)1 * 9 + f) * N + p] = a1;
+}
+
+template <typename T>
+void launch_density_reduce(const T *part, const Chunk *chunks, int nchunks,
+                           int nf, int N, double *dens, hipStream_t st) {
+    dim3 g((N + kBlock - 1) / kBlock, nf);
+    hipLaunchKernelGGL(k_density_reduce<T>, g, dim3(kBlock), 0, st, part, chunks,
+                       nchunks, nf, (long)N, dens);
+}
+
+// ---------------------------------------------------------------------------
+// Derivatives of fields (DerRho, LapRho, rot j, rot s, DerS ...:
+// Densities.f90:541-550,1003-1101), batched as a job list.
+__global__ __launch_bounds__(kBlock) void k_field_jobs(MeshK<double> m,
+                                                       const FieldJob *__restrict__ jobs) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m.N) return;
+    const FieldJob &jb = jobs[blockIdx.y];
+    const int x = p % m.nx, y = (p / m.nx) % m.ny, z = p / ((long)m.nx * m.ny);
+    double acc = 0;
+    for (int t = 0; t < jb.nterms; ++t) {
+        double val;
+        if (jb.axis[t] < 3) {
+            double d1, d2 = 0;
+            axis_derivs(m, jb.src[t], x, y, z, jb.axis[t], jb.sign[t][0],
+                        jb.sign[t][0], d1, d2, false);
+            val = d1;
+        } else {
+            double lap = 0;
+            for (int a = 0; a < 3; ++a) {
+                double d1, d2;
+                axis_derivs(m, jb.src[t], x, y, z, a, jb.sign[t][a], jb.sign[t][a],
+                            d1, d2, true);
+                lap += d2;
+            }
+            val = lap;
+        }
+        acc += jb.coef[t] * val;
+    }
+    jb.dst[p] = acc;
+}
+
+void launch_field_jobs(const MeshK<double> &m, const FieldJob *jobs, int njobs,
+                       hipStream_t st) {
+    if (njobs <= 0) return;
+    dim3 g((m.N + kBlock - 1) / kBlock, njobs);
+    hipLaunchKernelGGL(k_field_jobs, g, dim3(kBlock), 0, st, m, jobs);
+}
+
+// ---------------------------------------------------------------------------
+// Gram-Schmidt as Cholesky QR (SpwfStorage.f90:372-480).  Gram matrices of
+// each (isospin, parity, signature) block, G = dv * Psi^T Psi, in FP64.
+// Grid: x = K split, y = upper 64x64 tile of G, z = block.
+__device__ __forceinline__ bool tile_of(int tix, int nt, int &ti, int &tj) {
+    int cnt = 0;
+    for (int a = 0; a < nt; ++a)
+        for (int c = a; c < nt; ++c) {
+            if (cnt == tix) { ti = a; tj = c; return true; }
+            ++cnt;
+        }
+    return false;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gram(const T *__restrict__ psi, long K4,
+                                                 const int *__restrict__ blk_states,
+                                                 const int *__restrict__ blk_m,
+                                                 int max_m, double *__restrict__ partial,
+                                                 int nsplit, int ntile_max) {
+    const int w = blockIdx.x, tix = blockIdx.y, b = blockIdx.z;
+    const int m = blk_m[b];
+    int ti, tj;
+    if (!tile_of(tix, (m + kGramTile - 1) / kGramTile, ti, tj)) return;
+    __shared__ double As[kGramKC][kGramTile + 2];
+    __shared__ double Bs[kGramKC][kGramTile + 2];
+    const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0;
+    const long kw = ((K4 + nsplit - 1) / nsplit + kGramKC - 1) / kGramKC * kGramKC;
+    const long k0 = (long)w * kw, k1 = min(K4, k0 + kw);
+    const int *st = blk_states + (long)b * max_m;
+    for (long kb = k0; kb < k1; kb += kGramKC) {
+        for (int q = tid; q < kGramTile * kGramKC; q += kBlock) {
+            const int r = q / kGramKC, kk = q % kGramKC;
+            const long k = kb + kk;
+            const int ra = ti * kGramTile + r, rb = tj * kGramTile + r;
+            As[kk][r] = (ra < m && k < k1) ? (double)psi[(long)st[ra] * K4 + k] : 0.0;
+            Bs[kk][r] = (rb < m && k < k1) ? (double)psi[(long)st[rb] * K4 + k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGramKC; ++kk) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { a[i] = As[kk][tr * 4 + i]; bb[i] = Bs[kk][tc * 4 + i]; }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bb[j];
+        }
+        __syncthreads();
+    }
+    double *out = partial + (((long)b * ntile_max + tix) * nsplit + w) * kGramTile * kGramTile;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[(tr * 4 + i) * kGramTile + tc * 4 + j] = acc[i][j];
+}
+
+template <typename T>
+void launch_gram(const T *psi, long K4, const int *blk_states, const int *blk_m,
+                 int nblk, int max_m, double *partial, int nsplit,
+                 hipStream_t st) {
+    const int nt = (max_m + kGramTile - 1) / kGramTile;
+    const int ntile_max = nt * (nt + 1) / 2;
+    dim3 g(nsplit, ntile_max, nblk);
+    hipLaunchKernelGGL(k_gram<T>, g, dim3(kBlock), 0, st, psi, K4, blk_states, blk_m,
+                       max_m, partial, nsplit, ntile_max);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(const double *__restrict__ partial,
+                                                        const int *__restrict__ blk_m,
+                                                        int max_m, int nsplit,
+                                                        int ntile_max, double dv,
+                                                        double *__restrict__ G) {
+    const int tix = blockIdx.y, b = blockIdx.z;
+    const int m = blk_m[b];
+    int ti, tj;
+    if (!tile_of(tix, (m + kGramTile - 1) / kGramTile, ti, tj)) return;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= kGramTile * kGramTile) return;
+    const int r = e / kGramTile, c = e % kGramTile;
+    const int i = ti * kGramTile + r, j = tj * kGramTile + c;
+    if (i >= m || j >= m) return;
+    const double *src = partial + ((long)b * ntile_max + tix) * nsplit * kGramTile * kGramTile + e;
+    double s = 0;
+    for (int w = 0; w < nsplit; ++w) s += src[(long)w * kGramTile * kGramTile];
+    s *= dv;
+    double *g = G + (long)b * max_m * max_m;
+    g[(long)i * max_m + j] = s;
+    g[(long)j * max_m + i] = s;
+}
+
+void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
+                        int max_m, int nsplit, double dv, double *G,
+                        hipStream_t st) {
+    const int nt = (max_m + kGramTile - 1) / kGramTile;
+    const int ntile_max = nt * (nt + 1) / 2;
+    dim3 g((kGramTile * kGramTile + kBlock - 1) / kBlock, ntile_max, nblk);
+    hipLaunchKernelGGL(k_gram_reduce, g, dim3(kBlock), 0, st, partial, blk_m, max_m,
+                       nsplit, ntile_max, dv, G);
+}
+
+// Per block: ascending-energy order (the stable insertion sort of
+// OrderSpwfsISO, SpwfStorage.f90:795-847), Cholesky G = L L^T of the
+// reordered Gram matrix and T = (L^T)^{-1}, so that the orthonormal states
+// are Q = Psi_sorted T -- the same Q as the reference's modified
+// Gram-Schmidt in exact arithmetic.
+__global__ __launch_bounds__(kBlock) void k_cholesky(const double *__restrict__ G,
+                                                     const int *__restrict__ blk_states,
+                                                     const int *__restrict__ blk_m,
+                                                     int max_m, const double *__restrict__ E,
+                                                     double *__restrict__ Tm,
+                                                     int *__restrict__ perm,
+                                                     double *__restrict__ work,
+                                                     int *__restrict__ status) {
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int m = blk_m[b];
+    const int *st = blk_states + (long)b * max_m;
+    const double *g = G + (long)b * max_m * max_m;
+    double *A = work + (long)b * max_m * max_m;
+    double *Tb = Tm + (long)b * max_m * max_m;
+    int *pb = perm + (long)b * max_m;
+    __shared__ int rank[kMaxBlock];
+    for (int i = tid; i < m; i += kBlock) {
+        const double ei = E[st[i]];
+        int r = 0;
+        for (int j = 0; j < m; ++j) {
+            const double ej = E[st[j]];
+            if (ej < ei || (ej == ei && j < i)) ++r;
+        }
+        rank[i] = r;
+        pb[r] = st[i];
+    }
+    __syncthreads();
+    for (long q = tid; q < (long)m * m; q += kBlock) {
+        const int i = q / m, j = q % m;
+        A[(long)rank[i] * max_m + rank[j]] = g[(long)i * max_m + j];
+    }
+    __syncthreads();
+    __shared__ double diag;
+    for (int k = 0; k < m; ++k) {
+        if (tid == 0) {
+            const double d = A[(long)k * max_m + k];
+            if (!(d > 0.0)) atomicOr(status, 1);
+            diag = sqrt(d > 0.0 ? d : 1.0);
+            A[(long)k * max_m + k] = diag;
+        }
+        __syncthreads();
+        const double dk = diag;
+        for (int i = k + 1 + tid; i < m; i += kBlock) A[(long)i * max_m + k] /= dk;
+        __syncthreads();
+        const int r = m - k - 1;
+        for (long q = tid; q < (long)r * r; q += kBlock) {
+            const int i = k + 1 + q / r, j = k + 1 + q % r;
+            if (j <= i) A[(long)i * max_m + j] -= A[(long)i * max_m + k] * A[(long)j * max_m + k];
+        }
+        __syncthreads();
+    }
+    // Row j of T = column j of L^{-1} (forward substitution).
+    for (int j = tid; j < m; j += kBlock) {
+        double *X = Tb + (long)j * max_m;
+        for (int i = 0; i < j; ++i) X[i] = 0.0;
+        for (int i = j; i < m; ++i) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = j; k < i; ++k) s -= A[(long)i * max_m + k] * X[k];
+            X[i] = s / A[(long)i * max_m + i];
+        }
+    }
+}
+
+// Q = Psi_sorted T for each block, fused with the heavy-ball memory update
+// u = psi_new - u of GradDesc (ImaginaryTime.f90:497-499) when update_mode.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_apply(const T *__restrict__ src,
+                                                     T *__restrict__ dst,
+                                                     T *__restrict__ upd, long K4,
+                                                     const int *__restrict__ perm,
+                                                     const int *__restrict__ blk_m,
+                                                     int max_m,
+                                                     const double *__restrict__ Tm,
+                                                     int update_mode) {
+    extern __shared__ double Ts[];  // [kend][kApplyL]
+    const int b = blockIdx.z;
+    const int m = blk_m[b];
+    const int l0 = blockIdx.y * kApplyL;
+    if (l0 >= m) return;
+    const int kend = min(m, l0 + kApplyL);
+    const double *Tb = Tm + (long)b * max_m * max_m;
+    const int *pb = perm + (long)b * max_m;
+    for (int q = threadIdx.x; q < kend * kApplyL; q += kBlock) {
+        const int k = q / kApplyL, l = q % kApplyL;
+        Ts[q] = (l0 + l < m) ? Tb[(long)k * max_m + l0 + l] : 0.0;
+    }
+    __syncthreads();
+    const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= K4) return;
+    double acc[kApplyL];
+#pragma unroll
+    for (int l = 0; l < kApplyL; ++l) acc[l] = 0;
+    for (int k = 0; k < kend; ++k) {
+        const double v = (double)src[(long)pb[k] * K4 + e];
+        const double *tr = Ts + k * kApplyL;
+#pragma unroll
+        for (int l = 0; l < kApplyL; ++l) acc[l] += v * tr[l];
+    }
+#pragma unroll
+    for (int l = 0; l < kApplyL; ++l) {
+        if (l0 + l < m) {
+            const long o = (long)pb[l0 + l] * K4 + e;
+            const T r = (T)acc[l];
+            dst[o] = r;
+            if (update_mode) upd[o] = r - upd[o];
+        }
+    }
+}
+
+template <typename T>
+void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
+                     const int *blk_m, int nblk, int max_m, const double *Tm,
+                     int update_mode, hipStream_t st) {
+    dim3 g((K4 + kBlock - 1) / kBlock, (max_m + kApplyL - 1) / kApplyL, nblk);
+    const size_t shm = sizeof(double) * (size_t)max_m * kApplyL;
+    hipLaunchKernelGGL(k_gs_apply<T>, g, dim3(kBlock), shm, st, src, dst, upd, K4,
+                       perm, blk_m, max_m, Tm, update_mode);
+}
+
+// Explicit instantiations.
+#define MOCCA_INST(T)                                                                   \
+    template void launch_derive_all<T>(const MeshK<T> &, const T *, const int *, int,    \
+                                       double *, double *, hipStream_t);                \
+    template void launch_hpsi_step<T>(const MeshK<T> &, const T *, T *, T *, T *,        \
+                                      const T *, const int *, const int *,               \
+                                      const Chunk *, int, int, T, T, T, T, int,          \
+                                      double *, int, hipStream_t);                       \
+    template void launch_density<T>(const MeshK<T> &, const T *, const int *,            \
+                                    const double *, const int *, const Chunk *, int,     \
+                                    int, T *, hipStream_t);                              \
+    template void launch_density_reduce<T>(const T *, const Chunk *, int, int, int,      \
+                                           double *, hipStream_t);                       \
+    template void launch_gram<T>(const T *, long, const int *, const int *, int, int,    \
+                                 double *, int, hipStream_t);                            \
+    template void launch_gs_apply<T>(const T *, T *, T *, long, const int *,             \
+                                     const int *, int, int, const double *, int,         \
+                                     hipStream_t);
+MOCCA_INST(double)
+MOCCA_INST(float)
+
+void launch_cholesky_ld(const double *G, const int *blk_states, const int *blk_m,
+                        int nblk, int max_m, const double *E, double *Tm,
+                        int *perm, double *work, int *status, hipStream_t st) {
+    hipLaunchKernelGGL(k_cholesky, dim3(nblk), dim3(kBlock), 0, st, G, blk_states,
+                       blk_m, max_m, E, Tm, perm, work, status);
+}
+
+}  // namespace mocca

@@ -1,0 +1,132 @@
+"""Synthetic inputs of the benchmark workloads (numpy, seeded PCG64).
+
+``hpsi_microbench`` is the 32^3 / 200-state FP64 workload: states are a
+Gaussian envelope (width 3 fm) times a seeded sum of 8 plane waves with
+|k| <= 1.5 fm^-1; the mean fields are built from a Woods-Saxon profile
+(R = 4 fm, a = 0.6 fm): U = -50 f, B = 10 f, W_{mu nu} = -eps_{k mu nu} w d_k f,
+S = 2 f z-hat (time-odd, so time reversal is broken while parity, signature
+and time simplex are kept: the EV8 octant).  ``ho_states`` gives
+harmonic-oscillator seeds (b = 1.7 fm) with the per-axis parity each spinor
+component needs.
+"""
+import numpy as np
+
+HBM_HALF = 20.73551910       # hbar^2/2m (MeV fm^2), Force.f90 default hbm/2
+EPS = np.zeros((3, 3, 3))
+EPS[0, 1, 2] = EPS[1, 2, 0] = EPS[2, 0, 1] = 1
+EPS[0, 2, 1] = EPS[2, 1, 0] = EPS[1, 0, 2] = -1
+
+
+def mesh(nx, ny, nz, dx):
+    """Cell-centred octant coordinates x_i = (i - 1/2) dx (Mesh.f90:49-73)."""
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dx
+    z = (np.arange(nz) + 0.5) * dx
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return X, Y, Z
+
+
+def comp_signs(P, S, TS=1):
+    """Reflection sign of each spinor component along x, y, z in the EV8
+    octant (Derivatives.f90:348-439)."""
+    return ([S * TS, -S * TS, -S * TS, S * TS], [TS, -TS, TS, -TS],
+            [S * P, S * P, -S * P, -S * P])
+
+
+def woods_saxon_fields(nx, ny, nz, dx, R=4.0, a=0.6, time_odd=True):
+    X, Y, Z = mesh(nx, ny, nz, dx)
+    r = np.sqrt(X**2 + Y**2 + Z**2)
+    f = 1.0 / (1.0 + np.exp((r - R) / a))
+    df = -f * (1 - f) / a
+    grad = np.stack([df * X / r, df * Y / r, df * Z / r])
+    sh = (nz, ny, nx)
+    N = nx * ny * nz
+    U = np.stack([-50 * f, -50 * f])
+    B = np.stack([10 * f, 10 * f])
+    NablaB = np.stack([10 * grad, 10 * grad])
+    w = 60 * 0.16
+    W = np.zeros((2, 3, 3) + sh)     # [it][nu][mu]
+    for mu in range(3):
+        for nu in range(3):
+            W[:, nu, mu] = -sum(EPS[k, mu, nu] * w * grad[k] for k in range(3))
+    DivW = np.zeros((2, 3) + sh)     # sum_mu d_mu W_mu,nu = 0 for this W
+    out = dict(U=U, B=B, NablaB=NablaB, W=W, DivW=DivW)
+    if time_odd:
+        S = np.zeros((2, 3) + sh)
+        S[:, 2] = 2 * f
+        out.update(S=S, A=np.zeros((2, 3) + sh), DivA=np.zeros((2,) + sh))
+    return {k: v.reshape(v.shape[:-3] + (N,)) for k, v in out.items()}
+
+
+def plane_wave_states(rng, nx, ny, nz, dx, M, width=3.0, nwaves=8, kmax=1.5):
+    X, Y, Z = mesh(nx, ny, nz, dx)
+    env = np.exp(-(X**2 + Y**2 + Z**2) / (2 * width**2))
+    out = np.empty((M, 4, nz * ny * nx))
+    for i in range(M):
+        for c in range(4):
+            k = rng.uniform(-1, 1, (nwaves, 3))
+            k *= (kmax * rng.uniform(0, 1, (nwaves, 1)) /
+                  np.maximum(np.linalg.norm(k, axis=1, keepdims=True), 1e-12))
+            a = rng.uniform(-1, 1, nwaves)
+            ph = rng.uniform(0, 2 * np.pi, nwaves)
+            f = np.zeros_like(X)
+            for q in range(nwaves):
+                f += a[q] * np.cos(k[q, 0] * X + k[q, 1] * Y + k[q, 2] * Z + ph[q])
+            out[i, c] = (env * f).reshape(-1)
+    return out
+
+
+def hermite(n, t):
+    h0, h1 = np.ones_like(t), 2 * t
+    if n == 0:
+        return h0
+    for k in range(1, n):
+        h0, h1 = h1, 2 * t * h1 - 2 * k * h0
+    return h1
+
+
+def ho_states(rng, nx, ny, nz, dx, qns, b=1.7, noise=1e-3):
+    """Harmonic-oscillator products (lowest shells) whose per-axis parity
+    matches each component's reflection sign, plus uniform noise."""
+    X, Y, Z = mesh(nx, ny, nz, dx)
+    env = np.exp(-(X**2 + Y**2 + Z**2) / (2 * b * b))
+    out = np.empty((len(qns), 4, nz * ny * nx))
+    for i, (iso, P, S, TS) in enumerate(qns):
+        sx, sy, sz = comp_signs(P, S, TS)
+        for c in range(4):
+            pick = lambda s: [n for n in range(4) if (1 if n % 2 == 0 else -1) == s]
+            f = np.zeros_like(X)
+            for _ in range(2):
+                f += rng.uniform(-1, 1) * hermite(rng.choice(pick(sx[c])), X / b) * \
+                    hermite(rng.choice(pick(sy[c])), Y / b) * hermite(rng.choice(pick(sz[c])), Z / b)
+            out[i, c] = (f * env + noise * rng.uniform(-1, 1, f.shape)).reshape(-1)
+    return out
+
+
+def balanced_qn(M, time_reversal):
+    """Quantum numbers spread evenly over isospin, parity (and signature
+    when time reversal is broken); time simplex +1 (EV8)."""
+    qn = []
+    for i in range(M):
+        iso = -1 if i < M // 2 else 1
+        j = i % (M // 2) if M >= 2 else 0
+        P = 1 if j % 2 == 0 else -1
+        S = 1 if (time_reversal or (j // 2) % 2 == 0) else -1
+        qn.append((iso, P, S, 1))
+    return np.array(qn, np.int32)
+
+
+def hpsi_microbench(seed=2, n=32, dx=0.8, M=200):
+    """The 32^3 / 200-state FP64 hPsi workload (time-odd S field)."""
+    rng = np.random.default_rng(seed)
+    qn = balanced_qn(M, time_reversal=False)
+    occ = rng.uniform(0.5, 1.0, M)
+    psi = plane_wave_states(rng, n, n, n, dx, M)
+    fields = woods_saxon_fields(n, n, n, dx, time_odd=True)
+    dv = dx**3 * 8
+    # orthonormalise-friendly scale: unit norm per state
+    psi /= np.sqrt((psi.reshape(M, -1)**2).sum(1) * dv)[:, None, None]
+    kin = np.array([HBM_HALF, HBM_HALF])
+    return dict(nx=n, ny=n, nz=n, dx=dx, dv=dv, M=M, trc=False, lagrange=False,
+                qn=qn, occ=occ, psi=psi, fields=fields, kin=kin,
+                dt_over_hbar=0.010 / 6.58211928, momentum=0.5)

@@ -1,0 +1,127 @@
+"""GPU parity of the HIP library (through the C ABI) against the reference's
+own golden vectors and the C restatement (oracle).
+
+Tolerances (max-norm relative to the largest reference value):
+  FP64 path   derivatives, hPsi, densities: 1e-11; states after the step +
+              Gram-Schmidt: 1e-10; energies 1e-11.
+  mixed path  FP32 stencils/hPsi/densities: 2e-5 (a few FP32 ulps of the
+              accumulated sums); Gram-Schmidt stays FP64.
+"""
+import numpy as np
+import pytest
+
+import mocca_amd
+from mocca_amd import workloads
+from oracle_lib import (GOLDEN_CASES, Mesh, density, density_derived, fields_of,
+                        golden, graddesc, hpsi as or_hpsi, kin_coefs, relerr)
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(g, precision=mocca_amd.FP64):
+    m = g["meta"]
+    s = mocca_amd.Solve(m["nx"], m["ny"], m["nz"], m["dx"], g["consts"][6], m["nwt"],
+                        time_reversal=m["trc"], lagrange=m["lagrange"],
+                        precision=precision)
+    s.set_states_meta(g["qn"], g["occ"])
+    s.upload_states(g["psi"])
+    F = {k: g[k] for k in ("U", "B", "NablaB", "W", "DivW", "S", "A", "DivA")}
+    s.set_potentials(F["U"], F["B"], F["NablaB"], F["W"], F["DivW"], kin_coefs(g),
+                     S=F["S"], A=F["A"], DivA=F["DivA"])
+    return s
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("prec,tol", [(mocca_amd.FP64, 1e-11), (mocca_amd.MIXED, 2e-5)])
+def test_derivatives_hpsi_density_vs_reference(case, prec, tol):
+    g = golden(case)
+    m = g["meta"]
+    N = m["nx"] * m["ny"] * m["nz"]
+    s = _solve(g, prec)
+    s.derive_all()
+    for i in range(m["nwt"]):
+        der, lap = s.derivatives(i)
+        assert relerr(der, g["der"][i].reshape(3, 4, N)) < tol
+        assert relerr(lap, g["lap"][i].reshape(4, N)) < tol
+        assert relerr(s.hpsi(i), g["hpsi"][i].reshape(4, N)) < tol
+    d = s.compute_density()
+    for k in ("rho", "tau", "nablaJ", "derrho", "laprho"):
+        assert relerr(d[k], g[k].reshape(d[k].shape)) < tol, k
+    if not m["trc"]:
+        for k, gk in (("vecj", "vecj"), ("vecs", "vecs"), ("rotvecj", "rotvecj"),
+                      ("rots", "rots"), ("ders", "ders")):
+            assert relerr(d[k], g[gk].reshape(d[k].shape)) < tol, k
+    s.close()
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_grad_desc_gram_schmidt_vs_reference(case):
+    g = golden(case)
+    m = g["meta"]
+    N = m["nx"] * m["ny"] * m["nz"]
+    s = _solve(g)
+    dth = m["dt"] / g["consts"][5]
+    for step in (1, 2):
+        s.grad_desc(dth, m["momentum"])
+        psi = s.download_states()
+        e, d = s.energies()
+        assert relerr(psi, g[f"step{step}_psi"].reshape(m["nwt"], 4, N)) < 1e-10
+        assert relerr(e, g[f"step{step}_ed"][:, 0]) < 1e-11
+        assert relerr(d, g[f"step{step}_ed"][:, 1]) < 1e-8
+    s.close()
+
+
+def test_gram_schmidt_standalone_orthonormal():
+    g = golden("ev8_todd_fd")
+    m = g["meta"]
+    s = _solve(g)
+    e = np.linspace(-30, 5, m["nwt"])
+    s.set_energies(e)
+    s.gram_schmidt()
+    q = s.download_states().reshape(m["nwt"], -1)
+    dv = g["consts"][6]
+    qn = g["qn"]
+    for key in set(map(tuple, qn[:, :3])):
+        idx = [i for i in range(m["nwt"]) if tuple(qn[i, :3]) == key]
+        Q = q[idx]
+        assert np.allclose(Q @ Q.T * dv, np.eye(len(idx)), atol=1e-12)
+    s.close()
+
+
+def test_microbench_workload_vs_oracle():
+    """The full 32^3 / 200-state workload: one GradDesc (hPsi of every state,
+    step, Gram-Schmidt) and the densities, against the oracle at full size."""
+    w = workloads.hpsi_microbench()
+    n, M, N = w["nx"], w["M"], w["nx"] ** 3
+    s = mocca_amd.Solve(n, n, n, w["dx"], w["dv"], M, time_reversal=False)
+    s.set_states_meta(w["qn"], w["occ"])
+    s.upload_states(w["psi"])
+    F = w["fields"]
+    s.set_potentials(F["U"], F["B"], F["NablaB"], F["W"], F["DivW"], w["kin"],
+                     S=F["S"], A=F["A"], DivA=F["DivA"])
+    # densities of the input states
+    d = s.compute_density()
+    mesh = Mesh(n, n, n, w["dx"], False)
+    der = np.zeros((M, 3, 4, N))
+    lap = np.zeros((M, 4, N))
+    for i, q in enumerate(w["qn"]):
+        der[i], lap[i] = mesh.derive_spinor(w["psi"][i], q[1], q[2], q[3])
+    rho, tau, nj, vj, vs = density(w["psi"], der, w["qn"], w["occ"], False, N)
+    for k, ref in (("rho", rho), ("tau", tau), ("nablaJ", nj), ("vecj", vj), ("vecs", vs)):
+        assert relerr(d[k], ref) < 1e-11, k
+    # hPsi of a few states
+    Ff = {k: np.ascontiguousarray(v).reshape(-1) for k, v in F.items()}
+    for i in (0, 57, 133, M - 1):
+        it = (w["qn"][i][0] + 3) // 2 - 1
+        ref = or_hpsi(w["psi"][i], der[i], lap[i], it, False, w["kin"][it], Ff, N)
+        assert relerr(s.hpsi(i), ref) < 1e-11
+    # one GradDesc
+    s.grad_desc(w["dt_over_hbar"], w["momentum"])
+    psi = s.download_states()
+    e, _ = s.energies()
+    ref_psi, _, ref_e, _ = graddesc(w["psi"], der, lap, np.zeros_like(w["psi"]), w["qn"],
+                                    False, w["kin"], Ff, w["dt_over_hbar"],
+                                    w["momentum"], w["dv"])
+    assert relerr(e, ref_e) < 1e-11
+    assert relerr(psi, ref_psi.reshape(M, 4, N)) < 1e-10
+    s.close()
