@@ -45,6 +45,9 @@ struct Buffers {
     FieldJob *jobs = nullptr;
     Chunk *one_chunk = nullptr;
     int *one_order = nullptr;
+    Task *tasks = nullptr, *one_tasks = nullptr;
+    Chunk *dchunks = nullptr;
+    DTask *dtasks = nullptr;
 };
 
 }  // namespace
@@ -64,6 +67,10 @@ struct mocca_solve {
     int nchunks = 0, ntiles = 0;
     int nblk = 0, max_m = 0, nsplit = 1;
     int njobs = 0;
+    // z-marching FD kernels: y-tile rows, tiles per plane, task count
+    bool march = false;
+    int TY = 0, ntile_y = 0, ntasks = 0;
+    int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
     // profiling
     bool prof = false;
     std::vector<Pending> pending;
@@ -333,13 +340,13 @@ template <typename T>
 int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_mode) {
     const long K4 = 4L * h->N;
     int rc = run(h, "gram", [&] {
-        launch_gram<T>(src, K4, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m,
-                       h->b.gpart, h->nsplit, h->st);
+        launch_gram<T>(src, K4, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m, h->mmax,
+                       h->gtile, h->b.gpart, h->nsplit, h->st);
     });
     if (rc) return rc;
     rc = run(h, "gram_reduce", [&] {
-        launch_gram_reduce(h->b.gpart, h->b.blk_m, h->nblk, h->max_m, h->nsplit,
-                           h->mesh.dv, h->b.G, h->st);
+        launch_gram_reduce(h->b.gpart, h->b.blk_m, h->nblk, h->max_m, h->mmax, h->gtile,
+                           h->nsplit, h->mesh.dv, h->b.G, h->st);
     });
     if (rc) return rc;
     rc = run(h, "cholesky", [&] {
@@ -358,15 +365,27 @@ template <typename T>
 int grad_desc_impl(mocca_solve *h, double dth, double mu) {
     const MeshK<T> &m = mesh_of<T>(h);
     T *psi = (T *)h->b.psi, *psi2 = (T *)h->b.psi2, *upd = (T *)h->b.upd;
-    int rc = run(h, "hpsi_step", [&] {
-        launch_hpsi_step<T>(m, psi, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
-                            h->b.order, h->b.chunks, h->nchunks, h->mesh.time_reversal,
-                            T(h->kin[0]), T(h->kin[1]), T(dth), T(mu), kStep,
-                            h->b.eparts, h->ntiles, h->st);
-    });
+    int rc, slots;
+    if (h->march) {
+        slots = h->ntile_y;
+        rc = run(h, "hpsi_step", [&] {
+            launch_hpsi_fd<T>(m, psi, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
+                              h->b.tasks, h->ntasks, h->TY, h->ntile_y,
+                              h->mesh.time_reversal, T(h->kin[0]), T(h->kin[1]), T(dth),
+                              T(mu), kStep, h->b.eparts, h->st);
+        });
+    } else {
+        slots = h->ntiles;
+        rc = run(h, "hpsi_step", [&] {
+            launch_hpsi_step<T>(m, psi, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
+                                h->b.order, h->b.chunks, h->nchunks, h->mesh.time_reversal,
+                                T(h->kin[0]), T(h->kin[1]), T(dth), T(mu), kStep,
+                                h->b.eparts, h->ntiles, h->st);
+        });
+    }
     if (rc) return rc;
     rc = run(h, "reduce_ed", [&] {
-        launch_reduce_ed(h->b.eparts, h->ntiles, h->b.all_states, h->M, h->mesh.dv,
+        launch_reduce_ed(h->b.eparts, slots, h->b.all_states, h->M, h->mesh.dv,
                          h->b.E, h->b.D, h->st);
     });
     if (rc) return rc;
@@ -377,6 +396,18 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
 template <typename T>
 int hpsi_one_impl(mocca_solve *h, int state) {
     const MeshK<T> &m = mesh_of<T>(h);
+    if (h->march) {
+        std::vector<Task> tk(h->ntile_y);
+        for (int t = 0; t < h->ntile_y; ++t) tk[t] = Task{state, t};
+        HIPCHK(h, hipMemcpy(h->b.one_tasks, tk.data(), tk.size() * sizeof(Task),
+                            hipMemcpyHostToDevice));
+        return run(h, "hpsi", [&] {
+            launch_hpsi_fd<T>(m, (const T *)h->b.psi, nullptr, nullptr, (T *)h->b.hbuf,
+                              (const T *)h->b.fields, h->b.qn, h->b.one_tasks, h->ntile_y,
+                              h->TY, h->ntile_y, h->mesh.time_reversal, T(h->kin[0]),
+                              T(h->kin[1]), T(0), T(0), kWriteH, nullptr, h->st);
+        });
+    }
     Chunk c{0, 1, (h->qn[4 * state] + 3) / 2 - 1, 0};
     HIPCHK(h, hipMemcpy(h->b.one_chunk, &c, sizeof c, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->b.one_order, &state, sizeof(int), hipMemcpyHostToDevice));
@@ -392,16 +423,30 @@ template <typename T>
 int density_impl(mocca_solve *h) {
     const MeshK<T> &m = mesh_of<T>(h);
     const int nf = h->mesh.time_reversal ? 3 : 9;
-    int rc = run(h, "density", [&] {
-        launch_density<T>(m, (const T *)h->b.psi, h->b.qn, h->b.occ, h->b.order,
-                          h->b.chunks, h->nchunks, h->mesh.time_reversal,
-                          (T *)h->b.dpart, h->st);
-    });
-    if (rc) return rc;
-    rc = run(h, "density_reduce", [&] {
-        launch_density_reduce<T>((const T *)h->b.dpart, h->b.chunks, h->nchunks, nf, h->N,
-                                 h->b.dens, h->st);
-    });
+    int rc;
+    if (h->march) {
+        rc = run(h, "density", [&] {
+            launch_density_fd<T>(m, (const T *)h->b.psi, h->b.qn, h->b.occ, h->b.order,
+                                 h->b.dchunks, h->b.dtasks, h->ndtasks, h->TY,
+                                 h->mesh.time_reversal, (T *)h->b.dpart, h->st);
+        });
+        if (rc) return rc;
+        rc = run(h, "density_reduce", [&] {
+            launch_density_reduce<T>((const T *)h->b.dpart, h->b.dchunks, h->ndchunks, nf,
+                                     h->N, h->b.dens, h->st);
+        });
+    } else {
+        rc = run(h, "density", [&] {
+            launch_density<T>(m, (const T *)h->b.psi, h->b.qn, h->b.occ, h->b.order,
+                              h->b.chunks, h->nchunks, h->mesh.time_reversal,
+                              (T *)h->b.dpart, h->st);
+        });
+        if (rc) return rc;
+        rc = run(h, "density_reduce", [&] {
+            launch_density_reduce<T>((const T *)h->b.dpart, h->b.chunks, h->nchunks, nf,
+                                     h->N, h->b.dens, h->st);
+        });
+    }
     if (rc) return rc;
     return run(h, "field_jobs", [&] {
         launch_field_jobs(h->md, h->b.jobs, h->njobs, h->st);
@@ -449,7 +494,7 @@ void free_all(mocca_solve *h) {
                     b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.eparts,
                     b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
                     b.G, b.Tm, b.work, b.gpart, b.status, b.jobs, b.one_chunk,
-                    b.one_order};
+                    b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &p : h->pending) { h->pool.push_back(p.a); h->pool.push_back(p.b); }
@@ -519,6 +564,10 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     TRY(dalloc(h, &b.one_chunk, sizeof(Chunk)));
     TRY(dalloc(h, &b.one_order, sizeof(int)));
     h->ntiles = (int)((N + kBlock - 1) / kBlock);
+    h->march = !mesh->lagrange && mesh->nx <= 64;
+    h->TY = plane_tile_rows(mesh->nx, mesh->ny);
+    h->ntile_y = (mesh->ny + h->TY - 1) / h->TY;
+    TRY(dalloc(h, &b.one_tasks, h->ntile_y * sizeof(Task)));
     TRY(dalloc(h, &b.eparts, M * (size_t)h->ntiles * 2 * sizeof(double)));
     HIPCHK(h, hipMemset(b.upd, 0, M * 4 * N * ts));
     HIPCHK(h, hipMemset(b.fields, 0, 48 * N * ts));
@@ -599,6 +648,32 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
             chunks.push_back(c);
         }
     }
+    // Density chunks (kDensChunk states of one isospin) over the same order.
+    std::vector<Chunk> dchunks;
+    for (const Chunk &c : chunks) {
+        if (!dchunks.empty() && dchunks.back().it == c.it &&
+            dchunks.back().count + c.count <= kDensChunk)
+            dchunks.back().count += c.count;
+        else
+            dchunks.push_back(c);
+    }
+    // Density tasks: (tile, slab, chunk); all tasks of one chunk on one XCD.
+    std::vector<DTask> dtasks;
+    {
+        const int nslab = (h->mesh.nz + kDensSlab - 1) / kDensSlab;
+        std::vector<std::vector<DTask>> q(8);
+        for (int c = 0; c < (int)dchunks.size(); ++c)
+            for (int t = 0; t < h->ntile_y; ++t)
+                for (int sl = 0; sl < nslab; ++sl) q[c % 8].push_back(DTask{t, sl, c, 0});
+        std::vector<size_t> pos(8, 0);
+        size_t total = 0;
+        for (auto &v : q) total += v.size();
+        for (size_t w = 0; dtasks.size() < total; ++w) {
+            int k = (int)(w % 8);
+            for (int tries = 0; tries < 8 && pos[k] >= q[k].size(); ++tries) k = (k + 1) % 8;
+            dtasks.push_back(q[k][pos[k]++]);
+        }
+    }
     // Gram-Schmidt blocks: (isospin, parity, signature) (Spwf.f90:842-866).
     std::vector<std::vector<int>> blocks;
     for (int iso = -1; iso <= 1; iso += 2)
@@ -625,14 +700,33 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     }
     std::vector<int> all(M);
     for (int i = 0; i < M; ++i) all[i] = i;
+    // Task list of the z-marching kernels, XCD-aware: workgroup w runs on XCD
+    // w % 8, so all tasks of one (isospin, y-tile) pair go to one XCD and share
+    // that XCD's L2 copy of the tile's potentials.
+    std::vector<Task> tasks;
+    {
+        std::vector<std::vector<Task>> q(8);
+        for (int i = 0; i < M; ++i) {
+            const int it = (qn[4 * i] + 3) / 2 - 1;
+            for (int t = 0; t < h->ntile_y; ++t) q[(it * h->ntile_y + t) % 8].push_back(Task{i, t});
+        }
+        std::vector<size_t> pos(8, 0);
+        const size_t total = (size_t)M * h->ntile_y;
+        for (size_t w = 0; tasks.size() < total; ++w) {
+            int k = (int)(w % 8);
+            for (int tries = 0; tries < 8 && pos[k] >= q[k].size(); ++tries) k = (k + 1) % 8;
+            tasks.push_back(q[k][pos[k]++]);
+        }
+    }
     // (Re)allocate index and GS buffers.
     void *old[] = {b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
-                   b.G, b.Tm, b.work, b.gpart, b.dpart};
+                   b.G, b.Tm, b.work, b.gpart, b.dpart, b.tasks, b.dchunks, b.dtasks};
     for (void *p : old)
         if (p) (void)hipFree(p);
     b.order = nullptr; b.chunks = nullptr; b.all_states = nullptr;
     b.blk_states = nullptr; b.blk_m = nullptr; b.perm = nullptr;
-    b.G = b.Tm = b.work = b.gpart = nullptr; b.dpart = nullptr;
+    b.G = b.Tm = b.work = b.gpart = nullptr; b.dpart = nullptr; b.tasks = nullptr;
+    b.dchunks = nullptr; b.dtasks = nullptr;
     int rc;
     if ((rc = dalloc(h, &b.order, order.size() * sizeof(int)))) return rc;
     if ((rc = dalloc(h, &b.chunks, chunks.size() * sizeof(Chunk)))) return rc;
@@ -644,15 +738,32 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     if ((rc = dalloc(h, &b.G, gsz))) return rc;
     if ((rc = dalloc(h, &b.Tm, gsz))) return rc;
     if ((rc = dalloc(h, &b.work, gsz))) return rc;
-    const int nt = max_m / kGramTile, ntile_max = nt * (nt + 1) / 2;
+    h->gtile = mmax <= 32 ? 32 : 64;
+    // Tiles are counted from the largest block (the leading dimension max_m
+    // may be larger); each block gets enough K splits to fill the GPU.
+    const int nt = (mmax + h->gtile - 1) / h->gtile, ntile_max = nt * (nt + 1) / 2;
+    h->mmax = mmax;
     const long K4 = 4L * h->N;
     int nsplit = std::max(1, 2048 / std::max(1, nblk * ntile_max));
-    nsplit = (int)std::min<long>(nsplit, std::max<long>(1, K4 / (kGramKC * 16)));
+    nsplit = (int)std::min<long>(nsplit, std::max<long>(1, K4 / (kGramKC * 8)));
     h->nsplit = nsplit;
-    if ((rc = dalloc(h, &b.gpart, (size_t)nblk * ntile_max * nsplit * kGramTile * kGramTile *
+    if ((rc = dalloc(h, &b.gpart, (size_t)nblk * ntile_max * nsplit * h->gtile * h->gtile *
                                       sizeof(double))))
         return rc;
-    if ((rc = dalloc(h, &b.dpart, chunks.size() * 9 * (size_t)h->N * h->tsize))) return rc;
+    const size_t npart = std::max(chunks.size(), dchunks.size());
+    if ((rc = dalloc(h, &b.dpart, npart * 9 * (size_t)h->N * h->tsize))) return rc;
+    if ((rc = dalloc(h, &b.dchunks, dchunks.size() * sizeof(Chunk)))) return rc;
+    if ((rc = dalloc(h, &b.dtasks, dtasks.size() * sizeof(DTask)))) return rc;
+    HIPCHK(h, hipMemcpy(b.dchunks, dchunks.data(), dchunks.size() * sizeof(Chunk),
+                        hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(b.dtasks, dtasks.data(), dtasks.size() * sizeof(DTask),
+                        hipMemcpyHostToDevice));
+    h->ndchunks = (int)dchunks.size();
+    h->ndtasks = (int)dtasks.size();
+    if ((rc = dalloc(h, &b.tasks, tasks.size() * sizeof(Task)))) return rc;
+    HIPCHK(h, hipMemcpy(b.tasks, tasks.data(), tasks.size() * sizeof(Task),
+                        hipMemcpyHostToDevice));
+    h->ntasks = (int)tasks.size();
     HIPCHK(h, hipMemcpy(b.order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(b.chunks, chunks.data(), chunks.size() * sizeof(Chunk),
                         hipMemcpyHostToDevice));

@@ -50,6 +50,39 @@ struct FieldJob {
 
 enum HpsiMode { kWriteH = 1, kStep = 2 };
 
+// Field buffer offsets, in units of N (one contiguous buffer per handle):
+// U[2] B[2] NablaB[2][3] W[2][nu][mu] DivW[2][3] S[2][3] A[2][3] DivA[2]
+constexpr int kOffU = 0, kOffB = 2, kOffNB = 4, kOffW = 10, kOffDW = 28,
+              kOffS = 34, kOffA = 40, kOffDA = 46, kNumFieldSlots = 48;
+
+// One workgroup task of the z-marching stencil kernels: a y-tile of one
+// state (all x, rows [tile*TY, tile*TY+TY), all z).
+struct Task {
+    int state, tile;
+};
+
+// Density task of the z-marching density kernel.
+struct DTask {
+    int tile, slab, chunk, pad;
+};
+constexpr int kDensSlab = 4;     // z planes per density task
+constexpr int kDensChunk = 16;   // states per density chunk
+
+// Rows per y-tile of the z-marching kernels (x spans the full line).
+inline int plane_tile_rows(int nx, int ny) {
+    int ty = kBlock / nx;
+    return ty < ny ? ty : ny;
+}
+
+// Finite-difference hPsi (+ heavy-ball step) marching along z: register
+// queue of 9 planes per component, LDS tile of the current plane with y
+// halos.  Requires nx <= kBlock.
+template <typename T>
+void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
+                    const T *fields, const int *qn, const Task *tasks, int ntasks,
+                    int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
+                    int mode, double *eparts, hipStream_t st);
+
 template <typename T>
 void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
                        double *der, double *lap, hipStream_t st);
@@ -77,14 +110,16 @@ void launch_density_reduce(const T *part, const Chunk *chunks, int nchunks,
 void launch_field_jobs(const MeshK<double> &m, const FieldJob *jobs, int njobs,
                        hipStream_t st);
 
+// Gram matrices with TILE x TILE output tiles (tile = 32 or 64).
+// max_m: leading dimension of the per-block arrays; mmax: largest block.
 template <typename T>
 void launch_gram(const T *psi, long K4, const int *blk_states, const int *blk_m,
-                 int nblk, int max_m, double *partial, int nsplit,
-                 hipStream_t st);
+                 int nblk, int max_m, int mmax, int tile, double *partial,
+                 int nsplit, hipStream_t st);
 
 void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
-                        int max_m, int nsplit, double dv, double *G,
-                        hipStream_t st);
+                        int max_m, int mmax, int tile, int nsplit, double dv,
+                        double *G, hipStream_t st);
 
 void launch_cholesky_ld(const double *G, const int *blk_states,
                         const int *blk_m, int nblk, int max_m, const double *E,
@@ -95,5 +130,10 @@ template <typename T>
 void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
                      const int *blk_m, int nblk, int max_m, const double *Tm,
                      int update_mode, hipStream_t st);
+
+template <typename T>
+void launch_density_fd(const MeshK<T> &m, const T *psi, const int *qn, const double *occ,
+                       const int *order, const Chunk *dchunks, const DTask *tasks,
+                       int ntasks, int TY, int trc, T *part, hipStream_t st);
 
 }  // namespace mocca

@@ -6,10 +6,6 @@
 
 namespace mocca {
 
-// Field buffer offsets, in units of N (one contiguous buffer per handle):
-// U[2] B[2] NablaB[2][3] W[2][nu][mu] DivW[2][3] S[2][3] A[2][3] DivA[2]
-constexpr int kOffU = 0, kOffB = 2, kOffNB = 4, kOffW = 10, kOffDW = 28,
-              kOffS = 34, kOffA = 40, kOffDA = 46, kNumFieldSlots = 48;
 
 // ---------------------------------------------------------------------------
 // Sign tables of the EV8 octant (OptimizedDerivatives.f90:24-44,115-132,
@@ -474,66 +470,72 @@ __device__ __forceinline__ bool tile_of(int tix, int nt, int &ti, int &tj) {
     return false;
 }
 
-template <typename T>
+template <typename T, int TILE>
 __global__ __launch_bounds__(kBlock) void k_gram(const T *__restrict__ psi, long K4,
                                                  const int *__restrict__ blk_states,
                                                  const int *__restrict__ blk_m,
                                                  int max_m, double *__restrict__ partial,
                                                  int nsplit, int ntile_max) {
+    constexpr int R = TILE / 16;  // register tile edge per thread
     const int w = blockIdx.x, tix = blockIdx.y, b = blockIdx.z;
     const int m = blk_m[b];
     int ti, tj;
-    if (!tile_of(tix, (m + kGramTile - 1) / kGramTile, ti, tj)) return;
-    __shared__ double As[kGramKC][kGramTile + 2];
-    __shared__ double Bs[kGramKC][kGramTile + 2];
+    if (!tile_of(tix, (m + TILE - 1) / TILE, ti, tj)) return;
+    __shared__ double As[kGramKC][TILE + 2];
+    __shared__ double Bs[kGramKC][TILE + 2];
     const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
-    double acc[4][4];
+    double acc[R][R];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0;
+        for (int j = 0; j < R; ++j) acc[i][j] = 0;
     const long kw = ((K4 + nsplit - 1) / nsplit + kGramKC - 1) / kGramKC * kGramKC;
     const long k0 = (long)w * kw, k1 = min(K4, k0 + kw);
     const int *st = blk_states + (long)b * max_m;
     for (long kb = k0; kb < k1; kb += kGramKC) {
-        for (int q = tid; q < kGramTile * kGramKC; q += kBlock) {
+        for (int q = tid; q < TILE * kGramKC; q += kBlock) {
             const int r = q / kGramKC, kk = q % kGramKC;
             const long k = kb + kk;
-            const int ra = ti * kGramTile + r, rb = tj * kGramTile + r;
+            const int ra = ti * TILE + r, rb = tj * TILE + r;
             As[kk][r] = (ra < m && k < k1) ? (double)psi[(long)st[ra] * K4 + k] : 0.0;
             Bs[kk][r] = (rb < m && k < k1) ? (double)psi[(long)st[rb] * K4 + k] : 0.0;
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kGramKC; ++kk) {
-            double a[4], bb[4];
+            double a[R], bb[R];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) { a[i] = As[kk][tr * 4 + i]; bb[i] = Bs[kk][tc * 4 + i]; }
+            for (int i = 0; i < R; ++i) { a[i] = As[kk][tr * R + i]; bb[i] = Bs[kk][tc * R + i]; }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < R; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * bb[j];
+                for (int j = 0; j < R; ++j) acc[i][j] += a[i] * bb[j];
         }
         __syncthreads();
     }
-    double *out = partial + (((long)b * ntile_max + tix) * nsplit + w) * kGramTile * kGramTile;
+    double *out = partial + (((long)b * ntile_max + tix) * nsplit + w) * TILE * TILE;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) out[(tr * 4 + i) * kGramTile + tc * 4 + j] = acc[i][j];
+        for (int j = 0; j < R; ++j) out[(tr * R + i) * TILE + tc * R + j] = acc[i][j];
 }
 
 template <typename T>
 void launch_gram(const T *psi, long K4, const int *blk_states, const int *blk_m,
-                 int nblk, int max_m, double *partial, int nsplit,
-                 hipStream_t st) {
-    const int nt = (max_m + kGramTile - 1) / kGramTile;
+                 int nblk, int max_m, int mmax, int tile, double *partial,
+                 int nsplit, hipStream_t st) {
+    const int nt = (mmax + tile - 1) / tile;
     const int ntile_max = nt * (nt + 1) / 2;
     dim3 g(nsplit, ntile_max, nblk);
-    hipLaunchKernelGGL(k_gram<T>, g, dim3(kBlock), 0, st, psi, K4, blk_states, blk_m,
-                       max_m, partial, nsplit, ntile_max);
+    if (tile == 32)
+        hipLaunchKernelGGL((k_gram<T, 32>), g, dim3(kBlock), 0, st, psi, K4, blk_states,
+                           blk_m, max_m, partial, nsplit, ntile_max);
+    else
+        hipLaunchKernelGGL((k_gram<T, 64>), g, dim3(kBlock), 0, st, psi, K4, blk_states,
+                           blk_m, max_m, partial, nsplit, ntile_max);
 }
 
+template <int TILE>
 __global__ __launch_bounds__(kBlock) void k_gram_reduce(const double *__restrict__ partial,
                                                         const int *__restrict__ blk_m,
                                                         int max_m, int nsplit,
@@ -542,15 +544,15 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce(const double *__restrict
     const int tix = blockIdx.y, b = blockIdx.z;
     const int m = blk_m[b];
     int ti, tj;
-    if (!tile_of(tix, (m + kGramTile - 1) / kGramTile, ti, tj)) return;
+    if (!tile_of(tix, (m + TILE - 1) / TILE, ti, tj)) return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= kGramTile * kGramTile) return;
-    const int r = e / kGramTile, c = e % kGramTile;
-    const int i = ti * kGramTile + r, j = tj * kGramTile + c;
+    if (e >= TILE * TILE) return;
+    const int r = e / TILE, c = e % TILE;
+    const int i = ti * TILE + r, j = tj * TILE + c;
     if (i >= m || j >= m) return;
-    const double *src = partial + ((long)b * ntile_max + tix) * nsplit * kGramTile * kGramTile + e;
+    const double *src = partial + ((long)b * ntile_max + tix) * nsplit * TILE * TILE + e;
     double s = 0;
-    for (int w = 0; w < nsplit; ++w) s += src[(long)w * kGramTile * kGramTile];
+    for (int w = 0; w < nsplit; ++w) s += src[(long)w * TILE * TILE];
     s *= dv;
     double *g = G + (long)b * max_m * max_m;
     g[(long)i * max_m + j] = s;
@@ -558,13 +560,17 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce(const double *__restrict
 }
 
 void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
-                        int max_m, int nsplit, double dv, double *G,
-                        hipStream_t st) {
-    const int nt = (max_m + kGramTile - 1) / kGramTile;
+                        int max_m, int mmax, int tile, int nsplit, double dv,
+                        double *G, hipStream_t st) {
+    const int nt = (mmax + tile - 1) / tile;
     const int ntile_max = nt * (nt + 1) / 2;
-    dim3 g((kGramTile * kGramTile + kBlock - 1) / kBlock, ntile_max, nblk);
-    hipLaunchKernelGGL(k_gram_reduce, g, dim3(kBlock), 0, st, partial, blk_m, max_m,
-                       nsplit, ntile_max, dv, G);
+    dim3 g((tile * tile + kBlock - 1) / kBlock, ntile_max, nblk);
+    if (tile == 32)
+        hipLaunchKernelGGL(k_gram_reduce<32>, g, dim3(kBlock), 0, st, partial, blk_m, max_m,
+                           nsplit, ntile_max, dv, G);
+    else
+        hipLaunchKernelGGL(k_gram_reduce<64>, g, dim3(kBlock), 0, st, partial, blk_m, max_m,
+                           nsplit, ntile_max, dv, G);
 }
 
 // Per block: ascending-energy order (the stable insertion sort of
@@ -705,7 +711,7 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
     template void launch_density_reduce<T>(const T *, const Chunk *, int, int, int,      \
                                            double *, hipStream_t);                       \
     template void launch_gram<T>(const T *, long, const int *, const int *, int, int,    \
-                                 double *, int, hipStream_t);                            \
+                                 int, int, double *, int, hipStream_t);                  \
     template void launch_gs_apply<T>(const T *, T *, T *, long, const int *,             \
                                      const int *, int, int, const double *, int,         \
                                      hipStream_t);

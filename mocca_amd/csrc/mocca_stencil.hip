@@ -1,0 +1,476 @@
+// Z-marching finite-difference kernels for the EV8 octant (gfx950, wave64).
+//
+// One workgroup owns a y-tile (full x lines, TY rows) of one state and walks
+// along z.  The 9-point z stencil comes from a per-thread register queue of
+// 9 planes per spinor component; the x and y stencils read the current plane
+// from LDS, where the tile is stored with 4-row y halos.  Reflection ghosts
+// (below the symmetry plane) and zeros (beyond the box) follow the EV8 rules
+// of derx/dery/derz and lapla_{X,Y,Z}_ev8 (OptimizedDerivatives.f90:46-113,
+// 230-304, 362-430, 617-795).  Each state is read from HBM once per launch.
+#include "mocca_kernels.h"
+
+namespace mocca {
+
+namespace {
+
+__device__ __forceinline__ int comp_sign_x(int c, int S) { return (c == 0 || c == 3) ? S : -S; }
+__device__ __forceinline__ int comp_sign_y(int c, int TS) { return (c == 0 || c == 2) ? TS : -TS; }
+__device__ __forceinline__ int comp_sign_z(int c, int S, int P) { return c < 2 ? S * P : -S * P; }
+
+// 7-point first and 9-point second derivative from the 9 values v[0..8]
+// centred on v[4].
+template <typename T>
+__device__ __forceinline__ void fd_pair(const MeshK<T> &m, const T v[9], T &d1, T &d2) {
+    d1 = (T(45) * (v[5] - v[3]) - T(9) * (v[6] - v[2]) + (v[7] - v[1])) * m.inv60dx;
+    d2 = m.l2scale * (m.l2m * v[4] + (v[5] + v[3]) + m.l2c2 * (v[6] + v[2]) +
+                      m.l2c3 * (v[7] + v[1]) + m.l2c4 * (v[8] + v[0]));
+}
+
+__device__ __forceinline__ void block_sum2_fd(double &a, double &b, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { sh[2 * w] = a; sh[2 * w + 1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int k = 0; k < kBlock / 64; ++k) { sa += sh[2 * k]; sb += sh[2 * k + 1]; }
+        a = sa; b = sb;
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+// LDS row layout of the z-marching kernels: one row holds x = -G .. nx+G-1
+// at offsets 0 .. nx+2G-1: the G left cells are the mirrored reflection
+// ghosts (with the component's sign), the G right cells are zeros, so the
+// x stencils read without branches.
+template <typename T, int HMAX, bool TRC>
+__global__ __launch_bounds__(kBlock) void k_hpsi_fd(
+    MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
+    T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
+    const int *__restrict__ qn, const Task *__restrict__ tasks, int TY,
+    int ntile_y, T kin0, T kin1, T dth, T mu, int mode,
+    double *__restrict__ eparts) {
+    constexpr int G = 4;  // ghost width; HMAX = halo loads per thread (32 nx / 256)
+    constexpr bool trc = TRC;
+    extern __shared__ double smem_d[];
+    T *sp = reinterpret_cast<T *>(smem_d);  // [4][TY+8][nx+8]
+    __shared__ double red[2 * kBlock / 64];
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const long N = m.N, plane = (long)nx * ny;
+    const int W = nx + 2 * G;
+    const Task tk = tasks[blockIdx.x];
+    const int s = tk.state, tile = tk.tile;
+    const int it = (qn[4 * s] + 3) / 2 - 1;
+    const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
+    const T kin = it == 0 ? kin0 : kin1;
+    const int tid = threadIdx.x, tx = tid % nx, tyl = tid / nx;
+    const int y0 = tile * TY, y = y0 + tyl;
+    const bool active = (tyl < TY) && (y < ny);
+    const int rows = TY + 8;
+    T sgx[4], sgy[4], sgz[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        sgx[c] = T(comp_sign_x(c, S));
+        sgy[c] = T(comp_sign_y(c, TS));
+        sgz[c] = T(comp_sign_z(c, S, P));
+    }
+    const T *ps = psi + (long)s * 4 * N;
+    const long col = (long)(active ? y : 0) * nx + tx;
+    T q[4][9];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int zz = k - 4;
+            const int zi = zz < 0 ? -1 - zz : (zz < nz ? zz : nz - 1);
+            const T val = ps[c * N + zi * plane + col];
+            q[c][k] = !active || zz >= nz ? T(0) : (zz < 0 ? sgz[c] * val : val);
+        }
+    const T *Fu = F + (long)(kOffU + it) * N, *Fb = F + (long)(kOffB + it) * N;
+    const T *Fnb = F + (long)(kOffNB + 3 * it) * N, *Fw = F + (long)(kOffW + 9 * it) * N;
+    const T *Fdw = F + (long)(kOffDW + 3 * it) * N, *Fs = F + (long)(kOffS + 3 * it) * N;
+    const T *Fa = F + (long)(kOffA + 3 * it) * N, *Fda = F + (long)(kOffDA + it) * N;
+    T *po = psi_out ? psi_out + (long)s * 4 * N : nullptr;
+    T *us = upd ? upd + (long)s * 4 * N : nullptr;
+    double eacc = 0, dacc = 0;
+    const int nhalo = 4 * 8 * nx;
+    for (int z = 0; z < nz; ++z) {
+        // ---- all global loads of this plane first ----
+        T hv[HMAX];
+#pragma unroll
+        for (int r = 0; r < HMAX; ++r) {
+            const int idx = tid + r * kBlock;
+            const int c = idx / (8 * nx), rem = idx % (8 * nx);
+            const int hr = rem / nx, xx = rem % nx;
+            const int yy = y0 - 4 + (hr < 4 ? hr : TY + hr);
+            const int yi = yy < 0 ? -1 - yy : (yy < ny ? yy : ny - 1);
+            const int cc = idx < nhalo ? c : 0;
+            hv[r] = ps[cc * N + z * plane + yi * nx + xx];
+        }
+        const long p = (long)z * plane + col;
+        const int zn = z + 5 < nz ? z + 5 : nz - 1;
+        T nxt[4], uo[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) nxt[c] = ps[c * N + zn * plane + col];
+        const T fu = Fu[p], fb = Fb[p] + kin;
+        const T nb0 = Fnb[p], nb1 = Fnb[N + p], nb2 = Fnb[2 * N + p];
+        T fw[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int mu_ = 0; mu_ < 3; ++mu_) fw[a][mu_] = Fw[(a * 3 + mu_) * N + p];
+        const T hx = T(0.5) * Fdw[p], hy = T(0.5) * Fdw[N + p], hz = T(0.5) * Fdw[2 * N + p];
+        T s0 = 0, s1 = 0, s2 = 0, a0 = 0, a1 = 0, a2 = 0, hda = 0;
+        if (!trc) {
+            s0 = Fs[p]; s1 = Fs[N + p]; s2 = Fs[2 * N + p];
+            a0 = Fa[p]; a1 = Fa[N + p]; a2 = Fa[2 * N + p];
+            hda = T(0.5) * Fda[p];
+        }
+        if (mode & kStep) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) uo[c] = us[c * N + p];
+        }
+        // ---- current plane -> LDS with ghosts and pads ----
+        if (tyl < TY) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                T *row = sp + (c * rows + tyl + 4) * W;
+                const T v = active ? q[c][4] : T(0);
+                row[tx + G] = v;
+                if (tx < G) row[G - 1 - tx] = sgx[c] * v;
+                if (tx >= nx - G) row[tx + 2 * G] = T(0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < HMAX; ++r) {
+            const int idx = tid + r * kBlock;
+            if (idx < nhalo) {
+                const int c = idx / (8 * nx), rem = idx % (8 * nx);
+                const int hr = rem / nx, xx = rem % nx;
+                const int rr = hr < 4 ? hr : TY + hr;
+                const int yy = y0 - 4 + rr;
+                const T v = yy < 0 ? sgy[c] * hv[r] : (yy < ny ? hv[r] : T(0));
+                T *row = sp + (c * rows + rr) * W;
+                row[xx + G] = v;
+                if (xx < G) row[G - 1 - xx] = sgx[c] * v;
+                if (xx >= nx - G) row[xx + 2 * G] = T(0);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            // Component by component: derivatives, then their contributions to
+            // h (ActionOfBNew, ActionOfU: MeanFields.f90:792-851), to the
+            // spin-orbit sum aw (ActionOfW, :1029-1065) and to the current
+            // term aa (ActionOfA, :1107-1129).
+            T h[4] = {0, 0, 0, 0}, aw[4] = {0, 0, 0, 0}, aa[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const T *row = sp + (c * rows + tyl + 4) * W + tx + G;
+                T w[9], dxc, dyc, dzc, lx, ly, lz;
+#pragma unroll
+                for (int k = -4; k <= 4; ++k) w[k + 4] = row[k];
+                fd_pair(m, w, dxc, lx);
+#pragma unroll
+                for (int k = -4; k <= 4; ++k) w[k + 4] = row[k * W];
+                fd_pair(m, w, dyc, ly);
+                fd_pair(m, q[c], dzc, lz);
+                const T lc = lx + ly + lz, vc = q[c][4];
+                h[c] += -fb * lc - (nb0 * dxc + nb1 * dyc + nb2 * dzc) + fu * vc;
+                const T ex = fw[0][0] * dxc + fw[0][1] * dyc + fw[0][2] * dzc + hx * vc;
+                const T ey = fw[1][0] * dxc + fw[1][1] * dyc + fw[1][2] * dzc + hy * vc;
+                const T ez = fw[2][0] * dxc + fw[2][1] * dyc + fw[2][2] * dzc + hz * vc;
+                if (c == 0) { aw[0] += ez; aw[2] += ex; aw[3] += ey; }
+                if (c == 1) { aw[1] += ez; aw[2] -= ey; aw[3] += ex; }
+                if (c == 2) { aw[0] += ex; aw[1] -= ey; aw[2] -= ez; }
+                if (c == 3) { aw[0] += ey; aw[1] += ex; aw[3] -= ez; }
+                if (!trc) {
+                    // S . sigma psi (ActionOfS, :1131-1147)
+                    if (c == 0) { h[0] += s2 * vc; h[2] += s0 * vc; h[3] += s1 * vc; }
+                    if (c == 1) { h[1] += s2 * vc; h[2] -= s1 * vc; h[3] += s0 * vc; }
+                    if (c == 2) { h[0] += s0 * vc; h[1] -= s1 * vc; h[2] -= s2 * vc; }
+                    if (c == 3) { h[0] += s1 * vc; h[1] += s0 * vc; h[3] -= s2 * vc; }
+                    aa[c] = a0 * dxc + a1 * dyc + a2 * dzc + hda * vc;
+                }
+            }
+            h[0] += aw[1]; h[1] -= aw[0]; h[2] += aw[3]; h[3] -= aw[2];
+            if (!trc) { h[0] += aa[1]; h[1] -= aa[0]; h[2] += aa[3]; h[3] -= aa[2]; }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const T vc = q[c][4];
+                eacc += (double)vc * (double)h[c];
+                dacc += (double)h[c] * (double)h[c];
+            }
+            if (mode & kWriteH) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) hout[c * N + p] = h[c];
+            }
+            if (mode & kStep) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const T vc = q[c][4];
+                    po[c * N + p] = vc + (-(dth * h[c]) + mu * uo[c]);
+                    us[c * N + p] = vc;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[c][k] = q[c][k + 1];
+            q[c][8] = (active && z + 5 < nz) ? nxt[c] : T(0);
+        }
+    }
+    if (eparts) {
+        block_sum2_fd(eacc, dacc, red);
+        if (tid == 0) {
+            eparts[((long)s * ntile_y + tile) * 2] = eacc;
+            eparts[((long)s * ntile_y + tile) * 2 + 1] = dacc;
+        }
+    }
+}
+
+template <typename T>
+void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
+                    const T *fields, const int *qn, const Task *tasks, int ntasks,
+                    int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
+                    int mode, double *eparts, hipStream_t st) {
+    const size_t shm = sizeof(T) * 4 * (size_t)(TY + 8) * (m.nx + 8);
+#define MOCCA_HPSI_LAUNCH(HM, TR)                                                           \
+    hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR>), dim3(ntasks), dim3(kBlock), shm, st, m, psi,  \
+                       psi_out, upd, hout, fields, qn, tasks, TY, ntile_y, kin0, kin1, dth, \
+                       mu, mode, eparts)
+    if (m.nx <= 32) {
+        if (trc) MOCCA_HPSI_LAUNCH(4, true); else MOCCA_HPSI_LAUNCH(4, false);
+    } else {
+        if (trc) MOCCA_HPSI_LAUNCH(8, true); else MOCCA_HPSI_LAUNCH(8, false);
+    }
+#undef MOCCA_HPSI_LAUNCH
+}
+
+template void launch_hpsi_fd<double>(const MeshK<double> &, const double *, double *,
+                                     double *, double *, const double *, const int *,
+                                     const Task *, int, int, int, int, double, double,
+                                     double, double, int, double *, hipStream_t);
+template void launch_hpsi_fd<float>(const MeshK<float> &, const float *, float *, float *,
+                                    float *, const float *, const int *, const Task *, int,
+                                    int, int, int, float, float, float, float, int,
+                                    double *, hipStream_t);
+
+
+// ---------------------------------------------------------------------------
+// Densities (ComputeDensity, Densities.f90:464-534) with the same z-march:
+// a task is (y-tile, z-slab of TZ planes, chunk of states of one isospin).
+// For every state of the chunk the slab is walked with a 7-plane register
+// queue (first derivatives only) and the occupation-weighted contributions
+// are summed in registers; the chunk's partial fields go to
+// part[chunk][f][N], f = rho, tau, nabla.J [, j_x, j_y, j_z, s_x, s_y, s_z].
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T imd(const T a[4], const T b[4]) {
+    return a[0] * b[1] - a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+}
+
+template <typename T>
+__device__ __forceinline__ void sig(const T a[4], int dir, T o[4]) {
+    if (dir == 0) { o[0] = a[2]; o[1] = a[3]; o[2] = a[0]; o[3] = a[1]; }
+    else if (dir == 1) { o[0] = a[3]; o[1] = -a[2]; o[2] = -a[1]; o[3] = a[0]; }
+    else { o[0] = a[0]; o[1] = a[1]; o[2] = -a[2]; o[3] = -a[3]; }
+}
+
+template <typename T>
+__device__ __forceinline__ T fd1_7(const MeshK<T> &m, const T v[7]) {
+    return (T(45) * (v[4] - v[2]) - T(9) * (v[5] - v[1]) + (v[6] - v[0])) * m.inv60dx;
+}
+
+}  // namespace
+
+template <typename T, int NF, int TZ, int HMAX>
+__global__ __launch_bounds__(kBlock) void k_density_fd(
+    MeshK<T> m, const T *__restrict__ psi, const int *__restrict__ qn,
+    const double *__restrict__ occ, const int *__restrict__ order,
+    const Chunk *__restrict__ dchunks, const DTask *__restrict__ tasks, int TY,
+    T *__restrict__ part) {
+    constexpr int G = 3;  // ghost width; HMAX = halo loads per thread (24 nx / 256)
+    extern __shared__ double smem_d[];
+    T *sp = reinterpret_cast<T *>(smem_d);  // [4][TY+6][nx+6]
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const long N = m.N, plane = (long)nx * ny;
+    const int W = nx + 2 * G;
+    const DTask tk = tasks[blockIdx.x];
+    const Chunk ch = dchunks[tk.chunk];
+    const int tid = threadIdx.x, tx = tid % nx, tyl = tid / nx;
+    const int y0 = tk.tile * TY, y = y0 + tyl;
+    const bool active = (tyl < TY) && (y < ny);
+    const int rows = TY + 6;
+    const int z0 = tk.slab * TZ;
+    const long col = (long)(active ? y : 0) * nx + tx;
+    T acc[TZ][NF];
+#pragma unroll
+    for (int a = 0; a < TZ; ++a)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) acc[a][f] = 0;
+    const int nhalo = 4 * 6 * nx;
+    for (int q = 0; q < ch.count; ++q) {
+        const int s = order[ch.start + q];
+        if (occ[s] == 0.0) continue;
+        const T o = (T)occ[s];
+        const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
+        T sgx[4], sgy[4], sgz[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sgx[c] = T(comp_sign_x(c, S));
+            sgy[c] = T(comp_sign_y(c, TS));
+            sgz[c] = T(comp_sign_z(c, S, P));
+        }
+        const T *ps = psi + (long)s * 4 * N;
+        T qz[4][7];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const int zz = z0 - 3 + k;
+                const int zi = zz < 0 ? -1 - zz : (zz < nz ? zz : nz - 1);
+                const T val = ps[c * N + zi * plane + col];
+                qz[c][k] = !active || zz >= nz ? T(0) : (zz < 0 ? sgz[c] * val : val);
+            }
+#pragma unroll
+        for (int zl = 0; zl < TZ; ++zl) {
+            const int z = z0 + zl;
+            if (z >= nz) break;
+            T hv[HMAX];
+#pragma unroll
+            for (int r = 0; r < HMAX; ++r) {
+                const int idx = tid + r * kBlock;
+                const int c = idx / (6 * nx), rem = idx % (6 * nx);
+                const int hr = rem / nx, xx = rem % nx;
+                const int yy = y0 - 3 + (hr < 3 ? hr : TY + hr);
+                const int yi = yy < 0 ? -1 - yy : (yy < ny ? yy : ny - 1);
+                const int cc = idx < nhalo ? c : 0;
+                hv[r] = ps[cc * N + z * plane + yi * nx + xx];
+            }
+            const int zn = z + 4 < nz ? z + 4 : nz - 1;
+            T nxt[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) nxt[c] = ps[c * N + zn * plane + col];
+            if (tyl < TY) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    T *row = sp + (c * rows + tyl + 3) * W;
+                    const T v = active ? qz[c][3] : T(0);
+                    row[tx + G] = v;
+                    if (tx < G) row[G - 1 - tx] = sgx[c] * v;
+                    if (tx >= nx - G) row[tx + 2 * G] = T(0);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < HMAX; ++r) {
+                const int idx = tid + r * kBlock;
+                if (idx < nhalo) {
+                    const int c = idx / (6 * nx), rem = idx % (6 * nx);
+                    const int hr = rem / nx, xx = rem % nx;
+                    const int rr = hr < 3 ? hr : TY + hr;
+                    const int yy = y0 - 3 + rr;
+                    const T v = yy < 0 ? sgy[c] * hv[r] : (yy < ny ? hv[r] : T(0));
+                    T *row = sp + (c * rows + rr) * W;
+                    row[xx + G] = v;
+                    if (xx < G) row[G - 1 - xx] = sgx[c] * v;
+                    if (xx >= nx - G) row[xx + 2 * G] = T(0);
+                }
+            }
+            __syncthreads();
+            if (active) {
+                T v[4], d[3][4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const T *row = sp + (c * rows + tyl + 3) * W + tx + G;
+                    T w[7];
+#pragma unroll
+                    for (int k = -3; k <= 3; ++k) w[k + 3] = row[k];
+                    d[0][c] = fd1_7(m, w);
+#pragma unroll
+                    for (int k = -3; k <= 3; ++k) w[k + 3] = row[k * W];
+                    d[1][c] = fd1_7(m, w);
+                    d[2][c] = fd1_7(m, qz[c]);
+                    v[c] = qz[c][3];
+                }
+                T t[4];
+                acc[zl][0] += o * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+                T ta = 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    ta += d[a][0] * d[a][0] + d[a][1] * d[a][1] + d[a][2] * d[a][2] + d[a][3] * d[a][3];
+                acc[zl][1] += o * ta;
+                // nabla.J = sum eps_ijk Im(d_i psi^+ sigma_k d_j psi) (Spwf.f90:2206-2243)
+                T nj = 0;
+                sig(d[1], 2, t); nj += imd(d[0], t);
+                sig(d[2], 1, t); nj -= imd(d[0], t);
+                sig(d[1], 0, t); nj -= imd(d[2], t);
+                sig(d[0], 1, t); nj += imd(d[2], t);
+                sig(d[0], 2, t); nj -= imd(d[1], t);
+                sig(d[2], 0, t); nj += imd(d[1], t);
+                acc[zl][2] += o * nj;
+                if (NF == 9) {
+                    // j (Spwf.f90:2034) and s (Spwf.f90:2071)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) acc[zl][3 + a] += o * imd(v, d[a]);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        sig(v, a, t);
+                        acc[zl][6 + a] += o * (v[0] * t[0] + v[1] * t[1] + v[2] * t[2] + v[3] * t[3]);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) qz[c][k] = qz[c][k + 1];
+                qz[c][6] = (active && z + 4 < nz) ? nxt[c] : T(0);
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int zl = 0; zl < TZ; ++zl) {
+            const int z = z0 + zl;
+            if (z >= nz) break;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                part[(long)(tk.chunk * NF + f) * N + z * plane + col] = acc[zl][f];
+        }
+    }
+}
+
+template <typename T>
+void launch_density_fd(const MeshK<T> &m, const T *psi, const int *qn, const double *occ,
+                       const int *order, const Chunk *dchunks, const DTask *tasks,
+                       int ntasks, int TY, int trc, T *part, hipStream_t st) {
+    const size_t shm = sizeof(T) * 4 * (size_t)(TY + 6) * (m.nx + 6);
+    constexpr int TZ = kDensSlab;
+#define MOCCA_DENS_LAUNCH(NF, HM)                                                          \
+    hipLaunchKernelGGL((k_density_fd<T, NF, TZ, HM>), dim3(ntasks), dim3(kBlock), shm, st, m, \
+                       psi, qn, occ, order, dchunks, tasks, TY, part)
+    if (m.nx <= 32) {
+        if (trc) MOCCA_DENS_LAUNCH(3, 3); else MOCCA_DENS_LAUNCH(9, 3);
+    } else {
+        if (trc) MOCCA_DENS_LAUNCH(3, 6); else MOCCA_DENS_LAUNCH(9, 6);
+    }
+#undef MOCCA_DENS_LAUNCH
+}
+
+template void launch_density_fd<double>(const MeshK<double> &, const double *, const int *,
+                                        const double *, const int *, const Chunk *,
+                                        const DTask *, int, int, int, double *, hipStream_t);
+template void launch_density_fd<float>(const MeshK<float> &, const float *, const int *,
+                                       const double *, const int *, const Chunk *,
+                                       const DTask *, int, int, int, float *, hipStream_t);
+
+}  // namespace mocca
