@@ -16,7 +16,11 @@
 !   real8 grid(nx,ny,nz,4,nwt)
 !   real8 U(n,2) B(n,2) NablaB(n,3,2) W(n,3,3,2) DivW(n,3,2)
 !   real8 S(n,3,2) A(n,3,2) DivA(n,2)
-! Tasks (comma separated): derive, hpsi, dens, step, force, scf
+! Tasks (comma separated): derive, hpsi, dens, step, force (synthetic mean
+! fields from the input file), pot (densities -> moments -> Coulomb ->
+! potentials -> energy, reference routines), scf (the Evolve loop of
+! Main.f90:255-328 for HF with fixed dt/momentum; iteration count = niter
+! from the 4th argument).
 !-------------------------------------------------------------------------------
 program ref_driver
   use CompilationInfo
@@ -30,10 +34,16 @@ program ref_driver
   use MeanFields
   use ImaginaryTime
   use InOutput, only : ReadRunInfo
+  use Moments
+  use Coulomb
+  use Energy
+  use PotentialMixing
+  use Pairing, only : SolvePairing
+  use DensityMixing, only : MixDensities
   implicit none
 
-  character(len=512) :: infile, prefix, tasks
-  integer            :: nwtf, i, m, it, u, n
+  character(len=512) :: infile, prefix, tasks, arg4
+  integer            :: nwtf, i, m, it, uu, n, niter
   integer, allocatable :: qn(:,:)
   real(KIND=dp), allocatable :: occ(:), grids(:,:,:,:,:)
   type(Spinor)       :: sp, h
@@ -41,17 +51,20 @@ program ref_driver
   call get_command_argument(1, infile)
   call get_command_argument(2, prefix)
   call get_command_argument(3, tasks)
+  niter = 0
+  call get_command_argument(4, arg4)
+  if(len_trim(arg4).gt.0) read(arg4,*) niter
 
   call ReadRunInfo()
 
-  open(newunit=u, file=trim(infile), access='stream', form='unformatted',      &
+  open(newunit=uu, file=trim(infile), access='stream', form='unformatted',      &
   &    status='old')
-  read(u) nwtf
+  read(uu) nwtf
   if(nwtf .ne. nwt) stop 'nwt mismatch between namelist and input file'
   allocate(qn(4,nwt), occ(nwt), grids(nx,ny,nz,4,nwt))
-  read(u) qn
-  read(u) occ
-  read(u) grids
+  read(uu) qn
+  read(uu) occ
+  read(uu) grids
 
   allocate(HFBasis(nwt))
   do i=1,nwt
@@ -62,79 +75,191 @@ program ref_driver
   enddo
   DensityBasis => HFBasis
 
-  ! Mean-field arrays (allocated the way ConstructPotentials does,
-  ! PotentialMixing.f90:32-54) and filled with the synthetic fields.
-  allocate(BPot(nx,ny,nz,2), NablaBPot(nx,ny,nz,3,2), UPot(nx,ny,nz,2))
-  allocate(WPot(nx,ny,nz,3,3,2), DivWPot(nx,ny,nz,3,2))
-  allocate(SPot(nx,ny,nz,3,2), APot(nx,ny,nz,3,2), DivAPot(nx,ny,nz,2))
-  read(u) UPot, BPot, NablaBPot, WPot, DivWPot, SPot, APot, DivAPot
-  close(u)
-  ActionOfB => ActionOfBNew
+  if(index(tasks,'hpsi').gt.0 .or. index(tasks,'step').gt.0) then
+    ! Mean-field arrays (allocated the way ConstructPotentials does,
+    ! PotentialMixing.f90:32-54) and filled with the synthetic fields.
+    allocate(BPot(nx,ny,nz,2), NablaBPot(nx,ny,nz,3,2), UPot(nx,ny,nz,2))
+    allocate(WPot(nx,ny,nz,3,3,2), DivWPot(nx,ny,nz,3,2))
+    allocate(SPot(nx,ny,nz,3,2), APot(nx,ny,nz,3,2), DivAPot(nx,ny,nz,2))
+    read(uu) UPot, BPot, NablaBPot, WPot, DivWPot, SPot, APot, DivAPot
+    ActionOfB => ActionOfBNew
+  endif
+  close(uu)
 
   n = nx*ny*nz
 
   if(index(tasks,'force').gt.0) then
-    open(newunit=u, file=trim(prefix)//'_force.bin', access='stream',         &
+    open(newunit=uu, file=trim(prefix)//'_force.bin', access='stream',         &
     &    form='unformatted', status='replace')
-    write(u) B1,B2,B3,B4,B5,B6,B7a,B7b,B8a,B8b,Byt3a,Byt3b,B9,B9q,B10,B11,     &
+    write(uu) B1,B2,B3,B4,B5,B6,B7a,B7b,B8a,B8b,Byt3a,Byt3b,B9,B9q,B10,B11,     &
     &        B12a,B12b,B13a,B13b,B14,B15,B16,B17,B18,B19,B20,B21
-    write(u) hbm, nucleonmass, e2, hbar, dv, Neutrons, Protons
-    write(u) COM1Body
-    close(u)
+    write(uu) hbm, nucleonmass, e2, hbar, dv, Neutrons, Protons
+    write(uu) COM1Body
+    close(uu)
   endif
 
   call DeriveAll()
 
   if(index(tasks,'derive').gt.0) then
-    open(newunit=u, file=trim(prefix)//'_derive.bin', access='stream',        &
+    open(newunit=uu, file=trim(prefix)//'_derive.bin', access='stream',        &
     &    form='unformatted', status='replace')
     do i=1,nwt
       do m=1,3
-        write(u) HFBasis(i)%Der(m)%Grid(:,:,:,:,1)
+        write(uu) HFBasis(i)%Der(m)%Grid(:,:,:,:,1)
       enddo
-      write(u) HFBasis(i)%Lap%Grid(:,:,:,:,1)
+      write(uu) HFBasis(i)%Lap%Grid(:,:,:,:,1)
     enddo
-    close(u)
+    close(uu)
   endif
 
   if(index(tasks,'hpsi').gt.0) then
-    open(newunit=u, file=trim(prefix)//'_hpsi.bin', access='stream',          &
+    open(newunit=uu, file=trim(prefix)//'_hpsi.bin', access='stream',          &
     &    form='unformatted', status='replace')
     do i=1,nwt
       h = hPsi(HFBasis(i))
-      write(u) h%Grid(:,:,:,:,1)
+      write(uu) h%Grid(:,:,:,:,1)
     enddo
-    close(u)
+    close(uu)
   endif
 
   if(index(tasks,'dens').gt.0) then
     call ComputeDensity(Density, .false.)
-    open(newunit=u, file=trim(prefix)//'_dens.bin', access='stream',          &
+    open(newunit=uu, file=trim(prefix)//'_dens.bin', access='stream',          &
     &    form='unformatted', status='replace')
-    write(u) Density%Rho, Density%Tau, Density%NablaJ, Density%DerRho,         &
+    write(uu) Density%Rho, Density%Tau, Density%NablaJ, Density%DerRho,         &
     &        Density%LapRho
     if(.not. TRC) then
-      write(u) Density%Vecj, Density%Vecs, Density%RotVecJ, Density%RotS,      &
+      write(uu) Density%Vecj, Density%Vecs, Density%RotVecJ, Density%RotS,      &
       &        Density%DerS
     endif
-    close(u)
+    close(uu)
   endif
 
   if(index(tasks,'step').gt.0) then
     ! Two heavy-ball steps with fixed dt/momentum (ParameterEstimation=0),
     ! each followed by Gram-Schmidt, as GradDesc does (ImaginaryTime.f90:439).
-    open(newunit=u, file=trim(prefix)//'_step.bin', access='stream',          &
+    open(newunit=uu, file=trim(prefix)//'_step.bin', access='stream',          &
     &    form='unformatted', status='replace')
     do it=1,2
       call GradDesc(it)
       do i=1,nwt
-        write(u) HFBasis(i)%Value%Grid(:,:,:,:,1)
+        write(uu) HFBasis(i)%Value%Grid(:,:,:,:,1)
       enddo
       do i=1,nwt
-        write(u) HFBasis(i)%GetEnergy(), HFBasis(i)%GetDispersion()
+        write(uu) HFBasis(i)%GetEnergy(), HFBasis(i)%GetDispersion()
       enddo
       call DeriveAll()
     enddo
-    close(u)
+    close(uu)
   endif
+
+  if(index(tasks,'pot').gt.0) then
+    ! Densities of the given states (occupations as given), then the
+    ! reference's own chain: moments -> Coulomb -> potentials -> energy.
+    call ComputeDensity(Density, .false.)
+    call CalculateAllMoments(1)
+    call ConstructPotentials
+    call CompEnergy
+    call dump_moments(trim(prefix)//'_moments.bin')
+    call dump_fields(trim(prefix)//'_pot.bin')
+    call dump_energy(trim(prefix)//'_energy.bin')
+  endif
+
+  if(index(tasks,'scf').gt.0) then
+    ! Start-up and loop of Evolve (Main.f90:172-225, 255-328) for HF without
+    ! constraints or cranking, ParameterEstimation = 0.
+    EvolveSpwf => GradDesc
+    call SolvePairing
+    call DeriveAll
+    call UpdateDensities(0)
+    call CalculateAllMoments(1)
+    call CalculateAllMoments(1)
+    call CalculateAllMoments(1)
+    call ConstructPotentials
+    call CompEnergy
+    open(newunit=uu, file=trim(prefix)//'_scf.bin', access='stream',            &
+    &    form='unformatted', status='replace')
+    write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,        &
+    &        sum(CoMCorrection), Skyrmeterms
+    do it=1,niter
+      call EvolveSpwf(it)
+      call SolvePairing
+      call DeriveAll
+      call UpdateDensities(0)
+      call MixDensities(it)
+      call CalculateAllMoments(1)
+      call ReadjustAllMoments()
+      call ConstructPotentials
+      call CompEnergy
+      write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,      &
+      &        sum(CoMCorrection), Skyrmeterms
+    enddo
+    close(uu)
+    open(newunit=uu, file=trim(prefix)//'_scfstates.bin', access='stream',      &
+    &    form='unformatted', status='replace')
+    do i=1,nwt
+      write(uu) HFBasis(i)%Value%Grid(:,:,:,:,1)
+    enddo
+    do i=1,nwt
+      write(uu) HFBasis(i)%GetEnergy(), HFBasis(i)%GetOcc()
+    enddo
+    close(uu)
+    call dump_fields(trim(prefix)//'_scfpot.bin')
+  endif
+
+contains
+
+  subroutine dump_moments(fn)
+    character(len=*), intent(in) :: fn
+    type(Moment), pointer :: Cur
+    integer :: v, cnt
+    cnt = 0
+    Cur => Root
+    do while(associated(Cur))
+      if(Cur%l .lt. 0) exit
+      cnt = cnt + 1
+      Cur => Cur%Next
+    enddo
+    open(newunit=v, file=fn, access='stream', form='unformatted',              &
+    &    status='replace')
+    write(v) cnt
+    Cur => Root
+    do while(associated(Cur))
+      if(Cur%l .lt. 0) exit
+      write(v) Cur%l, Cur%m, merge(1, 0, Cur%ImPart), Cur%Value
+      Cur => Cur%Next
+    enddo
+    close(v)
+  end subroutine dump_moments
+
+  subroutine dump_fields(fn)
+    character(len=*), intent(in) :: fn
+    integer :: v
+    open(newunit=v, file=fn, access='stream', form='unformatted',              &
+    &    status='replace')
+    write(v) Density%Rho, Density%Tau, Density%NablaJ, Density%DerRho,         &
+    &        Density%LapRho
+    if(.not. TRC) then
+      write(v) Density%Vecj, Density%Vecs, Density%RotVecJ, Density%RotS
+    endif
+    write(v) CoulombPotential, CoulExchange
+    write(v) size(CPotComplete,1), size(CPotComplete,2), size(CPotComplete,3)
+    write(v) CPotComplete
+    write(v) UPot, BPot, NablaBPot, WPot, DivWPot
+    if(.not. TRC) then
+      write(v) SPot, APot, DivAPot
+    endif
+    close(v)
+  end subroutine dump_fields
+
+  subroutine dump_energy(fn)
+    character(len=*), intent(in) :: fn
+    integer :: v
+    open(newunit=v, file=fn, access='stream', form='unformatted',              &
+    &    status='replace')
+    write(v) TotalEnergy, Kinetic, CoulombEnergy, CoulombExchange,             &
+    &        CoMCorrection, Skyrmeterms
+    close(v)
+  end subroutine dump_energy
+
 end program ref_driver

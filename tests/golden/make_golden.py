@@ -113,7 +113,7 @@ CASES = {
 }
 
 
-def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom):
+def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0):
     fd = "-1" if lag else "3"
     fdl = "-1" if lag else "4"
     return f"""&GenInfo
@@ -125,7 +125,7 @@ MaxIter=2, ParameterEstimation=0, dt={dt}, momentum={mom}
 nwt={nwt}
 /
 &Densit
-MixingScheme=0
+MixingScheme={mixing}
 /
 &Pairing
 Type="HF"
@@ -183,6 +183,9 @@ def run_case(name):
     with open(os.path.join(wd, "input.nml")) as fin:
         subprocess.run([DRIVER, "input.bin", "out", "force,derive,hpsi,dens,step"],
                        cwd=wd, stdin=fin, check=True, stdout=subprocess.DEVNULL)
+    with open(os.path.join(wd, "input.nml")) as fin:
+        subprocess.run([DRIVER, "input.bin", "out", "pot"],
+                       cwd=wd, stdin=fin, check=True, stdout=subprocess.DEVNULL)
 
     def rd(fn, count=-1, offset=0):
         return np.fromfile(os.path.join(wd, fn), "<f8", count=count, offset=offset)
@@ -224,8 +227,9 @@ def run_case(name):
                 com1body=com1, seed=seed,
                 source="reference MOCCa modules via oracle/ref_driver.f90 "
                        "(amdflang -O2)")
+    potres = read_pot(wd, "out", nx, ny, nz, trc)
     np.savez_compressed(
-        os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
+        os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta), **potres,
         qn=np.array(qns, np.int32), occ=np.array(occ), psi=psi, **fields,
         der=d[:, :3], lap=d[:, 3], hpsi=h, **res,
         step1_psi=steps[0][0], step1_ed=steps[0][1],
@@ -234,6 +238,120 @@ def run_case(name):
     print(name, "written")
 
 
+def read_fields(fn, nx, ny, nz, trc):
+    """Parse dump_fields of oracle/ref_driver.f90."""
+    raw = open(fn, "rb").read()
+    n = nx * ny * nz
+    k = 0
+    out = {}
+
+    def take(name, shape):
+        nonlocal k
+        cnt = int(np.prod(shape))
+        out[name] = np.frombuffer(raw[k:k + 8 * cnt], "<f8").reshape(shape).copy()
+        k += 8 * cnt
+    sh = (nz, ny, nx)
+    take("rho", (2,) + sh); take("tau", (2,) + sh); take("nablaJ", (2,) + sh)
+    take("derrho", (2, 3) + sh); take("laprho", (2,) + sh)
+    if not trc:
+        take("vecj", (2, 3) + sh); take("vecs", (2, 3) + sh)
+        take("rotvecj", (2, 3) + sh); take("rots", (2, 3) + sh)
+    take("vcoul", sh); take("vcoulx", sh)
+    mx, my, mz = np.frombuffer(raw[k:k + 12], "<i4")
+    k += 12
+    take("cpot", (mz, my, mx))
+    take("Upot", (2,) + sh); take("Bpot", (2,) + sh); take("NBpot", (2, 3) + sh)
+    take("Wpot", (2, 3, 3) + sh); take("DWpot", (2, 3) + sh)
+    if not trc:
+        take("Spot", (2, 3) + sh); take("Apot", (2, 3) + sh); take("DApot", (2,) + sh)
+    assert k == len(raw), (k, len(raw))
+    return out
+
+
+def read_pot(wd, prefix, nx, ny, nz, trc):
+    f = read_fields(os.path.join(wd, prefix + "_pot.bin"), nx, ny, nz, trc)
+    res = {"pot_" + key: v for key, v in f.items()}
+    raw = open(os.path.join(wd, prefix + "_moments.bin"), "rb").read()
+    cnt = int(np.frombuffer(raw[:4], "<i4")[0])
+    rec = np.dtype([("l", "<i4"), ("m", "<i4"), ("im", "<i4"), ("v", "<f8", 2)])
+    mom = np.frombuffer(raw[4:4 + cnt * rec.itemsize], rec)
+    res["pot_moment_lm"] = np.stack([mom["l"], mom["m"], mom["im"]], 1).astype(np.int32)
+    res["pot_moment_val"] = mom["v"].copy()
+    e = np.fromfile(os.path.join(wd, prefix + "_energy.bin"), "<f8")
+    # TotalEnergy, Kinetic(2), CoulombEnergy, CoulombExchange, CoMCorrection(2,2), terms(36)
+    res["pot_energy"] = e
+    return res
+
+
+SCF_CASES = {
+    # name: (n, dx, trc, lag, neutrons, protons, nwt_per_isospin, niter)
+    "scf_trc_fd": (10, 0.8, True, False, 8, 8, 6, 12),
+    "scf_trc_lag": (10, 0.8, True, True, 8, 8, 6, 12),
+    "scf_todd_fd": (8, 0.9, False, False, 4, 4, 6, 8),
+}
+
+
+def run_scf(name):
+    n, dx, trc, lag, neutrons, protons, per_iso, niter = SCF_CASES[name]
+    seed = sum(map(ord, name))
+    rng = np.random.default_rng(seed)
+    pattern = [(1, 1), (-1, 1), (1, 1), (-1, 1), (1, 1), (-1, 1)] if trc else \
+        [(1, 1), (1, -1), (-1, 1), (-1, -1), (1, 1), (1, -1)]
+    qns = [(iso, P, S, 1) for iso in (-1, 1) for (P, S) in pattern[:per_iso]]
+    nwt = len(qns)
+    occ = [0.0] * nwt
+    psi = seeded_states(rng, n, n, n, dx, qns)
+    # Orthonormalise the seeds per (isospin, parity, signature) block in the
+    # octant inner product (dv = 8 dx^3), as a start-up would.
+    dv8 = 8 * dx**3
+    flat = psi.reshape(nwt, -1)
+    for key in sorted(set(q[:3] for q in qns)):
+        idx = [i for i, q in enumerate(qns) if q[:3] == key]
+        for a, i in enumerate(idx):
+            for j in idx[:a]:
+                flat[i] -= (flat[i] @ flat[j]) * dv8 * flat[j]
+            flat[i] /= np.sqrt((flat[i] @ flat[i]) * dv8)
+    psi = flat.reshape(psi.shape)
+    dt, mom = 0.010, 0.3
+    wd = os.path.join(RUNDIR, name)
+    os.makedirs(wd, exist_ok=True)
+    if not os.path.exists(os.path.join(wd, "forces.param")):
+        os.symlink(FORCES, os.path.join(wd, "forces.param"))
+    with open(os.path.join(wd, "input.bin"), "wb") as fh:
+        np.array([nwt], "<i4").tofile(fh)
+        np.array(qns, "<i4").tofile(fh)
+        np.array(occ, "<f8").tofile(fh)
+        psi.astype("<f8").tofile(fh)
+    nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom)
+    with open(os.path.join(wd, "input.nml"), "w") as fh:
+        fh.write(nml)
+    with open(os.path.join(wd, "input.nml")) as fin:
+        subprocess.run([DRIVER, "input.bin", "out", "force,scf", str(niter)], cwd=wd,
+                       stdin=fin, check=True, stdout=subprocess.DEVNULL)
+    traj = np.fromfile(os.path.join(wd, "out_scf.bin"), "<f8").reshape(niter + 1, 41)
+    st = np.fromfile(os.path.join(wd, "out_scfstates.bin"), "<f8")
+    N = n ** 3
+    fin_psi = st[:nwt * 4 * N].reshape(nwt, 4, n, n, n)
+    fin_eo = st[nwt * 4 * N:].reshape(nwt, 2)
+    f = read_fields(os.path.join(wd, "out_scfpot.bin"), n, n, n, trc)
+    fb = open(os.path.join(wd, "out_force.bin"), "rb").read()
+    coef = np.frombuffer(fb[:28 * 8], "<f8")
+    rest = np.frombuffer(fb[28 * 8:37 * 8], "<f8")
+    com1 = int(np.frombuffer(fb[37 * 8:37 * 8 + 4], "<i4")[0])
+    meta = dict(nx=n, ny=n, nz=n, dx=dx, trc=trc, lagrange=lag, nwt=nwt, dt=dt,
+                momentum=mom, neutrons=neutrons, protons=protons, niter=niter,
+                com1body=com1, seed=seed, mixing="scheme 0, DampingParam 0.75",
+                source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")
+    np.savez_compressed(os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
+                        qn=np.array(qns, np.int32), psi=psi, traj=traj,
+                        final_psi=fin_psi, final_eo=fin_eo, coef=coef, consts=rest,
+                        **{"final_" + k: v for k, v in f.items()})
+    print(name, "written", traj[:, 0])
+
+
 if __name__ == "__main__":
-    for c in (sys.argv[1:] or CASES):
-        run_case(c)
+    for c in (sys.argv[1:] or list(CASES) + list(SCF_CASES)):
+        if c in CASES:
+            run_case(c)
+        else:
+            run_scf(c)
