@@ -24,6 +24,7 @@
  */
 #ifndef MOCCA_ORACLE_H
 #define MOCCA_ORACLE_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -104,6 +105,61 @@ void or_graddesc(double *psi, const double *der, const double *lap,
                  const double *S, const double *A, const double *DivA,
                  double dt_over_hbar, double momentum, double dv,
                  double *energy, double *disp);
+
+/* ---- self-consistent part (mocca_oracle_scf.c) --------------------------- */
+
+/* r^l Re Y_lm at (X,Y,Z) (SphericalHarmonics, Moments.f90:584-691). */
+double or_rlylm(int l, int m, double X, double Y, double Z);
+
+/* EV8 electric multipole moments (even l <= lmax, even m, real part) in the
+ * list order of IniMoments (Moments.f90:372-485); lm[2k], lm[2k+1] = l, m;
+ * val[2k+it] = dv sum rho_it r^l Y_lm (Calculate_Multipole :1907-2004).
+ * Returns the number of moments. */
+int or_moments(const double *rho, int nx, int ny, int nz, double dx, double dv,
+               int lmax, int *lm, double *val);
+
+/* SolveCoulomb with the conjugate-gradient solver (Coulomb.f90:255-340,
+ * 583-674, 755-900).  cpot[(nz+2)][(ny+2)][(nx+2)] is the persistent complete
+ * potential; first != 0 fills it with InitialGuess (:542-581) first.
+ * qp[k] = proton moment k of the list of or_moments.  Returns the number of
+ * CG iterations taken. */
+int or_coulomb(const double *rho_p, int nx, int ny, int nz, double dx, double e2,
+               double nucleons, double protons, int nmom, const int *lm,
+               const double *qp, double *cpot, int first, double *vcoul,
+               double *vcoulx, int maxit);
+
+/* Mean-field potentials of an SLy4-type functional (MeanFields.f90:115-150,
+ * 269-409, 411-492, 494-597, 618-712).  coef = B1..B21 in Force.f90 order
+ * (28 values).  Layouts as in or_hpsi. */
+void or_potentials(const double *coef, int nx, int ny, int nz, double dx, int lag,
+                   const double *lagx, const double *lagy, const double *lagz, int trc,
+                   const double *rho, const double *tau, const double *nablaJ,
+                   const double *derrho, const double *laprho, const double *vecj,
+                   const double *vecs, const double *rotvecj, const double *rots,
+                   const double *vcoul, const double *vcoulx, int cexchange,
+                   double *U, double *B, double *NablaB, double *W, double *DivW,
+                   double *S, double *A, double *DivA);
+
+/* Energy (CompEnergy, Energy.f90:62-179 with compSkyrme :623-859 and
+ * CompCOMCorrection :1608-1660).  consts = hbm(2), mass(2), e2, hbar, dv, N,
+ * Z.  out[0] total, [1..2] kinetic, [3] Coulomb, [4] exchange, [5..6]
+ * one-body c.o.m., [7..42] the 36 Skyrme terms. */
+void or_energy(const double *coef, const double *consts, int com1body, int trc,
+               int nx, int ny, int nz, const double *rho, const double *tau,
+               const double *nablaJ, const double *laprho, const double *vecj,
+               const double *vecs, const double *rots, const double *vcoul,
+               const double *kin_sp, int cexchange, double *out);
+
+/* sum_l occ_l <psi_l|lap_l> per isospin (CompKinetic, Energy.f90:1426-1456). */
+void or_kinetic_sums(const double *psi, const double *lap, int M, int N,
+                     const int *qn, const double *occ, double dv, double *kin_sp);
+
+/* d = (1-a) d + a h (MixDensities scheme 0, DensityMixing.f90:39-45). */
+void or_mix(double *d, const double *h, size_t count, double a);
+
+/* HF occupations by NaiveFill (HartreeFock.f90:177-262). */
+void or_hf_fill(const double *energy, const int *qn, int M, double neutrons,
+                double protons, int trc, double *occ);
 
 #ifdef __cplusplus
 }

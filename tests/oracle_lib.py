@@ -22,6 +22,7 @@ def lib():
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
         _LIB = ctypes.CDLL(path)
+        _LIB.or_rlylm.restype = ctypes.c_double
     return _LIB
 
 
@@ -178,3 +179,168 @@ def relerr(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ---- self-consistent part -------------------------------------------------
+
+def rlylm(l, m, X, Y, Z):
+    return lib().or_rlylm(int(l), int(m), ctypes.c_double(X), ctypes.c_double(Y),
+                          ctypes.c_double(Z))
+
+
+def moments(rho, mesh, dv, lmax=10):
+    lm = np.zeros(2 * 64, np.int32)
+    val = np.zeros(2 * 64)
+    rho = np.ascontiguousarray(rho.reshape(2, mesh.N))
+    cnt = lib().or_moments(P(rho), mesh.nx, mesh.ny, mesh.nz, ctypes.c_double(mesh.dx),
+                           ctypes.c_double(dv), int(lmax), P(lm), P(val))
+    return lm[:2 * cnt].reshape(cnt, 2), val[:2 * cnt].reshape(cnt, 2)
+
+
+class Coulomb:
+    """Persistent Coulomb state (CPotComplete) of SolveCoulomb."""
+
+    def __init__(self, mesh, e2, nucleons, protons, maxit=500):
+        self.mesh, self.e2, self.A, self.Z, self.maxit = mesh, e2, nucleons, protons, maxit
+        self.cpot = np.zeros((mesh.nz + 2) * (mesh.ny + 2) * (mesh.nx + 2))
+        self.first = True
+        self.iters = 0
+
+    def solve(self, rho_p, lm, qp):
+        m = self.mesh
+        vc, vx = np.zeros(m.N), np.zeros(m.N)
+        lm = np.ascontiguousarray(lm, dtype=np.int32)
+        qp = np.ascontiguousarray(qp, dtype=np.float64)
+        self.iters = lib().or_coulomb(
+            P(np.ascontiguousarray(rho_p.reshape(m.N))), m.nx, m.ny, m.nz,
+            ctypes.c_double(m.dx), ctypes.c_double(self.e2), ctypes.c_double(self.A),
+            ctypes.c_double(self.Z), len(qp), P(lm), P(qp), P(self.cpot),
+            int(self.first), P(vc), P(vx), int(self.maxit))
+        self.first = False
+        return vc, vx
+
+
+def potentials(coef, mesh, trc, d, vcoul, vcoulx, cexchange=2):
+    """d: dict with rho, tau, nablaJ, derrho, laprho (+ vecj, vecs, rotvecj,
+    rots when trc is False), shapes as in the reference dumps."""
+    N = mesh.N
+    c = lambda k, shp: (np.ascontiguousarray(d[k].reshape(shp)) if k in d and d[k] is not None
+                        else None)
+    out = {"U": np.zeros((2, N)), "B": np.zeros((2, N)), "NablaB": np.zeros((2, 3, N)),
+           "W": np.zeros((2, 3, 3, N)), "DivW": np.zeros((2, 3, N)),
+           "S": np.zeros((2, 3, N)), "A": np.zeros((2, 3, N)), "DivA": np.zeros((2, N))}
+    lib().or_potentials(
+        P(np.ascontiguousarray(coef)), mesh.nx, mesh.ny, mesh.nz, ctypes.c_double(mesh.dx),
+        mesh.lag, *mesh.lagmats(), int(trc), P(c("rho", (2, N))), P(c("tau", (2, N))),
+        P(c("nablaJ", (2, N))), P(c("derrho", (2, 3, N))), P(c("laprho", (2, N))),
+        P(c("vecj", (2, 3, N))), P(c("vecs", (2, 3, N))), P(c("rotvecj", (2, 3, N))),
+        P(c("rots", (2, 3, N))), P(np.ascontiguousarray(vcoul.reshape(N))),
+        P(np.ascontiguousarray(vcoulx.reshape(N))), int(cexchange),
+        P(out["U"]), P(out["B"]), P(out["NablaB"]), P(out["W"]), P(out["DivW"]),
+        P(out["S"]), P(out["A"]), P(out["DivA"]))
+    return out
+
+
+def kinetic_sums(psi, lap, qn, occ, dv):
+    M = psi.shape[0]
+    N = psi[0].size // 4
+    k = np.zeros(2)
+    lib().or_kinetic_sums(P(np.ascontiguousarray(psi.reshape(M, 4, N))),
+                          P(np.ascontiguousarray(lap.reshape(M, 4, N))), M, N,
+                          P(np.ascontiguousarray(qn, dtype=np.int32)),
+                          P(np.ascontiguousarray(occ, dtype=np.float64)),
+                          ctypes.c_double(dv), P(k))
+    return k
+
+
+def energy(coef, consts, com1body, trc, mesh, d, vcoul, kin_sp, cexchange=2):
+    """Returns [total, kin_n, kin_p, E_C, E_Cx, com_n, com_p, terms(36)]."""
+    N = mesh.N
+    c = lambda k, shp: (np.ascontiguousarray(d[k].reshape(shp)) if k in d and d[k] is not None
+                        else None)
+    out = np.zeros(43)
+    lib().or_energy(P(np.ascontiguousarray(coef)), P(np.ascontiguousarray(consts)),
+                    int(com1body), int(trc), mesh.nx, mesh.ny, mesh.nz,
+                    P(c("rho", (2, N))), P(c("tau", (2, N))), P(c("nablaJ", (2, N))),
+                    P(c("laprho", (2, N))), P(c("vecj", (2, 3, N))), P(c("vecs", (2, 3, N))),
+                    P(c("rots", (2, 3, N))), P(np.ascontiguousarray(vcoul.reshape(N))),
+                    P(np.ascontiguousarray(kin_sp, dtype=np.float64)), int(cexchange), P(out))
+    return out
+
+
+def hf_fill(energies, qn, neutrons, protons, trc):
+    M = len(energies)
+    occ = np.zeros(M)
+    lib().or_hf_fill(P(np.ascontiguousarray(energies, dtype=np.float64)),
+                     P(np.ascontiguousarray(qn, dtype=np.int32)), M,
+                     ctypes.c_double(neutrons), ctypes.c_double(protons), int(trc), P(occ))
+    return occ
+
+
+DENSITY_KEYS = ("rho", "tau", "nablaJ", "derrho", "laprho")
+DENSITY_KEYS_ODD = ("vecj", "rotvecj", "vecs", "rots")
+
+
+def all_densities(mesh, psi, qn, occ, trc):
+    """ComputeDensity + the derived fields, and each state's derivatives."""
+    M, N = psi.shape[0], mesh.N
+    der = np.zeros((M, 3, 4, N))
+    lap = np.zeros((M, 4, N))
+    for j, q in enumerate(qn):
+        der[j], lap[j] = mesh.derive_spinor(psi[j], q[1], q[2], q[3])
+    rho, tau, nj, vj, vs = density(psi, der, qn, occ, trc, N)
+    derrho, laprho, rotj, rots, _ = density_derived(mesh, rho, vj, vs, trc)
+    d = dict(rho=rho, tau=tau, nablaJ=nj, derrho=derrho, laprho=laprho)
+    if not trc:
+        d.update(vecj=vj, vecs=vs, rotvecj=rotj, rots=rots)
+    return d, der, lap
+
+
+def scf_run(g, niter):
+    """The Evolve loop of the reference (Main.f90:172-225, 255-328) for HF with
+    fixed dt/momentum and linear density mixing, restated on the oracle.
+    Returns the per-iteration [E, sum kin, E_C, E_Cx, sum com, terms(36)]
+    rows (start-up row first), the final states and potentials."""
+    m = g["meta"]
+    mesh = Mesh(m["nx"], m["ny"], m["nz"], m["dx"], m["lagrange"])
+    coef, consts = g["coef"], g["consts"]
+    dv, trc, qn = consts[6], m["trc"], g["qn"]
+    M, N = m["nwt"], mesh.N
+    kin2 = kin_coefs(g)
+    psi = np.ascontiguousarray(g["psi"].reshape(M, 4, N)).copy()
+    coul = Coulomb(mesh, consts[4], consts[7] + consts[8], consts[8])
+    mixing = 0.75
+
+    def rows(E):
+        return np.concatenate([[E[0], E[1] + E[2], E[3], E[4], E[5] + E[6]], E[7:]])
+
+    def construct(d):
+        lm, val = moments(d["rho"], mesh, dv)
+        vc, vx = coul.solve(d["rho"][1], lm, val[:, 1])
+        F = potentials(coef, mesh, trc, d, vc, vx)
+        return F, vc, vx
+
+    occ = hf_fill(np.zeros(M), qn, m["neutrons"], m["protons"], trc)
+    d, der, lap = all_densities(mesh, psi, qn, occ, trc)
+    F, vc, vx = construct(d)
+    E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
+               kinetic_sums(psi, lap, qn, occ, dv))
+    traj = [rows(E)]
+    upd = np.zeros_like(psi)
+    dt_over_hbar = m["dt"] / consts[5]
+    for _ in range(niter):
+        Ff = {k: np.ascontiguousarray(v).reshape(-1) for k, v in F.items()}
+        psi, upd, e_sp, _ = graddesc(psi, der, lap, upd, qn, trc, kin2, Ff, dt_over_hbar,
+                                      m["momentum"], dv)
+        occ = hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
+        old = d
+        d, der, lap = all_densities(mesh, psi, qn, occ, trc)
+        for k in d:
+            lib().or_mix(P(d[k]), P(np.ascontiguousarray(old[k])), ctypes.c_size_t(d[k].size),
+                         ctypes.c_double(mixing))
+        F, vc, vx = construct(d)
+        E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
+                   kinetic_sums(psi, lap, qn, occ, dv))
+        traj.append(rows(E))
+    return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
+                vcoul=vc, vcoulx=vx, cpot=coul.cpot)

@@ -1,0 +1,159 @@
+"""Pin the self-consistent part of the C restatement (moments, Coulomb,
+mean-field potentials, energy, density mixing, HF filling, and the whole
+iteration loop) against the reference's own outputs, and against what the
+mathematics fixes independently of the reference (spherical harmonics from
+scipy, the potential of a Gaussian charge, a brute-force filling).
+
+Tolerances: rounding level (1e-12) where the oracle keeps the reference's
+operation order; 1e-11 for the Coulomb solution, whose conjugate-gradient
+iterate is only defined to the solver's stopping residual; energies along an
+iteration trajectory to 1e-9 MeV absolute.
+"""
+import numpy as np
+import pytest
+from scipy import special
+
+import oracle_lib as ol
+from oracle_lib import GOLDEN_CASES, Mesh, golden, relerr
+
+TOL = 1e-12
+SCF_CASES = ["scf_trc_fd", "scf_trc_lag", "scf_todd_fd"]
+
+
+def _mesh(g):
+    m = g["meta"]
+    return Mesh(m["nx"], m["ny"], m["nz"], m["dx"], m["lagrange"])
+
+
+def _dens(g, prefix="pot_"):
+    keys = ol.DENSITY_KEYS + (() if g["meta"]["trc"] else ol.DENSITY_KEYS_ODD)
+    return {k: g[prefix + k] for k in keys}
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_moments_vs_reference(case):
+    g = golden(case)
+    lm, val = ol.moments(g["pot_rho"], _mesh(g), g["consts"][6])
+    assert np.array_equal(lm, g["pot_moment_lm"][:, :2])
+    assert not g["pot_moment_lm"][:, 2].any()      # real parts only in EV8
+    assert relerr(val, g["pot_moment_val"]) < TOL
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_coulomb_vs_reference(case):
+    """First SolveCoulomb call (initial guess + multipole boundary + CG)."""
+    g = golden(case)
+    mesh, c = _mesh(g), g["consts"]
+    cs = ol.Coulomb(mesh, c[4], c[7] + c[8], c[8])
+    vc, vx = cs.solve(g["pot_rho"][1], g["pot_moment_lm"][:, :2], g["pot_moment_val"][:, 1])
+    assert 0 < cs.iters < 500
+    assert relerr(vc, g["pot_vcoul"].reshape(-1)) < 1e-11
+    assert relerr(cs.cpot, g["pot_cpot"].reshape(-1)) < 1e-11
+    assert relerr(vx, g["pot_vcoulx"].reshape(-1)) < TOL
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_potentials_vs_reference(case):
+    g = golden(case)
+    trc = g["meta"]["trc"]
+    F = ol.potentials(g["coef"], _mesh(g), trc, _dens(g), g["pot_vcoul"], g["pot_vcoulx"])
+    pairs = [("U", "Upot"), ("B", "Bpot"), ("NablaB", "NBpot"), ("W", "Wpot"),
+             ("DivW", "DWpot")]
+    if not trc:
+        pairs += [("S", "Spot"), ("A", "Apot"), ("DivA", "DApot")]
+    for k, gk in pairs:
+        assert relerr(F[k].reshape(-1), g["pot_" + gk].reshape(-1)) < TOL, k
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_energy_vs_reference(case):
+    g = golden(case)
+    m, c = g["meta"], g["consts"]
+    ks = ol.kinetic_sums(g["psi"], g["lap"], g["qn"], g["occ"], c[6])
+    E = ol.energy(g["coef"], c, m["com1body"], m["trc"], _mesh(g), _dens(g),
+                  g["pot_vcoul"], ks)
+    ge = g["pot_energy"]   # total, kin(2), E_C, E_Cx, CoM(2,2) (Fortran order), terms
+    assert not ge[6] and not ge[8]                 # no two-body c.o.m. term
+    ref = np.concatenate([ge[0:6], [ge[7]], ge[9:]])
+    assert np.max(np.abs(E - ref) / np.maximum(np.abs(ref), 1e-300)) < TOL
+
+
+@pytest.mark.parametrize("case", SCF_CASES)
+def test_scf_trajectory_vs_reference(case):
+    """The whole loop: GradDesc, HF filling, densities, mixing, moments,
+    Coulomb, potentials, energy; every iteration's energy decomposition."""
+    g = golden(case)
+    r = ol.scf_run(g, g["meta"]["niter"])
+    assert r["traj"].shape == g["traj"].shape
+    assert np.max(np.abs(r["traj"][:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(r["traj"], g["traj"]) < 1e-12
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-11
+    assert relerr(r["energies"], g["final_eo"][:, 0]) < 1e-11
+    assert np.array_equal(r["occ"], g["final_eo"][:, 1])
+    assert relerr(r["fields"]["U"].reshape(-1), g["final_Upot"].reshape(-1)) < 1e-11
+    assert relerr(r["cpot"], g["final_cpot"].reshape(-1)) < 1e-11
+    # the energy goes down along the imaginary-time trajectory
+    assert g["traj"][-1, 0] < g["traj"][0, 0]
+
+
+# ---- checks fixed by the mathematics ---------------------------------------
+
+@pytest.mark.parametrize("l,m", [(0, 0), (2, 0), (2, 2), (4, 0), (4, 2), (4, 4),
+                                 (6, 4), (8, 6), (10, 10)])
+def test_rlylm_vs_scipy(l, m):
+    """r^l Re Y_lm with the Condon-Shortley phase, as scipy defines it."""
+    rng = np.random.default_rng(l * 11 + m)
+    for X, Y, Z in rng.uniform(0.1, 5.0, size=(20, 3)):
+        r = np.sqrt(X * X + Y * Y + Z * Z)
+        y = special.sph_harm_y(l, m, np.arccos(Z / r), np.arctan2(Y, X))
+        assert abs(ol.rlylm(l, m, X, Y, Z) - r**l * y.real) < 1e-12 * max(1.0, r**l)
+
+
+def _gaussian_charge_error(n, dx, a=2.0, Z=20.0, e2=1.43996446):
+    mesh = Mesh(n, n, n, dx, False)
+    x = (np.arange(n) + 0.5) * dx
+    zz, yy, xx = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(xx**2 + yy**2 + zz**2)
+    rho_p = Z / (np.pi**1.5 * a**3) * np.exp(-(r / a) ** 2)
+    rho = np.stack([np.zeros_like(rho_p), rho_p]).reshape(2, -1)
+    lm, val = ol.moments(rho, mesh, 8 * dx**3)
+    assert abs(val[0, 1] * np.sqrt(4 * np.pi) - Z) < 1e-6 * Z     # Q00 = Z/sqrt(4 pi)
+    cs = ol.Coulomb(mesh, e2, 2 * Z, Z)
+    vc, _ = cs.solve(rho[1], lm, val[:, 1])
+    return relerr(vc, (e2 * Z * special.erf(r / a) / r).reshape(-1))
+
+
+def test_coulomb_gaussian_charge():
+    """Potential of a Gaussian proton distribution, V = e2 Z erf(r/a)/r:
+    fixes the source sign and scale, the Laplacian and the multipole
+    boundary; the error falls at close to fourth order in dx, as the
+    5-point stencil gives."""
+    coarse = _gaussian_charge_error(20, 0.8)
+    fine = _gaussian_charge_error(40, 0.4)
+    assert coarse < 1.5e-3
+    assert fine < coarse / 10
+
+
+def test_hf_fill_brute_force():
+    rng = np.random.default_rng(5)
+    M = 14
+    qn = np.array([[(-1) ** (i % 2), 1, 1, 1] for i in range(M)], np.int32)
+    e = rng.normal(size=M)
+    e[3] = e[5]                                     # a tie keeps index order
+    for trc, (nn, zz) in [(True, (6, 4)), (False, (3, 5)), (True, (7, 5))]:
+        occ = ol.hf_fill(e, qn, nn, zz, trc)
+        for iso, count in ((-1, nn), (1, zz)):
+            k = count // 2 if trc else count
+            idx = [i for i in sorted(range(M), key=lambda i: (e[i], i)) if qn[i, 0] == iso][:k]
+            want = np.zeros(M)
+            want[idx] = 2.0 if trc else 1.0
+            sel = qn[:, 0] == iso
+            assert np.array_equal(occ[sel], want[sel])
+
+
+def test_mix_linear():
+    rng = np.random.default_rng(9)
+    d, h = rng.normal(size=100), rng.normal(size=100)
+    want = 0.25 * d + 0.75 * h
+    ol.lib().or_mix(ol.P(d), ol.P(h), ol.ctypes.c_size_t(100), ol.ctypes.c_double(0.75))
+    assert np.max(np.abs(d - want)) < 1e-15
