@@ -128,6 +128,84 @@ int mocca_gram_schmidt(mocca_solve *h);
  * functional: no tensor or N2LO densities). */
 int mocca_compute_density(mocca_solve *h, mocca_density *out);
 
+/* ---- self-consistent part of the iteration --------------------------------
+ * The loop of Evolve (Main.f90:255-328) for Hartree-Fock with fixed dt and
+ * momentum (ParameterEstimation = 0) and linear density mixing
+ * (MixingScheme = 0): EvolveSpwf (GradDesc), SolvePairing (NaiveFill),
+ * UpdateDensities, MixDensities, CalculateAllMoments, ConstructPotentials
+ * (SolveCoulomb + Calc*Pot), CompEnergy.  Everything stays on the device;
+ * only the energies come back. */
+
+/* Functional and constants (Force.f90:38-41, ReadForceInfo): B[] = B1, B2,
+ * B3, B4, B5, B6, B7a, B7b, B8a, B8b, byt3a, byt3b, B9, B9q, B10, B11, B12a,
+ * B12b, B13a, B13b, B14 .. B21 (28 values, Force.f90 order).  Tensor and
+ * N2LO terms (B14 .. B21, t1n2, t2n2) are not supported: non-zero B14 .. B21
+ * returns MOCCA_ERR_UNSUPPORTED. */
+typedef struct mocca_functional {
+    double B[28];
+    double hbm[2];        /* hbar^2/2m per isospin (Force.f90 hbm) */
+    double mass[2];       /* nucleonmass */
+    double e2;            /* e^2 (Force.f90 e2) */
+    double neutrons, protons;
+    int com1body;         /* COM1Body: 2 = one-body c.o.m. in the kinetic term */
+    int cexchange;        /* Cexchange: 0 none, 1 perturbative, 2 self-consistent */
+    int coulomb;          /* CoulombSolver: 1 = conjugate gradients, 0 = none */
+    int coulomb_maxit;    /* CoulombMax (500) */
+    int max_moment;       /* MaxMoment = MaxLCoul (10, even, <= 12) */
+    double damping;       /* DampingParam of MixingScheme 0 (0.75) */
+} mocca_functional;
+
+/* Set the functional, the kinetic coefficients hbm/2 * reduced-mass factor
+ * (MeanFields.f90:826-838), the multipole tables (IniMoments,
+ * Moments.f90:372-485) and the Coulomb box with its initial guess
+ * (SetupCoulomb + InitialGuess, Coulomb.f90:342-581). */
+int mocca_scf_setup(mocca_solve *h, const mocca_functional *f);
+
+/* SolvePairing for HF: NaiveFill (HartreeFock.f90:177-262) from the
+ * current single-particle energies. */
+int mocca_hf_fill(mocca_solve *h);
+int mocca_get_occupations(mocca_solve *h, double *occ);
+
+/* UpdateDensities(0) (Densities.f90:406-429): the current densities become
+ * DensityHistory(1), then ComputeDensity. */
+int mocca_update_densities(mocca_solve *h);
+
+/* MixDensities, MixingScheme 0 (DensityMixing.f90:22-95):
+ * density = (1 - damping) density + damping DensityHistory(1). */
+int mocca_mix_densities(mocca_solve *h);
+
+/* CalculateAllMoments (Moments.f90:1907-2004, electric moments) and
+ * ConstructPotentials (PotentialMixing.f90:17-203): SolveCoulomb with the
+ * conjugate-gradient solver (Coulomb.f90:255-340, 583-674, 755-900), then
+ * CalcUPot, CalcWPot, CalcBPot, CalcSPot, CalcAPot (MeanFields.f90:115-712)
+ * into the potentials used by hPsi / GradDesc. */
+int mocca_construct_potentials(mocca_solve *h);
+
+/* CompEnergy (Energy.f90:62-179): e[43] = total, kinetic(2), Coulomb,
+ * Coulomb exchange, one-body c.o.m.(2), Skyrme terms 1..36
+ * (compSkyrme, Energy.f90:623-859). */
+int mocca_comp_energy(mocca_solve *h, double *e);
+
+/* Start-up of Evolve (Main.f90:172-225): SolvePairing, UpdateDensities,
+ * CalculateAllMoments, ConstructPotentials, CompEnergy. */
+int mocca_scf_start(mocca_solve *h);
+
+/* niter iterations of the Evolve loop (Main.f90:255-328) on the device.
+ * energies (may be NULL): [niter+1][43] in the layout of mocca_comp_energy,
+ * row 0 for the states on entry, row k after iteration k. */
+int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
+                  double *energies);
+
+/* Results of the last ConstructPotentials.  lm[k][2] = (l, m) and
+ * q[k][2] = (neutron, proton) moments; *count moments (<= 28).
+ * vcoul, vcoulx [N]; cpot [(nz+2)][(ny+2)][(nx+2)] (CPotComplete);
+ * potentials in the layouts of mocca_fields.  Any pointer may be NULL. */
+int mocca_get_moments(mocca_solve *h, int *lm, double *q, int *count);
+int mocca_get_coulomb(mocca_solve *h, double *vcoul, double *vcoulx, double *cpot,
+                      int *cg_iterations);
+int mocca_get_potentials(mocca_solve *h, double *U, double *B, double *NablaB,
+                         double *W, double *DivW, double *S, double *A, double *DivA);
+
 /* Kernel timing over the handle's stream with HIP events (for benchmarks):
  * enable = 1 starts recording each launch; mocca_profile_get returns the
  * summed milliseconds and launch count for a kernel family name

@@ -28,6 +28,10 @@ EXPORTS = (
     "mocca_derive_all", "mocca_download_derivatives", "mocca_hpsi",
     "mocca_grad_desc", "mocca_gram_schmidt", "mocca_compute_density",
     "mocca_profile_enable", "mocca_profile_get",
+    "mocca_scf_setup", "mocca_hf_fill", "mocca_get_occupations", "mocca_update_densities",
+    "mocca_mix_densities", "mocca_construct_potentials", "mocca_comp_energy",
+    "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
+    "mocca_get_potentials",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -48,6 +52,20 @@ class FieldsC(ctypes.Structure):
 class DensityC(ctypes.Structure):
     _fields_ = [(k, _dp) for k in ("rho", "tau", "nablaJ", "derrho", "laprho",
                                    "vecj", "vecs", "rotvecj", "rots", "ders")]
+
+
+class FunctionalC(ctypes.Structure):
+    _fields_ = [("B", ctypes.c_double * 28), ("hbm", ctypes.c_double * 2),
+                ("mass", ctypes.c_double * 2), ("e2", ctypes.c_double),
+                ("neutrons", ctypes.c_double), ("protons", ctypes.c_double),
+                ("com1body", ctypes.c_int), ("cexchange", ctypes.c_int),
+                ("coulomb", ctypes.c_int), ("coulomb_maxit", ctypes.c_int),
+                ("max_moment", ctypes.c_int), ("damping", ctypes.c_double)]
+
+
+# Energy row layout of mocca_comp_energy / mocca_iterate.
+ENERGY_FIELDS = ("total", "kinetic_n", "kinetic_p", "coulomb", "coulomb_exchange",
+                 "com_n", "com_p") + tuple(f"term{k}" for k in range(1, 37))
 
 
 class MoccaError(RuntimeError):
@@ -201,6 +219,72 @@ class Solve:
 
     def synchronize(self):
         self._chk(self.lib.mocca_synchronize(self.h))
+
+    # Self-consistent part -------------------------------------------------
+    def scf_setup(self, B, hbm, mass, e2, neutrons, protons, com1body=2, cexchange=2,
+                  coulomb=1, coulomb_maxit=500, max_moment=10, damping=0.75):
+        f = FunctionalC((ctypes.c_double * 28)(*[float(x) for x in B]),
+                        (ctypes.c_double * 2)(*hbm), (ctypes.c_double * 2)(*mass),
+                        float(e2), float(neutrons), float(protons), int(com1body),
+                        int(cexchange), int(coulomb), int(coulomb_maxit), int(max_moment),
+                        float(damping))
+        self._chk(self.lib.mocca_scf_setup(self.h, ctypes.byref(f)))
+
+    def hf_fill(self):
+        self._chk(self.lib.mocca_hf_fill(self.h))
+
+    def occupations(self):
+        occ = np.zeros(self.M)
+        self._chk(self.lib.mocca_get_occupations(self.h, _p(occ)))
+        return occ
+
+    def update_densities(self):
+        self._chk(self.lib.mocca_update_densities(self.h))
+
+    def mix_densities(self):
+        self._chk(self.lib.mocca_mix_densities(self.h))
+
+    def construct_potentials(self):
+        self._chk(self.lib.mocca_construct_potentials(self.h))
+
+    def comp_energy(self):
+        e = np.zeros(43)
+        self._chk(self.lib.mocca_comp_energy(self.h, _p(e)))
+        return e
+
+    def scf_start(self):
+        self._chk(self.lib.mocca_scf_start(self.h))
+
+    def iterate(self, niter, dt_over_hbar, momentum, fetch=True):
+        e = np.zeros((niter + 1, 43)) if fetch else None
+        self._chk(self.lib.mocca_iterate(self.h, int(niter), ctypes.c_double(dt_over_hbar),
+                                         ctypes.c_double(momentum), _p(e)))
+        return e
+
+    def moments(self):
+        lm = np.zeros(2 * 28, np.int32)
+        q = np.zeros(2 * 28)
+        cnt = ctypes.c_int()
+        self._chk(self.lib.mocca_get_moments(self.h, _p(lm), _p(q), ctypes.byref(cnt)))
+        n = cnt.value
+        return lm[:2 * n].reshape(n, 2), q[:2 * n].reshape(n, 2)
+
+    def coulomb(self):
+        N = self.N
+        vc, vx = np.zeros(N), np.zeros(N)
+        cp = np.zeros((self.nz + 2) * (self.ny + 2) * (self.nx + 2))
+        it = ctypes.c_int()
+        self._chk(self.lib.mocca_get_coulomb(self.h, _p(vc), _p(vx), _p(cp), ctypes.byref(it)))
+        return vc, vx, cp, it.value
+
+    def potentials(self):
+        N = self.N
+        out = dict(U=np.zeros((2, N)), B=np.zeros((2, N)), NablaB=np.zeros((2, 3, N)),
+                   W=np.zeros((2, 3, 3, N)), DivW=np.zeros((2, 3, N)), S=np.zeros((2, 3, N)),
+                   A=np.zeros((2, 3, N)), DivA=np.zeros((2, N)))
+        keys = ("U", "B", "NablaB", "W", "DivW", "S", "A", "DivA")
+        self._chk(self.lib.mocca_get_potentials(self.h, *[_p(out[k]) for k in keys]))
+        return out
 
     def profile_enable(self, on=True):
         self._chk(self.lib.mocca_profile_enable(self.h, int(on)))

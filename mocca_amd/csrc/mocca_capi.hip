@@ -1,84 +1,20 @@
 // C ABI of the mocca_amd library: solve handles, device memory, launches.
 // The entry points and their reference counterparts are documented in
 // include/mocca_amd.h.
-#include "../../include/mocca_amd.h"
-#include "mocca_kernels.h"
+#include "mocca_internal.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <map>
-#include <string>
-#include <vector>
 
 using namespace mocca;
 
-namespace {
+namespace mocca {
+namespace detail {
 
 thread_local std::string g_err;
-
-struct Pending {
-    const char *name;
-    hipEvent_t a, b;
-};
-
-struct Buffers {
-    void *psi = nullptr, *psi2 = nullptr, *upd = nullptr, *hbuf = nullptr;
-    void *fields = nullptr;        // T [48][N]
-    void *dpart = nullptr;         // T [nchunks][9][N]
-    void *lagT = nullptr;          // T lagrange matrices
-    double *lagD = nullptr;        // double lagrange matrices
-    double *der = nullptr, *lap = nullptr;
-    double *dens = nullptr;        // [2][9][N]
-    double *derived = nullptr;     // derrho[2][3] laprho[2] rotvecj[2][3] rots[2][3] ders[2][3][3]
-    int *qn = nullptr;
-    double *occ = nullptr, *E = nullptr, *D = nullptr;
-    double *eparts = nullptr;
-    int *order = nullptr;
-    Chunk *chunks = nullptr;
-    int *all_states = nullptr;
-    int *blk_states = nullptr, *blk_m = nullptr, *perm = nullptr;
-    double *G = nullptr, *Tm = nullptr, *work = nullptr, *gpart = nullptr;
-    int *status = nullptr;
-    FieldJob *jobs = nullptr;
-    Chunk *one_chunk = nullptr;
-    int *one_order = nullptr;
-    Task *tasks = nullptr, *one_tasks = nullptr;
-    Chunk *dchunks = nullptr;
-    DTask *dtasks = nullptr;
-};
-
-}  // namespace
-
-struct mocca_solve {
-    mocca_mesh mesh{};
-    int M = 0, N = 0, prec = MOCCA_FP64, tsize = 8;
-    hipStream_t st = nullptr;
-    std::string err;
-    Buffers b;
-    MeshK<double> md{};
-    MeshK<float> mf{};
-    bool meta_set = false, fields_set = false;
-    double kin[2] = {0, 0};
-    std::vector<int> qn;
-    std::vector<double> occ;
-    int nchunks = 0, ntiles = 0;
-    int nblk = 0, max_m = 0, nsplit = 1;
-    int njobs = 0;
-    // z-marching FD kernels: y-tile rows, tiles per plane, task count
-    bool march = false;
-    int TY = 0, ntile_y = 0, ntasks = 0;
-    int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
-    // profiling
-    bool prof = false;
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> pool;
-    std::map<std::string, std::pair<double, long>> acc;
-};
-
-namespace {
 
 int fail(mocca_solve *h, int code, const char *fmt, ...) {
     char buf[512];
@@ -91,25 +27,6 @@ int fail(mocca_solve *h, int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIPCHK(h, call)                                                           \
-    do {                                                                          \
-        hipError_t e_ = (call);                                                   \
-        if (e_ != hipSuccess)                                                     \
-            return fail(h, MOCCA_ERR_HIP, "%s failed: %s", #call,                 \
-                        hipGetErrorString(e_));                                   \
-    } while (0)
-
-template <typename P>
-int dalloc(mocca_solve *h, P **p, size_t bytes) {
-    void *q = nullptr;
-    hipError_t e = hipMalloc(&q, bytes ? bytes : 8);
-    if (e != hipSuccess)
-        return fail(h, MOCCA_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes,
-                    hipGetErrorString(e));
-    *p = static_cast<P *>(q);
-    return 0;
-}
-
 hipEvent_t take_event(mocca_solve *h) {
     if (!h->pool.empty()) {
         hipEvent_t e = h->pool.back();
@@ -119,24 +36,6 @@ hipEvent_t take_event(mocca_solve *h) {
     hipEvent_t e;
     (void)hipEventCreate(&e);
     return e;
-}
-
-template <typename F>
-int run(mocca_solve *h, const char *name, F &&f) {
-    if (h->prof) {
-        hipEvent_t a = take_event(h), b = take_event(h);
-        (void)hipEventRecord(a, h->st);
-        f();
-        (void)hipEventRecord(b, h->st);
-        h->pending.push_back({name, a, b});
-    } else {
-        f();
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return fail(h, MOCCA_ERR_HIP, "launch of %s failed: %s", name,
-                    hipGetErrorString(e));
-    return 0;
 }
 
 int drain_profile(mocca_solve *h) {
@@ -386,11 +285,50 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
     if (rc) return rc;
     rc = run(h, "reduce_ed", [&] {
         launch_reduce_ed(h->b.eparts, slots, h->b.all_states, h->M, h->mesh.dv,
-                         h->b.E, h->b.D, h->st);
+                         h->b.E, h->b.D, h->b.K, h->st);
     });
     if (rc) return rc;
     // Gram-Schmidt of the stepped states psi2 -> psi, with u = psi_new - u.
     return gram_schmidt_impl<T>(h, psi2, psi, upd, 1);
+}
+
+// hPsi of every state without outputs other than the per-state sums; only
+// K = dv <psi|Lap psi> is kept (CompKinetic's inner products).
+template <typename T>
+int kinetic_impl(mocca_solve *h) {
+    const MeshK<T> &m = mesh_of<T>(h);
+    int rc, slots;
+    if (h->march) {
+        slots = h->ntile_y;
+        rc = run(h, "kinetic", [&] {
+            launch_hpsi_fd<T>(m, (const T *)h->b.psi, nullptr, nullptr, nullptr,
+                              (const T *)h->b.fields, h->b.qn, h->b.tasks, h->ntasks, h->TY,
+                              h->ntile_y, h->mesh.time_reversal, T(h->kin[0]), T(h->kin[1]),
+                              T(0), T(0), 0, h->b.eparts, h->st);
+        });
+    } else {
+        slots = h->ntiles;
+        rc = run(h, "kinetic", [&] {
+            launch_hpsi_step<T>(m, (const T *)h->b.psi, nullptr, nullptr, nullptr,
+                                (const T *)h->b.fields, h->b.qn, h->b.order, h->b.chunks,
+                                h->nchunks, h->mesh.time_reversal, T(h->kin[0]),
+                                T(h->kin[1]), T(0), T(0), 0, h->b.eparts, h->ntiles, h->st);
+        });
+    }
+    if (rc) return rc;
+    return run(h, "reduce_ed", [&] {
+        launch_reduce_ed(h->b.eparts, slots, h->b.all_states, h->M, h->mesh.dv, nullptr,
+                         nullptr, h->b.K, h->st);
+    });
+}
+
+int grad_desc_any(mocca_solve *h, double dth, double mu) {
+    return h->prec == MOCCA_FP64 ? grad_desc_impl<double>(h, dth, mu)
+                                 : grad_desc_impl<float>(h, dth, mu);
+}
+
+int kinetic_any(mocca_solve *h) {
+    return h->prec == MOCCA_FP64 ? kinetic_impl<double>(h) : kinetic_impl<float>(h);
 }
 
 template <typename T>
@@ -453,6 +391,10 @@ int density_impl(mocca_solve *h) {
     });
 }
 
+int density_any(mocca_solve *h) {
+    return h->prec == MOCCA_FP64 ? density_impl<double>(h) : density_impl<float>(h);
+}
+
 template <typename T>
 int upload_convert(mocca_solve *h, T *dst, const double *src, size_t n) {
     if (sizeof(T) == sizeof(double)) {
@@ -491,18 +433,22 @@ int down(mocca_solve *h, double *dst, const void *src, size_t n) {
 void free_all(mocca_solve *h) {
     Buffers &b = h->b;
     void *ptrs[] = {b.psi, b.psi2, b.upd, b.hbuf, b.fields, b.dpart, b.lagT, b.lagD,
-                    b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.eparts,
+                    b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.K, b.eparts,
                     b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
                     b.G, b.Tm, b.work, b.gpart, b.status, b.jobs, b.one_chunk,
                     b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    scf_free(h);
     for (auto &p : h->pending) { h->pool.push_back(p.a); h->pool.push_back(p.b); }
     for (auto e : h->pool) (void)hipEventDestroy(e);
     if (h->st) (void)hipStreamDestroy(h->st);
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace mocca
+
+using namespace mocca::detail;
 
 // ===========================================================================
 extern "C" {
@@ -568,7 +514,9 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     h->TY = plane_tile_rows(mesh->nx, mesh->ny);
     h->ntile_y = (mesh->ny + h->TY - 1) / h->TY;
     TRY(dalloc(h, &b.one_tasks, h->ntile_y * sizeof(Task)));
-    TRY(dalloc(h, &b.eparts, M * (size_t)h->ntiles * 2 * sizeof(double)));
+    TRY(dalloc(h, &b.eparts, M * (size_t)h->ntiles * 3 * sizeof(double)));
+    TRY(dalloc(h, &b.K, M * sizeof(double)));
+    HIPCHK(h, hipMemset(b.K, 0, M * sizeof(double)));
     HIPCHK(h, hipMemset(b.upd, 0, M * 4 * N * ts));
     HIPCHK(h, hipMemset(b.fields, 0, 48 * N * ts));
     HIPCHK(h, hipMemset(b.dens, 0, 2 * 9 * N * sizeof(double)));
@@ -902,8 +850,7 @@ int mocca_hpsi(mocca_solve *h, int state, double *out) {
 int mocca_grad_desc(mocca_solve *h, double dt_over_hbar, double momentum) {
     if (int rc = check_ready(h)) return rc;
     if (!h->fields_set) return fail(h, MOCCA_ERR_STATE, "mocca_set_potentials not called");
-    return h->prec == MOCCA_FP64 ? grad_desc_impl<double>(h, dt_over_hbar, momentum)
-                                 : grad_desc_impl<float>(h, dt_over_hbar, momentum);
+    return grad_desc_any(h, dt_over_hbar, momentum);
 }
 
 int mocca_gram_schmidt(mocca_solve *h) {
@@ -920,7 +867,7 @@ int mocca_gram_schmidt(mocca_solve *h) {
 
 int mocca_compute_density(mocca_solve *h, mocca_density *out) {
     if (int rc = check_ready(h)) return rc;
-    int rc = h->prec == MOCCA_FP64 ? density_impl<double>(h) : density_impl<float>(h);
+    int rc = density_any(h);
     if (rc) return rc;
     if (!out) return 0;
     HIPCHK(h, hipStreamSynchronize(h->st));

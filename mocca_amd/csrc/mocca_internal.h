@@ -1,0 +1,164 @@
+// Internal state of a solve handle, shared by the C-ABI translation units
+// (mocca_capi.hip: states, hPsi/GradDesc, Gram-Schmidt, densities;
+// mocca_scf_capi.hip: filling, mixing, moments, Coulomb, potentials, energy).
+#pragma once
+
+#include "../../include/mocca_amd.h"
+#include "mocca_kernels.h"
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace mocca {
+namespace detail {
+
+struct Pending {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct Buffers {
+    void *psi = nullptr, *psi2 = nullptr, *upd = nullptr, *hbuf = nullptr;
+    void *fields = nullptr;        // T [48][N]
+    void *dpart = nullptr;         // T [nchunks][9][N]
+    void *lagT = nullptr;          // T lagrange matrices
+    double *lagD = nullptr;        // double lagrange matrices
+    double *der = nullptr, *lap = nullptr;
+    double *dens = nullptr;        // [2][9][N]: rho tau nablaJ j(3) s(3)
+    double *derived = nullptr;     // derrho[2][3] laprho[2] rotvecj[2][3] rots[2][3] ders[2][3][3]
+    int *qn = nullptr;
+    double *occ = nullptr, *E = nullptr, *D = nullptr, *K = nullptr;
+    double *eparts = nullptr;      // [M][tiles][3]: <psi|h psi>, <h psi|h psi>, <psi|Lap psi>
+    int *order = nullptr;
+    Chunk *chunks = nullptr;
+    int *all_states = nullptr;
+    int *blk_states = nullptr, *blk_m = nullptr, *perm = nullptr;
+    double *G = nullptr, *Tm = nullptr, *work = nullptr, *gpart = nullptr;
+    int *status = nullptr;
+    FieldJob *jobs = nullptr;
+    Chunk *one_chunk = nullptr;
+    int *one_order = nullptr;
+    Task *tasks = nullptr, *one_tasks = nullptr;
+    Chunk *dchunks = nullptr;
+    DTask *dtasks = nullptr;
+};
+
+// Device buffers and settings of the self-consistent part.
+struct Scf {
+    bool set = false, started = false;
+    SkyrmeK sk{};
+    CoulombK ck{};
+    double damping = 0.75;
+    int ub_n = 0, ub_p = 0;
+    double fill_fac = 1.0;
+    int nmom = 0, lm[2 * kMaxMom] = {};
+    int nb = 0, cg_nwg = 0, mom_nblk = 0, e_nblk = 0, npjobs = 0;
+    bool coulomb_first = true;
+    double *hist = nullptr;        // [18N] dens + [20N] derived (derrho laprho rotvecj rots)
+    double *Y = nullptr;           // [nmom][N] r^l Y_lm on the mesh
+    double *mpart = nullptr;       // [mom_nblk][kMaxMom][2]
+    double *Q = nullptr;           // [kMaxMom][2]
+    double *lfac = nullptr;        // [nmom] 4 pi / (2l+1)
+    int *bidx = nullptr;           // [nb] boundary points of the Coulomb box
+    double *bsh = nullptr, *brp = nullptr;  // [nmom][nb] r^l Y_lm and r^(2l+1) there
+    double *cx = nullptr, *cr = nullptr, *cpa = nullptr, *cpb = nullptr, *cap = nullptr;
+    double *cpart = nullptr;       // [2 cg_nwg]
+    double *vcoul = nullptr, *vcoulx = nullptr;
+    int *cg_iters = nullptr;
+    double *pot = nullptr;         // FP64 potentials [48][N] (aliases fields in FP64)
+    FieldJob *pjobs = nullptr;
+    double *epart = nullptr;       // [e_nblk][kEnergySlots]
+    double *ecur = nullptr;        // [43] energy of the current state
+    double *erows = nullptr;       // [erow_cap][43]
+    int erow_cap = 0;
+};
+
+}  // namespace detail
+}  // namespace mocca
+
+struct mocca_solve {
+    mocca_mesh mesh{};
+    int M = 0, N = 0, prec = MOCCA_FP64, tsize = 8;
+    hipStream_t st = nullptr;
+    std::string err;
+    mocca::detail::Buffers b;
+    mocca::detail::Scf scf;
+    mocca::MeshK<double> md{};
+    mocca::MeshK<float> mf{};
+    bool meta_set = false, fields_set = false;
+    double kin[2] = {0, 0};
+    std::vector<int> qn;
+    std::vector<double> occ;
+    int nchunks = 0, ntiles = 0;
+    int nblk = 0, max_m = 0, nsplit = 1;
+    int njobs = 0;
+    // z-marching FD kernels: y-tile rows, tiles per plane, task count
+    bool march = false;
+    int TY = 0, ntile_y = 0, ntasks = 0;
+    int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
+    // profiling
+    bool prof = false;
+    std::vector<mocca::detail::Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, long>> acc;
+};
+
+namespace mocca {
+namespace detail {
+
+int fail(mocca_solve *h, int code, const char *fmt, ...);
+
+#define HIPCHK(h, call)                                                           \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return ::mocca::detail::fail(h, MOCCA_ERR_HIP, "%s failed: %s", #call, \
+                                         hipGetErrorString(e_));                  \
+    } while (0)
+
+template <typename P>
+int dalloc(mocca_solve *h, P **p, size_t bytes) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes ? bytes : 8);
+    if (e != hipSuccess)
+        return fail(h, MOCCA_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes,
+                    hipGetErrorString(e));
+    *p = static_cast<P *>(q);
+    return 0;
+}
+
+hipEvent_t take_event(mocca_solve *h);
+
+// Launch f on the handle's stream, timed with events when profiling is on.
+template <typename F>
+int run(mocca_solve *h, const char *name, F &&f) {
+    if (h->prof) {
+        hipEvent_t a = take_event(h), b = take_event(h);
+        (void)hipEventRecord(a, h->st);
+        f();
+        (void)hipEventRecord(b, h->st);
+        h->pending.push_back({name, a, b});
+    } else {
+        f();
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(h, MOCCA_ERR_HIP, "launch of %s failed: %s", name, hipGetErrorString(e));
+    return 0;
+}
+
+int check_ready(mocca_solve *h);
+int sign_first_h(int axis, int P, int S, int TS, int c, int lag);
+// GradDesc on the device (hPsi + step + energies + Gram-Schmidt); also
+// leaves K[i] = dv <psi_i|Lap psi_i> of the states before the step.
+int grad_desc_any(mocca_solve *h, double dth, double mu);
+// Densities of the current states into dens/derived.
+int density_any(mocca_solve *h);
+// K[i] = dv <psi_i|Lap psi_i> of the current states (no other output).
+int kinetic_any(mocca_solve *h);
+// Releases the buffers of the self-consistent part.
+void scf_free(mocca_solve *h);
+
+}  // namespace detail
+}  // namespace mocca

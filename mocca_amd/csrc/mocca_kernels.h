@@ -94,8 +94,10 @@ void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
                       T dth, T mu, int mode, double *eparts, int ntiles,
                       hipStream_t st);
 
+// E = dv sum psi.h, D = dv sum h.h - E^2, K = dv sum psi.Lap psi per state
+// (any of E, D, K may be null).
 void launch_reduce_ed(const double *eparts, int ntiles, const int *states,
-                      int nstates, double dv, double *E, double *D,
+                      int nstates, double dv, double *E, double *D, double *K,
                       hipStream_t st);
 
 template <typename T>
@@ -135,5 +137,52 @@ template <typename T>
 void launch_density_fd(const MeshK<T> &m, const T *psi, const int *qn, const double *occ,
                        const int *order, const Chunk *dchunks, const DTask *tasks,
                        int ntasks, int TY, int trc, T *part, hipStream_t st);
+
+// ---- self-consistent part (mocca_scf.hip) ----------------------------------
+
+constexpr int kMaxMom = 28;      // multipole moments (even l <= 12, even m)
+
+// Skyrme-type functional: the B coefficients of Force.f90:38-41 and the
+// constants the potentials and the energy need.
+struct SkyrmeK {
+    double B1, B2, B3, B4, B5, B6, B7a, B7b, B8a, B8b, byt3a, byt3b, B9, B9q, B10, B11,
+        B12a, B12b, B13a, B13b;
+    double hbm[2], mass[2], e2, neutrons, protons;
+    int com1body, cexchange, coulomb, pad;
+};
+
+// Coulomb box: interior nx*ny*nz, box strides mx = nx+2, my = ny+2.
+struct CoulombK {
+    int nx, ny, nz, mx, my, mz;
+    double f;      // 1/dx^2
+    double prec;   // stop when sum r^2 <= prec
+    double e2;
+    int maxit, pad;
+};
+
+void launch_hf_fill(const double *E, const int *qn, int M, int ub_n, int ub_p, double fac,
+                    double *occ, hipStream_t st);
+void launch_mix(double *d, double *h, long n, double a, int keep, hipStream_t st);
+void launch_moments(const double *rho_n, const double *rho_p, const double *Y, int nmom, long N,
+                    double dv, double *part, int nblk, double *Q, hipStream_t st);
+void launch_coulomb_bc(const double *Q, int nmom, const double *lfac, const int *bidx,
+                       const double *bsh, const double *brp, int nb, double e2, double *x,
+                       hipStream_t st);
+// Cooperative launch of the whole CG solve; returns the hipError_t.
+int launch_coulomb_cg(const CoulombK &c, double *x, const double *rho_p, double *r, double *pa,
+                      double *pb, double *ap, double *part, int nwg, double *vcoul,
+                      double *vcoulx, int *iters, hipStream_t st);
+int coulomb_cg_max_groups();
+void launch_potentials(const SkyrmeK &f, long N, int trc, const double *dens,
+                       const double *derived, const double *vcoul, const double *vcoulx,
+                       double *pot, hipStream_t st);
+void launch_convert_f32(const double *a, float *b, long n, hipStream_t st);
+// e[43] = [total, kin(2), E_C, E_Cx, com(2), terms 1..36]
+void launch_energy_terms(const SkyrmeK &f, long N, int trc, const double *dens,
+                         const double *derived, const double *vcoul, double *part, int nblk,
+                         double dv, double *e, hipStream_t st);
+void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ, const int *qn,
+                           int M, double *e, hipStream_t st);
+constexpr int kEnergySlots = 35;   // partial sums per workgroup of the energy pass
 
 }  // namespace mocca

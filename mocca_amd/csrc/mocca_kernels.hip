@@ -139,19 +139,20 @@ void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
 
 // ---------------------------------------------------------------------------
 // Block-wide sum of two values (wave64 shuffles, then LDS over the waves).
-__device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o, 64);
         b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
     }
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { sh[2 * w] = a; sh[2 * w + 1] = b; }
+    if (l == 0) { sh[3 * w] = a; sh[3 * w + 1] = b; sh[3 * w + 2] = c; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double sa = 0, sb = 0;
-        for (int k = 0; k < kBlock / 64; ++k) { sa += sh[2 * k]; sb += sh[2 * k + 1]; }
-        a = sa; b = sb;
+        double sa = 0, sb = 0, sc = 0;
+        for (int k = 0; k < kBlock / 64; ++k) { sa += sh[3 * k]; sb += sh[3 * k + 1]; sc += sh[3 * k + 2]; }
+        a = sa; b = sb; c = sc;
     }
     __syncthreads();
 }
@@ -159,8 +160,9 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
 // hPsi (ImaginaryTime.f90:213-304) fused with the heavy-ball step of GradDesc
 // (ImaginaryTime.f90:467-491).  One thread per mesh point, looping over the
 // states of one chunk (one isospin), so the isospin's potentials are read
-// once per point and reused.  Partial sums of <psi|h psi> and <h psi|h psi>
-// go to eparts[state][tile].
+// once per point and reused.  Partial sums of <psi|h psi>, <h psi|h psi> and
+// <psi|Lap psi> (the kinetic sum of CompKinetic, Energy.f90:1426-1456) go to
+// eparts[state][tile][3].
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hpsi_step(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
     const int *__restrict__ qn, const int *__restrict__ order,
     const Chunk *__restrict__ chunks, int trc, T kin0, T kin1, T dth, T mu,
     int mode, double *__restrict__ eparts, int ntiles) {
-    __shared__ double sh[2 * kBlock / 64];
+    __shared__ double sh[3 * kBlock / 64];
     const long N = m.N;
     const long p = (long)blockIdx.x * kBlock + threadIdx.x;
     const Chunk ch = chunks[blockIdx.y];
@@ -245,11 +247,12 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
                 h[0] += aa[1]; h[1] -= aa[0]; h[2] += aa[3]; h[3] -= aa[2];
             }
         }
-        double e = 0, dd = 0;
+        double e = 0, dd = 0, kk = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             e += (double)v[c] * (double)h[c];
             dd += (double)h[c] * (double)h[c];
+            kk += (double)v[c] * (double)l[c];
         }
         if (valid) {
             if (mode & kWriteH) {
@@ -268,10 +271,12 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
             }
         }
         if (eparts) {
-            block_sum2(e, dd, sh);
+            block_sum3(e, dd, kk, sh);
             if (threadIdx.x == 0) {
-                eparts[((long)s * ntiles + blockIdx.x) * 2] = e;
-                eparts[((long)s * ntiles + blockIdx.x) * 2 + 1] = dd;
+                double *ep = eparts + ((long)s * ntiles + blockIdx.x) * 3;
+                ep[0] = e;
+                ep[1] = dd;
+                ep[2] = kk;
             }
         }
     }
@@ -294,27 +299,31 @@ void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
 __global__ __launch_bounds__(kBlock) void k_reduce_ed(const double *__restrict__ eparts,
                                                       int ntiles,
                                                       const int *__restrict__ states,
-                                                      double dv, double *E, double *D) {
-    __shared__ double sh[2 * kBlock / 64];
+                                                      double dv, double *E, double *D,
+                                                      double *K) {
+    __shared__ double sh[3 * kBlock / 64];
     const int s = states[blockIdx.x];
-    double a = 0, b = 0;
+    double a = 0, b = 0, c = 0;
     for (int t = threadIdx.x; t < ntiles; t += kBlock) {
-        a += eparts[((long)s * ntiles + t) * 2];
-        b += eparts[((long)s * ntiles + t) * 2 + 1];
+        const double *ep = eparts + ((long)s * ntiles + t) * 3;
+        a += ep[0];
+        b += ep[1];
+        c += ep[2];
     }
-    block_sum2(a, b, sh);
+    block_sum3(a, b, c, sh);
     if (threadIdx.x == 0) {
         const double e = a * dv;
-        E[s] = e;
-        D[s] = b * dv - e * e;
+        if (E) E[s] = e;
+        if (D) D[s] = b * dv - e * e;
+        if (K) K[s] = c * dv;
     }
 }
 
 void launch_reduce_ed(const double *eparts, int ntiles, const int *states,
-                      int nstates, double dv, double *E, double *D,
+                      int nstates, double dv, double *E, double *D, double *K,
                       hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_ed, dim3(nstates), dim3(kBlock), 0, st, eparts,
-                       ntiles, states, dv, E, D);
+                       ntiles, states, dv, E, D, K);
 }
 
 // ---------------------------------------------------------------------------

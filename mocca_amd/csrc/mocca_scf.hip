@@ -1,0 +1,631 @@
+// Self-consistent part of the imaginary-time iteration on the device
+// (gfx950, wave64): HF filling, density mixing, multipole moments, the
+// Coulomb Poisson solve, the mean-field potentials and the energy.
+//
+// All of it is FP64 (it is O(N) per iteration against the O(M N) of the
+// spinor kernels).  Reductions are deterministic: per-workgroup partial sums
+// in a fixed layout, combined in a fixed order.
+#include <hip/hip_cooperative_groups.h>
+
+#include "mocca_kernels.h"
+
+namespace mocca {
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
+}
+
+// Sum of one value over the workgroup (kBlock threads); result on every thread.
+__device__ __forceinline__ double block_sum(double a, double *sh) {
+    a = wave_sum(a);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    __syncthreads();
+    if (l == 0) sh[w] = a;
+    __syncthreads();
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) s += sh[k];
+    return s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// NaiveFill (HartreeFock.f90:177-262): the states are ordered by a stable
+// ascending sort of the energies (OrderSpwfs, SpwfStorage.f90:754-793); the
+// first ub[it] states of each isospin in that order get occupation fac
+// (2 under time reversal, which doubles every occupation), the rest 0.
+// A state's place in the order is the number of states before it, so one
+// thread per state decides its own occupation.
+__global__ __launch_bounds__(kBlock) void k_hf_fill(const double *__restrict__ E,
+                                                     const int *__restrict__ qn, int M,
+                                                     int ub_n, int ub_p, double fac,
+                                                     double *__restrict__ occ) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    const int iso = qn[4 * i];
+    const double e = E[i];
+    int before = 0;
+    for (int j = 0; j < M; ++j) {
+        if (qn[4 * j] != iso) continue;
+        const double f = E[j];
+        before += (f < e || (f == e && j < i)) ? 1 : 0;
+    }
+    const int ub = iso == -1 ? ub_n : ub_p;
+    occ[i] = before < ub ? fac : 0.0;
+}
+
+void launch_hf_fill(const double *E, const int *qn, int M, int ub_n, int ub_p, double fac,
+                    double *occ, hipStream_t st) {
+    hipLaunchKernelGGL(k_hf_fill, dim3((M + kBlock - 1) / kBlock), dim3(kBlock), 0, st, E, qn,
+                       M, ub_n, ub_p, fac, occ);
+}
+
+// ---------------------------------------------------------------------------
+// MixDensities, scheme 0 (DensityMixing.f90:39-45): d = (1-a) d + a h over
+// every mixed density; with keep != 0 the history takes the mixed value too
+// (what the next UpdateDensities stores as DensityHistory(1)).
+__global__ __launch_bounds__(kBlock) void k_mix(double *__restrict__ d, double *__restrict__ h,
+                                                 long n, double a, int keep) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double v = (1.0 - a) * d[i] + a * h[i];
+    d[i] = v;
+    if (keep) h[i] = v;
+}
+
+void launch_mix(double *d, double *h, long n, double a, int keep, hipStream_t st) {
+    hipLaunchKernelGGL(k_mix, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d, h, n, a,
+                       keep);
+}
+
+// ---------------------------------------------------------------------------
+// Multipole moments (Calculate_Multipole, Moments.f90:1907-2004):
+// Q[k][it] = dv * sum_p Y[k][p] rho_it(p).  Pass 1: per-workgroup partial
+// sums part[wg][k][it]; pass 2: one thread per (k, it) sums the partials in
+// workgroup order.
+__global__ __launch_bounds__(kBlock) void k_moments(const double *__restrict__ rho_n,
+                                                     const double *__restrict__ rho_p,
+                                                     const double *__restrict__ Y, int nmom,
+                                                     long N, double *__restrict__ part) {
+    __shared__ double sh[kBlock / 64];
+    double an[kMaxMom], ap[kMaxMom];
+#pragma unroll
+    for (int k = 0; k < kMaxMom; ++k) { an[k] = 0; ap[k] = 0; }
+    for (long p = (long)blockIdx.x * kBlock + threadIdx.x; p < N; p += (long)gridDim.x * kBlock) {
+        const double rn = rho_n[p], rp = rho_p[p];
+#pragma unroll
+        for (int k = 0; k < kMaxMom; ++k)
+            if (k < nmom) {
+                const double y = Y[k * N + p];
+                an[k] += y * rn;
+                ap[k] += y * rp;
+            }
+    }
+    for (int k = 0; k < nmom; ++k) {
+        const double sn = block_sum(an[k], sh);
+        const double sp = block_sum(ap[k], sh);
+        if (threadIdx.x == 0) {
+            part[((long)blockIdx.x * kMaxMom + k) * 2] = sn;
+            part[((long)blockIdx.x * kMaxMom + k) * 2 + 1] = sp;
+        }
+    }
+}
+
+__global__ void k_moments_reduce(const double *__restrict__ part, int nblk, int nmom, double dv,
+                                 double *__restrict__ Q) {
+    const int t = threadIdx.x;
+    if (t >= 2 * nmom) return;
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += part[(long)b * kMaxMom * 2 + t];
+    Q[t] = s * dv;
+}
+
+void launch_moments(const double *rho_n, const double *rho_p, const double *Y, int nmom, long N,
+                    double dv, double *part, int nblk, double *Q, hipStream_t st) {
+    hipLaunchKernelGGL(k_moments, dim3(nblk), dim3(kBlock), 0, st, rho_n, rho_p, Y, nmom, N,
+                       part);
+    hipLaunchKernelGGL(k_moments_reduce, dim3(1), dim3(2 * kMaxMom), 0, st, part, nblk, nmom, dv,
+                       Q);
+}
+
+// ---------------------------------------------------------------------------
+// Coulomb boundary values (CoulombBound, Coulomb.f90:583-674): on the
+// boundary planes of the Coulomb box, V = sum_lm Q_lm(p) 4pi/(2l+1) e2
+// r^l Y_lm / r^(2l+1); a non-zero value replaces the potential there.
+__global__ __launch_bounds__(kBlock) void k_coulomb_bc(const double *__restrict__ Q, int nmom,
+                                                        const double *__restrict__ lfac,
+                                                        const int *__restrict__ bidx,
+                                                        const double *__restrict__ bsh,
+                                                        const double *__restrict__ brp, int nb,
+                                                        double e2, double *__restrict__ x) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nb) return;
+    double bc = 0.0;
+    for (int k = 0; k < nmom; ++k) {
+        const double value = Q[2 * k + 1] * lfac[k];
+        bc = bc + value * bsh[(long)k * nb + b] * e2 / brp[(long)k * nb + b];
+    }
+    if (bc != 0.0) x[bidx[b]] = bc;
+}
+
+void launch_coulomb_bc(const double *Q, int nmom, const double *lfac, const int *bidx,
+                       const double *bsh, const double *brp, int nb, double e2, double *x,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_coulomb_bc, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, Q,
+                       nmom, lfac, bidx, bsh, brp, nb, e2, x);
+}
+
+// ---------------------------------------------------------------------------
+// Conjugate gradients for Lap V = -4 pi e2 rho_p on the Coulomb box
+// (ConjugGrad, Coulomb.f90:755-900; CoulombLaplacian / Coulomb_1D,
+// CoulombDerivatives.f90:144-281): 5-point second derivative per axis,
+// mirror (sign +1) below the symmetry planes, boundary planes held fixed.
+// The reference step length a = r.r / r.Ap and stop sum r^2 <= prec.
+//
+// One cooperative launch runs the whole solve.  Per iteration: phase A
+// forms p = r + c p_old on the fly (double-buffered p), Ap and the partial
+// r.Ap; grid barrier; phase B updates V and r and the partial r.r; grid
+// barrier.  Every workgroup sums the partials in the same order, so all
+// take the same decisions and leave the loop together.
+namespace {
+
+__device__ __forceinline__ int mirror(int a) { return a < 0 ? -1 - a : a; }
+
+// Fixed-order sum of nwg partials; identical on every workgroup.
+__device__ __forceinline__ double grid_total(const double *part, int nwg, double *sh) {
+    double s = 0;
+    if (threadIdx.x < 64) {
+        for (int b = threadIdx.x; b < nwg; b += 64) s += part[b];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) sh[0] = s;
+    }
+    __syncthreads();
+    s = sh[0];
+    __syncthreads();
+    return s;
+}
+
+}  // namespace
+
+template <typename PF>
+__device__ __forceinline__ double coul_lap(const CoulombK &c, int i, int j, int k, PF &&val) {
+    constexpr double c2 = -1.0 / 12.0, c1 = 4.0 / 3.0, c0 = -5.0 / 2.0;
+    const double cf[5] = {c2, c1, c0, c1, c2};
+    auto at = [&](int a, int b, int d) { return val(((long)d * c.my + b) * c.mx + a); };
+    double dx = 0, dy = 0, dz = 0;
+#pragma unroll
+    for (int q = -2; q <= 2; ++q) {
+        dx = dx + cf[q + 2] * at(mirror(i + q), j, k);
+        dy = dy + cf[q + 2] * at(i, mirror(j + q), k);
+        dz = dz + cf[q + 2] * at(i, j, mirror(k + q));
+    }
+    return (dx * c.f + dy * c.f) + dz * c.f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_coulomb_cg(CoulombK c, double *__restrict__ x,
+                                                        const double *__restrict__ rho_p,
+                                                        double *__restrict__ r,
+                                                        double *__restrict__ pa,
+                                                        double *__restrict__ pb,
+                                                        double *__restrict__ ap,
+                                                        double *__restrict__ part,
+                                                        double *__restrict__ vcoul,
+                                                        double *__restrict__ vcoulx,
+                                                        int *__restrict__ iters) {
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ double sh[kBlock / 64 + 1];
+    const int nwg = gridDim.x;
+    const long stride = (long)nwg * kBlock;
+    const long nint = (long)c.nx * c.ny * c.nz;
+    const double src_f = -4.0 * 3.14159265358979323846 * c.e2;
+    auto coords = [&](long q, int &i, int &j, int &k) {
+        i = q % c.nx;
+        j = (q / c.nx) % c.ny;
+        k = q / ((long)c.nx * c.ny);
+    };
+    auto bidx = [&](int i, int j, int k) { return ((long)k * c.my + j) * c.mx + i; };
+    // r = -Lap V + source, p = r, norm = r.r
+    double loc = 0;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nint; q += stride) {
+        int i, j, k;
+        coords(q, i, j, k);
+        const double lv = coul_lap(c, i, j, k, [&](long t) { return x[t]; });
+        double rv = -1.0 * lv;
+        rv = rv + src_f * rho_p[q];
+        r[bidx(i, j, k)] = rv;
+        pa[bidx(i, j, k)] = rv;
+        loc += rv * rv;
+    }
+    // (partials of r.r live in part[nwg..2nwg), those of r.Ap in part[0..nwg):
+    // a workgroup may only overwrite a region after the barrier that follows
+    // every other workgroup's read of it)
+    double s = block_sum(loc, sh);
+    if (threadIdx.x == 0) part[nwg + blockIdx.x] = s;
+    grid.sync();
+    double norm = grid_total(part + nwg, nwg, sh);
+    double cc = 0;
+    double *pcur = pa, *pnext = pb;
+    int it;
+    for (it = 1; it <= c.maxit; ++it) {
+        // phase A: p (on the fly for it > 1), Ap, r.Ap
+        loc = 0;
+        const bool first = it == 1;
+        for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nint; q += stride) {
+            int i, j, k;
+            coords(q, i, j, k);
+            double lv;
+            if (first)
+                lv = coul_lap(c, i, j, k, [&](long t) { return pcur[t]; });
+            else
+                lv = coul_lap(c, i, j, k, [&](long t) { return r[t] + cc * pcur[t]; });
+            const long b = bidx(i, j, k);
+            if (!first) pnext[b] = r[b] + cc * pcur[b];
+            ap[b] = lv;
+            loc += r[b] * lv;
+        }
+        s = block_sum(loc, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+        grid.sync();
+        if (!first) { double *t = pcur; pcur = pnext; pnext = t; }
+        const double integral = grid_total(part, nwg, sh);
+        const double a = norm / integral;
+        // phase B: V += a p, r -= a Ap, r.r
+        loc = 0;
+        for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nint; q += stride) {
+            int i, j, k;
+            coords(q, i, j, k);
+            const long b = bidx(i, j, k);
+            x[b] = x[b] + a * pcur[b];
+            const double rv = r[b] - a * ap[b];
+            r[b] = rv;
+            loc += rv * rv;
+        }
+        s = block_sum(loc, sh);
+        if (threadIdx.x == 0) part[nwg + blockIdx.x] = s;
+        grid.sync();
+        const double nn = grid_total(part + nwg, nwg, sh);
+        if (nn <= c.prec) break;
+        cc = nn / norm;
+        norm = nn;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *iters = it > c.maxit ? c.maxit : it;
+    // Coulomb potential on the mesh and the Slater exchange potential
+    // (SolveCoulomb, Coulomb.f90:329-339).
+    const double fx = -pow(3.0 / 3.14159265358979323846, 1.0 / 3.0) * c.e2;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nint; q += stride) {
+        int i, j, k;
+        coords(q, i, j, k);
+        vcoul[q] = x[bidx(i, j, k)];
+        vcoulx[q] = fx * pow(rho_p[q], 1.0 / 3.0);
+    }
+}
+
+int launch_coulomb_cg(const CoulombK &c, double *x, const double *rho_p, double *r, double *pa,
+                      double *pb, double *ap, double *part, int nwg, double *vcoul,
+                      double *vcoulx, int *iters, hipStream_t st) {
+    CoulombK cc = c;
+    void *args[] = {&cc, &x, &rho_p, &r, &pa, &pb, &ap, &part, &vcoul, &vcoulx, &iters};
+    return (int)hipLaunchCooperativeKernel((const void *)k_coulomb_cg, dim3(nwg), dim3(kBlock),
+                                           args, 0, st);
+}
+
+int coulomb_cg_max_groups() {
+    int dev = 0, per_cu = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_coulomb_cg, kBlock,
+                                                     0) != hipSuccess)
+        return 0;
+    return per_cu * ncu;
+}
+
+// ---------------------------------------------------------------------------
+// Mean-field potentials, pointwise part: U (CalcUPot, MeanFields.f90:269-409),
+// B (CalcBPot :115-150), W (CalcWPot :494-541), S (CalcSPot :618-712) and
+// A (CalcAPot :411-446) into pot[48][N] at the field offsets of the hPsi
+// kernels.  Their derivative fields (NablaB, DivW, DivA) are field jobs.
+__global__ __launch_bounds__(kBlock) void k_potentials(SkyrmeK f, long N, int trc,
+                                                        const double *__restrict__ dens,
+                                                        const double *__restrict__ derived,
+                                                        const double *__restrict__ vcoul,
+                                                        const double *__restrict__ vcoulx,
+                                                        double *__restrict__ pot) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= N) return;
+    const double eps = 1.0e-20;
+    auto D = [&](int it, int slot) { return dens[((long)it * 9 + slot) * N + p]; };
+    const double *derrho = derived, *laprho = derived + 6 * N, *rotj = derived + 8 * N,
+                 *rots = derived + 14 * N;
+    const double rn = D(0, 0), rp = D(1, 0);
+    const double rt = rn + rp;
+    const double njt = D(0, 2) + D(1, 2);
+    const double a3 = f.byt3a, a3b = f.byt3b;
+    double U[2];
+    U[0] = (2.0 * f.B1 + 2.0 * f.B2) * rn + 2.0 * f.B1 * rp;
+    U[1] = (2.0 * f.B1 + 2.0 * f.B2) * rp + 2.0 * f.B1 * rn;
+    const double rta = pow(rt, a3), rt1 = pow(rt + eps, 1 + a3);
+    const double rtb = pow(rt, a3b), rt1b = pow(rt + eps, 1 + a3b);
+    for (int it = 0; it < 2; ++it) {
+        const int at = 1 - it;
+        const double ri = D(it, 0), ra = D(at, 0);
+        U[it] = U[it] + (f.B3 + f.B4) * D(it, 1) + f.B3 * D(at, 1) +
+                2.0 * ((f.B5 + f.B6) * laprho[it * N + p] + f.B5 * laprho[at * N + p]) +
+                (2 + a3) * f.B7a * rt1 +
+                f.B8a * (a3 * rta / (rt + eps) * (ri * ri + ra * ra) + 2.0 * rta * ri) +
+                f.B9 * njt + f.B9q * D(it, 2);
+        if (f.B7b != 0.0 || f.B8b != 0.0)
+            U[it] = U[it] + (2 + a3b) * f.B7b * rt1b +
+                    f.B8b * (a3b * rtb / (rt + eps) * (ri * ri + ra * ra) + 2.0 * rtb * ri);
+    }
+    double s[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    if (!trc) {
+        double st2 = 0, s2[2] = {0, 0};
+        for (int k = 0; k < 3; ++k) {
+            s[0][k] = D(0, 6 + k);
+            s[1][k] = D(1, 6 + k);
+            st2 = st2 + (s[0][k] + s[1][k]) * (s[0][k] + s[1][k]);
+            s2[0] = s2[0] + s[0][k] * s[0][k];
+            s2[1] = s2[1] + s[1][k] * s[1][k];
+        }
+        for (int it = 0; it < 2; ++it)
+            U[it] = U[it] + a3 * rta / (rt + eps) *
+                                (f.B12a * st2 + f.B13a * s2[it] + f.B13a * s2[1 - it]);
+        if (f.B12b != 0.0 || f.B13b != 0.0)
+            for (int it = 0; it < 2; ++it)
+                U[it] = U[it] + a3b * rtb / (rt + eps) *
+                                    (f.B12b * st2 + f.B13b * s2[it] + f.B13b * s2[1 - it]);
+    }
+    U[1] = U[1] + vcoul[p];
+    if (f.cexchange == 2) U[1] = U[1] + vcoulx[p];
+    pot[(kOffU + 0) * N + p] = U[0];
+    pot[(kOffU + 1) * N + p] = U[1];
+    // B (CalcBPot)
+    for (int it = 0; it < 2; ++it)
+        pot[(kOffB + it) * N + p] = f.B3 * D(1 - it, 0) + (f.B3 + f.B4) * D(it, 0);
+    // W_{mu nu} = -sum_k eps_{k mu nu} ((B9+B9q) d_k rho_q + B9 d_k rho_q'),
+    // stored W[it][nu][mu].
+    for (int it = 0; it < 2; ++it) {
+        double g[3];
+        for (int k = 0; k < 3; ++k)
+            g[k] = (f.B9 + f.B9q) * derrho[(it * 3 + k) * N + p] +
+                   f.B9 * derrho[((1 - it) * 3 + k) * N + p];
+        for (int nu = 0; nu < 3; ++nu)
+            for (int mu = 0; mu < 3; ++mu) {
+                double w = 0.0;
+                if (mu != nu) {
+                    const int k = 3 - mu - nu;
+                    // eps_{k mu nu} = +1 for (k, mu, nu) an even permutation of (0,1,2)
+                    const int e = ((mu - k + 3) % 3 == 1) ? 1 : -1;
+                    w = e > 0 ? 0.0 - g[k] : 0.0 + g[k];
+                }
+                pot[(kOffW + (it * 3 + nu) * 3 + mu) * N + p] = w;
+            }
+    }
+    if (trc) return;
+    // S (CalcSPot) and A (CalcAPot)
+    for (int it = 0; it < 2; ++it) {
+        const int at = 1 - it;
+        for (int k = 0; k < 3; ++k) {
+            double sv = +(f.B9 + f.B9q) * rotj[(it * 3 + k) * N + p] +
+                        f.B9 * rotj[(at * 3 + k) * N + p] +
+                        2.0 * ((f.B10 + f.B11) * s[it][k] + f.B10 * s[at][k]) +
+                        2.0 * ((f.B12a + f.B13a) * (rta * s[it][k]) + f.B12a * (rta * s[at][k]));
+            if (f.B12b != 0.0 || f.B13b != 0.0)
+                sv = sv + 2.0 * ((f.B12b + f.B13b) * (rtb * s[it][k]) + f.B12b * (rtb * s[at][k]));
+            pot[(kOffS + it * 3 + k) * N + p] = sv;
+            double av = -2.0 * ((f.B3 + f.B4) * D(it, 3 + k) + f.B3 * D(at, 3 + k));
+            av = av + ((f.B9 + f.B9q) * rots[(it * 3 + k) * N + p] +
+                       f.B9 * rots[(at * 3 + k) * N + p]);
+            pot[(kOffA + it * 3 + k) * N + p] = av;
+        }
+    }
+}
+
+void launch_potentials(const SkyrmeK &f, long N, int trc, const double *dens,
+                       const double *derived, const double *vcoul, const double *vcoulx,
+                       double *pot, hipStream_t st) {
+    hipLaunchKernelGGL(k_potentials, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, f, N,
+                       trc, dens, derived, vcoul, vcoulx, pot);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_convert(const double *__restrict__ a,
+                                                     T *__restrict__ b, long n) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (T)a[i];
+}
+
+void launch_convert_f32(const double *a, float *b, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_convert<float>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, a,
+                       b, n);
+}
+
+// ---------------------------------------------------------------------------
+// Energy (compSkyrme, Energy.f90:623-859; Coulomb energies Coulomb.f90:
+// 215-253).  Pass 1: per-workgroup partial sums of the integrands, slot
+// layout kE*; pass 2 (one workgroup): fixed-order sums, coefficients, dv.
+enum {
+    kE_rt2, kE_rn2, kE_rp2, kE_rttt, kE_rntn, kE_rptp, kE_jt2, kE_jn2, kE_jp2, kE_rtlt,
+    kE_rnln, kE_rplp, kE_rta2, kE_rn2a, kE_rp2a, kE_rtnj, kE_rnnj, kE_rpnj, kE_jrst, kE_jrsn,
+    kE_jrsp, kE_st2, kE_sn2, kE_sp2, kE_ast2, kE_asn2, kE_asp2, kE_rtb2, kE_rn2b, kE_rp2b,
+    kE_bst2, kE_bsn2, kE_bsp2, kE_coul, kE_cx, kE_count
+};
+static_assert(kE_count == kEnergySlots, "energy partial-sum slots");
+
+__global__ __launch_bounds__(kBlock) void k_energy_partial(SkyrmeK f, long N, int trc,
+                                                            const double *__restrict__ dens,
+                                                            const double *__restrict__ derived,
+                                                            const double *__restrict__ vcoul,
+                                                            double *__restrict__ part) {
+    __shared__ double sh[kBlock / 64];
+    double acc[kE_count];
+#pragma unroll
+    for (int k = 0; k < kE_count; ++k) acc[k] = 0;
+    const double *laprho = derived + 6 * N, *rots = derived + 14 * N;
+    for (long p = (long)blockIdx.x * kBlock + threadIdx.x; p < N; p += (long)gridDim.x * kBlock) {
+        auto D = [&](int it, int slot) { return dens[((long)it * 9 + slot) * N + p]; };
+        const double rn = D(0, 0), rp = D(1, 0), rt = rn + rp;
+        const double tn = D(0, 1), tp = D(1, 1);
+        const double ln = laprho[p], lp = laprho[N + p];
+        const double nn = D(0, 2), np_ = D(1, 2);
+        const double rta = pow(rt, f.byt3a);
+        acc[kE_rt2] += rt * rt;
+        acc[kE_rn2] += rn * rn;
+        acc[kE_rp2] += rp * rp;
+        acc[kE_rttt] += rt * (tn + tp);
+        acc[kE_rntn] += rn * tn;
+        acc[kE_rptp] += rp * tp;
+        acc[kE_rtlt] += rt * (ln + lp);
+        acc[kE_rnln] += rn * ln;
+        acc[kE_rplp] += rp * lp;
+        acc[kE_rta2] += pow(rt, 2 + f.byt3a);
+        acc[kE_rn2a] += rn * rn * rta;
+        acc[kE_rp2a] += rp * rp * rta;
+        acc[kE_rtnj] += rt * (nn + np_);
+        acc[kE_rnnj] += rn * nn;
+        acc[kE_rpnj] += rp * np_;
+        if (f.B7b != 0.0) acc[kE_rtb2] += pow(rt, 2 + f.byt3b);
+        if (f.B8b != 0.0) {
+            const double rtb = pow(rt, f.byt3b);
+            acc[kE_rn2b] += rn * rn * rtb;
+            acc[kE_rp2b] += rp * rp * rtb;
+        }
+        if (!trc) {
+            double jt2 = 0, jn2 = 0, jp2 = 0, jrst = 0, jrsn = 0, jrsp = 0, st2 = 0, sn2 = 0,
+                   sp2 = 0;
+            for (int k = 0; k < 3; ++k) {
+                const double jn = D(0, 3 + k), jp = D(1, 3 + k);
+                const double sn = D(0, 6 + k), sp = D(1, 6 + k);
+                const double rsn = rots[k * N + p], rsp = rots[(3 + k) * N + p];
+                jt2 += (jn + jp) * (jn + jp);
+                jn2 += jn * jn;
+                jp2 += jp * jp;
+                jrst += (jn + jp) * (rsn + rsp);
+                jrsn += jn * rsn;
+                jrsp += jp * rsp;
+                st2 += (sn + sp) * (sn + sp);
+                sn2 += sn * sn;
+                sp2 += sp * sp;
+            }
+            acc[kE_jt2] += jt2;
+            acc[kE_jn2] += jn2;
+            acc[kE_jp2] += jp2;
+            acc[kE_jrst] += jrst;
+            acc[kE_jrsn] += jrsn;
+            acc[kE_jrsp] += jrsp;
+            acc[kE_st2] += st2;
+            acc[kE_sn2] += sn2;
+            acc[kE_sp2] += sp2;
+            acc[kE_ast2] += rta * st2;
+            acc[kE_asn2] += rta * sn2;
+            acc[kE_asp2] += rta * sp2;
+            if (f.B12b != 0.0 || f.B13b != 0.0) {
+                const double rtb = pow(rt, f.byt3b);
+                acc[kE_bst2] += rtb * st2;
+                acc[kE_bsn2] += rtb * sn2;
+                acc[kE_bsp2] += rtb * sp2;
+            }
+        }
+        acc[kE_coul] += rp * vcoul[p];
+        acc[kE_cx] += rp * pow(rp, 1.0 / 3.0);
+    }
+    for (int k = 0; k < kE_count; ++k) {
+        const double s = block_sum(acc[k], sh);
+        if (threadIdx.x == 0) part[(long)blockIdx.x * kE_count + k] = s;
+    }
+}
+
+// Terms, Coulomb energies into e[0..42] = [total, kin(2), E_C, E_Cx, com(2),
+// terms 1..36]; the kinetic and c.o.m. slots are filled by k_energy_kinetic.
+__global__ void k_energy_terms(SkyrmeK f, const double *__restrict__ part, int nblk, int trc,
+                               double dv, double *__restrict__ e) {
+    __shared__ double S[kE_count];
+    const int t = threadIdx.x;
+    if (t < kE_count) {
+        double s = 0;
+        for (int b = 0; b < nblk; ++b) s += part[(long)b * kE_count + t];
+        S[t] = s;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double T[37];
+    for (int k = 0; k < 37; ++k) T[k] = 0;
+    T[1] = f.B1 * S[kE_rt2];
+    T[2] = f.B2 * ((0.0 + S[kE_rn2]) + S[kE_rp2]);
+    T[3] = f.B3 * S[kE_rttt];
+    T[4] = (0.0 + f.B4 * S[kE_rntn]) + f.B4 * S[kE_rptp];
+    T[7] = f.B5 * S[kE_rtlt];
+    T[8] = (0.0 + f.B6 * S[kE_rnln]) + f.B6 * S[kE_rplp];
+    T[9] = f.B7a * S[kE_rta2];
+    T[10] = (0.0 + f.B8a * S[kE_rn2a]) + f.B8a * S[kE_rp2a];
+    if (f.B7b != 0.0) T[33] = f.B7b * S[kE_rtb2];
+    if (f.B8b != 0.0) T[34] = (0.0 + f.B8b * S[kE_rn2b]) + f.B8b * S[kE_rp2b];
+    T[11] = f.B9 * S[kE_rtnj];
+    T[12] = (0.0 + f.B9q * S[kE_rnnj]) + f.B9q * S[kE_rpnj];
+    if (!trc) {
+        T[5] = -f.B3 * S[kE_jt2];
+        T[6] = (0.0 - f.B4 * S[kE_jn2]) - f.B4 * S[kE_jp2];
+        T[13] = f.B9 * S[kE_jrst];
+        T[14] = (0.0 + f.B9q * S[kE_jrsn]) + f.B9q * S[kE_jrsp];
+        T[15] = f.B10 * S[kE_st2];
+        T[16] = (0.0 + f.B11 * S[kE_sn2]) + f.B11 * S[kE_sp2];
+        T[17] = f.B12a * S[kE_ast2];
+        T[18] = (0.0 + f.B13a * S[kE_asn2]) + f.B13a * S[kE_asp2];
+        if (f.B12b != 0.0) T[35] = f.B12b * S[kE_bst2];
+        if (f.B13b != 0.0) T[36] = (0.0 + f.B13b * S[kE_bsn2]) + f.B13b * S[kE_bsp2];
+    }
+    for (int k = 1; k <= 36; ++k) e[6 + k] = T[k] * dv;
+    e[3] = f.coulomb ? S[kE_coul] * dv * 0.5 : 0.0;
+    e[4] = (f.coulomb && f.cexchange >= 1)
+               ? -pow(3.0 / 3.14159265358979323846, 1.0 / 3.0) * f.e2 * 0.75 * dv * S[kE_cx]
+               : 0.0;
+}
+
+// Kinetic energy (CompKinetic, Energy.f90:1426-1456) from the per-state sums
+// K_i = dv <psi_i|Lap psi_i>, the one-body c.o.m. correction
+// (CompCOMCorrection :1608-1660) and the total (CompEnergy :62-179).
+__global__ void k_energy_kinetic(SkyrmeK f, const double *__restrict__ K,
+                                 const double *__restrict__ occ, const int *__restrict__ qn,
+                                 int M, double *__restrict__ e) {
+    if (threadIdx.x != 0) return;
+    double kin[2] = {0, 0};
+    for (int l = 0; l < M; ++l) {
+        const int it = (qn[4 * l] + 3) / 2 - 1;
+        kin[it] = kin[it] + occ[l] * K[l];
+    }
+    double com[2] = {0, 0};
+    for (int it = 0; it < 2; ++it) {
+        kin[it] = -kin[it] / 2.0 * f.hbm[it];
+        if (f.com1body > 0)
+            com[it] = -kin[it] * f.mass[it] / (f.neutrons * f.mass[0] + f.protons * f.mass[1]);
+    }
+    double sky = 0;
+    for (int k = 1; k <= 36; ++k) sky = sky + e[6 + k];
+    e[1] = kin[0];
+    e[2] = kin[1];
+    e[5] = com[0];
+    e[6] = com[1];
+    e[0] = sky + (kin[0] + kin[1]) + e[3] + (com[0] + com[1]) + e[4];
+}
+
+void launch_energy_terms(const SkyrmeK &f, long N, int trc, const double *dens,
+                         const double *derived, const double *vcoul, double *part, int nblk,
+                         double dv, double *e, hipStream_t st) {
+    hipLaunchKernelGGL(k_energy_partial, dim3(nblk), dim3(kBlock), 0, st, f, N, trc, dens,
+                       derived, vcoul, part);
+    hipLaunchKernelGGL(k_energy_terms, dim3(1), dim3(64), 0, st, f, part, nblk, trc, dv, e);
+}
+
+void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ, const int *qn,
+                           int M, double *e, hipStream_t st) {
+    hipLaunchKernelGGL(k_energy_kinetic, dim3(1), dim3(64), 0, st, f, K, occ, qn, M, e);
+}
+
+}  // namespace mocca

@@ -1,0 +1,544 @@
+// C ABI of the self-consistent part: functional set-up, HF filling,
+// density mixing, moments, Coulomb, potentials, energy and the fused
+// iteration loop.  Entry points and their reference counterparts are listed
+// in include/mocca_amd.h.
+#include "mocca_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace mocca;
+using namespace mocca::detail;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Associated Legendre functions P_n^m(x), n = m..nmax, with the Condon-Shortley
+// phase, by the recurrence legendre_pnm uses (Mesh.f90:92-268):
+// P_m^m = (-1)^m (2m-1)!! (1-x^2)^(m/2), P_{m+1}^m = x (2m+1) P_m^m,
+// P_n^m = ((2n-1) x P_{n-1}^m - (n+m-1) P_{n-2}^m) / (n-m).
+void legendre(int nmax, int m, double x, double *p) {
+    for (int i = 0; i < m; ++i) p[i] = 0.0;
+    p[m] = 1.0;
+    const double s = std::sqrt(1.0 - x * x);
+    double odd = 1.0;
+    for (int i = 1; i <= m; ++i) {
+        p[m] = -p[m] * odd * s;
+        odd += 2.0;
+    }
+    if (m == nmax) return;
+    p[m + 1] = x * double(2 * m + 1) * p[m];
+    for (int n = m + 2; n <= nmax; ++n)
+        p[n] = (double(2 * n - 1) * x * p[n - 1] - double(n + m - 1) * p[n - 2]) / double(n - m);
+}
+
+// r^l Re Y_lm(X, Y, Z) with z the quantisation axis and phi = atan2(Y, X)
+// (SphericalHarmonics, Moments.f90:584-691).
+double rlylm(int l, int m, double X, double Y, double Z) {
+    double p[32];
+    const double r = std::sqrt(X * X + Y * Y + Z * Z);
+    legendre(l, m, Z / r, p);
+    double q = 1.0;
+    for (int k = 1; k <= 2 * m; ++k) q = q / double(l + m - k + 1);
+    double fac = ((m & 1) ? -1.0 : 1.0) * std::sqrt(q) * std::pow(r, l);
+    fac = fac * std::sqrt(double(2 * l + 1) / (4 * kPi));
+    return fac * p[l] * std::cos(m * std::atan2(Y, X));
+}
+
+void free_ptr(void *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int check_scf(mocca_solve *h) {
+    if (int rc = check_ready(h)) return rc;
+    if (!h->scf.set) return fail(h, MOCCA_ERR_STATE, "mocca_scf_setup not called");
+    return 0;
+}
+
+// DivWPot and DivAPot terms (MeanFields.f90:463-468, 580-592): for each
+// output, the field derivative and the (P, S, TS, component) it is taken with.
+void build_potential_jobs(mocca_solve *h, std::vector<FieldJob> &jobs) {
+    const long N = h->N;
+    const int lag = h->mesh.lagrange, trc = h->mesh.time_reversal;
+    double *pot = h->scf.pot;
+    auto term = [&](FieldJob &j, int t, const double *src, int axis, int P, int S, int TS,
+                    int comp) {
+        j.src[t] = src;
+        j.axis[t] = axis;
+        j.sign[t][0] = j.sign[t][1] = j.sign[t][2] = sign_first_h(axis, P, S, TS, comp - 1, lag);
+        j.coef[t] = 1.0;
+        j.nterms = t + 1;
+    };
+    for (int it = 0; it < 2; ++it)
+        for (int a = 0; a < 3; ++a) {
+            FieldJob j{};
+            term(j, 0, pot + (kOffB + it) * N, a, 1, 1, 1, 1);
+            j.dst = pot + (kOffNB + it * 3 + a) * N;
+            jobs.push_back(j);
+        }
+    // DivW_nu = sum_mu d_mu W_{mu nu}; (P, S, TS, comp) per (nu, mu)
+    static const int q[3][3][4] = {{{-1, 1, 1, 2}, {-1, 1, 1, 1}, {-1, -1, 1, 2}},
+                                   {{-1, 1, 1, 1}, {-1, 1, 1, 2}, {-1, -1, 1, 1}},
+                                   {{-1, -1, 1, 2}, {-1, -1, 1, 1}, {-1, 1, 1, 2}}};
+    for (int it = 0; it < 2; ++it)
+        for (int nu = 0; nu < 3; ++nu) {
+            FieldJob j{};
+            for (int mu = 0; mu < 3; ++mu)
+                term(j, mu, pot + (kOffW + (it * 3 + nu) * 3 + mu) * N, mu, q[nu][mu][0],
+                     q[nu][mu][1], q[nu][mu][2], q[nu][mu][3]);
+            j.dst = pot + (kOffDW + it * 3 + nu) * N;
+            jobs.push_back(j);
+        }
+    if (trc) return;
+    for (int it = 0; it < 2; ++it) {
+        FieldJob j{};
+        term(j, 0, pot + (kOffA + it * 3 + 0) * N, 0, -1, -1, 1, 2);
+        term(j, 1, pot + (kOffA + it * 3 + 1) * N, 1, -1, -1, 1, 1);
+        term(j, 2, pot + (kOffA + it * 3 + 2) * N, 2, -1, 1, 1, 2);
+        j.dst = pot + (kOffDA + it) * N;
+        jobs.push_back(j);
+    }
+}
+
+// CalculateAllMoments + ConstructPotentials.
+int construct_impl(mocca_solve *h) {
+    Scf &s = h->scf;
+    const long N = h->N;
+    double *dens = h->b.dens;
+    int rc = run(h, "moments", [&] {
+        launch_moments(dens, dens + 9 * N, s.Y, s.nmom, N, h->mesh.dv, s.mpart, s.mom_nblk, s.Q,
+                       h->st);
+    });
+    if (rc) return rc;
+    if (s.sk.coulomb) {
+        rc = run(h, "coulomb_bc", [&] {
+            launch_coulomb_bc(s.Q, s.nmom, s.lfac, s.bidx, s.bsh, s.brp, s.nb, s.sk.e2, s.cx,
+                              h->st);
+        });
+        if (rc) return rc;
+        int herr = 0;
+        rc = run(h, "coulomb_cg", [&] {
+            herr = launch_coulomb_cg(s.ck, s.cx, dens + 9 * N, s.cr, s.cpa, s.cpb, s.cap, s.cpart,
+                                     s.cg_nwg, s.vcoul, s.vcoulx, s.cg_iters, h->st);
+        });
+        if (rc) return rc;
+        if (herr != (int)hipSuccess)
+            return fail(h, MOCCA_ERR_HIP, "cooperative launch of the Coulomb solver failed: %s",
+                        hipGetErrorString((hipError_t)herr));
+    }
+    rc = run(h, "potentials", [&] {
+        launch_potentials(s.sk, N, h->mesh.time_reversal, dens, h->b.derived, s.vcoul, s.vcoulx,
+                          s.pot, h->st);
+    });
+    if (rc) return rc;
+    rc = run(h, "pot_jobs", [&] { launch_field_jobs(h->md, s.pjobs, s.npjobs, h->st); });
+    if (rc) return rc;
+    if (h->prec != MOCCA_FP64) {
+        rc = run(h, "pot_convert", [&] {
+            launch_convert_f32(s.pot, (float *)h->b.fields, (long)kNumFieldSlots * N, h->st);
+        });
+        if (rc) return rc;
+    }
+    h->fields_set = true;
+    return 0;
+}
+
+int energy_terms_impl(mocca_solve *h, double *row) {
+    Scf &s = h->scf;
+    return run(h, "energy", [&] {
+        launch_energy_terms(s.sk, h->N, h->mesh.time_reversal, h->b.dens, h->b.derived, s.vcoul,
+                            s.epart, s.e_nblk, h->mesh.dv, row, h->st);
+    });
+}
+
+int energy_kinetic_impl(mocca_solve *h, double *row) {
+    return run(h, "energy_kinetic", [&] {
+        launch_energy_kinetic(h->scf.sk, h->b.K, h->b.occ, h->b.qn, h->M, row, h->st);
+    });
+}
+
+int fill_impl(mocca_solve *h) {
+    Scf &s = h->scf;
+    return run(h, "hf_fill", [&] {
+        launch_hf_fill(h->b.E, h->b.qn, h->M, s.ub_n, s.ub_p, s.fill_fac, h->b.occ, h->st);
+    });
+}
+
+int save_history(mocca_solve *h) {
+    const size_t N = h->N;
+    HIPCHK(h, hipMemcpyAsync(h->scf.hist, h->b.dens, 18 * N * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->scf.hist + 18 * N, h->b.derived, 20 * N * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->st));
+    return 0;
+}
+
+int mix_impl(mocca_solve *h, int keep) {
+    const long N = h->N;
+    Scf &s = h->scf;
+    return run(h, "mix", [&] {
+        launch_mix(h->b.dens, s.hist, 18 * N, s.damping, keep, h->st);
+        launch_mix(h->b.derived, s.hist + 18 * N, 20 * N, s.damping, keep, h->st);
+    });
+}
+
+int check_gs_status(mocca_solve *h) {
+    int st = 0;
+    HIPCHK(h, hipMemcpyAsync(&st, h->b.status, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (st) {
+        HIPCHK(h, hipMemsetAsync(h->b.status, 0, sizeof(int), h->st));
+        return fail(h, MOCCA_ERR_NUMERIC,
+                    "Gram-Schmidt met a non-positive norm (linearly dependent or NaN states)");
+    }
+    return 0;
+}
+
+}  // namespace
+
+namespace mocca {
+namespace detail {
+
+void scf_free(mocca_solve *h) {
+    Scf &s = h->scf;
+    void *ptrs[] = {s.hist, s.Y, s.mpart, s.Q, s.lfac, s.bidx, s.bsh, s.brp, s.cx, s.cr, s.cpa,
+                    s.cpb, s.cap, s.cpart, s.vcoul, s.vcoulx, s.cg_iters, s.pjobs, s.epart,
+                    s.ecur, s.erows};
+    for (void *p : ptrs) free_ptr(p);
+    if (s.pot && s.pot != (double *)h->b.fields) (void)hipFree(s.pot);
+    s = Scf{};
+}
+
+}  // namespace detail
+}  // namespace mocca
+
+extern "C" {
+
+int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
+    if (!h || !f) return fail(h, MOCCA_ERR_ARG, "null argument");
+    for (int k = 20; k < 28; ++k)
+        if (f->B[k] != 0.0)
+            return fail(h, MOCCA_ERR_UNSUPPORTED,
+                        "tensor terms (B14..B21) are not supported (B[%d] = %g)", k, f->B[k]);
+    if (f->max_moment < 0 || f->max_moment > 12 || (f->max_moment & 1))
+        return fail(h, MOCCA_ERR_ARG, "max_moment must be even and <= 12 (got %d)",
+                    f->max_moment);
+    if (f->coulomb != 0 && f->coulomb != 1)
+        return fail(h, MOCCA_ERR_UNSUPPORTED, "only the conjugate-gradient Coulomb solver (1)");
+    if (f->coulomb && f->coulomb_maxit < 1)
+        return fail(h, MOCCA_ERR_ARG, "coulomb_maxit must be positive");
+    if (f->cexchange < 0 || f->cexchange > 2) return fail(h, MOCCA_ERR_ARG, "cexchange 0..2");
+    if (!(f->neutrons >= 0) || !(f->protons >= 0))
+        return fail(h, MOCCA_ERR_ARG, "particle numbers must be non-negative");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    scf_free(h);
+    Scf &s = h->scf;
+    const mocca_mesh &g = h->mesh;
+    const long N = h->N;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    // functional
+    SkyrmeK &k = s.sk;
+    const double *B = f->B;
+    k.B1 = B[0]; k.B2 = B[1]; k.B3 = B[2]; k.B4 = B[3]; k.B5 = B[4]; k.B6 = B[5];
+    k.B7a = B[6]; k.B7b = B[7]; k.B8a = B[8]; k.B8b = B[9]; k.byt3a = B[10]; k.byt3b = B[11];
+    k.B9 = B[12]; k.B9q = B[13]; k.B10 = B[14]; k.B11 = B[15]; k.B12a = B[16];
+    k.B12b = B[17]; k.B13a = B[18]; k.B13b = B[19];
+    for (int it = 0; it < 2; ++it) { k.hbm[it] = f->hbm[it]; k.mass[it] = f->mass[it]; }
+    k.e2 = f->e2; k.neutrons = f->neutrons; k.protons = f->protons;
+    k.com1body = f->com1body; k.cexchange = f->cexchange; k.coulomb = f->coulomb;
+    s.damping = f->damping;
+    // kinetic coefficients hbm/2 (1 - m_q / (N m_n + Z m_p)) with COM1Body = 2
+    for (int it = 0; it < 2; ++it) {
+        const double red = f->com1body == 2
+                               ? 1.0 - f->mass[it] / (f->neutrons * f->mass[0] +
+                                                      f->protons * f->mass[1])
+                               : 1.0;
+        h->kin[it] = f->hbm[it] / 2.0 * red;
+    }
+    // filling bounds (NaiveFill): pairs under time reversal
+    const int trc = g.time_reversal;
+    s.ub_p = trc ? (int)std::floor(f->protons / 2) : (int)std::floor(f->protons);
+    s.ub_n = trc ? (int)std::floor(f->neutrons / 2) : (int)std::floor(f->neutrons);
+    s.fill_fac = trc ? 2.0 : 1.0;
+    int rc;
+#define SCF_ALLOC(p, bytes)                                   \
+    do {                                                      \
+        if ((rc = dalloc(h, &(p), (bytes))) != 0) {           \
+            scf_free(h);                                      \
+            return rc;                                        \
+        }                                                     \
+    } while (0)
+    // moments: even l <= max_moment, even m <= l, in IniMoments order
+    s.nmom = 0;
+    for (int l = 0; l <= f->max_moment; l += 2)
+        for (int m = 0; m <= l; m += 2) {
+            s.lm[2 * s.nmom] = l;
+            s.lm[2 * s.nmom + 1] = m;
+            ++s.nmom;
+        }
+    {
+        std::vector<double> Y((size_t)s.nmom * N);
+        for (int q = 0; q < s.nmom; ++q) {
+            const int l = s.lm[2 * q], m = s.lm[2 * q + 1];
+            for (int z = 0; z < nz; ++z)
+                for (int y = 0; y < ny; ++y)
+                    for (int x = 0; x < nx; ++x) {
+                        const long p = ((long)z * ny + y) * nx + x;
+                        // the monopole form factor is 1/sqrt(4 pi) (IniMoments)
+                        Y[q * N + p] = l == 0 ? 1.0 / std::sqrt(4.0 * kPi)
+                                              : rlylm(l, m, (x + 0.5) * g.dx, (y + 0.5) * g.dx,
+                                                      (z + 0.5) * g.dx);
+                    }
+        }
+        SCF_ALLOC(s.Y, Y.size() * sizeof(double));
+        HIPCHK(h, hipMemcpy(s.Y, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    s.mom_nblk = (int)std::min<long>(256, (N + kBlock - 1) / kBlock);
+    SCF_ALLOC(s.mpart, (size_t)s.mom_nblk * kMaxMom * 2 * sizeof(double));
+    SCF_ALLOC(s.Q, kMaxMom * 2 * sizeof(double));
+    HIPCHK(h, hipMemset(s.Q, 0, kMaxMom * 2 * sizeof(double)));
+    // Coulomb box: (nx+2)(ny+2)(nz+2), boundary planes on the far faces
+    const int mx = nx + 2, my = ny + 2, mz = nz + 2;
+    const size_t M3 = (size_t)mx * my * mz;
+    s.ck.nx = nx; s.ck.ny = ny; s.ck.nz = nz; s.ck.mx = mx; s.ck.my = my; s.ck.mz = mz;
+    s.ck.f = 1.0 / (g.dx * g.dx);
+    s.ck.prec = 1.0e-9 / (g.dx * g.dx * g.dx * nx * ny * nz);
+    s.ck.e2 = f->e2;
+    s.ck.maxit = f->coulomb_maxit;
+    {
+        std::vector<int> bidx;
+        std::vector<double> bx;
+        for (int z = 0; z < mz; ++z)
+            for (int y = 0; y < my; ++y)
+                for (int x = 0; x < mx; ++x)
+                    if (x >= nx || y >= ny || z >= nz) bidx.push_back((z * my + y) * mx + x);
+        s.nb = (int)bidx.size();
+        std::vector<double> bsh((size_t)s.nmom * s.nb), brp((size_t)s.nmom * s.nb), lf(s.nmom);
+        for (int q = 0; q < s.nmom; ++q) {
+            const int l = s.lm[2 * q], m = s.lm[2 * q + 1];
+            lf[q] = 4 * kPi / (2 * l + 1);
+            for (int b = 0; b < s.nb; ++b) {
+                const int i = bidx[b] % mx, j = (bidx[b] / mx) % my, kk = bidx[b] / (mx * my);
+                const double X = (i + 0.5) * g.dx, Yc = (j + 0.5) * g.dx, Z = (kk + 0.5) * g.dx;
+                const double r = std::sqrt(X * X + Yc * Yc + Z * Z);
+                bsh[(size_t)q * s.nb + b] = rlylm(l, m, X, Yc, Z);
+                brp[(size_t)q * s.nb + b] = std::pow(r, 2 * l + 1);
+            }
+        }
+        SCF_ALLOC(s.bidx, bidx.size() * sizeof(int));
+        SCF_ALLOC(s.bsh, bsh.size() * sizeof(double));
+        SCF_ALLOC(s.brp, brp.size() * sizeof(double));
+        SCF_ALLOC(s.lfac, lf.size() * sizeof(double));
+        HIPCHK(h, hipMemcpy(s.bidx, bidx.data(), bidx.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(s.bsh, bsh.data(), bsh.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(s.brp, brp.data(), brp.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(s.lfac, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
+        // InitialGuess: potential of a uniformly charged sphere of radius
+        // 1.25 A^(1/3) holding the protons, on the interior of the box.
+        std::vector<double> x0(M3, 0.0);
+        const double R = 1.25 * std::pow(f->neutrons + f->protons, 1.0 / 3.0);
+        const double Qc = f->protons * f->e2, F = Qc / (2 * R);
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const double X = (x + 0.5) * g.dx, Yc = (y + 0.5) * g.dx, Z = (z + 0.5) * g.dx;
+                    const double r = std::sqrt(X * X + Yc * Yc + Z * Z);
+                    x0[((size_t)z * my + y) * mx + x] =
+                        r <= R ? F * (3 - r * r / (R * R)) : Qc / r;
+                }
+        SCF_ALLOC(s.cx, M3 * sizeof(double));
+        HIPCHK(h, hipMemcpy(s.cx, x0.data(), M3 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    SCF_ALLOC(s.cr, M3 * sizeof(double));
+    SCF_ALLOC(s.cpa, M3 * sizeof(double));
+    SCF_ALLOC(s.cpb, M3 * sizeof(double));
+    SCF_ALLOC(s.cap, M3 * sizeof(double));
+    HIPCHK(h, hipMemset(s.cr, 0, M3 * sizeof(double)));
+    HIPCHK(h, hipMemset(s.cpa, 0, M3 * sizeof(double)));
+    HIPCHK(h, hipMemset(s.cpb, 0, M3 * sizeof(double)));
+    HIPCHK(h, hipMemset(s.cap, 0, M3 * sizeof(double)));
+    {
+        const long nint = (long)nx * ny * nz;
+        const int cap = coulomb_cg_max_groups();
+        if (cap < 1) {
+            scf_free(h);
+            return fail(h, MOCCA_ERR_HIP, "cooperative launch not available for the Coulomb solver");
+        }
+        s.cg_nwg = (int)std::min<long>(std::min(cap, 256), (nint + kBlock - 1) / kBlock);
+    }
+    SCF_ALLOC(s.cpart, 2 * (size_t)s.cg_nwg * sizeof(double));
+    SCF_ALLOC(s.vcoul, N * sizeof(double));
+    SCF_ALLOC(s.vcoulx, N * sizeof(double));
+    SCF_ALLOC(s.cg_iters, sizeof(int));
+    HIPCHK(h, hipMemset(s.vcoul, 0, N * sizeof(double)));
+    HIPCHK(h, hipMemset(s.vcoulx, 0, N * sizeof(double)));
+    HIPCHK(h, hipMemset(s.cg_iters, 0, sizeof(int)));
+    // potentials (FP64 master copy; the FP64 path uses it directly)
+    if (h->prec == MOCCA_FP64) {
+        s.pot = (double *)h->b.fields;
+    } else {
+        SCF_ALLOC(s.pot, (size_t)kNumFieldSlots * N * sizeof(double));
+        HIPCHK(h, hipMemset(s.pot, 0, (size_t)kNumFieldSlots * N * sizeof(double)));
+    }
+    {
+        std::vector<FieldJob> jobs;
+        build_potential_jobs(h, jobs);
+        s.npjobs = (int)jobs.size();
+        SCF_ALLOC(s.pjobs, jobs.size() * sizeof(FieldJob));
+        HIPCHK(h, hipMemcpy(s.pjobs, jobs.data(), jobs.size() * sizeof(FieldJob),
+                            hipMemcpyHostToDevice));
+    }
+    s.e_nblk = (int)std::min<long>(512, (N + kBlock - 1) / kBlock);
+    SCF_ALLOC(s.epart, (size_t)s.e_nblk * kEnergySlots * sizeof(double));
+    SCF_ALLOC(s.ecur, 43 * sizeof(double));
+    HIPCHK(h, hipMemset(s.ecur, 0, 43 * sizeof(double)));
+    SCF_ALLOC(s.hist, 38 * (size_t)N * sizeof(double));
+    HIPCHK(h, hipMemset(s.hist, 0, 38 * (size_t)N * sizeof(double)));
+#undef SCF_ALLOC
+    s.set = true;
+    return 0;
+}
+
+int mocca_hf_fill(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    return fill_impl(h);
+}
+
+int mocca_get_occupations(mocca_solve *h, double *occ) {
+    if (int rc = check_ready(h)) return rc;
+    if (!occ) return fail(h, MOCCA_ERR_ARG, "null argument");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(occ, h->b.occ, h->M * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mocca_update_densities(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    if (int rc = save_history(h)) return rc;
+    return density_any(h);
+}
+
+int mocca_mix_densities(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    return mix_impl(h, 0);
+}
+
+int mocca_construct_potentials(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    return construct_impl(h);
+}
+
+int mocca_comp_energy(mocca_solve *h, double *e) {
+    if (int rc = check_scf(h)) return rc;
+    if (!h->fields_set) return fail(h, MOCCA_ERR_STATE, "no potentials yet");
+    int rc;
+    if ((rc = kinetic_any(h))) return rc;
+    if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
+    if ((rc = energy_kinetic_impl(h, h->scf.ecur))) return rc;
+    if (e) {
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(e, h->scf.ecur, 43 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int mocca_scf_start(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    int rc;
+    if ((rc = fill_impl(h))) return rc;
+    if ((rc = density_any(h))) return rc;
+    if ((rc = save_history(h))) return rc;
+    if ((rc = construct_impl(h))) return rc;
+    if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
+    h->scf.started = true;
+    return 0;
+}
+
+int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
+                  double *energies) {
+    if (int rc = check_scf(h)) return rc;
+    if (!h->scf.started) return fail(h, MOCCA_ERR_STATE, "mocca_scf_start not called");
+    if (niter < 0) return fail(h, MOCCA_ERR_ARG, "niter must be >= 0");
+    Scf &s = h->scf;
+    if (s.erow_cap < niter + 1) {
+        if (s.erows) {
+            HIPCHK(h, hipStreamSynchronize(h->st));
+            (void)hipFree(s.erows);
+            s.erows = nullptr;
+        }
+        if (int rc = dalloc(h, &s.erows, (size_t)(niter + 1) * 43 * sizeof(double))) return rc;
+        s.erow_cap = niter + 1;
+    }
+    HIPCHK(h, hipMemcpyAsync(s.erows, s.ecur, 43 * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->st));
+    int rc;
+    for (int k = 1; k <= niter; ++k) {
+        double *prev = s.erows + (size_t)(k - 1) * 43, *row = s.erows + (size_t)k * 43;
+        if ((rc = grad_desc_any(h, dt_over_hbar, momentum))) return rc;
+        // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
+        // previous row's densities came from: that row's kinetic energy
+        if ((rc = energy_kinetic_impl(h, prev))) return rc;
+        if ((rc = fill_impl(h))) return rc;
+        if ((rc = density_any(h))) return rc;
+        if ((rc = mix_impl(h, 1))) return rc;
+        if ((rc = construct_impl(h))) return rc;
+        if ((rc = energy_terms_impl(h, row))) return rc;
+    }
+    // kinetic energy of the final states: one hPsi-type pass with no outputs
+    double *last = s.erows + (size_t)niter * 43;
+    if ((rc = kinetic_any(h))) return rc;
+    if ((rc = energy_kinetic_impl(h, last))) return rc;
+    HIPCHK(h, hipMemcpyAsync(s.ecur, last, 43 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    if (energies) {
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(energies, s.erows, (size_t)(niter + 1) * 43 * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    return check_gs_status(h);
+}
+
+int mocca_get_moments(mocca_solve *h, int *lm, double *q, int *count) {
+    if (int rc = check_scf(h)) return rc;
+    const Scf &s = h->scf;
+    if (count) *count = s.nmom;
+    if (lm) std::memcpy(lm, s.lm, 2 * s.nmom * sizeof(int));
+    if (q) {
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(q, s.Q, 2 * s.nmom * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int mocca_get_coulomb(mocca_solve *h, double *vcoul, double *vcoulx, double *cpot,
+                      int *cg_iterations) {
+    if (int rc = check_scf(h)) return rc;
+    const Scf &s = h->scf;
+    const size_t N = h->N, M3 = (size_t)s.ck.mx * s.ck.my * s.ck.mz;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (vcoul) HIPCHK(h, hipMemcpy(vcoul, s.vcoul, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (vcoulx) HIPCHK(h, hipMemcpy(vcoulx, s.vcoulx, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (cpot) HIPCHK(h, hipMemcpy(cpot, s.cx, M3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (cg_iterations)
+        HIPCHK(h, hipMemcpy(cg_iterations, s.cg_iters, sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mocca_get_potentials(mocca_solve *h, double *U, double *B, double *NablaB, double *W,
+                         double *DivW, double *S, double *A, double *DivA) {
+    if (int rc = check_scf(h)) return rc;
+    const size_t N = h->N;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    struct { double *dst; int off, n; } parts[] = {
+        {U, kOffU, 2}, {B, kOffB, 2}, {NablaB, kOffNB, 6}, {W, kOffW, 18}, {DivW, kOffDW, 6},
+        {S, kOffS, 6}, {A, kOffA, 6}, {DivA, kOffDA, 2}};
+    for (auto &q : parts)
+        if (q.dst)
+            HIPCHK(h, hipMemcpy(q.dst, h->scf.pot + q.off * N, q.n * N * sizeof(double),
+                                hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

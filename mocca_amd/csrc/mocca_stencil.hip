@@ -26,19 +26,20 @@ __device__ __forceinline__ void fd_pair(const MeshK<T> &m, const T v[9], T &d1, 
                       m.l2c3 * (v[7] + v[1]) + m.l2c4 * (v[8] + v[0]));
 }
 
-__device__ __forceinline__ void block_sum2_fd(double &a, double &b, double *sh) {
+__device__ __forceinline__ void block_sum3_fd(double &a, double &b, double &c, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o, 64);
         b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
     }
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { sh[2 * w] = a; sh[2 * w + 1] = b; }
+    if (l == 0) { sh[3 * w] = a; sh[3 * w + 1] = b; sh[3 * w + 2] = c; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double sa = 0, sb = 0;
-        for (int k = 0; k < kBlock / 64; ++k) { sa += sh[2 * k]; sb += sh[2 * k + 1]; }
-        a = sa; b = sb;
+        double sa = 0, sb = 0, sc = 0;
+        for (int k = 0; k < kBlock / 64; ++k) { sa += sh[3 * k]; sb += sh[3 * k + 1]; sc += sh[3 * k + 2]; }
+        a = sa; b = sb; c = sc;
     }
     __syncthreads();
 }
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
     constexpr bool trc = TRC;
     extern __shared__ double smem_d[];
     T *sp = reinterpret_cast<T *>(smem_d);  // [4][TY+8][nx+8]
-    __shared__ double red[2 * kBlock / 64];
+    __shared__ double red[3 * kBlock / 64];
     const int nx = m.nx, ny = m.ny, nz = m.nz;
     const long N = m.N, plane = (long)nx * ny;
     const int W = nx + 2 * G;
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
     const T *Fa = F + (long)(kOffA + 3 * it) * N, *Fda = F + (long)(kOffDA + it) * N;
     T *po = psi_out ? psi_out + (long)s * 4 * N : nullptr;
     T *us = upd ? upd + (long)s * 4 * N : nullptr;
-    double eacc = 0, dacc = 0;
+    double eacc = 0, dacc = 0, kacc = 0;
     const int nhalo = 4 * 8 * nx;
     for (int z = 0; z < nz; ++z) {
         // ---- all global loads of this plane first ----
@@ -181,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
                 fd_pair(m, w, dyc, ly);
                 fd_pair(m, q[c], dzc, lz);
                 const T lc = lx + ly + lz, vc = q[c][4];
+                kacc += (double)vc * (double)lc;
                 h[c] += -fb * lc - (nb0 * dxc + nb1 * dyc + nb2 * dzc) + fu * vc;
                 const T ex = fw[0][0] * dxc + fw[0][1] * dyc + fw[0][2] * dzc + hx * vc;
                 const T ey = fw[1][0] * dxc + fw[1][1] * dyc + fw[1][2] * dzc + hy * vc;
@@ -228,10 +230,12 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
         }
     }
     if (eparts) {
-        block_sum2_fd(eacc, dacc, red);
+        block_sum3_fd(eacc, dacc, kacc, red);
         if (tid == 0) {
-            eparts[((long)s * ntile_y + tile) * 2] = eacc;
-            eparts[((long)s * ntile_y + tile) * 2 + 1] = dacc;
+            double *ep = eparts + ((long)s * ntile_y + tile) * 3;
+            ep[0] = eacc;
+            ep[1] = dacc;
+            ep[2] = kacc;
         }
     }
 }

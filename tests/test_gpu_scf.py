@@ -1,0 +1,171 @@
+"""GPU parity of the self-consistent part (HF filling, densities, mixing,
+moments, Coulomb, potentials, energy, the fused iteration loop) through the
+C ABI, against the reference's own golden vectors and the oracle.
+
+Tolerances (FP64 path, max-norm relative to the largest reference value):
+  moments, potentials without Coulomb, energy terms     1e-11
+  Coulomb potential (CG iterate, stopped at the same residual)  1e-10
+  energies along an iteration trajectory                 1e-9 MeV absolute
+Mixed path (FP32 spinors): total energy within 1 keV of the reference.
+"""
+import numpy as np
+import pytest
+
+import mocca_amd
+import oracle_lib as ol
+from oracle_lib import GOLDEN_CASES, golden, kin_coefs, relerr
+
+pytestmark = pytest.mark.gpu
+SCF_CASES = ["scf_trc_fd", "scf_trc_lag", "scf_todd_fd"]
+
+
+def _functional(s, g):
+    c, m = g["consts"], g["meta"]
+    s.scf_setup(g["coef"], c[0:2], c[2:4], c[4], c[7], c[8], com1body=m["com1body"],
+                cexchange=2, coulomb=1, coulomb_maxit=500, max_moment=10, damping=0.75)
+
+
+def _solve(g, occ, precision=mocca_amd.FP64):
+    m = g["meta"]
+    s = mocca_amd.Solve(m["nx"], m["ny"], m["nz"], m["dx"], g["consts"][6], m["nwt"],
+                        time_reversal=m["trc"], lagrange=m["lagrange"], precision=precision)
+    s.set_states_meta(g["qn"], occ)
+    s.upload_states(g["psi"])
+    _functional(s, g)
+    return s
+
+
+def _ref_row(ge):
+    """Reference energy dump -> the library's row layout."""
+    return np.concatenate([ge[0:6], [ge[7]], ge[9:]])
+
+
+def _traj_cols(rows):
+    """Library rows -> the reference trajectory columns
+    [E, sum kin, E_C, E_Cx, sum com, terms 1..36]."""
+    return np.concatenate([rows[:, 0:1], rows[:, 1:2] + rows[:, 2:3], rows[:, 3:5],
+                           rows[:, 5:6] + rows[:, 6:7], rows[:, 7:]], axis=1)
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_potentials_energy_vs_reference(case):
+    g = golden(case)
+    m = g["meta"]
+    N = m["nx"] * m["ny"] * m["nz"]
+    print(f"scf pieces: case {case}", flush=True)
+    s = _solve(g, g["occ"])
+    s.update_densities()
+    s.construct_potentials()
+    lm, q = s.moments()
+    assert np.array_equal(lm, g["pot_moment_lm"][:, :2])
+    assert relerr(q, g["pot_moment_val"]) < 1e-11
+    vc, vx, cp, iters = s.coulomb()
+    assert 0 < iters < 500
+    assert relerr(vc, g["pot_vcoul"].reshape(N)) < 1e-10
+    assert relerr(cp, g["pot_cpot"].reshape(-1)) < 1e-10
+    assert relerr(vx, g["pot_vcoulx"].reshape(N)) < 1e-11
+    P = s.potentials()
+    pairs = [("U", "Upot", 1e-10), ("B", "Bpot", 1e-11), ("NablaB", "NBpot", 1e-11),
+             ("W", "Wpot", 1e-11)]
+    if not m["trc"]:
+        pairs += [("S", "Spot", 1e-11), ("A", "Apot", 1e-11), ("DivA", "DApot", 1e-11)]
+    for k, gk, tol in pairs:
+        assert relerr(P[k].reshape(-1), g["pot_" + gk].reshape(-1)) < tol, k
+    # div W of the curl-type spin-orbit W vanishes analytically: both sides
+    # hold rounding noise, compared on the scale of the terms, max|W| / dx
+    scale = np.max(np.abs(g["pot_Wpot"])) / m["dx"]
+    assert np.max(np.abs(P["DivW"].reshape(-1) - g["pot_DWpot"].reshape(-1))) < 1e-13 * scale
+    e = s.comp_energy()
+    ref = _ref_row(g["pot_energy"])
+    assert np.max(np.abs(e - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-10
+    assert abs(e[0] - ref[0]) < 1e-10 * abs(ref[0])
+    s.close()
+
+
+@pytest.mark.parametrize("case", SCF_CASES)
+def test_iterate_trajectory_vs_reference(case):
+    g = golden(case)
+    m = g["meta"]
+    n = m["niter"]
+    print(f"scf loop: case {case} niter {n}", flush=True)
+    s = _solve(g, np.zeros(m["nwt"]))
+    s.scf_start()
+    rows = s.iterate(n, m["dt"] / g["consts"][5], m["momentum"])
+    traj = _traj_cols(rows)
+    assert np.max(np.abs(traj[:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(traj, g["traj"]) < 1e-11
+    psi = s.download_states()
+    assert relerr(psi.reshape(-1), g["final_psi"].reshape(-1)) < 1e-10
+    e, _ = s.energies()
+    assert relerr(e, g["final_eo"][:, 0]) < 1e-10
+    assert np.array_equal(s.occupations(), g["final_eo"][:, 1])
+    s.close()
+
+
+@pytest.mark.parametrize("case", ["scf_trc_fd", "scf_todd_fd"])
+def test_entry_points_match_fused_loop(case):
+    """The reference's call sequence through the separate entry points gives
+    the same trajectory as mocca_iterate."""
+    g = golden(case)
+    m = g["meta"]
+    n = 4
+    dth = m["dt"] / g["consts"][5]
+    a = _solve(g, np.zeros(m["nwt"]))
+    a.scf_start()
+    fused = a.iterate(n, dth, m["momentum"])
+    b = _solve(g, np.zeros(m["nwt"]))
+    b.hf_fill()
+    b.update_densities()
+    b.construct_potentials()
+    rows = [b.comp_energy()]
+    for _ in range(n):
+        b.grad_desc(dth, m["momentum"])
+        b.hf_fill()
+        b.update_densities()
+        b.mix_densities()
+        b.construct_potentials()
+        rows.append(b.comp_energy())
+    assert relerr(np.array(rows), fused) < 1e-13
+    assert relerr(b.download_states(), a.download_states()) < 1e-13
+    a.close()
+    b.close()
+
+
+def test_iterate_in_pieces_continues_the_loop():
+    g = golden("scf_trc_fd")
+    m = g["meta"]
+    dth = m["dt"] / g["consts"][5]
+    s = _solve(g, np.zeros(m["nwt"]))
+    s.scf_start()
+    r1 = s.iterate(5, dth, m["momentum"])
+    r2 = s.iterate(7, dth, m["momentum"])
+    assert relerr(r2[0], r1[-1]) < 1e-14
+    traj = _traj_cols(np.concatenate([r1, r2[1:]]))
+    assert np.max(np.abs(traj[:, 0] - g["traj"][:, 0])) < 1e-9
+    s.close()
+
+
+@pytest.mark.parametrize("case", ["scf_trc_fd", "scf_todd_fd"])
+def test_mixed_precision_energy_within_1keV(case):
+    g = golden(case)
+    m = g["meta"]
+    s = _solve(g, np.zeros(m["nwt"]), mocca_amd.MIXED)
+    s.scf_start()
+    rows = s.iterate(m["niter"], m["dt"] / g["consts"][5], m["momentum"])
+    err = np.max(np.abs(rows[:, 0] - g["traj"][:, 0]))
+    print(f"mixed precision: max |E - E_ref| = {err * 1e3:.4f} keV", flush=True)
+    assert err < 1e-3
+    s.close()
+
+
+def test_hf_fill_vs_oracle():
+    g = golden("scf_todd_fd")
+    m = g["meta"]
+    rng = np.random.default_rng(3)
+    e = np.round(rng.normal(size=m["nwt"]), 1)      # ties on purpose
+    s = _solve(g, np.zeros(m["nwt"]))
+    s.set_energies(e)
+    s.hf_fill()
+    want = ol.hf_fill(e, g["qn"], m["neutrons"], m["protons"], m["trc"])
+    assert np.array_equal(s.occupations(), want)
+    s.close()
