@@ -10,7 +10,14 @@ with the states resident in HBM.  The FP64 rate is `value`; the mixed
 (FP32 stencils/hPsi/densities, FP64 Gram-Schmidt) rate and its energy error
 against FP64 are reported beside it.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+The self-consistent 40^3 / 400-state workload (SLy4 + Coulomb, HF filling,
+density mixing; one step = the whole Evolve iteration on the device) is
+timed after it in FP64 and mixed precision from the same start, with the
+total-energy difference after the same number of iterations
+(``full_iteration`` in the JSON line).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+                    [--full-steps K3] [--skip-full]
 """
 import argparse
 import ctypes
@@ -71,6 +78,56 @@ def timed_run(w, precision, steps, warmup, profile):
     return (t1 - t0) / steps, prof, e
 
 
+def full_iteration_run(w, precision, steps, warmup, profile):
+    """The self-consistent loop on the device: start-up, warmup iterations,
+    then `steps` timed iterations (mocca_iterate)."""
+    import mocca_amd
+    s = mocca_amd.Solve(w["nx"], w["ny"], w["nz"], w["dx"], w["dv"], w["M"],
+                        time_reversal=w["trc"], lagrange=w["lagrange"], precision=precision)
+    s.set_states_meta(w["qn"], w["occ"])
+    s.upload_states(w["psi"])
+    s.scf_setup(**w["functional"])
+    s.gram_schmidt()
+    s.scf_start()
+    s.iterate(warmup, w["dt_over_hbar"], w["momentum"], fetch=False)
+    s.synchronize()
+    if profile:
+        s.profile_enable(True)
+    t0 = time.perf_counter()
+    rows = s.iterate(steps, w["dt_over_hbar"], w["momentum"])
+    t1 = time.perf_counter()
+    prof = {}
+    if profile:
+        for k in ("hpsi_step", "reduce_ed", "gram", "gram_reduce", "cholesky", "gs_apply",
+                  "hf_fill", "density", "density_reduce", "field_jobs", "mix", "moments",
+                  "coulomb_bc", "coulomb_cg", "potentials", "pot_jobs", "pot_convert",
+                  "energy", "energy_kinetic", "kinetic"):
+            ms, cnt = s.profile_get(k)
+            if cnt:
+                prof[k] = ms / steps
+        s.profile_enable(False)
+    _, _, _, cg_iters = s.coulomb()
+    s.close()
+    return (t1 - t0) / steps, prof, rows, cg_iters
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (2 x FETCH_SIZE + WRITE_SIZE, in KiB per dispatch; the factor 2
+    is the FETCH_SIZE calibration against a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hpsi_step_gram_density_fd32_200states.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].strip("`").startswith(kernel_prefix):
+            try:
+                return (2 * float(cells[1]) + float(cells[3])) * 1024
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(w, min_seconds=10.0, max_seconds=30.0):
     """The C restatement (oracle, one core) on one Gram-Schmidt block of the
     same workload: derivatives, hPsi + step + Gram-Schmidt, densities of its
@@ -117,7 +174,21 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--full-steps", type=int, default=200)
+    ap.add_argument("--skip-full", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (the "
+                         "reference kernel for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
@@ -162,7 +233,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("mocca::k_hpsi_fd<double"),
             "kernel": "hpsi_step", "kernel_ms": t_kernel * 1e3,
             "alg_bytes_per_launch": alg_bytes,
         },
@@ -177,6 +248,28 @@ def main():
                     "from the same start, vs the FP64 run (frozen mean fields)",
         },
     }
+    if not args.skip_full:
+        wf = workloads.full_iteration()
+        k3 = args.full_steps
+        tf64, pf64, r64, cg64 = full_iteration_run(wf, FP64, k3, args.warmup, True)
+        tf32, pf32, r32, cg32 = full_iteration_run(wf, MIXED, k3, args.warmup, True)
+        out["full_iteration"] = {
+            "workload": "40x40x40 octant mesh, dx=0.8 fm, 400 states (200 n + 200 p), "
+                        "HO seeds, SLy4 + Coulomb (CG), HF filling N=126 Z=82, "
+                        "MixingScheme 0; step = the whole Evolve iteration on the device "
+                        "(GradDesc + fill + densities + mixing + moments + Coulomb + "
+                        "potentials + energy)",
+            "steps": k3, "warmup": args.warmup,
+            "fp64": {"value": 1.0 / tf64, "unit": "iters/s", "ms_per_step": tf64 * 1e3,
+                     "kernel_ms_per_step": pf64, "coulomb_cg_iterations_last": cg64},
+            "mixed": {"value": 1.0 / tf32, "unit": "iters/s", "ms_per_step": tf32 * 1e3,
+                      "speedup_vs_fp64": tf64 / tf32, "kernel_ms_per_step": pf32,
+                      "coulomb_cg_iterations_last": cg32},
+            "energy_fp64_MeV": float(r64[-1, 0]), "energy_mixed_MeV": float(r32[-1, 0]),
+            "total_energy_err_keV": float(abs(r32[-1, 0] - r64[-1, 0]) * 1e3),
+            "max_total_energy_err_keV_along_trajectory":
+                float(np.max(np.abs(r32[:, 0] - r64[:, 0])) * 1e3),
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

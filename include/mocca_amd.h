@@ -143,14 +143,19 @@ int mocca_compute_density(mocca_solve *h, mocca_density *out);
  * returns MOCCA_ERR_UNSUPPORTED. */
 typedef struct mocca_functional {
     double B[28];
-    double hbm[2];        /* hbar^2/2m per isospin (Force.f90 hbm) */
+    double hbm[2];        /* Force.f90 hbm after ReadForceInfo doubles it: hbar^2/m,
+                             so the kinetic coefficient is hbm/2 */
     double mass[2];       /* nucleonmass */
     double e2;            /* e^2 (Force.f90 e2) */
     double neutrons, protons;
     int com1body;         /* COM1Body: 2 = one-body c.o.m. in the kinetic term */
     int cexchange;        /* Cexchange: 0 none, 1 perturbative, 2 self-consistent */
-    int coulomb;          /* CoulombSolver: 1 = conjugate gradients, 0 = none */
-    int coulomb_maxit;    /* CoulombMax (500) */
+    int coulomb;          /* 0 = no Coulomb (CoulombSolver 0);
+                             1 = conjugate gradients, the reference's iterate
+                                 (CoulombSolver 1, ConjugGrad);
+                             2 = direct solve of the same discrete problem
+                                 (exact to rounding; no iterations) */
+    int coulomb_maxit;    /* CoulombMax (500), for coulomb = 1 */
     int max_moment;       /* MaxMoment = MaxLCoul (10, even, <= 12) */
     double damping;       /* DampingParam of MixingScheme 0 (0.75) */
 } mocca_functional;
@@ -175,8 +180,9 @@ int mocca_update_densities(mocca_solve *h);
 int mocca_mix_densities(mocca_solve *h);
 
 /* CalculateAllMoments (Moments.f90:1907-2004, electric moments) and
- * ConstructPotentials (PotentialMixing.f90:17-203): SolveCoulomb with the
- * conjugate-gradient solver (Coulomb.f90:255-340, 583-674, 755-900), then
+ * ConstructPotentials (PotentialMixing.f90:17-201): SolveCoulomb with the
+ * multipole boundary values (Coulomb.f90:255-340, 583-674) and the
+ * conjugate-gradient (ConjugGrad :755-900) or direct solve, then
  * CalcUPot, CalcWPot, CalcBPot, CalcSPot, CalcAPot (MeanFields.f90:115-712)
  * into the potentials used by hPsi / GradDesc. */
 int mocca_construct_potentials(mocca_solve *h);

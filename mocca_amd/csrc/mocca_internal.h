@@ -66,6 +66,9 @@ struct Scf {
     double *cpart = nullptr;       // [2 cg_nwg]
     double *vcoul = nullptr, *vcoulx = nullptr;
     int *cg_iters = nullptr;
+    double *cq[3] = {nullptr, nullptr, nullptr};    // direct solve: eigenvectors per axis
+    double *clam[3] = {nullptr, nullptr, nullptr};  // and eigenvalues
+    double *cw0 = nullptr, *cw1 = nullptr;          // [N] scratch
     double *pot = nullptr;         // FP64 potentials [48][N] (aliases fields in FP64)
     FieldJob *pjobs = nullptr;
     double *epart = nullptr;       // [e_nblk][kEnergySlots]

@@ -173,6 +173,11 @@ int launch_coulomb_cg(const CoulombK &c, double *x, const double *rho_p, double 
                       double *pb, double *ap, double *part, int nwg, double *vcoul,
                       double *vcoulx, int *iters, hipStream_t st);
 int coulomb_cg_max_groups();
+// Direct solve: Q[a] (n_a x n_a, row-major, columns = eigenvectors) and
+// lam[a] of the 1-D operators; w0, w1 scratch of nx*ny*nz doubles.
+void launch_coulomb_direct(const CoulombK &c, double *x, const double *rho_p,
+                           const double *const Q[3], const double *const lam[3], double *w0,
+                           double *w1, double *vcoul, double *vcoulx, hipStream_t st);
 void launch_potentials(const SkyrmeK &f, long N, int trc, const double *dens,
                        const double *derived, const double *vcoul, const double *vcoulx,
                        double *pot, hipStream_t st);

@@ -327,6 +327,108 @@ int coulomb_cg_max_groups() {
 }
 
 // ---------------------------------------------------------------------------
+// Direct solve of the same discrete Poisson problem.  The box operator is a
+// sum of three 1-D operators, A = T_x + T_y + T_z, each the 5-point second
+// derivative with the mirror rows at the symmetry plane; with T_a = Q_a
+// diag(lambda_a) Q_a^T (symmetric, Q_a orthogonal, computed once on the
+// host) the interior solution is V = (Q_z Q_y Q_x) D^-1 (Q_x^T Q_y^T Q_z^T) b,
+// D = lambda_x + lambda_y + lambda_z, with b the source minus what the fixed
+// boundary planes contribute through the stencil.  Six small transforms
+// along the axes replace the conjugate-gradient iterations; the result is
+// the exact discrete solution (to rounding), which the reference's CG
+// approaches to its stopping residual.
+__global__ __launch_bounds__(kBlock) void k_coulomb_rhs(CoulombK c, const double *__restrict__ x,
+                                                         const double *__restrict__ rho_p,
+                                                         double *__restrict__ b) {
+    const long q = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long nint = (long)c.nx * c.ny * c.nz;
+    if (q >= nint) return;
+    const int i = q % c.nx, j = (q / c.nx) % c.ny, k = q / ((long)c.nx * c.ny);
+    const double cf[3] = {-5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0};
+    auto X = [&](int a, int bb, int d) { return x[((long)d * c.my + bb) * c.mx + a]; };
+    double acc = -4.0 * 3.14159265358979323846 * c.e2 * rho_p[q];
+    for (int s = 1; s <= 2; ++s) {
+        if (i + s >= c.nx) acc -= cf[s] * c.f * X(i + s, j, k);
+        if (j + s >= c.ny) acc -= cf[s] * c.f * X(i, j + s, k);
+        if (k + s >= c.nz) acc -= cf[s] * c.f * X(i, j, k + s);
+    }
+    b[q] = acc;
+}
+
+// out = Q^T line (forward) or Q line (inverse) along one axis of length n
+// with element stride `inner`; with lam != null the forward result is
+// divided by lam_x + lam_y + lam_z.
+__global__ __launch_bounds__(kBlock) void k_axis_transform(const double *__restrict__ in,
+                                                            double *__restrict__ out,
+                                                            const double *__restrict__ Q, int n,
+                                                            long inner, long total, int inverse,
+                                                            const double *__restrict__ lam_x,
+                                                            const double *__restrict__ lam_y,
+                                                            const double *__restrict__ lam_z,
+                                                            int nx, int ny) {
+    extern __shared__ double qs[];
+    const bool lds = n <= 64;
+    if (lds) {
+        for (int t = threadIdx.x; t < n * n; t += kBlock) qs[t] = Q[t];
+        __syncthreads();
+    }
+    const double *M = lds ? qs : Q;
+    const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= total) return;
+    const long line = e / n;           // lines are enumerated (outer, inner-offset)
+    const int a = e % n;
+    const long base = (line / inner) * n * inner + (line % inner);
+    double acc = 0;
+    if (!inverse) {
+        for (int i = 0; i < n; ++i) acc += M[i * n + a] * in[base + i * inner];
+    } else {
+        for (int i = 0; i < n; ++i) acc += M[a * n + i] * in[base + i * inner];
+    }
+    const long o = base + a * inner;
+    if (lam_x) {
+        const int xi = o % nx, yi = (o / nx) % ny, zi = o / ((long)nx * ny);
+        acc = acc / (lam_x[xi] + lam_y[yi] + lam_z[zi]);
+    }
+    out[o] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_coulomb_finish(CoulombK c, double *__restrict__ x,
+                                                            const double *__restrict__ v,
+                                                            const double *__restrict__ rho_p,
+                                                            double *__restrict__ vcoul,
+                                                            double *__restrict__ vcoulx) {
+    const long q = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long nint = (long)c.nx * c.ny * c.nz;
+    if (q >= nint) return;
+    const int i = q % c.nx, j = (q / c.nx) % c.ny, k = q / ((long)c.nx * c.ny);
+    x[((long)k * c.my + j) * c.mx + i] = v[q];
+    vcoul[q] = v[q];
+    vcoulx[q] = -pow(3.0 / 3.14159265358979323846, 1.0 / 3.0) * c.e2 * pow(rho_p[q], 1.0 / 3.0);
+}
+
+void launch_coulomb_direct(const CoulombK &c, double *x, const double *rho_p,
+                           const double *const Q[3], const double *const lam[3], double *w0,
+                           double *w1, double *vcoul, double *vcoulx, hipStream_t st) {
+    const long nint = (long)c.nx * c.ny * c.nz;
+    const dim3 g((nint + kBlock - 1) / kBlock), b(kBlock);
+    const int n[3] = {c.nx, c.ny, c.nz};
+    const long inner[3] = {1, c.nx, (long)c.nx * c.ny};
+    auto shm = [&](int a) { return n[a] <= 64 ? sizeof(double) * n[a] * n[a] : 0; };
+    hipLaunchKernelGGL(k_coulomb_rhs, g, b, 0, st, c, x, rho_p, w0);
+    // forward x, y, z (scaled), inverse z, y, x: w0 -> w1 -> w0 -> w1 -> w0 -> w1 -> w0
+    double *buf[2] = {w0, w1};
+    int cur = 0;
+    for (int a = 0; a < 3; ++a, cur ^= 1)
+        hipLaunchKernelGGL(k_axis_transform, g, b, shm(a), st, buf[cur], buf[cur ^ 1], Q[a], n[a],
+                           inner[a], nint, 0, a == 2 ? lam[0] : nullptr, lam[1], lam[2], c.nx,
+                           c.ny);
+    for (int a = 2; a >= 0; --a, cur ^= 1)
+        hipLaunchKernelGGL(k_axis_transform, g, b, shm(a), st, buf[cur], buf[cur ^ 1], Q[a], n[a],
+                           inner[a], nint, 1, nullptr, nullptr, nullptr, c.nx, c.ny);
+    hipLaunchKernelGGL(k_coulomb_finish, g, b, 0, st, c, x, buf[cur], rho_p, vcoul, vcoulx);
+}
+
+// ---------------------------------------------------------------------------
 // Mean-field potentials, pointwise part: U (CalcUPot, MeanFields.f90:269-409),
 // B (CalcBPot :115-150), W (CalcWPot :494-541), S (CalcSPot :618-712) and
 // A (CalcAPot :411-446) into pot[48][N] at the field offsets of the hPsi

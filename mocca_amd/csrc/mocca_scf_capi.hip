@@ -47,6 +47,67 @@ double rlylm(int l, int m, double X, double Y, double Z) {
     return fac * p[l] * std::cos(m * std::atan2(Y, X));
 }
 
+// Eigen-decomposition of a symmetric n x n matrix by cyclic Jacobi rotations
+// (Golub & Van Loan, Matrix Computations, sec. 8.5): a = Q diag(w) Q^T with
+// Q[i][k] the i-th component of eigenvector k.
+void jacobi_eigen(int n, std::vector<double> a, std::vector<double> &Q, std::vector<double> &w) {
+    Q.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+    auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += A(i, j) * A(i, j);
+                if (i != j) off += A(i, j) * A(i, j);
+            }
+        if (off <= 1e-32 * tot) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A(p, q);
+                if (apq == 0.0) continue;
+                const double theta = (A(q, q) - A(p, p)) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) /
+                                 (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A(k, p), akq = A(k, q);
+                    A(k, p) = c * akp - sn * akq;
+                    A(k, q) = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A(p, k), aqk = A(q, k);
+                    A(p, k) = c * apk - sn * aqk;
+                    A(q, k) = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double &qkp = Q[(size_t)k * n + p], &qkq = Q[(size_t)k * n + q];
+                    const double x = qkp, y = qkq;
+                    qkp = c * x - sn * y;
+                    qkq = sn * x + c * y;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; ++i) w[i] = A(i, i);
+}
+
+// The 1-D Coulomb operator on n interior points: 5-point second derivative
+// (-1/12, 4/3, -5/2, 4/3, -1/12)/dx^2, mirrored (+) below the symmetry plane
+// (Coulomb_1D, CoulombDerivatives.f90:228-262); the two boundary planes
+// beyond the far edge are known values, not unknowns.
+std::vector<double> coulomb_1d(int n, double dx) {
+    const double cf[5] = {-1.0 / 12.0, 4.0 / 3.0, -5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0};
+    std::vector<double> t((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int q = -2; q <= 2; ++q) {
+            int a = i + q;
+            if (a < 0) a = -1 - a;
+            if (a < n) t[(size_t)i * n + a] += cf[q + 2] / (dx * dx);
+        }
+    return t;
+}
+
 void free_ptr(void *&p) {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -113,7 +174,20 @@ int construct_impl(mocca_solve *h) {
                        h->st);
     });
     if (rc) return rc;
-    if (s.sk.coulomb) {
+    if (s.sk.coulomb == 2) {
+        rc = run(h, "coulomb_bc", [&] {
+            launch_coulomb_bc(s.Q, s.nmom, s.lfac, s.bidx, s.bsh, s.brp, s.nb, s.sk.e2, s.cx,
+                              h->st);
+        });
+        if (rc) return rc;
+        rc = run(h, "coulomb_direct", [&] {
+            const double *Qa[3] = {s.cq[0], s.cq[1], s.cq[2]};
+            const double *La[3] = {s.clam[0], s.clam[1], s.clam[2]};
+            launch_coulomb_direct(s.ck, s.cx, dens + 9 * N, Qa, La, s.cw0, s.cw1, s.vcoul,
+                                  s.vcoulx, h->st);
+        });
+        if (rc) return rc;
+    } else if (s.sk.coulomb == 1) {
         rc = run(h, "coulomb_bc", [&] {
             launch_coulomb_bc(s.Q, s.nmom, s.lfac, s.bidx, s.bsh, s.brp, s.nb, s.sk.e2, s.cx,
                               h->st);
@@ -206,7 +280,8 @@ void scf_free(mocca_solve *h) {
     Scf &s = h->scf;
     void *ptrs[] = {s.hist, s.Y, s.mpart, s.Q, s.lfac, s.bidx, s.bsh, s.brp, s.cx, s.cr, s.cpa,
                     s.cpb, s.cap, s.cpart, s.vcoul, s.vcoulx, s.cg_iters, s.pjobs, s.epart,
-                    s.ecur, s.erows};
+                    s.ecur, s.erows, s.cq[0], s.cq[1], s.cq[2], s.clam[0], s.clam[1],
+                    s.clam[2], s.cw0, s.cw1};
     for (void *p : ptrs) free_ptr(p);
     if (s.pot && s.pot != (double *)h->b.fields) (void)hipFree(s.pot);
     s = Scf{};
@@ -226,8 +301,9 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
     if (f->max_moment < 0 || f->max_moment > 12 || (f->max_moment & 1))
         return fail(h, MOCCA_ERR_ARG, "max_moment must be even and <= 12 (got %d)",
                     f->max_moment);
-    if (f->coulomb != 0 && f->coulomb != 1)
-        return fail(h, MOCCA_ERR_UNSUPPORTED, "only the conjugate-gradient Coulomb solver (1)");
+    if (f->coulomb < 0 || f->coulomb > 2)
+        return fail(h, MOCCA_ERR_UNSUPPORTED,
+                    "coulomb: 0 none, 1 conjugate gradients, 2 direct (got %d)", f->coulomb);
     if (f->coulomb && f->coulomb_maxit < 1)
         return fail(h, MOCCA_ERR_ARG, "coulomb_maxit must be positive");
     if (f->cexchange < 0 || f->cexchange > 2) return fail(h, MOCCA_ERR_ARG, "cexchange 0..2");
@@ -370,6 +446,22 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
         s.cg_nwg = (int)std::min<long>(std::min(cap, 256), (nint + kBlock - 1) / kBlock);
     }
     SCF_ALLOC(s.cpart, 2 * (size_t)s.cg_nwg * sizeof(double));
+    // direct solve: eigen-decomposition of the 1-D operators
+    {
+        const int ns[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a) {
+            std::vector<double> Qh, wh;
+            jacobi_eigen(ns[a], coulomb_1d(ns[a], g.dx), Qh, wh);
+            SCF_ALLOC(s.cq[a], Qh.size() * sizeof(double));
+            SCF_ALLOC(s.clam[a], wh.size() * sizeof(double));
+            HIPCHK(h, hipMemcpy(s.cq[a], Qh.data(), Qh.size() * sizeof(double),
+                                hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(s.clam[a], wh.data(), wh.size() * sizeof(double),
+                                hipMemcpyHostToDevice));
+        }
+        SCF_ALLOC(s.cw0, N * sizeof(double));
+        SCF_ALLOC(s.cw1, N * sizeof(double));
+    }
     SCF_ALLOC(s.vcoul, N * sizeof(double));
     SCF_ALLOC(s.vcoulx, N * sizeof(double));
     SCF_ALLOC(s.cg_iters, sizeof(int));

@@ -6,8 +6,8 @@ Gaussian envelope (width 3 fm) times a seeded sum of 8 plane waves with
 (R = 4 fm, a = 0.6 fm): U = -50 f, B = 10 f, W_{mu nu} = -eps_{k mu nu} w d_k f,
 S = 2 f z-hat (time-odd, so time reversal is broken while parity, signature
 and time simplex are kept: the EV8 octant).  ``ho_states`` gives
-harmonic-oscillator seeds (b = 1.7 fm) with the per-axis parity each spinor
-component needs.
+harmonic-oscillator seeds with the per-axis parity each spinor component
+needs; ``full_iteration`` is the 40^3 / 400-state self-consistent workload.
 """
 import numpy as np
 
@@ -85,21 +85,31 @@ def hermite(n, t):
     return h1
 
 
-def ho_states(rng, nx, ny, nz, dx, qns, b=1.7, noise=1e-3):
-    """Harmonic-oscillator products (lowest shells) whose per-axis parity
-    matches each component's reflection sign, plus uniform noise."""
-    X, Y, Z = mesh(nx, ny, nz, dx)
-    env = np.exp(-(X**2 + Y**2 + Z**2) / (2 * b * b))
+def ho_states(rng, nx, ny, nz, dx, qns, b=1.7, noise=1e-3, nmax=10, terms=3):
+    """Harmonic-oscillator seeds: per spinor component, a random combination
+    of `terms` Hermite products H_a(x/b) H_b(y/b) H_c(z/b) exp(-r^2/2b^2)
+    with degrees below nmax whose per-axis parity matches the component's
+    reflection sign, plus uniform noise."""
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dx
+    z = (np.arange(nz) + 0.5) * dx
+    hx = [hermite(k, x / b) * np.exp(-x * x / (2 * b * b)) for k in range(nmax)]
+    hy = [hermite(k, y / b) * np.exp(-y * y / (2 * b * b)) for k in range(nmax)]
+    hz = [hermite(k, z / b) * np.exp(-z * z / (2 * b * b)) for k in range(nmax)]
+    # scale each degree to comparable size (H_n grows like 2^n n!)
+    norm = [1.0 / np.sqrt(2.0**k * np.prod(np.arange(1, k + 1))) for k in range(nmax)]
+    pick = {s: [k for k in range(nmax) if (1 if k % 2 == 0 else -1) == s] for s in (1, -1)}
     out = np.empty((len(qns), 4, nz * ny * nx))
     for i, (iso, P, S, TS) in enumerate(qns):
         sx, sy, sz = comp_signs(P, S, TS)
         for c in range(4):
-            pick = lambda s: [n for n in range(4) if (1 if n % 2 == 0 else -1) == s]
-            f = np.zeros_like(X)
-            for _ in range(2):
-                f += rng.uniform(-1, 1) * hermite(rng.choice(pick(sx[c])), X / b) * \
-                    hermite(rng.choice(pick(sy[c])), Y / b) * hermite(rng.choice(pick(sz[c])), Z / b)
-            out[i, c] = (f * env + noise * rng.uniform(-1, 1, f.shape)).reshape(-1)
+            f = np.zeros((nz, ny, nx))
+            for _ in range(terms):
+                a, bb, cc = (rng.choice(pick[sx[c]]), rng.choice(pick[sy[c]]),
+                             rng.choice(pick[sz[c]]))
+                f += (rng.uniform(-1, 1) * norm[a] * norm[bb] * norm[cc] *
+                      hz[cc][:, None, None] * hy[bb][None, :, None] * hx[a][None, None, :])
+            out[i, c] = (f + noise * rng.uniform(-1, 1, f.shape)).reshape(-1)
     return out
 
 
@@ -130,3 +140,20 @@ def hpsi_microbench(seed=2, n=32, dx=0.8, M=200):
     return dict(nx=n, ny=n, nz=n, dx=dx, dv=dv, M=M, trc=False, lagrange=False,
                 qn=qn, occ=occ, psi=psi, fields=fields, kin=kin,
                 dt_over_hbar=0.010 / 6.58211928, momentum=0.5)
+
+
+def full_iteration(seed=3, n=40, dx=0.8, M=400, neutrons=126, protons=82, b=2.4):
+    """The 40^3 / 400-state self-consistent workload: 200 neutron and 200
+    proton states (time reversal, parity, signature and time simplex
+    conserved; parity alternating), harmonic-oscillator seeds, SLy4 with
+    Coulomb, HF filling of 126 neutrons and 82 protons, fixed dt and
+    momentum, linear density mixing."""
+    from . import functional
+    rng = np.random.default_rng(seed)
+    qn = np.array([(iso, 1 if j % 2 == 0 else -1, 1, 1)
+                   for iso in (-1, 1) for j in range(M // 2)], np.int32)
+    psi = ho_states(rng, n, n, n, dx, [tuple(q) for q in qn], b=b)
+    return dict(nx=n, ny=n, nz=n, dx=dx, dv=dx**3 * 8, M=M, trc=True, lagrange=False,
+                qn=qn, occ=np.zeros(M), psi=psi,
+                functional=functional.scf_arguments("SLy4", neutrons, protons, True),
+                dt_over_hbar=0.010 / functional.HBAR, momentum=0.3)

@@ -19,20 +19,36 @@ pytestmark = pytest.mark.gpu
 SCF_CASES = ["scf_trc_fd", "scf_trc_lag", "scf_todd_fd"]
 
 
-def _functional(s, g):
+def _functional(s, g, coulomb=1):
     c, m = g["consts"], g["meta"]
     s.scf_setup(g["coef"], c[0:2], c[2:4], c[4], c[7], c[8], com1body=m["com1body"],
-                cexchange=2, coulomb=1, coulomb_maxit=500, max_moment=10, damping=0.75)
+                cexchange=2, coulomb=coulomb, coulomb_maxit=500, max_moment=10, damping=0.75)
 
 
-def _solve(g, occ, precision=mocca_amd.FP64):
+def _solve(g, occ, precision=mocca_amd.FP64, coulomb=1):
     m = g["meta"]
     s = mocca_amd.Solve(m["nx"], m["ny"], m["nz"], m["dx"], g["consts"][6], m["nwt"],
                         time_reversal=m["trc"], lagrange=m["lagrange"], precision=precision)
     s.set_states_meta(g["qn"], occ)
     s.upload_states(g["psi"])
-    _functional(s, g)
+    _functional(s, g, coulomb)
     return s
+
+
+def _box_residual(cpot, rho_p, nx, ny, nz, dx, e2):
+    """max |Lap V + 4 pi e2 rho_p| / max |4 pi e2 rho_p| of a Coulomb-box
+    potential under the 5-point-per-axis operator with mirror planes."""
+    c = [-1 / 12, 4 / 3, -5 / 2, 4 / 3, -1 / 12]
+    x = cpot.reshape(nz + 2, ny + 2, nx + 2)
+    lap = np.zeros((nz, ny, nx))
+    for q in range(-2, 3):
+        ix = np.arange(nx) + q
+        iy = np.arange(ny) + q
+        iz = np.arange(nz) + q
+        ix, iy, iz = (np.where(a < 0, -1 - a, a) for a in (ix, iy, iz))
+        lap += c[q + 2] / dx**2 * (x[:nz, :ny, ix] + x[:nz, iy, :nx] + x[iz, :ny, :nx])
+    src = -4 * np.pi * e2 * rho_p.reshape(nz, ny, nx)
+    return np.max(np.abs(lap - src)) / np.max(np.abs(src))
 
 
 def _ref_row(ge):
@@ -79,6 +95,40 @@ def test_potentials_energy_vs_reference(case):
     ref = _ref_row(g["pot_energy"])
     assert np.max(np.abs(e - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-10
     assert abs(e[0] - ref[0]) < 1e-10 * abs(ref[0])
+    s.close()
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_direct_coulomb_solve(case):
+    """The direct solve meets the discrete equation to rounding and agrees
+    with the reference's CG iterate to that iterate's own convergence."""
+    g = golden(case)
+    m = g["meta"]
+    N = m["nx"] * m["ny"] * m["nz"]
+    s = _solve(g, g["occ"], coulomb=2)
+    s.update_densities()
+    s.construct_potentials()
+    vc, vx, cp, _ = s.coulomb()
+    d = s.compute_density()
+    res = _box_residual(cp, d["rho"][1], m["nx"], m["ny"], m["nz"], m["dx"], g["consts"][4])
+    assert res < 1e-12
+    assert relerr(vc, g["pot_vcoul"].reshape(N)) < 1e-6
+    assert relerr(cp, g["pot_cpot"].reshape(-1)) < 1e-6
+    s.close()
+
+
+@pytest.mark.parametrize("case", SCF_CASES)
+def test_iterate_with_direct_coulomb_vs_reference(case):
+    """The loop with the direct Coulomb solve: total energy after every
+    iteration within 1e-9 relative of the reference's (CG) trajectory."""
+    g = golden(case)
+    m = g["meta"]
+    s = _solve(g, np.zeros(m["nwt"]), coulomb=2)
+    s.scf_start()
+    rows = s.iterate(m["niter"], m["dt"] / g["consts"][5], m["momentum"])
+    rel = np.abs(rows[:, 0] - g["traj"][:, 0]) / np.abs(g["traj"][:, 0])
+    print(f"direct Coulomb: max relative energy difference {rel.max():.2e}", flush=True)
+    assert rel.max() < 1e-9
     s.close()
 
 
