@@ -249,8 +249,9 @@ int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_m
     });
     if (rc) return rc;
     rc = run(h, "cholesky", [&] {
-        launch_cholesky_ld(h->b.G, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m,
-                           h->b.E, h->b.Tm, h->b.perm, h->b.work, h->b.status, h->st);
+        launch_cholesky_ld(h->b.G, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m, h->mmax,
+                           h->b.E, h->b.Tm, h->b.perm, h->b.work, h->b.status,
+                           h->chol_lds, h->st);
     });
     if (rc) return rc;
     return run(h, "gs_apply", [&] {
@@ -509,6 +510,7 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     TRY(dalloc(h, &b.status, sizeof(int)));
     TRY(dalloc(h, &b.one_chunk, sizeof(Chunk)));
     TRY(dalloc(h, &b.one_order, sizeof(int)));
+    h->chol_lds = cholesky_lds_limit();
     h->ntiles = (int)((N + kBlock - 1) / kBlock);
     h->march = !mesh->lagrange && mesh->nx <= 64;
     h->TY = plane_tile_rows(mesh->nx, mesh->ny);

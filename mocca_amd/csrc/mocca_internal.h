@@ -100,6 +100,7 @@ struct mocca_solve {
     bool march = false;
     int TY = 0, ntile_y = 0, ntasks = 0;
     int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
+    size_t chol_lds = 0;   // dynamic LDS bytes available to the in-LDS Cholesky
     // profiling
     bool prof = false;
     std::vector<mocca::detail::Pending> pending;

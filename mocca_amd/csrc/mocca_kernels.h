@@ -123,10 +123,13 @@ void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
                         int max_m, int mmax, int tile, int nsplit, double dv,
                         double *G, hipStream_t st);
 
+// Energy order, Cholesky and T = L^{-T} per block; in LDS when the largest
+// block fits in lds_limit bytes (cholesky_lds_limit()), else in global memory.
 void launch_cholesky_ld(const double *G, const int *blk_states,
-                        const int *blk_m, int nblk, int max_m, const double *E,
+                        const int *blk_m, int nblk, int max_m, int mmax, const double *E,
                         double *Tm, int *perm, double *work, int *status,
-                        hipStream_t st);
+                        size_t lds_limit, hipStream_t st);
+size_t cholesky_lds_limit();
 
 template <typename T>
 void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,

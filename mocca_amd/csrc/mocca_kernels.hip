@@ -650,6 +650,88 @@ __global__ __launch_bounds__(kBlock) void k_cholesky(const double *__restrict__ 
     }
 }
 
+// The same factorisation with the block resident in LDS (leading dimension
+// m+1): right-looking Cholesky with one barrier per column (the division of
+// column k by its pivot is deferred to step k+1, when nothing reads it),
+// then L^{-1} by rows into the free upper triangle, so that T = L^{-T} is
+// that upper triangle with the reciprocal pivots on the diagonal.
+__global__ __launch_bounds__(1024) void k_cholesky_lds(const double *__restrict__ G,
+                                                       const int *__restrict__ blk_states,
+                                                       const int *__restrict__ blk_m,
+                                                       int max_m, const double *__restrict__ E,
+                                                       double *__restrict__ Tm,
+                                                       int *__restrict__ perm,
+                                                       int *__restrict__ status) {
+    extern __shared__ double A[];           // [m][m+1]
+    __shared__ int rank[kMaxBlock];
+    __shared__ double dinv[kMaxBlock];
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int m = blk_m[b], ld = m + 1;
+    const int *st = blk_states + (long)b * max_m;
+    const double *g = G + (long)b * max_m * max_m;
+    double *Tb = Tm + (long)b * max_m * max_m;
+    int *pb = perm + (long)b * max_m;
+    for (int i = tid; i < m; i += nt) {
+        const double ei = E[st[i]];
+        int r = 0;
+        for (int j = 0; j < m; ++j) {
+            const double ej = E[st[j]];
+            if (ej < ei || (ej == ei && j < i)) ++r;
+        }
+        rank[i] = r;
+        pb[r] = st[i];
+    }
+    __syncthreads();
+    for (int q = tid; q < m * m; q += nt) {
+        const int i = q / m, j = q % m;
+        A[rank[i] * ld + rank[j]] = g[(long)i * max_m + j];
+    }
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        const double akk = A[k * ld + k];
+        if (k > 0) {
+            // finish column k-1: L[i][k-1] = A[i][k-1] / d_{k-1}
+            const double d = sqrt(A[(k - 1) * ld + k - 1] > 0 ? A[(k - 1) * ld + k - 1] : 1.0);
+            for (int i = k + tid; i < m; i += nt) A[i * ld + k - 1] /= d;
+        }
+        if (tid == 0 && !(akk > 0.0)) atomicOr(status, 1);
+        const double inv = 1.0 / (akk > 0.0 ? akk : 1.0);
+        const int r = m - k - 1;
+        for (int q = tid; q < r * r; q += nt) {
+            const int i = k + 1 + q / r, j = k + 1 + q % r;
+            if (j <= i) A[i * ld + j] -= A[i * ld + k] * A[j * ld + k] * inv;
+        }
+        __syncthreads();
+        if (k > 0 && tid == 0) {
+            const double pv = A[(k - 1) * ld + k - 1];
+            A[(k - 1) * ld + k - 1] = sqrt(pv > 0 ? pv : 1.0);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double pv = A[(m - 1) * ld + m - 1];
+        A[(m - 1) * ld + m - 1] = sqrt(pv > 0 ? pv : 1.0);
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += nt) dinv[i] = 1.0 / A[i * ld + i];
+    __syncthreads();
+    // rows of X = L^{-1}: X[i][j] = -(sum_{k=j}^{i-1} L[i][k] X[k][j]) / L[i][i],
+    // X[i][i] = 1/L[i][i]; X[k][j] (j < k) lives at A[j][k].
+    for (int i = 1; i < m; ++i) {
+        for (int j = tid; j < i; j += nt) {
+            double s = A[i * ld + j] * dinv[j];
+            for (int k = j + 1; k < i; ++k) s += A[i * ld + k] * A[j * ld + k];
+            A[j * ld + i] = -s * dinv[i];
+        }
+        __syncthreads();
+    }
+    // T = L^{-T}: row j of T holds X[i][j] for i >= j
+    for (int q = tid; q < m * m; q += nt) {
+        const int j = q / m, i = q % m;
+        Tb[(long)j * max_m + i] = i < j ? 0.0 : (i == j ? dinv[j] : A[j * ld + i]);
+    }
+}
+
 // Q = Psi_sorted T for each block, fused with the heavy-ball memory update
 // u = psi_new - u of GradDesc (ImaginaryTime.f90:497-499) when update_mode.
 template <typename T>
@@ -728,10 +810,33 @@ MOCCA_INST(double)
 MOCCA_INST(float)
 
 void launch_cholesky_ld(const double *G, const int *blk_states, const int *blk_m,
-                        int nblk, int max_m, const double *E, double *Tm,
-                        int *perm, double *work, int *status, hipStream_t st) {
+                        int nblk, int max_m, int mmax, const double *E, double *Tm,
+                        int *perm, double *work, int *status, size_t lds_limit,
+                        hipStream_t st) {
+    const size_t lds = sizeof(double) * (size_t)mmax * (mmax + 1);
+    if (lds <= lds_limit) {
+        hipLaunchKernelGGL(k_cholesky_lds, dim3(nblk), dim3(1024), lds, st, G, blk_states,
+                           blk_m, max_m, E, Tm, perm, status);
+        return;
+    }
     hipLaunchKernelGGL(k_cholesky, dim3(nblk), dim3(kBlock), 0, st, G, blk_states,
                        blk_m, max_m, E, Tm, perm, work, status);
+}
+
+size_t cholesky_lds_limit() {
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+        hipSuccess)
+        return 0;
+    // static LDS of the kernel (rank, dinv)
+    const size_t stat = kMaxBlock * (sizeof(int) + sizeof(double));
+    if ((size_t)lim <= stat) return 0;
+    const size_t dyn = (size_t)lim - stat;
+    if (hipFuncSetAttribute((const void *)k_cholesky_lds,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
+        return 0;
+    return dyn;
 }
 
 }  // namespace mocca
