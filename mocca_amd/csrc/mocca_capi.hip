@@ -238,16 +238,26 @@ int check_status(mocca_solve *h) {
 template <typename T>
 int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_mode) {
     const long K4 = 4L * h->N;
-    int rc = run(h, "gram", [&] {
-        launch_gram<T>(src, K4, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m, h->mmax,
-                       h->gtile, h->b.gpart, h->nsplit, h->st);
-    });
-    if (rc) return rc;
-    rc = run(h, "gram_reduce", [&] {
-        launch_gram_reduce(h->b.gpart, h->b.blk_m, h->nblk, h->max_m, h->mmax, h->gtile,
-                           h->nsplit, h->mesh.dv, h->b.G, h->st);
-    });
-    if (rc) return rc;
+    int rc;
+    if (h->mfma_gram) {
+        rc = run(h, "gram", [&] {
+            launch_gram_mfma<T>(src, K4, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m,
+                                h->mpad, h->b.gpart, h->nsplit_mfma, h->mesh.dv, h->b.G,
+                                h->st);
+        });
+        if (rc) return rc;
+    } else {
+        rc = run(h, "gram", [&] {
+            launch_gram<T>(src, K4, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m, h->mmax,
+                           h->gtile, h->b.gpart, h->nsplit, h->st);
+        });
+        if (rc) return rc;
+        rc = run(h, "gram_reduce", [&] {
+            launch_gram_reduce(h->b.gpart, h->b.blk_m, h->nblk, h->max_m, h->mmax, h->gtile,
+                               h->nsplit, h->mesh.dv, h->b.G, h->st);
+        });
+        if (rc) return rc;
+    }
     rc = run(h, "cholesky", [&] {
         launch_cholesky_ld(h->b.G, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m, h->mmax,
                            h->b.E, h->b.Tm, h->b.perm, h->b.work, h->b.status,
@@ -697,8 +707,24 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     int nsplit = std::max(1, 2048 / std::max(1, nblk * ntile_max));
     nsplit = (int)std::min<long>(nsplit, std::max<long>(1, K4 / (kGramKC * 8)));
     h->nsplit = nsplit;
-    if ((rc = dalloc(h, &b.gpart, (size_t)nblk * ntile_max * nsplit * h->gtile * h->gtile *
-                                      sizeof(double))))
+    // matrix-core Gram for blocks of up to kMfmaMaxRows states: K splits for
+    // ~2048 workgroups, partial matrices kept under 5% of the states' bytes
+    h->mpad = (mmax + 15) / 16 * 16;
+    // (below 64 states the vector kernel streams the block faster)
+    h->mfma_gram = h->mpad >= 64 && h->mpad <= kMfmaMaxRows;
+    const int nt16 = h->mpad / 16, ntile16 = nt16 * (nt16 + 1) / 2;
+    {
+        // ~4096 single-wave workgroups, partial tiles under 15% of the
+        // states' bytes, at least 4 chunks of 16 values per wave
+        const double data = (double)h->M * K4 * h->tsize;
+        long ns = std::max(1, 4096 / nblk);
+        ns = std::min<long>(ns, (long)(0.15 * data / ((double)nblk * ntile16 * 256 * 8)));
+        ns = std::min<long>(ns, std::max<long>(1, K4 / 64));
+        h->nsplit_mfma = (int)std::max<long>(1, ns);
+    }
+    const size_t gpart_old = (size_t)nblk * ntile_max * nsplit * h->gtile * h->gtile;
+    const size_t gpart_new = (size_t)nblk * h->nsplit_mfma * ntile16 * 256;
+    if ((rc = dalloc(h, &b.gpart, std::max(gpart_old, gpart_new) * sizeof(double))))
         return rc;
     const size_t npart = std::max(chunks.size(), dchunks.size());
     if ((rc = dalloc(h, &b.dpart, npart * 9 * (size_t)h->N * h->tsize))) return rc;

@@ -101,6 +101,8 @@ struct mocca_solve {
     int TY = 0, ntile_y = 0, ntasks = 0;
     int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
     size_t chol_lds = 0;   // dynamic LDS bytes available to the in-LDS Cholesky
+    bool mfma_gram = false;  // Gram on the matrix cores (blocks <= kMfmaMaxRows)
+    int mpad = 0, nsplit_mfma = 1;
     // profiling
     bool prof = false;
     std::vector<mocca::detail::Pending> pending;

@@ -136,6 +136,15 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
                      const int *blk_m, int nblk, int max_m, const double *Tm,
                      int update_mode, hipStream_t st);
 
+// Gram matrices on the FP64 matrix cores (v_mfma_f64_16x16x4f64) for blocks
+// of 64..kMfmaMaxRows states (one wave holds all upper 16x16 tiles).
+constexpr int kMfmaMaxRows = 128;
+
+template <typename T>
+void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *blk_m, int nblk,
+                      int max_m, int mpad_max, double *part, int nsplit, double dv, double *G,
+                      hipStream_t st);
+
 template <typename T>
 void launch_density_fd(const MeshK<T> &m, const T *psi, const int *qn, const double *occ,
                        const int *order, const Chunk *dchunks, const DTask *tasks,

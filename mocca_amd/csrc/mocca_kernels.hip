@@ -4,6 +4,8 @@
 // reference MOCCa tree.
 #include "mocca_kernels.h"
 
+#include <algorithm>
+
 namespace mocca {
 
 
@@ -761,11 +763,19 @@ __global__ __launch_bounds__(kBlock) void k_gs_apply(const T *__restrict__ src,
     double acc[kApplyL];
 #pragma unroll
     for (int l = 0; l < kApplyL; ++l) acc[l] = 0;
-    for (int k = 0; k < kend; ++k) {
-        const double v = (double)src[(long)pb[k] * K4 + e];
-        const double *tr = Ts + k * kApplyL;
+    // the values of 8 states are loaded before their products
+    for (int k0 = 0; k0 < kend; k0 += 8) {
+        double v[8];
 #pragma unroll
-        for (int l = 0; l < kApplyL; ++l) acc[l] += v * tr[l];
+        for (int q = 0; q < 8; ++q)
+            v[q] = k0 + q < kend ? (double)src[(long)pb[k0 + q] * K4 + e] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (k0 + q >= kend) break;
+            const double *tr = Ts + (k0 + q) * kApplyL;
+#pragma unroll
+            for (int l = 0; l < kApplyL; ++l) acc[l] += v[q] * tr[l];
+        }
     }
 #pragma unroll
     for (int l = 0; l < kApplyL; ++l) {
@@ -788,6 +798,127 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
                        perm, blk_m, max_m, Tm, update_mode);
 }
 
+// ---------------------------------------------------------------------------
+// Gram matrices and the Gram-Schmidt apply on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: D[16x16] += A[16x4] B[4x16]; operand lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; result lane l holds D[l/16 + 4r][l%16] in
+// register r -- measured on gfx950).
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mfma_d4 mfma16x16x4(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Gram partials: one wave per (K slice, block) holds all upper 16x16 tiles
+// of the block (NT row tiles, <= 8).  Per 16 mesh values each lane loads 4
+// consecutive values of its state row (16 lanes = 16 states, 4 lane groups =
+// 16 consecutive k): the k order inside a 16-chunk is permuted the same way
+// for both operands, which leaves the sum over k unchanged.  Output: the
+// raw accumulator layout part[b][split][tile][lane][reg].
+template <typename T, int NT>
+__global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, long K4,
+                                                  const int *__restrict__ blk_states,
+                                                  const int *__restrict__ blk_m, int max_m,
+                                                  double *__restrict__ part, int nsplit) {
+    constexpr int NTILE = NT * (NT + 1) / 2;
+    const int w = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
+    const int m = blk_m[b];
+    const int *st = blk_states + (long)b * max_m;
+    const long nch = (K4 + 15) / 16;
+    const long c0 = nch * w / nsplit, c1 = nch * (w + 1) / nsplit;
+    const T *rows[NT];
+    bool ok[NT];
+#pragma unroll
+    for (int I = 0; I < NT; ++I) {
+        const int r = I * 16 + lane % 16;
+        ok[I] = r < m;
+        rows[I] = psi + (long)(ok[I] ? st[r] : 0) * K4;
+    }
+    mfma_d4 acc[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) acc[t] = mfma_d4{0, 0, 0, 0};
+    // chunk c+1 is loaded while the MFMAs of chunk c run
+    auto load = [&](long c, double (&v)[NT][4]) {
+        const long kb = c * 16 + 4 * (lane / 16);
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+            if (ok[I] && kb + 3 < K4) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[I][q] = (double)rows[I][kb + q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    v[I][q] = (ok[I] && kb + q < K4) ? (double)rows[I][kb + q] : 0.0;
+            }
+        }
+    };
+    double v[NT][4], vn[NT][4];
+    if (c0 < c1) load(c0, v);
+    for (long c = c0; c < c1; ++c) {
+        if (c + 1 < c1) load(c + 1, vn);
+        int t = 0;
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int J = I; J < NT; ++J, ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(v[I][q], v[J][q], acc[t]);
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[I][q] = vn[I][q];
+    }
+    double *P = part + (((long)b * nsplit + w) * NTILE) * 256 + lane * 4;
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) P[t * 256 + r] = acc[t][r];
+}
+
+// G[b][i][j] = dv sum_split of the tile element holding (min(i,j), max(i,j)).
+__global__ __launch_bounds__(kBlock) void k_gram_mfma_reduce(const double *__restrict__ part,
+                                                             const int *__restrict__ blk_m,
+                                                             int max_m, int nt, int nsplit,
+                                                             double dv, double *__restrict__ G) {
+    const int b = blockIdx.y;
+    const int m = blk_m[b];
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= m * m) return;
+    const int i = q / m, j = q % m;
+    const int lo = min(i, j), hi = max(i, j);
+    const int I = lo / 16, J = hi / 16, li = lo % 16, lj = hi % 16;
+    const int t = I * nt - I * (I - 1) / 2 + (J - I);
+    const int ntile = nt * (nt + 1) / 2;
+    // lane l, register r hold row l/16 + 4r, column l%16 of the tile
+    const long off = (long)t * 256 + (lj + 16 * (li % 4)) * 4 + li / 4;
+    const double *P = part + (long)b * nsplit * ntile * 256 + off;
+    double s = 0;
+    for (int w = 0; w < nsplit; ++w) s += P[(long)w * ntile * 256];
+    G[((long)b * max_m + i) * max_m + j] = s * dv;
+}
+
+template <typename T>
+void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *blk_m, int nblk,
+                      int max_m, int mpad_max, double *part, int nsplit, double dv, double *G,
+                      hipStream_t st) {
+    const int nt = mpad_max / 16;
+    const dim3 g(nsplit, nblk), blk(64);
+#define MOCCA_GRAM_NT(K)                                                                    \
+    case K:                                                                                 \
+        hipLaunchKernelGGL((k_gram_mfma<T, K>), g, blk, 0, st, psi, K4, blk_states, blk_m, \
+                           max_m, part, nsplit);                                           \
+        break;
+    switch (nt) {
+        MOCCA_GRAM_NT(1) MOCCA_GRAM_NT(2) MOCCA_GRAM_NT(3) MOCCA_GRAM_NT(4)
+        MOCCA_GRAM_NT(5) MOCCA_GRAM_NT(6) MOCCA_GRAM_NT(7) MOCCA_GRAM_NT(8)
+        default: return;
+    }
+#undef MOCCA_GRAM_NT
+    const int mm = max_m * max_m;
+    hipLaunchKernelGGL(k_gram_mfma_reduce, dim3((mm + kBlock - 1) / kBlock, nblk), dim3(kBlock),
+                       0, st, part, blk_m, max_m, nt, nsplit, dv, G);
+}
+
 // Explicit instantiations.
 #define MOCCA_INST(T)                                                                   \
     template void launch_derive_all<T>(const MeshK<T> &, const T *, const int *, int,    \
@@ -805,7 +936,11 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
                                  int, int, double *, int, hipStream_t);                  \
     template void launch_gs_apply<T>(const T *, T *, T *, long, const int *,             \
                                      const int *, int, int, const double *, int,         \
-                                     hipStream_t);
+                                     hipStream_t);                                       \
+    template void launch_gram_mfma<T>(const T *, long, const int *, const int *, int,    \
+                                      int, int, double *, int, double, double *,         \
+                                      hipStream_t);
+
 MOCCA_INST(double)
 MOCCA_INST(float)
 
