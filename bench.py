@@ -100,7 +100,8 @@ def full_iteration_run(w, precision, steps, warmup, profile):
     if profile:
         for k in ("hpsi_step", "reduce_ed", "gram", "gram_reduce", "cholesky", "gs_apply",
                   "hf_fill", "density", "density_reduce", "field_jobs", "mix", "moments",
-                  "coulomb_bc", "coulomb_cg", "potentials", "pot_jobs", "pot_convert",
+                  "coulomb_bc", "coulomb_cg", "coulomb_direct", "potentials", "pot_jobs",
+                  "pot_convert",
                   "energy", "energy_kinetic", "kinetic"):
             ms, cnt = s.profile_get(k)
             if cnt:

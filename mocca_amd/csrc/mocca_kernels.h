@@ -14,6 +14,7 @@ constexpr int kChunk = 8;        // states per chunk (one isospin per chunk)
 constexpr int kGramTile = 64;    // Gram tile edge (16x16 threads x 4x4)
 constexpr int kGramKC = 16;      // K sub-chunk staged in LDS by the Gram kernel
 constexpr int kApplyL = 32;      // output states per workgroup in the apply
+constexpr int kApplyChunks = 1;  // chunks of kBlock values per apply workgroup
 constexpr int kMaxBlock = 512;   // largest (isospin, parity, signature) block
 
 // Mesh description passed by value to kernels.

@@ -666,18 +666,20 @@ __global__ __launch_bounds__(1024) void k_cholesky_lds(const double *__restrict_
                                                        int *__restrict__ status) {
     extern __shared__ double A[];           // [m][m+1]
     __shared__ int rank[kMaxBlock];
-    __shared__ double dinv[kMaxBlock];
+    __shared__ double dinv[kMaxBlock];      // first the block's energies, then 1/L[i][i]
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int m = blk_m[b], ld = m + 1;
     const int *st = blk_states + (long)b * max_m;
     const double *g = G + (long)b * max_m * max_m;
     double *Tb = Tm + (long)b * max_m * max_m;
     int *pb = perm + (long)b * max_m;
+    for (int i = tid; i < m; i += nt) dinv[i] = E[st[i]];
+    __syncthreads();
     for (int i = tid; i < m; i += nt) {
-        const double ei = E[st[i]];
+        const double ei = dinv[i];
         int r = 0;
         for (int j = 0; j < m; ++j) {
-            const double ej = E[st[j]];
+            const double ej = dinv[j];
             if (ej < ei || (ej == ei && j < i)) ++r;
         }
         rank[i] = r;
@@ -698,10 +700,10 @@ __global__ __launch_bounds__(1024) void k_cholesky_lds(const double *__restrict_
         }
         if (tid == 0 && !(akk > 0.0)) atomicOr(status, 1);
         const double inv = 1.0 / (akk > 0.0 ? akk : 1.0);
-        const int r = m - k - 1;
-        for (int q = tid; q < r * r; q += nt) {
-            const int i = k + 1 + q / r, j = k + 1 + q % r;
-            if (j <= i) A[i * ld + j] -= A[i * ld + k] * A[j * ld + k] * inv;
+        // trailing update on a 32 x 32 thread grid (blockDim = 1024)
+        for (int i = k + 1 + tid / 32; i < m; i += 32) {
+            const double aik = A[i * ld + k] * inv;
+            for (int j = k + 1 + tid % 32; j <= i; j += 32) A[i * ld + j] -= aik * A[j * ld + k];
         }
         __syncthreads();
         if (k > 0 && tid == 0) {
@@ -716,14 +718,25 @@ __global__ __launch_bounds__(1024) void k_cholesky_lds(const double *__restrict_
     }
     __syncthreads();
     for (int i = tid; i < m; i += nt) dinv[i] = 1.0 / A[i * ld + i];
+    // X = L^{-1} by right-looking substitution: R = I - (sum over finished
+    // rows), row k of X = R[k][:] / L[k][k], then R[i][j] -= L[i][k] X[k][j]
+    // for i > k, j <= k.  R and X (j < i) live in the free upper triangle at
+    // A[j][i]; the diagonal of R stays 1, so X[k][k] = 1/L[k][k].
+    for (int q = tid; q < m * m; q += nt) {
+        const int j = q / m, i = q % m;
+        if (j < i) A[j * ld + i] = 0.0;
+    }
     __syncthreads();
-    // rows of X = L^{-1}: X[i][j] = -(sum_{k=j}^{i-1} L[i][k] X[k][j]) / L[i][i],
-    // X[i][i] = 1/L[i][i]; X[k][j] (j < k) lives at A[j][k].
-    for (int i = 1; i < m; ++i) {
-        for (int j = tid; j < i; j += nt) {
-            double s = A[i * ld + j] * dinv[j];
-            for (int k = j + 1; k < i; ++k) s += A[i * ld + k] * A[j * ld + k];
-            A[j * ld + i] = -s * dinv[i];
+    for (int k = 0; k < m; ++k) {
+        const double dk = dinv[k];
+        for (int j = tid; j < k; j += nt) A[j * ld + k] *= dk;   // X[k][j]
+        __syncthreads();
+        for (int i = k + 1 + tid / 32; i < m; i += 32) {
+            const double lik = A[i * ld + k];
+            for (int j = tid % 32; j <= k; j += 32) {
+                const double xkj = j == k ? dk : A[j * ld + k];
+                A[j * ld + i] -= lik * xkj;
+            }
         }
         __syncthreads();
     }
@@ -758,32 +771,35 @@ __global__ __launch_bounds__(kBlock) void k_gs_apply(const T *__restrict__ src,
         Ts[q] = (l0 + l < m) ? Tb[(long)k * max_m + l0 + l] : 0.0;
     }
     __syncthreads();
-    const long e = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= K4) return;
-    double acc[kApplyL];
+    // the staged T columns serve kApplyChunks consecutive chunks of kBlock values
+    for (int ch = 0; ch < kApplyChunks; ++ch) {
+        const long e = ((long)blockIdx.x * kApplyChunks + ch) * kBlock + threadIdx.x;
+        if (e >= K4) return;
+        double acc[kApplyL];
 #pragma unroll
-    for (int l = 0; l < kApplyL; ++l) acc[l] = 0;
-    // the values of 8 states are loaded before their products
-    for (int k0 = 0; k0 < kend; k0 += 8) {
-        double v[8];
+        for (int l = 0; l < kApplyL; ++l) acc[l] = 0;
+        // the values of 8 states are loaded before their products
+        for (int k0 = 0; k0 < kend; k0 += 8) {
+            double v[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            v[q] = k0 + q < kend ? (double)src[(long)pb[k0 + q] * K4 + e] : 0.0;
+            for (int q = 0; q < 8; ++q)
+                v[q] = k0 + q < kend ? (double)src[(long)pb[k0 + q] * K4 + e] : 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            if (k0 + q >= kend) break;
-            const double *tr = Ts + (k0 + q) * kApplyL;
+            for (int q = 0; q < 8; ++q) {
+                if (k0 + q >= kend) break;
+                const double *tr = Ts + (k0 + q) * kApplyL;
 #pragma unroll
-            for (int l = 0; l < kApplyL; ++l) acc[l] += v[q] * tr[l];
+                for (int l = 0; l < kApplyL; ++l) acc[l] += v[q] * tr[l];
+            }
         }
-    }
 #pragma unroll
-    for (int l = 0; l < kApplyL; ++l) {
-        if (l0 + l < m) {
-            const long o = (long)pb[l0 + l] * K4 + e;
-            const T r = (T)acc[l];
-            dst[o] = r;
-            if (update_mode) upd[o] = r - upd[o];
+        for (int l = 0; l < kApplyL; ++l) {
+            if (l0 + l < m) {
+                const long o = (long)pb[l0 + l] * K4 + e;
+                const T r = (T)acc[l];
+                dst[o] = r;
+                if (update_mode) upd[o] = r - upd[o];
+            }
         }
     }
 }
@@ -792,7 +808,8 @@ template <typename T>
 void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
                      const int *blk_m, int nblk, int max_m, const double *Tm,
                      int update_mode, hipStream_t st) {
-    dim3 g((K4 + kBlock - 1) / kBlock, (max_m + kApplyL - 1) / kApplyL, nblk);
+    const long per = (long)kBlock * kApplyChunks;
+    dim3 g((K4 + per - 1) / per, (max_m + kApplyL - 1) / kApplyL, nblk);
     const size_t shm = sizeof(double) * (size_t)max_m * kApplyL;
     hipLaunchKernelGGL(k_gs_apply<T>, g, dim3(kBlock), shm, st, src, dst, upd, K4,
                        perm, blk_m, max_m, Tm, update_mode);
@@ -875,26 +892,37 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
         for (int r = 0; r < 4; ++r) P[t * 256 + r] = acc[t][r];
 }
 
-// G[b][i][j] = dv sum_split of the tile element holding (min(i,j), max(i,j)).
+// G[b][i][j] = G[b][j][i] = dv sum_split of the tile element holding
+// (i, j), i <= j: one wave per element, lanes over the splits, then a fixed
+// shuffle tree.
 __global__ __launch_bounds__(kBlock) void k_gram_mfma_reduce(const double *__restrict__ part,
                                                              const int *__restrict__ blk_m,
                                                              int max_m, int nt, int nsplit,
                                                              double dv, double *__restrict__ G) {
     const int b = blockIdx.y;
     const int m = blk_m[b];
-    const int q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= m * m) return;
-    const int i = q / m, j = q % m;
-    const int lo = min(i, j), hi = max(i, j);
-    const int I = lo / 16, J = hi / 16, li = lo % 16, lj = hi % 16;
+    const long pidx = (long)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (pidx >= (long)m * (m + 1) / 2) return;
+    // pidx -> (i, j), i <= j, row-major over the upper triangle
+    int i = 0;
+    long rem = pidx;
+    while (rem >= m - i) { rem -= m - i; ++i; }
+    const int j = i + (int)rem;
+    const int I = i / 16, J = j / 16, li = i % 16, lj = j % 16;
     const int t = I * nt - I * (I - 1) / 2 + (J - I);
     const int ntile = nt * (nt + 1) / 2;
-    // lane l, register r hold row l/16 + 4r, column l%16 of the tile
+    // lane l, register r of a tile hold row l/16 + 4r, column l%16
     const long off = (long)t * 256 + (lj + 16 * (li % 4)) * 4 + li / 4;
     const double *P = part + (long)b * nsplit * ntile * 256 + off;
     double s = 0;
-    for (int w = 0; w < nsplit; ++w) s += P[(long)w * ntile * 256];
-    G[((long)b * max_m + i) * max_m + j] = s * dv;
+    for (int w = lane; w < nsplit; w += 64) s += P[(long)w * ntile * 256];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        G[((long)b * max_m + i) * max_m + j] = s * dv;
+        G[((long)b * max_m + j) * max_m + i] = s * dv;
+    }
 }
 
 template <typename T>
@@ -914,9 +942,10 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
         default: return;
     }
 #undef MOCCA_GRAM_NT
-    const int mm = max_m * max_m;
-    hipLaunchKernelGGL(k_gram_mfma_reduce, dim3((mm + kBlock - 1) / kBlock, nblk), dim3(kBlock),
-                       0, st, part, blk_m, max_m, nt, nsplit, dv, G);
+    const long pairs = (long)max_m * (max_m + 1) / 2;
+    const long per = kBlock / 64;
+    hipLaunchKernelGGL(k_gram_mfma_reduce, dim3((pairs + per - 1) / per, nblk), dim3(kBlock), 0,
+                       st, part, blk_m, max_m, nt, nsplit, dv, G);
 }
 
 // Explicit instantiations.

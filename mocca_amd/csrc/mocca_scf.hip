@@ -45,15 +45,21 @@ __global__ __launch_bounds__(kBlock) void k_hf_fill(const double *__restrict__ E
                                                      const int *__restrict__ qn, int M,
                                                      int ub_n, int ub_p, double fac,
                                                      double *__restrict__ occ) {
+    extern __shared__ double se[];          // [M] energies, then [M] isospins
+    int *si = reinterpret_cast<int *>(se + M);
+    for (int j = threadIdx.x; j < M; j += kBlock) {
+        se[j] = E[j];
+        si[j] = qn[4 * j];
+    }
+    __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= M) return;
-    const int iso = qn[4 * i];
-    const double e = E[i];
+    const int iso = si[i];
+    const double e = se[i];
     int before = 0;
     for (int j = 0; j < M; ++j) {
-        if (qn[4 * j] != iso) continue;
-        const double f = E[j];
-        before += (f < e || (f == e && j < i)) ? 1 : 0;
+        const double f = se[j];
+        before += (si[j] == iso && (f < e || (f == e && j < i))) ? 1 : 0;
     }
     const int ub = iso == -1 ? ub_n : ub_p;
     occ[i] = before < ub ? fac : 0.0;
@@ -61,7 +67,8 @@ __global__ __launch_bounds__(kBlock) void k_hf_fill(const double *__restrict__ E
 
 void launch_hf_fill(const double *E, const int *qn, int M, int ub_n, int ub_p, double fac,
                     double *occ, hipStream_t st) {
-    hipLaunchKernelGGL(k_hf_fill, dim3((M + kBlock - 1) / kBlock), dim3(kBlock), 0, st, E, qn,
+    const size_t shm = (size_t)M * (sizeof(double) + sizeof(int));
+    hipLaunchKernelGGL(k_hf_fill, dim3((M + kBlock - 1) / kBlock), dim3(kBlock), shm, st, E, qn,
                        M, ub_n, ub_p, fac, occ);
 }
 
@@ -116,20 +123,24 @@ __global__ __launch_bounds__(kBlock) void k_moments(const double *__restrict__ r
     }
 }
 
-__global__ void k_moments_reduce(const double *__restrict__ part, int nblk, int nmom, double dv,
-                                 double *__restrict__ Q) {
-    const int t = threadIdx.x;
+// One wave per output (moment, isospin): lanes sum strided partials, then a
+// fixed shuffle tree.
+__global__ __launch_bounds__(64) void k_moments_reduce(const double *__restrict__ part, int nblk,
+                                                      int nmom, double dv,
+                                                      double *__restrict__ Q) {
+    const int t = blockIdx.x;
     if (t >= 2 * nmom) return;
     double s = 0;
-    for (int b = 0; b < nblk; ++b) s += part[(long)b * kMaxMom * 2 + t];
-    Q[t] = s * dv;
+    for (int b = threadIdx.x; b < nblk; b += 64) s += part[(long)b * kMaxMom * 2 + t];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) Q[t] = s * dv;
 }
 
 void launch_moments(const double *rho_n, const double *rho_p, const double *Y, int nmom, long N,
                     double dv, double *part, int nblk, double *Q, hipStream_t st) {
     hipLaunchKernelGGL(k_moments, dim3(nblk), dim3(kBlock), 0, st, rho_n, rho_p, Y, nmom, N,
                        part);
-    hipLaunchKernelGGL(k_moments_reduce, dim3(1), dim3(2 * kMaxMom), 0, st, part, nblk, nmom, dv,
+    hipLaunchKernelGGL(k_moments_reduce, dim3(2 * nmom), dim3(64), 0, st, part, nblk, nmom, dv,
                        Q);
 }
 
@@ -648,15 +659,17 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(SkyrmeK f, long N, in
 // terms 1..36]; the kinetic and c.o.m. slots are filled by k_energy_kinetic.
 __global__ void k_energy_terms(SkyrmeK f, const double *__restrict__ part, int nblk, int trc,
                                double dv, double *__restrict__ e) {
+    // one wave per slot: lanes sum strided partials, fixed shuffle tree
     __shared__ double S[kE_count];
-    const int t = threadIdx.x;
-    if (t < kE_count) {
+    const int wv = threadIdx.x / 64, ln = threadIdx.x % 64;
+    for (int slot = wv; slot < kE_count; slot += blockDim.x / 64) {
         double s = 0;
-        for (int b = 0; b < nblk; ++b) s += part[(long)b * kE_count + t];
-        S[t] = s;
+        for (int b = ln; b < nblk; b += 64) s += part[(long)b * kE_count + slot];
+        s = wave_sum(s);
+        if (ln == 0) S[slot] = s;
     }
     __syncthreads();
-    if (t != 0) return;
+    if (threadIdx.x != 0) return;
     double T[37];
     for (int k = 0; k < 37; ++k) T[k] = 0;
     T[1] = f.B1 * S[kE_rt2];
@@ -693,15 +706,20 @@ __global__ void k_energy_terms(SkyrmeK f, const double *__restrict__ part, int n
 // Kinetic energy (CompKinetic, Energy.f90:1426-1456) from the per-state sums
 // K_i = dv <psi_i|Lap psi_i>, the one-body c.o.m. correction
 // (CompCOMCorrection :1608-1660) and the total (CompEnergy :62-179).
-__global__ void k_energy_kinetic(SkyrmeK f, const double *__restrict__ K,
-                                 const double *__restrict__ occ, const int *__restrict__ qn,
-                                 int M, double *__restrict__ e) {
-    if (threadIdx.x != 0) return;
-    double kin[2] = {0, 0};
-    for (int l = 0; l < M; ++l) {
-        const int it = (qn[4 * l] + 3) / 2 - 1;
-        kin[it] = kin[it] + occ[l] * K[l];
+__global__ __launch_bounds__(kBlock) void k_energy_kinetic(SkyrmeK f, const double *__restrict__ K,
+                                                          const double *__restrict__ occ,
+                                                          const int *__restrict__ qn, int M,
+                                                          double *__restrict__ e) {
+    __shared__ double sh[kBlock / 64];
+    double kn = 0, kp = 0;
+    for (int l = threadIdx.x; l < M; l += kBlock) {
+        const double v = occ[l] * K[l];
+        if (qn[4 * l] == -1) kn += v; else kp += v;
     }
+    const double sn = block_sum(kn, sh);
+    const double sp = block_sum(kp, sh);
+    if (threadIdx.x != 0) return;
+    double kin[2] = {sn, sp};
     double com[2] = {0, 0};
     for (int it = 0; it < 2; ++it) {
         kin[it] = -kin[it] / 2.0 * f.hbm[it];
@@ -722,12 +740,12 @@ void launch_energy_terms(const SkyrmeK &f, long N, int trc, const double *dens,
                          double dv, double *e, hipStream_t st) {
     hipLaunchKernelGGL(k_energy_partial, dim3(nblk), dim3(kBlock), 0, st, f, N, trc, dens,
                        derived, vcoul, part);
-    hipLaunchKernelGGL(k_energy_terms, dim3(1), dim3(64), 0, st, f, part, nblk, trc, dv, e);
+    hipLaunchKernelGGL(k_energy_terms, dim3(1), dim3(1024), 0, st, f, part, nblk, trc, dv, e);
 }
 
 void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ, const int *qn,
                            int M, double *e, hipStream_t st) {
-    hipLaunchKernelGGL(k_energy_kinetic, dim3(1), dim3(64), 0, st, f, K, occ, qn, M, e);
+    hipLaunchKernelGGL(k_energy_kinetic, dim3(1), dim3(kBlock), 0, st, f, K, occ, qn, M, e);
 }
 
 }  // namespace mocca
