@@ -202,6 +202,33 @@ int mocca_scf_start(mocca_solve *h);
 int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
                   double *energies);
 
+/* IterativeEstimation (ImaginaryTime.f90:28-211), called by GradDesc when
+ * ParameterEstimation = 1 (:464): up to 500 power iterations of an extra
+ * state towards the largest eigenvalue of h, heavy-ball steps with the
+ * given dt and momentum, stopping at |change of <psi|h psi>| < 1e-2 MeV; then
+ * relE (HF: lowest particle-hole energy of one isospin, at least 0.1 MeV),
+ * and out[6] = { dt, momentum, maxE, relE, last change, iterations }.
+ * At iteration 1 (or the first call on the handle) the extra state starts
+ * from uniform [0,1) numbers (random_number in the reference) with the
+ * quantum numbers of the last state (of the last neutron state when
+ * protons = 0); later calls continue from where the previous one ended. */
+int mocca_estimate_parameters(mocca_solve *h, int iteration, double dt, double momentum,
+                              double *out);
+
+/* Start vector grid[4][N] (the layout of mocca_upload_states) for the next
+ * call that starts the estimation state, instead of random numbers; NULL
+ * clears it.  Lets a caller reproduce a run whose random start it knows. */
+int mocca_estimation_seed(mocca_solve *h, const double *grid);
+
+/* mocca_iterate with ParameterEstimation = 1: before each GradDesc,
+ * mocca_estimate_parameters with the current dt, momentum (dt in units of
+ * 1e-22 s, not divided by hbar), whose results carry over to the next
+ * iteration.  first_iteration numbers the first iteration (1 at the start
+ * of a run); params (may be NULL) receives [niter][2] = the dt and momentum
+ * used by each GradDesc. */
+int mocca_iterate_estimated(mocca_solve *h, int first_iteration, int niter, double dt,
+                            double momentum, double *energies, double *params);
+
 /* Results of the last ConstructPotentials.  lm[k][2] = (l, m) and
  * q[k][2] = (neutron, proton) moments; *count moments (<= 28).
  * vcoul, vcoulx [N]; cpot [(nz+2)][(ny+2)][(nx+2)] (CPotComplete);

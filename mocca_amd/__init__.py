@@ -31,7 +31,8 @@ EXPORTS = (
     "mocca_scf_setup", "mocca_hf_fill", "mocca_get_occupations", "mocca_update_densities",
     "mocca_mix_densities", "mocca_construct_potentials", "mocca_comp_energy",
     "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
-    "mocca_get_potentials",
+    "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
+    "mocca_iterate_estimated",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -260,6 +261,30 @@ class Solve:
         self._chk(self.lib.mocca_iterate(self.h, int(niter), ctypes.c_double(dt_over_hbar),
                                          ctypes.c_double(momentum), _p(e)))
         return e
+
+    def estimate_parameters(self, iteration, dt, momentum):
+        """IterativeEstimation: (dt, momentum, maxE, relE, con, iterations)."""
+        out = np.zeros(6)
+        self._chk(self.lib.mocca_estimate_parameters(self.h, int(iteration), ctypes.c_double(dt),
+                                                     ctypes.c_double(momentum), _p(out)))
+        return out
+
+    def estimation_seed(self, grid):
+        """Start vector [4][N] of the estimation state for its next start."""
+        g = None if grid is None else np.ascontiguousarray(grid, dtype=np.float64)
+        if g is not None and g.size != 4 * self.N:
+            raise ValueError("seed grid must hold 4 N values")
+        self._chk(self.lib.mocca_estimation_seed(self.h, _p(g)))
+
+    def iterate_estimated(self, niter, dt, momentum, first_iteration=1):
+        """mocca_iterate with IterativeEstimation before every step: returns
+        (energy rows [niter+1][43], params [niter][2] = dt, momentum used)."""
+        e = np.zeros((niter + 1, 43))
+        prm = np.zeros((max(niter, 1), 2))
+        self._chk(self.lib.mocca_iterate_estimated(self.h, int(first_iteration), int(niter),
+                                                   ctypes.c_double(dt), ctypes.c_double(momentum),
+                                                   _p(e), _p(prm)))
+        return e, prm[:niter]
 
     def moments(self):
         lm = np.zeros(2 * 28, np.int32)

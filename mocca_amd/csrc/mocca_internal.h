@@ -77,6 +77,22 @@ struct Scf {
     int erow_cap = 0;
 };
 
+// The extra state of IterativeEstimation (maxspwf) and its records.
+struct Est {
+    bool ready = false;            // psi holds the state of an earlier call
+    void *psi = nullptr, *h = nullptr, *u = nullptr;  // T [4N]
+    double *eparts = nullptr;      // [slots][3] hPsi partial sums
+    double *npart = nullptr;       // [nblk] norm partial sums
+    double *rec = nullptr;         // [kEstSlots]
+    int *qn = nullptr;             // [4]
+    Task *tasks = nullptr;         // z-marching hPsi: state 0, every y tile
+    int *order = nullptr;          // chunked hPsi: the one state
+    Chunk *chunk = nullptr;
+    int nblk = 0;
+    std::vector<double> seed;      // start vector for the next first call
+    unsigned long long calls = 0;
+};
+
 }  // namespace detail
 }  // namespace mocca
 
@@ -87,6 +103,7 @@ struct mocca_solve {
     std::string err;
     mocca::detail::Buffers b;
     mocca::detail::Scf scf;
+    mocca::detail::Est est;
     mocca::MeshK<double> md{};
     mocca::MeshK<float> mf{};
     bool meta_set = false, fields_set = false;
@@ -163,6 +180,9 @@ int grad_desc_any(mocca_solve *h, double dth, double mu);
 int density_any(mocca_solve *h);
 // K[i] = dv <psi_i|Lap psi_i> of the current states (no other output).
 int kinetic_any(mocca_solve *h);
+// IterativeEstimation: out = dt, momentum, maxE, relE, con, iterations.
+int estimate_any(mocca_solve *h, int iteration, double dt, double mu, double *out);
+constexpr double kHbar = 6.58211928;   // Force.f90:103
 // Releases the buffers of the self-consistent part.
 void scf_free(mocca_solve *h);
 

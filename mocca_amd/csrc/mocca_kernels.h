@@ -203,4 +203,19 @@ void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ,
                            int M, double *e, hipStream_t st);
 constexpr int kEnergySlots = 35;   // partial sums per workgroup of the energy pass
 
+// IterativeEstimation (power iteration for dt and momentum).
+constexpr int kEstSlots = 8;
+template <typename T>
+void launch_est_reset(T *psi, T *u, long n, int fill, unsigned long long seed, double *est,
+                      int nblk, hipStream_t st);
+template <typename T>
+void launch_est_start(T *psi, long n, double *npart, int nblk, double dv, double *est,
+                      hipStream_t st);
+template <typename T>
+void launch_est_iteration(T *psi, const T *hpsi, T *u, long n, const double *eparts, int slots,
+                          double dv, double dth, double mu, int iter, int first, double tol,
+                          double *est, double *npart, int nblk, hipStream_t st);
+void launch_est_finalize(const double *E, const double *occ, const int *qn, int M, int pairing,
+                         const double *qpe, int nqp, double hbar, double *est, hipStream_t st);
+
 }  // namespace mocca

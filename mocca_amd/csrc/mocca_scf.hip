@@ -748,4 +748,196 @@ void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ,
     hipLaunchKernelGGL(k_energy_kinetic, dim3(1), dim3(kBlock), 0, st, f, K, occ, qn, M, e);
 }
 
+// ---------------------------------------------------------------------------
+// IterativeEstimation (ImaginaryTime.f90:28-211): power iteration of one
+// extra state (maxspwf) towards the largest eigenvalue of h, then dt and the
+// heavy-ball momentum from that eigenvalue and the lowest excitation.
+//
+// Device record est[kEstSlots]: est[0] = maxE of the last executed power
+// iteration, est[1] = con = previous maxE - maxE, est[2] = iterations
+// executed, est[3] = iteration at which |con| < tol was met (0 while
+// running), est[4..7] = dt, momentum, relE, maxE - minh from k_est_finalize.
+// The host enqueues power iterations in batches; every kernel of iteration
+// k returns at once when est[3] holds an earlier iteration, so a batch that
+// runs past convergence leaves the state as it was at convergence.
+
+// Counter-based uniform [0,1) numbers (splitmix64 of seed + index), the
+// start vector of the first call (random_number in the reference).
+__device__ __forceinline__ double uniform01(unsigned long long seed, unsigned long long i) {
+    unsigned long long z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_est_reset(T *__restrict__ psi, T *__restrict__ u,
+                                                     long n, int fill, unsigned long long seed,
+                                                     double *__restrict__ est) {
+    for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        if (fill) psi[i] = T(uniform01(seed, (unsigned long long)i));
+        u[i] = T(0);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kEstSlots)
+        est[threadIdx.x] = threadIdx.x == 1 ? 1e300 : 0.0;
+}
+
+// One power iteration, first half: maxE = dv <psi|h psi> from the hPsi
+// partial sums, u = dth (h psi - maxE psi) + mu u, psi += u, and the
+// workgroup's part of <psi|psi>.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_est_update(
+    T *__restrict__ psi, const T *__restrict__ hpsi, T *__restrict__ u, long n,
+    const double *__restrict__ eparts, int slots, double dv, double dth, double mu, int iter,
+    int first, double *__restrict__ est, double *__restrict__ npart) {
+    __shared__ double sh[kBlock / 64];
+    const double stop = est[3];
+    if (stop != 0.0 && stop < iter) return;
+    double a = 0;
+    for (int t = threadIdx.x; t < slots; t += kBlock) a += eparts[3 * t];
+    const double maxE = block_sum(a, sh) * dv;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        est[1] = first ? 1e300 : est[0] - maxE;
+        est[0] = maxE;
+    }
+    double nn = 0;
+    for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        const double v = (double)psi[i];
+        const double w = dth * ((double)hpsi[i] - maxE * v) + mu * (double)u[i];
+        const T nv = T(v + w);
+        u[i] = T(w);
+        psi[i] = nv;
+        nn += (double)nv * (double)nv;
+    }
+    nn = block_sum(nn, sh);
+    if (threadIdx.x == 0) npart[blockIdx.x] = nn;
+}
+
+// Second half: psi /= sqrt(dv <psi|psi>); records the iteration and, when
+// |con| < tol, the stop.  With n4 partials from k_est_sumsq (iter 0) it is
+// the normalisation of the start vector.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_est_normalize(T *__restrict__ psi, long n,
+                                                         const double *__restrict__ npart,
+                                                         int nblk, double dv, double tol,
+                                                         int iter, double *__restrict__ est) {
+    __shared__ double sh[kBlock / 64];
+    const double stop = est[3];
+    if (stop != 0.0 && stop < iter) return;
+    double a = 0;
+    for (int t = threadIdx.x; t < nblk; t += kBlock) a += npart[t];
+    const double scale = 1.0 / sqrt(block_sum(a, sh) * dv);
+    for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+        psi[i] = T((double)psi[i] * scale);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && iter > 0) {
+        est[2] = iter;
+        if (fabs(est[1]) < tol) est[3] = iter;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_est_sumsq(const T *__restrict__ psi, long n,
+                                                     double *__restrict__ npart) {
+    __shared__ double sh[kBlock / 64];
+    double nn = 0;
+    for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+        nn += (double)psi[i] * (double)psi[i];
+    nn = block_sum(nn, sh);
+    if (threadIdx.x == 0) npart[blockIdx.x] = nn;
+}
+
+// relE, minh and the parameters (ImaginaryTime.f90:151-209).  HF (pairing
+// 0): relE = min over occupied i, empty ii of the same isospin of
+// E_ii - E_i > 0, at least 0.1; HFB (pairing 2): relE = min of the positive
+// quasiparticle energies qpe[0..nqp).  dt uses the single-precision 0.90 of
+// the reference (0.899999976158142...).
+__global__ __launch_bounds__(kBlock) void k_est_finalize(const double *__restrict__ E,
+                                                        const double *__restrict__ occ,
+                                                        const int *__restrict__ qn, int M,
+                                                        int pairing, const double *__restrict__ qpe,
+                                                        int nqp, double hbar,
+                                                        double *__restrict__ est) {
+    __shared__ double sr[kBlock], sm[kBlock];
+    double rel = 10000.0, mh = 10000.0;
+    if (pairing == 0) {
+        for (int i = 0; i < M; ++i) {
+            if (fabs(occ[i]) < 0.5) continue;
+            const int iso = qn[4 * i];
+            for (int ii = threadIdx.x; ii < M; ii += kBlock) {
+                if (fabs(occ[ii]) > 0.5 || qn[4 * ii] != iso) continue;
+                const double c = -E[i] + E[ii];
+                if (c > 0.0) rel = fmin(rel, c);
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < nqp; i += kBlock)
+            if (qpe[i] > 0.0) rel = fmin(rel, fabs(qpe[i]));
+    }
+    for (int i = threadIdx.x; i < M; i += kBlock) mh = fmin(mh, E[i]);
+    sr[threadIdx.x] = rel;
+    sm[threadIdx.x] = mh;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] = fmin(sr[threadIdx.x], sr[threadIdx.x + o]);
+            sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double relE = sr[0];
+    if (pairing == 0 && relE < 0.1) relE = 0.1;
+    const double maxE = est[0] - sm[0];
+    const double kappa = relE / maxE;
+    const double sk = sqrt(kappa);
+    const double m = ((sk - 1) / (sk + 1)) * ((sk - 1) / (sk + 1));
+    est[4] = 4.0 / (maxE + relE + 2 * sqrt(maxE * relE)) * hbar * (double)0.90f;
+    est[5] = m;
+    est[6] = relE;
+    est[7] = maxE;
+}
+
+template <typename T>
+void launch_est_reset(T *psi, T *u, long n, int fill, unsigned long long seed, double *est,
+                      int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_est_reset<T>, dim3(nblk), dim3(kBlock), 0, st, psi, u, n, fill, seed,
+                       est);
+}
+
+template <typename T>
+void launch_est_start(T *psi, long n, double *npart, int nblk, double dv, double *est,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_est_sumsq<T>, dim3(nblk), dim3(kBlock), 0, st, psi, n, npart);
+    hipLaunchKernelGGL(k_est_normalize<T>, dim3(nblk), dim3(kBlock), 0, st, psi, n, npart, nblk,
+                       dv, 0.0, 0, est);
+}
+
+template <typename T>
+void launch_est_iteration(T *psi, const T *hpsi, T *u, long n, const double *eparts, int slots,
+                          double dv, double dth, double mu, int iter, int first, double tol,
+                          double *est, double *npart, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(k_est_update<T>, dim3(nblk), dim3(kBlock), 0, st, psi, hpsi, u, n, eparts,
+                       slots, dv, dth, mu, iter, first, est, npart);
+    hipLaunchKernelGGL(k_est_normalize<T>, dim3(nblk), dim3(kBlock), 0, st, psi, n, npart, nblk,
+                       dv, tol, iter, est);
+}
+
+void launch_est_finalize(const double *E, const double *occ, const int *qn, int M, int pairing,
+                         const double *qpe, int nqp, double hbar, double *est, hipStream_t st) {
+    hipLaunchKernelGGL(k_est_finalize, dim3(1), dim3(kBlock), 0, st, E, occ, qn, M, pairing, qpe,
+                       nqp, hbar, est);
+}
+
+#define MOCCA_EST_INST(T)                                                                      \
+    template void launch_est_reset<T>(T *, T *, long, int, unsigned long long, double *, int,  \
+                                      hipStream_t);                                            \
+    template void launch_est_start<T>(T *, long, double *, int, double, double *, hipStream_t); \
+    template void launch_est_iteration<T>(T *, const T *, T *, long, const double *, int,      \
+                                          double, double, double, int, int, double, double *,  \
+                                          double *, int, hipStream_t);
+MOCCA_EST_INST(double)
+MOCCA_EST_INST(float)
+#undef MOCCA_EST_INST
+
 }  // namespace mocca

@@ -271,6 +271,65 @@ int check_gs_status(mocca_solve *h) {
     return 0;
 }
 
+// The Evolve loop (Main.f90:255-328) for niter iterations.  With estimate,
+// IterativeEstimation runs before each step (ParameterEstimation = 1,
+// ImaginaryTime.f90:464) and dt, momentum carry over from one iteration to
+// the next; params (when not null) receives the pair used at each.
+int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, double dth,
+                 double momentum, bool estimate, double *energies, double *params) {
+    if (int rc = check_scf(h)) return rc;
+    if (!h->scf.started) return fail(h, MOCCA_ERR_STATE, "mocca_scf_start not called");
+    if (niter < 0) return fail(h, MOCCA_ERR_ARG, "niter must be >= 0");
+    Scf &s = h->scf;
+    if (s.erow_cap < niter + 1) {
+        if (s.erows) {
+            HIPCHK(h, hipStreamSynchronize(h->st));
+            (void)hipFree(s.erows);
+            s.erows = nullptr;
+        }
+        if (int rc = dalloc(h, &s.erows, (size_t)(niter + 1) * 43 * sizeof(double))) return rc;
+        s.erow_cap = niter + 1;
+    }
+    HIPCHK(h, hipMemcpyAsync(s.erows, s.ecur, 43 * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->st));
+    int rc;
+    for (int k = 1; k <= niter; ++k) {
+        double *prev = s.erows + (size_t)(k - 1) * 43, *row = s.erows + (size_t)k * 43;
+        if (estimate) {
+            double out[6];
+            if ((rc = estimate_any(h, first_iteration + k - 1, dt, momentum, out))) return rc;
+            dt = out[0];
+            dth = dt / kHbar;
+            momentum = out[1];
+        }
+        if (params) {
+            params[2 * (k - 1)] = dt;
+            params[2 * (k - 1) + 1] = momentum;
+        }
+        if ((rc = grad_desc_any(h, dth, momentum))) return rc;
+        // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
+        // previous row's densities came from: that row's kinetic energy
+        if ((rc = energy_kinetic_impl(h, prev))) return rc;
+        if ((rc = fill_impl(h))) return rc;
+        if ((rc = density_any(h))) return rc;
+        if ((rc = mix_impl(h, 1))) return rc;
+        if ((rc = construct_impl(h))) return rc;
+        if ((rc = energy_terms_impl(h, row))) return rc;
+    }
+    // kinetic energy of the final states: one hPsi-type pass with no outputs
+    double *last = s.erows + (size_t)niter * 43;
+    if ((rc = kinetic_any(h))) return rc;
+    if ((rc = energy_kinetic_impl(h, last))) return rc;
+    HIPCHK(h, hipMemcpyAsync(s.ecur, last, 43 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    if (energies) {
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(energies, s.erows, (size_t)(niter + 1) * 43 * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    return check_gs_status(h);
+}
+
+
 }  // namespace
 
 namespace mocca {
@@ -551,46 +610,37 @@ int mocca_scf_start(mocca_solve *h) {
 
 int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
                   double *energies) {
-    if (int rc = check_scf(h)) return rc;
-    if (!h->scf.started) return fail(h, MOCCA_ERR_STATE, "mocca_scf_start not called");
-    if (niter < 0) return fail(h, MOCCA_ERR_ARG, "niter must be >= 0");
-    Scf &s = h->scf;
-    if (s.erow_cap < niter + 1) {
-        if (s.erows) {
-            HIPCHK(h, hipStreamSynchronize(h->st));
-            (void)hipFree(s.erows);
-            s.erows = nullptr;
-        }
-        if (int rc = dalloc(h, &s.erows, (size_t)(niter + 1) * 43 * sizeof(double))) return rc;
-        s.erow_cap = niter + 1;
-    }
-    HIPCHK(h, hipMemcpyAsync(s.erows, s.ecur, 43 * sizeof(double), hipMemcpyDeviceToDevice,
-                             h->st));
-    int rc;
-    for (int k = 1; k <= niter; ++k) {
-        double *prev = s.erows + (size_t)(k - 1) * 43, *row = s.erows + (size_t)k * 43;
-        if ((rc = grad_desc_any(h, dt_over_hbar, momentum))) return rc;
-        // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
-        // previous row's densities came from: that row's kinetic energy
-        if ((rc = energy_kinetic_impl(h, prev))) return rc;
-        if ((rc = fill_impl(h))) return rc;
-        if ((rc = density_any(h))) return rc;
-        if ((rc = mix_impl(h, 1))) return rc;
-        if ((rc = construct_impl(h))) return rc;
-        if ((rc = energy_terms_impl(h, row))) return rc;
-    }
-    // kinetic energy of the final states: one hPsi-type pass with no outputs
-    double *last = s.erows + (size_t)niter * 43;
-    if ((rc = kinetic_any(h))) return rc;
-    if ((rc = energy_kinetic_impl(h, last))) return rc;
-    HIPCHK(h, hipMemcpyAsync(s.ecur, last, 43 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
-    if (energies) {
-        HIPCHK(h, hipStreamSynchronize(h->st));
-        HIPCHK(h, hipMemcpy(energies, s.erows, (size_t)(niter + 1) * 43 * sizeof(double),
-                            hipMemcpyDeviceToHost));
-    }
-    return check_gs_status(h);
+    return iterate_impl(h, niter, 1, dt_over_hbar * kHbar, dt_over_hbar, momentum, false,
+                        energies, nullptr);
 }
+
+int mocca_iterate_estimated(mocca_solve *h, int first_iteration, int niter, double dt,
+                            double momentum, double *energies, double *params) {
+    if (first_iteration < 1) return fail(h, MOCCA_ERR_ARG, "first_iteration must be >= 1");
+    return iterate_impl(h, niter, first_iteration, dt, dt / kHbar, momentum, true, energies,
+                        params);
+}
+
+int mocca_estimate_parameters(mocca_solve *h, int iteration, double dt, double momentum,
+                              double *out) {
+    if (int rc = check_ready(h)) return rc;
+    if (!h->fields_set) return fail(h, MOCCA_ERR_STATE, "no potentials set");
+    if (!out) return fail(h, MOCCA_ERR_ARG, "null output");
+    if (iteration < 1) return fail(h, MOCCA_ERR_ARG, "iteration must be >= 1");
+    return estimate_any(h, iteration, dt, momentum, out);
+}
+
+int mocca_estimation_seed(mocca_solve *h, const double *grid) {
+    if (int rc = check_ready(h)) return rc;
+    if (!grid) {
+        h->est.seed.clear();
+        return 0;
+    }
+    h->est.seed.assign(grid, grid + 4 * (size_t)h->N);
+    return 0;
+}
+
+
 
 int mocca_get_moments(mocca_solve *h, int *lm, double *q, int *count) {
     if (int rc = check_scf(h)) return rc;

@@ -175,6 +175,26 @@ module mocca_amd_c
       real(c_double), value :: dt_over_hbar, momentum
       real(c_double), intent(out) :: energies(*)
     end function
+    integer(c_int) function mocca_estimate_parameters(h, iteration, dt, momentum, out) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: iteration
+      real(c_double), value :: dt, momentum
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function mocca_estimation_seed(h, grid) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: grid(*)
+    end function
+    integer(c_int) function mocca_iterate_estimated(h, first_iteration, niter, dt, momentum,   &
+    &                                               energies, params) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: first_iteration, niter
+      real(c_double), value :: dt, momentum
+      real(c_double), intent(out) :: energies(*), params(*)
+    end function
     integer(c_int) function mocca_get_moments(h, lm, q, count) bind(C)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

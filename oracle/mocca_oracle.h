@@ -161,6 +161,17 @@ void or_mix(double *d, const double *h, size_t count, double a);
 void or_hf_fill(const double *energy, const int *qn, int M, double neutrons,
                 double protons, int trc, double *occ);
 
+/* IterativeEstimation for HF (ImaginaryTime.f90:28-211) on maxspwf psi[4][N]
+ * (normalised first when start != 0), quantum numbers qn_est = iso, P, S,
+ * TS; out[6] = { dt, momentum, maxE, relE, con, iterations }. */
+void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int lag,
+                 const double *lagx, const double *lagy, const double *lagz,
+                 const int *qn_est, int trc, double kin, const double *U, const double *B,
+                 const double *NablaB, const double *W, const double *DivW,
+                 const double *S, const double *A, const double *DivA, double dt,
+                 double momentum, double dv, double hbar, int M, const int *qn,
+                 const double *occ, const double *energy, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -491,3 +491,82 @@ void or_hf_fill(const double *energy, const int *qn, int M, double neutrons,
         for (int i = 0; i < M; ++i) occ[i] = 2.0 * occ[i];
     free(ord);
 }
+
+/* ------------------------------------------------------------------------ */
+/* IterativeEstimation (ImaginaryTime.f90:28-211).  psi[4][N] is maxspwf;
+ * when start != 0 it is first normalised as the reference does after
+ * random_number (:63-65).  Then up to 500 power iterations
+ * (:73-100): h = hPsi, maxE = <psi|h>, con = previous maxE - maxE (the
+ * first iteration of a call never converges: the reference's previous maxE
+ * is an undefined local there), u = dt/hbar (h - maxE psi) + momentum u
+ * with u = 0 on entry, psi += u, psi /= sqrt(<psi|psi>), exit at
+ * |con| < 1e-2.  relE for HF (:109-128), minh (:189-195), then
+ * out[6] = { dt, momentum, maxE, relE, con, iterations } (:199-208). */
+static double dotdv(const double *a, const double *b, size_t n, double dv)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s = s + a[i] * b[i];
+    return s * dv;
+}
+
+void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int lag,
+                 const double *lagx, const double *lagy, const double *lagz,
+                 const int *qn_est, int trc, double kin, const double *U, const double *B,
+                 const double *NablaB, const double *W, const double *DivW,
+                 const double *S, const double *A, const double *DivA, double dt,
+                 double momentum, double dv, double hbar, int M, const int *qn,
+                 const double *occ, const double *energy, double *out)
+{
+    const int N = nx * ny * nz;
+    const size_t n4 = (size_t)4 * N;
+    double *der = malloc(sizeof(double) * 3 * n4), *lap = malloc(sizeof(double) * n4);
+    double *h = malloc(sizeof(double) * n4), *u = calloc(n4, sizeof(double));
+    const int it = (qn_est[0] + 3) / 2 - 1;
+    if (start) {
+        const double sc = 1.0 / sqrt(dotdv(psi, psi, n4, dv));
+        for (size_t k = 0; k < n4; ++k) psi[k] = sc * psi[k];
+    }
+    double maxE = 0.0, con = 1e300;
+    int iter;
+    for (iter = 1; iter <= 500; ++iter) {
+        or_derive_spinor(psi, der, lap, nx, ny, nz, dx, qn_est[1], qn_est[2], qn_est[3], lag,
+                         lagx, lagy, lagz);
+        or_hpsi(psi, der, lap, N, it, trc, kin, U, B, NablaB, W, DivW, S, A, DivA, h);
+        const double prev = maxE;
+        maxE = dotdv(psi, h, n4, dv);
+        con = iter == 1 ? 1e300 : prev - maxE;
+        const double f = dt / hbar;
+        for (size_t k = 0; k < n4; ++k) {
+            const double d = h[k] - maxE * psi[k];
+            u[k] = f * d + momentum * u[k];
+            psi[k] = psi[k] + u[k];
+        }
+        const double sc = 1.0 / sqrt(dotdv(psi, psi, n4, dv));
+        for (size_t k = 0; k < n4; ++k) psi[k] = sc * psi[k];
+        if (fabs(con) < 1e-2) break;
+    }
+    if (iter > 500) iter = 500;
+    double relE = 10000.0, minh = 10000.0;
+    for (int i = 0; i < M; ++i) {
+        if (fabs(occ[i]) < 0.5) continue;
+        for (int ii = 0; ii < M; ++ii) {
+            if (fabs(occ[ii]) > 0.5) continue;
+            if (qn[4 * ii] != qn[4 * i]) continue;
+            const double c = -energy[i] + energy[ii];
+            if (c > 0.0 && c < relE) relE = c;
+        }
+    }
+    if (relE < 0.1) relE = 0.1;
+    for (int i = 0; i < M; ++i)
+        if (energy[i] < minh) minh = energy[i];
+    const double mE = maxE - minh;
+    const double kappa = relE / mE;
+    const double r = (sqrt(kappa) - 1) / (sqrt(kappa) + 1);
+    out[0] = 4.0 / (mE + relE + 2 * sqrt(mE * relE)) * hbar * (double)0.90f;
+    out[1] = r * r;
+    out[2] = maxE;
+    out[3] = relE;
+    out[4] = con;
+    out[5] = iter;
+    free(der); free(lap); free(h); free(u);
+}

@@ -19,8 +19,11 @@
 ! Tasks (comma separated): derive, hpsi, dens, step, force (synthetic mean
 ! fields from the input file), pot (densities -> moments -> Coulomb ->
 ! potentials -> energy, reference routines), scf (the Evolve loop of
-! Main.f90:255-328 for HF with fixed dt/momentum; iteration count = niter
-! from the 4th argument).
+! Main.f90:255-328 for HF; iteration count = niter from the 4th argument).
+! With ParameterEstimation = 1 the scf task reads maxspwf's start grid from
+! seed.bin (real8 grid(nx,ny,nz,4)) in place of IterativeEstimation's
+! random_number start, and writes dt, momentum after each GradDesc to
+! <prefix>_params.bin.
 !-------------------------------------------------------------------------------
 program ref_driver
   use CompilationInfo
@@ -43,7 +46,7 @@ program ref_driver
   implicit none
 
   character(len=512) :: infile, prefix, tasks, arg4
-  integer            :: nwtf, i, m, it, uu, n, niter
+  integer            :: nwtf, i, m, it, uu, n, niter, up
   integer, allocatable :: qn(:,:)
   real(KIND=dp), allocatable :: occ(:), grids(:,:,:,:,:)
   type(Spinor)       :: sp, h
@@ -169,6 +172,21 @@ program ref_driver
     ! Start-up and loop of Evolve (Main.f90:172-225, 255-328) for HF without
     ! constraints or cranking, ParameterEstimation = 0.
     EvolveSpwf => GradDesc
+    if(ParameterEstimation.eq.1) then
+      ! maxspwf as the first call of IterativeEstimation makes it
+      ! (ImaginaryTime.f90:58-70), from the seed grid instead of random_number;
+      ! the loop below then passes iteration numbers >= 2 so that it is kept.
+      maxspwf = CopyWaveFunction(HFBasis(nwt))
+      open(newunit=up, file='seed.bin', access='stream', form='unformatted',   &
+      &    status='old')
+      read(up) maxspwf%value%grid
+      close(up)
+      call maxspwf%compnorm()
+      maxspwf%value = 1.0/sqrt(maxspwf%norm) * maxspwf%value
+      call maxspwf%compder()
+      open(newunit=up, file=trim(prefix)//'_params.bin', access='stream',      &
+      &    form='unformatted', status='replace')
+    endif
     call SolvePairing
     call DeriveAll
     call UpdateDensities(0)
@@ -182,7 +200,8 @@ program ref_driver
     write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,        &
     &        sum(CoMCorrection), Skyrmeterms
     do it=1,niter
-      call EvolveSpwf(it)
+      call EvolveSpwf(max(it, 2))
+      if(ParameterEstimation.eq.1) write(up) dt, momentum
       call SolvePairing
       call DeriveAll
       call UpdateDensities(0)
@@ -195,6 +214,7 @@ program ref_driver
       &        sum(CoMCorrection), Skyrmeterms
     enddo
     close(uu)
+    if(ParameterEstimation.eq.1) close(up)
     open(newunit=uu, file=trim(prefix)//'_scfstates.bin', access='stream',      &
     &    form='unformatted', status='replace')
     do i=1,nwt

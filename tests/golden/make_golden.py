@@ -113,13 +113,13 @@ CASES = {
 }
 
 
-def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0):
+def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0, pe=0):
     fd = "-1" if lag else "3"
     fdl = "-1" if lag else "4"
     return f"""&GenInfo
 TimeReversal={1 if trc else 0}, neutrons={neutrons}, protons={protons}
 nx={nx}, ny={ny}, nz={nz}, dx={dx}
-MaxIter=2, ParameterEstimation=0, dt={dt}, momentum={mom}
+MaxIter=2, ParameterEstimation={pe}, dt={dt}, momentum={mom}
 /
 &SpwfStorage
 nwt={nwt}
@@ -284,15 +284,19 @@ def read_pot(wd, prefix, nx, ny, nz, trc):
 
 
 SCF_CASES = {
-    # name: (n, dx, trc, lag, neutrons, protons, nwt_per_isospin, niter)
-    "scf_trc_fd": (10, 0.8, True, False, 8, 8, 6, 12),
-    "scf_trc_lag": (10, 0.8, True, True, 8, 8, 6, 12),
-    "scf_todd_fd": (8, 0.9, False, False, 4, 4, 6, 8),
+    # name: (n, dx, trc, lag, neutrons, protons, nwt_per_isospin, niter,
+    #        ParameterEstimation)
+    "scf_trc_fd": (10, 0.8, True, False, 8, 8, 6, 12, 0),
+    "scf_trc_lag": (10, 0.8, True, True, 8, 8, 6, 12, 0),
+    "scf_todd_fd": (8, 0.9, False, False, 4, 4, 6, 8, 0),
+    # IterativeEstimation before every step; maxspwf starts from seed.bin
+    "scfest_trc_fd": (10, 0.8, True, False, 8, 8, 6, 10, 1),
+    "scfest_todd_lag": (8, 0.9, False, True, 4, 4, 6, 8, 1),
 }
 
 
 def run_scf(name):
-    n, dx, trc, lag, neutrons, protons, per_iso, niter = SCF_CASES[name]
+    n, dx, trc, lag, neutrons, protons, per_iso, niter, pe = SCF_CASES[name]
     seed = sum(map(ord, name))
     rng = np.random.default_rng(seed)
     pattern = [(1, 1), (-1, 1), (1, 1), (-1, 1), (1, 1), (-1, 1)] if trc else \
@@ -322,12 +326,21 @@ def run_scf(name):
         np.array(qns, "<i4").tofile(fh)
         np.array(occ, "<f8").tofile(fh)
         psi.astype("<f8").tofile(fh)
-    nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom)
+    extra = {}
+    if pe:
+        # stands in for random_number in IterativeEstimation's first call
+        seed_grid = rng.uniform(0.0, 1.0, size=(4, n, n, n))
+        seed_grid.astype("<f8").tofile(os.path.join(wd, "seed.bin"))
+        extra["est_seed"] = seed_grid
+    nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom, pe=pe)
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
     with open(os.path.join(wd, "input.nml")) as fin:
         subprocess.run([DRIVER, "input.bin", "out", "force,scf", str(niter)], cwd=wd,
                        stdin=fin, check=True, stdout=subprocess.DEVNULL)
+    if pe:
+        extra["params"] = np.fromfile(os.path.join(wd, "out_params.bin"),
+                                      "<f8").reshape(niter, 2)
     traj = np.fromfile(os.path.join(wd, "out_scf.bin"), "<f8").reshape(niter + 1, 41)
     st = np.fromfile(os.path.join(wd, "out_scfstates.bin"), "<f8")
     N = n ** 3
@@ -341,11 +354,12 @@ def run_scf(name):
     meta = dict(nx=n, ny=n, nz=n, dx=dx, trc=trc, lagrange=lag, nwt=nwt, dt=dt,
                 momentum=mom, neutrons=neutrons, protons=protons, niter=niter,
                 com1body=com1, seed=seed, mixing="scheme 0, DampingParam 0.75",
+                parameter_estimation=pe,
                 source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")
     np.savez_compressed(os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
                         qn=np.array(qns, np.int32), psi=psi, traj=traj,
                         final_psi=fin_psi, final_eo=fin_eo, coef=coef, consts=rest,
-                        **{"final_" + k: v for k, v in f.items()})
+                        **extra, **{"final_" + k: v for k, v in f.items()})
     print(name, "written", traj[:, 0])
 
 

@@ -281,6 +281,23 @@ DENSITY_KEYS = ("rho", "tau", "nablaJ", "derrho", "laprho")
 DENSITY_KEYS_ODD = ("vecj", "rotvecj", "vecs", "rots")
 
 
+def estimate(mesh, psi, start, qn_est, trc, kin, F, dt, mom, dv, hbar, qn, occ, energy):
+    """IterativeEstimation on maxspwf psi (updated in place); returns
+    (dt, momentum, maxE, relE, con, iterations)."""
+    out = np.zeros(6)
+    M = len(qn)
+    lib().or_estimate(P(psi), int(start), mesh.nx, mesh.ny, mesh.nz, ctypes.c_double(mesh.dx),
+                      mesh.lag, *mesh.lagmats(),
+                      P(np.ascontiguousarray(qn_est, dtype=np.int32)), int(trc),
+                      ctypes.c_double(kin), P(F["U"]), P(F["B"]), P(F["NablaB"]),
+                      P(F["W"]), P(F["DivW"]), P(F["S"]), P(F["A"]), P(F["DivA"]),
+                      ctypes.c_double(dt), ctypes.c_double(mom), ctypes.c_double(dv),
+                      ctypes.c_double(hbar), M, P(np.ascontiguousarray(qn, dtype=np.int32)),
+                      P(np.ascontiguousarray(occ, dtype=np.float64)),
+                      P(np.ascontiguousarray(energy, dtype=np.float64)), P(out))
+    return out
+
+
 def all_densities(mesh, psi, qn, occ, trc):
     """ComputeDensity + the derived fields, and each state's derivatives."""
     M, N = psi.shape[0], mesh.N
@@ -327,11 +344,26 @@ def scf_run(g, niter):
                kinetic_sums(psi, lap, qn, occ, dv))
     traj = [rows(E)]
     upd = np.zeros_like(psi)
-    dt_over_hbar = m["dt"] / consts[5]
-    for _ in range(niter):
-        Ff = {k: np.ascontiguousarray(v).reshape(-1) for k, v in F.items()}
+    hbar = consts[5]
+    dt, mom = m["dt"], m["momentum"]
+    dt_over_hbar = dt / hbar
+    est = m.get("parameter_estimation", 0) == 1
+    if est:
+        est_psi = np.ascontiguousarray(g["est_seed"], dtype=np.float64).reshape(-1).copy()
+        qn_est = qn[M - 1]
+    e_sp = np.zeros(M)
+    params = []
+    for k in range(niter):
+        Ff = {key: np.ascontiguousarray(v).reshape(-1) for key, v in F.items()}
+        if est:
+            it_est = (qn_est[0] + 3) // 2 - 1
+            r = estimate(mesh, est_psi, k == 0, qn_est, trc, kin2[it_est], Ff, dt, mom, dv,
+                         hbar, qn, occ, e_sp)
+            dt, mom = r[0], r[1]
+            dt_over_hbar = dt / hbar
+            params.append((dt, mom))
         psi, upd, e_sp, _ = graddesc(psi, der, lap, upd, qn, trc, kin2, Ff, dt_over_hbar,
-                                      m["momentum"], dv)
+                                      mom, dv)
         occ = hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
         old = d
         d, der, lap = all_densities(mesh, psi, qn, occ, trc)
@@ -343,4 +375,4 @@ def scf_run(g, niter):
                    kinetic_sums(psi, lap, qn, occ, dv))
         traj.append(rows(E))
     return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
-                vcoul=vc, vcoulx=vx, cpot=coul.cpot)
+                vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params))

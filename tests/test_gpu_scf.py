@@ -152,6 +152,28 @@ def test_iterate_trajectory_vs_reference(case):
     s.close()
 
 
+@pytest.mark.parametrize("case", ["scfest_trc_fd", "scfest_todd_lag"])
+def test_iterate_estimated_vs_reference(case):
+    """ParameterEstimation = 1: IterativeEstimation before every GradDesc on
+    the device (its first call from the seed the reference run used in place
+    of random_number); dt, momentum and energies of every iteration."""
+    g = golden(case)
+    m = g["meta"]
+    n = m["niter"]
+    print(f"scf loop with estimation: case {case} niter {n}", flush=True)
+    s = _solve(g, np.zeros(m["nwt"]))
+    s.estimation_seed(g["est_seed"])
+    s.scf_start()
+    rows, prm = s.iterate_estimated(n, m["dt"], m["momentum"])
+    assert relerr(prm, g["params"]) < 1e-10
+    traj = _traj_cols(rows)
+    assert np.max(np.abs(traj[:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(traj, g["traj"]) < 1e-10
+    psi = s.download_states()
+    assert relerr(psi.reshape(-1), g["final_psi"].reshape(-1)) < 1e-9
+    s.close()
+
+
 @pytest.mark.parametrize("case", ["scf_trc_fd", "scf_todd_fd"])
 def test_entry_points_match_fused_loop(case):
     """The reference's call sequence through the separate entry points gives

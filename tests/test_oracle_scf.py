@@ -96,7 +96,67 @@ def test_scf_trajectory_vs_reference(case):
     assert g["traj"][-1, 0] < g["traj"][0, 0]
 
 
+@pytest.mark.parametrize("case", ["scfest_trc_fd", "scfest_todd_lag"])
+def test_scf_estimation_trajectory_vs_reference(case):
+    """ParameterEstimation = 1: IterativeEstimation before every GradDesc
+    (its first call started from the seed the reference run used).  The dt
+    and momentum of every iteration and the trajectory match the reference."""
+    g = golden(case)
+    r = ol.scf_run(g, g["meta"]["niter"])
+    assert relerr(r["params"], g["params"]) < 1e-11
+    assert np.max(np.abs(r["traj"][:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(r["traj"], g["traj"]) < 1e-11
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-10
+    # dt grows from the first estimate (relE = 10^4 MeV while every energy is
+    # still 0) and the momentum stays in (0, 1)
+    assert g["params"][0, 0] < g["params"][1, 0]
+    assert np.all((g["params"][:, 1] > 0) & (g["params"][:, 1] < 1))
+
+
 # ---- checks fixed by the mathematics ---------------------------------------
+
+def test_estimation_finds_largest_eigenvalue():
+    """The power iteration of IterativeEstimation converges to the largest
+    eigenvalue of h, from below (a Rayleigh quotient): the dense matrix of h
+    on a 5^3 octant is built column by column and diagonalised.  dt and the
+    momentum are the heavy-ball optimum for the spectrum [relE, maxE - minh]:
+    dt/hbar (sqrt(maxE') + sqrt(relE))^2 = 4 * 0.9, momentum =
+    ((1 - sqrt(k)) / (1 + sqrt(k)))^2 with k = relE / maxE'."""
+    n, dx = 5, 0.8
+    mesh = ol.Mesh(n, n, n, dx, 0)
+    N = n ** 3
+    x = (np.arange(n) + 0.5) * dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    f = 1 / (1 + np.exp((r - 3) / 0.6))
+    df = -f * (1 - f) / 0.6
+    grad = [df * X / r, df * Y / r, df * Z / r]
+    F = dict(U=np.stack([-50 * f, -45 * f]).reshape(-1), B=np.stack([10 * f, 11 * f]).reshape(-1),
+             NablaB=np.stack([[10 * g for g in grad], [11 * g for g in grad]]).reshape(-1))
+    for k, cnt in (("W", 18), ("DivW", 6), ("S", 6), ("A", 6), ("DivA", 2)):
+        F[k] = np.zeros(cnt * N)
+    kin, hbar, dv = 20.7355191, 6.58211928, 8 * dx ** 3
+    qe = np.array([-1, 1, 1, 1], np.int32)
+    H = np.zeros((4 * N, 4 * N))
+    for j in range(4 * N):
+        e = np.zeros(4 * N)
+        e[j] = 1
+        der, lap = mesh.derive_spinor(e, 1, 1, 1)
+        H[:, j] = ol.hpsi(e, der, lap, 0, 1, kin, F, N).reshape(-1)
+    lmax = np.linalg.eigvals(H).real.max()
+    psi = np.random.default_rng(3).uniform(0, 1, 4 * N)
+    qn = np.array([[-1, 1, 1, 1], [-1, 1, 1, 1], [1, 1, 1, 1]], np.int32)
+    energy = np.array([-30.0, -10.0, -40.0])      # relE 20 (neutrons), minh -40
+    dt, mom, maxE, relE, con, iters = ol.estimate(mesh, psi, 1, qe, 1, kin, F, 0.01, 0.3, dv,
+                                                  hbar, qn, np.array([1.0, 0.0, 1.0]), energy)
+    assert maxE <= lmax * (1 + 1e-12) and lmax - maxE < 0.05
+    assert abs(con) < 1e-2 and 1 < iters < 500
+    assert relE == 20.0
+    me = maxE + 40.0
+    assert abs(dt / hbar * (np.sqrt(me) + np.sqrt(relE)) ** 2 - 4 * np.float32(0.9)) < 1e-12
+    k = relE / me
+    assert abs(mom - ((1 - np.sqrt(k)) / (1 + np.sqrt(k))) ** 2) < 1e-14
+    assert abs(np.sum(psi * psi) * dv - 1) < 1e-12
 
 @pytest.mark.parametrize("l,m", [(0, 0), (2, 0), (2, 2), (4, 0), (4, 2), (4, 4),
                                  (6, 4), (8, 6), (10, 10)])
