@@ -158,6 +158,13 @@ typedef struct mocca_functional {
     int coulomb_maxit;    /* CoulombMax (500), for coulomb = 1 */
     int max_moment;       /* MaxMoment = MaxLCoul (10, even, <= 12) */
     double damping;       /* DampingParam of MixingScheme 0 (0.75) */
+    int mixing_scheme;    /* MixingScheme (Densities.f90:27): 0 = linear density mixing
+                             with damping; 2 = no density mixing, U and S (without
+                             time reversal) preconditioned in ConstructPotentials
+                             (PotentialMixing.f90:143-171) */
+    int pad_;
+    double preconu;       /* preconu (Densities.f90:34, 1.0): the preconditioner is
+                             (1 - preconu Lap)^-1 */
 } mocca_functional;
 
 /* Set the functional, the kinetic coefficients hbm/2 * reduced-mass factor

@@ -61,7 +61,9 @@ class FunctionalC(ctypes.Structure):
                 ("neutrons", ctypes.c_double), ("protons", ctypes.c_double),
                 ("com1body", ctypes.c_int), ("cexchange", ctypes.c_int),
                 ("coulomb", ctypes.c_int), ("coulomb_maxit", ctypes.c_int),
-                ("max_moment", ctypes.c_int), ("damping", ctypes.c_double)]
+                ("max_moment", ctypes.c_int), ("damping", ctypes.c_double),
+                ("mixing_scheme", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("preconu", ctypes.c_double)]
 
 
 # Energy row layout of mocca_comp_energy / mocca_iterate.
@@ -223,12 +225,13 @@ class Solve:
 
     # Self-consistent part -------------------------------------------------
     def scf_setup(self, B, hbm, mass, e2, neutrons, protons, com1body=2, cexchange=2,
-                  coulomb=1, coulomb_maxit=500, max_moment=10, damping=0.75):
+                  coulomb=1, coulomb_maxit=500, max_moment=10, damping=0.75, mixing_scheme=0,
+                  preconu=1.0):
         f = FunctionalC((ctypes.c_double * 28)(*[float(x) for x in B]),
                         (ctypes.c_double * 2)(*hbm), (ctypes.c_double * 2)(*mass),
                         float(e2), float(neutrons), float(protons), int(com1body),
                         int(cexchange), int(coulomb), int(coulomb_maxit), int(max_moment),
-                        float(damping))
+                        float(damping), int(mixing_scheme), 0, float(preconu))
         self._chk(self.lib.mocca_scf_setup(self.h, ctypes.byref(f)))
 
     def hf_fill(self):

@@ -75,6 +75,15 @@ struct Scf {
     double *ecur = nullptr;        // [43] energy of the current state
     double *erows = nullptr;       // [erow_cap][43]
     int erow_cap = 0;
+    // MixingScheme 2 (potential preconditioning)
+    int mixing_scheme = 0;
+    double preconu = 1.0;
+    bool pot_built = false;        // a ConstructPotentials has run on the handle
+    double *pold = nullptr;        // [8N] U(2), S(2x3) before the update
+    double *pcwork = nullptr;      // [npc][4N] CG vectors
+    PrecJob *pcjobs = nullptr;
+    int *pc_iters = nullptr;       // [npc]
+    int npc = 0;
 };
 
 // The extra state of IterativeEstimation (maxspwf) and its records.

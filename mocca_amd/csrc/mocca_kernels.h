@@ -49,6 +49,15 @@ struct FieldJob {
     int pad;
 };
 
+// One MixingScheme 2 solve: pot <- old + (beta + alpha Lap)^-1 (pot - old),
+// Laplacian ghost signs sign[axis].
+struct PrecJob {
+    double *pot;
+    const double *old;
+    int sign[3];
+    int pad;
+};
+
 enum HpsiMode { kWriteH = 1, kStep = 2 };
 
 // Field buffer offsets, in units of N (one contiguous buffer per handle):
@@ -202,6 +211,9 @@ void launch_energy_terms(const SkyrmeK &f, long N, int trc, const double *dens,
 void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ, const int *qn,
                            int M, double *e, hipStream_t st);
 constexpr int kEnergySlots = 35;   // partial sums per workgroup of the energy pass
+
+void launch_precondition(const MeshK<double> &m, const PrecJob *jobs, int njobs, double *work,
+                         double alpha, double beta, double dv, int *iters, hipStream_t st);
 
 // IterativeEstimation (power iteration for dt and momentum).
 constexpr int kEstSlots = 8;

@@ -468,6 +468,97 @@ void launch_field_jobs(const MeshK<double> &m, const FieldJob *jobs, int njobs,
 }
 
 // ---------------------------------------------------------------------------
+// PreconditionPotential (Damping.f90:158-198) with preconoperator
+// (:200-228), the MixingScheme 2 step of ConstructPotentials
+// (PotentialMixing.f90:143-171): for each job, du = pot - old, then the
+// reference's conjugate gradients on (beta + alpha Lap) x = du from x = 0
+// (at most 300 iterations, ending after the direction update once
+// dv |r|^2 < 1e-8), and pot = old + x.  Lap is the wavefunction Laplacian
+// (FD or Lagrange, axis_derivs) with the job's ghost signs.  One workgroup
+// per job keeps every reduction inside the workgroup, so the whole solve is
+// one launch with no host round trip; work holds x, r, d, q per job.
+constexpr int kPrecBlock = 1024;
+
+__device__ __forceinline__ double prec_sum(double a, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    __syncthreads();
+    if (l == 0) sh[w] = a;
+    __syncthreads();
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kPrecBlock / 64; ++k) s += sh[k];
+    return s;
+}
+
+__global__ __launch_bounds__(kPrecBlock) void k_precondition(MeshK<double> m,
+                                                            const PrecJob *__restrict__ jobs,
+                                                            double *__restrict__ work,
+                                                            double alpha, double beta,
+                                                            double dv, int *__restrict__ iters) {
+    __shared__ double sh[kPrecBlock / 64];
+    const PrecJob jb = jobs[blockIdx.x];
+    const long N = m.N;
+    double *x = work + (long)blockIdx.x * 4 * N, *r = x + N, *d = r + N, *q = d + N;
+    double rr = 0;
+    for (long p = threadIdx.x; p < N; p += kPrecBlock) {
+        const double v = jb.pot[p] - jb.old[p];
+        x[p] = 0.0;
+        r[p] = v;
+        d[p] = v;
+        rr += v * v;
+    }
+    double newres = prec_sum(rr, sh) * dv;
+    int iter = 0;
+    // du = 0 exactly (a field that vanishes by symmetry) leaves pot as it
+    // is; the reference divides 0 by 0 there.
+    if (newres > 0.0) {
+        for (iter = 1; iter <= 300; ++iter) {
+            double dq = 0;
+            for (long p = threadIdx.x; p < N; p += kPrecBlock) {
+                const int xx = p % m.nx, yy = (p / m.nx) % m.ny, zz = p / ((long)m.nx * m.ny);
+                double lap = 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    double d1, d2;
+                    axis_derivs(m, (const double *)d, xx, yy, zz, a, jb.sign[a], jb.sign[a], d1,
+                                d2, true);
+                    lap += d2;
+                }
+                const double qv = beta * d[p] + alpha * lap;
+                q[p] = qv;
+                dq += d[p] * qv;
+            }
+            const double a = newres / (prec_sum(dq, sh) * dv);
+            double r2 = 0;
+            for (long p = threadIdx.x; p < N; p += kPrecBlock) {
+                x[p] = x[p] + a * d[p];
+                const double rv = r[p] - a * q[p];
+                r[p] = rv;
+                r2 += rv * rv;
+            }
+            const double oldres = newres;
+            newres = prec_sum(r2, sh) * dv;
+            const double b = newres / oldres;
+            for (long p = threadIdx.x; p < N; p += kPrecBlock) d[p] = r[p] + b * d[p];
+            __syncthreads();
+            if (newres < 1e-8) break;
+        }
+        if (iter > 300) iter = 300;
+        for (long p = threadIdx.x; p < N; p += kPrecBlock) jb.pot[p] = jb.old[p] + x[p];
+    }
+    if (threadIdx.x == 0 && iters) iters[blockIdx.x] = iter;
+}
+
+void launch_precondition(const MeshK<double> &m, const PrecJob *jobs, int njobs, double *work,
+                         double alpha, double beta, double dv, int *iters, hipStream_t st) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_precondition, dim3(njobs), dim3(kPrecBlock), 0, st, m, jobs, work,
+                       alpha, beta, dv, iters);
+}
+
+// ---------------------------------------------------------------------------
 // Gram-Schmidt as Cholesky QR (SpwfStorage.f90:372-480).  Gram matrices of
 // each (isospin, parity, signature) block, G = dv * Psi^T Psi, in FP64.
 // Grid: x = K split, y = upper 64x64 tile of G, z = block.

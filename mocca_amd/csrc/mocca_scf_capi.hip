@@ -203,6 +203,17 @@ int construct_impl(mocca_solve *h) {
             return fail(h, MOCCA_ERR_HIP, "cooperative launch of the Coulomb solver failed: %s",
                         hipGetErrorString((hipError_t)herr));
     }
+    // MixingScheme 2 updates U and S from their previous values (UpotOld,
+    // SpotOld, PotentialMixing.f90:114-124), skipped on the first
+    // construction, when those are all zero (:143)
+    const bool precond = s.mixing_scheme == 2 && s.pot_built;
+    if (precond) {
+        HIPCHK(h, hipMemcpyAsync(s.pold, s.pot + kOffU * N, 2 * N * sizeof(double),
+                                 hipMemcpyDeviceToDevice, h->st));
+        if (!h->mesh.time_reversal)
+            HIPCHK(h, hipMemcpyAsync(s.pold + 2 * N, s.pot + kOffS * N, 6 * N * sizeof(double),
+                                     hipMemcpyDeviceToDevice, h->st));
+    }
     rc = run(h, "potentials", [&] {
         launch_potentials(s.sk, N, h->mesh.time_reversal, dens, h->b.derived, s.vcoul, s.vcoulx,
                           s.pot, h->st);
@@ -210,6 +221,14 @@ int construct_impl(mocca_solve *h) {
     if (rc) return rc;
     rc = run(h, "pot_jobs", [&] { launch_field_jobs(h->md, s.pjobs, s.npjobs, h->st); });
     if (rc) return rc;
+    if (precond) {
+        rc = run(h, "precondition", [&] {
+            launch_precondition(h->md, s.pcjobs, s.npc, s.pcwork, -s.preconu, 1.0, h->mesh.dv,
+                                s.pc_iters, h->st);
+        });
+        if (rc) return rc;
+    }
+    s.pot_built = true;
     if (h->prec != MOCCA_FP64) {
         rc = run(h, "pot_convert", [&] {
             launch_convert_f32(s.pot, (float *)h->b.fields, (long)kNumFieldSlots * N, h->st);
@@ -253,6 +272,7 @@ int save_history(mocca_solve *h) {
 int mix_impl(mocca_solve *h, int keep) {
     const long N = h->N;
     Scf &s = h->scf;
+    if (s.mixing_scheme == 2) return 0;   // MixDensities: potentials are mixed instead
     return run(h, "mix", [&] {
         launch_mix(h->b.dens, s.hist, 18 * N, s.damping, keep, h->st);
         launch_mix(h->b.derived, s.hist + 18 * N, 20 * N, s.damping, keep, h->st);
@@ -340,7 +360,7 @@ void scf_free(mocca_solve *h) {
     void *ptrs[] = {s.hist, s.Y, s.mpart, s.Q, s.lfac, s.bidx, s.bsh, s.brp, s.cx, s.cr, s.cpa,
                     s.cpb, s.cap, s.cpart, s.vcoul, s.vcoulx, s.cg_iters, s.pjobs, s.epart,
                     s.ecur, s.erows, s.cq[0], s.cq[1], s.cq[2], s.clam[0], s.clam[1],
-                    s.clam[2], s.cw0, s.cw1};
+                    s.clam[2], s.cw0, s.cw1, s.pold, s.pcwork, s.pcjobs, s.pc_iters};
     for (void *p : ptrs) free_ptr(p);
     if (s.pot && s.pot != (double *)h->b.fields) (void)hipFree(s.pot);
     s = Scf{};
@@ -360,6 +380,12 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
     if (f->max_moment < 0 || f->max_moment > 12 || (f->max_moment & 1))
         return fail(h, MOCCA_ERR_ARG, "max_moment must be even and <= 12 (got %d)",
                     f->max_moment);
+    if (f->mixing_scheme != 0 && f->mixing_scheme != 2)
+        return fail(h, MOCCA_ERR_UNSUPPORTED,
+                    "MixingScheme %d is not supported (0: linear mixing, 2: potential "
+                    "preconditioning)", f->mixing_scheme);
+    if (f->mixing_scheme == 2 && !(f->preconu > 0.0))
+        return fail(h, MOCCA_ERR_ARG, "preconu must be positive");
     if (f->coulomb < 0 || f->coulomb > 2)
         return fail(h, MOCCA_ERR_UNSUPPORTED,
                     "coulomb: 0 none, 1 conjugate gradients, 2 direct (got %d)", f->coulomb);
@@ -385,6 +411,8 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
     k.e2 = f->e2; k.neutrons = f->neutrons; k.protons = f->protons;
     k.com1body = f->com1body; k.cexchange = f->cexchange; k.coulomb = f->coulomb;
     s.damping = f->damping;
+    s.mixing_scheme = f->mixing_scheme;
+    s.preconu = f->preconu;
     // kinetic coefficients hbm/2 (1 - m_q / (N m_n + Z m_p)) with COM1Body = 2
     for (int it = 0; it < 2; ++it) {
         const double red = f->com1body == 2
@@ -541,6 +569,39 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
         SCF_ALLOC(s.pjobs, jobs.size() * sizeof(FieldJob));
         HIPCHK(h, hipMemcpy(s.pjobs, jobs.data(), jobs.size() * sizeof(FieldJob),
                             hipMemcpyHostToDevice));
+    }
+    if (s.mixing_scheme == 2) {
+        // U (P, S, TS, comp 1) and, without time reversal, S_x (P, -S, TS, 1),
+        // S_y (P, -S, TS, 2), S_z (P, S, TS, 1) for P = S = TS = 1
+        // (PotentialMixing.f90:146-170); ghost signs from the component table
+        // of Lapla_EV8 (OptimizedDerivatives.f90:533-542).
+        static const int su[3] = {1, 1, 1};
+        static const int ss[3][3] = {{-1, 1, -1}, {1, -1, -1}, {1, 1, 1}};
+        SCF_ALLOC(s.pold, 8 * (size_t)N * sizeof(double));
+        std::vector<PrecJob> pj;
+        for (int it = 0; it < 2; ++it) {
+            PrecJob j{};
+            j.pot = s.pot + (kOffU + it) * N;
+            j.old = s.pold + it * N;
+            for (int a = 0; a < 3; ++a) j.sign[a] = su[a];
+            pj.push_back(j);
+        }
+        if (!trc)
+            for (int it = 0; it < 2; ++it)
+                for (int c = 0; c < 3; ++c) {
+                    PrecJob j{};
+                    j.pot = s.pot + (kOffS + 3 * it + c) * N;
+                    j.old = s.pold + (2 + 3 * it + c) * N;
+                    for (int a = 0; a < 3; ++a) j.sign[a] = ss[c][a];
+                    pj.push_back(j);
+                }
+        s.npc = (int)pj.size();
+        SCF_ALLOC(s.pcwork, (size_t)s.npc * 4 * N * sizeof(double));
+        SCF_ALLOC(s.pcjobs, pj.size() * sizeof(PrecJob));
+        SCF_ALLOC(s.pc_iters, pj.size() * sizeof(int));
+        HIPCHK(h, hipMemcpy(s.pcjobs, pj.data(), pj.size() * sizeof(PrecJob),
+                            hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemset(s.pc_iters, 0, pj.size() * sizeof(int)));
     }
     s.e_nblk = (int)std::min<long>(512, (N + kBlock - 1) / kBlock);
     SCF_ALLOC(s.epart, (size_t)s.e_nblk * kEnergySlots * sizeof(double));

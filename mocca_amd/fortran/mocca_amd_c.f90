@@ -40,6 +40,8 @@ module mocca_amd_c
     real(c_double) :: e2, neutrons, protons
     integer(c_int) :: com1body, cexchange, coulomb, coulomb_maxit, max_moment
     real(c_double) :: damping
+    integer(c_int) :: mixing_scheme, pad_
+    real(c_double) :: preconu
   end type
 
   interface

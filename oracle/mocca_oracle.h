@@ -172,6 +172,13 @@ void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int 
                  double momentum, double dv, double hbar, int M, const int *qn,
                  const double *occ, const double *energy, double *out);
 
+/* PreconditionPotential (Damping.f90:158-228): per isospin, CG on
+ * (beta + alpha Lap) x = pot[it] (Lap with ghost signs sx, sy, sz), x = 0
+ * start, <= 300 iterations, stop at dv |r|^2 < 1e-8.  x[2][N], iters[2]. */
+void or_precondition(const double *pot, double *x, int nx, int ny, int nz, double dx, int lag,
+                     const double *lagx, const double *lagy, const double *lagz, int sx, int sy,
+                     int sz, double alpha, double beta, double dv, int *iters);
+
 #ifdef __cplusplus
 }
 #endif

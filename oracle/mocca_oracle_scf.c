@@ -570,3 +570,45 @@ void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int 
     out[5] = iter;
     free(der); free(lap); free(h); free(u);
 }
+
+/* ------------------------------------------------------------------------ */
+/* PreconditionPotential (Damping.f90:158-198) with preconoperator
+ * (:200-228): per isospin, conjugate gradients on
+ * (beta + alpha Lap) x = pot[it] from x = 0; Lap is the wavefunction
+ * Laplacian (FD or Lagrange) with ghost signs (sx, sy, sz); at most 300
+ * iterations; the loop ends after the direction update once
+ * dv |r|^2 < 1e-8.  pot[2][N] in, x[2][N] out, iters[2] iterations. */
+void or_precondition(const double *pot, double *x, int nx, int ny, int nz, double dx, int lag,
+                     const double *lagx, const double *lagy, const double *lagz, int sx, int sy,
+                     int sz, double alpha, double beta, double dv, int *iters)
+{
+    const size_t N = (size_t)nx * ny * nz;
+    double *r = malloc(sizeof(double) * N), *d = malloc(sizeof(double) * N);
+    double *q = malloc(sizeof(double) * N);
+    for (int it = 0; it < 2; ++it) {
+        double *xi = x + it * N;
+        for (size_t k = 0; k < N; ++k) {
+            xi[k] = 0.0;
+            r[k] = pot[it * N + k];
+            d[k] = r[k];
+        }
+        double newres = dotdv(d, d, N, dv);
+        int iter;
+        for (iter = 1; iter <= 300; ++iter) {
+            or_laplace_field(d, q, nx, ny, nz, dx, sx, sy, sz, lag, lagx, lagy, lagz);
+            for (size_t k = 0; k < N; ++k) q[k] = beta * d[k] + alpha * q[k];
+            const double a = newres / dotdv(d, q, N, dv);
+            for (size_t k = 0; k < N; ++k) {
+                xi[k] = xi[k] + a * d[k];
+                r[k] = r[k] - a * q[k];
+            }
+            const double oldres = newres;
+            newres = dotdv(r, r, N, dv);
+            const double b = newres / oldres;
+            for (size_t k = 0; k < N; ++k) d[k] = r[k] + b * d[k];
+            if (newres < 1e-8) break;
+        }
+        iters[it] = iter > 300 ? 300 : iter;
+    }
+    free(r); free(d); free(q);
+}

@@ -113,7 +113,8 @@ CASES = {
 }
 
 
-def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0, pe=0):
+def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0, pe=0,
+              preconw=0.0):
     fd = "-1" if lag else "3"
     fdl = "-1" if lag else "4"
     return f"""&GenInfo
@@ -125,7 +126,7 @@ MaxIter=2, ParameterEstimation={pe}, dt={dt}, momentum={mom}
 nwt={nwt}
 /
 &Densit
-MixingScheme={mixing}
+MixingScheme={mixing}, PreConW={preconw}
 /
 &Pairing
 Type="HF"
@@ -292,11 +293,17 @@ SCF_CASES = {
     # IterativeEstimation before every step; maxspwf starts from seed.bin
     "scfest_trc_fd": (10, 0.8, True, False, 8, 8, 6, 10, 1),
     "scfest_todd_lag": (8, 0.9, False, True, 4, 4, 6, 8, 1),
+    # MixingScheme 2: potentials preconditioned by (1 - Lap)^-1, no density
+    # mixing
+    "scfpc_trc_fd": (10, 0.8, True, False, 8, 8, 6, 10, 0, dict(mixing=2)),
+    "scfpc_todd_lag": (8, 0.9, False, True, 4, 4, 6, 8, 0, dict(mixing=2)),
 }
 
 
 def run_scf(name):
-    n, dx, trc, lag, neutrons, protons, per_iso, niter, pe = SCF_CASES[name]
+    n, dx, trc, lag, neutrons, protons, per_iso, niter, pe = SCF_CASES[name][:9]
+    opts = SCF_CASES[name][9] if len(SCF_CASES[name]) > 9 else {}
+    mixing, preconw = opts.get("mixing", 0), opts.get("preconw", 0.0)
     seed = sum(map(ord, name))
     rng = np.random.default_rng(seed)
     pattern = [(1, 1), (-1, 1), (1, 1), (-1, 1), (1, 1), (-1, 1)] if trc else \
@@ -332,7 +339,8 @@ def run_scf(name):
         seed_grid = rng.uniform(0.0, 1.0, size=(4, n, n, n))
         seed_grid.astype("<f8").tofile(os.path.join(wd, "seed.bin"))
         extra["est_seed"] = seed_grid
-    nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom, pe=pe)
+    nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom, pe=pe,
+                    mixing=mixing, preconw=preconw)
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
     with open(os.path.join(wd, "input.nml")) as fin:
@@ -353,8 +361,10 @@ def run_scf(name):
     com1 = int(np.frombuffer(fb[37 * 8:37 * 8 + 4], "<i4")[0])
     meta = dict(nx=n, ny=n, nz=n, dx=dx, trc=trc, lagrange=lag, nwt=nwt, dt=dt,
                 momentum=mom, neutrons=neutrons, protons=protons, niter=niter,
-                com1body=com1, seed=seed, mixing="scheme 0, DampingParam 0.75",
-                parameter_estimation=pe,
+                com1body=com1, seed=seed,
+                mixing=("scheme 0, DampingParam 0.75" if mixing == 0 else
+                        f"scheme {mixing}, PreConW {preconw}"),
+                mixing_scheme=mixing, preconw=preconw, parameter_estimation=pe,
                 source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")
     np.savez_compressed(os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
                         qn=np.array(qns, np.int32), psi=psi, traj=traj,

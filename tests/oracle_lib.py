@@ -298,6 +298,53 @@ def estimate(mesh, psi, start, qn_est, trc, kin, F, dt, mom, dv, hbar, qn, occ, 
     return out
 
 
+def precondition(mesh, pot, signs, dv, alpha=-1.0, beta=1.0):
+    """PreconditionPotential: (beta + alpha Lap)^-1 pot for both isospins by
+    the reference's CG; returns (x [2][N], iterations [2])."""
+    x = np.zeros((2, mesh.N))
+    iters = np.zeros(2, np.int32)
+    lib().or_precondition(P(np.ascontiguousarray(pot, dtype=np.float64).reshape(2, mesh.N)),
+                          P(x), mesh.nx, mesh.ny, mesh.nz, ctypes.c_double(mesh.dx), mesh.lag,
+                          *mesh.lagmats(), int(signs[0]), int(signs[1]), int(signs[2]),
+                          ctypes.c_double(alpha), ctypes.c_double(beta), ctypes.c_double(dv),
+                          P(iters))
+    return x, iters
+
+
+# Ghost signs (x, y, z) of the Laplacian for the potential components that
+# MixingScheme 2 preconditions (PotentialMixing.f90:143-198 with the
+# component table of Lapla_EV8, OptimizedDerivatives.f90:533-542), for
+# P = S = TS = 1: U; S_x, S_y, S_z; W_{mu nu} as [mu][nu].
+PRECON_SIGNS_U = (1, 1, 1)
+PRECON_SIGNS_S = ((-1, 1, -1), (1, -1, -1), (1, 1, 1))
+PRECON_SIGNS_W = (((-1, -1, -1), (1, 1, -1), (1, -1, 1)),
+                  ((1, 1, -1), (-1, -1, -1), (-1, 1, 1)),
+                  ((1, -1, 1), (-1, 1, 1), (-1, -1, -1)))
+
+
+def precondition_potentials(mesh, F, Fold, trc, dv, preconw):
+    """The MixingScheme 2 step of ConstructPotentials (PotentialMixing.f90:
+    143-198): U, S (and W when PreConW > 0) <- old + precond(new - old)."""
+    N = mesh.N
+    U = F["U"].reshape(2, N)
+    x, _ = precondition(mesh, U - Fold["U"].reshape(2, N), PRECON_SIGNS_U, dv)
+    F["U"] = (Fold["U"].reshape(2, N) + x).reshape(F["U"].shape)
+    if not trc:
+        S, So = F["S"].reshape(2, 3, N).copy(), Fold["S"].reshape(2, 3, N)
+        for c in range(3):
+            x, _ = precondition(mesh, S[:, c] - So[:, c], PRECON_SIGNS_S[c], dv)
+            S[:, c] = So[:, c] + x
+        F["S"] = S.reshape(F["S"].shape)
+    if preconw > 0:
+        W, Wo = F["W"].reshape(2, 3, 3, N).copy(), Fold["W"].reshape(2, 3, 3, N)
+        for mu in range(3):
+            for nu in range(3):   # W[it][nu][mu] holds Wpot(:,:,:,mu,nu,it)
+                x, _ = precondition(mesh, W[:, nu, mu] - Wo[:, nu, mu],
+                                    PRECON_SIGNS_W[mu][nu], dv)
+                W[:, nu, mu] = Wo[:, nu, mu] + x
+        F["W"] = W.reshape(F["W"].shape)
+
+
 def all_densities(mesh, psi, qn, occ, trc):
     """ComputeDensity + the derived fields, and each state's derivatives."""
     M, N = psi.shape[0], mesh.N
@@ -337,6 +384,8 @@ def scf_run(g, niter):
         F = potentials(coef, mesh, trc, d, vc, vx)
         return F, vc, vx
 
+    scheme = m.get("mixing_scheme", 0)
+    preconw = m.get("preconw", 0.0)
     occ = hf_fill(np.zeros(M), qn, m["neutrons"], m["protons"], trc)
     d, der, lap = all_densities(mesh, psi, qn, occ, trc)
     F, vc, vx = construct(d)
@@ -367,10 +416,14 @@ def scf_run(g, niter):
         occ = hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
         old = d
         d, der, lap = all_densities(mesh, psi, qn, occ, trc)
-        for k in d:
-            lib().or_mix(P(d[k]), P(np.ascontiguousarray(old[k])), ctypes.c_size_t(d[k].size),
-                         ctypes.c_double(mixing))
+        if scheme == 0:
+            for key in d:
+                lib().or_mix(P(d[key]), P(np.ascontiguousarray(old[key])),
+                             ctypes.c_size_t(d[key].size), ctypes.c_double(mixing))
+        Fold = F
         F, vc, vx = construct(d)
+        if scheme == 2:
+            precondition_potentials(mesh, F, Fold, trc, dv, preconw)
         E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
                    kinetic_sums(psi, lap, qn, occ, dv))
         traj.append(rows(E))

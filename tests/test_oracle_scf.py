@@ -113,7 +113,48 @@ def test_scf_estimation_trajectory_vs_reference(case):
     assert np.all((g["params"][:, 1] > 0) & (g["params"][:, 1] < 1))
 
 
+@pytest.mark.parametrize("case", ["scfpc_trc_fd", "scfpc_todd_lag"])
+def test_scf_potential_preconditioning_vs_reference(case):
+    """MixingScheme 2: no density mixing; U (and S without time reversal)
+    <- old + (1 - Lap)^-1 (new - old) by the reference's CG."""
+    g = golden(case)
+    r = ol.scf_run(g, g["meta"]["niter"])
+    assert np.max(np.abs(r["traj"][:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(r["traj"], g["traj"]) < 1e-11
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-10
+    assert relerr(r["fields"]["U"].reshape(-1), g["final_Upot"].reshape(-1)) < 1e-11
+    if not g["meta"]["trc"]:
+        assert relerr(r["fields"]["S"].reshape(-1), g["final_Spot"].reshape(-1)) < 1e-10
+
+
 # ---- checks fixed by the mathematics ---------------------------------------
+
+@pytest.mark.parametrize("lag", [0, 1])
+def test_precondition_solves_the_screened_equation(lag):
+    """The CG of PreconditionPotential solves (1 - Lap) x = f to its stopping
+    rule dv |r|^2 < 1e-8: checked by applying the operator to x, and against
+    a dense solve of the same operator (assembled column by column)."""
+    n, dx = 5, 0.8
+    mesh = ol.Mesh(n, n, n, dx, lag)
+    N, dv = n ** 3, 8 * dx ** 3
+    rng = np.random.default_rng(5 + lag)
+    f = rng.uniform(-3, 3, (2, N))
+    for signs in [ol.PRECON_SIGNS_U] + list(ol.PRECON_SIGNS_S):
+        x, iters = ol.precondition(mesh, f, signs, dv)
+        assert (iters > 2).all() and (iters < 300).all()
+        A = np.zeros((N, N))
+        for j in range(N):
+            e = np.zeros(N)
+            e[j] = 1.0
+            A[:, j] = e - mesh.laplace_field(e, *signs)
+        for it in range(2):
+            r = f[it] - A @ x[it]
+            assert np.sum(r * r) * dv < 1e-8
+            exact = np.linalg.solve(A, f[it])
+            assert np.max(np.abs(x[it] - exact)) < 1e-3
+        # the screened operator is symmetric positive definite
+        assert np.allclose(A, A.T, atol=1e-10)
+        assert np.linalg.eigvalsh(A).min() > 0.99
 
 def test_estimation_finds_largest_eigenvalue():
     """The power iteration of IterativeEstimation converges to the largest
