@@ -179,6 +179,11 @@ void or_precondition(const double *pot, double *x, int nx, int ny, int nz, doubl
                      const double *lagx, const double *lagy, const double *lagz, int sx, int sy,
                      int sz, double alpha, double beta, double dv, int *iters);
 
+/* RutzCutOff (Moments.f90:2703-2849): density cutoff function of the
+ * multipole constraints, rho[2][N] -> cutoff[2][N]. */
+void or_rutz_cutoff(const double *rho, int nx, int ny, int nz, double dx, double dv,
+                    double radd, double acut, double *cutoff);
+
 #ifdef __cplusplus
 }
 #endif

@@ -612,3 +612,74 @@ void or_precondition(const double *pot, double *x, int nx, int ny, int nz, doubl
     }
     free(r); free(d); free(q);
 }
+
+/* ------------------------------------------------------------------------ */
+/* RutzCutOff (Moments.f90:2703-2849): per isospin, threshold = max(rho)/10;
+ * surface points by linear interpolation between neighbours (i, i+1),
+ * (j, j+1), (k, k+1) of the cells i < nx-1, j < ny-1, k < nz-1 of the
+ * octant whose densities straddle the threshold; DeltaR = -+ distance to
+ * the nearest surface point (minus inside, rho > threshold);
+ * cutoff = 1 / (1 + exp((DeltaR - radd) / acut)).  An isospin with
+ * dv sum rho < 1e-10 gets cutoff 0.  Mesh points at (i + 1/2) dx. */
+void or_rutz_cutoff(const double *rho, int nx, int ny, int nz, double dx, double dv,
+                    double radd, double acut, double *cutoff)
+{
+    const size_t N = (size_t)nx * ny * nz;
+    double *surf = malloc(sizeof(double) * 3 * 3 * N);
+    for (int it = 0; it < 2; ++it) {
+        const double *r = rho + it * N;
+        double *cut = cutoff + it * N;
+        double tot = 0.0, mx = r[0];
+        for (size_t p = 0; p < N; ++p) {
+            tot += r[p];
+            if (r[p] > mx) mx = r[p];
+        }
+        if (dv * tot < 0.0000000001) {
+            for (size_t p = 0; p < N; ++p) cut[p] = 0.0;
+            continue;
+        }
+        const double tr = mx / 10.0;
+        size_t T = 0;
+#define RHO(i, j, k) r[((size_t)(k) * ny + (j)) * nx + (i)]
+        for (int k = 0; k < nz - 1; ++k) {
+            const double Z = dx / 2.0 + dx * k;
+            for (int j = 0; j < ny - 1; ++j) {
+                const double Y = dx / 2.0 + dx * j;
+                for (int i = 0; i < nx - 1; ++i) {
+                    const double X = dx / 2.0 + dx * i;
+                    const double a = RHO(i, j, k);
+                    const double nbr[3] = {RHO(i + 1, j, k), RHO(i, j + 1, k), RHO(i, j, k + 1)};
+                    for (int ax = 0; ax < 3; ++ax) {
+                        const double b = nbr[ax];
+                        if ((a >= tr && b <= tr) || (a <= tr && b >= tr)) {
+                            double P[3] = {X, Y, Z};
+                            P[ax] = P[ax] + dx * (tr - a) / (b - a);
+                            surf[3 * T] = P[0];
+                            surf[3 * T + 1] = P[1];
+                            surf[3 * T + 2] = P[2];
+                            ++T;
+                        }
+                    }
+                }
+            }
+        }
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const double X = dx / 2.0 + dx * i, Y = dx / 2.0 + dx * j,
+                                 Z = dx / 2.0 + dx * k;
+                    double dmin = 1e12;
+                    for (size_t l = 0; l < T; ++l) {
+                        const double ex = X - surf[3 * l], ey = Y - surf[3 * l + 1],
+                                     ez = Z - surf[3 * l + 2];
+                        const double d = ex * ex + ey * ey + ez * ez;
+                        if (d < dmin) dmin = d;
+                    }
+                    const double sig = RHO(i, j, k) > tr ? -1.0 : 1.0;
+                    const double dr = sig * sqrt(dmin);
+                    cut[((size_t)k * ny + j) * nx + i] = 1.0 / (1.0 + exp((dr - radd) / acut));
+                }
+#undef RHO
+    }
+    free(surf);
+}

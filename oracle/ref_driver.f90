@@ -19,7 +19,8 @@
 ! Tasks (comma separated): derive, hpsi, dens, step, force (synthetic mean
 ! fields from the input file), pot (densities -> moments -> Coulomb ->
 ! potentials -> energy, reference routines), scf (the Evolve loop of
-! Main.f90:255-328 for HF; iteration count = niter from the 4th argument).
+! Main.f90:255-328 for HF, with AlternateStep while alternating constraints
+! are on; iteration count = niter from the 4th argument).
 ! With ParameterEstimation = 1 the scf task reads maxspwf's start grid from
 ! seed.bin (real8 grid(nx,ny,nz,4)) in place of IterativeEstimation's
 ! random_number start, and writes dt, momentum after each GradDesc to
@@ -47,6 +48,7 @@ program ref_driver
 
   character(len=512) :: infile, prefix, tasks, arg4
   integer            :: nwtf, i, m, it, uu, n, niter, up
+  logical            :: altcheck
   integer, allocatable :: qn(:,:)
   real(KIND=dp), allocatable :: occ(:), grids(:,:,:,:,:)
   type(Spinor)       :: sp, h
@@ -161,6 +163,10 @@ program ref_driver
     ! reference's own chain: moments -> Coulomb -> potentials -> energy.
     call ComputeDensity(Density, .false.)
     call CalculateAllMoments(1)
+    open(newunit=uu, file=trim(prefix)//'_cutoff.bin', access='stream',        &
+    &    form='unformatted', status='replace')
+    write(uu) Cutoff
+    close(uu)
     call ConstructPotentials
     call CompEnergy
     call dump_moments(trim(prefix)//'_moments.bin')
@@ -199,15 +205,19 @@ program ref_driver
     &    form='unformatted', status='replace')
     write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,        &
     &        sum(CoMCorrection), Skyrmeterms
+    altcheck = CheckForAlternateConstraints()
     do it=1,niter
+      if(altcheck) call AlternateStep()
       call EvolveSpwf(max(it, 2))
       if(ParameterEstimation.eq.1) write(up) dt, momentum
       call SolvePairing
+      altcheck = CheckForAlternateConstraints()
       call DeriveAll
       call UpdateDensities(0)
       call MixDensities(it)
       call CalculateAllMoments(1)
       call ReadjustAllMoments()
+      call TurnOffConstraints(it)
       call ConstructPotentials
       call CompEnergy
       write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,      &

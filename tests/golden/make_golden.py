@@ -114,7 +114,7 @@ CASES = {
 
 
 def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0, pe=0,
-              preconw=0.0):
+              preconw=0.0, constraints=()):
     fd = "-1" if lag else "3"
     fdl = "-1" if lag else "4"
     return f"""&GenInfo
@@ -135,8 +135,9 @@ Type="HF"
 MaxFDOrder={fd}, MaxFDLapOrder={fdl}, BStack=.false.
 /
 &MomentParam
+{"MoreConstraints=.true." if constraints else ""}
 /
-&InAndOutput
+{"".join("&MomentConstraint" + chr(10) + c + (", MoreConstraints=.true." if k + 1 < len(constraints) else "") + chr(10) + "/" + chr(10) for k, c in enumerate(constraints))}&InAndOutput
 /
 &Force
 afor="SLy4"
@@ -278,6 +279,7 @@ def read_pot(wd, prefix, nx, ny, nz, trc):
     mom = np.frombuffer(raw[4:4 + cnt * rec.itemsize], rec)
     res["pot_moment_lm"] = np.stack([mom["l"], mom["m"], mom["im"]], 1).astype(np.int32)
     res["pot_moment_val"] = mom["v"].copy()
+    res["pot_cutoff"] = np.fromfile(os.path.join(wd, prefix + "_cutoff.bin"), "<f8")
     e = np.fromfile(os.path.join(wd, prefix + "_energy.bin"), "<f8")
     # TotalEnergy, Kinetic(2), CoulombEnergy, CoulombExchange, CoMCorrection(2,2), terms(36)
     res["pot_energy"] = e
@@ -297,6 +299,16 @@ SCF_CASES = {
     # mixing
     "scfpc_trc_fd": (10, 0.8, True, False, 8, 8, 6, 10, 0, dict(mixing=2)),
     "scfpc_todd_lag": (8, 0.9, False, True, 4, 4, 6, 8, 0, dict(mixing=2)),
+    # multipole constraints: alternating (ConstraintType 2) on Q20 of the
+    # sum of both isospins, switched off after iteration 6; then neutron and
+    # proton Q20 separately (alternating) with an augmented-Lagrangian
+    # (ConstraintType 1) Q22
+    "scfcon_trc_fd": (10, 0.8, True, False, 8, 8, 6, 10, 0,
+                      dict(constraints=["l=2, m=0, Constraint=30.0, iteration=6"])),
+    "scfcon_todd_lag": (8, 0.9, False, True, 4, 4, 6, 8, 0,
+                        dict(constraints=["l=2, m=0, Isoswitch=2, ConstraintNeutrons=9.0, "
+                                          "ConstraintProtons=7.0",
+                                          "l=2, m=2, ConstraintType=1, Constraint=2.0"])),
 }
 
 
@@ -304,6 +316,7 @@ def run_scf(name):
     n, dx, trc, lag, neutrons, protons, per_iso, niter, pe = SCF_CASES[name][:9]
     opts = SCF_CASES[name][9] if len(SCF_CASES[name]) > 9 else {}
     mixing, preconw = opts.get("mixing", 0), opts.get("preconw", 0.0)
+    constraints = opts.get("constraints", ())
     seed = sum(map(ord, name))
     rng = np.random.default_rng(seed)
     pattern = [(1, 1), (-1, 1), (1, 1), (-1, 1), (1, 1), (-1, 1)] if trc else \
@@ -340,7 +353,7 @@ def run_scf(name):
         seed_grid.astype("<f8").tofile(os.path.join(wd, "seed.bin"))
         extra["est_seed"] = seed_grid
     nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom, pe=pe,
-                    mixing=mixing, preconw=preconw)
+                    mixing=mixing, preconw=preconw, constraints=constraints)
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
     with open(os.path.join(wd, "input.nml")) as fin:
@@ -365,6 +378,7 @@ def run_scf(name):
                 mixing=("scheme 0, DampingParam 0.75" if mixing == 0 else
                         f"scheme {mixing}, PreConW {preconw}"),
                 mixing_scheme=mixing, preconw=preconw, parameter_estimation=pe,
+                constraints=list(constraints),
                 source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")
     np.savez_compressed(os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
                         qn=np.array(qns, np.int32), psi=psi, traj=traj,

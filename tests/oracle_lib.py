@@ -345,6 +345,144 @@ def precondition_potentials(mesh, F, Fold, trc, dv, preconw):
         F["W"] = W.reshape(F["W"].shape)
 
 
+def rutz_cutoff(mesh, rho, dv, radd=4.0, acut=0.4):
+    out = np.zeros((2, mesh.N))
+    lib().or_rutz_cutoff(P(np.ascontiguousarray(rho, dtype=np.float64).reshape(2, mesh.N)),
+                         mesh.nx, mesh.ny, mesh.nz, ctypes.c_double(mesh.dx),
+                         ctypes.c_double(dv), ctypes.c_double(radd), ctypes.c_double(acut),
+                         P(out))
+    return out
+
+
+def ylm_field(mesh, l, m):
+    """SpherHarm of the moment (l, m) on the octant mesh (r^l Re Y_lm;
+    1/sqrt(4 pi) for l = 0), points at (i + 1/2) dx."""
+    out = np.zeros(mesh.N)
+    q = 0
+    for k in range(mesh.nz):
+        for j in range(mesh.ny):
+            for i in range(mesh.nx):
+                X, Y, Z = (i + 0.5) * mesh.dx, (j + 0.5) * mesh.dx, (k + 0.5) * mesh.dx
+                out[q] = 1.0 / np.sqrt(4 * np.pi) if l == 0 else rlylm(l, m, X, Y, Z)
+                q += 1
+    return out
+
+
+def parse_namelist_items(text):
+    """'l=2, m=0, Constraint=30.0' -> {'l': 2, 'm': 0, 'constraint': 30.0}."""
+    out = {}
+    for item in text.split(","):
+        k, v = item.split("=")
+        k, v = k.strip().lower(), v.strip()
+        out[k] = int(v) if v.lstrip("-").isdigit() else float(v)
+    return out
+
+
+D0 = float(np.float32(0.1))     # d0 of Moments.f90:258, a default-real literal
+
+
+class Constraints:
+    """Multipole constraints on the density moments (Moments.f90):
+    Calculate_Multipole values and <Q^2> (:1907-2004), the first-call targets
+    of CalculateAllMoments (:2250-2334), CalcConstraintEnergy (:2368-2531),
+    Readjust (:2929-3058), TurnOffConstraints (:3516-3549) and the multipole
+    field of AlternateStep (ImaginaryTime.f90:306-437).  Types 1 and 2,
+    Isoswitch 1 and 2; Rutz cutoff, damping 0, ReadjustSlowDown 0.01."""
+
+    def __init__(self, specs, mesh, lm, dv):
+        self.mesh, self.dv = mesh, dv
+        self.damping, self.slowdown = 0.0, 0.01
+        self.items = []
+        for text in specs:
+            p = parse_namelist_items(text)
+            l, m = p["l"], p.get("m", 0)
+            idx = [k for k, (a, b) in enumerate(lm) if a == l and b == m][0]
+            iso = p.get("isoswitch", 1)
+            if iso == 1:
+                c = np.array([p.get("constraint", 0.0)] * 2)
+            else:
+                c = np.array([p.get("constraintneutrons", 0.0), p.get("constraintprotons", 0.0)])
+            self.items.append(dict(idx=idx, type=p.get("constrainttype", 2), iso=iso,
+                                   iteration=p.get("iteration", -1), con=c.copy(),
+                                   true=c.copy(), inten=np.array([p.get("intensity", 0.0)] * 2),
+                                   Y=ylm_field(mesh, l, m)))
+        self.items.sort(key=lambda c: c["idx"])     # the moment list's order
+        self.ce = np.zeros((2, mesh.N))
+        self.first = True
+
+    def calculate(self, rho, val):
+        rho = rho.reshape(2, self.mesh.N)
+        self.cut = rutz_cutoff(self.mesh, rho, self.dv)
+        for c in self.items:
+            c["value"] = np.array(val[c["idx"]])
+            c["sq"] = np.array([np.sum(c["Y"] ** 2 * rho[it]) * self.dv for it in range(2)])
+            if self.first and c["type"] != 0:
+                c["con"] = (np.array([c["value"].sum()] * 2) if c["iso"] == 1
+                            else c["value"].copy())
+        self.first = False
+        self.ce = self.damping * self.ce
+        for c in self.items:
+            if c["type"] == 0:
+                continue
+            if c["iso"] == 1:
+                fac = np.array([2.0 * c["inten"][0]] * 2)
+                value = np.array([c["value"].sum()] * 2)
+                des = np.array([c["con"][0]] * 2)
+            else:
+                fac, value, des = 2.0 * c["inten"], c["value"], c["con"]
+            if np.any(self.ce != 0.0):
+                fac = fac * (1.0 - self.damping)
+            for it in range(2):
+                self.ce[it] = self.ce[it] + fac[it] * (value[it] - des[it]) * c["Y"] * \
+                    self.cut[it]
+
+    def readjust(self, momentum):
+        for c in self.items:
+            if c["type"] == 1:
+                if c["iso"] == 1:
+                    if c["inten"][0] == 0.0:
+                        o2 = np.array([c["sq"].sum()] * 2)
+                        c["inten"][:] = 2 / o2.sum()
+                        self.slowdown = 2 * c["inten"][0]
+                    c["con"][:] = c["con"][0] - self.slowdown * (c["value"].sum() - c["true"][0])
+                else:
+                    c["con"] = c["con"] - self.slowdown * (c["value"] - c["true"])
+            elif c["type"] == 2:
+                if c["inten"][0] == 0.0:
+                    if c["iso"] == 1:
+                        c["inten"][:] = 2 / c["sq"].sum()
+                        if momentum != 0.0:
+                            c["inten"][:] = 0.5 * c["inten"][0]
+                    else:
+                        c["inten"] = 2 / c["sq"]
+                        if momentum != 0.0:
+                            c["inten"] = 0.5 * c["inten"]
+                if c["iso"] == 1:
+                    c["con"][:] = c["con"][0] - (c["value"].sum() - c["true"][0])
+                else:
+                    c["con"] = c["con"] - (c["value"] - c["true"])
+
+    def turn_off(self, iteration):
+        for c in self.items:
+            if c["type"] != 0 and c["iteration"] > 0 and iteration > c["iteration"]:
+                c["type"] = 0
+
+    def alternating(self):
+        return any(c["type"] == 2 for c in self.items)
+
+    def alternate_field(self):
+        mult = np.zeros((2, self.mesh.N))
+        for c in self.items:
+            if c["type"] != 2:
+                continue
+            o2 = c["sq"].sum()
+            for it in range(2):
+                v = c["value"].sum() if c["iso"] == 1 else c["value"][it]
+                d = c["true"][0] if c["iso"] == 1 else c["true"][it]
+                mult[it] = mult[it] + 0.5 * (v - d) / (o2 + D0) * self.cut[it] * c["Y"]
+        return mult
+
+
 def all_densities(mesh, psi, qn, occ, trc):
     """ComputeDensity + the derived fields, and each state's derivatives."""
     M, N = psi.shape[0], mesh.N
@@ -378,20 +516,32 @@ def scf_run(g, niter):
     def rows(E):
         return np.concatenate([[E[0], E[1] + E[2], E[3], E[4], E[5] + E[6]], E[7:]])
 
-    def construct(d):
+    cons = None
+
+    def construct(d, calc=1):
         lm, val = moments(d["rho"], mesh, dv)
+        if cons is not None:
+            for _ in range(calc):
+                cons.calculate(d["rho"], val)
         vc, vx = coul.solve(d["rho"][1], lm, val[:, 1])
         F = potentials(coef, mesh, trc, d, vc, vx)
+        if cons is not None:
+            F["U"] = (F["U"].reshape(2, N) + cons.ce).reshape(F["U"].shape)
         return F, vc, vx
+
+    if m.get("constraints"):
+        lm0, _ = moments(np.zeros((2, N)), mesh, dv)
+        cons = Constraints(m["constraints"], mesh, [tuple(x) for x in lm0], dv)
 
     scheme = m.get("mixing_scheme", 0)
     preconw = m.get("preconw", 0.0)
     occ = hf_fill(np.zeros(M), qn, m["neutrons"], m["protons"], trc)
     d, der, lap = all_densities(mesh, psi, qn, occ, trc)
-    F, vc, vx = construct(d)
+    F, vc, vx = construct(d, calc=3)      # CalculateAllMoments three times (Main.f90:213-215)
     E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
                kinetic_sums(psi, lap, qn, occ, dv))
     traj = [rows(E)]
+    altcheck = cons is not None and cons.alternating()
     upd = np.zeros_like(psi)
     hbar = consts[5]
     dt, mom = m["dt"], m["momentum"]
@@ -403,6 +553,16 @@ def scf_run(g, niter):
     e_sp = np.zeros(M)
     params = []
     for k in range(niter):
+        if altcheck:
+            mult = cons.alternate_field()
+            for i in range(M):
+                it = (qn[i][0] + 3) // 2 - 1
+                psi[i] = psi[i] - mult[it] * psi[i]
+            # AlternateStep (ImaginaryTime.f90:306-437) changes the values
+            # only: neither SetGrid nor Gramschmidt recomputes the
+            # derivatives, so the next GradDesc's hPsi combines the new
+            # values with the derivatives of the states before the step
+            psi = gram_schmidt(psi, qn, e_sp, dv)
         Ff = {key: np.ascontiguousarray(v).reshape(-1) for key, v in F.items()}
         if est:
             it_est = (qn_est[0] + 3) // 2 - 1
@@ -414,6 +574,8 @@ def scf_run(g, niter):
         psi, upd, e_sp, _ = graddesc(psi, der, lap, upd, qn, trc, kin2, Ff, dt_over_hbar,
                                       mom, dv)
         occ = hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
+        if cons is not None:
+            altcheck = cons.alternating()
         old = d
         d, der, lap = all_densities(mesh, psi, qn, occ, trc)
         if scheme == 0:
@@ -421,7 +583,14 @@ def scf_run(g, niter):
                 lib().or_mix(P(d[key]), P(np.ascontiguousarray(old[key])),
                              ctypes.c_size_t(d[key].size), ctypes.c_double(mixing))
         Fold = F
-        F, vc, vx = construct(d)
+        if cons is not None:
+            lm_, val_ = moments(d["rho"], mesh, dv)
+            cons.calculate(d["rho"], val_)
+            cons.readjust(mom)
+            cons.turn_off(k + 1)
+            F, vc, vx = construct(d, calc=0)
+        else:
+            F, vc, vx = construct(d)
         if scheme == 2:
             precondition_potentials(mesh, F, Fold, trc, dv, preconw)
         E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,

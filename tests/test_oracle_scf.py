@@ -127,7 +127,50 @@ def test_scf_potential_preconditioning_vs_reference(case):
         assert relerr(r["fields"]["S"].reshape(-1), g["final_Spot"].reshape(-1)) < 1e-10
 
 
+@pytest.mark.parametrize("case", ["scfcon_trc_fd", "scfcon_todd_lag"])
+def test_scf_constraints_vs_reference(case):
+    """Multipole constraints: the AlternateStep before GradDesc (whose hPsi
+    then uses the derivatives of the states before that step, as in the
+    reference), the constraint field in U, readjusted targets, estimated
+    intensities and the switch-off after the given iteration."""
+    g = golden(case)
+    r = ol.scf_run(g, g["meta"]["niter"])
+    assert np.max(np.abs(r["traj"][:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(r["traj"], g["traj"]) < 1e-12
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-11
+    assert relerr(r["fields"]["U"].reshape(-1), g["final_Upot"].reshape(-1)) < 1e-11
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_rutz_cutoff_vs_reference(case):
+    g = golden(case)
+    m = g["meta"]
+    mesh = ol.Mesh(m["nx"], m["ny"], m["nz"], m["dx"], m["lagrange"])
+    cut = ol.rutz_cutoff(mesh, g["rho"], g["consts"][6])
+    assert relerr(cut.reshape(-1), g["pot_cutoff"]) < 1e-14
+
+
 # ---- checks fixed by the mathematics ---------------------------------------
+
+def test_rutz_cutoff_of_a_spherical_density():
+    """For rho = exp(-r^2/b^2) the equidensity surface at max/10 is the sphere
+    r0 = b sqrt(ln 10 + ...) through the mesh maximum; DeltaR is the distance
+    to it, |r - r0|, up to the linear interpolation and the surface sampling
+    (a fraction of dx)."""
+    n, dx, b = 14, 0.5, 2.5
+    mesh = ol.Mesh(n, n, n, dx, 0)
+    x = (np.arange(n) + 0.5) * dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z).reshape(-1)
+    rho = np.exp(-(r / b) ** 2)
+    r0 = b * np.sqrt(np.log(10.0) - (r.min() / b) ** 2)    # max of the mesh is at r.min()
+    cut = ol.rutz_cutoff(mesh, np.stack([rho, 0.5 * rho]), 8 * dx ** 3, radd=4.0, acut=0.4)
+    dr = 4.0 + 0.4 * np.log(1.0 / cut[0] - 1.0)            # invert the Fermi function
+    inner = r < 0.8 * n * dx                                # away from the box faces
+    assert np.max(np.abs(dr[inner] - (r[inner] - r0))) < 0.6 * dx
+    assert np.array_equal(cut[0], cut[1])                   # scale-free threshold
+    # no particles: cutoff 0
+    assert not ol.rutz_cutoff(mesh, np.zeros((2, mesh.N)), 8 * dx ** 3).any()
 
 @pytest.mark.parametrize("lag", [0, 1])
 def test_precondition_solves_the_screened_equation(lag):
