@@ -236,6 +236,49 @@ int mocca_estimation_seed(mocca_solve *h, const double *grid);
 int mocca_iterate_estimated(mocca_solve *h, int first_iteration, int niter, double dt,
                             double momentum, double *energies, double *params);
 
+/* One multipole constraint (namelist MomentConstraint, Moments.f90:3576-3800):
+ * the moment (l, m) of the density (real part, even l and m under the EV8
+ * symmetries), ConstraintType 1 (augmented Lagrangian) or 2 (alternating
+ * step, the default), Isoswitch 1 (neutrons + protons: value[0]) or 2
+ * (value[0] neutrons, value[1] protons), the iteration after which it is
+ * switched off (<= 0: never) and Intensity (0: estimated at the first
+ * readjustment). */
+typedef struct mocca_constraint {
+    int l, m;
+    int type;
+    int isoswitch;
+    int iteration;
+    int pad_;
+    double value[2];
+    double intensity;
+} mocca_constraint;
+
+/* Multipole constraints for the run (ReadMomentData, Moments.f90:3558-3800),
+ * with the Rutz cutoff parameters radd, acut (4.0, 0.4), the Damping of the
+ * constraint field (0.0) and ReadjustSlowDown (0.01).  After
+ * mocca_scf_setup and before mocca_scf_start; count = 0 removes them.  They
+ * act through CalculateAllMoments in mocca_construct_potentials (cutoff,
+ * <Q^2>, the constraint field added to U, MeanFields.f90:402-403),
+ * mocca_readjust_constraints and mocca_alternate_step; mocca_iterate and
+ * mocca_iterate_estimated run all three in the reference's order. */
+int mocca_set_constraints(mocca_solve *h, const mocca_constraint *c, int count, double radd,
+                          double acut, double damping, double readjust_slowdown);
+
+/* ReadjustAllMoments + TurnOffConstraints(iteration) (Moments.f90:2336-2366,
+ * 3516-3549); momentum is the current heavy-ball momentum (it halves the
+ * estimated intensity of alternating constraints). */
+int mocca_readjust_constraints(mocca_solve *h, int iteration, double momentum);
+
+/* AlternateStep (ImaginaryTime.f90:306-437): psi <- psi - sum_c
+ * 0.5 (<Q_c> - Q_c) / (<Q_c^2> + d0) cutoff Y_c psi, then Gramschmidt.  As in
+ * the reference, the next mocca_grad_desc takes the derivatives of the
+ * states before this step (their values after it). */
+int mocca_alternate_step(mocca_solve *h);
+
+/* State of the constraints: out[count][10] = l, m, type (0 once switched
+ * off), target(2), intensity(2), <Q^2>(2), isoswitch. */
+int mocca_get_constraints(mocca_solve *h, double *out);
+
 /* Results of the last ConstructPotentials.  lm[k][2] = (l, m) and
  * q[k][2] = (neutron, proton) moments; *count moments (<= 28).
  * vcoul, vcoulx [N]; cpot [(nz+2)][(ny+2)][(nx+2)] (CPotComplete);

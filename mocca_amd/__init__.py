@@ -32,7 +32,8 @@ EXPORTS = (
     "mocca_mix_densities", "mocca_construct_potentials", "mocca_comp_energy",
     "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
     "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
-    "mocca_iterate_estimated",
+    "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
+    "mocca_alternate_step", "mocca_get_constraints",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -64,6 +65,12 @@ class FunctionalC(ctypes.Structure):
                 ("max_moment", ctypes.c_int), ("damping", ctypes.c_double),
                 ("mixing_scheme", ctypes.c_int), ("pad_", ctypes.c_int),
                 ("preconu", ctypes.c_double)]
+
+
+class ConstraintC(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_int), ("m", ctypes.c_int), ("type", ctypes.c_int),
+                ("isoswitch", ctypes.c_int), ("iteration", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("value", ctypes.c_double * 2), ("intensity", ctypes.c_double)]
 
 
 # Energy row layout of mocca_comp_energy / mocca_iterate.
@@ -264,6 +271,40 @@ class Solve:
         self._chk(self.lib.mocca_iterate(self.h, int(niter), ctypes.c_double(dt_over_hbar),
                                          ctypes.c_double(momentum), _p(e)))
         return e
+
+    def set_constraints(self, constraints, radd=4.0, acut=0.4, damping=0.0,
+                        readjust_slowdown=0.01):
+        """constraints: dicts with l, m, and optionally type (2), isoswitch (1),
+        iteration (-1: never switched off), value (isoswitch 1) or
+        value_n, value_p (isoswitch 2), intensity (0: estimated)."""
+        arr = (ConstraintC * max(1, len(constraints)))()
+        self._nconstraints = len(constraints)
+        for k, c in enumerate(constraints):
+            iso = int(c.get("isoswitch", 1))
+            v = ((c.get("value", 0.0), 0.0) if iso == 1
+                 else (c.get("value_n", 0.0), c.get("value_p", 0.0)))
+            arr[k] = ConstraintC(int(c["l"]), int(c.get("m", 0)), int(c.get("type", 2)), iso,
+                                 int(c.get("iteration", -1)), 0,
+                                 (ctypes.c_double * 2)(*[float(x) for x in v]),
+                                 float(c.get("intensity", 0.0)))
+        self._chk(self.lib.mocca_set_constraints(self.h, arr, len(constraints),
+                                                 ctypes.c_double(radd), ctypes.c_double(acut),
+                                                 ctypes.c_double(damping),
+                                                 ctypes.c_double(readjust_slowdown)))
+
+    def readjust_constraints(self, iteration, momentum):
+        self._chk(self.lib.mocca_readjust_constraints(self.h, int(iteration),
+                                                      ctypes.c_double(momentum)))
+
+    def alternate_step(self):
+        self._chk(self.lib.mocca_alternate_step(self.h))
+
+    def constraints(self):
+        """[count][10]: l, m, type, target(2), intensity(2), <Q^2>(2), isoswitch."""
+        n = getattr(self, "_nconstraints", 0)
+        out = np.zeros((max(n, 1), 10))
+        self._chk(self.lib.mocca_get_constraints(self.h, _p(out)))
+        return out[:n]
 
     def estimate_parameters(self, iteration, dt, momentum):
         """IterativeEstimation: (dt, momentum, maxE, relE, con, iterations)."""

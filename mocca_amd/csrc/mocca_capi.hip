@@ -275,22 +275,29 @@ template <typename T>
 int grad_desc_impl(mocca_solve *h, double dth, double mu) {
     const MeshK<T> &m = mesh_of<T>(h);
     T *psi = (T *)h->b.psi, *psi2 = (T *)h->b.psi2, *upd = (T *)h->b.upd;
+    // After an AlternateStep the stencils read the states before it and the
+    // values come from the current ones (the reference keeps the derivatives
+    // of DeriveAll until the next one, ImaginaryTime.f90:306-437).
+    const bool stale = h->scf.alt_pending;
+    const T *dsrc = stale ? (const T *)h->scf.alt_old : psi;
+    const T *vsrc = stale ? psi : nullptr;
+    h->scf.alt_pending = false;
     int rc, slots;
     if (h->march) {
         slots = h->ntile_y;
         rc = run(h, "hpsi_step", [&] {
-            launch_hpsi_fd<T>(m, psi, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
+            launch_hpsi_fd<T>(m, dsrc, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
                               h->b.tasks, h->ntasks, h->TY, h->ntile_y,
                               h->mesh.time_reversal, T(h->kin[0]), T(h->kin[1]), T(dth),
-                              T(mu), kStep, h->b.eparts, h->st);
+                              T(mu), kStep, h->b.eparts, h->st, vsrc);
         });
     } else {
         slots = h->ntiles;
         rc = run(h, "hpsi_step", [&] {
-            launch_hpsi_step<T>(m, psi, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
+            launch_hpsi_step<T>(m, dsrc, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
                                 h->b.order, h->b.chunks, h->nchunks, h->mesh.time_reversal,
                                 T(h->kin[0]), T(h->kin[1]), T(dth), T(mu), kStep,
-                                h->b.eparts, h->ntiles, h->st);
+                                h->b.eparts, h->ntiles, h->st, vsrc);
         });
     }
     if (rc) return rc;
@@ -465,6 +472,33 @@ int estimate_impl(mocca_solve *h, int iteration, double dt, double mu, double *o
 int estimate_any(mocca_solve *h, int iteration, double dt, double mu, double *out) {
     return h->prec == MOCCA_FP64 ? estimate_impl<double>(h, iteration, dt, mu, out)
                                  : estimate_impl<float>(h, iteration, dt, mu, out);
+}
+
+// AlternateStep (ImaginaryTime.f90:306-437): psi <- psi - multipole psi,
+// then Gramschmidt; the states before the step stay in alt_old for the
+// derivatives of the next GradDesc.
+template <typename T>
+int alternate_impl(mocca_solve *h) {
+    Scf &s = h->scf;
+    const long N = h->N;
+    if (!s.alt_old) {
+        int rc = dalloc(h, &s.alt_old, (size_t)h->M * 4 * N * h->tsize);
+        if (rc) return rc;
+    }
+    int rc = run(h, "alternate", [&] {
+        launch_alternate<T>(s.con, s.ncon, s.Y, s.calt, s.cut, N, s.mult, (const T *)h->b.psi,
+                            (T *)h->b.psi2, h->b.qn, h->M, h->st);
+    });
+    if (rc) return rc;
+    if ((rc = gram_schmidt_impl<T>(h, (const T *)h->b.psi2, (T *)s.alt_old, nullptr, 0)))
+        return rc;
+    std::swap(h->b.psi, s.alt_old);
+    s.alt_pending = true;
+    return 0;
+}
+
+int alternate_any(mocca_solve *h) {
+    return h->prec == MOCCA_FP64 ? alternate_impl<double>(h) : alternate_impl<float>(h);
 }
 
 int grad_desc_any(mocca_solve *h, double dth, double mu) {
@@ -925,6 +959,7 @@ int mocca_upload_states(mocca_solve *h, int first, int count, const double *grid
                     first + count, h->M);
     HIPCHK(h, hipStreamSynchronize(h->st));
     const size_t n4 = 4 * (size_t)h->N;
+    h->scf.alt_pending = false;
     return up(h, (char *)h->b.psi + first * n4 * h->tsize, grid, count * n4);
 }
 
@@ -973,6 +1008,7 @@ int mocca_set_potentials(mocca_solve *h, const mocca_fields *f) {
 int mocca_derive_all(mocca_solve *h) {
     if (int rc = check_ready(h)) return rc;
     const size_t N = h->N, M = h->M;
+    h->scf.alt_pending = false;   // DeriveAll refreshes the derivatives
     if (!h->b.der) {
         if (int rc = dalloc(h, &h->b.der, M * 12 * N * sizeof(double))) return rc;
         if (int rc = dalloc(h, &h->b.lap, M * 4 * N * sizeof(double))) return rc;

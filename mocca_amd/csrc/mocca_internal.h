@@ -84,6 +84,20 @@ struct Scf {
     PrecJob *pcjobs = nullptr;
     int *pc_iters = nullptr;       // [npc]
     int npc = 0;
+    // multipole constraints
+    int ncon = 0;
+    ConstraintK *con = nullptr;    // [ncon] device records
+    ConstraintCtl *cctl = nullptr;
+    double *csq = nullptr, *ccoef = nullptr, *calt = nullptr;  // [ncon][2]
+    double *cut = nullptr, *ce = nullptr, *mult = nullptr;     // [2][N]
+    double *cstat = nullptr;       // [2][2] threshold, particles present
+    int *ccnt = nullptr;           // [2] surface points
+    double *csurf = nullptr;       // [2][3N][3]
+    double radd = 4.0, acut = 0.4;
+    std::vector<int> htype, hiter;  // host copy of the types and switch-off iterations
+    void *alt_old = nullptr;       // states before the last AlternateStep (T [M][4][N])
+    bool alt_pending = false;      // the next GradDesc takes its derivatives from alt_old
+    bool alt_next = false;         // Evolve's AlternateCheck for the next iteration
 };
 
 // The extra state of IterativeEstimation (maxspwf) and its records.
@@ -189,6 +203,8 @@ int grad_desc_any(mocca_solve *h, double dth, double mu);
 int density_any(mocca_solve *h);
 // K[i] = dv <psi_i|Lap psi_i> of the current states (no other output).
 int kinetic_any(mocca_solve *h);
+// AlternateStep (constraints of type 2) on the device.
+int alternate_any(mocca_solve *h);
 // IterativeEstimation: out = dt, momentum, maxE, relE, con, iterations.
 int estimate_any(mocca_solve *h, int iteration, double dt, double mu, double *out);
 constexpr double kHbar = 6.58211928;   // Force.f90:103

@@ -91,7 +91,7 @@ template <typename T>
 void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
                     const T *fields, const int *qn, const Task *tasks, int ntasks,
                     int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
-                    int mode, double *eparts, hipStream_t st);
+                    int mode, double *eparts, hipStream_t st, const T *vsrc = nullptr);
 
 template <typename T>
 void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
@@ -102,7 +102,7 @@ void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
                       T *hout, const T *fields, const int *qn, const int *order,
                       const Chunk *chunks, int nchunks, int trc, T kin0, T kin1,
                       T dth, T mu, int mode, double *eparts, int ntiles,
-                      hipStream_t st);
+                      hipStream_t st, const T *vsrc = nullptr);
 
 // E = dv sum psi.h, D = dv sum h.h - E^2, K = dv sum psi.Lap psi per state
 // (any of E, D, K may be null).
@@ -202,7 +202,7 @@ void launch_coulomb_direct(const CoulombK &c, double *x, const double *rho_p,
                            double *w1, double *vcoul, double *vcoulx, hipStream_t st);
 void launch_potentials(const SkyrmeK &f, long N, int trc, const double *dens,
                        const double *derived, const double *vcoul, const double *vcoulx,
-                       double *pot, hipStream_t st);
+                       const double *ce, double *pot, hipStream_t st);
 void launch_convert_f32(const double *a, float *b, long n, hipStream_t st);
 // e[43] = [total, kin(2), E_C, E_Cx, com(2), terms 1..36]
 void launch_energy_terms(const SkyrmeK &f, long N, int trc, const double *dens,
@@ -214,6 +214,38 @@ constexpr int kEnergySlots = 35;   // partial sums per workgroup of the energy p
 
 void launch_precondition(const MeshK<double> &m, const PrecJob *jobs, int njobs, double *work,
                          double alpha, double beta, double dv, int *iters, hipStream_t st);
+
+// Multipole constraints (Moments.f90): one record per constraint, in the
+// order of the moment list.
+struct ConstraintK {
+    int idx;             // moment slot (row of the Y table, of Q)
+    int type;            // ConstraintType: 0 off, 1 augmented Lagrangian, 2 alternating
+    int iso;             // Isoswitch: 1 neutrons + protons, 2 each isospin
+    int iteration;       // switched off after this iteration (<= 0: never)
+    double con[2], truec[2], inten[2];   // Constraint, TrueConstraint, Intensity
+};
+struct ConstraintCtl {
+    double damping;      // Damping of the constraint field
+    double slowdown;     // ReadjustSlowDown
+    int first;           // next CalculateAllMoments is the first call
+    int ce_nz;           // the constraint field is not all zero
+};
+enum ConMode { kConCalc = 1, kConReadjust = 2, kConTurnOff = 4 };
+void launch_rutz_cutoff(const double *dens, int nx, int ny, int nz, double dx, double dv,
+                        double radd, double acut, double *stat, int *cnt, double *surf,
+                        double *cut, hipStream_t st);
+void launch_con_squared(const ConstraintK *con, int nc, const double *Y, const double *dens,
+                        long N, double dv, double *sq, hipStream_t st);
+void launch_con_update(ConstraintK *con, int nc, ConstraintCtl *ctl, const double *Q,
+                       const double *sq, const double *cstat, int mode, int iteration,
+                       double momentum, double *ce_coef, double *alt, hipStream_t st);
+void launch_con_field(const ConstraintK *con, int nc, const double *Y, const double *coef,
+                      const ConstraintCtl *ctl, const double *cut, long N, double *ce,
+                      hipStream_t st);
+template <typename T>
+void launch_alternate(const ConstraintK *con, int nc, const double *Y, const double *alt,
+                      const double *cut, long N, double *mult, const T *in, T *out,
+                      const int *qn, int M, hipStream_t st);
 
 // IterativeEstimation (power iteration for dt and momentum).
 constexpr int kEstSlots = 8;

@@ -165,13 +165,14 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 // once per point and reused.  Partial sums of <psi|h psi>, <h psi|h psi> and
 // <psi|Lap psi> (the kinetic sum of CompKinetic, Energy.f90:1426-1456) go to
 // eparts[state][tile][3].
-template <typename T>
+// STALE: values from vsrc, derivatives from psi (see k_hpsi_fd).
+template <typename T, bool STALE>
 __global__ __launch_bounds__(kBlock) void k_hpsi_step(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
     const int *__restrict__ qn, const int *__restrict__ order,
     const Chunk *__restrict__ chunks, int trc, T kin0, T kin1, T dth, T mu,
-    int mode, double *__restrict__ eparts, int ntiles) {
+    int mode, double *__restrict__ eparts, int ntiles, const T *__restrict__ vsrc) {
     __shared__ double sh[3 * kBlock / 64];
     const long N = m.N;
     const long p = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -207,6 +208,10 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
         if (valid) {
             state_derivs(m, ps, qn[4 * s + 1], qn[4 * s + 2], qn[4 * s + 3], x, y, z,
                          p, v, d, l, true);
+            if constexpr (STALE) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[c] = vsrc[(long)s * 4 * N + (long)c * N + p];
+            }
             // ActionOfBNew (MeanFields.f90:811-851)
             // ActionOfU    (MeanFields.f90:792-809)
             // ActionOfW    (MeanFields.f90:1029-1065): -i[W_mn s_n d_m + 1/2 divW_n s_n] psi
@@ -289,11 +294,16 @@ void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
                       T *hout, const T *fields, const int *qn, const int *order,
                       const Chunk *chunks, int nchunks, int trc, T kin0, T kin1,
                       T dth, T mu, int mode, double *eparts, int ntiles,
-                      hipStream_t st) {
+                      hipStream_t st, const T *vsrc) {
     dim3 g(ntiles, nchunks);
-    hipLaunchKernelGGL(k_hpsi_step<T>, g, dim3(kBlock), 0, st, m, psi, psi_out, upd,
-                       hout, fields, qn, order, chunks, trc, kin0, kin1, dth, mu,
-                       mode, eparts, ntiles);
+    if (vsrc)
+        hipLaunchKernelGGL((k_hpsi_step<T, true>), g, dim3(kBlock), 0, st, m, psi, psi_out,
+                           upd, hout, fields, qn, order, chunks, trc, kin0, kin1, dth, mu,
+                           mode, eparts, ntiles, vsrc);
+    else
+        hipLaunchKernelGGL((k_hpsi_step<T, false>), g, dim3(kBlock), 0, st, m, psi, psi_out,
+                           upd, hout, fields, qn, order, chunks, trc, kin0, kin1, dth, mu,
+                           mode, eparts, ntiles, vsrc);
 }
 
 // Energies and dispersions (ImaginaryTime.f90:470-471): E = dv sum psi.h,
@@ -1046,7 +1056,7 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
     template void launch_hpsi_step<T>(const MeshK<T> &, const T *, T *, T *, T *,        \
                                       const T *, const int *, const int *,               \
                                       const Chunk *, int, int, T, T, T, T, int,          \
-                                      double *, int, hipStream_t);                       \
+                                      double *, int, hipStream_t, const T *);            \
     template void launch_density<T>(const MeshK<T> &, const T *, const int *,            \
                                     const double *, const int *, const Chunk *, int,     \
                                     int, T *, hipStream_t);                              \

@@ -449,6 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_potentials(SkyrmeK f, long N, int tr
                                                         const double *__restrict__ derived,
                                                         const double *__restrict__ vcoul,
                                                         const double *__restrict__ vcoulx,
+                                                        const double *__restrict__ ce,
                                                         double *__restrict__ pot) {
     const long p = (long)blockIdx.x * kBlock + threadIdx.x;
     if (p >= N) return;
@@ -497,6 +498,11 @@ __global__ __launch_bounds__(kBlock) void k_potentials(SkyrmeK f, long N, int tr
     }
     U[1] = U[1] + vcoul[p];
     if (f.cexchange == 2) U[1] = U[1] + vcoulx[p];
+    // multipole constraints (MeanFields.f90:402-403)
+    if (ce) {
+        U[0] = U[0] + ce[p];
+        U[1] = U[1] + ce[N + p];
+    }
     pot[(kOffU + 0) * N + p] = U[0];
     pot[(kOffU + 1) * N + p] = U[1];
     // B (CalcBPot)
@@ -543,9 +549,9 @@ __global__ __launch_bounds__(kBlock) void k_potentials(SkyrmeK f, long N, int tr
 
 void launch_potentials(const SkyrmeK &f, long N, int trc, const double *dens,
                        const double *derived, const double *vcoul, const double *vcoulx,
-                       double *pot, hipStream_t st) {
+                       const double *ce, double *pot, hipStream_t st) {
     hipLaunchKernelGGL(k_potentials, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, f, N,
-                       trc, dens, derived, vcoul, vcoulx, pot);
+                       trc, dens, derived, vcoul, vcoulx, ce, pot);
 }
 
 template <typename T>
@@ -747,6 +753,348 @@ void launch_energy_kinetic(const SkyrmeK &f, const double *K, const double *occ,
                            int M, double *e, hipStream_t st) {
     hipLaunchKernelGGL(k_energy_kinetic, dim3(1), dim3(kBlock), 0, st, f, K, occ, qn, M, e);
 }
+
+// ---------------------------------------------------------------------------
+// Multipole constraints (Moments.f90).  Scalars of every constraint live in
+// device records updated by one thread in the reference's order, so an
+// iteration needs no host round trip; the fields (cutoff, constraint field
+// in U, AlternateStep multipole) are point kernels.
+
+// RutzCutOff (Moments.f90:2703-2849), step 1: per isospin threshold
+// max(rho)/10 and the particle-number test; clears the surface counters.
+__global__ __launch_bounds__(1024) void k_cut_stats(const double *__restrict__ dens, long N,
+                                                   double dv, double *__restrict__ stat,
+                                                   int *__restrict__ cnt) {
+    __shared__ double smax[1024], ssum[16];
+    const int it = blockIdx.x;
+    const double *r = dens + (long)it * 9 * N;
+    double mx = r[0], sm = 0;
+    for (long p = threadIdx.x; p < N; p += 1024) {
+        mx = fmax(mx, r[p]);
+        sm += r[p];
+    }
+    smax[threadIdx.x] = mx;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sm += __shfl_xor(sm, o, 64);
+    if (threadIdx.x % 64 == 0) ssum[threadIdx.x / 64] = sm;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < o) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < 16; ++k) t += ssum[k];
+        stat[2 * it] = smax[0] / 10.0;
+        stat[2 * it + 1] = dv * t < 0.0000000001 ? 0.0 : 1.0;
+        cnt[it] = 0;
+    }
+}
+
+// Step 2: surface points between neighbours straddling the threshold, for
+// the cells i < nx-1, j < ny-1, k < nz-1 (the reference's loop bounds).  The
+// order of the points does not matter: only their minimum distance is used.
+__global__ __launch_bounds__(kBlock) void k_cut_surface(const double *__restrict__ dens, int nx,
+                                                       int ny, int nz, double dx,
+                                                       const double *__restrict__ stat,
+                                                       int *__restrict__ cnt,
+                                                       double *__restrict__ surf) {
+    const long N = (long)nx * ny * nz;
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int it = blockIdx.y;
+    if (p >= N || stat[2 * it + 1] == 0.0) return;
+    const int i = p % nx, j = (p / nx) % ny, k = p / ((long)nx * ny);
+    if (i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return;
+    const double *r = dens + (long)it * 9 * N;
+    const double tr = stat[2 * it];
+    const double a = r[p];
+    const long nb[3] = {p + 1, p + nx, p + (long)nx * ny};
+    for (int ax = 0; ax < 3; ++ax) {
+        const double b = r[nb[ax]];
+        if ((a >= tr && b <= tr) || (a <= tr && b >= tr)) {
+            double P[3] = {dx / 2.0 + dx * i, dx / 2.0 + dx * j, dx / 2.0 + dx * k};
+            P[ax] = P[ax] + dx * (tr - a) / (b - a);
+            const int slot = atomicAdd(&cnt[it], 1);
+            double *o = surf + ((long)it * 3 * N + slot) * 3;
+            o[0] = P[0];
+            o[1] = P[1];
+            o[2] = P[2];
+        }
+    }
+}
+
+// Step 3: signed distance to the nearest surface point and the Fermi
+// function 1 / (1 + exp((DeltaR - radd) / acut)).
+__global__ __launch_bounds__(kBlock) void k_cut_apply(const double *__restrict__ dens, int nx,
+                                                     int ny, int nz, double dx, double radd,
+                                                     double acut,
+                                                     const double *__restrict__ stat,
+                                                     const int *__restrict__ cnt,
+                                                     const double *__restrict__ surf,
+                                                     double *__restrict__ cut) {
+    __shared__ double sp[3 * kBlock];
+    const long N = (long)nx * ny * nz;
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int it = blockIdx.y;
+    if (stat[2 * it + 1] == 0.0) {
+        if (p < N) cut[(long)it * N + p] = 0.0;
+        return;
+    }
+    const int T = cnt[it];
+    const long pp = p < N ? p : 0;
+    const int i = pp % nx, j = (pp / nx) % ny, k = pp / ((long)nx * ny);
+    const double X = dx / 2.0 + dx * i, Y = dx / 2.0 + dx * j, Z = dx / 2.0 + dx * k;
+    const double *su = surf + (long)it * 3 * N * 3;
+    double dmin = 1e12;
+    for (int t0 = 0; t0 < T; t0 += kBlock) {
+        const int nt = min(kBlock, T - t0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < 3 * nt; q += kBlock) sp[q] = su[(long)t0 * 3 + q];
+        __syncthreads();
+        for (int l = 0; l < nt; ++l) {
+            const double ex = X - sp[3 * l], ey = Y - sp[3 * l + 1], ez = Z - sp[3 * l + 2];
+            const double d = ex * ex + ey * ey + ez * ez;
+            dmin = fmin(dmin, d);
+        }
+    }
+    if (p >= N) return;
+    const double sig = dens[(long)it * 9 * N + p] > stat[2 * it] ? -1.0 : 1.0;
+    cut[(long)it * N + p] = 1.0 / (1.0 + exp((sig * sqrt(dmin) - radd) / acut));
+}
+
+void launch_rutz_cutoff(const double *dens, int nx, int ny, int nz, double dx, double dv,
+                        double radd, double acut, double *stat, int *cnt, double *surf,
+                        double *cut, hipStream_t st) {
+    const long N = (long)nx * ny * nz;
+    const dim3 g((N + kBlock - 1) / kBlock, 2);
+    hipLaunchKernelGGL(k_cut_stats, dim3(2), dim3(1024), 0, st, dens, N, dv, stat, cnt);
+    hipLaunchKernelGGL(k_cut_surface, g, dim3(kBlock), 0, st, dens, nx, ny, nz, dx, stat, cnt,
+                       surf);
+    hipLaunchKernelGGL(k_cut_apply, g, dim3(kBlock), 0, st, dens, nx, ny, nz, dx, radd, acut,
+                       stat, cnt, surf, cut);
+}
+
+// <Q^2> = dv sum Y^2 rho per constraint and isospin (Calculate_Multipole,
+// Moments.f90:1943-1947): one workgroup per (constraint, isospin).
+__global__ __launch_bounds__(1024) void k_con_squared(const ConstraintK *__restrict__ con,
+                                                     const double *__restrict__ Y,
+                                                     const double *__restrict__ dens, long N,
+                                                     double dv, double *__restrict__ sq) {
+    __shared__ double sh[16];
+    const int c = blockIdx.x, it = blockIdx.y;
+    const double *y = Y + (long)con[c].idx * N, *r = dens + (long)it * 9 * N;
+    double a = 0;
+    for (long p = threadIdx.x; p < N; p += 1024) a += y[p] * y[p] * r[p];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < 16; ++k) t += sh[k];
+        sq[2 * c + it] = t * dv;
+    }
+}
+
+// The scalar part, one thread, constraints in the moment list's order:
+// kConCalc: first-call targets (CalculateAllMoments :2278-2301) and the
+// coefficients of the constraint field (CalcConstraintEnergy :2368-2531);
+// kConReadjust: intensity estimate and target update (Readjust
+// :2929-3058); kConTurnOff: switch-off after the given iteration
+// (TurnOffConstraints :3516-3549).  Always: the AlternateStep coefficients
+// 0.5 (<Q> - Q_0) / (<Q^2> + d0) (ImaginaryTime.f90:338-377).
+__global__ void k_con_update(ConstraintK *__restrict__ con, int nc, ConstraintCtl *ctl,
+                             const double *__restrict__ Q, const double *__restrict__ sq,
+                             const double *__restrict__ cstat, int mode, int iteration,
+                             double momentum, double *__restrict__ ce_coef,
+                             double *__restrict__ alt) {
+    if (threadIdx.x != 0) return;
+    const double d0 = (double)0.1f;      // a default-real literal (Moments.f90:258)
+    if (mode & kConCalc) {
+        if (ctl->first) {
+            for (int c = 0; c < nc; ++c) {
+                ConstraintK &k = con[c];
+                if (k.type == 0) continue;
+                const double v0 = Q[2 * k.idx], v1 = Q[2 * k.idx + 1];
+                if (k.iso == 1) { k.con[0] = k.con[1] = v0 + v1; }
+                else { k.con[0] = v0; k.con[1] = v1; }
+            }
+            ctl->first = 0;
+        }
+        const double damp = ctl->damping;
+        ctl->ce_nz = ctl->ce_nz && damp != 0.0;
+        for (int c = 0; c < nc; ++c) {
+            ConstraintK &k = con[c];
+            double fac[2], val[2], des[2];
+            const double v0 = Q[2 * k.idx], v1 = Q[2 * k.idx + 1];
+            if (k.type == 0) {
+                ce_coef[2 * c] = ce_coef[2 * c + 1] = 0.0;
+                continue;
+            }
+            if (k.iso == 1) {
+                fac[0] = fac[1] = 2.0 * k.inten[0];
+                val[0] = val[1] = v0 + v1;
+                des[0] = des[1] = k.con[0];
+            } else {
+                fac[0] = 2.0 * k.inten[0]; fac[1] = 2.0 * k.inten[1];
+                val[0] = v0; val[1] = v1;
+                des[0] = k.con[0]; des[1] = k.con[1];
+            }
+            if (ctl->ce_nz) { fac[0] = fac[0] * (1.0 - damp); fac[1] = fac[1] * (1.0 - damp); }
+            for (int it = 0; it < 2; ++it) {
+                const double cf = fac[it] * (val[it] - des[it]);
+                ce_coef[2 * c + it] = cf;
+                if (cf != 0.0 && cstat[2 * it + 1] != 0.0) ctl->ce_nz = 1;
+            }
+        }
+    }
+    if (mode & kConReadjust) {
+        for (int c = 0; c < nc; ++c) {
+            ConstraintK &k = con[c];
+            const double v0 = Q[2 * k.idx], v1 = Q[2 * k.idx + 1];
+            const double s0 = sq[2 * c], s1 = sq[2 * c + 1];
+            if (k.type == 1) {
+                if (k.iso == 1) {
+                    if (k.inten[0] == 0.0) {
+                        const double o2 = s0 + s1;         // O2(1:2) = sum(Squared)
+                        k.inten[0] = k.inten[1] = 2 / (o2 + o2);
+                        ctl->slowdown = 2 * k.inten[0];
+                    }
+                    k.con[0] = k.con[1] = k.con[0] - ctl->slowdown * ((v0 + v1) - k.truec[0]);
+                } else {
+                    k.con[0] = k.con[0] - ctl->slowdown * (v0 - k.truec[0]);
+                    k.con[1] = k.con[1] - ctl->slowdown * (v1 - k.truec[1]);
+                }
+            } else if (k.type == 2) {
+                if (k.inten[0] == 0.0) {
+                    if (k.iso == 1) {
+                        k.inten[0] = k.inten[1] = 2 / (s0 + s1);
+                        if (momentum != 0.0) k.inten[0] = k.inten[1] = 0.5 * k.inten[0];
+                    } else {
+                        k.inten[0] = 2 / s0;
+                        k.inten[1] = 2 / s1;
+                        if (momentum != 0.0) {
+                            k.inten[0] = 0.5 * k.inten[0];
+                            k.inten[1] = 0.5 * k.inten[1];
+                        }
+                    }
+                }
+                if (k.iso == 1) {
+                    k.con[0] = k.con[1] = k.con[0] - ((v0 + v1) - k.truec[0]);
+                } else {
+                    k.con[0] = k.con[0] - (v0 - k.truec[0]);
+                    k.con[1] = k.con[1] - (v1 - k.truec[1]);
+                }
+            }
+        }
+    }
+    if (mode & kConTurnOff)
+        for (int c = 0; c < nc; ++c) {
+            ConstraintK &k = con[c];
+            if (k.type != 0 && k.iteration > 0 && iteration > k.iteration) k.type = 0;
+        }
+    for (int c = 0; c < nc; ++c) {
+        const ConstraintK &k = con[c];
+        const double v0 = Q[2 * k.idx], v1 = Q[2 * k.idx + 1];
+        const double o2 = sq[2 * c] + sq[2 * c + 1];
+        for (int it = 0; it < 2; ++it) {
+            double a = 0.0;
+            if (k.type == 2) {
+                const double v = k.iso == 1 ? v0 + v1 : (it == 0 ? v0 : v1);
+                const double d = k.iso == 1 ? k.truec[0] : k.truec[it];
+                a = 0.5 * (v - d) / (o2 + d0);
+            }
+            alt[2 * c + it] = a;
+        }
+    }
+}
+
+// ConstraintEnergy = Damping * ConstraintEnergy + sum_c coef_c Y_c cutoff.
+__global__ __launch_bounds__(kBlock) void k_con_field(const ConstraintK *__restrict__ con,
+                                                     int nc, const double *__restrict__ Y,
+                                                     const double *__restrict__ coef,
+                                                     const ConstraintCtl *__restrict__ ctl,
+                                                     const double *__restrict__ cut, long N,
+                                                     double *__restrict__ ce) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= N) return;
+    const double damp = ctl->damping;
+    for (int it = 0; it < 2; ++it) {
+        double v = damp * ce[(long)it * N + p];
+        const double cu = cut[(long)it * N + p];
+        for (int c = 0; c < nc; ++c) v = v + coef[2 * c + it] * Y[(long)con[c].idx * N + p] * cu;
+        ce[(long)it * N + p] = v;
+    }
+}
+
+// AlternateStep's multipole field sum_c alt_c cutoff Y_c.
+__global__ __launch_bounds__(kBlock) void k_alt_field(const ConstraintK *__restrict__ con,
+                                                     int nc, const double *__restrict__ Y,
+                                                     const double *__restrict__ alt,
+                                                     const double *__restrict__ cut, long N,
+                                                     double *__restrict__ mult) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= N) return;
+    for (int it = 0; it < 2; ++it) {
+        double v = 0.0;
+        const double cu = cut[(long)it * N + p];
+        for (int c = 0; c < nc; ++c) v = v + alt[2 * c + it] * cu * Y[(long)con[c].idx * N + p];
+        mult[(long)it * N + p] = v;
+    }
+}
+
+// psi <- psi - multipole(isospin) psi (ImaginaryTime.f90:415-426).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_alt_apply(const T *__restrict__ in,
+                                                     T *__restrict__ out,
+                                                     const int *__restrict__ qn,
+                                                     const double *__restrict__ mult, long N) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int s = blockIdx.y;
+    if (p >= N) return;
+    const int it = (qn[4 * s] + 3) / 2 - 1;
+    const double mv = mult[(long)it * N + p];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const long q = ((long)s * 4 + c) * N + p;
+        const double v = (double)in[q];
+        out[q] = T(v - mv * v);
+    }
+}
+
+void launch_con_squared(const ConstraintK *con, int nc, const double *Y, const double *dens,
+                        long N, double dv, double *sq, hipStream_t st) {
+    hipLaunchKernelGGL(k_con_squared, dim3(nc, 2), dim3(1024), 0, st, con, Y, dens, N, dv, sq);
+}
+
+void launch_con_update(ConstraintK *con, int nc, ConstraintCtl *ctl, const double *Q,
+                       const double *sq, const double *cstat, int mode, int iteration,
+                       double momentum, double *ce_coef, double *alt, hipStream_t st) {
+    hipLaunchKernelGGL(k_con_update, dim3(1), dim3(64), 0, st, con, nc, ctl, Q, sq, cstat, mode,
+                       iteration, momentum, ce_coef, alt);
+}
+
+void launch_con_field(const ConstraintK *con, int nc, const double *Y, const double *coef,
+                      const ConstraintCtl *ctl, const double *cut, long N, double *ce,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_con_field, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, con,
+                       nc, Y, coef, ctl, cut, N, ce);
+}
+
+template <typename T>
+void launch_alternate(const ConstraintK *con, int nc, const double *Y, const double *alt,
+                      const double *cut, long N, double *mult, const T *in, T *out,
+                      const int *qn, int M, hipStream_t st) {
+    const unsigned gx = (unsigned)((N + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_alt_field, dim3(gx), dim3(kBlock), 0, st, con, nc, Y, alt, cut, N, mult);
+    hipLaunchKernelGGL(k_alt_apply<T>, dim3(gx, M), dim3(kBlock), 0, st, in, out, qn, mult, N);
+}
+template void launch_alternate<double>(const ConstraintK *, int, const double *, const double *,
+                                       const double *, long, double *, const double *,
+                                       double *, const int *, int, hipStream_t);
+template void launch_alternate<float>(const ConstraintK *, int, const double *, const double *,
+                                      const double *, long, double *, const float *, float *,
+                                      const int *, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // IterativeEstimation (ImaginaryTime.f90:28-211): power iteration of one

@@ -165,7 +165,7 @@ void build_potential_jobs(mocca_solve *h, std::vector<FieldJob> &jobs) {
 }
 
 // CalculateAllMoments + ConstructPotentials.
-int construct_impl(mocca_solve *h) {
+int construct_impl(mocca_solve *h, int passes = 1) {
     Scf &s = h->scf;
     const long N = h->N;
     double *dens = h->b.dens;
@@ -174,6 +174,25 @@ int construct_impl(mocca_solve *h) {
                        h->st);
     });
     if (rc) return rc;
+    if (s.ncon) {
+        // the constraint part of CalculateAllMoments (Moments.f90:2250-2334):
+        // cutoff, <Q^2>, targets and the constraint field; the start-up
+        // calls it three times (Main.f90:213-215)
+        rc = run(h, "cutoff", [&] {
+            launch_rutz_cutoff(dens, h->mesh.nx, h->mesh.ny, h->mesh.nz, h->mesh.dx, h->mesh.dv,
+                               s.radd, s.acut, s.cstat, s.ccnt, s.csurf, s.cut, h->st);
+        });
+        if (rc) return rc;
+        rc = run(h, "constraints", [&] {
+            launch_con_squared(s.con, s.ncon, s.Y, dens, N, h->mesh.dv, s.csq, h->st);
+            for (int k = 0; k < passes; ++k) {
+                launch_con_update(s.con, s.ncon, s.cctl, s.Q, s.csq, s.cstat, kConCalc, 0, 0.0,
+                                  s.ccoef, s.calt, h->st);
+                launch_con_field(s.con, s.ncon, s.Y, s.ccoef, s.cctl, s.cut, N, s.ce, h->st);
+            }
+        });
+        if (rc) return rc;
+    }
     if (s.sk.coulomb == 2) {
         rc = run(h, "coulomb_bc", [&] {
             launch_coulomb_bc(s.Q, s.nmom, s.lfac, s.bidx, s.bsh, s.brp, s.nb, s.sk.e2, s.cx,
@@ -216,7 +235,7 @@ int construct_impl(mocca_solve *h) {
     }
     rc = run(h, "potentials", [&] {
         launch_potentials(s.sk, N, h->mesh.time_reversal, dens, h->b.derived, s.vcoul, s.vcoulx,
-                          s.pot, h->st);
+                          s.ncon ? s.ce : nullptr, s.pot, h->st);
     });
     if (rc) return rc;
     rc = run(h, "pot_jobs", [&] { launch_field_jobs(h->md, s.pjobs, s.npjobs, h->st); });
@@ -279,6 +298,26 @@ int mix_impl(mocca_solve *h, int keep) {
     });
 }
 
+// ReadjustAllMoments + TurnOffConstraints(iteration) (Moments.f90:2336-2366,
+// 3516-3549), and the AlternateStep coefficients for the next iteration.
+int readjust_impl(mocca_solve *h, int iteration, double momentum) {
+    Scf &s = h->scf;
+    if (!s.ncon) return 0;
+    for (size_t c = 0; c < s.htype.size(); ++c)
+        if (s.htype[c] != 0 && s.hiter[c] > 0 && iteration > s.hiter[c]) s.htype[c] = 0;
+    return run(h, "constraints", [&] {
+        launch_con_update(s.con, s.ncon, s.cctl, s.Q, s.csq, s.cstat,
+                          kConReadjust | kConTurnOff, iteration, momentum, s.ccoef, s.calt,
+                          h->st);
+    });
+}
+
+bool alternating(const mocca_solve *h) {
+    for (int t : h->scf.htype)
+        if (t == 2) return true;
+    return false;
+}
+
 int check_gs_status(mocca_solve *h) {
     int st = 0;
     HIPCHK(h, hipMemcpyAsync(&st, h->b.status, sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -315,9 +354,19 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
     int rc;
     for (int k = 1; k <= niter; ++k) {
         double *prev = s.erows + (size_t)(k - 1) * 43, *row = s.erows + (size_t)k * 43;
+        const int iter = first_iteration + k - 1;
+        // AlternateStep when alternating constraints were on after the last
+        // SolvePairing (Main.f90:262-264); the previous row's kinetic energy
+        // then comes from the states before it
+        const bool alt = s.alt_next;
+        if (alt) {
+            if ((rc = kinetic_any(h))) return rc;
+            if ((rc = energy_kinetic_impl(h, prev))) return rc;
+            if ((rc = alternate_any(h))) return rc;
+        }
         if (estimate) {
             double out[6];
-            if ((rc = estimate_any(h, first_iteration + k - 1, dt, momentum, out))) return rc;
+            if ((rc = estimate_any(h, iter, dt, momentum, out))) return rc;
             dt = out[0];
             dth = dt / kHbar;
             momentum = out[1];
@@ -329,11 +378,13 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
         if ((rc = grad_desc_any(h, dth, momentum))) return rc;
         // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
         // previous row's densities came from: that row's kinetic energy
-        if ((rc = energy_kinetic_impl(h, prev))) return rc;
+        if (!alt && (rc = energy_kinetic_impl(h, prev))) return rc;
         if ((rc = fill_impl(h))) return rc;
+        s.alt_next = alternating(h);
         if ((rc = density_any(h))) return rc;
         if ((rc = mix_impl(h, 1))) return rc;
         if ((rc = construct_impl(h))) return rc;
+        if ((rc = readjust_impl(h, iter, momentum))) return rc;
         if ((rc = energy_terms_impl(h, row))) return rc;
     }
     // kinetic energy of the final states: one hPsi-type pass with no outputs
@@ -360,7 +411,9 @@ void scf_free(mocca_solve *h) {
     void *ptrs[] = {s.hist, s.Y, s.mpart, s.Q, s.lfac, s.bidx, s.bsh, s.brp, s.cx, s.cr, s.cpa,
                     s.cpb, s.cap, s.cpart, s.vcoul, s.vcoulx, s.cg_iters, s.pjobs, s.epart,
                     s.ecur, s.erows, s.cq[0], s.cq[1], s.cq[2], s.clam[0], s.clam[1],
-                    s.clam[2], s.cw0, s.cw1, s.pold, s.pcwork, s.pcjobs, s.pc_iters};
+                    s.clam[2], s.cw0, s.cw1, s.pold, s.pcwork, s.pcjobs, s.pc_iters, s.con,
+                    s.cctl, s.csq, s.ccoef, s.calt, s.cut, s.ce, s.mult, s.cstat, s.ccnt,
+                    s.csurf, s.alt_old};
     for (void *p : ptrs) free_ptr(p);
     if (s.pot && s.pot != (double *)h->b.fields) (void)hipFree(s.pot);
     s = Scf{};
@@ -663,8 +716,9 @@ int mocca_scf_start(mocca_solve *h) {
     if ((rc = fill_impl(h))) return rc;
     if ((rc = density_any(h))) return rc;
     if ((rc = save_history(h))) return rc;
-    if ((rc = construct_impl(h))) return rc;
+    if ((rc = construct_impl(h, 3))) return rc;
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
+    h->scf.alt_next = alternating(h);
     h->scf.started = true;
     return 0;
 }
@@ -727,6 +781,128 @@ int mocca_get_coulomb(mocca_solve *h, double *vcoul, double *vcoulx, double *cpo
     if (cg_iterations)
         HIPCHK(h, hipMemcpy(cg_iterations, s.cg_iters, sizeof(int), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int mocca_set_constraints(mocca_solve *h, const mocca_constraint *c, int count, double radd,
+                          double acut, double damping, double readjust_slowdown) {
+    if (int rc = check_scf(h)) return rc;
+    if (count < 0 || (count > 0 && !c)) return fail(h, MOCCA_ERR_ARG, "bad constraint list");
+    if (!(acut > 0.0)) return fail(h, MOCCA_ERR_ARG, "acut must be positive");
+    Scf &s = h->scf;
+    const long N = h->N;
+    std::vector<ConstraintK> ck;
+    for (int k = 0; k < count; ++k) {
+        const mocca_constraint &q = c[k];
+        int idx = -1;
+        for (int j = 0; j < s.nmom; ++j)
+            if (s.lm[2 * j] == q.l && s.lm[2 * j + 1] == q.m) idx = j;
+        if (idx < 0)
+            return fail(h, MOCCA_ERR_UNSUPPORTED,
+                        "no moment (l=%d, m=%d) under the symmetries (even l, even m <= l <= "
+                        "max_moment)", q.l, q.m);
+        if (q.type != 1 && q.type != 2)
+            return fail(h, MOCCA_ERR_UNSUPPORTED, "ConstraintType %d (1 or 2)", q.type);
+        if (q.isoswitch != 1 && q.isoswitch != 2)
+            return fail(h, MOCCA_ERR_UNSUPPORTED, "Isoswitch %d (1 or 2)", q.isoswitch);
+        for (const ConstraintK &o : ck)
+            if (o.idx == idx)
+                return fail(h, MOCCA_ERR_ARG, "two constraints on (l=%d, m=%d)", q.l, q.m);
+        ConstraintK r{};
+        r.idx = idx;
+        r.type = q.type;
+        r.iso = q.isoswitch;
+        r.iteration = q.iteration;
+        // Constraint / sqrt(4 pi) for the monopole (Moments.f90:3719-3720)
+        const double f = q.l == 0 ? 1.0 / std::sqrt(4.0 * kPi) : 1.0;
+        if (q.isoswitch == 1) {
+            r.con[0] = r.con[1] = q.value[0] * f;
+        } else {
+            r.con[0] = q.value[0];
+            r.con[1] = q.value[1];
+        }
+        r.truec[0] = r.con[0];
+        r.truec[1] = r.con[1];
+        r.inten[0] = r.inten[1] = q.intensity;
+        ck.push_back(r);
+    }
+    // the order of the moment list (IniMoments, Moments.f90:372-485)
+    std::sort(ck.begin(), ck.end(),
+              [](const ConstraintK &a, const ConstraintK &b) { return a.idx < b.idx; });
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    void *old[] = {s.con, s.cctl, s.csq, s.ccoef, s.calt, s.cut, s.ce, s.mult, s.cstat, s.ccnt,
+                   s.csurf};
+    for (void *p : old) free_ptr(p);
+    s.con = nullptr; s.cctl = nullptr; s.csq = s.ccoef = s.calt = nullptr;
+    s.cut = s.ce = s.mult = s.cstat = s.csurf = nullptr; s.ccnt = nullptr;
+    s.ncon = (int)ck.size();
+    s.htype.clear();
+    s.hiter.clear();
+    if (!s.ncon) return 0;
+    for (const ConstraintK &r : ck) {
+        s.htype.push_back(r.type);
+        s.hiter.push_back(r.iteration);
+    }
+    const ConstraintCtl ctl{damping, readjust_slowdown, 1, 0};
+    s.radd = radd;
+    s.acut = acut;
+    int rc;
+    if ((rc = dalloc(h, &s.con, ck.size() * sizeof(ConstraintK))) ||
+        (rc = dalloc(h, &s.cctl, sizeof(ConstraintCtl))) ||
+        (rc = dalloc(h, &s.csq, 2 * ck.size() * sizeof(double))) ||
+        (rc = dalloc(h, &s.ccoef, 2 * ck.size() * sizeof(double))) ||
+        (rc = dalloc(h, &s.calt, 2 * ck.size() * sizeof(double))) ||
+        (rc = dalloc(h, &s.cut, 2 * N * sizeof(double))) ||
+        (rc = dalloc(h, &s.ce, 2 * N * sizeof(double))) ||
+        (rc = dalloc(h, &s.mult, 2 * N * sizeof(double))) ||
+        (rc = dalloc(h, &s.cstat, 4 * sizeof(double))) ||
+        (rc = dalloc(h, &s.ccnt, 2 * sizeof(int))) ||
+        (rc = dalloc(h, &s.csurf, 2 * 3 * (size_t)N * 3 * sizeof(double))))
+        return rc;
+    HIPCHK(h, hipMemcpy(s.con, ck.data(), ck.size() * sizeof(ConstraintK), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(s.cctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(s.ce, 0, 2 * N * sizeof(double)));
+    HIPCHK(h, hipMemset(s.cut, 0, 2 * N * sizeof(double)));
+    HIPCHK(h, hipMemset(s.csq, 0, 2 * ck.size() * sizeof(double)));
+    HIPCHK(h, hipMemset(s.calt, 0, 2 * ck.size() * sizeof(double)));
+    HIPCHK(h, hipMemset(s.ccoef, 0, 2 * ck.size() * sizeof(double)));
+    return 0;
+}
+
+int mocca_get_constraints(mocca_solve *h, double *out) {
+    if (int rc = check_scf(h)) return rc;
+    Scf &s = h->scf;
+    if (!s.ncon) return 0;
+    if (!out) return fail(h, MOCCA_ERR_ARG, "null output");
+    std::vector<ConstraintK> ck(s.ncon);
+    std::vector<double> sq(2 * s.ncon);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(ck.data(), s.con, ck.size() * sizeof(ConstraintK), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(sq.data(), s.csq, sq.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < s.ncon; ++c) {
+        double *o = out + 10 * c;
+        o[0] = s.lm[2 * ck[c].idx];
+        o[1] = s.lm[2 * ck[c].idx + 1];
+        o[2] = ck[c].type;
+        o[3] = ck[c].con[0];
+        o[4] = ck[c].con[1];
+        o[5] = ck[c].inten[0];
+        o[6] = ck[c].inten[1];
+        o[7] = sq[2 * c];
+        o[8] = sq[2 * c + 1];
+        o[9] = ck[c].iso;
+    }
+    return 0;
+}
+
+int mocca_alternate_step(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    if (!h->scf.ncon) return 0;
+    return alternate_any(h);
+}
+
+int mocca_readjust_constraints(mocca_solve *h, int iteration, double momentum) {
+    if (int rc = check_scf(h)) return rc;
+    return readjust_impl(h, iteration, momentum);
 }
 
 int mocca_get_potentials(mocca_solve *h, double *U, double *B, double *NablaB, double *W,

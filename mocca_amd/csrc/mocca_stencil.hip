@@ -50,13 +50,17 @@ __device__ __forceinline__ void block_sum3_fd(double &a, double &b, double &c, d
 // at offsets 0 .. nx+2G-1: the G left cells are the mirrored reflection
 // ghosts (with the component's sign), the G right cells are zeros, so the
 // x stencils read without branches.
-template <typename T, int HMAX, bool TRC>
+//
+// STALE: the values come from vsrc instead of psi, whose stencils give the
+// derivatives (the first GradDesc after an AlternateStep, which changes the
+// values but not the stored derivatives, ImaginaryTime.f90:306-437).
+template <typename T, int HMAX, bool TRC, bool STALE>
 __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
     const int *__restrict__ qn, const Task *__restrict__ tasks, int TY,
     int ntile_y, T kin0, T kin1, T dth, T mu, int mode,
-    double *__restrict__ eparts) {
+    double *__restrict__ eparts, const T *__restrict__ vsrc) {
     constexpr int G = 4;  // ghost width; HMAX = halo loads per thread (32 nx / 256)
     constexpr bool trc = TRC;
     extern __shared__ double smem_d[];
@@ -170,6 +174,12 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
             // spin-orbit sum aw (ActionOfW, :1029-1065) and to the current
             // term aa (ActionOfA, :1107-1129).
             T h[4] = {0, 0, 0, 0}, aw[4] = {0, 0, 0, 0}, aa[4] = {0, 0, 0, 0};
+            T val[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if constexpr (STALE) val[c] = vsrc[(long)s * 4 * N + c * N + p];
+                else val[c] = q[c][4];
+            }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const T *row = sp + (c * rows + tyl + 4) * W + tx + G;
@@ -181,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
                 for (int k = -4; k <= 4; ++k) w[k + 4] = row[k * W];
                 fd_pair(m, w, dyc, ly);
                 fd_pair(m, q[c], dzc, lz);
-                const T lc = lx + ly + lz, vc = q[c][4];
+                const T lc = lx + ly + lz, vc = val[c];
                 kacc += (double)vc * (double)lc;
                 h[c] += -fb * lc - (nb0 * dxc + nb1 * dyc + nb2 * dzc) + fu * vc;
                 const T ex = fw[0][0] * dxc + fw[0][1] * dyc + fw[0][2] * dzc + hx * vc;
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
             if (!trc) { h[0] += aa[1]; h[1] -= aa[0]; h[2] += aa[3]; h[3] -= aa[2]; }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const T vc = q[c][4];
+                const T vc = val[c];
                 eacc += (double)vc * (double)h[c];
                 dacc += (double)h[c] * (double)h[c];
             }
@@ -215,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
             if (mode & kStep) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const T vc = q[c][4];
+                    const T vc = val[c];
                     po[c * N + p] = vc + (-(dth * h[c]) + mu * uo[c]);
                     us[c * N + p] = vc;
                 }
@@ -244,28 +254,32 @@ template <typename T>
 void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
                     const T *fields, const int *qn, const Task *tasks, int ntasks,
                     int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
-                    int mode, double *eparts, hipStream_t st) {
+                    int mode, double *eparts, hipStream_t st, const T *vsrc) {
     const size_t shm = sizeof(T) * 4 * (size_t)(TY + 8) * (m.nx + 8);
-#define MOCCA_HPSI_LAUNCH(HM, TR)                                                           \
-    hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR>), dim3(ntasks), dim3(kBlock), shm, st, m, psi,  \
-                       psi_out, upd, hout, fields, qn, tasks, TY, ntile_y, kin0, kin1, dth, \
-                       mu, mode, eparts)
-    if (m.nx <= 32) {
-        if (trc) MOCCA_HPSI_LAUNCH(4, true); else MOCCA_HPSI_LAUNCH(4, false);
+#define MOCCA_HPSI_LAUNCH(HM, TR, ST)                                                         \
+    hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, ST>), dim3(ntasks), dim3(kBlock), shm, st, m,   \
+                       psi, psi_out, upd, hout, fields, qn, tasks, TY, ntile_y, kin0, kin1,  \
+                       dth, mu, mode, eparts, vsrc)
+#define MOCCA_HPSI_TRC(HM, ST)                                                                \
+    if (trc) MOCCA_HPSI_LAUNCH(HM, true, ST); else MOCCA_HPSI_LAUNCH(HM, false, ST)
+    if (vsrc) {
+        if (m.nx <= 32) { MOCCA_HPSI_TRC(4, true); } else { MOCCA_HPSI_TRC(8, true); }
     } else {
-        if (trc) MOCCA_HPSI_LAUNCH(8, true); else MOCCA_HPSI_LAUNCH(8, false);
+        if (m.nx <= 32) { MOCCA_HPSI_TRC(4, false); } else { MOCCA_HPSI_TRC(8, false); }
     }
+#undef MOCCA_HPSI_TRC
 #undef MOCCA_HPSI_LAUNCH
 }
 
 template void launch_hpsi_fd<double>(const MeshK<double> &, const double *, double *,
                                      double *, double *, const double *, const int *,
                                      const Task *, int, int, int, int, double, double,
-                                     double, double, int, double *, hipStream_t);
+                                     double, double, int, double *, hipStream_t,
+                                     const double *);
 template void launch_hpsi_fd<float>(const MeshK<float> &, const float *, float *, float *,
                                     float *, const float *, const int *, const Task *, int,
                                     int, int, int, float, float, float, float, int,
-                                    double *, hipStream_t);
+                                    double *, hipStream_t, const float *);
 
 
 // ---------------------------------------------------------------------------

@@ -44,6 +44,12 @@ module mocca_amd_c
     real(c_double) :: preconu
   end type
 
+  type, bind(C) :: mocca_constraint
+    integer(c_int) :: l, m, type, isoswitch, iteration, pad_
+    real(c_double) :: value(2)
+    real(c_double) :: intensity
+  end type
+
   interface
     integer(c_int) function mocca_solve_create(mesh, nwt, precision, h) bind(C)
       import :: c_int, c_ptr, mocca_mesh
@@ -196,6 +202,29 @@ module mocca_amd_c
       integer(c_int), value :: first_iteration, niter
       real(c_double), value :: dt, momentum
       real(c_double), intent(out) :: energies(*), params(*)
+    end function
+    integer(c_int) function mocca_set_constraints(h, c, count, radd, acut, damping,        &
+    &                                             readjust_slowdown) bind(C)
+      import :: c_int, c_ptr, c_double, mocca_constraint
+      type(c_ptr), value :: h
+      type(mocca_constraint), intent(in) :: c(*)
+      integer(c_int), value :: count
+      real(c_double), value :: radd, acut, damping, readjust_slowdown
+    end function
+    integer(c_int) function mocca_readjust_constraints(h, iteration, momentum) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: iteration
+      real(c_double), value :: momentum
+    end function
+    integer(c_int) function mocca_alternate_step(h) bind(C)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function mocca_get_constraints(h, out) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
     end function
     integer(c_int) function mocca_get_moments(h, lm, q, count) bind(C)
       import :: c_int, c_ptr, c_double

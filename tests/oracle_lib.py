@@ -597,4 +597,7 @@ def scf_run(g, niter):
                    kinetic_sums(psi, lap, qn, occ, dv))
         traj.append(rows(E))
     return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
-                vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params))
+                vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params),
+                constraints=[] if cons is None else
+                [dict(type=c["type"], con=c["con"].copy(), inten=c["inten"].copy(),
+                      sq=c["sq"].copy()) for c in cons.items])

@@ -26,6 +26,19 @@ def _functional(s, g, coulomb=1):
                 mixing_scheme=m.get("mixing_scheme", 0))
 
 
+def _constraints(m):
+    """The reference run's MomentConstraint namelists as library constraints."""
+    out = []
+    for text in m.get("constraints", []):
+        p = ol.parse_namelist_items(text)
+        out.append(dict(l=p["l"], m=p.get("m", 0), type=p.get("constrainttype", 2),
+                        isoswitch=p.get("isoswitch", 1), iteration=p.get("iteration", -1),
+                        value=p.get("constraint", 0.0), value_n=p.get("constraintneutrons", 0.0),
+                        value_p=p.get("constraintprotons", 0.0),
+                        intensity=p.get("intensity", 0.0)))
+    return out
+
+
 def _solve(g, occ, precision=mocca_amd.FP64, coulomb=1):
     m = g["meta"]
     s = mocca_amd.Solve(m["nx"], m["ny"], m["nz"], m["dx"], g["consts"][6], m["nwt"],
@@ -33,6 +46,8 @@ def _solve(g, occ, precision=mocca_amd.FP64, coulomb=1):
     s.set_states_meta(g["qn"], occ)
     s.upload_states(g["psi"])
     _functional(s, g, coulomb)
+    if m.get("constraints"):
+        s.set_constraints(_constraints(m))
     return s
 
 
@@ -202,6 +217,37 @@ def test_iterate_potential_preconditioning_vs_reference(case, coulomb):
     assert relerr(pot["U"].reshape(-1), g["final_Upot"].reshape(-1)) < tol
     if not m["trc"]:
         assert relerr(pot["S"].reshape(-1), g["final_Spot"].reshape(-1)) < tol
+    s.close()
+
+
+@pytest.mark.parametrize("case", ["scfcon_trc_fd", "scfcon_todd_lag"])
+def test_iterate_constraints_vs_reference(case):
+    """Multipole constraints in the loop: AlternateStep (then GradDesc with
+    the derivatives of the states before it), the Rutz cutoff, the
+    constraint field in U, readjusted targets and the switch-off."""
+    g = golden(case)
+    m = g["meta"]
+    n = m["niter"]
+    print(f"scf loop with constraints: case {case} niter {n}", flush=True)
+    s = _solve(g, np.zeros(m["nwt"]))
+    s.scf_start()
+    rows = s.iterate(n, m["dt"] / g["consts"][5], m["momentum"])
+    traj = _traj_cols(rows)
+    assert np.max(np.abs(traj[:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(traj, g["traj"]) < 1e-10
+    psi = s.download_states()
+    assert relerr(psi.reshape(-1), g["final_psi"].reshape(-1)) < 1e-9
+    pot = s.potentials()
+    assert relerr(pot["U"].reshape(-1), g["final_Upot"].reshape(-1)) < 1e-9
+    # targets, intensities and <Q^2> after the run, against the oracle's
+    st = s.constraints()
+    r = ol.scf_run(g, n)
+    assert st.shape[0] == len(r["constraints"]) == len(m["constraints"])
+    for row, c in zip(st, r["constraints"]):
+        assert row[2] == c["type"]
+        assert relerr(row[3:5], c["con"]) < 1e-10
+        assert relerr(row[5:7], c["inten"]) < 1e-10
+        assert relerr(row[7:9], c["sq"]) < 1e-10
     s.close()
 
 
