@@ -24,7 +24,9 @@
 ! With ParameterEstimation = 1 the scf task reads maxspwf's start grid from
 ! seed.bin (real8 grid(nx,ny,nz,4)) in place of IterativeEstimation's
 ! random_number start, and writes dt, momentum after each GradDesc to
-! <prefix>_params.bin.
+! <prefix>_params.bin.  An optional energies.bin (real8 e(nwt)) gives the
+! single-particle energies of the start states, fermi.bin (real8 (2)) the
+! start Fermi energies, as a wavefunction file would provide them.
 !-------------------------------------------------------------------------------
 program ref_driver
   use CompilationInfo
@@ -42,13 +44,17 @@ program ref_driver
   use Coulomb
   use Energy
   use PotentialMixing
-  use Pairing, only : SolvePairing
+  use Pairing, only : SolvePairing, Fermi, PairingEnergy, PairingType, Delta
+  use PairingInteraction, only : PairingStrength, Alpha, DDexp, PairingCut, PairingMu,   &
+  &                              RhoSat, CutType, DensityFactor, PCutoffs
+  use HFB, only : HFBMix, HFBIter, RhoHFB, KappaHFB, blocksizes, QuasiEnergies
   use DensityMixing, only : MixDensities
   implicit none
 
   character(len=512) :: infile, prefix, tasks, arg4
-  integer            :: nwtf, i, m, it, uu, n, niter, up
-  logical            :: altcheck
+  integer            :: nwtf, i, m, it, uu, n, niter, up, upp
+  logical            :: altcheck, havee
+  real(KIND=dp), allocatable :: estart(:)
   integer, allocatable :: qn(:,:)
   real(KIND=dp), allocatable :: occ(:), grids(:,:,:,:,:)
   type(Spinor)       :: sp, h
@@ -100,6 +106,8 @@ program ref_driver
     &        B12a,B12b,B13a,B13b,B14,B15,B16,B17,B18,B19,B20,B21
     write(uu) hbm, nucleonmass, e2, hbar, dv, Neutrons, Protons
     write(uu) COM1Body
+    write(uu) PairingType, CutType, HFBIter
+    write(uu) PairingStrength, Alpha, DDexp, PairingCut, PairingMu, RhoSat, HFBMix
     close(uu)
   endif
 
@@ -178,6 +186,25 @@ program ref_driver
     ! Start-up and loop of Evolve (Main.f90:172-225, 255-328) for HF without
     ! constraints or cranking, ParameterEstimation = 0.
     EvolveSpwf => GradDesc
+    ! Optional single-particle energies of the start states (energies.bin,
+    ! real8 e(nwt)), as a wavefunction file would provide them.
+    inquire(file='energies.bin', exist=havee)
+    if(havee) then
+      allocate(estart(nwt))
+      open(newunit=up, file='energies.bin', access='stream', form='unformatted', status='old')
+      read(up) estart
+      close(up)
+      do i=1,nwt
+        call HFBasis(i)%SetEnergy(estart(i))
+      enddo
+    endif
+    ! Optional start Fermi energies (fermi.bin, real8 Fermi(2)).
+    inquire(file='fermi.bin', exist=havee)
+    if(havee) then
+      open(newunit=up, file='fermi.bin', access='stream', form='unformatted', status='old')
+      read(up) Fermi
+      close(up)
+    endif
     if(ParameterEstimation.eq.1) then
       ! maxspwf as the first call of IterativeEstimation makes it
       ! (ImaginaryTime.f90:58-70), from the seed grid instead of random_number;
@@ -194,6 +221,13 @@ program ref_driver
       &    form='unformatted', status='replace')
     endif
     call SolvePairing
+    if(PairingType.eq.2) then
+      ! the start-up pairing step: gaps, kappa, rho, density factor, cutoffs
+      open(newunit=up, file=trim(prefix)//'_hfbstart.bin', access='stream',    &
+      &    form='unformatted', status='replace')
+      write(up) dble(Delta), dble(KappaHFB), dble(RhoHFB), DensityFactor, PCutoffs
+      close(up)
+    endif
     call DeriveAll
     call UpdateDensities(0)
     call CalculateAllMoments(1)
@@ -205,6 +239,9 @@ program ref_driver
     &    form='unformatted', status='replace')
     write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,        &
     &        sum(CoMCorrection), Skyrmeterms
+    open(newunit=upp, file=trim(prefix)//'_pairing.bin', access='stream',       &
+    &    form='unformatted', status='replace')
+    write(upp) Fermi, PairingEnergy
     altcheck = CheckForAlternateConstraints()
     do it=1,niter
       if(altcheck) call AlternateStep()
@@ -222,8 +259,10 @@ program ref_driver
       call CompEnergy
       write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,      &
       &        sum(CoMCorrection), Skyrmeterms
+      write(upp) Fermi, PairingEnergy
     enddo
     close(uu)
+    close(upp)
     if(ParameterEstimation.eq.1) close(up)
     open(newunit=uu, file=trim(prefix)//'_scfstates.bin', access='stream',      &
     &    form='unformatted', status='replace')

@@ -114,7 +114,7 @@ CASES = {
 
 
 def namelists(nx, ny, nz, dx, trc, lag, nwt, neutrons, protons, dt, mom, mixing=0, pe=0,
-              preconw=0.0, constraints=()):
+              preconw=0.0, constraints=(), pairing='Type="HF"'):
     fd = "-1" if lag else "3"
     fdl = "-1" if lag else "4"
     return f"""&GenInfo
@@ -129,7 +129,7 @@ nwt={nwt}
 MixingScheme={mixing}, PreConW={preconw}
 /
 &Pairing
-Type="HF"
+{pairing}
 /
 &Derivatives
 MaxFDOrder={fd}, MaxFDLapOrder={fdl}, BStack=.false.
@@ -309,6 +309,11 @@ SCF_CASES = {
                         dict(constraints=["l=2, m=0, Isoswitch=2, ConstraintNeutrons=9.0, "
                                           "ConstraintProtons=7.0",
                                           "l=2, m=2, ConstraintType=1, Constraint=2.0"])),
+    # HFB pairing (the sample input's pairing namelist): 4 + 4 particles in
+    # 6 + 6 states and their time-reversed partners
+    "scfhfb_trc_fd": (10, 0.8, True, False, 6, 4, 6, 10, 0,
+                      dict(pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
+                                   'GuessKappa=.true.', start="hf")),
 }
 
 
@@ -317,6 +322,8 @@ def run_scf(name):
     opts = SCF_CASES[name][9] if len(SCF_CASES[name]) > 9 else {}
     mixing, preconw = opts.get("mixing", 0), opts.get("preconw", 0.0)
     constraints = opts.get("constraints", ())
+    pairing = opts.get("pairing", 'Type="HF"')
+    e0 = f0 = None
     seed = sum(map(ord, name))
     rng = np.random.default_rng(seed)
     pattern = [(1, 1), (-1, 1), (1, 1), (-1, 1), (1, 1), (-1, 1)] if trc else \
@@ -341,19 +348,45 @@ def run_scf(name):
     os.makedirs(wd, exist_ok=True)
     if not os.path.exists(os.path.join(wd, "forces.param")):
         os.symlink(FORCES, os.path.join(wd, "forces.param"))
+    for fn in ("energies.bin", "fermi.bin"):
+        if os.path.exists(os.path.join(wd, fn)):
+            os.remove(os.path.join(wd, fn))
+    extra = {}
     with open(os.path.join(wd, "input.bin"), "wb") as fh:
         np.array([nwt], "<i4").tofile(fh)
         np.array(qns, "<i4").tofile(fh)
         np.array(occ, "<f8").tofile(fh)
         psi.astype("<f8").tofile(fh)
-    extra = {}
     if pe:
         # stands in for random_number in IterativeEstimation's first call
         seed_grid = rng.uniform(0.0, 1.0, size=(4, n, n, n))
         seed_grid.astype("<f8").tofile(os.path.join(wd, "seed.bin"))
         extra["est_seed"] = seed_grid
+    if opts.get("start") == "hf":
+        # Start data as a wavefunction file holds them: the single-particle
+        # energies <h> of the start states (the energies of a one-step HF run)
+        # and Fermi energies halfway between the last filled and the first
+        # empty level of the HF filling.
+        nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom)
+        with open(os.path.join(wd, "input.nml"), "w") as fh:
+            fh.write(nml)
+        with open(os.path.join(wd, "input.nml")) as fin:
+            subprocess.run([DRIVER, "input.bin", "hf1", "scf", "1"], cwd=wd, stdin=fin,
+                           check=True, stdout=subprocess.DEVNULL)
+        st1 = np.fromfile(os.path.join(wd, "hf1_scfstates.bin"), "<f8")
+        e0 = st1[nwt * 4 * n ** 3:].reshape(nwt, 2)[:, 0].copy()
+        f0 = []
+        for iso, npart in ((-1, neutrons), (1, protons)):
+            es = np.sort([e for e, q in zip(e0, qns) if q[0] == iso])
+            k = int(npart // 2 if trc else npart)
+            f0.append(0.5 * (es[k - 1] + es[k]))
+        f0 = np.array(f0)
+        e0.astype("<f8").tofile(os.path.join(wd, "energies.bin"))
+        f0.astype("<f8").tofile(os.path.join(wd, "fermi.bin"))
+        extra["start_energies"] = e0
+        extra["start_fermi"] = f0
     nml = namelists(n, n, n, dx, trc, lag, nwt, neutrons, protons, dt, mom, pe=pe,
-                    mixing=mixing, preconw=preconw, constraints=constraints)
+                    mixing=mixing, preconw=preconw, constraints=constraints, pairing=pairing)
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
     with open(os.path.join(wd, "input.nml")) as fin:
@@ -372,13 +405,31 @@ def run_scf(name):
     coef = np.frombuffer(fb[:28 * 8], "<f8")
     rest = np.frombuffer(fb[28 * 8:37 * 8], "<f8")
     com1 = int(np.frombuffer(fb[37 * 8:37 * 8 + 4], "<i4")[0])
+    ptype, cuttype, hfbiter = (int(x) for x in np.frombuffer(fb[37 * 8 + 4:37 * 8 + 16], "<i4"))
+    pp = np.frombuffer(fb[37 * 8 + 16:], "<f8")
+    if ptype:
+        extra["pairing"] = np.fromfile(os.path.join(wd, "out_pairing.bin"),
+                                       "<f8").reshape(niter + 1, 4)
+        raw = np.fromfile(os.path.join(wd, "out_hfbstart.bin"), "<f8")
+        S = 2 * nwt if trc else nwt
+        k = 0
+        for key, cnt in (("delta", S * S * 4), ("kappa", S * S * 4), ("rho", S * S * 4),
+                         ("dfac", 2 * n ** 3), ("pcut", nwt)):
+            v = raw[k:k + cnt]
+            k += cnt
+            extra["start_" + key] = (v.reshape(2, 2, S, S) if cnt == S * S * 4 else
+                                     v.reshape(2, n ** 3) if key == "dfac" else v)
+        assert k == raw.size
     meta = dict(nx=n, ny=n, nz=n, dx=dx, trc=trc, lagrange=lag, nwt=nwt, dt=dt,
                 momentum=mom, neutrons=neutrons, protons=protons, niter=niter,
                 com1body=com1, seed=seed,
                 mixing=("scheme 0, DampingParam 0.75" if mixing == 0 else
                         f"scheme {mixing}, PreConW {preconw}"),
                 mixing_scheme=mixing, preconw=preconw, parameter_estimation=pe,
-                constraints=list(constraints),
+                constraints=list(constraints), pairing_type=ptype, cut_type=cuttype,
+                hfb_iter=hfbiter, pairing_strength=list(pp[0:2]), pairing_alpha=list(pp[2:4]),
+                pairing_ddexp=list(pp[4:6]), pairing_cut=list(pp[6:8]),
+                pairing_mu=list(pp[8:10]), rhosat=float(pp[10]), hfb_mix=float(pp[11]),
                 source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")
     np.savez_compressed(os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
                         qn=np.array(qns, np.int32), psi=psi, traj=traj,

@@ -535,11 +535,35 @@ def scf_run(g, niter):
 
     scheme = m.get("mixing_scheme", 0)
     preconw = m.get("preconw", 0.0)
-    occ = hf_fill(np.zeros(M), qn, m["neutrons"], m["protons"], trc)
-    d, der, lap = all_densities(mesh, psi, qn, occ, trc)
+    hfb = HFB(qn, trc, m) if m.get("pairing_type", 0) == 2 else None
+    e_start = np.asarray(g["start_energies"]) if "start_energies" in g else np.zeros(M)
+    pairs = []
+
+    def pairing_step(psi, e_sp, rho):
+        """SolvePairing: HF filling, or HFB with the canonical basis as the
+        density basis (Pairing.f90:812)."""
+        if hfb is None:
+            return psi, hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
+        can, occ_c = hfb.solve(psi.reshape(M, 4, N), e_sp, rho, dv)
+        return can, occ_c
+
+    if hfb is not None:
+        hfb.guess()
+        if "start_fermi" in g:
+            hfb.fermi = np.array(g["start_fermi"], dtype=float)
+    dbasis, occ = pairing_step(psi, e_start, np.zeros((2, N)))
+    d, der, lap = all_densities(mesh, dbasis, qn, occ, trc)
     F, vc, vx = construct(d, calc=3)      # CalculateAllMoments three times (Main.f90:213-215)
     E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
-               kinetic_sums(psi, lap, qn, occ, dv))
+               kinetic_sums(dbasis, lap, qn, occ, dv))
+    if hfb is not None:
+        ep = hfb.energy()
+        E[0] += ep.sum()
+        pairs.append(np.concatenate([hfb.fermi, ep]))
+        der = np.zeros((M, 3, 4, N))
+        lap = np.zeros((M, 4, N))
+        for i in range(M):
+            der[i], lap[i] = mesh.derive_spinor(psi[i], qn[i][1], qn[i][2], qn[i][3])
     traj = [rows(E)]
     altcheck = cons is not None and cons.alternating()
     upd = np.zeros_like(psi)
@@ -550,7 +574,7 @@ def scf_run(g, niter):
     if est:
         est_psi = np.ascontiguousarray(g["est_seed"], dtype=np.float64).reshape(-1).copy()
         qn_est = qn[M - 1]
-    e_sp = np.zeros(M)
+    e_sp = e_start.copy()
     params = []
     for k in range(niter):
         if altcheck:
@@ -573,11 +597,11 @@ def scf_run(g, niter):
             params.append((dt, mom))
         psi, upd, e_sp, _ = graddesc(psi, der, lap, upd, qn, trc, kin2, Ff, dt_over_hbar,
                                       mom, dv)
-        occ = hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
+        dbasis, occ = pairing_step(psi, e_sp, d["rho"])
         if cons is not None:
             altcheck = cons.alternating()
         old = d
-        d, der, lap = all_densities(mesh, psi, qn, occ, trc)
+        d, der, lap = all_densities(mesh, dbasis, qn, occ, trc)
         if scheme == 0:
             for key in d:
                 lib().or_mix(P(d[key]), P(np.ascontiguousarray(old[key])),
@@ -594,10 +618,315 @@ def scf_run(g, niter):
         if scheme == 2:
             precondition_potentials(mesh, F, Fold, trc, dv, preconw)
         E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
-                   kinetic_sums(psi, lap, qn, occ, dv))
+                   kinetic_sums(dbasis, lap, qn, occ, dv))
+        if hfb is not None:
+            ep = hfb.energy()
+            E[0] += ep.sum()
+            pairs.append(np.concatenate([hfb.fermi, ep]))
+            for i in range(M):
+                der[i], lap[i] = mesh.derive_spinor(psi[i], qn[i][1], qn[i][2], qn[i][3])
         traj.append(rows(E))
     return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
                 vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params),
+                pairing=np.array(pairs), hfb=hfb,
                 constraints=[] if cons is None else
                 [dict(type=c["type"], con=c["con"].copy(), inten=c["inten"].copy(),
                       sq=c["sq"].copy()) for c in cons.items])
+
+
+# ---- HFB pairing (HFB.f90, Pairing.f90, PairingInteraction.f90) -----------
+
+def pair_density(w1, w2, tr):
+    """GetPairDensity (PairingInteraction.f90:101-180) of two spinors [4][N]:
+    complex pair amplitude; tr selects the time-reversed form."""
+    if not tr:
+        re = w1[0] * w2[2] - w1[1] * w2[3] - w1[2] * w2[0] + w1[3] * w2[1]
+        im = w1[0] * w2[3] + w1[1] * w2[2] - w1[2] * w2[1] - w1[3] * w2[0]
+    else:
+        re = -w1[0] * w2[0] - w1[1] * w2[1] - w1[2] * w2[2] - w1[3] * w2[3]
+        im = w1[0] * w2[1] - w1[1] * w2[0] + w1[2] * w2[3] - w1[3] * w2[2]
+    return re + 1j * im
+
+
+def time_reverse(w):
+    """TimeReverse (Spinor.f90:371-389): -i sigma_y K psi."""
+    return np.stack([w[2], -w[3], -w[0], w[1]])
+
+
+HFB_NUM_CUT = 1e-12
+
+
+class HFB:
+    """The HFB pairing step of SolvePairing (Pairing.f90:743-890) under time
+    reversal with conserved signature: blocks (parity, isospin) of the states
+    and their time-reversed partners (PrepareHFBModule, HFB.f90:248-295),
+    GuessHFBMatrices (:3645-3836), the pairing field and gaps (:350-632),
+    the Broyden Fermi solver (:1849-2161) over HFBNumberofParticles
+    (:810-1073, ConstructHFBHamiltonian :2554-2631,
+    DiagonaliseHFBHamiltonian_Signature :2695-2801, constructRhoHFB,
+    ConstructKappaHFB :3128-3360), HFBOccupations with the canonical basis
+    (:3361-3644) and HFBEnergy (:296-348).  Dense diagonalisations use
+    numpy's eigh (eigenvalues ascending, as diagoncr8 sorts them)."""
+
+    def __init__(self, qn, trc, m):
+        assert trc, "HFB restatement covers time-reversal-conserving runs"
+        self.qn = np.asarray(qn)
+        self.nwt = len(qn)
+        self.size = 2 * self.nwt
+        self.blocks = {}
+        for i in range(self.size):
+            ii = i % self.nwt
+            P = (self.qn[ii][1] + 3) // 2
+            it = (self.qn[ii][0] + 3) // 2
+            self.blocks.setdefault((P, it), []).append(i)
+        self.keys = sorted(self.blocks, key=lambda k: (k[1], k[0]))   # it outer, P inner
+        self.G = np.array(m["pairing_strength"])
+        self.alpha = np.array(m["pairing_alpha"])
+        self.cut = np.array(m["pairing_cut"])
+        self.mu = np.array(m["pairing_mu"])
+        self.rhosat = m["rhosat"]
+        self.mix = m["hfb_mix"]
+        self.iters = m["hfb_iter"]
+        self.particles = np.array([m["neutrons"], m["protons"]], dtype=float)
+        z = {k: np.zeros((len(v), len(v))) for k, v in self.blocks.items()}
+        self.rho = {k: v.copy() for k, v in z.items()}
+        self.kappa = {k: v.copy() for k, v in z.items()}
+        self.old_rho = {k: v.copy() for k, v in z.items()}
+        self.old_kappa = {k: v.copy() for k, v in z.items()}
+        self.delta = {k: v.copy() for k, v in z.items()}
+        self.fermi = np.array([-10.0, -10.0])
+
+    def sig(self, i):
+        s = self.qn[i % self.nwt][2]
+        return -s if i >= self.nwt else s
+
+    def guess(self):
+        for k, idx in self.blocks.items():
+            N = len(idx)
+            ovl = np.zeros((N, N))
+            for i in range(N // 2):
+                ovl[i, N // 2 + i] = ovl[N // 2 + i, i] = 1.0
+            kap = np.zeros((N, N))
+            found = np.zeros(N, bool)
+            for i in range(N):
+                best, maxo = 0, -9999.999
+                for j in range(N):
+                    if found[j]:
+                        continue
+                    if abs(ovl[i, j]) > maxo:
+                        best, maxo = j, abs(ovl[i, j])
+                found[best] = True
+                kap[i, best] = 0.25
+                kap[best, i] = -0.25
+            self.kappa[k] = kap
+
+    def cutoffs(self, energy, fermi):
+        pc = np.zeros(self.nwt)
+        for i in range(self.nwt):
+            it = (self.qn[i][0] + 3) // 2 - 1
+            up = (energy[i] - fermi[it] - self.cut[it]) / self.mu[it]
+            down = -(energy[i] - fermi[it] + self.cut[it]) / self.mu[it]
+            c = np.sqrt(np.sqrt(1.0 / (1.0 + np.exp(up))))
+            pc[i] = c * np.sqrt(np.sqrt(1.0 / (1.0 + np.exp(down))))
+        return pc
+
+    def density_factor(self, rho):
+        rho0 = rho[0] + rho[1]
+        return np.stack([-self.G[it] * (1.0 - self.alpha[it] / self.rhosat * rho0)
+                         for it in range(2)])
+
+    def _pairs(self, k, pc, need_kappa):
+        idx = self.blocks[k]
+        N = len(idx)
+        for j in range(N):
+            jj = idx[j]
+            jjj = jj % self.nwt
+            if pc[jjj] < HFB_NUM_CUT:
+                continue
+            for i in range(j + 1, N):
+                ii = idx[i]
+                iii = ii % self.nwt
+                if self.sig(jj) != -self.sig(ii):
+                    continue
+                if need_kappa and pc[jjj] * pc[iii] * abs(self.kappa[k][i, j]) < HFB_NUM_CUT:
+                    continue
+                tr = (ii != iii or jj != jjj) and not (ii != iii and jj != jjj)
+                yield i, j, iii, jjj, tr
+
+    def pairing_field(self, psi, pc, df):
+        N = psi.shape[-1]
+        field = np.zeros((2, N), complex)
+        for k in self.keys:
+            it = k[1] - 1
+            for i, j, iii, jjj, tr in self._pairs(k, pc, True):
+                a = pair_density(psi[jjj], psi[iii], tr)
+                f = pc[jjj] * pc[iii] * self.kappa[k][i, j]
+                field[it] = field[it] - f * a
+        return field * df
+
+    def gaps(self, psi, field, pc, dv):
+        for k in self.keys:
+            it = k[1] - 1
+            idx = self.blocks[k]
+            N = len(idx)
+            d = np.zeros((N, N))
+            for i in range(N):
+                ii = idx[i]
+                iii = ii % self.nwt
+                for j in range(i + 1, N):
+                    jj = idx[j]
+                    jjj = jj % self.nwt
+                    if self.sig(ii) != -self.sig(jj):
+                        continue
+                    if pc[iii] * pc[jjj] < HFB_NUM_CUT:
+                        continue
+                    tr = (ii != iii or jj != jjj) and not (ii != iii and jj != jjj)
+                    t = pair_density(psi[iii], psi[jjj], tr)
+                    d[i, j] = dv * pc[iii] * pc[jjj] * (np.sum(t.real * field[it].real) +
+                                                         np.sum(t.imag * field[it].imag))
+                    d[j, i] = -d[i, j]
+            self.delta[k] = d
+
+    def number(self, lam, energy):
+        """HFBNumberofParticles: rho, kappa, quasiparticle energies at lam."""
+        n = np.zeros(2)
+        self.qpe = {}
+        for k in self.keys:
+            it = k[1] - 1
+            idx = self.blocks[k]
+            N = len(idx)
+            H = np.zeros((2 * N, 2 * N))
+            for i in range(N):
+                H[i, i] = energy[idx[i] % self.nwt] - lam[it]
+                for j in range(N):
+                    H[i + N, j + N] = -H[i, j]
+                for j in range(N):
+                    H[i, j + N] = self.delta[k][i, j]
+            for i in range(2 * N):
+                for j in range(i + 1, 2 * N):
+                    H[j, i] = H[i, j]
+            h = N // 2
+            T = np.zeros((N, N))
+            for j in range(h):
+                for i in range(h):
+                    T[i, j] = T[j, i] = H[i, j]
+                    T[i + h, j + h] = T[j + h, i + h] = H[i + 3 * h, j + 3 * h]
+                    T[i, j + h] = T[j + h, i] = H[i, j + 3 * h]
+            w, vec = np.linalg.eigh(T)
+            U = np.zeros((N, 2 * N))
+            V = np.zeros((N, 2 * N))
+            U[:h, :N] = vec[:h, :N]
+            V[h:, :N] = vec[h:, :N]
+            U[h:, N:] = -vec[:h, :N]
+            V[:h, N:] = vec[h:, :N]
+            qpe = np.concatenate([w, w])
+            cols = [2 * N - i + 1 for i in range(1, h + 1)] + list(range(h + 1, N + 1))
+            cols = np.array(cols) - 1
+            rho = V[:, cols] @ V[:, cols].T
+            kap = np.zeros((N, N))
+            for j in range(N):
+                for i in range(j + 1, N):
+                    kap[i, j] = np.sum(V[i, cols] * U[j, cols])
+                    kap[j, i] = -kap[i, j]
+            self.rho[k], self.kappa[k] = rho, kap
+            self.qpe[k] = qpe[cols]
+            n[it] += np.trace(rho)
+        return n
+
+    def broyden(self, energy, prec=1e-12):
+        fermi = self.fermi.copy()
+        hist = fermi + 0.01
+        N = self.number(fermi, energy) - self.particles
+        conv = np.abs(N) < prec
+        NX = self.number(hist, energy) - self.particles
+        jac = (N - NX) / (fermi - hist)
+        upd = np.zeros(2)
+        norm = np.zeros(2)
+        for it_ in range(1, self.iters + 1):
+            det = jac.copy()
+            inv = np.where(np.abs(det) < 0.001, np.copysign(0.99, det), 1.0 / det)
+            for it in range(2):
+                if conv[it]:
+                    continue
+                upd[it] = -inv[it] * N[it]
+                norm[it] = upd[it] ** 2
+            upd = 0.5 * upd
+            for it in range(2):
+                if conv[it]:
+                    continue
+                hist[it] = fermi[it]
+                fermi[it] = fermi[it] + upd[it]
+            N = self.number(fermi, energy) - self.particles
+            conv = conv | (np.abs(N) < prec)
+            if conv.all():
+                break
+            if it_ > 1:
+                for it in range(2):
+                    if conv[it]:
+                        continue
+                    jac[it] = jac[it] + N[it] * upd[it] / norm[it]
+        self.fermi = fermi
+        self.iterations = it_
+
+    def occupations(self, psi):
+        """HFBOccupations: canonical basis and occupations (TRC: twice the
+        eigenvalues of rho), placed in the slots of the original states of
+        each block; then the mixing of rho and kappa."""
+        can = np.zeros_like(psi)
+        occ = np.zeros(self.nwt)
+        for k in self.keys:
+            idx = self.blocks[k]
+            N = len(idx)
+            h = N // 2
+            T = self.rho[k].copy()
+            for i in range(N):
+                if idx[i] >= self.nwt or self.qn[idx[i] % self.nwt][2] == -1:
+                    T[i, i] -= 2.0
+            w, vec = np.linalg.eigh(T)
+            vec = np.where(np.abs(vec) < 1e-11, 0.0, vec)
+            w = np.where(w < -0.1, w + 2, w)
+            w = np.where(w < 0.0, 0.0, w)
+            w = np.minimum(w, 1.0)
+            w = 2.0 * w
+            cols = [c for c in range(N) if not np.any(np.abs(vec[h:, c]) > HFB_NUM_CUT)]
+            slots = [i for i in idx if i < self.nwt]
+            assert len(cols) == len(slots)
+            for slot, c in zip(slots, cols):
+                v = np.zeros_like(psi[0])
+                for j in range(N):
+                    if abs(vec[j, c]) < HFB_NUM_CUT:
+                        continue
+                    jjj = idx[j] % self.nwt
+                    src = psi[jjj] if idx[j] < self.nwt else time_reverse(psi[jjj])
+                    v = v + vec[j, c] * src
+                can[slot] = v
+                occ[slot] = w[c]
+        if any(np.any(self.old_rho[k] != 0.0) for k in self.keys):
+            for k in self.keys:
+                self.rho[k] = self.mix * self.rho[k] + (1.0 - self.mix) * self.old_rho[k]
+        if any(np.any(self.kappa[k] != 0.0) for k in self.keys):
+            for k in self.keys:
+                self.kappa[k] = self.mix * self.kappa[k] + (1.0 - self.mix) * self.old_kappa[k]
+        for k in self.keys:
+            self.old_rho[k] = self.rho[k].copy()
+            self.old_kappa[k] = self.kappa[k].copy()
+        return can, occ
+
+    def energy(self):
+        e = np.zeros(2)
+        for k in self.keys:
+            e[k[1] - 1] -= np.sum(self.delta[k] * self.kappa[k])
+        return -0.5 * e
+
+    def solve(self, psi, energy, rho, dv):
+        """SolvePairing for PairingType 2 with PairingIter = 1."""
+        if all(not np.any(self.old_rho[k]) for k in self.keys):
+            for k in self.keys:
+                self.old_rho[k] = self.rho[k].copy()
+                self.old_kappa[k] = self.kappa[k].copy()
+        df = self.density_factor(rho)
+        pc = self.cutoffs(energy, self.fermi)
+        field = self.pairing_field(psi, pc, df)
+        self.gaps(psi, field, pc, dv)
+        self.broyden(energy)
+        return self.occupations(psi)

@@ -150,7 +150,62 @@ def test_rutz_cutoff_vs_reference(case):
     assert relerr(cut.reshape(-1), g["pot_cutoff"]) < 1e-14
 
 
+def test_scf_hfb_vs_reference():
+    """HFB pairing (the sample input's pairing): the pairing field from kappa,
+    gaps, the Broyden Fermi solver over the signature-reduced HFB matrices,
+    the canonical basis as density basis, rho/kappa mixing, pairing energy."""
+    g = golden("scfhfb_trc_fd")
+    r = ol.scf_run(g, g["meta"]["niter"])
+    assert np.max(np.abs(r["traj"][:, 0] - g["traj"][:, 0])) < 1e-9
+    assert relerr(r["traj"], g["traj"]) < 1e-11
+    assert np.max(np.abs(r["pairing"] - g["pairing"])) < 1e-9       # Fermi, E_pair
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-11
+    assert np.all(g["pairing"][:, 2:] < -0.5)                        # pairing stays on
+
+
+def test_hfb_start_matrices_vs_reference():
+    """The start-up SolvePairing: gaps Delta, mixed kappa and rho of every
+    (parity, isospin) block against the reference's arrays."""
+    g = golden("scfhfb_trc_fd")
+    m = g["meta"]
+    M, N = m["nwt"], m["nx"] ** 3
+    h = ol.HFB(g["qn"], True, m)
+    h.guess()
+    h.fermi = np.array(g["start_fermi"], float)
+    h.solve(g["psi"].reshape(M, 4, N), g["start_energies"], np.zeros((2, N)), g["consts"][6])
+    for (P, it) in h.keys:
+        n = len(h.blocks[(P, it)])
+        for name, mine in (("delta", h.delta), ("kappa", h.kappa), ("rho", h.rho)):
+            ref = g["start_" + name][it - 1, P - 1][:n, :n].T      # Fortran (i, j) order
+            assert np.max(np.abs(mine[(P, it)] - ref)) < 1e-11 * max(1.0, np.abs(ref).max())
+
+
 # ---- checks fixed by the mathematics ---------------------------------------
+
+def test_hfb_generalized_density_is_a_projector():
+    """For every lambda the quasiparticle vacuum's rho and kappa satisfy
+    rho - rho^2 = kappa kappa^T (R^2 = R for the generalised density), the
+    selected quasiparticle energies are positive and N(lambda) grows with
+    lambda."""
+    g = golden("scfhfb_trc_fd")
+    m = g["meta"]
+    M, N = m["nwt"], m["nx"] ** 3
+    h = ol.HFB(g["qn"], True, m)
+    h.guess()
+    psi = g["psi"].reshape(M, 4, N)
+    e = g["start_energies"]
+    pc = h.cutoffs(e, np.array(g["start_fermi"]))
+    h.gaps(psi, h.pairing_field(psi, pc, h.density_factor(np.zeros((2, N)))), pc, g["consts"][6])
+    last = None
+    for lam in np.linspace(30.0, 60.0, 7):
+        n = h.number(np.array([lam, lam]), e)
+        for k in h.keys:
+            rho, kap = h.rho[k], h.kappa[k]
+            assert np.max(np.abs(rho - rho @ rho - kap @ kap.T)) < 1e-12
+            assert np.all(h.qpe[k] > 0)
+        if last is not None:
+            assert np.all(n >= last - 1e-12)
+        last = n
 
 def test_rutz_cutoff_of_a_spherical_density():
     """For rho = exp(-r^2/b^2) the equidensity surface at max/10 is the sphere
