@@ -279,6 +279,45 @@ int mocca_alternate_step(mocca_solve *h);
  * off), target(2), intensity(2), <Q^2>(2), isoswitch. */
 int mocca_get_constraints(mocca_solve *h, double *out);
 
+/* Pairing (namelist Pairing, Pairing.f90:143-560).  type 0 = HF (NaiveFill,
+ * the default), 2 = HFB with a density-dependent contact interaction
+ * -G (1 - alpha rho / rho_sat) (CompDensityFactor,
+ * PairingInteraction.f90:265-354), SymmetricFermi cutoff factors, the
+ * Broyden Fermi solver and HFBMix mixing of rho and kappa.  HFB here needs
+ * time reversal and states of signature +1.  fermi: the start Fermi
+ * energies (-10, -10 in the reference); guess_kappa: GuessKappa. */
+typedef struct mocca_pairing {
+    int type;
+    int guess_kappa;
+    int iterations;       /* HFBIter (50 for the Broyden solver) */
+    int pad_;
+    double strength[2];   /* PairingNeutron, PairingProton (MeV fm^3) */
+    double alpha[2];      /* AlphaNeutron, AlphaProton */
+    double cut[2];        /* CutNeutron, CutProton (5 MeV) */
+    double mu[2];         /* PairingMu (0.5 MeV) */
+    double rhosat;        /* RhoSat (0.16 fm^-3) */
+    double hfb_mix;       /* HFBMix (0.1) */
+    double fermi_prec;    /* FermiPrec (1e-12) */
+    double fermi[2];
+} mocca_pairing;
+
+/* Select the pairing treatment of SolvePairing (after mocca_scf_setup,
+ * before mocca_scf_start).  With HFB the canonical basis is the density
+ * basis (Pairing.f90:812) and energy rows carry the pairing energies and
+ * Fermi energies in elements 43..46. */
+int mocca_set_pairing(mocca_solve *h, const mocca_pairing *p);
+
+/* SolvePairing (Pairing.f90:743-890): HF filling, or the HFB step (pairing
+ * field, gaps, Fermi energies, canonical basis, mixing).  Replaces
+ * mocca_hf_fill when HFB is selected (mocca_hf_fill calls it then). */
+int mocca_solve_pairing(mocca_solve *h);
+
+/* Fermi energies, pairing energies and selected quasiparticle energies of
+ * the last pairing step: out[4 + nwt] = Fermi(2), E_pair(2), qp energies
+ * (the h largest of each block, block by block), and canonical occupations
+ * occ[nwt] (may be NULL). */
+int mocca_get_pairing(mocca_solve *h, double *out, double *canonical_occ);
+
 /* Results of the last ConstructPotentials.  lm[k][2] = (l, m) and
  * q[k][2] = (neutron, proton) moments; *count moments (<= 28).
  * vcoul, vcoulx [N]; cpot [(nz+2)][(ny+2)][(nx+2)] (CPotComplete);

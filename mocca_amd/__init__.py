@@ -33,7 +33,8 @@ EXPORTS = (
     "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
     "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
     "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
-    "mocca_alternate_step", "mocca_get_constraints",
+    "mocca_alternate_step", "mocca_get_constraints", "mocca_set_pairing",
+    "mocca_solve_pairing", "mocca_get_pairing",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -75,7 +76,18 @@ class ConstraintC(ctypes.Structure):
 
 # Energy row layout of mocca_comp_energy / mocca_iterate.
 ENERGY_FIELDS = ("total", "kinetic_n", "kinetic_p", "coulomb", "coulomb_exchange",
-                 "com_n", "com_p") + tuple(f"term{k}" for k in range(1, 37))
+                 "com_n", "com_p") + tuple(f"term{k}" for k in range(1, 37)) + \
+    ("pairing_n", "pairing_p", "fermi_n", "fermi_p")
+ROW_LEN = len(ENERGY_FIELDS)
+
+
+class PairingC(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("guess_kappa", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("strength", ctypes.c_double * 2), ("alpha", ctypes.c_double * 2),
+                ("cut", ctypes.c_double * 2), ("mu", ctypes.c_double * 2),
+                ("rhosat", ctypes.c_double), ("hfb_mix", ctypes.c_double),
+                ("fermi_prec", ctypes.c_double), ("fermi", ctypes.c_double * 2)]
 
 
 class MoccaError(RuntimeError):
@@ -259,7 +271,7 @@ class Solve:
         self._chk(self.lib.mocca_construct_potentials(self.h))
 
     def comp_energy(self):
-        e = np.zeros(43)
+        e = np.zeros(ROW_LEN)
         self._chk(self.lib.mocca_comp_energy(self.h, _p(e)))
         return e
 
@@ -267,7 +279,7 @@ class Solve:
         self._chk(self.lib.mocca_scf_start(self.h))
 
     def iterate(self, niter, dt_over_hbar, momentum, fetch=True):
-        e = np.zeros((niter + 1, 43)) if fetch else None
+        e = np.zeros((niter + 1, ROW_LEN)) if fetch else None
         self._chk(self.lib.mocca_iterate(self.h, int(niter), ctypes.c_double(dt_over_hbar),
                                          ctypes.c_double(momentum), _p(e)))
         return e
@@ -306,6 +318,26 @@ class Solve:
         self._chk(self.lib.mocca_get_constraints(self.h, _p(out)))
         return out[:n]
 
+    def set_pairing(self, type=2, strength=(1250.0, 1250.0), alpha=(1.0, 1.0), cut=(5.0, 5.0),
+                    mu=(0.5, 0.5), rhosat=0.16, hfb_mix=0.1, iterations=50, fermi_prec=1e-12,
+                    fermi=(-10.0, -10.0), guess_kappa=True):
+        """Pairing of SolvePairing: type 0 HF, 2 HFB (namelist Pairing defaults)."""
+        d2 = lambda v: (ctypes.c_double * 2)(*[float(x) for x in v])
+        p = PairingC(int(type), int(bool(guess_kappa)), int(iterations), 0, d2(strength),
+                     d2(alpha), d2(cut), d2(mu), float(rhosat), float(hfb_mix),
+                     float(fermi_prec), d2(fermi))
+        self._chk(self.lib.mocca_set_pairing(self.h, ctypes.byref(p)))
+
+    def solve_pairing(self):
+        self._chk(self.lib.mocca_solve_pairing(self.h))
+
+    def pairing(self):
+        """(fermi[2], pairing energy[2], qp energies[nwt], canonical occupations[nwt])."""
+        out = np.zeros(4 + self.M)
+        occ = np.zeros(self.M)
+        self._chk(self.lib.mocca_get_pairing(self.h, _p(out), _p(occ)))
+        return out[0:2], out[2:4], out[4:], occ
+
     def estimate_parameters(self, iteration, dt, momentum):
         """IterativeEstimation: (dt, momentum, maxE, relE, con, iterations)."""
         out = np.zeros(6)
@@ -323,7 +355,7 @@ class Solve:
     def iterate_estimated(self, niter, dt, momentum, first_iteration=1):
         """mocca_iterate with IterativeEstimation before every step: returns
         (energy rows [niter+1][43], params [niter][2] = dt, momentum used)."""
-        e = np.zeros((niter + 1, 43))
+        e = np.zeros((niter + 1, ROW_LEN))
         prm = np.zeros((max(niter, 1), 2))
         self._chk(self.lib.mocca_iterate_estimated(self.h, int(first_iteration), int(niter),
                                                    ctypes.c_double(dt), ctypes.c_double(momentum),

@@ -315,11 +315,13 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
 template <typename T>
 int kinetic_impl(mocca_solve *h) {
     const MeshK<T> &m = mesh_of<T>(h);
+    // the density basis: the canonical states under HFB
+    const T *psi = (const T *)(h->hfb.on ? h->hfb.can_psi : h->b.psi);
     int rc, slots;
     if (h->march) {
         slots = h->ntile_y;
         rc = run(h, "kinetic", [&] {
-            launch_hpsi_fd<T>(m, (const T *)h->b.psi, nullptr, nullptr, nullptr,
+            launch_hpsi_fd<T>(m, psi, nullptr, nullptr, nullptr,
                               (const T *)h->b.fields, h->b.qn, h->b.tasks, h->ntasks, h->TY,
                               h->ntile_y, h->mesh.time_reversal, T(h->kin[0]), T(h->kin[1]),
                               T(0), T(0), 0, h->b.eparts, h->st);
@@ -327,7 +329,7 @@ int kinetic_impl(mocca_solve *h) {
     } else {
         slots = h->ntiles;
         rc = run(h, "kinetic", [&] {
-            launch_hpsi_step<T>(m, (const T *)h->b.psi, nullptr, nullptr, nullptr,
+            launch_hpsi_step<T>(m, psi, nullptr, nullptr, nullptr,
                                 (const T *)h->b.fields, h->b.qn, h->b.order, h->b.chunks,
                                 h->nchunks, h->mesh.time_reversal, T(h->kin[0]),
                                 T(h->kin[1]), T(0), T(0), 0, h->b.eparts, h->ntiles, h->st);
@@ -452,7 +454,8 @@ int estimate_impl(mocca_solve *h, int iteration, double dt, double mu, double *o
         batch = std::min(2 * batch, 64);
     }
     rc = run(h, "est_finalize", [&] {
-        launch_est_finalize(h->b.E, h->b.occ, h->b.qn, h->M, 0, nullptr, 0, kHbar, e.rec, h->st);
+        launch_est_finalize(h->b.E, h->b.occ, h->b.qn, h->M, h->hfb.on ? 2 : 0, h->hfb.qpe,
+                            h->hfb.on ? h->M : 0, kHbar, e.rec, h->st);
     });
     if (rc) return rc;
     HIPCHK(h, hipMemcpyAsync(rec, e.rec, sizeof rec, hipMemcpyDeviceToHost, h->st));
@@ -539,11 +542,14 @@ int hpsi_one_impl(mocca_solve *h, int state) {
 template <typename T>
 int density_impl(mocca_solve *h) {
     const MeshK<T> &m = mesh_of<T>(h);
+    // the density basis (DensityBasis): the canonical states under HFB
+    const T *psi = (const T *)(h->hfb.on ? h->hfb.can_psi : h->b.psi);
+    const double *occ = h->hfb.on ? h->hfb.can_occ : h->b.occ;
     const int nf = h->mesh.time_reversal ? 3 : 9;
     int rc;
     if (h->march) {
         rc = run(h, "density", [&] {
-            launch_density_fd<T>(m, (const T *)h->b.psi, h->b.qn, h->b.occ, h->b.order,
+            launch_density_fd<T>(m, psi, h->b.qn, occ, h->b.order,
                                  h->b.dchunks, h->b.dtasks, h->ndtasks, h->TY,
                                  h->mesh.time_reversal, (T *)h->b.dpart, h->st);
         });
@@ -554,7 +560,7 @@ int density_impl(mocca_solve *h) {
         });
     } else {
         rc = run(h, "density", [&] {
-            launch_density<T>(m, (const T *)h->b.psi, h->b.qn, h->b.occ, h->b.order,
+            launch_density<T>(m, psi, h->b.qn, occ, h->b.order,
                               h->b.chunks, h->nchunks, h->mesh.time_reversal,
                               (T *)h->b.dpart, h->st);
         });
@@ -623,6 +629,7 @@ void free_all(mocca_solve *h) {
     for (void *p : eptrs)
         if (p) (void)hipFree(p);
     scf_free(h);
+    hfb_free(h);
     for (auto &p : h->pending) { h->pool.push_back(p.a); h->pool.push_back(p.b); }
     for (auto e : h->pool) (void)hipEventDestroy(e);
     if (h->st) (void)hipStreamDestroy(h->st);
@@ -685,6 +692,10 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     TRY(dalloc(h, &b.fields, 48 * N * ts));
     TRY(dalloc(h, &b.dens, 2 * 9 * N * sizeof(double)));
     TRY(dalloc(h, &b.derived, 38 * N * sizeof(double)));
+    // no densities yet: zero (the pairing factor of a first SolvePairing
+    // reads them)
+    HIPCHK(h, hipMemset(b.dens, 0, 2 * 9 * N * sizeof(double)));
+    HIPCHK(h, hipMemset(b.derived, 0, 38 * N * sizeof(double)));
     TRY(dalloc(h, &b.qn, M * 4 * sizeof(int)));
     TRY(dalloc(h, &b.occ, M * sizeof(double)));
     TRY(dalloc(h, &b.E, M * sizeof(double)));

@@ -116,6 +116,27 @@ struct Est {
     unsigned long long calls = 0;
 };
 
+// HFB pairing state (PairingType 2).
+struct Hfb {
+    bool on = false, started = false, unconverged = false;
+    HfbK k{};
+    int total = 0, hmax = 0;
+    size_t lds_broyden = 0, lds_canonical = 0;
+    double *kappa = nullptr, *rho = nullptr, *orho = nullptr, *okappa = nullptr;
+    double *delta = nullptr;       // packed (2h)^2 per block
+    double *fermi = nullptr;       // [2]
+    double *pc = nullptr;          // [M] cutoff factors
+    double *dfac = nullptr;        // [2][N] density factor
+    double *field = nullptr;       // [2][2][N] pairing field (re, im)
+    double *qpe = nullptr;         // [M] selected quasiparticle energies
+    double *tcan = nullptr;        // packed h^2 canonical transforms
+    void *can_psi = nullptr;       // T [M][4][N] canonical states (the density basis)
+    double *can_occ = nullptr;     // [M] canonical occupations
+    double *work = nullptr;
+    int *status = nullptr;
+    int *tables = nullptr;         // [3][M] state, slot_block, slot_index
+};
+
 }  // namespace detail
 }  // namespace mocca
 
@@ -127,6 +148,7 @@ struct mocca_solve {
     mocca::detail::Buffers b;
     mocca::detail::Scf scf;
     mocca::detail::Est est;
+    mocca::detail::Hfb hfb;
     mocca::MeshK<double> md{};
     mocca::MeshK<float> mf{};
     bool meta_set = false, fields_set = false;
@@ -203,6 +225,14 @@ int grad_desc_any(mocca_solve *h, double dth, double mu);
 int density_any(mocca_solve *h);
 // K[i] = dv <psi_i|Lap psi_i> of the current states (no other output).
 int kinetic_any(mocca_solve *h);
+// HFB pairing: setup from the C-ABI parameters, one SolvePairing step, the
+// pairing energy and Fermi energies into an energy row, the status check.
+int hfb_setup(mocca_solve *h, const mocca_pairing *q);
+int hfb_free(mocca_solve *h);
+int hfb_solve_any(mocca_solve *h, bool first);
+int hfb_energy_impl(mocca_solve *h, double *row);
+int hfb_check_status(mocca_solve *h);
+constexpr int kRowLen = 47;   // energy row: 43 of CompEnergy + pairing energy(2) + Fermi(2)
 // AlternateStep (constraints of type 2) on the device.
 int alternate_any(mocca_solve *h);
 // IterativeEstimation: out = dt, momentum, maxE, relE, con, iterations.

@@ -247,6 +247,27 @@ void launch_alternate(const ConstraintK *con, int nc, const double *Y, const dou
                       const double *cut, long N, double *mult, const T *in, T *out,
                       const int *qn, int M, hipStream_t st);
 
+// HFB pairing (time reversal, conserved signature): one block per
+// (isospin, parity) with h states and their h partners.
+constexpr double kHfbNumCut = 1e-12;   // HFBNumCut (HFB.f90)
+constexpr int kHfbMaxN = 64;           // block size 2h of the in-LDS solvers
+struct HfbBlockK {
+    int it, P, h;
+    int first;      // index of the block's first state in the state table
+    int off;        // offset of its (2h)^2 matrices in the packed arrays
+    int toff;       // offset of its h^2 canonical transform
+};
+struct HfbK {
+    int nblk;
+    HfbBlockK blk[4];
+    double G[2], alpha[2], cut[2], mu[2];   // PairingStrength, Alpha, PairingCut, PairingMu
+    double rhosat, mix, prec, particles[2];  // RhoSat, HFBMix, FermiPrec, N and Z
+    int iters;                               // HFBIter
+    const int *state;        // [M] states block by block
+    const int *slot_block;   // [M] block of each state
+    const int *slot_index;   // [M] position of each state in its block
+};
+
 // IterativeEstimation (power iteration for dt and momentum).
 constexpr int kEstSlots = 8;
 template <typename T>

@@ -738,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_kinetic(SkyrmeK f, const doub
     e[2] = kin[1];
     e[5] = com[0];
     e[6] = com[1];
-    e[0] = sky + (kin[0] + kin[1]) + e[3] + (com[0] + com[1]) + e[4];
+    e[0] = sky + (kin[0] + kin[1]) + e[3] + (com[0] + com[1]) + e[4] + (e[43] + e[44]);
 }
 
 void launch_energy_terms(const SkyrmeK &f, long N, int trc, const double *dens,

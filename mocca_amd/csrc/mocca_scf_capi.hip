@@ -267,13 +267,19 @@ int energy_terms_impl(mocca_solve *h, double *row) {
 }
 
 int energy_kinetic_impl(mocca_solve *h, double *row) {
+    const double *occ = h->hfb.on ? h->hfb.can_occ : h->b.occ;
     return run(h, "energy_kinetic", [&] {
-        launch_energy_kinetic(h->scf.sk, h->b.K, h->b.occ, h->b.qn, h->M, row, h->st);
+        launch_energy_kinetic(h->scf.sk, h->b.K, occ, h->b.qn, h->M, row, h->st);
     });
 }
 
 int fill_impl(mocca_solve *h) {
     Scf &s = h->scf;
+    if (h->hfb.on) {
+        const bool first = !h->hfb.started;
+        h->hfb.started = true;
+        return hfb_solve_any(h, first);
+    }
     return run(h, "hf_fill", [&] {
         launch_hf_fill(h->b.E, h->b.qn, h->M, s.ub_n, s.ub_p, s.fill_fac, h->b.occ, h->st);
     });
@@ -346,22 +352,28 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
             (void)hipFree(s.erows);
             s.erows = nullptr;
         }
-        if (int rc = dalloc(h, &s.erows, (size_t)(niter + 1) * 43 * sizeof(double))) return rc;
+        if (int rc = dalloc(h, &s.erows, (size_t)(niter + 1) * kRowLen * sizeof(double))) return rc;
+        HIPCHK(h, hipMemsetAsync(s.erows, 0, (size_t)(niter + 1) * kRowLen * sizeof(double),
+                                 h->st));
         s.erow_cap = niter + 1;
     }
-    HIPCHK(h, hipMemcpyAsync(s.erows, s.ecur, 43 * sizeof(double), hipMemcpyDeviceToDevice,
+    HIPCHK(h, hipMemcpyAsync(s.erows, s.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
                              h->st));
     int rc;
+    // HF: a row's kinetic energy comes from the next GradDesc's hPsi pass
+    const bool deferred = !h->hfb.on;
     for (int k = 1; k <= niter; ++k) {
-        double *prev = s.erows + (size_t)(k - 1) * 43, *row = s.erows + (size_t)k * 43;
+        double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
         const int iter = first_iteration + k - 1;
         // AlternateStep when alternating constraints were on after the last
         // SolvePairing (Main.f90:262-264); the previous row's kinetic energy
         // then comes from the states before it
         const bool alt = s.alt_next;
         if (alt) {
-            if ((rc = kinetic_any(h))) return rc;
-            if ((rc = energy_kinetic_impl(h, prev))) return rc;
+            if (deferred) {
+                if ((rc = kinetic_any(h))) return rc;
+                if ((rc = energy_kinetic_impl(h, prev))) return rc;
+            }
             if ((rc = alternate_any(h))) return rc;
         }
         if (estimate) {
@@ -378,7 +390,7 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
         if ((rc = grad_desc_any(h, dth, momentum))) return rc;
         // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
         // previous row's densities came from: that row's kinetic energy
-        if (!alt && (rc = energy_kinetic_impl(h, prev))) return rc;
+        if (deferred && !alt && (rc = energy_kinetic_impl(h, prev))) return rc;
         if ((rc = fill_impl(h))) return rc;
         s.alt_next = alternating(h);
         if ((rc = density_any(h))) return rc;
@@ -386,18 +398,29 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
         if ((rc = construct_impl(h))) return rc;
         if ((rc = readjust_impl(h, iter, momentum))) return rc;
         if ((rc = energy_terms_impl(h, row))) return rc;
+        if (!deferred) {
+            // HFB: the density basis is the canonical one, whose kinetic sums
+            // the GradDesc pass does not give
+            if ((rc = hfb_energy_impl(h, row))) return rc;
+            if ((rc = kinetic_any(h))) return rc;
+            if ((rc = energy_kinetic_impl(h, row))) return rc;
+        }
     }
     // kinetic energy of the final states: one hPsi-type pass with no outputs
-    double *last = s.erows + (size_t)niter * 43;
-    if ((rc = kinetic_any(h))) return rc;
-    if ((rc = energy_kinetic_impl(h, last))) return rc;
-    HIPCHK(h, hipMemcpyAsync(s.ecur, last, 43 * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    double *last = s.erows + (size_t)niter * kRowLen;
+    if (deferred) {
+        if ((rc = kinetic_any(h))) return rc;
+        if ((rc = energy_kinetic_impl(h, last))) return rc;
+    }
+    HIPCHK(h, hipMemcpyAsync(s.ecur, last, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->st));
     if (energies) {
         HIPCHK(h, hipStreamSynchronize(h->st));
-        HIPCHK(h, hipMemcpy(energies, s.erows, (size_t)(niter + 1) * 43 * sizeof(double),
+        HIPCHK(h, hipMemcpy(energies, s.erows, (size_t)(niter + 1) * kRowLen * sizeof(double),
                             hipMemcpyDeviceToHost));
     }
-    return check_gs_status(h);
+    if ((rc = check_gs_status(h))) return rc;
+    return h->hfb.on ? hfb_check_status(h) : 0;
 }
 
 
@@ -449,6 +472,7 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
         return fail(h, MOCCA_ERR_ARG, "particle numbers must be non-negative");
     HIPCHK(h, hipStreamSynchronize(h->st));
     scf_free(h);
+    hfb_free(h);
     Scf &s = h->scf;
     const mocca_mesh &g = h->mesh;
     const long N = h->N;
@@ -658,8 +682,8 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
     }
     s.e_nblk = (int)std::min<long>(512, (N + kBlock - 1) / kBlock);
     SCF_ALLOC(s.epart, (size_t)s.e_nblk * kEnergySlots * sizeof(double));
-    SCF_ALLOC(s.ecur, 43 * sizeof(double));
-    HIPCHK(h, hipMemset(s.ecur, 0, 43 * sizeof(double)));
+    SCF_ALLOC(s.ecur, kRowLen * sizeof(double));
+    HIPCHK(h, hipMemset(s.ecur, 0, kRowLen * sizeof(double)));
     SCF_ALLOC(s.hist, 38 * (size_t)N * sizeof(double));
     HIPCHK(h, hipMemset(s.hist, 0, 38 * (size_t)N * sizeof(double)));
 #undef SCF_ALLOC
@@ -702,10 +726,11 @@ int mocca_comp_energy(mocca_solve *h, double *e) {
     int rc;
     if ((rc = kinetic_any(h))) return rc;
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
+    if (h->hfb.on && (rc = hfb_energy_impl(h, h->scf.ecur))) return rc;
     if ((rc = energy_kinetic_impl(h, h->scf.ecur))) return rc;
     if (e) {
         HIPCHK(h, hipStreamSynchronize(h->st));
-        HIPCHK(h, hipMemcpy(e, h->scf.ecur, 43 * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(e, h->scf.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToHost));
     }
     return 0;
 }
@@ -718,6 +743,11 @@ int mocca_scf_start(mocca_solve *h) {
     if ((rc = save_history(h))) return rc;
     if ((rc = construct_impl(h, 3))) return rc;
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
+    if (h->hfb.on) {
+        if ((rc = hfb_energy_impl(h, h->scf.ecur))) return rc;
+        if ((rc = kinetic_any(h))) return rc;
+        if ((rc = energy_kinetic_impl(h, h->scf.ecur))) return rc;
+    }
     h->scf.alt_next = alternating(h);
     h->scf.started = true;
     return 0;
@@ -903,6 +933,53 @@ int mocca_alternate_step(mocca_solve *h) {
 int mocca_readjust_constraints(mocca_solve *h, int iteration, double momentum) {
     if (int rc = check_scf(h)) return rc;
     return readjust_impl(h, iteration, momentum);
+}
+
+int mocca_set_pairing(mocca_solve *h, const mocca_pairing *p) {
+    if (int rc = check_scf(h)) return rc;
+    if (!p) return fail(h, MOCCA_ERR_ARG, "null pairing parameters");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return hfb_setup(h, p);
+}
+
+int mocca_solve_pairing(mocca_solve *h) {
+    if (int rc = check_scf(h)) return rc;
+    int rc = fill_impl(h);
+    if (rc) return rc;
+    return h->hfb.on ? hfb_check_status(h) : 0;
+}
+
+int mocca_get_pairing(mocca_solve *h, double *out, double *canonical_occ) {
+    if (int rc = check_scf(h)) return rc;
+    Hfb &p = h->hfb;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (out) {
+        std::vector<double> row(kRowLen, 0.0);
+        if (p.on) {
+            double *tmp = nullptr;
+            if (int rc = dalloc(h, &tmp, kRowLen * sizeof(double))) return rc;
+            int rc = hfb_energy_impl(h, tmp);
+            if (!rc) {
+                (void)hipMemcpyAsync(row.data(), tmp, kRowLen * sizeof(double),
+                                     hipMemcpyDeviceToHost, h->st);
+                (void)hipStreamSynchronize(h->st);
+            }
+            (void)hipFree(tmp);
+            if (rc) return rc;
+            HIPCHK(h, hipMemcpy(out + 4, p.qpe, h->M * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            for (int i = 0; i < h->M; ++i) out[4 + i] = 0.0;
+        }
+        out[0] = row[45];
+        out[1] = row[46];
+        out[2] = row[43];
+        out[3] = row[44];
+    }
+    if (canonical_occ) {
+        HIPCHK(h, hipMemcpy(canonical_occ, p.on ? p.can_occ : h->b.occ, h->M * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    return 0;
 }
 
 int mocca_get_potentials(mocca_solve *h, double *U, double *B, double *NablaB, double *W,

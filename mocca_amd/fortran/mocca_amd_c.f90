@@ -50,6 +50,13 @@ module mocca_amd_c
     real(c_double) :: intensity
   end type
 
+  type, bind(C) :: mocca_pairing
+    integer(c_int) :: type, guess_kappa, iterations, pad_
+    real(c_double) :: strength(2), alpha(2), cut(2), mu(2)
+    real(c_double) :: rhosat, hfb_mix, fermi_prec
+    real(c_double) :: fermi(2)
+  end type
+
   interface
     integer(c_int) function mocca_solve_create(mesh, nwt, precision, h) bind(C)
       import :: c_int, c_ptr, mocca_mesh
@@ -225,6 +232,20 @@ module mocca_amd_c
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function mocca_set_pairing(h, p) bind(C)
+      import :: c_int, c_ptr, mocca_pairing
+      type(c_ptr), value :: h
+      type(mocca_pairing), intent(in) :: p
+    end function
+    integer(c_int) function mocca_solve_pairing(h) bind(C)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function mocca_get_pairing(h, out, canonical_occ) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*), canonical_occ(*)
     end function
     integer(c_int) function mocca_get_moments(h, lm, q, count) bind(C)
       import :: c_int, c_ptr, c_double
