@@ -506,12 +506,15 @@ __global__ __launch_bounds__(kHfbThreads) void k_hfb_can_states(HfbK k, const T 
 
 // rho <- mix rho + (1 - mix) rho_old when rho_old is not all zero, kappa
 // likewise when kappa is not all zero (HFBOccupations :3492-3510); then the
-// old matrices take the mixed ones.  At the first call rho_old, kappa_old
-// are set to the current ones when rho_old is all zero (SolvePairing
-// :800-806) -- the caller passes first = 1 for that copy.
+// old matrices take the mixed ones.  The reference's new kappa is exactly
+// zero only when every gap is (its Householder-QL eigenvectors of a matrix
+// with any non-zero coupling carry rounding-level pair components), so the
+// test is made on the gaps as well: Jacobi leaves couplings below its
+// stopping threshold untouched.  first = 1 instead sets rho_old, kappa_old
+// to the current ones while rho_old is all zero (SolvePairing :800-806).
 __global__ void k_hfb_mix(int total, double mix, int first, double *__restrict__ rho,
                           double *__restrict__ kappa, double *__restrict__ orho,
-                          double *__restrict__ okappa) {
+                          double *__restrict__ okappa, const double *__restrict__ delta) {
     __shared__ int flags[2];
     const int tid = threadIdx.x;
     if (first) {
@@ -531,7 +534,7 @@ __global__ void k_hfb_mix(int total, double mix, int first, double *__restrict__
     __syncthreads();
     for (int e = tid; e < total; e += blockDim.x) {
         if (orho[e] != 0.0) flags[0] = 1;
-        if (kappa[e] != 0.0) flags[1] = 1;
+        if (kappa[e] != 0.0 || delta[e] != 0.0) flags[1] = 1;
     }
     __syncthreads();
     for (int e = tid; e < total; e += blockDim.x) {
@@ -606,7 +609,7 @@ int hfb_solve_impl(mocca_solve *h, bool first) {
     // old rho is all zero (the first call)
     int rc = run(h, "hfb_mix", [&] {
         hipLaunchKernelGGL(k_hfb_mix, dim3(1), dim3(256), 0, h->st, p.total, p.k.mix, 1, p.rho,
-                           p.kappa, p.orho, p.okappa);
+                           p.kappa, p.orho, p.okappa, p.delta);
     });
     if (rc) return rc;
     rc = run(h, "hfb_field", [&] {
@@ -637,7 +640,7 @@ int hfb_solve_impl(mocca_solve *h, bool first) {
                            dim3(kHfbThreads), 0, h->st, p.k, (const T *)h->b.psi, N, p.tcan,
                            (T *)p.can_psi);
         hipLaunchKernelGGL(k_hfb_mix, dim3(1), dim3(256), 0, h->st, p.total, p.k.mix, 0, p.rho,
-                           p.kappa, p.orho, p.okappa);
+                           p.kappa, p.orho, p.okappa, p.delta);
     });
     return rc;
 }

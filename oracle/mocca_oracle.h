@@ -170,7 +170,7 @@ void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int 
                  const double *NablaB, const double *W, const double *DivW,
                  const double *S, const double *A, const double *DivA, double dt,
                  double momentum, double dv, double hbar, int M, const int *qn,
-                 const double *occ, const double *energy, double *out);
+                 const double *occ, const double *energy, double rel_hfb, double *out);
 
 /* PreconditionPotential (Damping.f90:158-228): per isospin, CG on
  * (beta + alpha Lap) x = pot[it] (Lap with ghost signs sx, sy, sz), x = 0

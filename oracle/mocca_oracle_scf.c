@@ -500,8 +500,10 @@ void or_hf_fill(const double *energy, const int *qn, int M, double neutrons,
  * first iteration of a call never converges: the reference's previous maxE
  * is an undefined local there), u = dt/hbar (h - maxE psi) + momentum u
  * with u = 0 on entry, psi += u, psi /= sqrt(<psi|psi>), exit at
- * |con| < 1e-2.  relE for HF (:109-128), minh (:189-195), then
- * out[6] = { dt, momentum, maxE, relE, con, iterations } (:199-208). */
+ * |con| < 1e-2.  relE for HF (:109-128) or, when rel_hfb > 0, the given
+ * HFB value (the smallest positive quasiparticle energy, :136-152); minh
+ * (:189-195), then out[6] = { dt, momentum, maxE, relE, con, iterations }
+ * (:199-208). */
 static double dotdv(const double *a, const double *b, size_t n, double dv)
 {
     double s = 0.0;
@@ -515,7 +517,7 @@ void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int 
                  const double *NablaB, const double *W, const double *DivW,
                  const double *S, const double *A, const double *DivA, double dt,
                  double momentum, double dv, double hbar, int M, const int *qn,
-                 const double *occ, const double *energy, double *out)
+                 const double *occ, const double *energy, double rel_hfb, double *out)
 {
     const int N = nx * ny * nz;
     const size_t n4 = (size_t)4 * N;
@@ -547,16 +549,20 @@ void or_estimate(double *psi, int start, int nx, int ny, int nz, double dx, int 
     }
     if (iter > 500) iter = 500;
     double relE = 10000.0, minh = 10000.0;
-    for (int i = 0; i < M; ++i) {
-        if (fabs(occ[i]) < 0.5) continue;
-        for (int ii = 0; ii < M; ++ii) {
-            if (fabs(occ[ii]) > 0.5) continue;
-            if (qn[4 * ii] != qn[4 * i]) continue;
-            const double c = -energy[i] + energy[ii];
-            if (c > 0.0 && c < relE) relE = c;
+    if (rel_hfb > 0.0) {
+        relE = rel_hfb;
+    } else {
+        for (int i = 0; i < M; ++i) {
+            if (fabs(occ[i]) < 0.5) continue;
+            for (int ii = 0; ii < M; ++ii) {
+                if (fabs(occ[ii]) > 0.5) continue;
+                if (qn[4 * ii] != qn[4 * i]) continue;
+                const double c = -energy[i] + energy[ii];
+                if (c > 0.0 && c < relE) relE = c;
+            }
         }
+        if (relE < 0.1) relE = 0.1;
     }
-    if (relE < 0.1) relE = 0.1;
     for (int i = 0; i < M; ++i)
         if (energy[i] < minh) minh = energy[i];
     const double mE = maxE - minh;

@@ -52,7 +52,7 @@ program ref_driver
   implicit none
 
   character(len=512) :: infile, prefix, tasks, arg4
-  integer            :: nwtf, i, m, it, uu, n, niter, up, upp
+  integer            :: nwtf, i, m, it, uu, n, niter, up, upp, uh
   logical            :: altcheck, havee
   real(KIND=dp), allocatable :: estart(:)
   integer, allocatable :: qn(:,:)
@@ -223,10 +223,10 @@ program ref_driver
     call SolvePairing
     if(PairingType.eq.2) then
       ! the start-up pairing step: gaps, kappa, rho, density factor, cutoffs
-      open(newunit=up, file=trim(prefix)//'_hfbstart.bin', access='stream',    &
+      open(newunit=uh, file=trim(prefix)//'_hfbstart.bin', access='stream',    &
       &    form='unformatted', status='replace')
-      write(up) dble(Delta), dble(KappaHFB), dble(RhoHFB), DensityFactor, PCutoffs
-      close(up)
+      write(uh) dble(Delta), dble(KappaHFB), dble(RhoHFB), DensityFactor, PCutoffs
+      close(uh)
     endif
     call DeriveAll
     call UpdateDensities(0)

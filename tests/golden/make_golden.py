@@ -314,6 +314,15 @@ SCF_CASES = {
     "scfhfb_trc_fd": (10, 0.8, True, False, 6, 4, 6, 10, 0,
                       dict(pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
                                    'GuessKappa=.true.', start="hf")),
+    # The sample input of the reference's README (24Mg): 12 + 12 nucleons,
+    # 20 + 20 states on the 10^3 octant of a 20^3 mesh at dx = 0.8 fm,
+    # Lagrange derivatives, ParameterEstimation 1, MixingScheme 2, HFB with
+    # 1250 MeV fm^3 and GuessKappa, Q20 = 50 fm^2 constrained for the first 10
+    # iterations, SLy4; 100 iterations from fresh states (energies 0).
+    "sample_mg24": (10, 0.8, True, True, 12, 12, 20, 100, 1,
+                    dict(mixing=2, constraints=["l=2, m=0, Constraint=50, iteration=10"],
+                         pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
+                                 'GuessKappa=.true.')),
 }
 
 
@@ -326,7 +335,7 @@ def run_scf(name):
     e0 = f0 = None
     seed = sum(map(ord, name))
     rng = np.random.default_rng(seed)
-    pattern = [(1, 1), (-1, 1), (1, 1), (-1, 1), (1, 1), (-1, 1)] if trc else \
+    pattern = [(1, 1), (-1, 1)] * (per_iso // 2) if trc else \
         [(1, 1), (1, -1), (-1, 1), (-1, -1), (1, 1), (1, -1)]
     qns = [(iso, P, S, 1) for iso in (-1, 1) for (P, S) in pattern[:per_iso]]
     nwt = len(qns)

@@ -281,7 +281,8 @@ DENSITY_KEYS = ("rho", "tau", "nablaJ", "derrho", "laprho")
 DENSITY_KEYS_ODD = ("vecj", "rotvecj", "vecs", "rots")
 
 
-def estimate(mesh, psi, start, qn_est, trc, kin, F, dt, mom, dv, hbar, qn, occ, energy):
+def estimate(mesh, psi, start, qn_est, trc, kin, F, dt, mom, dv, hbar, qn, occ, energy,
+             rel_hfb=0.0):
     """IterativeEstimation on maxspwf psi (updated in place); returns
     (dt, momentum, maxE, relE, con, iterations)."""
     out = np.zeros(6)
@@ -294,7 +295,8 @@ def estimate(mesh, psi, start, qn_est, trc, kin, F, dt, mom, dv, hbar, qn, occ, 
                       ctypes.c_double(dt), ctypes.c_double(mom), ctypes.c_double(dv),
                       ctypes.c_double(hbar), M, P(np.ascontiguousarray(qn, dtype=np.int32)),
                       P(np.ascontiguousarray(occ, dtype=np.float64)),
-                      P(np.ascontiguousarray(energy, dtype=np.float64)), P(out))
+                      P(np.ascontiguousarray(energy, dtype=np.float64)),
+                      ctypes.c_double(rel_hfb), P(out))
     return out
 
 
@@ -590,8 +592,12 @@ def scf_run(g, niter):
         Ff = {key: np.ascontiguousarray(v).reshape(-1) for key, v in F.items()}
         if est:
             it_est = (qn_est[0] + 3) // 2 - 1
+            rel = 0.0
+            if hfb is not None:
+                pos = np.concatenate([q[q > 0.0] for q in hfb.qpe.values()])
+                rel = min(10000.0, pos.min()) if pos.size else 10000.0
             r = estimate(mesh, est_psi, k == 0, qn_est, trc, kin2[it_est], Ff, dt, mom, dv,
-                         hbar, qn, occ, e_sp)
+                         hbar, qn, occ, e_sp, rel)
             dt, mom = r[0], r[1]
             dt_over_hbar = dt / hbar
             params.append((dt, mom))
@@ -844,7 +850,8 @@ class HFB:
         norm = np.zeros(2)
         for it_ in range(1, self.iters + 1):
             det = jac.copy()
-            inv = np.where(np.abs(det) < 0.001, np.copysign(0.99, det), 1.0 / det)
+            with np.errstate(divide="ignore"):
+                inv = np.where(np.abs(det) < 0.001, np.copysign(0.99, det), 1.0 / det)
             for it in range(2):
                 if conv[it]:
                     continue

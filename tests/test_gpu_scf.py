@@ -290,6 +290,44 @@ def test_iterate_hfb_vs_reference(coulomb):
     s.close()
 
 
+@pytest.mark.parametrize("prec", [mocca_amd.FP64, mocca_amd.MIXED])
+def test_sample_input_vs_reference(prec):
+    """The reference's README sample input end to end on the device: 24Mg on
+    the Lagrange mesh with IterativeEstimation, MixingScheme 2, HFB pairing
+    and a Q20 constraint for the first 10 of 100 iterations.  FP64: energy
+    of every iteration within 1e-9 relative; mixed: within 1 keV."""
+    g = golden("sample_mg24")
+    m = g["meta"]
+    n = m["niter"]
+    print(f"sample input: precision {prec}", flush=True)
+    s = _solve(g, np.zeros(m["nwt"]), precision=prec)
+    s.set_pairing(type=2, strength=m["pairing_strength"], alpha=m["pairing_alpha"],
+                  cut=m["pairing_cut"], mu=m["pairing_mu"], rhosat=m["rhosat"],
+                  hfb_mix=m["hfb_mix"], iterations=m["hfb_iter"], guess_kappa=True)
+    s.estimation_seed(g["est_seed"])
+    s.scf_start()
+    rows, prm = s.iterate_estimated(n, m["dt"], m["momentum"])
+    traj = _traj_cols(rows)
+    if prec == mocca_amd.FP64:
+        # rows are compared on the scale of the run's energies (the total
+        # energy passes through zero near iteration 20), the last one
+        # relative to itself
+        diff = np.abs(traj[:, 0] - g["traj"][:, 0])
+        scale = np.maximum(np.abs(g["traj"][:, 0]), 100.0)
+        print(f"  max scaled energy difference {(diff / scale).max():.2e}, final "
+              f"{diff[-1] / abs(g['traj'][-1, 0]):.2e}", flush=True)
+        assert (diff / scale).max() < 1e-9
+        assert diff[-1] / abs(g["traj"][-1, 0]) < 1e-10
+        # dt, momentum of the power iteration (stopped at a 1e-2 MeV change)
+        assert relerr(prm, g["params"]) < 1e-8
+        assert np.max(np.abs(rows[:, 43:47] - g["pairing"][:, [2, 3, 0, 1]])) < 1e-7
+    else:
+        err = abs(traj[-1, 0] - g["traj"][-1, 0]) * 1000.0
+        print(f"  mixed precision: final energy error {err:.3f} keV", flush=True)
+        assert err < 1.0
+    s.close()
+
+
 @pytest.mark.parametrize("case", ["scf_trc_fd", "scf_todd_fd"])
 def test_entry_points_match_fused_loop(case):
     """The reference's call sequence through the separate entry points gives

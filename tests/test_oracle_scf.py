@@ -180,6 +180,22 @@ def test_hfb_start_matrices_vs_reference():
             assert np.max(np.abs(mine[(P, it)] - ref)) < 1e-11 * max(1.0, np.abs(ref).max())
 
 
+def test_sample_input_vs_reference():
+    """The reference's README sample (24Mg, Lagrange mesh, IterativeEstimation,
+    MixingScheme 2, HFB with GuessKappa, a Q20 constraint for 10 iterations):
+    all 100 iterations, energy to 1e-9 of the run's energy scale, the final
+    one to 1e-10 relative."""
+    g = golden("sample_mg24")
+    r = ol.scf_run(g, g["meta"]["niter"])
+    diff = np.abs(r["traj"][:, 0] - g["traj"][:, 0])
+    assert (diff / np.maximum(np.abs(g["traj"][:, 0]), 100.0)).max() < 1e-9
+    assert diff[-1] / abs(g["traj"][-1, 0]) < 1e-10
+    assert np.max(np.abs(r["pairing"] - g["pairing"])) < 1e-8
+    assert relerr(r["params"], g["params"]) < 1e-9
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-9
+    assert g["traj"][-1, 0] < -190.0             # bound 24Mg after 100 iterations
+
+
 # ---- checks fixed by the mathematics ---------------------------------------
 
 def test_hfb_generalized_density_is_a_projector():
