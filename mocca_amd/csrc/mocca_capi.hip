@@ -282,6 +282,12 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
     const T *dsrc = stale ? (const T *)h->scf.alt_old : psi;
     const T *vsrc = stale ? psi : nullptr;
     h->scf.alt_pending = false;
+    // no previous states yet (start, upload, reset): u = 0
+    if (!h->prev_valid) {
+        HIPCHK(h, hipMemcpyAsync(upd, dsrc, (size_t)h->M * 4 * h->N * sizeof(T),
+                                 hipMemcpyDeviceToDevice, h->st));
+        h->prev_valid = true;
+    }
     int rc, slots;
     if (h->march) {
         slots = h->ntile_y;
@@ -306,8 +312,12 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
                          h->b.E, h->b.D, h->b.K, h->st);
     });
     if (rc) return rc;
-    // Gram-Schmidt of the stepped states psi2 -> psi, with u = psi_new - u.
-    return gram_schmidt_impl<T>(h, psi2, psi, upd, 1);
+    // Gram-Schmidt of the stepped states psi2 into the buffer of the previous
+    // states, which then become the current ones; the states before the
+    // step are the new previous ones (u = psi_new - psi_old next time).
+    if ((rc = gram_schmidt_impl<T>(h, psi2, upd, nullptr, 0))) return rc;
+    std::swap(h->b.psi, h->b.upd);
+    return 0;
 }
 
 // hPsi of every state without outputs other than the per-state sums; only
@@ -971,6 +981,7 @@ int mocca_upload_states(mocca_solve *h, int first, int count, const double *grid
     HIPCHK(h, hipStreamSynchronize(h->st));
     const size_t n4 = 4 * (size_t)h->N;
     h->scf.alt_pending = false;
+    h->prev_valid = false;
     return up(h, (char *)h->b.psi + first * n4 * h->tsize, grid, count * n4);
 }
 
@@ -987,7 +998,7 @@ int mocca_download_states(mocca_solve *h, int first, int count, double *grid) {
 
 int mocca_reset_updates(mocca_solve *h) {
     if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
-    HIPCHK(h, hipMemsetAsync(h->b.upd, 0, (size_t)h->M * 4 * h->N * h->tsize, h->st));
+    h->prev_valid = false;   // the next GradDesc starts with u = 0
     return 0;
 }
 

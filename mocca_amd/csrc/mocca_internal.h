@@ -19,6 +19,8 @@ struct Pending {
 };
 
 struct Buffers {
+    // psi: current states; psi2: stepped states; upd: the states before the
+    // last GradDesc step (the heavy-ball memory is u = psi - upd)
     void *psi = nullptr, *psi2 = nullptr, *upd = nullptr, *hbuf = nullptr;
     void *fields = nullptr;        // T [48][N]
     void *dpart = nullptr;         // T [nchunks][9][N]
@@ -152,6 +154,7 @@ struct mocca_solve {
     mocca::MeshK<double> md{};
     mocca::MeshK<float> mf{};
     bool meta_set = false, fields_set = false;
+    bool prev_valid = false;   // b.upd holds the states before the last GradDesc step
     double kin[2] = {0, 0};
     std::vector<int> qn;
     std::vector<double> occ;

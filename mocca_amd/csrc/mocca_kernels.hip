@@ -205,9 +205,12 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
         const int s = order[ch.start + q];
         const T *ps = psi + (long)s * 4 * N;
         T v[4] = {0, 0, 0, 0}, d[3][4] = {{0}}, l[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0};
+        T vd[4] = {0, 0, 0, 0};   // values of the derivative source
         if (valid) {
             state_derivs(m, ps, qn[4 * s + 1], qn[4 * s + 2], qn[4 * s + 3], x, y, z,
                          p, v, d, l, true);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) vd[c] = v[c];
             if constexpr (STALE) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) v[c] = vsrc[(long)s * 4 * N + (long)c * N + p];
@@ -271,9 +274,9 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
                 T *po = psi_out + (long)s * 4 * N;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const T uo = us[(long)c * N + p];
-                    po[(long)c * N + p] = v[c] + (-(dth * h[c]) + mu * uo);
-                    us[(long)c * N + p] = v[c];
+                    // u = psi - psi_prev (ImaginaryTime.f90:497-499)
+                    const T u = vd[c] - us[(long)c * N + p];
+                    po[(long)c * N + p] = v[c] + (-(dth * h[c]) + mu * u);
                 }
             }
         }
@@ -850,6 +853,9 @@ __global__ __launch_bounds__(1024) void k_cholesky_lds(const double *__restrict_
 
 // Q = Psi_sorted T for each block, fused with the heavy-ball memory update
 // u = psi_new - u of GradDesc (ImaginaryTime.f90:497-499) when update_mode.
+// The T entries are the same for every lane: read from memory with
+// workgroup-uniform addresses they become scalar loads, so the products
+// take T from scalar registers instead of staging it through LDS.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gs_apply(const T *__restrict__ src,
                                                      T *__restrict__ dst,

@@ -138,6 +138,8 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
             hda = T(0.5) * Fda[p];
         }
         if (mode & kStep) {
+            // heavy-ball memory u = psi - psi_prev (ImaginaryTime.f90:497-499):
+            // the previous states are kept instead of u
 #pragma unroll
             for (int c = 0; c < 4; ++c) uo[c] = us[c * N + p];
         }
@@ -226,8 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const T vc = val[c];
-                    po[c * N + p] = vc + (-(dth * h[c]) + mu * uo[c]);
-                    us[c * N + p] = vc;
+                    const T u = q[c][4] - uo[c];
+                    po[c * N + p] = vc + (-(dth * h[c]) + mu * u);
                 }
             }
         }
