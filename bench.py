@@ -112,16 +112,22 @@ def full_iteration_run(w, precision, steps, warmup, profile):
     return (t1 - t0) / steps, prof, rows, cg_iters
 
 
-def measured_traffic(kernel_prefix):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
-    summary (2 x FETCH_SIZE + WRITE_SIZE, in KiB per dispatch; the factor 2
-    is the FETCH_SIZE calibration against a plain copy), or None."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the counter table of the
+    committed rocprofv3 summary (2 x FETCH_SIZE + WRITE_SIZE, in KiB per
+    dispatch; the factor 2 is the FETCH_SIZE calibration against a plain
+    copy), or None."""
     path = os.path.join(ROOT, "profiles", "hpsi_step_gram_density_fd32_200states.md")
     if not os.path.exists(path):
         return None
+    in_counters = False
     for line in open(path):
+        if line.startswith("## "):
+            in_counters = "traffic counters" in line
+        if not in_counters:
+            continue
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].strip("`").startswith(kernel_prefix):
+        if len(cells) == 5 and cells[0].strip("`") == kernel:
             try:
                 return (2 * float(cells[1]) + float(cells[3])) * 1024
             except ValueError:
@@ -202,17 +208,19 @@ def main():
     t32, prof32, e32 = timed_run(w, MIXED, args.steps, args.warmup, True)
 
     # roofline of the dominant kernel (hpsi_step): algorithmic bytes per launch
-    # = read psi + read u + write psi' + write u for all states, plus the
-    # potentials of both isospins once (25 fields per isospin, time-odd).
+    # = read psi + read the states before the last step (the heavy-ball
+    # memory) + write psi' for all states, plus the potentials of both
+    # isospins once (25 fields per isospin, time-odd).
     s_bytes = 4 * N * 8
     nfields = 18 if w["trc"] else 25
-    alg_bytes = 4 * M * s_bytes + 2 * nfields * N * 8
+    alg_bytes = 3 * M * s_bytes + 2 * nfields * N * 8
     ms, cnt = prof["hpsi_step"]
     t_kernel = ms / cnt / 1e3
     achieved = alg_bytes / t_kernel / 1e9
     tot_ms = sum(v[0] for v in prof.values()) / args.steps
 
-    iter_bytes = 10 * M * s_bytes          # ten spinor passes per iteration
+    # seven state passes per iteration: hPsi/step 3, Gram 1, apply 2, densities 1
+    iter_bytes = 7 * M * s_bytes
     out = {
         "metric": METRIC,
         "value": 1.0 / t64,
@@ -234,7 +242,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("mocca::k_hpsi_fd<double"),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(
+                "mocca::k_hpsi_fd<double, 4, %s, false>" % ("true" if w["trc"] else "false")),
             "kernel": "hpsi_step", "kernel_ms": t_kernel * 1e3,
             "alg_bytes_per_launch": alg_bytes,
         },

@@ -194,9 +194,10 @@ int mocca_mix_densities(mocca_solve *h);
  * into the potentials used by hPsi / GradDesc. */
 int mocca_construct_potentials(mocca_solve *h);
 
-/* CompEnergy (Energy.f90:62-179): e[43] = total, kinetic(2), Coulomb,
+/* CompEnergy (Energy.f90:62-179): e[47] = total, kinetic(2), Coulomb,
  * Coulomb exchange, one-body c.o.m.(2), Skyrme terms 1..36
- * (compSkyrme, Energy.f90:623-859). */
+ * (compSkyrme, Energy.f90:623-859), pairing energy(2) and Fermi energy(2)
+ * (zero for HF; HFBEnergy, HFB.f90:296-348). */
 int mocca_comp_energy(mocca_solve *h, double *e);
 
 /* Start-up of Evolve (Main.f90:172-225): SolvePairing, UpdateDensities,
@@ -204,7 +205,7 @@ int mocca_comp_energy(mocca_solve *h, double *e);
 int mocca_scf_start(mocca_solve *h);
 
 /* niter iterations of the Evolve loop (Main.f90:255-328) on the device.
- * energies (may be NULL): [niter+1][43] in the layout of mocca_comp_energy,
+ * energies (may be NULL): [niter+1][47] in the layout of mocca_comp_energy,
  * row 0 for the states on entry, row k after iteration k. */
 int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
                   double *energies);

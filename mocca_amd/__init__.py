@@ -354,7 +354,7 @@ class Solve:
 
     def iterate_estimated(self, niter, dt, momentum, first_iteration=1):
         """mocca_iterate with IterativeEstimation before every step: returns
-        (energy rows [niter+1][43], params [niter][2] = dt, momentum used)."""
+        (energy rows [niter+1][47], params [niter][2] = dt, momentum used)."""
         e = np.zeros((niter + 1, ROW_LEN))
         prm = np.zeros((max(niter, 1), 2))
         self._chk(self.lib.mocca_iterate_estimated(self.h, int(first_iteration), int(niter),

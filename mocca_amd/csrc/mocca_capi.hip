@@ -264,6 +264,11 @@ int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_m
                            h->chol_lds, h->st);
     });
     if (rc) return rc;
+    if (h->mfma_gram && !update_mode)
+        return run(h, "gs_apply", [&] {
+            launch_gs_apply_mfma<T>(src, dst, K4, h->b.perm, h->b.blk_m, h->nblk, h->max_m,
+                                    h->mpad, h->b.Tm, h->st);
+        });
     return run(h, "gs_apply", [&] {
         launch_gs_apply<T>(src, dst, upd, K4, h->b.perm, h->b.blk_m, h->nblk, h->max_m,
                            h->b.Tm, update_mode, h->st);
@@ -907,10 +912,11 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     h->mfma_gram = h->mpad >= 64 && h->mpad <= kMfmaMaxRows;
     const int nt16 = h->mpad / 16, ntile16 = nt16 * (nt16 + 1) / 2;
     {
-        // ~4096 single-wave workgroups, partial tiles under 15% of the
-        // states' bytes, at least 4 chunks of 16 values per wave
+        // one round of single-wave workgroups (1024 = 256 CUs x 4 SIMDs at
+        // one wave per SIMD), partial tiles under 15% of the states' bytes,
+        // at least 4 chunks of 16 values per wave
         const double data = (double)h->M * K4 * h->tsize;
-        long ns = std::max(1, 4096 / nblk);
+        long ns = std::max(1, 1024 / nblk);
         ns = std::min<long>(ns, (long)(0.15 * data / ((double)nblk * ntile16 * 256 * 8)));
         ns = std::min<long>(ns, std::max<long>(1, K4 / 64));
         h->nsplit_mfma = (int)std::max<long>(1, ns);

@@ -145,6 +145,12 @@ template <typename T>
 void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
                      const int *blk_m, int nblk, int max_m, const double *Tm,
                      int update_mode, hipStream_t st);
+// The apply on the FP64 matrix cores for blocks of 64..128 states
+// (mpad_max = the largest block padded to 16, a multiple of 16 in 64..128).
+template <typename T>
+void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
+                          int nblk, int max_m, int mpad_max, const double *Tm,
+                          hipStream_t st);
 
 // Gram matrices on the FP64 matrix cores (v_mfma_f64_16x16x4f64) for blocks
 // of 64..kMfmaMaxRows states (one wave holds all upper 16x16 tiles).

@@ -929,6 +929,7 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
 // register r -- measured on gfx950).
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
+
 __device__ __forceinline__ mfma_d4 mfma16x16x4(double a, double b, mfma_d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -961,25 +962,34 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
     mfma_d4 acc[NTILE];
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) acc[t] = mfma_d4{0, 0, 0, 0};
-    // chunk c+1 is loaded while the MFMAs of chunk c run
-    auto load = [&](long c, double (&v)[NT][4]) {
+    // Chunk c+1 is loaded in the storage type, without branches (rows past
+    // the block and k past the end read row 0 / k 0), while the MFMAs of
+    // chunk c run; it is widened to FP64 afterwards, with those entries
+    // zeroed.  K4 is a multiple of 4, so a lane's 4 values are all in range
+    // or all out.
+    auto load = [&](long c, T (&raw)[NT][4]) {
         const long kb = c * 16 + 4 * (lane / 16);
+        const long ks = kb < K4 ? kb : 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I) {
-            if (ok[I] && kb + 3 < K4) {
+        for (int I = 0; I < NT; ++I)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v[I][q] = (double)rows[I][kb + q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    v[I][q] = (ok[I] && kb + q < K4) ? (double)rows[I][kb + q] : 0.0;
-            }
-        }
+            for (int q = 0; q < 4; ++q) raw[I][q] = rows[I][ks + q];
     };
-    double v[NT][4], vn[NT][4];
-    if (c0 < c1) load(c0, v);
+    auto widen = [&](long c, const T (&raw)[NT][4], double (&v)[NT][4]) {
+        const bool kin = c * 16 + 4 * (lane / 16) < K4;
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[I][q] = (kin && ok[I]) ? (double)raw[I][q] : 0.0;
+    };
+    double v[NT][4];
+    T raw[NT][4];
+    if (c0 < c1) {
+        load(c0, raw);
+        widen(c0, raw, v);
+    }
     for (long c = c0; c < c1; ++c) {
-        if (c + 1 < c1) load(c + 1, vn);
+        if (c + 1 < c1) load(c + 1, raw);
         int t = 0;
 #pragma unroll
         for (int I = 0; I < NT; ++I)
@@ -987,16 +997,148 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
             for (int J = I; J < NT; ++J, ++t)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(v[I][q], v[J][q], acc[t]);
-#pragma unroll
-        for (int I = 0; I < NT; ++I)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[I][q] = vn[I][q];
+        widen(c + 1, raw, v);   // after the last chunk: unused
     }
     double *P = part + (((long)b * nsplit + w) * NTILE) * 256 + lane * 4;
 #pragma unroll
     for (int t = 0; t < NTILE; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) P[t * 256 + r] = acc[t][r];
+}
+
+// Q = Psi_sorted T on the FP64 matrix cores, for blocks of 64..128 states,
+// as a GEMM [K4 x m] = [K4 x m] [m x m] tiled for LDS: a workgroup (4 waves)
+// owns 256 consecutive values of K and all m outputs; the sorted states and
+// T pass through LDS in chunks of 16 states, double-buffered (chunk c+1 is
+// loaded into registers while the MFMAs of chunk c run).
+//   D[16 l][16 e] += A[16 l][4 k] B[4 k][16 e]: A = T^T (lane L holds
+//   T[k0 + L/16][l0 + L%16]), B = the states (lane L holds psi_sorted
+//   [k0 + L/16][e0 + L%16]); result lane L, register r: output state
+//   l0 + L/16 + 4r at e0 + L%16, so 16 lanes store 16 consecutive values.
+// T is upper triangular (T[k][l] = 0 for k > l): chunk c skips the l tiles
+// below c.  Each wave accumulates 64 values of K x NT*16 outputs.
+constexpr int kApplyEW = 256;             // K values per workgroup
+constexpr int kApplyPS = kApplyEW + 8;    // padded LDS row of the states
+template <int NT>
+constexpr size_t gs_apply_mfma_lds() {
+    return sizeof(double) * 2 * 16 * (kApplyPS + NT * 16 + 8);
+}
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_gs_apply_mfma(const T *__restrict__ src,
+                                                       T *__restrict__ dst, long K4,
+                                                       const int *__restrict__ perm,
+                                                       const int *__restrict__ blk_m, int max_m,
+                                                       const double *__restrict__ Tm) {
+    constexpr int LT = NT * 16, PT = LT + 8;
+    extern __shared__ double lds[];   // psi [2][16][kApplyPS], T [2][16][PT]
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    const int m = blk_m[b];
+    const long eb = (long)blockIdx.x * kApplyEW;
+    const double *Tb = Tm + (long)b * max_m * max_m;
+    const int *pb = perm + (long)b * max_m;
+    const int kr = lane / 16, col = lane % 16;
+    const int nchunk = (m + 15) / 16;
+    const long my_e = eb + tid;
+    const bool my_in = my_e < K4;
+    const long my_es = my_in ? my_e : 0;
+    // global -> registers (addresses clamped in range; masked when stored)
+    T ps[16];
+    double tt[NT];
+    auto gload = [&](int c) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int k = c * 16 + r;
+            ps[r] = src[(long)pb[k < m ? k : 0] * K4 + my_es];
+        }
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int idx = tid + q * 256, r = idx / LT, l = idx % LT, k = c * 16 + r;
+            const bool in = k < m && l < m;
+            const double v = Tb[in ? (long)k * max_m + l : 0];
+            tt[q] = in ? v : 0.0;
+        }
+    };
+    auto lstore = [&](int c, int buf) {
+        double *P = lds + buf * 16 * kApplyPS;
+        double *Tl = lds + 2 * 16 * kApplyPS + buf * 16 * PT;
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            P[r * kApplyPS + tid] = (my_in && c * 16 + r < m) ? (double)ps[r] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int idx = tid + q * 256;
+            Tl[(idx / LT) * PT + idx % LT] = tt[q];
+        }
+    };
+    mfma_d4 acc[NT][4];
+#pragma unroll
+    for (int L = 0; L < NT; ++L)
+#pragma unroll
+        for (int E = 0; E < 4; ++E) acc[L][E] = mfma_d4{0, 0, 0, 0};
+    gload(0);
+    lstore(0, 0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunk) gload(c + 1);
+        const double *P = lds + buf * 16 * kApplyPS + wv * 64 + col;
+        const double *Tl = lds + 2 * 16 * kApplyPS + buf * 16 * PT + col;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int row = 4 * kk + kr;
+            double bv[4];
+#pragma unroll
+            for (int E = 0; E < 4; ++E) bv[E] = P[row * kApplyPS + 16 * E];
+#pragma unroll
+            for (int L = 0; L < NT; ++L) {
+                if (L < c) continue;
+                const double av = Tl[row * PT + 16 * L];
+#pragma unroll
+                for (int E = 0; E < 4; ++E) acc[L][E] = mfma16x16x4(av, bv[E], acc[L][E]);
+            }
+        }
+        if (c + 1 < nchunk) lstore(c + 1, buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int L = 0; L < NT; ++L)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int l = 16 * L + kr + 4 * r;
+            if (l < m) {
+                T *out = dst + (long)pb[l] * K4;
+#pragma unroll
+                for (int E = 0; E < 4; ++E) {
+                    const long e = eb + wv * 64 + 16 * E + col;
+                    if (e < K4) out[e] = (T)acc[L][E][r];
+                }
+            }
+        }
+}
+
+template <typename T>
+void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
+                          int nblk, int max_m, int mpad_max, const double *Tm,
+                          hipStream_t st) {
+    const dim3 g((K4 + kApplyEW - 1) / kApplyEW, nblk), blk(256);
+    // (the double-buffered LDS tiles exceed the default 64 KiB)
+#define MOCCA_APPLY_NT(K)                                                                   \
+    case K: {                                                                               \
+        constexpr size_t shm = gs_apply_mfma_lds<K>();                                      \
+        static const bool lds_set = hipFuncSetAttribute(                                   \
+            (const void *)k_gs_apply_mfma<T, K>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+            (int)shm) == hipSuccess;                                                        \
+        (void)lds_set;                                                                      \
+        hipLaunchKernelGGL((k_gs_apply_mfma<T, K>), g, blk, shm, st, src, dst, K4, perm,   \
+                           blk_m, max_m, Tm);                                               \
+        break;                                                                              \
+    }
+    switch (mpad_max / 16) {
+        MOCCA_APPLY_NT(4) MOCCA_APPLY_NT(5) MOCCA_APPLY_NT(6) MOCCA_APPLY_NT(7)
+        MOCCA_APPLY_NT(8)
+        default: return;
+    }
+#undef MOCCA_APPLY_NT
 }
 
 // G[b][i][j] = G[b][j][i] = dv sum_split of the tile element holding
@@ -1070,6 +1212,8 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
                                            double *, hipStream_t);                       \
     template void launch_gram<T>(const T *, long, const int *, const int *, int, int,    \
                                  int, int, double *, int, hipStream_t);                  \
+    template void launch_gs_apply_mfma<T>(const T *, T *, long, const int *, const int *, \
+                                          int, int, int, const double *, hipStream_t);    \
     template void launch_gs_apply<T>(const T *, T *, T *, long, const int *,             \
                                      const int *, int, int, const double *, int,         \
                                      hipStream_t);                                       \
