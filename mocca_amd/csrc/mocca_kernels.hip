@@ -1016,15 +1016,16 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
 //   [k0 + L/16][e0 + L%16]); result lane L, register r: output state
 //   l0 + L/16 + 4r at e0 + L%16, so 16 lanes store 16 consecutive values.
 // T is upper triangular (T[k][l] = 0 for k > l): chunk c skips the l tiles
-// below c.  Each wave accumulates 64 values of K x NT*16 outputs.
-constexpr int kApplyEW = 256;             // K values per workgroup
+// below c.  Each wave accumulates kApplyEW/4 values of K x NT*16 outputs.
+constexpr int kApplyEW = 128;             // K values per workgroup
+constexpr int kApplyNE = kApplyEW / 64;   // 16-value K tiles per wave
 constexpr int kApplyPS = kApplyEW + 8;    // padded LDS row of the states
 template <int NT>
 constexpr size_t gs_apply_mfma_lds() {
     return sizeof(double) * 2 * 16 * (kApplyPS + NT * 16 + 8);
 }
 template <typename T, int NT>
-__global__ __launch_bounds__(256) void k_gs_apply_mfma(const T *__restrict__ src,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gs_apply_mfma(const T *__restrict__ src,
                                                        T *__restrict__ dst, long K4,
                                                        const int *__restrict__ perm,
                                                        const int *__restrict__ blk_m, int max_m,
@@ -1038,17 +1039,21 @@ __global__ __launch_bounds__(256) void k_gs_apply_mfma(const T *__restrict__ src
     const int *pb = perm + (long)b * max_m;
     const int kr = lane / 16, col = lane % 16;
     const int nchunk = (m + 15) / 16;
-    const long my_e = eb + tid;
+    // staging: thread tid moves K value eb + tid % kApplyEW of the chunk rows
+    // tid / kApplyEW + RS j
+    constexpr int RS = 256 / kApplyEW, RPT = 16 / RS;
+    const int my_col = tid % kApplyEW, my_row = tid / kApplyEW;
+    const long my_e = eb + my_col;
     const bool my_in = my_e < K4;
     const long my_es = my_in ? my_e : 0;
     // global -> registers (addresses clamped in range; masked when stored)
-    T ps[16];
+    T ps[RPT];
     double tt[NT];
     auto gload = [&](int c) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int k = c * 16 + r;
-            ps[r] = src[(long)pb[k < m ? k : 0] * K4 + my_es];
+        for (int j = 0; j < RPT; ++j) {
+            const int k = c * 16 + my_row + RS * j;
+            ps[j] = src[(long)pb[k < m ? k : 0] * K4 + my_es];
         }
 #pragma unroll
         for (int q = 0; q < NT; ++q) {
@@ -1062,39 +1067,41 @@ __global__ __launch_bounds__(256) void k_gs_apply_mfma(const T *__restrict__ src
         double *P = lds + buf * 16 * kApplyPS;
         double *Tl = lds + 2 * 16 * kApplyPS + buf * 16 * PT;
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-            P[r * kApplyPS + tid] = (my_in && c * 16 + r < m) ? (double)ps[r] : 0.0;
+        for (int j = 0; j < RPT; ++j) {
+            const int r = my_row + RS * j;
+            P[r * kApplyPS + my_col] = (my_in && c * 16 + r < m) ? (double)ps[j] : 0.0;
+        }
 #pragma unroll
         for (int q = 0; q < NT; ++q) {
             const int idx = tid + q * 256;
             Tl[(idx / LT) * PT + idx % LT] = tt[q];
         }
     };
-    mfma_d4 acc[NT][4];
+    mfma_d4 acc[NT][kApplyNE];
 #pragma unroll
     for (int L = 0; L < NT; ++L)
 #pragma unroll
-        for (int E = 0; E < 4; ++E) acc[L][E] = mfma_d4{0, 0, 0, 0};
+        for (int E = 0; E < kApplyNE; ++E) acc[L][E] = mfma_d4{0, 0, 0, 0};
     gload(0);
     lstore(0, 0);
     __syncthreads();
     for (int c = 0; c < nchunk; ++c) {
         const int buf = c & 1;
         if (c + 1 < nchunk) gload(c + 1);
-        const double *P = lds + buf * 16 * kApplyPS + wv * 64 + col;
+        const double *P = lds + buf * 16 * kApplyPS + wv * (16 * kApplyNE) + col;
         const double *Tl = lds + 2 * 16 * kApplyPS + buf * 16 * PT + col;
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int row = 4 * kk + kr;
-            double bv[4];
+            double bv[kApplyNE];
 #pragma unroll
-            for (int E = 0; E < 4; ++E) bv[E] = P[row * kApplyPS + 16 * E];
+            for (int E = 0; E < kApplyNE; ++E) bv[E] = P[row * kApplyPS + 16 * E];
 #pragma unroll
             for (int L = 0; L < NT; ++L) {
                 if (L < c) continue;
                 const double av = Tl[row * PT + 16 * L];
 #pragma unroll
-                for (int E = 0; E < 4; ++E) acc[L][E] = mfma16x16x4(av, bv[E], acc[L][E]);
+                for (int E = 0; E < kApplyNE; ++E) acc[L][E] = mfma16x16x4(av, bv[E], acc[L][E]);
             }
         }
         if (c + 1 < nchunk) lstore(c + 1, buf ^ 1);
@@ -1108,8 +1115,8 @@ __global__ __launch_bounds__(256) void k_gs_apply_mfma(const T *__restrict__ src
             if (l < m) {
                 T *out = dst + (long)pb[l] * K4;
 #pragma unroll
-                for (int E = 0; E < 4; ++E) {
-                    const long e = eb + wv * 64 + 16 * E + col;
+                for (int E = 0; E < kApplyNE; ++E) {
+                    const long e = eb + wv * (16 * kApplyNE) + 16 * E + col;
                     if (e < K4) out[e] = (T)acc[L][E][r];
                 }
             }
