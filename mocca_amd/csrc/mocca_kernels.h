@@ -31,6 +31,25 @@ struct MeshK {
     const T *lag2[3][2];
 };
 
+// 9-point second derivative of v[0..8] centred on v[4] (lapla_*_ev8,
+// OptimizedDerivatives.f90:617-795).  FP64 keeps the reference's sum with
+// the centre weight; FP32 sums the differences from the centre instead, which
+// is the same operator (l2m = -2 (1 + l2c2 + l2c3 + l2c4)) but maps a constant
+// to exactly 0: the rounded FP32 centre weight would leave an offset of
+// ~1e-7 |v| / dx^2, a systematic bias of the kinetic energy (~1 keV for 200
+// nucleons).
+template <typename T>
+__device__ __forceinline__ T lap9(const MeshK<T> &m, const T v[9]) {
+    if constexpr (sizeof(T) == 4) {
+        const T c = v[4] + v[4];
+        return m.l2scale * (((v[5] + v[3]) - c) + m.l2c2 * ((v[6] + v[2]) - c) +
+                            m.l2c3 * ((v[7] + v[1]) - c) + m.l2c4 * ((v[8] + v[0]) - c));
+    } else {
+        return m.l2scale * (m.l2m * v[4] + (v[5] + v[3]) + m.l2c2 * (v[6] + v[2]) +
+                            m.l2c3 * (v[7] + v[1]) + m.l2c4 * (v[8] + v[0]));
+    }
+}
+
 // Chunk of states of one isospin: states order[start .. start+count-1].
 struct Chunk {
     int start, count, it, pad;

@@ -64,8 +64,7 @@ __device__ __forceinline__ void axis_derivs(const MeshK<T> &m,
 #pragma unroll
         for (int k = -4; k <= 4; ++k) v[k + 4] = line_at(line, i + k, n, stride, s);
         if (need2)
-            d2 = m.l2scale * (m.l2m * v[4] + (v[5] + v[3]) + m.l2c2 * (v[6] + v[2]) +
-                              m.l2c3 * (v[7] + v[1]) + m.l2c4 * (v[8] + v[0]));
+            d2 = lap9(m, v);
         if (s1 != s2) {
             const T s1t = s1 > 0 ? T(1) : T(-1);
 #pragma unroll

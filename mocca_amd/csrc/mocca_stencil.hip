@@ -22,8 +22,7 @@ __device__ __forceinline__ int comp_sign_z(int c, int S, int P) { return c < 2 ?
 template <typename T>
 __device__ __forceinline__ void fd_pair(const MeshK<T> &m, const T v[9], T &d1, T &d2) {
     d1 = (T(45) * (v[5] - v[3]) - T(9) * (v[6] - v[2]) + (v[7] - v[1])) * m.inv60dx;
-    d2 = m.l2scale * (m.l2m * v[4] + (v[5] + v[3]) + m.l2c2 * (v[6] + v[2]) +
-                      m.l2c3 * (v[7] + v[1]) + m.l2c4 * (v[8] + v[0]));
+    d2 = lap9(m, v);
 }
 
 __device__ __forceinline__ void block_sum3_fd(double &a, double &b, double &c, double *sh) {
