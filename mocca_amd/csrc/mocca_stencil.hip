@@ -251,11 +251,365 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The GradDesc step (kStep, no stale derivatives) with a two-stage pipeline:
+// while plane z is computed, the y-halo rows and the potentials of plane z+1
+// travel to LDS by direct-to-LDS loads (global_load_lds_dwordx4: lane L of a
+// wave moves 16 bytes to LDS base + 16 L), so no registers hold data in
+// flight and the loads overlap the arithmetic.  One barrier per plane: the
+// compiler waits for the outstanding direct-to-LDS loads there.
+//   LDS (elements of T), each double-buffered by plane parity:
+//     X[4][TY][nx+8]   the tile's own rows with x ghosts (reflection, zeros),
+//                      written from the register queue (x stencil);
+//     Y[4][TY+8][nx]   the same rows plus 4 halo rows above and below (y
+//                      stencil); halo rows below the symmetry plane arrive
+//                      unsigned from the mirrored rows and rows past the box
+//                      from the last row: the y stencil of the edge tiles
+//                      applies the reflection sign / zero when it reads them;
+//     P[NF][TNP]       the potentials of the tile (TNP: TY nx rounded up to
+//                      whole 1 KiB loads).
+// The new values of plane z are stored at the start of plane z+1, so their
+// completion is not waited for at the next barrier.  Requires nx * sizeof(T)
+// to be a multiple of 16 (whole 16-byte lane loads inside a row).
+typedef __attribute__((address_space(3))) void lds_raw_t;
+
+template <typename T>
+__device__ __forceinline__ void dma16(const T *g, T *lds_base) {
+    __builtin_amdgcn_global_load_lds((const void *)g, (lds_raw_t *)lds_base, 16, 0, 0);
+}
+
+__device__ __forceinline__ int pot_slot(int j, int it) {
+    if (j == 0) return kOffU + it;
+    if (j == 1) return kOffB + it;
+    if (j < 5) return kOffNB + 3 * it + (j - 2);
+    if (j < 14) return kOffW + 9 * it + (j - 5);
+    if (j < 17) return kOffDW + 3 * it + (j - 14);
+    if (j < 20) return kOffS + 3 * it + (j - 17);
+    if (j < 23) return kOffA + 3 * it + (j - 20);
+    return kOffDA + it;
+}
+
+template <typename T, bool TRC>
+constexpr int hpsi_dma_fields() { return TRC ? 17 : 24; }
+
+template <typename T, bool TRC>
+size_t hpsi_dma_lds(int nx, int TY) {
+    constexpr int EPI = 1024 / (int)sizeof(T);
+    const int TNP = (TY * nx + EPI - 1) / EPI * EPI;
+    return sizeof(T) * ((size_t)2 * 4 * TY * (nx + 8) + (size_t)2 * 4 * (TY + 8) * nx +
+                        (size_t)2 * hpsi_dma_fields<T, TRC>() * TNP);
+}
+
+// Two threads per point: lanes 0-31 of a wave own spinor components 0 and
+// 1 of 32 points, lanes 32-63 components 2 and 3 of the same points.  Each
+// thread forms the derivatives of its two components and their
+// contributions to all four components of h psi; the partners then swap the
+// partial sums of each other's components (a lane-distance-32 shuffle).  This
+// halves the register queue per thread, so two waves share a SIMD.
+constexpr int kDmaThreads = 2 * kBlock;
+
+template <typename T, bool TRC>
+__global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
+    MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
+    const T *__restrict__ prev, const T *__restrict__ F, const int *__restrict__ qn,
+    const Task *__restrict__ tasks, int TY, int ntile_y, T kin0, T kin1, T dth, T mu,
+    double *__restrict__ eparts) {
+    constexpr int G = 4;
+    constexpr bool trc = TRC;
+    constexpr int NF = hpsi_dma_fields<T, TRC>();
+    constexpr int NW = kDmaThreads / 64;       // waves
+    constexpr int VPL = 16 / (int)sizeof(T);   // values per lane of one 16-byte load
+    constexpr int EPI = 64 * VPL;              // values per wave load (1 KiB)
+    extern __shared__ double smem_d[];
+    __shared__ double red[3 * NW];
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const long N = m.N, plane = (long)nx * ny;
+    const int W = nx + 2 * G, YR = TY + 8;
+    const int TNP = (TY * nx + EPI - 1) / EPI * EPI;
+    T *const Xb = reinterpret_cast<T *>(smem_d);
+    T *const Yb = Xb + 2 * 4 * TY * W;
+    T *const Pb = Yb + 2 * 4 * YR * nx;
+    const Task tk = tasks[blockIdx.x];
+    const int s = tk.state, tile = tk.tile;
+    const int it = (qn[4 * s] + 3) / 2 - 1;
+    const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
+    const T kin = it == 0 ? kin0 : kin1;
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int half = lane / 32;                // components 2 half, 2 half + 1
+    const int pt = wave * 32 + lane % 32;      // point of the tile plane
+    const int tyl = pt / nx, tx = pt % nx;
+    const int y0 = tile * TY, y = y0 + tyl;
+    const bool inplane = tyl < TY;
+    const bool active = inplane && (y < ny);
+    const bool edge = y0 < 4 || y0 + TY + 4 > ny;
+    const int rows_valid = min(TY, ny - y0);
+    T sgx[2], sgy[2], sgz[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int c = 2 * half + a;
+        sgx[a] = T(comp_sign_x(c, S));
+        sgy[a] = T(comp_sign_y(c, TS));
+        sgz[a] = T(comp_sign_z(c, S, P));
+    }
+    const long cN = (long)(2 * half) * N;      // offset of this thread's first component
+    const T *ps = psi + (long)s * 4 * N;
+    const T *pv = prev + (long)s * 4 * N + cN;
+    T *po = psi_out + (long)s * 4 * N + cN;
+    const long col = (long)(active ? y : 0) * nx + tx;
+    // the register queue of the z stencil: planes z-4 .. z+4 of both components
+    T q[2][9];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int zz = k - 4;
+            const int zi = zz < 0 ? -1 - zz : (zz < nz ? zz : nz - 1);
+            const T val = ps[cN + a * N + zi * plane + col];
+            q[a][k] = !active || zz >= nz ? T(0) : (zz < 0 ? sgz[a] * val : val);
+        }
+    // Direct-to-LDS loads of plane zz into buffer b.  Wave w < 8 moves one
+    // halo block (component w/2, 4 rows above (w even) or below the tile:
+    // nhb <= 2 loads) and the potentials j = w, w+NW, ... (nfc <= 2 loads
+    // each).  The lane offsets inside a load are fixed, so they are set up
+    // once here.
+    const int nhb = (4 * nx + EPI - 1) / EPI, nfc = TNP / EPI;
+    int hrow[2], hx[2];
+    bool hval[2], fval[2];
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        const int e = ch * EPI + lane * VPL;
+        hval[ch] = ch < nhb && e < 4 * nx;
+        hrow[ch] = e / nx;
+        hx[ch] = e % nx;
+        fval[ch] = ch < nfc && e < rows_valid * nx;
+    }
+    const int hc = wave >> 1, hrr0 = (wave & 1) ? TY + 4 : 0;
+    const T *psw = ps + (long)hc * N;
+    auto issue = [&](int zz, int b) {
+        if (wave < 8) {
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch) {
+                T *dst = Yb + ((b * 4 + hc) * YR + hrr0) * nx + ch * EPI;
+                if (hval[ch]) {
+                    const int yy = y0 - 4 + hrr0 + hrow[ch];
+                    const int yi = yy < 0 ? -1 - yy : (yy < ny ? yy : ny - 1);
+                    dma16(psw + (long)zz * plane + (long)yi * nx + hx[ch], dst);
+                }
+            }
+        }
+        for (int j = wave; j < NF; j += NW) {
+            const T *src = F + (long)pot_slot(j, it) * N + (long)zz * plane + (long)y0 * nx +
+                           lane * VPL;
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch) {
+                T *dst = Pb + (b * NF + j) * TNP + ch * EPI;
+                if (fval[ch]) dma16(src + ch * EPI, dst);
+            }
+        }
+    };
+    issue(0, 0);
+    double eacc = 0, dacc = 0, kacc = 0;
+    T pst[2] = {0, 0};
+    long pst_p = -1;
+    for (int z = 0; z < nz; ++z) {
+        const int b = z & 1;
+        T *const Xc = Xb + b * 4 * TY * W;
+        T *const Yc = Yb + b * 4 * YR * nx;
+        const T *const Pc = Pb + b * NF * TNP;
+        // ---- the tile's own rows of plane z from the queue ----
+        if (inplane) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int c = 2 * half + a;
+                const T v = active ? q[a][4] : T(0);
+                T *xr = Xc + (c * TY + tyl) * W;
+                xr[tx + G] = v;
+                if (tx < G) xr[G - 1 - tx] = sgx[a] * v;
+                if (tx >= nx - G) xr[tx + 2 * G] = T(0);
+                Yc[(c * YR + tyl + 4) * nx + tx] = v;
+            }
+        }
+        __syncthreads();
+        // ---- stores of plane z-1, loads of plane z+1 and of the queue ----
+        if (pst_p >= 0) {
+            po[pst_p] = pst[0];
+            po[N + pst_p] = pst[1];
+        }
+        if (z + 1 < nz) issue(z + 1, b ^ 1);
+        const long p = (long)z * plane + col;
+        const int zn = z + 5 < nz ? z + 5 : nz - 1;
+        T nxt[2], uo[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            nxt[a] = ps[cN + a * N + zn * plane + col];
+            uo[a] = pv[a * N + p];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // Contributions of this thread's two components (a = 0, 1: c = 2 half
+        // + a) to h psi (ActionOfBNew, ActionOfU: MeanFields.f90:792-851;
+        // ActionOfW :1029-1065; ActionOfS, ActionOfA :1107-1147).  With
+        // sg = +1 for components (0, 1) and -1 for (2, 3) the couplings have
+        // the same form in both halves:
+        //   own0  = hown_0 + sg ez_1 + sg s2 v_0 + aa_1
+        //   own1  = hown_1 - sg ez_0 + sg s2 v_1 - aa_0
+        //   send0 = ex_1 + sg ey_0 + s0 v_0 - sg s1 v_1   (partner's first)
+        //   send1 = -ex_0 + sg ey_1 + sg s1 v_0 + s0 v_1  (partner's second)
+        // Lanes outside the tile compute on zeros.
+        const T sg = half ? T(-1) : T(1);
+        T own0 = 0, own1 = 0, send0 = 0, send1 = 0;
+        {
+            const T *pp = Pc + (inplane ? pt : 0);
+            const T fu = pp[0], fb = pp[TNP] + kin;
+            const T nb0 = pp[2 * TNP], nb1 = pp[3 * TNP], nb2 = pp[4 * TNP];
+            const int rowi = inplane ? tyl : 0;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int c = 2 * half + a;
+                const T *row = Xc + (c * TY + rowi) * W + tx + G;
+                T w[9], dxc, dyc, dzc, lx, ly, lz;
+#pragma unroll
+                for (int k = -4; k <= 4; ++k) w[k + 4] = row[k];
+                fd_pair(m, w, dxc, lx);
+                const T *colp = Yc + (c * YR + rowi + 4) * nx + tx;
+#pragma unroll
+                for (int k = -4; k <= 4; ++k) w[k + 4] = colp[k * nx];
+                if (edge) {
+#pragma unroll
+                    for (int k = -4; k <= 4; ++k) {
+                        const int yy = y + k;
+                        w[k + 4] *= yy < 0 ? sgy[a] : (yy < ny ? T(1) : T(0));
+                    }
+                }
+                fd_pair(m, w, dyc, ly);
+                fd_pair(m, q[a], dzc, lz);
+                const T lc = lx + ly + lz, vc = q[a][4];
+                kacc += (double)vc * (double)lc;
+                const T hown = -fb * lc - (nb0 * dxc + nb1 * dyc + nb2 * dzc) + fu * vc;
+                const T ex = pp[5 * TNP] * dxc + pp[6 * TNP] * dyc + pp[7 * TNP] * dzc +
+                             T(0.5) * pp[14 * TNP] * vc;
+                const T ey = pp[8 * TNP] * dxc + pp[9 * TNP] * dyc + pp[10 * TNP] * dzc +
+                             T(0.5) * pp[15 * TNP] * vc;
+                const T ez = pp[11 * TNP] * dxc + pp[12 * TNP] * dyc + pp[13 * TNP] * dzc +
+                             T(0.5) * pp[16 * TNP] * vc;
+                if (a == 0) {
+                    own0 += hown;
+                    own1 -= sg * ez;
+                    send0 += sg * ey;
+                    send1 -= ex;
+                } else {
+                    own1 += hown;
+                    own0 += sg * ez;
+                    send0 += ex;
+                    send1 += sg * ey;
+                }
+                if (!trc) {
+                    const T s0 = pp[17 * TNP], s1 = pp[18 * TNP], s2 = pp[19 * TNP];
+                    const T aac = pp[20 * TNP] * dxc + pp[21 * TNP] * dyc + pp[22 * TNP] * dzc +
+                                  T(0.5) * pp[23 * TNP] * vc;
+                    if (a == 0) {
+                        own0 += sg * s2 * vc;
+                        send0 += s0 * vc;
+                        send1 += sg * s1 * vc;
+                        own1 -= aac;
+                    } else {
+                        own1 += sg * s2 * vc;
+                        send0 -= sg * s1 * vc;
+                        send1 += s0 * vc;
+                        own0 += aac;
+                    }
+                }
+            }
+        }
+        // the partner's partial sums of this thread's components
+        const T h0 = own0 + __shfl_xor(send0, 32, 64);
+        const T h1 = own1 + __shfl_xor(send1, 32, 64);
+        if (active) {
+            const T hh[2] = {h0, h1};
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const T vc = q[a][4];
+                eacc += (double)vc * (double)hh[a];
+                dacc += (double)hh[a] * (double)hh[a];
+                // heavy-ball memory u = psi - psi_prev (ImaginaryTime.f90:497-499)
+                const T u = vc - uo[a];
+                pst[a] = vc + (-(dth * hh[a]) + mu * u);
+            }
+            pst_p = p;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[a][k] = q[a][k + 1];
+            q[a][8] = (active && z + 5 < nz) ? nxt[a] : T(0);
+        }
+    }
+    if (pst_p >= 0) {
+        po[pst_p] = pst[0];
+        po[N + pst_p] = pst[1];
+    }
+    if (eparts) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            eacc += __shfl_xor(eacc, o, 64);
+            dacc += __shfl_xor(dacc, o, 64);
+            kacc += __shfl_xor(kacc, o, 64);
+        }
+        if (lane == 0) { red[3 * wave] = eacc; red[3 * wave + 1] = dacc; red[3 * wave + 2] = kacc; }
+        __syncthreads();
+        if (tid == 0) {
+            double sa = 0, sb = 0, sc = 0;
+            for (int k = 0; k < NW; ++k) { sa += red[3 * k]; sb += red[3 * k + 1]; sc += red[3 * k + 2]; }
+            double *ep = eparts + ((long)s * ntile_y + tile) * 3;
+            ep[0] = sa;
+            ep[1] = sb;
+            ep[2] = sc;
+        }
+    }
+}
+
 template <typename T>
 void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
                     const T *fields, const int *qn, const Task *tasks, int ntasks,
                     int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
                     int mode, double *eparts, hipStream_t st, const T *vsrc) {
+    // (whole 16-byte lane loads inside a row; the halo rows of one component
+    // in at most two 1 KiB loads; the LDS within the device's per-workgroup
+    // limit less the kernel's static reduction array)
+    // (FP64 only: the FP32 kernel below already runs two waves per SIMD and
+    // is faster than the pipelined one in FP32)
+    if constexpr (sizeof(T) == 8) {
+    if (mode == kStep && !vsrc && !hout && upd && psi_out &&
+        (m.nx * sizeof(T)) % 16 == 0 && 4 * m.nx * sizeof(T) <= 2048) {
+        static const size_t lds_limit = [] {
+            int dev = 0, lim = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                    hipSuccess)
+                return (size_t)0;
+            const size_t stat = 3 * kDmaThreads / 64 * sizeof(double);
+            return (size_t)lim > stat ? (size_t)lim - stat : (size_t)0;
+        }();
+        const size_t dshm =
+            trc ? hpsi_dma_lds<T, true>(m.nx, TY) : hpsi_dma_lds<T, false>(m.nx, TY);
+        if (dshm <= lds_limit) {
+#define MOCCA_HPSI_DMA(TR)                                                                   \
+    {                                                                                        \
+        static const bool lds_set = hipFuncSetAttribute(                                    \
+            (const void *)k_hpsi_fd_dma<T, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+            (int)lds_limit) == hipSuccess;                                                   \
+        (void)lds_set;                                                                       \
+        hipLaunchKernelGGL((k_hpsi_fd_dma<T, TR>), dim3(ntasks), dim3(kDmaThreads), dshm, st, \
+                           m,                                                                \
+                           psi, psi_out, (const T *)upd, fields, qn, tasks, TY, ntile_y,    \
+                           kin0, kin1, dth, mu, eparts);                                     \
+    }
+            if (trc) MOCCA_HPSI_DMA(true) else MOCCA_HPSI_DMA(false)
+#undef MOCCA_HPSI_DMA
+            return;
+        }
+    }
+    }
     const size_t shm = sizeof(T) * 4 * (size_t)(TY + 8) * (m.nx + 8);
 #define MOCCA_HPSI_LAUNCH(HM, TR, ST)                                                         \
     hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, ST>), dim3(ntasks), dim3(kBlock), shm, st, m,   \
