@@ -294,6 +294,18 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
         h->prev_valid = true;
     }
     int rc, slots;
+    if constexpr (sizeof(T) == 8) {
+        if (h->march && h->dma_hpsi && !vsrc) {
+            slots = h->ntile_y * h->nzs;
+            rc = run(h, "hpsi_step", [&] {
+                launch_hpsi_fd_dma(m, dsrc, psi2, upd, (const T *)h->b.fields, h->b.qn,
+                                   h->b.tasks_dma, h->ntasks_dma, h->TY, h->ntile_y, h->nzs,
+                                   h->mesh.time_reversal, h->kin[0], h->kin[1], dth, mu,
+                                   h->b.eparts, h->st);
+            });
+            goto stepped;
+        }
+    }
     if (h->march) {
         slots = h->ntile_y;
         rc = run(h, "hpsi_step", [&] {
@@ -311,6 +323,7 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
                                 h->b.eparts, h->ntiles, h->st, vsrc);
         });
     }
+stepped:
     if (rc) return rc;
     rc = run(h, "reduce_ed", [&] {
         launch_reduce_ed(h->b.eparts, slots, h->b.all_states, h->M, h->mesh.dv,
@@ -636,7 +649,7 @@ void free_all(mocca_solve *h) {
                     b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.K, b.eparts,
                     b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
                     b.G, b.Tm, b.work, b.gpart, b.status, b.jobs, b.one_chunk,
-                    b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks};
+                    b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks, b.tasks_dma};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     Est &e = h->est;
@@ -876,15 +889,40 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
             tasks.push_back(q[k][pos[k]++]);
         }
     }
+    // The pipelined FP64 step splits each (state, tile) into nzs z slabs when
+    // that fills the CUs better: cost ~ rounds x (planes per slab + ~1.5
+    // planes to start the register queue and the first loads).
+    std::vector<Task> tasks_dma;
+    h->dma_hpsi = h->march && h->prec == MOCCA_FP64 &&
+                  hpsi_dma_ok(h->md, h->TY, h->mesh.time_reversal);
+    h->nzs = 1;
+    if (h->dma_hpsi) {
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const long T1 = (long)tasks.size();
+        const int nz = h->mesh.nz;
+        double best = 0;
+        for (int k = 1; k <= 4; ++k) {
+            if (k > 1 && (nz / k < 6 || h->ntile_y * k > h->ntiles)) break;
+            const double rounds = (double)((T1 * k + ncu - 1) / std::max(ncu, 1));
+            const double cost = rounds * ((double)nz / k + 1.5);
+            if (k == 1 || cost < best) { best = cost; h->nzs = k; }
+        }
+        // slab-major, so the first round holds every (state, tile) once
+        for (int sl = 0; sl < h->nzs; ++sl)
+            for (const Task &t : tasks) tasks_dma.push_back(Task{t.state, t.tile + h->ntile_y * sl});
+    }
     // (Re)allocate index and GS buffers.
     void *old[] = {b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
-                   b.G, b.Tm, b.work, b.gpart, b.dpart, b.tasks, b.dchunks, b.dtasks};
+                   b.G, b.Tm, b.work, b.gpart, b.dpart, b.tasks, b.dchunks, b.dtasks,
+                   b.tasks_dma};
     for (void *p : old)
         if (p) (void)hipFree(p);
     b.order = nullptr; b.chunks = nullptr; b.all_states = nullptr;
     b.blk_states = nullptr; b.blk_m = nullptr; b.perm = nullptr;
     b.G = b.Tm = b.work = b.gpart = nullptr; b.dpart = nullptr; b.tasks = nullptr;
-    b.dchunks = nullptr; b.dtasks = nullptr;
+    b.dchunks = nullptr; b.dtasks = nullptr; b.tasks_dma = nullptr;
     int rc;
     if ((rc = dalloc(h, &b.order, order.size() * sizeof(int)))) return rc;
     if ((rc = dalloc(h, &b.chunks, chunks.size() * sizeof(Chunk)))) return rc;
@@ -939,6 +977,12 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     HIPCHK(h, hipMemcpy(b.tasks, tasks.data(), tasks.size() * sizeof(Task),
                         hipMemcpyHostToDevice));
     h->ntasks = (int)tasks.size();
+    if ((rc = dalloc(h, &b.tasks_dma, std::max<size_t>(tasks_dma.size(), 1) * sizeof(Task))))
+        return rc;
+    if (!tasks_dma.empty())
+        HIPCHK(h, hipMemcpy(b.tasks_dma, tasks_dma.data(), tasks_dma.size() * sizeof(Task),
+                            hipMemcpyHostToDevice));
+    h->ntasks_dma = (int)tasks_dma.size();
     HIPCHK(h, hipMemcpy(b.order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(b.chunks, chunks.data(), chunks.size() * sizeof(Chunk),
                         hipMemcpyHostToDevice));

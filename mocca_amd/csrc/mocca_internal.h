@@ -42,6 +42,7 @@ struct Buffers {
     Chunk *one_chunk = nullptr;
     int *one_order = nullptr;
     Task *tasks = nullptr, *one_tasks = nullptr;
+    Task *tasks_dma = nullptr;     // pipelined FP64 step: (state, tile + ntile_y * slab)
     Chunk *dchunks = nullptr;
     DTask *dtasks = nullptr;
 };
@@ -164,6 +165,9 @@ struct mocca_solve {
     // z-marching FD kernels: y-tile rows, tiles per plane, task count
     bool march = false;
     int TY = 0, ntile_y = 0, ntasks = 0;
+    // the pipelined FP64 step kernel: usable, z slabs per state and tile, tasks
+    bool dma_hpsi = false;
+    int nzs = 1, ntasks_dma = 0;
     int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
     size_t chol_lds = 0;   // dynamic LDS bytes available to the in-LDS Cholesky
     bool mfma_gram = false;  // Gram on the matrix cores (blocks <= kMfmaMaxRows)

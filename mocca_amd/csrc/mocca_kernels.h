@@ -111,6 +111,16 @@ void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout
                     const T *fields, const int *qn, const Task *tasks, int ntasks,
                     int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
                     int mode, double *eparts, hipStream_t st, const T *vsrc = nullptr);
+// The FP64 GradDesc step (no stale derivatives) with the next plane's halo
+// rows and potentials moved to LDS by direct-to-LDS loads while the current
+// plane is computed; tasks carry tile + ntile_y * slab for nzs z slabs, and
+// eparts gets ntile_y * nzs slots per state.
+bool hpsi_dma_ok(const MeshK<double> &m, int TY, int trc);
+void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_out,
+                        const double *prev, const double *fields, const int *qn,
+                        const Task *tasks, int ntasks, int TY, int ntile_y, int nzs, int trc,
+                        double kin0, double kin1, double dth, double mu, double *eparts,
+                        hipStream_t st);
 
 template <typename T>
 void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,

@@ -312,8 +312,8 @@ template <typename T, bool TRC>
 __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     const T *__restrict__ prev, const T *__restrict__ F, const int *__restrict__ qn,
-    const Task *__restrict__ tasks, int TY, int ntile_y, T kin0, T kin1, T dth, T mu,
-    double *__restrict__ eparts) {
+    const Task *__restrict__ tasks, int TY, int ntile_y, int nzs, T kin0, T kin1, T dth,
+    T mu, double *__restrict__ eparts) {
     constexpr int G = 4;
     constexpr bool trc = TRC;
     constexpr int NF = hpsi_dma_fields<T, TRC>();
@@ -329,8 +329,10 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     T *const Xb = reinterpret_cast<T *>(smem_d);
     T *const Yb = Xb + 2 * 4 * TY * W;
     T *const Pb = Yb + 2 * 4 * YR * nx;
+    // task: state, and tile + ntile_y * slab (z slab slab of nzs)
     const Task tk = tasks[blockIdx.x];
-    const int s = tk.state, tile = tk.tile;
+    const int s = tk.state, tile = tk.tile % ntile_y, slab = tk.tile / ntile_y;
+    const int zb = nz * slab / nzs, ze = nz * (slab + 1) / nzs;
     const int it = (qn[4 * s] + 3) / 2 - 1;
     const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
     const T kin = it == 0 ? kin0 : kin1;
@@ -357,13 +359,13 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     const T *pv = prev + (long)s * 4 * N + cN;
     T *po = psi_out + (long)s * 4 * N + cN;
     const long col = (long)(active ? y : 0) * nx + tx;
-    // the register queue of the z stencil: planes z-4 .. z+4 of both components
+    // the register queue of the z stencil: planes zb-4 .. zb+4 of both components
     T q[2][9];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            const int zz = k - 4;
+            const int zz = zb + k - 4;
             const int zi = zz < 0 ? -1 - zz : (zz < nz ? zz : nz - 1);
             const T val = ps[cN + a * N + zi * plane + col];
             q[a][k] = !active || zz >= nz ? T(0) : (zz < 0 ? sgz[a] * val : val);
@@ -408,12 +410,12 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
             }
         }
     };
-    issue(0, 0);
+    issue(zb, 0);
     double eacc = 0, dacc = 0, kacc = 0;
     T pst[2] = {0, 0};
     long pst_p = -1;
-    for (int z = 0; z < nz; ++z) {
-        const int b = z & 1;
+    for (int z = zb; z < ze; ++z) {
+        const int b = (z - zb) & 1;
         T *const Xc = Xb + b * 4 * TY * W;
         T *const Yc = Yb + b * 4 * YR * nx;
         const T *const Pc = Pb + b * NF * TNP;
@@ -436,7 +438,7 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
             po[pst_p] = pst[0];
             po[N + pst_p] = pst[1];
         }
-        if (z + 1 < nz) issue(z + 1, b ^ 1);
+        if (z + 1 < ze) issue(z + 1, b ^ 1);
         const long p = (long)z * plane + col;
         const int zn = z + 5 < nz ? z + 5 : nz - 1;
         T nxt[2], uo[2];
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
         if (tid == 0) {
             double sa = 0, sb = 0, sc = 0;
             for (int k = 0; k < NW; ++k) { sa += red[3 * k]; sb += red[3 * k + 1]; sc += red[3 * k + 2]; }
-            double *ep = eparts + ((long)s * ntile_y + tile) * 3;
+            double *ep = eparts + ((long)s * ntile_y * nzs + tk.tile) * 3;
             ep[0] = sa;
             ep[1] = sb;
             ep[2] = sc;
@@ -576,40 +578,6 @@ void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout
     // (whole 16-byte lane loads inside a row; the halo rows of one component
     // in at most two 1 KiB loads; the LDS within the device's per-workgroup
     // limit less the kernel's static reduction array)
-    // (FP64 only: the FP32 kernel below already runs two waves per SIMD and
-    // is faster than the pipelined one in FP32)
-    if constexpr (sizeof(T) == 8) {
-    if (mode == kStep && !vsrc && !hout && upd && psi_out &&
-        (m.nx * sizeof(T)) % 16 == 0 && 4 * m.nx * sizeof(T) <= 2048) {
-        static const size_t lds_limit = [] {
-            int dev = 0, lim = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
-                    hipSuccess)
-                return (size_t)0;
-            const size_t stat = 3 * kDmaThreads / 64 * sizeof(double);
-            return (size_t)lim > stat ? (size_t)lim - stat : (size_t)0;
-        }();
-        const size_t dshm =
-            trc ? hpsi_dma_lds<T, true>(m.nx, TY) : hpsi_dma_lds<T, false>(m.nx, TY);
-        if (dshm <= lds_limit) {
-#define MOCCA_HPSI_DMA(TR)                                                                   \
-    {                                                                                        \
-        static const bool lds_set = hipFuncSetAttribute(                                    \
-            (const void *)k_hpsi_fd_dma<T, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
-            (int)lds_limit) == hipSuccess;                                                   \
-        (void)lds_set;                                                                       \
-        hipLaunchKernelGGL((k_hpsi_fd_dma<T, TR>), dim3(ntasks), dim3(kDmaThreads), dshm, st, \
-                           m,                                                                \
-                           psi, psi_out, (const T *)upd, fields, qn, tasks, TY, ntile_y,    \
-                           kin0, kin1, dth, mu, eparts);                                     \
-    }
-            if (trc) MOCCA_HPSI_DMA(true) else MOCCA_HPSI_DMA(false)
-#undef MOCCA_HPSI_DMA
-            return;
-        }
-    }
-    }
     const size_t shm = sizeof(T) * 4 * (size_t)(TY + 8) * (m.nx + 8);
 #define MOCCA_HPSI_LAUNCH(HM, TR, ST)                                                         \
     hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, ST>), dim3(ntasks), dim3(kBlock), shm, st, m,   \
@@ -624,6 +592,51 @@ void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout
     }
 #undef MOCCA_HPSI_TRC
 #undef MOCCA_HPSI_LAUNCH
+}
+
+// The pipelined FP64 step kernel is usable when a row splits into whole
+// 16-byte lane loads, the halo rows of one component fit two 1 KiB loads, and
+// its LDS fits the device's per-workgroup limit (less its static array).
+static size_t hpsi_dma_lds_limit() {
+    static const size_t lim = [] {
+        int dev = 0, l = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess)
+            return (size_t)0;
+        const size_t stat = 3 * kDmaThreads / 64 * sizeof(double);
+        return (size_t)l > stat ? (size_t)l - stat : (size_t)0;
+    }();
+    return lim;
+}
+
+bool hpsi_dma_ok(const MeshK<double> &m, int TY, int trc) {
+    if ((m.nx * sizeof(double)) % 16 != 0 || 4 * m.nx * sizeof(double) > 2048) return false;
+    const size_t dshm =
+        trc ? hpsi_dma_lds<double, true>(m.nx, TY) : hpsi_dma_lds<double, false>(m.nx, TY);
+    return dshm <= hpsi_dma_lds_limit();
+}
+
+void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_out,
+                        const double *prev, const double *fields, const int *qn,
+                        const Task *tasks, int ntasks, int TY, int ntile_y, int nzs, int trc,
+                        double kin0, double kin1, double dth, double mu, double *eparts,
+                        hipStream_t st) {
+    const size_t dshm =
+        trc ? hpsi_dma_lds<double, true>(m.nx, TY) : hpsi_dma_lds<double, false>(m.nx, TY);
+#define MOCCA_HPSI_DMA(TR)                                                                  \
+    {                                                                                       \
+        static const bool lds_set = hipFuncSetAttribute(                                   \
+            (const void *)k_hpsi_fd_dma<double, TR>,                                        \
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)hpsi_dma_lds_limit()) ==       \
+            hipSuccess;                                                                     \
+        (void)lds_set;                                                                      \
+        hipLaunchKernelGGL((k_hpsi_fd_dma<double, TR>), dim3(ntasks), dim3(kDmaThreads),    \
+                           dshm, st, m, psi, psi_out, prev, fields, qn, tasks, TY, ntile_y, \
+                           nzs, kin0, kin1, dth, mu, eparts);                               \
+    }
+    if (trc) MOCCA_HPSI_DMA(true) else MOCCA_HPSI_DMA(false)
+#undef MOCCA_HPSI_DMA
 }
 
 template void launch_hpsi_fd<double>(const MeshK<double> &, const double *, double *,
