@@ -210,9 +210,10 @@ def main():
     # roofline of the dominant kernel (hpsi_step): algorithmic bytes per launch
     # = read psi + read the states before the last step (the heavy-ball
     # memory) + write psi' for all states, plus the potentials of both
-    # isospins once (25 fields per isospin, time-odd).
+    # isospins once (24 fields per isospin time-odd: U B NB(3) W(9) DW(3) S(3)
+    # A(3) DA; 17 with time reversal).
     s_bytes = 4 * N * 8
-    nfields = 18 if w["trc"] else 25
+    nfields = 17 if w["trc"] else 24
     alg_bytes = 3 * M * s_bytes + 2 * nfields * N * 8
     ms, cnt = prof["hpsi_step"]
     t_kernel = ms / cnt / 1e3
@@ -243,7 +244,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(
-                "mocca::k_hpsi_fd<double, 4, %s, false>" % ("true" if w["trc"] else "false")),
+                "mocca::k_hpsi_fd_dma<double, %s>" % ("true" if w["trc"] else "false")),
             "kernel": "hpsi_step", "kernel_ms": t_kernel * 1e3,
             "alg_bytes_per_launch": alg_bytes,
         },
