@@ -316,11 +316,18 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
         });
     } else {
         slots = h->ntiles;
+        T *dsc = (T *)h->b.lagscr;
+        if (dsc) {
+            rc = run(h, "lag_derivs", [&] {
+                launch_lag_derivs<T>(h->md, dsrc, h->b.qn, h->M, dsc, h->st);
+            });
+            if (rc) return rc;
+        }
         rc = run(h, "hpsi_step", [&] {
             launch_hpsi_step<T>(m, dsrc, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
                                 h->b.order, h->b.chunks, h->nchunks, h->mesh.time_reversal,
                                 T(h->kin[0]), T(h->kin[1]), T(dth), T(mu), kStep,
-                                h->b.eparts, h->ntiles, h->st, vsrc);
+                                h->b.eparts, h->ntiles, h->st, vsrc, dsc);
         });
     }
 stepped:
@@ -356,11 +363,19 @@ int kinetic_impl(mocca_solve *h) {
         });
     } else {
         slots = h->ntiles;
+        T *dsc = (T *)h->b.lagscr;
+        if (dsc) {
+            rc = run(h, "lag_derivs", [&] {
+                launch_lag_derivs<T>(h->md, psi, h->b.qn, h->M, dsc, h->st);
+            });
+            if (rc) return rc;
+        }
         rc = run(h, "kinetic", [&] {
             launch_hpsi_step<T>(m, psi, nullptr, nullptr, nullptr,
                                 (const T *)h->b.fields, h->b.qn, h->b.order, h->b.chunks,
                                 h->nchunks, h->mesh.time_reversal, T(h->kin[0]),
-                                T(h->kin[1]), T(0), T(0), 0, h->b.eparts, h->ntiles, h->st);
+                                T(h->kin[1]), T(0), T(0), 0, h->b.eparts, h->ntiles, h->st,
+                                nullptr, dsc);
         });
     }
     if (rc) return rc;
@@ -649,7 +664,8 @@ void free_all(mocca_solve *h) {
                     b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.K, b.eparts,
                     b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
                     b.G, b.Tm, b.work, b.gpart, b.status, b.jobs, b.one_chunk,
-                    b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks, b.tasks_dma};
+                    b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks, b.tasks_dma,
+                    b.lagscr};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     Est &e = h->est;
@@ -738,6 +754,19 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     h->ntile_y = (mesh->ny + h->TY - 1) / h->TY;
     TRY(dalloc(h, &b.one_tasks, h->ntile_y * sizeof(Task)));
     TRY(dalloc(h, &b.eparts, M * (size_t)h->ntiles * 3 * sizeof(double)));
+    // Lagrange meshes with lines of 16..64 points take their derivatives from
+    // the matrix cores (k_lag_axis) through a scratch of 24 values per state
+    // and point; if that does not fit, the on-the-fly path is used.
+    if (mesh->lagrange) {
+        const int nmin = std::min(mesh->nx, std::min(mesh->ny, mesh->nz));
+        const int nmax = std::max(mesh->nx, std::max(mesh->ny, mesh->nz));
+        if (nmin >= 16 && nmax <= kLagMaxN) {
+            if (hipMalloc(&b.lagscr, (size_t)M * 24 * N * ts) != hipSuccess) {
+                b.lagscr = nullptr;
+                (void)hipGetLastError();
+            }
+        }
+    }
     TRY(dalloc(h, &b.K, M * sizeof(double)));
     HIPCHK(h, hipMemset(b.K, 0, M * sizeof(double)));
     HIPCHK(h, hipMemset(b.upd, 0, M * 4 * N * ts));

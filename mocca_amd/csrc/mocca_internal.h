@@ -43,6 +43,7 @@ struct Buffers {
     int *one_order = nullptr;
     Task *tasks = nullptr, *one_tasks = nullptr;
     Task *tasks_dma = nullptr;     // pipelined FP64 step: (state, tile + ntile_y * slab)
+    void *lagscr = nullptr;        // T [M][4][6][N]: matrix-core Lagrange derivatives
     Chunk *dchunks = nullptr;
     DTask *dtasks = nullptr;
 };

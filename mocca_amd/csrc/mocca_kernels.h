@@ -131,7 +131,14 @@ void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
                       T *hout, const T *fields, const int *qn, const int *order,
                       const Chunk *chunks, int nchunks, int trc, T kin0, T kin1,
                       T dth, T mu, int mode, double *eparts, int ntiles,
-                      hipStream_t st, const T *vsrc = nullptr);
+                      hipStream_t st, const T *vsrc = nullptr, const T *dsc = nullptr);
+constexpr int kLagMaxN = 64;   // longest line of the matrix-core Lagrange derivatives
+// Lagrange first and second derivatives of every state along x, y, z on the
+// FP64 matrix cores (lines of up to kLagMaxN = 64 points):
+// dsc[state][comp][6][N] = d/dx, d/dy, d/dz, d2/dx2, d2/dy2, d2/dz2.
+template <typename T>
+void launch_lag_derivs(const MeshK<double> &md, const T *psi, const int *qn, int M, T *dsc,
+                       hipStream_t st);
 
 // E = dv sum psi.h, D = dv sum h.h - E^2, K = dv sum psi.Lap psi per state
 // (any of E, D, K may be null).

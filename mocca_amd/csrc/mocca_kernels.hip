@@ -158,6 +158,134 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
     __syncthreads();
 }
 
+// FP64 matrix-core tile: D[16x16] += A[16x4] B[4x16].
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+
+__device__ __forceinline__ mfma_d4 mfma16x16x4(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Lagrange derivatives on the FP64 matrix cores (Opt_LAG{X,Y,Z}_EV8,
+// Lapla_LAG_EV8: OptimizedDerivatives.f90:115-148,306-338,432-464,551-615).
+// Along each axis the derivative of every line is a small GEMM with the
+// folded n x n matrix of the component's reflection sign: D = L Lambda, with
+// Lambda[k][p] the value at position k of line ("pencil") p.  A workgroup
+// (4 waves) owns one (state, component, axis): it keeps L1 and L2 (first and
+// second derivative) transposed in LDS and walks blocks of 16 pencils, each
+// staged in the wave's own LDS slice.  v_mfma_f64_16x16x4f64: lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; the result lane l, register r holds
+// D[l/16 + 4r][l%16].  For y and z a pencil block is 16 consecutive x (so the
+// results are stored 16 consecutive values at a time) and D = L Lambda; for x
+// a block is 16 rows and D^T = Lambda^T L^T, which stores along x as well.
+// Output: dsc[state][comp][6][N] = d/dx, d/dy, d/dz, d2/dx2, d2/dy2, d2/dz2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lag_axis(MeshK<double> md, const T *__restrict__ psi,
+                                                  const int *__restrict__ qn, int M,
+                                                  T *__restrict__ dsc) {
+    const int s = blockIdx.x / 4, c = blockIdx.x % 4, axis = blockIdx.y;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int nx = md.nx, ny = md.ny, nz = md.nz;
+    const long N = md.N;
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const int npad = (n + 15) / 16 * 16, ntile = npad / 16, nks = npad / 4;
+    extern __shared__ double lag_lds[];
+    double *Lt = lag_lds;                            // [2][npad][npad]: L^T (k rows)
+    double *Lam = lag_lds + 2 * npad * npad + wave * npad * 16;   // [npad][16]
+    const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
+    const int s1 = sign_first(axis, P, S, TS, c, 1), s2 = sign_lap(axis, P, S, TS, c);
+    const double *L1 = md.lag1[axis][s1 > 0 ? 1 : 0], *L2 = md.lag2[axis][s2 > 0 ? 1 : 0];
+    for (int q = threadIdx.x; q < 2 * npad * npad; q += blockDim.x) {
+        const int mt = q / (npad * npad), r = q % (npad * npad);
+        const int k = r / npad, i = r % npad;
+        Lt[q] = (i < n && k < n) ? (mt ? L2 : L1)[i * n + k] : 0.0;
+    }
+    __syncthreads();
+    const T *ps = psi + ((long)s * 4 + c) * N;
+    T *d1 = dsc + (((long)s * 4 + c) * 6 + axis) * N;
+    T *d2 = dsc + (((long)s * 4 + c) * 6 + 3 + axis) * N;
+    const int nxb = (nx + 15) / 16;
+    const int nblk = axis == 0 ? (ny * nz + 15) / 16 : nxb * (axis == 1 ? nz : ny);
+    const int kr = lane / 16, col = lane % 16;
+    for (int blk = wave; blk < nblk; blk += 4) {
+        // stage the block: Lam[k][p], zero past the line or the pencils
+        int outer = 0, x0 = 0, npen = 16;
+        if (axis == 0) {
+            npen = min(16, ny * nz - 16 * blk);
+        } else {
+            outer = blk / nxb;
+            x0 = (blk % nxb) * 16;
+            npen = min(16, nx - x0);
+        }
+        for (int e = lane; e < npad * 16; e += 64) {
+            int k, p;
+            long a = 0;
+            if (axis == 0) { p = e / npad; k = e % npad; a = (long)(16 * blk + p) * nx + k; }
+            else {
+                k = e / 16; p = e % 16;
+                a = axis == 1 ? ((long)outer * ny + k) * nx + x0 + p
+                              : ((long)k * ny + outer) * nx + x0 + p;
+            }
+            const bool ok = k < n && p < npen;
+            Lam[k * 16 + p] = ok ? (double)ps[ok ? a : 0] : 0.0;
+        }
+        // (the wave reads only its own slice: its LDS writes complete in order)
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        mfma_d4 acc[2][kLagMaxN / 16];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int t = 0; t < kLagMaxN / 16; ++t) acc[mt][t] = mfma_d4{0, 0, 0, 0};
+        for (int ks = 0; ks < nks; ++ks) {
+            const int k = 4 * ks + kr;
+            const double lam = Lam[k * 16 + col];
+#pragma unroll
+            for (int t = 0; t < kLagMaxN / 16; ++t) {
+                if (t >= ntile) break;
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    const double ell = Lt[(mt * npad + k) * npad + 16 * t + col];
+                    acc[mt][t] = axis == 0 ? mfma16x16x4(lam, ell, acc[mt][t])
+                                           : mfma16x16x4(ell, lam, acc[mt][t]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kLagMaxN / 16; ++t) {
+            if (t >= ntile) break;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i, p;
+                if (axis == 0) { p = kr + 4 * r; i = 16 * t + col; }
+                else { i = 16 * t + kr + 4 * r; p = col; }
+                if (i < n && p < npen) {
+                    long a;
+                    if (axis == 0) a = (long)(16 * blk + p) * nx + i;
+                    else if (axis == 1) a = ((long)outer * ny + i) * nx + x0 + p;
+                    else a = ((long)i * ny + outer) * nx + x0 + p;
+                    d1[a] = (T)acc[0][t][r];
+                    d2[a] = (T)acc[1][t][r];
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_lag_derivs(const MeshK<double> &md, const T *psi, const int *qn, int M, T *dsc,
+                       hipStream_t st) {
+    const int nmax = std::max(md.nx, std::max(md.ny, md.nz));
+    const int npad = (nmax + 15) / 16 * 16;
+    const size_t shm = sizeof(double) * (2 * (size_t)npad * npad + 4 * (size_t)npad * 16);
+    static const bool lds_set =
+        hipFuncSetAttribute((const void *)k_lag_axis<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(double) * (2 * kLagMaxN * kLagMaxN + 4 * kLagMaxN * 16))) ==
+        hipSuccess;
+    (void)lds_set;
+    hipLaunchKernelGGL(k_lag_axis<T>, dim3(4 * M, 3), dim3(256), shm, st, md, psi, qn, M, dsc);
+}
+
 // hPsi (ImaginaryTime.f90:213-304) fused with the heavy-ball step of GradDesc
 // (ImaginaryTime.f90:467-491).  One thread per mesh point, looping over the
 // states of one chunk (one isospin), so the isospin's potentials are read
@@ -171,7 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
     T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
     const int *__restrict__ qn, const int *__restrict__ order,
     const Chunk *__restrict__ chunks, int trc, T kin0, T kin1, T dth, T mu,
-    int mode, double *__restrict__ eparts, int ntiles, const T *__restrict__ vsrc) {
+    int mode, double *__restrict__ eparts, int ntiles, const T *__restrict__ vsrc,
+    const T *__restrict__ dsc) {
     __shared__ double sh[3 * kBlock / 64];
     const long N = m.N;
     const long p = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -206,8 +335,20 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_step(
         T v[4] = {0, 0, 0, 0}, d[3][4] = {{0}}, l[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0};
         T vd[4] = {0, 0, 0, 0};   // values of the derivative source
         if (valid) {
-            state_derivs(m, ps, qn[4 * s + 1], qn[4 * s + 2], qn[4 * s + 3], x, y, z,
-                         p, v, d, l, true);
+            if (dsc) {
+                // derivatives from the matrix-core Lagrange pass (k_lag_axis)
+                const T *dd = dsc + (long)s * 24 * N + p;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    v[c] = ps[(long)c * N + p];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) d[a][c] = dd[(c * 6 + a) * N];
+                    l[c] = dd[(c * 6 + 3) * N] + dd[(c * 6 + 4) * N] + dd[(c * 6 + 5) * N];
+                }
+            } else {
+                state_derivs(m, ps, qn[4 * s + 1], qn[4 * s + 2], qn[4 * s + 3], x, y, z,
+                             p, v, d, l, true);
+            }
 #pragma unroll
             for (int c = 0; c < 4; ++c) vd[c] = v[c];
             if constexpr (STALE) {
@@ -296,16 +437,16 @@ void launch_hpsi_step(const MeshK<T> &m, const T *psi, T *psi_out, T *upd,
                       T *hout, const T *fields, const int *qn, const int *order,
                       const Chunk *chunks, int nchunks, int trc, T kin0, T kin1,
                       T dth, T mu, int mode, double *eparts, int ntiles,
-                      hipStream_t st, const T *vsrc) {
+                      hipStream_t st, const T *vsrc, const T *dsc) {
     dim3 g(ntiles, nchunks);
     if (vsrc)
         hipLaunchKernelGGL((k_hpsi_step<T, true>), g, dim3(kBlock), 0, st, m, psi, psi_out,
                            upd, hout, fields, qn, order, chunks, trc, kin0, kin1, dth, mu,
-                           mode, eparts, ntiles, vsrc);
+                           mode, eparts, ntiles, vsrc, dsc);
     else
         hipLaunchKernelGGL((k_hpsi_step<T, false>), g, dim3(kBlock), 0, st, m, psi, psi_out,
                            upd, hout, fields, qn, order, chunks, trc, kin0, kin1, dth, mu,
-                           mode, eparts, ntiles, vsrc);
+                           mode, eparts, ntiles, vsrc, dsc);
 }
 
 // Energies and dispersions (ImaginaryTime.f90:470-471): E = dv sum psi.h,
@@ -926,12 +1067,6 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
 // (v_mfma_f64_16x16x4f64: D[16x16] += A[16x4] B[4x16]; operand lane l holds
 // A[l%16][l/16] and B[l/16][l%16]; result lane l holds D[l/16 + 4r][l%16] in
 // register r -- measured on gfx950).
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
-
-
-__device__ __forceinline__ mfma_d4 mfma16x16x4(double a, double b, mfma_d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 // Gram partials: one wave per (K slice, block) holds all upper 16x16 tiles
 // of the block (NT row tiles, <= 8).  Per 16 mesh values each lane loads 4
@@ -1211,7 +1346,9 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
     template void launch_hpsi_step<T>(const MeshK<T> &, const T *, T *, T *, T *,        \
                                       const T *, const int *, const int *,               \
                                       const Chunk *, int, int, T, T, T, T, int,          \
-                                      double *, int, hipStream_t, const T *);            \
+                                      double *, int, hipStream_t, const T *, const T *); \
+    template void launch_lag_derivs<T>(const MeshK<double> &, const T *, const int *,    \
+                                       int, T *, hipStream_t);                           \
     template void launch_density<T>(const MeshK<T> &, const T *, const int *,            \
                                     const double *, const int *, const Chunk *, int,     \
                                     int, T *, hipStream_t);                              \
