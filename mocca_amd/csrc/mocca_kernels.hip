@@ -180,110 +180,154 @@ __device__ __forceinline__ mfma_d4 mfma16x16x4(double a, double b, mfma_d4 c) {
 // results are stored 16 consecutive values at a time) and D = L Lambda; for x
 // a block is 16 rows and D^T = Lambda^T L^T, which stores along x as well.
 // Output: dsc[state][comp][6][N] = d/dx, d/dy, d/dz, d2/dx2, d2/dy2, d2/dz2.
-template <typename T>
+template <typename T, int NPAD, int AXIS>
 __global__ __launch_bounds__(256) void k_lag_axis(MeshK<double> md, const T *__restrict__ psi,
                                                   const int *__restrict__ qn, int M,
                                                   T *__restrict__ dsc) {
-    const int s = blockIdx.x / 4, c = blockIdx.x % 4, axis = blockIdx.y;
+    constexpr int NT = NPAD / 16, NKS = NPAD / 4, NEL = NPAD / 4;
+    const int s = blockIdx.x / 4, c = blockIdx.x % 4;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int nx = md.nx, ny = md.ny, nz = md.nz;
     const long N = md.N;
-    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
-    const int npad = (n + 15) / 16 * 16, ntile = npad / 16, nks = npad / 4;
+    const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
     extern __shared__ double lag_lds[];
-    double *Lt = lag_lds;                            // [2][npad][npad]: L^T (k rows)
-    double *Lam = lag_lds + 2 * npad * npad + wave * npad * 16;   // [npad][16]
+    double *Lt = lag_lds;                                 // [2][NPAD][NPAD]: L^T (k rows)
+    double *Lam = lag_lds + 2 * NPAD * NPAD + wave * NPAD * 16;   // [NPAD][16]
     const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
-    const int s1 = sign_first(axis, P, S, TS, c, 1), s2 = sign_lap(axis, P, S, TS, c);
-    const double *L1 = md.lag1[axis][s1 > 0 ? 1 : 0], *L2 = md.lag2[axis][s2 > 0 ? 1 : 0];
-    for (int q = threadIdx.x; q < 2 * npad * npad; q += blockDim.x) {
-        const int mt = q / (npad * npad), r = q % (npad * npad);
-        const int k = r / npad, i = r % npad;
+    const int s1 = sign_first(AXIS, P, S, TS, c, 1), s2 = sign_lap(AXIS, P, S, TS, c);
+    const double *L1 = md.lag1[AXIS][s1 > 0 ? 1 : 0], *L2 = md.lag2[AXIS][s2 > 0 ? 1 : 0];
+    for (int q = threadIdx.x; q < 2 * NPAD * NPAD; q += blockDim.x) {
+        const int mt = q / (NPAD * NPAD), r = q % (NPAD * NPAD);
+        const int k = r / NPAD, i = r % NPAD;
         Lt[q] = (i < n && k < n) ? (mt ? L2 : L1)[i * n + k] : 0.0;
     }
     __syncthreads();
     const T *ps = psi + ((long)s * 4 + c) * N;
-    T *d1 = dsc + (((long)s * 4 + c) * 6 + axis) * N;
-    T *d2 = dsc + (((long)s * 4 + c) * 6 + 3 + axis) * N;
+    T *d1 = dsc + (((long)s * 4 + c) * 6 + AXIS) * N;
+    T *d2 = dsc + (((long)s * 4 + c) * 6 + 3 + AXIS) * N;
     const int nxb = (nx + 15) / 16;
-    const int nblk = axis == 0 ? (ny * nz + 15) / 16 : nxb * (axis == 1 ? nz : ny);
+    const int nblk = AXIS == 0 ? (ny * nz + 15) / 16 : nxb * (AXIS == 1 ? nz : ny);
     const int kr = lane / 16, col = lane % 16;
-    for (int blk = wave; blk < nblk; blk += 4) {
-        // stage the block: Lam[k][p], zero past the line or the pencils
-        int outer = 0, x0 = 0, npen = 16;
-        if (axis == 0) {
-            npen = min(16, ny * nz - 16 * blk);
+    // geometry of pencil block b: x pencils are rows 16 b .. 16 b + 15; y and
+    // z pencils are 16 consecutive x at (z, or y) = outer
+    auto geom = [&](int b, int &outer, int &x0, int &npen) {
+        outer = 0; x0 = 0;
+        if (AXIS == 0) {
+            npen = min(16, ny * nz - 16 * b);
         } else {
-            outer = blk / nxb;
-            x0 = (blk % nxb) * 16;
+            outer = b / nxb;
+            x0 = (b % nxb) * 16;
             npen = min(16, nx - x0);
         }
-        for (int e = lane; e < npad * 16; e += 64) {
+    };
+    // element j of the lane in a block: Lam index e = lane + 64 j, position
+    // k and pencil p; the next block is loaded into registers while the
+    // MFMAs of the current one run
+    auto gather = [&](int b, double (&pf)[NEL]) {
+        int outer, x0, npen;
+        geom(b, outer, x0, npen);
+#pragma unroll
+        for (int j = 0; j < NEL; ++j) {
+            const int e = lane + 64 * j;
             int k, p;
-            long a = 0;
-            if (axis == 0) { p = e / npad; k = e % npad; a = (long)(16 * blk + p) * nx + k; }
+            long a;
+            if (AXIS == 0) { p = e / NPAD; k = e % NPAD; a = (long)(16 * b + p) * nx + k; }
             else {
                 k = e / 16; p = e % 16;
-                a = axis == 1 ? ((long)outer * ny + k) * nx + x0 + p
+                a = AXIS == 1 ? ((long)outer * ny + k) * nx + x0 + p
                               : ((long)k * ny + outer) * nx + x0 + p;
             }
             const bool ok = k < n && p < npen;
-            Lam[k * 16 + p] = ok ? (double)ps[ok ? a : 0] : 0.0;
+            const T v = ps[ok ? a : 0];
+            pf[j] = ok ? (double)v : 0.0;
         }
+    };
+    auto put = [&](const double (&pf)[NEL]) {
+#pragma unroll
+        for (int j = 0; j < NEL; ++j) {
+            const int e = lane + 64 * j;
+            const int k = AXIS == 0 ? e % NPAD : e / 16, p = AXIS == 0 ? e / NPAD : e % 16;
+            Lam[k * 16 + p] = pf[j];
+        }
+    };
+    double pf[NEL];
+    if (wave < nblk) {
+        gather(wave, pf);
+        put(pf);
+    }
+    for (int blk = wave; blk < nblk; blk += 4) {
+        const int nb = blk + 4;
+        if (nb < nblk) gather(nb, pf);
+        int outer, x0, npen;
+        geom(blk, outer, x0, npen);
         // (the wave reads only its own slice: its LDS writes complete in order)
         __builtin_amdgcn_s_waitcnt(0xc07f);
-        mfma_d4 acc[2][kLagMaxN / 16];
+        mfma_d4 acc[2][NT];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-            for (int t = 0; t < kLagMaxN / 16; ++t) acc[mt][t] = mfma_d4{0, 0, 0, 0};
-        for (int ks = 0; ks < nks; ++ks) {
+            for (int t = 0; t < NT; ++t) acc[mt][t] = mfma_d4{0, 0, 0, 0};
+#pragma unroll 4
+        for (int ks = 0; ks < NKS; ++ks) {
             const int k = 4 * ks + kr;
             const double lam = Lam[k * 16 + col];
 #pragma unroll
-            for (int t = 0; t < kLagMaxN / 16; ++t) {
-                if (t >= ntile) break;
+            for (int t = 0; t < NT; ++t)
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
-                    const double ell = Lt[(mt * npad + k) * npad + 16 * t + col];
-                    acc[mt][t] = axis == 0 ? mfma16x16x4(lam, ell, acc[mt][t])
+                    const double ell = Lt[(mt * NPAD + k) * NPAD + 16 * t + col];
+                    acc[mt][t] = AXIS == 0 ? mfma16x16x4(lam, ell, acc[mt][t])
                                            : mfma16x16x4(ell, lam, acc[mt][t]);
                 }
-            }
         }
 #pragma unroll
-        for (int t = 0; t < kLagMaxN / 16; ++t) {
-            if (t >= ntile) break;
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int i, p;
-                if (axis == 0) { p = kr + 4 * r; i = 16 * t + col; }
+                if (AXIS == 0) { p = kr + 4 * r; i = 16 * t + col; }
                 else { i = 16 * t + kr + 4 * r; p = col; }
                 if (i < n && p < npen) {
                     long a;
-                    if (axis == 0) a = (long)(16 * blk + p) * nx + i;
-                    else if (axis == 1) a = ((long)outer * ny + i) * nx + x0 + p;
+                    if (AXIS == 0) a = (long)(16 * blk + p) * nx + i;
+                    else if (AXIS == 1) a = ((long)outer * ny + i) * nx + x0 + p;
                     else a = ((long)i * ny + outer) * nx + x0 + p;
                     d1[a] = (T)acc[0][t][r];
                     d2[a] = (T)acc[1][t][r];
                 }
             }
-        }
+        if (nb < nblk) put(pf);
     }
 }
 
 template <typename T>
 void launch_lag_derivs(const MeshK<double> &md, const T *psi, const int *qn, int M, T *dsc,
                        hipStream_t st) {
-    const int nmax = std::max(md.nx, std::max(md.ny, md.nz));
-    const int npad = (nmax + 15) / 16 * 16;
-    const size_t shm = sizeof(double) * (2 * (size_t)npad * npad + 4 * (size_t)npad * 16);
-    static const bool lds_set =
-        hipFuncSetAttribute((const void *)k_lag_axis<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(sizeof(double) * (2 * kLagMaxN * kLagMaxN + 4 * kLagMaxN * 16))) ==
-        hipSuccess;
-    (void)lds_set;
-    hipLaunchKernelGGL(k_lag_axis<T>, dim3(4 * M, 3), dim3(256), shm, st, md, psi, qn, M, dsc);
+    const int ns[3] = {md.nx, md.ny, md.nz};
+#define MOCCA_LAG_AXIS(NP, AX)                                                               \
+    {                                                                                        \
+        constexpr size_t shm = sizeof(double) * (2 * NP * NP + 4 * NP * 16);               \
+        static const bool lds_set =                                                          \
+            hipFuncSetAttribute((const void *)k_lag_axis<T, NP, AX>,                         \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) ==     \
+            hipSuccess;                                                                      \
+        (void)lds_set;                                                                       \
+        hipLaunchKernelGGL((k_lag_axis<T, NP, AX>), dim3(4 * M), dim3(256), shm, st, md, psi, \
+                           qn, M, dsc);                                                      \
+    }
+#define MOCCA_LAG_NP(AX)                                                                     \
+    switch ((ns[AX] + 15) / 16) {                                                            \
+        case 1: MOCCA_LAG_AXIS(16, AX) break;                                                \
+        case 2: MOCCA_LAG_AXIS(32, AX) break;                                                \
+        case 3: MOCCA_LAG_AXIS(48, AX) break;                                                \
+        case 4: MOCCA_LAG_AXIS(64, AX) break;                                                \
+        default: break;                                                                      \
+    }
+    MOCCA_LAG_NP(0)
+    MOCCA_LAG_NP(1)
+    MOCCA_LAG_NP(2)
+#undef MOCCA_LAG_NP
+#undef MOCCA_LAG_AXIS
 }
 
 // hPsi (ImaginaryTime.f90:213-304) fused with the heavy-ball step of GradDesc
