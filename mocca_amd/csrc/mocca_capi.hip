@@ -264,7 +264,7 @@ int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_m
                            h->chol_lds, h->st);
     });
     if (rc) return rc;
-    if (h->mfma_gram && !update_mode)
+    if (h->mfma_apply && !update_mode)
         return run(h, "gs_apply", [&] {
             launch_gs_apply_mfma<T>(src, dst, K4, h->b.perm, h->b.blk_m, h->nblk, h->max_m,
                                     h->mpad, h->b.Tm, h->st);
@@ -977,6 +977,8 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     h->mpad = (mmax + 15) / 16 * 16;
     // (below 64 states the vector kernel streams the block faster)
     h->mfma_gram = h->mpad >= 32 && h->mpad <= kMfmaMaxRows;
+    // (the apply keeps one output row of tiles per wave: up to 192 states)
+    h->mfma_apply = h->mpad >= 32 && h->mpad <= kMfmaApplyMaxRows;
     const int nt16 = h->mpad / 16, ntile16 = nt16 * (nt16 + 1) / 2;
     {
         // one round of single-wave workgroups (1024 = 256 CUs x 4 SIMDs at

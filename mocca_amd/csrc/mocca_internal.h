@@ -172,6 +172,7 @@ struct mocca_solve {
     int ndchunks = 0, ndtasks = 0, gtile = 64, mmax = 0;
     size_t chol_lds = 0;   // dynamic LDS bytes available to the in-LDS Cholesky
     bool mfma_gram = false;  // Gram on the matrix cores (blocks <= kMfmaMaxRows)
+    bool mfma_apply = false; // apply on the matrix cores (blocks <= kMfmaApplyMaxRows)
     int mpad = 0, nsplit_mfma = 1;
     // profiling
     bool prof = false;

@@ -191,6 +191,7 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
 // Gram matrices on the FP64 matrix cores (v_mfma_f64_16x16x4f64) for blocks
 // of 64..kMfmaMaxRows states (one wave holds all upper 16x16 tiles).
 constexpr int kMfmaMaxRows = 128;
+constexpr int kMfmaApplyMaxRows = 192;
 
 template <typename T>
 void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *blk_m, int nblk,

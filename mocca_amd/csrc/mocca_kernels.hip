@@ -1203,7 +1203,7 @@ constexpr size_t gs_apply_mfma_lds() {
     return sizeof(double) * 2 * 16 * (kApplyPS + NT * 16 + 8);
 }
 template <typename T, int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gs_apply_mfma(const T *__restrict__ src,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 8 ? 2 : 1))) void k_gs_apply_mfma(const T *__restrict__ src,
                                                        T *__restrict__ dst, long K4,
                                                        const int *__restrict__ perm,
                                                        const int *__restrict__ blk_m, int max_m,
@@ -1321,7 +1321,8 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
     switch (mpad_max / 16) {
         MOCCA_APPLY_NT(2) MOCCA_APPLY_NT(3)
         MOCCA_APPLY_NT(4) MOCCA_APPLY_NT(5) MOCCA_APPLY_NT(6) MOCCA_APPLY_NT(7)
-        MOCCA_APPLY_NT(8)
+        MOCCA_APPLY_NT(8) MOCCA_APPLY_NT(9) MOCCA_APPLY_NT(10) MOCCA_APPLY_NT(11)
+        MOCCA_APPLY_NT(12)
         default: return;
     }
 #undef MOCCA_APPLY_NT

@@ -125,3 +125,33 @@ def test_microbench_workload_vs_oracle():
     assert relerr(e, ref_e) < 1e-11
     assert relerr(psi, ref_psi.reshape(M, 4, N)) < 1e-10
     s.close()
+
+
+@pytest.mark.parametrize("m", [25, 100, 150, 190, 250])
+def test_gram_schmidt_block_sizes_vs_householder_qr(m):
+    """Gram-Schmidt of one block of m states (energy order) through every
+    kernel family: MFMA Gram up to 128 states, MFMA apply up to 192, vector
+    kernels and the global-memory Cholesky beyond.  Modified Gram-Schmidt in
+    a given order equals the QR factorisation with a positive-diagonal R, so
+    the result is checked against numpy's Householder QR of the same
+    energy-sorted states (an independent algorithm), 1e-10."""
+    n, dx = 8, 0.8
+    N, dv = n ** 3, dx ** 3 * 8
+    rng = np.random.default_rng(100 + m)
+    psi = rng.standard_normal((m, 4, N)) * 0.1
+    qn = np.tile(np.array([-1, 1, 1, 1], np.int32), (m, 1))
+    e = rng.permutation(np.linspace(-40.0, 10.0, m))
+    s = mocca_amd.Solve(n, n, n, dx, dv, m, time_reversal=False)
+    s.set_states_meta(qn, np.ones(m))
+    s.upload_states(psi)
+    s.set_energies(e)
+    s.gram_schmidt()
+    q = s.download_states().reshape(m, -1)
+    s.close()
+    order = np.argsort(e, kind="stable")
+    A = psi.reshape(m, -1)[order] * np.sqrt(dv)
+    Qh, R = np.linalg.qr(A.T)
+    Qh = Qh * np.sign(np.diag(R))
+    ref = Qh.T / np.sqrt(dv)
+    assert relerr(q[order], ref) < 1e-10
+    assert np.allclose(q @ q.T * dv, np.eye(m), atol=1e-11)
