@@ -14,10 +14,12 @@ The self-consistent 40^3 / 400-state workload (SLy4 + Coulomb, HF filling,
 density mixing; one step = the whole Evolve iteration on the device) is
 timed after it in FP64 and mixed precision from the same start, with the
 total-energy difference after the same number of iterations
-(``full_iteration`` in the JSON line).
+(``full_iteration`` in the JSON line).  The 48^3 / 600-state Lagrange-mesh
+workload (derivatives on the matrix cores) follows (``lagrange_48``).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
                     [--full-steps K3] [--skip-full]
+                    [--lagrange-steps K5] [--skip-lagrange]
 """
 import argparse
 import ctypes
@@ -98,8 +100,9 @@ def full_iteration_run(w, precision, steps, warmup, profile):
     t1 = time.perf_counter()
     prof = {}
     if profile:
-        for k in ("hpsi_step", "reduce_ed", "gram", "gram_reduce", "cholesky", "gs_apply",
-                  "hf_fill", "density", "density_reduce", "field_jobs", "mix", "moments",
+        for k in ("lag_derivs", "hpsi_step", "reduce_ed", "gram", "gram_reduce", "cholesky",
+                  "gs_apply", "hf_fill", "density", "density_reduce", "field_jobs", "mix",
+                  "moments",
                   "coulomb_bc", "coulomb_cg", "coulomb_direct", "potentials", "pot_jobs",
                   "pot_convert",
                   "energy", "energy_kinetic", "kinetic"):
@@ -183,6 +186,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--full-steps", type=int, default=200)
     ap.add_argument("--skip-full", action="store_true")
+    ap.add_argument("--lagrange-steps", type=int, default=20)
+    ap.add_argument("--skip-lagrange", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the "
                          "reference kernel for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -280,6 +285,22 @@ def main():
             "total_energy_err_keV": float(abs(r32[-1, 0] - r64[-1, 0]) * 1e3),
             "max_total_energy_err_keV_along_trajectory":
                 float(np.max(np.abs(r32[:, 0] - r64[:, 0])) * 1e3),
+        }
+    if not args.skip_lagrange:
+        wl = workloads.lagrange_48()
+        kl = args.lagrange_steps
+        tl64, pl64, rl64, _ = full_iteration_run(wl, FP64, kl, args.warmup, True)
+        tl32, pl32, rl32, _ = full_iteration_run(wl, MIXED, kl, args.warmup, True)
+        out["lagrange_48"] = {
+            "workload": "48x48x48 Lagrange octant mesh (48-point derivative matrices), "
+                        "dx=0.7 fm, 600 states (300 n + 300 p), HO seeds, SLy4 + Coulomb, "
+                        "HF filling N=Z=12; step = the whole Evolve iteration on the device",
+            "steps": kl, "warmup": args.warmup,
+            "fp64": {"value": 1.0 / tl64, "unit": "iters/s", "ms_per_step": tl64 * 1e3,
+                     "kernel_ms_per_step": pl64},
+            "mixed": {"value": 1.0 / tl32, "unit": "iters/s", "ms_per_step": tl32 * 1e3,
+                      "speedup_vs_fp64": tl64 / tl32, "kernel_ms_per_step": pl32},
+            "total_energy_err_keV": float(abs(rl32[-1, 0] - rl64[-1, 0]) * 1e3),
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)

@@ -157,3 +157,13 @@ def full_iteration(seed=3, n=40, dx=0.8, M=400, neutrons=126, protons=82, b=2.4)
                 qn=qn, occ=np.zeros(M), psi=psi,
                 functional=functional.scf_arguments("SLy4", neutrons, protons, True),
                 dt_over_hbar=0.010 / functional.HBAR, momentum=0.3)
+
+
+def lagrange_48(seed=5, n=48, dx=0.7, M=600, neutrons=12, protons=12, b=1.7):
+    """The 48^3 / 600-state Lagrange-mesh workload (Lagrange derivatives of
+    48 points per line): 300 neutron and 300 proton states (time reversal,
+    parity, signature and time simplex conserved), harmonic-oscillator seeds,
+    SLy4 with Coulomb, HF filling of 12 + 12 nucleons."""
+    w = full_iteration(seed=seed, n=n, dx=dx, M=M, neutrons=neutrons, protons=protons, b=b)
+    w["lagrange"] = True
+    return w
