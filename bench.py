@@ -15,11 +15,13 @@ density mixing; one step = the whole Evolve iteration on the device) is
 timed after it in FP64 and mixed precision from the same start, with the
 total-energy difference after the same number of iterations
 (``full_iteration`` in the JSON line).  The 48^3 / 600-state Lagrange-mesh
-workload (derivatives on the matrix cores) follows (``lagrange_48``).
+workload (derivatives on the matrix cores) follows (``lagrange_48``), then
+64 independent constrained solves through mocca_batch_run (``batch_scan``).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
                     [--full-steps K3] [--skip-full]
                     [--lagrange-steps K5] [--skip-lagrange]
+                    [--batch-steps K4] [--skip-batch]
 """
 import argparse
 import ctypes
@@ -115,6 +117,61 @@ def full_iteration_run(w, precision, steps, warmup, profile):
     return (t1 - t0) / steps, prof, rows, cg_iters
 
 
+def batch_scan_run(steps, warmup):
+    """64 independent constrained solves (uneven meshes 16^3..32^3 and
+    20..160 states, alternating Q20 constraints) through mocca_batch_run, in
+    FP64 and mixed precision from the same start; the same solves iterated
+    one after another give the sequential rate."""
+    import mocca_amd
+    from mocca_amd import FP64, MIXED, workloads
+    ws = workloads.batch_scan()
+
+    def start(w, precision):
+        s = mocca_amd.Solve(w["nx"], w["ny"], w["nz"], w["dx"], w["dv"], w["M"],
+                            time_reversal=w["trc"], lagrange=w["lagrange"],
+                            precision=precision)
+        s.set_states_meta(w["qn"], w["occ"])
+        s.upload_states(w["psi"])
+        s.scf_setup(**w["functional"])
+        s.set_constraints(w["constraints"])
+        s.gram_schmidt()
+        s.scf_start()
+        return s
+
+    dts = [w["dt_over_hbar"] for w in ws]
+    mus = [w["momentum"] for w in ws]
+    res, last = {}, {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        solves = [start(w, prec) for w in ws]
+        mocca_amd.batch_run(solves, warmup, dts, mus, fetch=False)
+        for s in solves:
+            s.synchronize()
+        t0 = time.perf_counter()
+        rows = mocca_amd.batch_run(solves, steps, dts, mus)
+        t1 = time.perf_counter()
+        last[name] = np.array([r[-1, 0] for r in rows])
+        t2 = time.perf_counter()
+        for s, w in zip(solves, ws):
+            s.iterate(steps, w["dt_over_hbar"], w["momentum"])
+        t3 = time.perf_counter()
+        for s in solves:
+            s.close()
+        res[name] = {"value": len(ws) * steps / (t1 - t0), "unit": "solve-iters/s",
+                     "ms_per_batch_step": (t1 - t0) / steps * 1e3,
+                     "sequential_value": len(ws) * steps / (t3 - t2),
+                     "speedup_vs_sequential": (t3 - t2) / (t1 - t0)}
+    return {
+        "workload": "64 solves of 12 n + 12 p, cubic meshes n ~ U{16..32} (dx=0.8 fm), "
+                    "nwt ~ U{20..160}, HO seeds, SLy4 + Coulomb, HF, alternating Q20 "
+                    "constraint over -300..600 fm^2; step = one Evolve iteration of "
+                    "every solve, issued in turn on the solves' own streams",
+        "solves": len(ws), "steps": steps, "warmup": warmup,
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "mixed_speedup_vs_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+        "max_abs_total_energy_err_keV": float(np.max(np.abs(last["mixed"] - last["fp64"])) * 1e3),
+    }
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of a kernel from the counter table of the
     committed rocprofv3 summary (2 x FETCH_SIZE + WRITE_SIZE, in KiB per
@@ -188,6 +245,8 @@ def main():
     ap.add_argument("--skip-full", action="store_true")
     ap.add_argument("--lagrange-steps", type=int, default=20)
     ap.add_argument("--skip-lagrange", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the "
                          "reference kernel for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -302,6 +361,8 @@ def main():
                       "speedup_vs_fp64": tl64 / tl32, "kernel_ms_per_step": pl32},
             "total_energy_err_keV": float(abs(rl32[-1, 0] - rl64[-1, 0]) * 1e3),
         }
+    if not args.skip_batch:
+        out["batch_scan"] = batch_scan_run(args.batch_steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

@@ -237,6 +237,19 @@ int mocca_estimation_seed(mocca_solve *h, const double *grid);
 int mocca_iterate_estimated(mocca_solve *h, int first_iteration, int niter, double dt,
                             double momentum, double *energies, double *params);
 
+/* niter iterations of the Evolve loop (as mocca_iterate) for each of nsolves
+ * independent solves, each after its own mocca_scf_start: iteration k of
+ * every solve is issued in turn, each on its solve's stream, so the device
+ * runs the solves concurrently (the batched constraint scans that the
+ * reference runs as separate jobs, one process each).  dt_over_hbar[i] and
+ * momentum[i] per solve; energies (may be NULL, and any energies[i] may be
+ * NULL) receives [niter+1][47] rows per solve.  The first failing solve stops
+ * the batch; its mocca_last_error has the message, and mocca_last_error(NULL)
+ * names it. */
+int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
+                    const double *dt_over_hbar, const double *momentum,
+                    double *const *energies);
+
 /* One multipole constraint (namelist MomentConstraint, Moments.f90:3576-3800):
  * the moment (l, m) of the density (real part, even l and m under the EV8
  * symmetries), ConstraintType 1 (augmented Lagrangian) or 2 (alternating

@@ -34,7 +34,7 @@ EXPORTS = (
     "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
     "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
     "mocca_alternate_step", "mocca_get_constraints", "mocca_set_pairing",
-    "mocca_solve_pairing", "mocca_get_pairing",
+    "mocca_solve_pairing", "mocca_get_pairing", "mocca_batch_run",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -395,3 +395,22 @@ class Solve:
         self._chk(self.lib.mocca_profile_get(self.h, name.encode(), ctypes.byref(ms),
                                              ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+
+def batch_run(solves, niter, dt_over_hbar, momentum, fetch=True):
+    """mocca_batch_run: niter Evolve iterations of every solve (each after
+    scf_start), issued in turn on the solves' own streams.  dt_over_hbar and
+    momentum: one value or one per solve.  Returns the energy rows
+    [niter+1][ROW_LEN] per solve (or None)."""
+    lib = load()
+    n = len(solves)
+    dts = np.broadcast_to(np.asarray(dt_over_hbar, dtype=np.float64), (n,)).copy()
+    mus = np.broadcast_to(np.asarray(momentum, dtype=np.float64), (n,)).copy()
+    hs = (ctypes.c_void_p * n)(*[s.h for s in solves])
+    rows = [np.zeros((niter + 1, ROW_LEN)) for _ in range(n)] if fetch else None
+    ptrs = (_dp * n)(*[_p(r) for r in rows]) if fetch else None
+    rc = lib.mocca_batch_run(hs, n, int(niter), _p(dts), _p(mus), ptrs)
+    if rc != 0:
+        msg = lib.mocca_last_error(None)
+        raise MoccaError(f"mocca error {rc}: {msg.decode() if msg else ''}")
+    return rows

@@ -340,8 +340,12 @@ int check_gs_status(mocca_solve *h) {
 // IterativeEstimation runs before each step (ParameterEstimation = 1,
 // ImaginaryTime.f90:464) and dt, momentum carry over from one iteration to
 // the next; params (when not null) receives the pair used at each.
-int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, double dth,
-                 double momentum, bool estimate, double *energies, double *params) {
+// The Evolve loop in three parts, so that the iterations of several solves
+// can be issued in turn on their own streams (mocca_batch_run): rows for
+// niter iterations, one iteration, and the final kinetic energy with the
+// status checks.  Only IterativeEstimation reads results back inside an
+// iteration.
+int iter_begin(mocca_solve *h, int niter) {
     if (int rc = check_scf(h)) return rc;
     if (!h->scf.started) return fail(h, MOCCA_ERR_STATE, "mocca_scf_start not called");
     if (niter < 0) return fail(h, MOCCA_ERR_ARG, "niter must be >= 0");
@@ -359,56 +363,66 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
     }
     HIPCHK(h, hipMemcpyAsync(s.erows, s.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
                              h->st));
+    return 0;
+}
+
+// Iteration k (1-based within the call) with Evolve's iteration number iter.
+int iter_step(mocca_solve *h, int k, int iter, double &dt, double &dth, double &momentum,
+              bool estimate, double *params) {
+    Scf &s = h->scf;
     int rc;
     // HF: a row's kinetic energy comes from the next GradDesc's hPsi pass
     const bool deferred = !h->hfb.on;
-    for (int k = 1; k <= niter; ++k) {
-        double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
-        const int iter = first_iteration + k - 1;
-        // AlternateStep when alternating constraints were on after the last
-        // SolvePairing (Main.f90:262-264); the previous row's kinetic energy
-        // then comes from the states before it
-        const bool alt = s.alt_next;
-        if (alt) {
-            if (deferred) {
-                if ((rc = kinetic_any(h))) return rc;
-                if ((rc = energy_kinetic_impl(h, prev))) return rc;
-            }
-            if ((rc = alternate_any(h))) return rc;
-        }
-        if (estimate) {
-            double out[6];
-            if ((rc = estimate_any(h, iter, dt, momentum, out))) return rc;
-            dt = out[0];
-            dth = dt / kHbar;
-            momentum = out[1];
-        }
-        if (params) {
-            params[2 * (k - 1)] = dt;
-            params[2 * (k - 1) + 1] = momentum;
-        }
-        if ((rc = grad_desc_any(h, dth, momentum))) return rc;
-        // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
-        // previous row's densities came from: that row's kinetic energy
-        if (deferred && !alt && (rc = energy_kinetic_impl(h, prev))) return rc;
-        if ((rc = fill_impl(h))) return rc;
-        s.alt_next = alternating(h);
-        if ((rc = density_any(h))) return rc;
-        if ((rc = mix_impl(h, 1))) return rc;
-        if ((rc = construct_impl(h))) return rc;
-        if ((rc = readjust_impl(h, iter, momentum))) return rc;
-        if ((rc = energy_terms_impl(h, row))) return rc;
-        if (!deferred) {
-            // HFB: the density basis is the canonical one, whose kinetic sums
-            // the GradDesc pass does not give
-            if ((rc = hfb_energy_impl(h, row))) return rc;
+    double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
+    // AlternateStep when alternating constraints were on after the last
+    // SolvePairing (Main.f90:262-264); the previous row's kinetic energy
+    // then comes from the states before it
+    const bool alt = s.alt_next;
+    if (alt) {
+        if (deferred) {
             if ((rc = kinetic_any(h))) return rc;
-            if ((rc = energy_kinetic_impl(h, row))) return rc;
+            if ((rc = energy_kinetic_impl(h, prev))) return rc;
         }
+        if ((rc = alternate_any(h))) return rc;
     }
+    if (estimate) {
+        double out[6];
+        if ((rc = estimate_any(h, iter, dt, momentum, out))) return rc;
+        dt = out[0];
+        dth = dt / kHbar;
+        momentum = out[1];
+    }
+    if (params) {
+        params[2 * (k - 1)] = dt;
+        params[2 * (k - 1) + 1] = momentum;
+    }
+    if ((rc = grad_desc_any(h, dth, momentum))) return rc;
+    // the hPsi pass of GradDesc also gave <psi|Lap psi> of the states the
+    // previous row's densities came from: that row's kinetic energy
+    if (deferred && !alt && (rc = energy_kinetic_impl(h, prev))) return rc;
+    if ((rc = fill_impl(h))) return rc;
+    s.alt_next = alternating(h);
+    if ((rc = density_any(h))) return rc;
+    if ((rc = mix_impl(h, 1))) return rc;
+    if ((rc = construct_impl(h))) return rc;
+    if ((rc = readjust_impl(h, iter, momentum))) return rc;
+    if ((rc = energy_terms_impl(h, row))) return rc;
+    if (!deferred) {
+        // HFB: the density basis is the canonical one, whose kinetic sums
+        // the GradDesc pass does not give
+        if ((rc = hfb_energy_impl(h, row))) return rc;
+        if ((rc = kinetic_any(h))) return rc;
+        if ((rc = energy_kinetic_impl(h, row))) return rc;
+    }
+    return 0;
+}
+
+int iter_end(mocca_solve *h, int niter, double *energies) {
+    Scf &s = h->scf;
+    int rc;
     // kinetic energy of the final states: one hPsi-type pass with no outputs
     double *last = s.erows + (size_t)niter * kRowLen;
-    if (deferred) {
+    if (!h->hfb.on) {
         if ((rc = kinetic_any(h))) return rc;
         if ((rc = energy_kinetic_impl(h, last))) return rc;
     }
@@ -421,6 +435,16 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
     }
     if ((rc = check_gs_status(h))) return rc;
     return h->hfb.on ? hfb_check_status(h) : 0;
+}
+
+int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, double dth,
+                 double momentum, bool estimate, double *energies, double *params) {
+    int rc;
+    if ((rc = iter_begin(h, niter))) return rc;
+    for (int k = 1; k <= niter; ++k)
+        if ((rc = iter_step(h, k, first_iteration + k - 1, dt, dth, momentum, estimate, params)))
+            return rc;
+    return iter_end(h, niter, energies);
 }
 
 
@@ -764,6 +788,39 @@ int mocca_iterate_estimated(mocca_solve *h, int first_iteration, int niter, doub
     if (first_iteration < 1) return fail(h, MOCCA_ERR_ARG, "first_iteration must be >= 1");
     return iterate_impl(h, niter, first_iteration, dt, dt / kHbar, momentum, true, energies,
                         params);
+}
+
+int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
+                    const double *dt_over_hbar, const double *momentum, double *const *energies) {
+    if (!solves || nsolves < 1 || !dt_over_hbar || !momentum) return MOCCA_ERR_ARG;
+    for (int i = 0; i < nsolves; ++i)
+        if (!solves[i]) return MOCCA_ERR_ARG;
+    std::vector<double> dt(nsolves), dth(nsolves), mu(nsolves);
+    for (int i = 0; i < nsolves; ++i) {
+        dth[i] = dt_over_hbar[i];
+        dt[i] = dth[i] * kHbar;
+        mu[i] = momentum[i];
+    }
+    // every solve's rows first, then iteration k of every solve in turn (each
+    // on its own stream, so the device overlaps them), then the final passes
+    // and checks; the first failure stops the batch and is reported on the
+    // failing solve
+    // (a failure is also reported as mocca_last_error(NULL), naming the solve)
+    auto failed = [&](int i, int rc) {
+        return fail(nullptr, rc, "batch solve %d: %s", i, solves[i]->err.c_str());
+    };
+    for (int i = 0; i < nsolves; ++i)
+        if (int rc = iter_begin(solves[i], niter)) return failed(i, rc);
+    for (int k = 1; k <= niter; ++k)
+        for (int i = 0; i < nsolves; ++i)
+            if (int rc = iter_step(solves[i], k, k, dt[i], dth[i], mu[i], false, nullptr))
+                return failed(i, rc);
+    int first = 0;
+    for (int i = 0; i < nsolves; ++i) {
+        const int rc = iter_end(solves[i], niter, energies ? energies[i] : nullptr);
+        if (rc && !first) first = failed(i, rc);
+    }
+    return first;
 }
 
 int mocca_estimate_parameters(mocca_solve *h, int iteration, double dt, double momentum,

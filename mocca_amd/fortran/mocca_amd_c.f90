@@ -210,6 +210,16 @@ module mocca_amd_c
       real(c_double), value :: dt, momentum
       real(c_double), intent(out) :: energies(*), params(*)
     end function
+    ! solves(nsolves): handles; energies(nsolves): c_loc of each solve's
+    ! rows(47, niter + 1), or c_null_ptr
+    integer(c_int) function mocca_batch_run(solves, nsolves, niter, dt_over_hbar, momentum, &
+    &                                       energies) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), intent(in) :: solves(*)
+      integer(c_int), value :: nsolves, niter
+      real(c_double), intent(in) :: dt_over_hbar(*), momentum(*)
+      type(c_ptr), intent(in) :: energies(*)
+    end function
     integer(c_int) function mocca_set_constraints(h, c, count, radd, acut, damping,        &
     &                                             readjust_slowdown) bind(C)
       import :: c_int, c_ptr, c_double, mocca_constraint

@@ -167,3 +167,28 @@ def lagrange_48(seed=5, n=48, dx=0.7, M=600, neutrons=12, protons=12, b=1.7):
     w = full_iteration(seed=seed, n=n, dx=dx, M=M, neutrons=neutrons, protons=protons, b=b)
     w["lagrange"] = True
     return w
+
+
+def batch_scan(seed=4, nsolves=64, dx=0.8, nmin=16, nmax=32, mmin=20, mmax=160,
+               neutrons=12, protons=12, q20=(-300.0, 600.0)):
+    """The batched constraint scan: nsolves independent solves of 12 + 12
+    nucleons, cubic meshes n ~ U{nmin..nmax}, nwt ~ U{mmin..mmax} (even, half
+    neutrons, half protons), harmonic-oscillator seeds (b = 1.7 fm), SLy4 with
+    Coulomb, HF, and an alternating (type 2) Q20 constraint spread uniformly
+    over q20 (solve k: q20[0] + (q20[1] - q20[0]) k / (nsolves - 1))."""
+    from . import functional
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nsolves):
+        n = int(rng.integers(nmin, nmax + 1))
+        M = 2 * int(rng.integers(mmin // 2, mmax // 2 + 1))
+        qn = np.array([(iso, 1 if j % 2 == 0 else -1, 1, 1)
+                       for iso in (-1, 1) for j in range(M // 2)], np.int32)
+        psi = ho_states(rng, n, n, n, dx, [tuple(q) for q in qn], b=1.7)
+        q = q20[0] + (q20[1] - q20[0]) * k / max(nsolves - 1, 1)
+        out.append(dict(nx=n, ny=n, nz=n, dx=dx, dv=dx**3 * 8, M=M, trc=True, lagrange=False,
+                        qn=qn, occ=np.zeros(M), psi=psi,
+                        functional=functional.scf_arguments("SLy4", neutrons, protons, True),
+                        constraints=[dict(l=2, m=0, type=2, value=q)],
+                        dt_over_hbar=0.010 / functional.HBAR, momentum=0.3))
+    return out
