@@ -807,6 +807,8 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
 
 int mocca_solve_destroy(mocca_solve *h) {
     if (!h) return 0;
+    graphs_clear(h);
+    if (h->graph_rows) (void)hipFree(h->graph_rows);
     free_all(h);
     delete h;
     return 0;
@@ -819,6 +821,7 @@ int mocca_synchronize(mocca_solve *h) {
 }
 
 int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
+    if (h) ++h->graph_epoch;
     if (!h || !qn || !occ) return fail(h, MOCCA_ERR_ARG, "null argument");
     const int M = h->M;
     for (int i = 0; i < M; ++i) {
@@ -1055,6 +1058,7 @@ int mocca_get_energies(mocca_solve *h, double *energy, double *disp) {
 }
 
 int mocca_upload_states(mocca_solve *h, int first, int count, const double *grid) {
+    if (h) ++h->graph_epoch;
     if (!h || !grid) return fail(h, MOCCA_ERR_ARG, "null argument");
     if (first < 0 || count < 0 || first + count > h->M)
         return fail(h, MOCCA_ERR_ARG, "states [%d, %d) out of range (nwt=%d)", first,
@@ -1084,6 +1088,7 @@ int mocca_reset_updates(mocca_solve *h) {
 }
 
 int mocca_set_potentials(mocca_solve *h, const mocca_fields *f) {
+    if (h) ++h->graph_epoch;
     if (!h || !f) return fail(h, MOCCA_ERR_ARG, "null argument");
     if (!f->U || !f->B || !f->NablaB || !f->W || !f->DivW)
         return fail(h, MOCCA_ERR_ARG, "U, B, NablaB, W and DivW are required");

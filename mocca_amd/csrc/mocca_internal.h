@@ -144,6 +144,17 @@ struct Hfb {
 }  // namespace detail
 }  // namespace mocca
 
+namespace mocca {
+namespace detail {
+// A captured GradDesc-to-energy iteration of mocca_batch_run for one
+// configuration of the rotating state buffers (pre), which it leaves in post.
+struct IterGraph {
+    void *pre[4], *post[4];
+    hipGraphExec_t exec;
+};
+}  // namespace detail
+}  // namespace mocca
+
 struct mocca_solve {
     mocca_mesh mesh{};
     int M = 0, N = 0, prec = MOCCA_FP64, tsize = 8;
@@ -157,6 +168,13 @@ struct mocca_solve {
     mocca::MeshK<float> mf{};
     bool meta_set = false, fields_set = false;
     bool prev_valid = false;   // b.upd holds the states before the last GradDesc step
+    // mocca_batch_run's captured iterations, valid while no buffer is
+    // (re)allocated and no setting changes (graph_epoch counts both) and for
+    // the dt, momentum they were captured with
+    unsigned long graph_epoch = 0, graphs_epoch = 0;
+    double graphs_dth = 0, graphs_mu = 0;
+    std::vector<mocca::detail::IterGraph> graphs;
+    double *graph_rows = nullptr;   // [2][row]: previous, current
     double kin[2] = {0, 0};
     std::vector<int> qn;
     std::vector<double> occ;
@@ -196,6 +214,7 @@ int fail(mocca_solve *h, int code, const char *fmt, ...);
 
 template <typename P>
 int dalloc(mocca_solve *h, P **p, size_t bytes) {
+    ++h->graph_epoch;
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, bytes ? bytes : 8);
     if (e != hipSuccess)
@@ -247,6 +266,8 @@ int alternate_any(mocca_solve *h);
 // IterativeEstimation: out = dt, momentum, maxE, relE, con, iterations.
 int estimate_any(mocca_solve *h, int iteration, double dt, double mu, double *out);
 constexpr double kHbar = 6.58211928;   // Force.f90:103
+// Destroys mocca_batch_run's captured iterations of a solve.
+void graphs_clear(mocca_solve *h);
 // Releases the buffers of the self-consistent part.
 void scf_free(mocca_solve *h);
 

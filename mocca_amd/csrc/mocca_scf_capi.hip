@@ -366,14 +366,14 @@ int iter_begin(mocca_solve *h, int niter) {
     return 0;
 }
 
-// Iteration k (1-based within the call) with Evolve's iteration number iter.
+// Iteration k (1-based within the call) with Evolve's iteration number iter,
+// writing its energy row to row and completing the previous one in prev.
 int iter_step(mocca_solve *h, int k, int iter, double &dt, double &dth, double &momentum,
-              bool estimate, double *params) {
+              bool estimate, double *params, double *prev, double *row) {
     Scf &s = h->scf;
     int rc;
     // HF: a row's kinetic energy comes from the next GradDesc's hPsi pass
     const bool deferred = !h->hfb.on;
-    double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
     // AlternateStep when alternating constraints were on after the last
     // SolvePairing (Main.f90:262-264); the previous row's kinetic energy
     // then comes from the states before it
@@ -442,9 +442,99 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
     int rc;
     if ((rc = iter_begin(h, niter))) return rc;
     for (int k = 1; k <= niter; ++k)
-        if ((rc = iter_step(h, k, first_iteration + k - 1, dt, dth, momentum, estimate, params)))
+        if ((rc = iter_step(h, k, first_iteration + k - 1, dt, dth, momentum, estimate, params,
+                            h->scf.erows + (size_t)(k - 1) * kRowLen,
+                            h->scf.erows + (size_t)k * kRowLen)))
             return rc;
     return iter_end(h, niter, energies);
+}
+
+// Replayed iterations of mocca_batch_run.  An iteration's kernels depend on
+// the host only through the buffer pointers that rotate from one iteration
+// to the next (states, stepped states, previous states, the states before an
+// AlternateStep) and the energy rows; the rotation has period 2 (3 with
+// AlternateStep).  So one graph is captured per pointer configuration, with
+// the rows in two fixed slots that are copied out after each launch, and a
+// replay restores the configuration the captured iteration left behind.
+void ptr_state(mocca_solve *h, void *(&k)[4]) {
+    k[0] = h->b.psi; k[1] = h->b.psi2; k[2] = h->b.upd; k[3] = h->scf.alt_old;
+}
+
+// Whether the iterations of a solve may be replayed: HF (the HFB step and
+// its canonical basis keep host-side state), no profiling, and constraints
+// that are never switched off (TurnOffConstraints reads the iteration
+// number).
+bool graph_ok(const mocca_solve *h) {
+    if (h->hfb.on || h->prof) return false;
+    for (size_t c = 0; c < h->scf.htype.size(); ++c)
+        if (h->scf.htype[c] != 0 && h->scf.hiter[c] > 0) return false;
+    return true;
+}
+
+}  // namespace
+
+namespace mocca {
+namespace detail {
+void graphs_clear(mocca_solve *h) {
+    if (!h->graphs.empty()) (void)hipStreamSynchronize(h->st);
+    for (IterGraph &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+}  // namespace detail
+}  // namespace mocca
+
+namespace {
+
+// One replayed (or, for a new pointer configuration, captured) iteration k
+// of a batch; rows: two slots [2][kRowLen] (previous, current).
+int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, double dth,
+               double momentum, double *rows, bool &ok) {
+    Scf &s = h->scf;
+    void *cur[4];
+    ptr_state(h, cur);
+    IterGraph *g = nullptr;
+    for (IterGraph &c : cache)
+        if (std::equal(cur, cur + 4, c.pre)) g = &c;
+    if (!g) {
+        IterGraph n{};
+        std::copy(cur, cur + 4, n.pre);
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(h->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            ok = false;
+            return 0;
+        }
+        const int rc = iter_step(h, k, k, dt, dth, momentum, false, nullptr, rows,
+                                 rows + kRowLen);
+        const hipError_t ec = hipStreamEndCapture(h->st, &graph);
+        if (rc || ec != hipSuccess || !graph ||
+            hipGraphInstantiate(&n.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            h->err.clear();
+            // the host state moved on as if the iteration had run: undo the
+            // pointer rotation and run it eagerly
+            h->b.psi = cur[0]; h->b.psi2 = cur[1]; h->b.upd = cur[2]; s.alt_old = cur[3];
+            ok = false;
+            return 0;
+        }
+        (void)hipGraphDestroy(graph);
+        ptr_state(h, n.post);
+        cache.push_back(n);
+        g = &cache.back();
+    } else {
+        h->b.psi = g->post[0]; h->b.psi2 = g->post[1]; h->b.upd = g->post[2];
+        s.alt_old = g->post[3];
+    }
+    HIPCHK(h, hipGraphLaunch(g->exec, h->st));
+    double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
+    HIPCHK(h, hipMemcpyAsync(prev, rows, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->st));
+    HIPCHK(h, hipMemcpyAsync(row, rows + kRowLen, kRowLen * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(rows, rows + kRowLen, kRowLen * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->st));
+    return 0;
 }
 
 
@@ -472,6 +562,7 @@ void scf_free(mocca_solve *h) {
 extern "C" {
 
 int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
+    if (h) ++h->graph_epoch;
     if (!h || !f) return fail(h, MOCCA_ERR_ARG, "null argument");
     for (int k = 20; k < 28; ++k)
         if (f->B[k] != 0.0)
@@ -811,14 +902,59 @@ int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
     };
     for (int i = 0; i < nsolves; ++i)
         if (int rc = iter_begin(solves[i], niter)) return failed(i, rc);
-    for (int k = 1; k <= niter; ++k)
-        for (int i = 0; i < nsolves; ++i)
-            if (int rc = iter_step(solves[i], k, k, dt[i], dth[i], mu[i], false, nullptr))
-                return failed(i, rc);
+    // Iteration 1 of every solve runs eagerly (it sets up what later
+    // iterations reuse); later ones are graph replays where graph_ok holds.
+    // The captured iterations stay with the solve for later calls.
+    std::vector<char> use_graph(nsolves, 0);
+    int rc = 0, bad = -1;
+    for (int k = 1; k <= niter && !rc; ++k) {
+        for (int i = 0; i < nsolves && !rc; ++i) {
+            mocca_solve *h = solves[i];
+            if (k == 2 && niter >= 3 && graph_ok(h)) {
+                if (h->graphs_epoch != h->graph_epoch || h->graphs_dth != dth[i] ||
+                    h->graphs_mu != mu[i])
+                    graphs_clear(h);
+                if (!h->graph_rows &&
+                    hipMalloc(&h->graph_rows, 2 * kRowLen * sizeof(double)) != hipSuccess) {
+                    h->graph_rows = nullptr;
+                    (void)hipGetLastError();
+                }
+                // both slots start as row 1 (entries an iteration does not
+                // write, such as the HF pairing terms, keep their values)
+                if (h->graph_rows &&
+                    hipMemcpyAsync(h->graph_rows, h->scf.erows + kRowLen,
+                                   kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                                   h->st) == hipSuccess &&
+                    hipMemcpyAsync(h->graph_rows + kRowLen, h->scf.erows + kRowLen,
+                                   kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                                   h->st) == hipSuccess) {
+                    use_graph[i] = 1;
+                    h->graphs_epoch = h->graph_epoch;
+                    h->graphs_dth = dth[i];
+                    h->graphs_mu = mu[i];
+                } else {
+                    (void)hipGetLastError();
+                }
+            }
+            if (use_graph[i]) {
+                bool ok = true;
+                rc = graph_step(h, h->graphs, k, dt[i], dth[i], mu[i], h->graph_rows, ok);
+                if (!rc && ok) continue;
+                if (rc) { bad = i; break; }
+                use_graph[i] = 0;   // capture refused: this solve runs eagerly
+                graphs_clear(h);
+            }
+            rc = iter_step(h, k, k, dt[i], dth[i], mu[i], false, nullptr,
+                           h->scf.erows + (size_t)(k - 1) * kRowLen,
+                           h->scf.erows + (size_t)k * kRowLen);
+            if (rc) bad = i;
+        }
+    }
+    if (rc) return failed(bad, rc);
     int first = 0;
     for (int i = 0; i < nsolves; ++i) {
-        const int rc = iter_end(solves[i], niter, energies ? energies[i] : nullptr);
-        if (rc && !first) first = failed(i, rc);
+        const int r = iter_end(solves[i], niter, energies ? energies[i] : nullptr);
+        if (r && !first) first = failed(i, r);
     }
     return first;
 }
@@ -872,6 +1008,7 @@ int mocca_get_coulomb(mocca_solve *h, double *vcoul, double *vcoulx, double *cpo
 
 int mocca_set_constraints(mocca_solve *h, const mocca_constraint *c, int count, double radd,
                           double acut, double damping, double readjust_slowdown) {
+    if (h) ++h->graph_epoch;
     if (int rc = check_scf(h)) return rc;
     if (count < 0 || (count > 0 && !c)) return fail(h, MOCCA_ERR_ARG, "bad constraint list");
     if (!(acut > 0.0)) return fail(h, MOCCA_ERR_ARG, "acut must be positive");
@@ -993,6 +1130,7 @@ int mocca_readjust_constraints(mocca_solve *h, int iteration, double momentum) {
 }
 
 int mocca_set_pairing(mocca_solve *h, const mocca_pairing *p) {
+    if (h) ++h->graph_epoch;
     if (int rc = check_scf(h)) return rc;
     if (!p) return fail(h, MOCCA_ERR_ARG, "null pairing parameters");
     HIPCHK(h, hipStreamSynchronize(h->st));
