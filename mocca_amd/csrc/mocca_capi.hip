@@ -978,10 +978,9 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     // matrix-core Gram for blocks of up to kMfmaMaxRows states: K splits for
     // ~2048 workgroups, partial matrices kept under 5% of the states' bytes
     h->mpad = (mmax + 15) / 16 * 16;
-    // (below 64 states the vector kernel streams the block faster)
-    h->mfma_gram = h->mpad >= 32 && h->mpad <= kMfmaMaxRows;
+    h->mfma_gram = h->mpad <= kMfmaMaxRows;
     // (the apply keeps one output row of tiles per wave: up to 192 states)
-    h->mfma_apply = h->mpad >= 32 && h->mpad <= kMfmaApplyMaxRows;
+    h->mfma_apply = h->mpad <= kMfmaApplyMaxRows;
     const int nt16 = h->mpad / 16, ntile16 = nt16 * (nt16 + 1) / 2;
     {
         // one round of single-wave workgroups (1024 = 256 CUs x 4 SIMDs at

@@ -1319,7 +1319,7 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
         break;                                                                              \
     }
     switch (mpad_max / 16) {
-        MOCCA_APPLY_NT(2) MOCCA_APPLY_NT(3)
+        MOCCA_APPLY_NT(1) MOCCA_APPLY_NT(2) MOCCA_APPLY_NT(3)
         MOCCA_APPLY_NT(4) MOCCA_APPLY_NT(5) MOCCA_APPLY_NT(6) MOCCA_APPLY_NT(7)
         MOCCA_APPLY_NT(8) MOCCA_APPLY_NT(9) MOCCA_APPLY_NT(10) MOCCA_APPLY_NT(11)
         MOCCA_APPLY_NT(12)

@@ -5,6 +5,8 @@
 #include "mocca_internal.h"
 
 #include <algorithm>
+#include <functional>
+#include <thread>
 #include <cmath>
 #include <cstring>
 
@@ -904,52 +906,68 @@ int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
         if (int rc = iter_begin(solves[i], niter)) return failed(i, rc);
     // Iteration 1 of every solve runs eagerly (it sets up what later
     // iterations reuse); later ones are graph replays where graph_ok holds.
-    // The captured iterations stay with the solve for later calls.
+    // The captured iterations stay with the solve for later calls.  The
+    // solves are split over a few host threads, each issuing its share in
+    // turn: one thread cannot issue the kernels of many small solves as fast
+    // as the device runs them.
     std::vector<char> use_graph(nsolves, 0);
-    int rc = 0, bad = -1;
-    for (int k = 1; k <= niter && !rc; ++k) {
-        for (int i = 0; i < nsolves && !rc; ++i) {
-            mocca_solve *h = solves[i];
-            if (k == 2 && niter >= 3 && graph_ok(h)) {
-                if (h->graphs_epoch != h->graph_epoch || h->graphs_dth != dth[i] ||
-                    h->graphs_mu != mu[i])
+    auto run_range = [&](int i0, int i1, int &rc, int &bad) {
+        for (int k = 1; k <= niter && !rc; ++k) {
+            for (int i = i0; i < i1 && !rc; ++i) {
+                mocca_solve *h = solves[i];
+                if (k == 2 && niter >= 3 && graph_ok(h)) {
+                    if (h->graphs_epoch != h->graph_epoch || h->graphs_dth != dth[i] ||
+                        h->graphs_mu != mu[i])
+                        graphs_clear(h);
+                    if (!h->graph_rows &&
+                        hipMalloc(&h->graph_rows, 2 * kRowLen * sizeof(double)) != hipSuccess) {
+                        h->graph_rows = nullptr;
+                        (void)hipGetLastError();
+                    }
+                    // both slots start as row 1 (entries an iteration does not
+                    // write, such as the HF pairing terms, keep their values)
+                    if (h->graph_rows &&
+                        hipMemcpyAsync(h->graph_rows, h->scf.erows + kRowLen,
+                                       kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                                       h->st) == hipSuccess &&
+                        hipMemcpyAsync(h->graph_rows + kRowLen, h->scf.erows + kRowLen,
+                                       kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                                       h->st) == hipSuccess) {
+                        use_graph[i] = 1;
+                        h->graphs_epoch = h->graph_epoch;
+                        h->graphs_dth = dth[i];
+                        h->graphs_mu = mu[i];
+                    } else {
+                        (void)hipGetLastError();
+                    }
+                }
+                if (use_graph[i]) {
+                    bool ok = true;
+                    rc = graph_step(h, h->graphs, k, dt[i], dth[i], mu[i], h->graph_rows, ok);
+                    if (!rc && ok) continue;
+                    if (rc) { bad = i; break; }
+                    use_graph[i] = 0;   // capture refused: this solve runs eagerly
                     graphs_clear(h);
-                if (!h->graph_rows &&
-                    hipMalloc(&h->graph_rows, 2 * kRowLen * sizeof(double)) != hipSuccess) {
-                    h->graph_rows = nullptr;
-                    (void)hipGetLastError();
                 }
-                // both slots start as row 1 (entries an iteration does not
-                // write, such as the HF pairing terms, keep their values)
-                if (h->graph_rows &&
-                    hipMemcpyAsync(h->graph_rows, h->scf.erows + kRowLen,
-                                   kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
-                                   h->st) == hipSuccess &&
-                    hipMemcpyAsync(h->graph_rows + kRowLen, h->scf.erows + kRowLen,
-                                   kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
-                                   h->st) == hipSuccess) {
-                    use_graph[i] = 1;
-                    h->graphs_epoch = h->graph_epoch;
-                    h->graphs_dth = dth[i];
-                    h->graphs_mu = mu[i];
-                } else {
-                    (void)hipGetLastError();
-                }
+                rc = iter_step(h, k, k, dt[i], dth[i], mu[i], false, nullptr,
+                               h->scf.erows + (size_t)(k - 1) * kRowLen,
+                               h->scf.erows + (size_t)k * kRowLen);
+                if (rc) bad = i;
             }
-            if (use_graph[i]) {
-                bool ok = true;
-                rc = graph_step(h, h->graphs, k, dt[i], dth[i], mu[i], h->graph_rows, ok);
-                if (!rc && ok) continue;
-                if (rc) { bad = i; break; }
-                use_graph[i] = 0;   // capture refused: this solve runs eagerly
-                graphs_clear(h);
-            }
-            rc = iter_step(h, k, k, dt[i], dth[i], mu[i], false, nullptr,
-                           h->scf.erows + (size_t)(k - 1) * kRowLen,
-                           h->scf.erows + (size_t)k * kRowLen);
-            if (rc) bad = i;
         }
-    }
+    };
+    const int nthr = std::max(1, std::min({nsolves, 8,
+                                           (int)std::thread::hardware_concurrency() / 2}));
+    std::vector<int> trc(nthr, 0), tbad(nthr, -1);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; ++t)
+        pool.emplace_back(run_range, nsolves * t / nthr, nsolves * (t + 1) / nthr,
+                          std::ref(trc[t]), std::ref(tbad[t]));
+    run_range(0, nsolves / nthr, trc[0], tbad[0]);
+    for (std::thread &th : pool) th.join();
+    int rc = 0, bad = -1;
+    for (int t = 0; t < nthr && !rc; ++t)
+        if (trc[t]) { rc = trc[t]; bad = tbad[t]; }
     if (rc) return failed(bad, rc);
     int first = 0;
     for (int i = 0; i < nsolves; ++i) {

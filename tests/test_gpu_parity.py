@@ -127,11 +127,12 @@ def test_microbench_workload_vs_oracle():
     s.close()
 
 
-@pytest.mark.parametrize("m", [25, 100, 150, 190, 250])
+@pytest.mark.parametrize("m", [5, 25, 100, 150, 190, 250])
 def test_gram_schmidt_block_sizes_vs_householder_qr(m):
     """Gram-Schmidt of one block of m states (energy order) through every
-    kernel family: MFMA Gram up to 128 states, MFMA apply up to 192, vector
-    kernels and the global-memory Cholesky beyond.  Modified Gram-Schmidt in
+    kernel family: MFMA Gram up to 128 states (one 16-row tile from 1 to 16),
+    MFMA apply up to 192, vector kernels and the global-memory Cholesky
+    beyond.  Modified Gram-Schmidt in
     a given order equals the QR factorisation with a positive-diagonal R, so
     the result is checked against numpy's Householder QR of the same
     energy-sorted states (an independent algorithm), 1e-10."""
