@@ -170,7 +170,8 @@ void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
                         double *G, hipStream_t st);
 
 // Energy order, Cholesky and T = L^{-T} per block; in LDS when the largest
-// block fits in lds_limit bytes (cholesky_lds_limit()), else in global memory.
+// block fits in lds_limit bytes (cholesky_lds_limit()), else with its packed
+// lower triangle in LDS (up to 192 states), else in global memory.
 void launch_cholesky_ld(const double *G, const int *blk_states,
                         const int *blk_m, int nblk, int max_m, int mmax, const double *E,
                         double *Tm, int *perm, double *work, int *status,
@@ -189,8 +190,9 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
                           hipStream_t st);
 
 // Gram matrices on the FP64 matrix cores (v_mfma_f64_16x16x4f64) for blocks
-// of 64..kMfmaMaxRows states (one wave holds all upper 16x16 tiles).
-constexpr int kMfmaMaxRows = 128;
+// of up to kMfmaMaxRows states (one wave holds all upper 16x16 tiles of up
+// to 128 states; 129-160 states take three waves per K slice).
+constexpr int kMfmaMaxRows = 160;
 constexpr int kMfmaApplyMaxRows = 192;
 
 template <typename T>

@@ -1035,6 +1035,119 @@ __global__ __launch_bounds__(1024) void k_cholesky_lds(const double *__restrict_
     }
 }
 
+// The same factorisation for blocks of up to kCholPackedMax states, with only
+// the lower triangle resident in LDS (row i at i(i+1)/2), so 150-190 states
+// fit.  The factorisation is the right-looking one above, operation for
+// operation; X = L^{-1} then goes column by column, one wave per column with
+// the column in registers (lane l holds rows l, l+64, l+128), in the same
+// order of updates as the in-LDS substitution, and is stored as T = X^T.
+constexpr int kCholPackedMax = 192;
+
+__device__ __forceinline__ int tri_row(int i) { return i * (i + 1) / 2; }
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__global__ __launch_bounds__(1024) void k_cholesky_packed(const double *__restrict__ G,
+                                                          const int *__restrict__ blk_states,
+                                                          const int *__restrict__ blk_m,
+                                                          int max_m,
+                                                          const double *__restrict__ E,
+                                                          double *__restrict__ Tm,
+                                                          int *__restrict__ perm,
+                                                          int *__restrict__ status) {
+    extern __shared__ double A[];           // packed lower triangle
+    __shared__ int rank[kMaxBlock];
+    __shared__ double dinv[kMaxBlock];      // first the block's energies, then 1/L[i][i]
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid % 64, wave = tid / 64, nwave = nt / 64;
+    const int m = blk_m[b];
+    const int *st = blk_states + (long)b * max_m;
+    const double *g = G + (long)b * max_m * max_m;
+    double *Tb = Tm + (long)b * max_m * max_m;
+    int *pb = perm + (long)b * max_m;
+    for (int i = tid; i < m; i += nt) dinv[i] = E[st[i]];
+    __syncthreads();
+    for (int i = tid; i < m; i += nt) {
+        const double ei = dinv[i];
+        int r = 0;
+        for (int j = 0; j < m; ++j) {
+            const double ej = dinv[j];
+            if (ej < ei || (ej == ei && j < i)) ++r;
+        }
+        rank[i] = r;
+        pb[r] = st[i];
+    }
+    __syncthreads();
+    for (int q = tid; q < m * m; q += nt) {
+        const int i = q / m, j = q % m;
+        const int ri = rank[i], rj = rank[j];
+        if (rj <= ri) A[tri_row(ri) + rj] = g[(long)i * max_m + j];
+    }
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        const double akk = A[tri_row(k) + k];
+        if (k > 0) {
+            const double pv = A[tri_row(k - 1) + k - 1];
+            const double d = sqrt(pv > 0 ? pv : 1.0);
+            for (int i = k + tid; i < m; i += nt) A[tri_row(i) + k - 1] /= d;
+        }
+        if (tid == 0 && !(akk > 0.0)) atomicOr(status, 1);
+        const double inv = 1.0 / (akk > 0.0 ? akk : 1.0);
+        for (int i = k + 1 + tid / 32; i < m; i += 32) {
+            const int ri = tri_row(i);
+            const double aik = A[ri + k] * inv;
+            for (int j = k + 1 + tid % 32; j <= i; j += 32) A[ri + j] -= aik * A[tri_row(j) + k];
+        }
+        __syncthreads();
+        if (k > 0 && tid == 0) {
+            const double pv = A[tri_row(k - 1) + k - 1];
+            A[tri_row(k - 1) + k - 1] = sqrt(pv > 0 ? pv : 1.0);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double pv = A[tri_row(m - 1) + m - 1];
+        A[tri_row(m - 1) + m - 1] = sqrt(pv > 0 ? pv : 1.0);
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += nt) dinv[i] = 1.0 / A[tri_row(i) + i];
+    __syncthreads();
+    // column j of X: R = e_j; for k = j..m-1: X[k][j] = R[k] / L[k][k],
+    // R[i] -= L[i][k] X[k][j] for i > k.
+    const int nr = (m + 63) / 64;
+    for (int j = wave; j < m; j += nwave) {
+        double x[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) x[r] = (lane + 64 * r == j) ? 1.0 : 0.0;
+        for (int k = j; k < m; ++k) {
+            const int rk = k >> 6, lk = k & 63;
+            const double own = rk == 0 ? x[0] : (rk == 1 ? x[1] : x[2]);
+            const double dk = dinv[k];
+            const double xk = k == j ? dk : readlane_d(own, lk) * dk;
+            if (lane == lk) {
+                if (rk == 0) x[0] = xk; else if (rk == 1) x[1] = xk; else x[2] = xk;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const int i = lane + 64 * r;
+                if (r < nr && i > k && i < m) x[r] -= A[tri_row(i) + k] * xk;
+            }
+        }
+        // T = L^{-T}: row j of T holds X[i][j] for i >= j
+        double *trow = Tb + (long)j * max_m;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int i = lane + 64 * r;
+            if (r < nr && i < m) trow[i] = i < j ? 0.0 : x[r];
+        }
+    }
+}
+
 // Q = Psi_sorted T for each block, fused with the heavy-ball memory update
 // u = psi_new - u of GradDesc (ImaginaryTime.f90:497-499) when update_mode.
 // The T entries are the same for every lane: read from memory with
@@ -1112,28 +1225,37 @@ void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
 // A[l%16][l/16] and B[l/16][l%16]; result lane l holds D[l/16 + 4r][l%16] in
 // register r -- measured on gfx950).
 
-// Gram partials: one wave per (K slice, block) holds all upper 16x16 tiles
-// of the block (NT row tiles, <= 8).  Per 16 mesh values each lane loads 4
-// consecutive values of its state row (16 lanes = 16 states, 4 lane groups =
-// 16 consecutive k): the k order inside a 16-chunk is permuted the same way
-// for both operands, which leaves the sum over k unchanged.  Output: the
-// raw accumulator layout part[b][split][tile][lane][reg].
-template <typename T, int NT>
+// Gram partials: one wave per (K slice, block, job) holds a set of upper
+// 16x16 tiles of the block.  A symmetric job (SYM) owns the tiles I <= J of
+// NI row tiles starting at tile I0 = job * NI; the rectangular job owns the
+// NI x NJ tiles of row tiles [0, NI) against [NI, NI + NJ).  Blocks of up to
+// 8 row tiles are one symmetric job; 9-10 row tiles are two symmetric jobs
+// and one rectangular one (each state read twice).  Per 16 mesh values each
+// lane loads 4 consecutive values of its state row (16 lanes = 16 states, 4
+// lane groups = 16 consecutive k): the k order inside a 16-chunk is permuted
+// the same way for both operands, which leaves the sum over k unchanged.
+// Output: the raw accumulator layout part[b][split][tile][lane][reg], tiles
+// numbered over the upper triangle of all nt row tiles; tiles past nt (an
+// odd nt split in two) are skipped.
+template <typename T, int NI, int NJ, bool SYM>
 __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, long K4,
                                                   const int *__restrict__ blk_states,
                                                   const int *__restrict__ blk_m, int max_m,
-                                                  double *__restrict__ part, int nsplit) {
-    constexpr int NTILE = NT * (NT + 1) / 2;
+                                                  double *__restrict__ part, int nsplit,
+                                                  int nt) {
+    constexpr int NR = SYM ? NI : NI + NJ;                   // row tiles loaded
+    constexpr int NTILE = SYM ? NI * (NI + 1) / 2 : NI * NJ;  // tiles accumulated
     const int w = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
+    const int I0 = SYM ? blockIdx.z * NI : 0;
     const int m = blk_m[b];
     const int *st = blk_states + (long)b * max_m;
     const long nch = (K4 + 15) / 16;
     const long c0 = nch * w / nsplit, c1 = nch * (w + 1) / nsplit;
-    const T *rows[NT];
-    bool ok[NT];
+    const T *rows[NR];
+    bool ok[NR];
 #pragma unroll
-    for (int I = 0; I < NT; ++I) {
-        const int r = I * 16 + lane % 16;
+    for (int I = 0; I < NR; ++I) {
+        const int r = (I0 + I) * 16 + lane % 16;
         ok[I] = r < m;
         rows[I] = psi + (long)(ok[I] ? st[r] : 0) * K4;
     }
@@ -1145,23 +1267,23 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
     // chunk c run; it is widened to FP64 afterwards, with those entries
     // zeroed.  K4 is a multiple of 4, so a lane's 4 values are all in range
     // or all out.
-    auto load = [&](long c, T (&raw)[NT][4]) {
+    auto load = [&](long c, T (&raw)[NR][4]) {
         const long kb = c * 16 + 4 * (lane / 16);
         const long ks = kb < K4 ? kb : 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I)
+        for (int I = 0; I < NR; ++I)
 #pragma unroll
             for (int q = 0; q < 4; ++q) raw[I][q] = rows[I][ks + q];
     };
-    auto widen = [&](long c, const T (&raw)[NT][4], double (&v)[NT][4]) {
+    auto widen = [&](long c, const T (&raw)[NR][4], double (&v)[NR][4]) {
         const bool kin = c * 16 + 4 * (lane / 16) < K4;
 #pragma unroll
-        for (int I = 0; I < NT; ++I)
+        for (int I = 0; I < NR; ++I)
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[I][q] = (kin && ok[I]) ? (double)raw[I][q] : 0.0;
     };
-    double v[NT][4];
-    T raw[NT][4];
+    double v[NR][4];
+    T raw[NR][4];
     if (c0 < c1) {
         load(c0, raw);
         widen(c0, raw, v);
@@ -1170,18 +1292,26 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
         if (c + 1 < c1) load(c + 1, raw);
         int t = 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I)
+        for (int I = 0; I < NI; ++I)
 #pragma unroll
-            for (int J = I; J < NT; ++J, ++t)
+            for (int J = SYM ? I : NI; J < (SYM ? NI : NR); ++J, ++t)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[t] = mfma16x16x4(v[I][q], v[J][q], acc[t]);
         widen(c + 1, raw, v);   // after the last chunk: unused
     }
-    double *P = part + (((long)b * nsplit + w) * NTILE) * 256 + lane * 4;
+    const int ntile = nt * (nt + 1) / 2;
+    double *P = part + (((long)b * nsplit + w) * ntile) * 256 + lane * 4;
+    int t = 0;
 #pragma unroll
-    for (int t = 0; t < NTILE; ++t)
+    for (int I = 0; I < NI; ++I)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) P[t * 256 + r] = acc[t][r];
+        for (int J = SYM ? I : NI; J < (SYM ? NI : NR); ++J, ++t) {
+            const int gi = I0 + I, gj = I0 + J;   // row tiles of the whole block
+            if (gj >= nt) continue;
+            const int tg = gi * nt - gi * (gi - 1) / 2 + (gj - gi);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) P[tg * 256 + r] = acc[t][r];
+        }
 }
 
 // Q = Psi_sorted T on the FP64 matrix cores, for blocks of 64..128 states,
@@ -1366,17 +1496,26 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
                       int max_m, int mpad_max, double *part, int nsplit, double dv, double *G,
                       hipStream_t st) {
     const int nt = mpad_max / 16;
-    const dim3 g(nsplit, nblk), blk(64);
+    const dim3 g(nsplit, nblk), g2(nsplit, nblk, 2), blk(64);
 #define MOCCA_GRAM_NT(K)                                                                    \
     case K:                                                                                 \
-        hipLaunchKernelGGL((k_gram_mfma<T, K>), g, blk, 0, st, psi, K4, blk_states, blk_m, \
-                           max_m, part, nsplit);                                           \
+        hipLaunchKernelGGL((k_gram_mfma<T, K, K, true>), g, blk, 0, st, psi, K4, blk_states, \
+                           blk_m, max_m, part, nsplit, nt);                                \
+        break;
+#define MOCCA_GRAM_NT2(K, H)                                                                \
+    case K:                                                                                 \
+        hipLaunchKernelGGL((k_gram_mfma<T, H, H, true>), g2, blk, 0, st, psi, K4,          \
+                           blk_states, blk_m, max_m, part, nsplit, nt);                    \
+        hipLaunchKernelGGL((k_gram_mfma<T, H, H, false>), g, blk, 0, st, psi, K4,          \
+                           blk_states, blk_m, max_m, part, nsplit, nt);                    \
         break;
     switch (nt) {
         MOCCA_GRAM_NT(1) MOCCA_GRAM_NT(2) MOCCA_GRAM_NT(3) MOCCA_GRAM_NT(4)
         MOCCA_GRAM_NT(5) MOCCA_GRAM_NT(6) MOCCA_GRAM_NT(7) MOCCA_GRAM_NT(8)
+        MOCCA_GRAM_NT2(9, 5) MOCCA_GRAM_NT2(10, 5)
         default: return;
     }
+#undef MOCCA_GRAM_NT2
 #undef MOCCA_GRAM_NT
     const long pairs = (long)max_m * (max_m + 1) / 2;
     const long per = kBlock / 64;
@@ -1417,10 +1556,18 @@ void launch_cholesky_ld(const double *G, const int *blk_states, const int *blk_m
                         int nblk, int max_m, int mmax, const double *E, double *Tm,
                         int *perm, double *work, int *status, size_t lds_limit,
                         hipStream_t st) {
+    // the whole block in LDS while it fits (up to ~137 states), else the
+    // packed lower triangle (up to kCholPackedMax), else global memory
     const size_t lds = sizeof(double) * (size_t)mmax * (mmax + 1);
     if (lds <= lds_limit) {
         hipLaunchKernelGGL(k_cholesky_lds, dim3(nblk), dim3(1024), lds, st, G, blk_states,
                            blk_m, max_m, E, Tm, perm, status);
+        return;
+    }
+    const size_t packed = sizeof(double) * (size_t)mmax * (mmax + 1) / 2;
+    if (mmax <= kCholPackedMax && packed <= lds_limit) {
+        hipLaunchKernelGGL(k_cholesky_packed, dim3(nblk), dim3(1024), packed, st, G,
+                           blk_states, blk_m, max_m, E, Tm, perm, status);
         return;
     }
     hipLaunchKernelGGL(k_cholesky, dim3(nblk), dim3(kBlock), 0, st, G, blk_states,
@@ -1438,6 +1585,8 @@ size_t cholesky_lds_limit() {
     if ((size_t)lim <= stat) return 0;
     const size_t dyn = (size_t)lim - stat;
     if (hipFuncSetAttribute((const void *)k_cholesky_lds,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_cholesky_packed,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
         return 0;
     return dyn;

@@ -835,19 +835,230 @@ __global__ __launch_bounds__(kBlock) void k_density_fd(
     }
 }
 
+// The same densities with one barrier per plane: while plane z is computed,
+// the 3 + 3 halo rows of the next plane (this state's z+1, or the first
+// plane of the chunk's next occupied state) move to the other LDS buffer by
+// direct-to-LDS loads.  Wave w moves halo blocks w and w+4 (block k:
+// component k/2, the rows above (k even) or below the tile).  The tile's own
+// rows come from the register queue.  Requires nx * sizeof(T) to be a
+// multiple of 16 and three rows to fit two 1 KiB wave loads.
+template <typename T, int NF, int TZ>
+__global__ __launch_bounds__(kBlock) void k_density_fd_dma(
+    MeshK<T> m, const T *__restrict__ psi, const int *__restrict__ qn,
+    const double *__restrict__ occ, const int *__restrict__ order,
+    const Chunk *__restrict__ dchunks, const DTask *__restrict__ tasks, int TY,
+    T *__restrict__ part) {
+    constexpr int G = 3;
+    constexpr int VPL = 16 / (int)sizeof(T);   // values per lane of one 16-byte load
+    constexpr int EPI = 64 * VPL;              // values per wave load (1 KiB)
+    static_assert(kBlock == 256, "halo blocks are spread over four waves");
+    extern __shared__ double smem_d[];
+    const int nx = m.nx, ny = m.ny, nz = m.nz;
+    const long N = m.N, plane = (long)nx * ny;
+    const int W = nx + 2 * G, YR = TY + 2 * G;
+    T *const Yb = reinterpret_cast<T *>(smem_d);  // [2][4][TY+6][nx]
+    T *const Xb = Yb + 2 * 4 * YR * nx;           // [2][4][TY][nx+6]
+    const DTask tk = tasks[blockIdx.x];
+    const Chunk ch = dchunks[tk.chunk];
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int tx = tid % nx, tyl = tid / nx;
+    const int y0 = tk.tile * TY, y = y0 + tyl;
+    const bool inplane = tyl < TY;
+    const bool active = inplane && (y < ny);
+    const bool edge = y0 < G || y0 + TY + G > ny;
+    const int zb = tk.slab * TZ, ze = min(zb + TZ, nz);
+    const long col = (long)(active ? y : 0) * nx + tx;
+    const int nhb = (G * nx + EPI - 1) / EPI;
+    int hrow[2], hx[2];
+    bool hval[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int e = k * EPI + lane * VPL;
+        hval[k] = k < nhb && e < G * nx;
+        hrow[k] = e / nx;
+        hx[k] = e % nx;
+    }
+    auto issue = [&](const T *ps, int zz, int b) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int blk = wave + 4 * j;
+            const int c = blk >> 1, rr0 = (blk & 1) ? TY + G : 0;
+            T *const dst = Yb + ((b * 4 + c) * YR + rr0) * nx;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (hval[k]) {
+                    const int yy = y0 - G + rr0 + hrow[k];
+                    const int yi = yy < 0 ? -1 - yy : (yy < ny ? yy : ny - 1);
+                    dma16(ps + (long)c * N + (long)zz * plane + (long)yi * nx + hx[k],
+                          dst + k * EPI);
+                }
+            }
+        }
+    };
+    auto next_occupied = [&](int q) {
+        while (q < ch.count && occ[order[ch.start + q]] == 0.0) ++q;
+        return q;
+    };
+    T acc[TZ][NF];
+#pragma unroll
+    for (int a = 0; a < TZ; ++a)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) acc[a][f] = 0;
+    int q = next_occupied(0);
+    int b = 0;
+    if (q < ch.count) issue(psi + (long)order[ch.start + q] * 4 * N, zb, 0);
+    while (q < ch.count) {
+        const int s = order[ch.start + q];
+        const int qnext = next_occupied(q + 1);
+        const T *const psn =
+            qnext < ch.count ? psi + (long)order[ch.start + qnext] * 4 * N : nullptr;
+        const T o = (T)occ[s];
+        const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
+        T sgx[4], sgy[4], sgz[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sgx[c] = T(comp_sign_x(c, S));
+            sgy[c] = T(comp_sign_y(c, TS));
+            sgz[c] = T(comp_sign_z(c, S, P));
+        }
+        const T *ps = psi + (long)s * 4 * N;
+        T qz[4][7];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const int zz = zb - G + k;
+                const int zi = zz < 0 ? -1 - zz : (zz < nz ? zz : nz - 1);
+                const T val = ps[c * N + zi * plane + col];
+                qz[c][k] = !active || zz >= nz ? T(0) : (zz < 0 ? sgz[c] * val : val);
+            }
+#pragma unroll
+        for (int zl = 0; zl < TZ; ++zl) {
+            const int z = zb + zl;
+            if (z >= ze) break;
+            T *const Xc = Xb + b * 4 * TY * W;
+            T *const Yc = Yb + b * 4 * YR * nx;
+            if (inplane) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const T v = active ? qz[c][3] : T(0);
+                    T *xr = Xc + (c * TY + tyl) * W;
+                    xr[tx + G] = v;
+                    if (tx < G) xr[G - 1 - tx] = sgx[c] * v;
+                    if (tx >= nx - G) xr[tx + 2 * G] = T(0);
+                    Yc[(c * YR + tyl + G) * nx + tx] = v;
+                }
+            }
+            __syncthreads();
+            if (z + 1 < ze) issue(ps, z + 1, b ^ 1);
+            else if (psn) issue(psn, zb, b ^ 1);
+            const int zn = z + 4 < nz ? z + 4 : nz - 1;
+            T nxt[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) nxt[c] = ps[c * N + zn * plane + col];
+            if (active) {
+                T v[4], d[3][4];
+                const int rowi = tyl;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const T *row = Xc + (c * TY + rowi) * W + tx + G;
+                    T w[7];
+#pragma unroll
+                    for (int k = -3; k <= 3; ++k) w[k + 3] = row[k];
+                    d[0][c] = fd1_7(m, w);
+                    const T *colp = Yc + (c * YR + rowi + G) * nx + tx;
+#pragma unroll
+                    for (int k = -3; k <= 3; ++k) w[k + 3] = colp[k * nx];
+                    if (edge) {
+#pragma unroll
+                        for (int k = -3; k <= 3; ++k) {
+                            const int yy = y + k;
+                            w[k + 3] *= yy < 0 ? sgy[c] : (yy < ny ? T(1) : T(0));
+                        }
+                    }
+                    d[1][c] = fd1_7(m, w);
+                    d[2][c] = fd1_7(m, qz[c]);
+                    v[c] = qz[c][3];
+                }
+                T t[4];
+                acc[zl][0] += o * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+                T ta = 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    ta += d[a][0] * d[a][0] + d[a][1] * d[a][1] + d[a][2] * d[a][2] +
+                          d[a][3] * d[a][3];
+                acc[zl][1] += o * ta;
+                // nabla.J = sum eps_ijk Im(d_i psi^+ sigma_k d_j psi) (Spwf.f90:2206-2243)
+                T nj = 0;
+                sig(d[1], 2, t); nj += imd(d[0], t);
+                sig(d[2], 1, t); nj -= imd(d[0], t);
+                sig(d[1], 0, t); nj -= imd(d[2], t);
+                sig(d[0], 1, t); nj += imd(d[2], t);
+                sig(d[0], 2, t); nj -= imd(d[1], t);
+                sig(d[2], 0, t); nj += imd(d[1], t);
+                acc[zl][2] += o * nj;
+                if (NF == 9) {
+                    // j (Spwf.f90:2034) and s (Spwf.f90:2071)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) acc[zl][3 + a] += o * imd(v, d[a]);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        sig(v, a, t);
+                        acc[zl][6 + a] +=
+                            o * (v[0] * t[0] + v[1] * t[1] + v[2] * t[2] + v[3] * t[3]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) qz[c][k] = qz[c][k + 1];
+                qz[c][6] = (active && z + 4 < nz) ? nxt[c] : T(0);
+            }
+            b ^= 1;
+        }
+        q = qnext;
+    }
+    if (active) {
+#pragma unroll
+        for (int zl = 0; zl < TZ; ++zl) {
+            const int z = zb + zl;
+            if (z >= ze) break;
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                part[(long)(tk.chunk * NF + f) * N + z * plane + col] = acc[zl][f];
+        }
+    }
+}
+
+template <typename T>
+static bool density_dma_ok(const MeshK<T> &m, int TY) {
+    return (m.nx * sizeof(T)) % 16 == 0 && 3 * m.nx * sizeof(T) <= 2048 &&
+           TY * m.nx <= kBlock && m.ny > 3 && m.nz > 3;
+}
+
 template <typename T>
 void launch_density_fd(const MeshK<T> &m, const T *psi, const int *qn, const double *occ,
                        const int *order, const Chunk *dchunks, const DTask *tasks,
                        int ntasks, int TY, int trc, T *part, hipStream_t st) {
+    // (the time-odd variant keeps the two-barrier kernel: with its nine
+    // accumulators per plane the pipelined one holds a single wave per SIMD)
+    if (trc && density_dma_ok(m, TY)) {
+        const size_t dshm =
+            sizeof(T) * ((size_t)2 * 4 * (TY + 6) * m.nx + (size_t)2 * 4 * TY * (m.nx + 6));
+        hipLaunchKernelGGL((k_density_fd_dma<T, 3, kDensSlab>), dim3(ntasks), dim3(kBlock),
+                           dshm, st, m, psi, qn, occ, order, dchunks, tasks, TY, part);
+        return;
+    }
     const size_t shm = sizeof(T) * 4 * (size_t)(TY + 6) * (m.nx + 6);
-    constexpr int TZ = kDensSlab;
-#define MOCCA_DENS_LAUNCH(NF, HM)                                                          \
+#define MOCCA_DENS_LAUNCH(NF, TZ, HM)                                                      \
     hipLaunchKernelGGL((k_density_fd<T, NF, TZ, HM>), dim3(ntasks), dim3(kBlock), shm, st, m, \
                        psi, qn, occ, order, dchunks, tasks, TY, part)
     if (m.nx <= 32) {
-        if (trc) MOCCA_DENS_LAUNCH(3, 3); else MOCCA_DENS_LAUNCH(9, 3);
+        if (trc) MOCCA_DENS_LAUNCH(3, kDensSlab, 3); else MOCCA_DENS_LAUNCH(9, kDensSlab, 3);
     } else {
-        if (trc) MOCCA_DENS_LAUNCH(3, 6); else MOCCA_DENS_LAUNCH(9, 6);
+        if (trc) MOCCA_DENS_LAUNCH(3, kDensSlab, 6); else MOCCA_DENS_LAUNCH(9, kDensSlab, 6);
     }
 #undef MOCCA_DENS_LAUNCH
 }

@@ -127,32 +127,42 @@ def test_microbench_workload_vs_oracle():
     s.close()
 
 
-@pytest.mark.parametrize("m", [5, 25, 100, 150, 190, 250])
-def test_gram_schmidt_block_sizes_vs_householder_qr(m):
+@pytest.mark.parametrize("m,prec", [(5, mocca_amd.FP64), (25, mocca_amd.FP64),
+                                    (100, mocca_amd.FP64), (140, mocca_amd.FP64),
+                                    (150, mocca_amd.FP64), (150, mocca_amd.MIXED),
+                                    (190, mocca_amd.FP64), (250, mocca_amd.FP64)])
+def test_gram_schmidt_block_sizes_vs_householder_qr(m, prec):
     """Gram-Schmidt of one block of m states (energy order) through every
-    kernel family: MFMA Gram up to 128 states (one 16-row tile from 1 to 16),
-    MFMA apply up to 192, vector kernels and the global-memory Cholesky
-    beyond.  Modified Gram-Schmidt in
-    a given order equals the QR factorisation with a positive-diagonal R, so
-    the result is checked against numpy's Householder QR of the same
-    energy-sorted states (an independent algorithm), 1e-10."""
+    kernel family: MFMA Gram up to 128 states in one wave per K slice (one
+    16-row tile from 1 to 16), 129-160 states as three jobs (140: nine row
+    tiles, the second job half empty; 150: ten), MFMA apply up to 192, the
+    packed-LDS Cholesky up to 192, vector kernels and the global-memory
+    Cholesky beyond.  Modified Gram-Schmidt in a given order equals the QR
+    factorisation with a positive-diagonal R, so the result is checked
+    against numpy's Householder QR of the same energy-sorted states (an
+    independent algorithm): 1e-10 in FP64; in mixed precision (FP32 states,
+    FP64 Gram, Cholesky and accumulation) against the QR of the FP32-rounded
+    states, 1e-5."""
     n, dx = 8, 0.8
     N, dv = n ** 3, dx ** 3 * 8
     rng = np.random.default_rng(100 + m)
     psi = rng.standard_normal((m, 4, N)) * 0.1
+    if prec == mocca_amd.MIXED:
+        psi = psi.astype(np.float32).astype(np.float64)
+    tol, otol = (1e-10, 1e-11) if prec == mocca_amd.FP64 else (1e-5, 1e-5)
     qn = np.tile(np.array([-1, 1, 1, 1], np.int32), (m, 1))
     e = rng.permutation(np.linspace(-40.0, 10.0, m))
-    s = mocca_amd.Solve(n, n, n, dx, dv, m, time_reversal=False)
+    s = mocca_amd.Solve(n, n, n, dx, dv, m, time_reversal=False, precision=prec)
     s.set_states_meta(qn, np.ones(m))
     s.upload_states(psi)
     s.set_energies(e)
     s.gram_schmidt()
-    q = s.download_states().reshape(m, -1)
+    q = s.download_states().reshape(m, -1).astype(np.float64)
     s.close()
     order = np.argsort(e, kind="stable")
     A = psi.reshape(m, -1)[order] * np.sqrt(dv)
     Qh, R = np.linalg.qr(A.T)
     Qh = Qh * np.sign(np.diag(R))
     ref = Qh.T / np.sqrt(dv)
-    assert relerr(q[order], ref) < 1e-10
-    assert np.allclose(q @ q.T * dv, np.eye(m), atol=1e-11)
+    assert relerr(q[order], ref) < tol
+    assert np.allclose(q @ q.T * dv, np.eye(m), atol=otol)
