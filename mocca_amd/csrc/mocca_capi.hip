@@ -984,10 +984,11 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     const int nt16 = h->mpad / 16, ntile16 = nt16 * (nt16 + 1) / 2;
     {
         // one round of single-wave workgroups (1024 = 256 CUs x 4 SIMDs at
-        // one wave per SIMD), partial tiles under 15% of the states' bytes,
-        // at least 4 chunks of 16 values per wave
+        // one wave per SIMD; four per SIMD for blocks of up to 64 states,
+        // whose waves hold few registers), partial tiles under 15% of the
+        // states' bytes, at least 4 chunks of 16 values per wave
         const double data = (double)h->M * K4 * h->tsize;
-        long ns = std::max(1, 1024 / nblk);
+        long ns = std::max(1, (nt16 <= 4 ? 4096 : 1024) / nblk);
         ns = std::min<long>(ns, (long)(0.15 * data / ((double)nblk * ntile16 * 256 * 8)));
         ns = std::min<long>(ns, std::max<long>(1, K4 / 64));
         h->nsplit_mfma = (int)std::max<long>(1, ns);
