@@ -1314,49 +1314,54 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
         }
 }
 
-// Q = Psi_sorted T on the FP64 matrix cores, for blocks of 64..128 states,
-// as a GEMM [K4 x m] = [K4 x m] [m x m] tiled for LDS: a workgroup (4 waves)
-// owns 256 consecutive values of K and all m outputs; the sorted states and
-// T pass through LDS in chunks of 16 states, double-buffered (chunk c+1 is
-// loaded into registers while the MFMAs of chunk c run).
+// Q = Psi_sorted T on the FP64 matrix cores, for blocks of up to 192 states,
+// as a GEMM [K4 x m] = [K4 x m] [m x m] tiled for LDS: a workgroup of NW
+// waves owns 32 NW consecutive values of K and all m outputs; the sorted
+// states and T pass through LDS in chunks of 16 states, double-buffered
+// (chunk c+1 is loaded into registers while the MFMAs of chunk c run).
 //   D[16 l][16 e] += A[16 l][4 k] B[4 k][16 e]: A = T^T (lane L holds
 //   T[k0 + L/16][l0 + L%16]), B = the states (lane L holds psi_sorted
 //   [k0 + L/16][e0 + L%16]); result lane L, register r: output state
 //   l0 + L/16 + 4r at e0 + L%16, so 16 lanes store 16 consecutive values.
 // T is upper triangular (T[k][l] = 0 for k > l): chunk c skips the l tiles
-// below c.  Each wave accumulates kApplyEW/4 values of K x NT*16 outputs.
-constexpr int kApplyEW = 128;             // K values per workgroup
-constexpr int kApplyNE = kApplyEW / 64;   // 16-value K tiles per wave
-constexpr int kApplyPS = kApplyEW + 8;    // padded LDS row of the states
-template <int NT>
+// below c.  Each wave accumulates 32 values of K x NT*16 outputs.  Every
+// workgroup reads all of T, so wider workgroups (NW = 8) halve that traffic
+// for the larger blocks.
+constexpr int kApplyNE = 2;               // 16-value K tiles per wave
+template <int NW>
+constexpr int apply_ew() { return 16 * kApplyNE * NW; }   // K values per workgroup
+template <int NT, int NW>
 constexpr size_t gs_apply_mfma_lds() {
-    return sizeof(double) * 2 * 16 * (kApplyPS + NT * 16 + 8);
+    return sizeof(double) * 2 * 16 * (apply_ew<NW>() + 8 + NT * 16 + 8);
 }
-template <typename T, int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 8 ? 2 : 1))) void k_gs_apply_mfma(const T *__restrict__ src,
+template <typename T, int NT, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
+    NW == 4 && NT <= 8 ? 2 : (NW == 8 ? 2 : 1)))) void k_gs_apply_mfma(const T *__restrict__ src,
                                                        T *__restrict__ dst, long K4,
                                                        const int *__restrict__ perm,
                                                        const int *__restrict__ blk_m, int max_m,
                                                        const double *__restrict__ Tm) {
+    constexpr int NTH = 64 * NW, EW = apply_ew<NW>(), PS = EW + 8;
     constexpr int LT = NT * 16, PT = LT + 8;
-    extern __shared__ double lds[];   // psi [2][16][kApplyPS], T [2][16][PT]
+    constexpr int NTT = (16 * LT + NTH - 1) / NTH;   // T values staged per thread
+    extern __shared__ double lds[];   // psi [2][16][PS], T [2][16][PT]
     const int b = blockIdx.y, tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
     const int m = blk_m[b];
-    const long eb = (long)blockIdx.x * kApplyEW;
+    const long eb = (long)blockIdx.x * EW;
     const double *Tb = Tm + (long)b * max_m * max_m;
     const int *pb = perm + (long)b * max_m;
     const int kr = lane / 16, col = lane % 16;
     const int nchunk = (m + 15) / 16;
-    // staging: thread tid moves K value eb + tid % kApplyEW of the chunk rows
-    // tid / kApplyEW + RS j
-    constexpr int RS = 256 / kApplyEW, RPT = 16 / RS;
-    const int my_col = tid % kApplyEW, my_row = tid / kApplyEW;
+    // staging: thread tid moves K value eb + tid % EW of the chunk rows
+    // tid / EW + RS j
+    constexpr int RS = NTH / EW, RPT = 16 / RS;
+    const int my_col = tid % EW, my_row = tid / EW;
     const long my_e = eb + my_col;
     const bool my_in = my_e < K4;
     const long my_es = my_in ? my_e : 0;
     // global -> registers (addresses clamped in range; masked when stored)
     T ps[RPT];
-    double tt[NT];
+    double tt[NTT];
     auto gload = [&](int c) {
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
@@ -1364,25 +1369,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 8 ? 2
             ps[j] = src[(long)pb[k < m ? k : 0] * K4 + my_es];
         }
 #pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            const int idx = tid + q * 256, r = idx / LT, l = idx % LT, k = c * 16 + r;
-            const bool in = k < m && l < m;
+        for (int q = 0; q < NTT; ++q) {
+            const int idx = tid + q * NTH, r = idx / LT, l = idx % LT, k = c * 16 + r;
+            const bool in = idx < 16 * LT && k < m && l < m;
             const double v = Tb[in ? (long)k * max_m + l : 0];
             tt[q] = in ? v : 0.0;
         }
     };
     auto lstore = [&](int c, int buf) {
-        double *P = lds + buf * 16 * kApplyPS;
-        double *Tl = lds + 2 * 16 * kApplyPS + buf * 16 * PT;
+        double *P = lds + buf * 16 * PS;
+        double *Tl = lds + 2 * 16 * PS + buf * 16 * PT;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int r = my_row + RS * j;
-            P[r * kApplyPS + my_col] = (my_in && c * 16 + r < m) ? (double)ps[j] : 0.0;
+            P[r * PS + my_col] = (my_in && c * 16 + r < m) ? (double)ps[j] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            const int idx = tid + q * 256;
-            Tl[(idx / LT) * PT + idx % LT] = tt[q];
+        for (int q = 0; q < NTT; ++q) {
+            const int idx = tid + q * NTH;
+            if (idx < 16 * LT) Tl[(idx / LT) * PT + idx % LT] = tt[q];
         }
     };
     mfma_d4 acc[NT][kApplyNE];
@@ -1396,14 +1401,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 8 ? 2
     for (int c = 0; c < nchunk; ++c) {
         const int buf = c & 1;
         if (c + 1 < nchunk) gload(c + 1);
-        const double *P = lds + buf * 16 * kApplyPS + wv * (16 * kApplyNE) + col;
-        const double *Tl = lds + 2 * 16 * kApplyPS + buf * 16 * PT + col;
+        const double *P = lds + buf * 16 * PS + wv * (16 * kApplyNE) + col;
+        const double *Tl = lds + 2 * 16 * PS + buf * 16 * PT + col;
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int row = 4 * kk + kr;
             double bv[kApplyNE];
 #pragma unroll
-            for (int E = 0; E < kApplyNE; ++E) bv[E] = P[row * kApplyPS + 16 * E];
+            for (int E = 0; E < kApplyNE; ++E) bv[E] = P[row * PS + 16 * E];
 #pragma unroll
             for (int L = 0; L < NT; ++L) {
                 if (L < c) continue;
@@ -1435,24 +1440,25 @@ template <typename T>
 void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
                           int nblk, int max_m, int mpad_max, const double *Tm,
                           hipStream_t st) {
-    const dim3 g((K4 + kApplyEW - 1) / kApplyEW, nblk), blk(256);
     // (the double-buffered LDS tiles exceed the default 64 KiB)
-#define MOCCA_APPLY_NT(K)                                                                   \
+#define MOCCA_APPLY_NT(K, W)                                                                \
     case K: {                                                                               \
-        constexpr size_t shm = gs_apply_mfma_lds<K>();                                      \
+        constexpr size_t shm = gs_apply_mfma_lds<K, W>();                                   \
         static const bool lds_set = hipFuncSetAttribute(                                   \
-            (const void *)k_gs_apply_mfma<T, K>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-            (int)shm) == hipSuccess;                                                        \
+            (const void *)k_gs_apply_mfma<T, K, W>,                                         \
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess;            \
         (void)lds_set;                                                                      \
-        hipLaunchKernelGGL((k_gs_apply_mfma<T, K>), g, blk, shm, st, src, dst, K4, perm,   \
-                           blk_m, max_m, Tm);                                               \
+        const dim3 g((K4 + apply_ew<W>() - 1) / apply_ew<W>(), nblk);                       \
+        hipLaunchKernelGGL((k_gs_apply_mfma<T, K, W>), g, dim3(64 * W), shm, st, src, dst,  \
+                           K4, perm, blk_m, max_m, Tm);                                     \
         break;                                                                              \
     }
+    // eight waves from 5 row tiles on; 11-12 row tiles do not fit two waves
+    // per SIMD, and keep four
     switch (mpad_max / 16) {
-        MOCCA_APPLY_NT(1) MOCCA_APPLY_NT(2) MOCCA_APPLY_NT(3)
-        MOCCA_APPLY_NT(4) MOCCA_APPLY_NT(5) MOCCA_APPLY_NT(6) MOCCA_APPLY_NT(7)
-        MOCCA_APPLY_NT(8) MOCCA_APPLY_NT(9) MOCCA_APPLY_NT(10) MOCCA_APPLY_NT(11)
-        MOCCA_APPLY_NT(12)
+        MOCCA_APPLY_NT(1, 4) MOCCA_APPLY_NT(2, 4) MOCCA_APPLY_NT(3, 4) MOCCA_APPLY_NT(4, 4)
+        MOCCA_APPLY_NT(5, 8) MOCCA_APPLY_NT(6, 8) MOCCA_APPLY_NT(7, 8) MOCCA_APPLY_NT(8, 8)
+        MOCCA_APPLY_NT(9, 8) MOCCA_APPLY_NT(10, 8) MOCCA_APPLY_NT(11, 4) MOCCA_APPLY_NT(12, 4)
         default: return;
     }
 #undef MOCCA_APPLY_NT
