@@ -166,3 +166,59 @@ def test_gram_schmidt_block_sizes_vs_householder_qr(m, prec):
     ref = Qh.T / np.sqrt(dv)
     assert relerr(q[order], ref) < tol
     assert np.allclose(q @ q.T * dv, np.eye(m), atol=otol)
+
+
+@pytest.mark.parametrize("nx,ny,nz,trc", [(16, 40, 9, True), (32, 20, 13, True),
+                                          (15, 21, 10, False), (24, 18, 7, False)])
+@pytest.mark.parametrize("prec,tol", [(mocca_amd.FP64, 1e-11), (mocca_amd.MIXED, 2e-5)])
+def test_density_and_step_multitile_vs_oracle(nx, ny, nz, trc, prec, tol):
+    """Densities and one GradDesc on FD meshes with several y tiles (the last
+    one partial) and z extents that leave short slabs: the pipelined density
+    kernel (time reversal, even nx) and the two-barrier one (odd nx, time-odd
+    fields), the pipelined FP64 step (even nx) and the register-queue step,
+    against the oracle.  FP64 1e-11 (states after the step and Gram-Schmidt
+    1e-10), mixed 2e-5."""
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + nz)
+    dx, M = 0.8, 12
+    N = nx * ny * nz
+    qn = workloads.balanced_qn(M, time_reversal=trc)
+    psi = workloads.plane_wave_states(rng, nx, ny, nz, dx, M)
+    dv = dx ** 3 * 8
+    psi /= np.sqrt((psi.reshape(M, -1) ** 2).sum(1) * dv)[:, None, None]
+    occ = rng.uniform(0.5, 1.0, M)
+    occ[3] = 0.0   # an empty state is skipped
+    F = workloads.woods_saxon_fields(nx, ny, nz, dx, time_odd=not trc)
+    kin = np.array([workloads.HBM_HALF, workloads.HBM_HALF])
+    s = mocca_amd.Solve(nx, ny, nz, dx, dv, M, time_reversal=trc, precision=prec)
+    s.set_states_meta(qn, occ)
+    s.upload_states(psi)
+    if trc:
+        s.set_potentials(F["U"], F["B"], F["NablaB"], F["W"], F["DivW"], kin)
+    else:
+        s.set_potentials(F["U"], F["B"], F["NablaB"], F["W"], F["DivW"], kin,
+                         S=F["S"], A=F["A"], DivA=F["DivA"])
+    mesh = Mesh(nx, ny, nz, dx, False)
+    der = np.zeros((M, 3, 4, N))
+    lap = np.zeros((M, 4, N))
+    for i, q in enumerate(qn):
+        der[i], lap[i] = mesh.derive_spinor(psi[i], q[1], q[2], q[3])
+    d = s.compute_density()
+    rho, tau, nj, vj, vs = density(psi, der, qn, occ, trc, N)
+    refs = [("rho", rho), ("tau", tau), ("nablaJ", nj)]
+    if not trc:
+        refs += [("vecj", vj), ("vecs", vs)]
+    for k, ref in refs:
+        assert relerr(d[k], ref) < tol, k
+    if trc:   # no time-odd fields: zeros for the oracle
+        F = dict(F, S=np.zeros_like(F["NablaB"]), A=np.zeros_like(F["NablaB"]),
+                 DivA=np.zeros_like(F["U"]))
+    Ff = {k: np.ascontiguousarray(v).reshape(-1) for k, v in F.items()}
+    dt, mu = 0.010 / 6.58211928, 0.5
+    s.grad_desc(dt, mu)
+    out = s.download_states()
+    e, _ = s.energies()
+    ref_psi, _, ref_e, _ = graddesc(psi, der, lap, np.zeros_like(psi), qn, trc, kin, Ff, dt,
+                                    mu, dv)
+    assert relerr(e, ref_e) < tol
+    assert relerr(out, ref_psi.reshape(M, 4, N)) < (1e-10 if prec == mocca_amd.FP64 else tol)
+    s.close()
