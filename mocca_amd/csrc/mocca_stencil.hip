@@ -704,16 +704,33 @@ __global__ __launch_bounds__(kBlock) void k_density_fd(
 #pragma unroll
         for (int f = 0; f < NF; ++f) acc[a][f] = 0;
     const int nhalo = 4 * 6 * nx;
+    // halo element r of this thread (fixed for the task): source offset in
+    // the state less the plane, LDS offset, and component | yy < 0 |
+    // yy >= ny | left ghost band | right ghost band | x << 8
+    int hg[HMAX], hl[HMAX], hf[HMAX];
+#pragma unroll
+    for (int r = 0; r < HMAX; ++r) {
+        const int idx = tid + r * kBlock;
+        const int c = idx / (6 * nx), rem = idx % (6 * nx);
+        const int hr = rem / nx, xx = rem % nx;
+        const int rr = hr < 3 ? hr : TY + hr;
+        const int yy = y0 - 3 + rr;
+        const int yi = yy < 0 ? -1 - yy : (yy < ny ? yy : ny - 1);
+        const bool in = idx < nhalo;
+        hg[r] = in ? c * (int)N + yi * nx + xx : 0;
+        hl[r] = in ? (c * rows + rr) * W + xx + G : -1;
+        hf[r] = (in ? c : 0) | (yy < 0 ? 4 : 0) | (yy >= ny ? 8 : 0) | (xx < G ? 16 : 0) |
+                (xx >= nx - G ? 32 : 0) | (xx << 8);
+    }
     for (int q = 0; q < ch.count; ++q) {
         const int s = order[ch.start + q];
         if (occ[s] == 0.0) continue;
         const T o = (T)occ[s];
         const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
-        T sgx[4], sgy[4], sgz[4];
+        T sgx[4], sgz[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             sgx[c] = T(comp_sign_x(c, S));
-            sgy[c] = T(comp_sign_y(c, TS));
             sgz[c] = T(comp_sign_z(c, S, P));
         }
         const T *ps = psi + (long)s * 4 * N;
@@ -733,15 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_density_fd(
             if (z >= nz) break;
             T hv[HMAX];
 #pragma unroll
-            for (int r = 0; r < HMAX; ++r) {
-                const int idx = tid + r * kBlock;
-                const int c = idx / (6 * nx), rem = idx % (6 * nx);
-                const int hr = rem / nx, xx = rem % nx;
-                const int yy = y0 - 3 + (hr < 3 ? hr : TY + hr);
-                const int yi = yy < 0 ? -1 - yy : (yy < ny ? yy : ny - 1);
-                const int cc = idx < nhalo ? c : 0;
-                hv[r] = ps[cc * N + z * plane + yi * nx + xx];
-            }
+            for (int r = 0; r < HMAX; ++r) hv[r] = ps[(long)z * plane + hg[r]];
             const int zn = z + 4 < nz ? z + 4 : nz - 1;
             T nxt[4];
 #pragma unroll
@@ -758,17 +767,14 @@ __global__ __launch_bounds__(kBlock) void k_density_fd(
             }
 #pragma unroll
             for (int r = 0; r < HMAX; ++r) {
-                const int idx = tid + r * kBlock;
-                if (idx < nhalo) {
-                    const int c = idx / (6 * nx), rem = idx % (6 * nx);
-                    const int hr = rem / nx, xx = rem % nx;
-                    const int rr = hr < 3 ? hr : TY + hr;
-                    const int yy = y0 - 3 + rr;
-                    const T v = yy < 0 ? sgy[c] * hv[r] : (yy < ny ? hv[r] : T(0));
-                    T *row = sp + (c * rows + rr) * W;
-                    row[xx + G] = v;
-                    if (xx < G) row[G - 1 - xx] = sgx[c] * v;
-                    if (xx >= nx - G) row[xx + 2 * G] = T(0);
+                if (hl[r] >= 0) {
+                    const int f = hf[r], c = f & 3, xx = f >> 8;
+                    const T sy = T(comp_sign_y(c, TS)), sx = T(comp_sign_x(c, S));
+                    const T v = (f & 4) ? sy * hv[r] : ((f & 8) ? T(0) : hv[r]);
+                    T *e = sp + hl[r];
+                    e[0] = v;
+                    if (f & 16) e[-1 - 2 * xx] = sx * v;
+                    if (f & 32) e[G] = T(0);
                 }
             }
             __syncthreads();
