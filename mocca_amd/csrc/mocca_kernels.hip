@@ -5,6 +5,7 @@
 #include "mocca_kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace mocca {
 
@@ -1327,12 +1328,23 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
 // below c.  Each wave accumulates 32 values of K x NT*16 outputs.  Every
 // workgroup reads all of T, so wider workgroups (NW = 8) halve that traffic
 // for the larger blocks.
+// FP32 states (mixed precision) take the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32, twice the FP64 rate) with T rounded to FP32 and
+// FP32 accumulation; their results are stored in FP32 anyway.  Its result
+// lane l, register r holds D[4 (l/16) + r][l%16].
 constexpr int kApplyNE = 2;               // 16-value K tiles per wave
 template <int NW>
 constexpr int apply_ew() { return 16 * kApplyNE * NW; }   // K values per workgroup
-template <int NT, int NW>
+template <typename T, int NT, int NW>
 constexpr size_t gs_apply_mfma_lds() {
-    return sizeof(double) * 2 * 16 * (apply_ew<NW>() + 8 + NT * 16 + 8);
+    return sizeof(T) * 2 * 16 * (apply_ew<NW>() + 8 + NT * 16 + 8);
+}
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mfma_d4 mma_apply(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mfma_f4 mma_apply(float a, float b, mfma_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 template <typename T, int NT, int NW>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
@@ -1344,7 +1356,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
     constexpr int NTH = 64 * NW, EW = apply_ew<NW>(), PS = EW + 8;
     constexpr int LT = NT * 16, PT = LT + 8;
     constexpr int NTT = (16 * LT + NTH - 1) / NTH;   // T values staged per thread
-    extern __shared__ double lds[];   // psi [2][16][PS], T [2][16][PT]
+    typedef std::conditional_t<sizeof(T) == 8, mfma_d4, mfma_f4> Acc;
+    extern __shared__ double lds_raw[];
+    T *const lds = reinterpret_cast<T *>(lds_raw);   // psi [2][16][PS], T [2][16][PT]
     const int b = blockIdx.y, tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
     const int m = blk_m[b];
     const long eb = (long)blockIdx.x * EW;
@@ -1361,7 +1375,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
     const long my_es = my_in ? my_e : 0;
     // global -> registers (addresses clamped in range; masked when stored)
     T ps[RPT];
-    double tt[NTT];
+    T tt[NTT];
     auto gload = [&](int c) {
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
@@ -1373,16 +1387,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
             const int idx = tid + q * NTH, r = idx / LT, l = idx % LT, k = c * 16 + r;
             const bool in = idx < 16 * LT && k < m && l < m;
             const double v = Tb[in ? (long)k * max_m + l : 0];
-            tt[q] = in ? v : 0.0;
+            tt[q] = in ? (T)v : T(0);
         }
     };
     auto lstore = [&](int c, int buf) {
-        double *P = lds + buf * 16 * PS;
-        double *Tl = lds + 2 * 16 * PS + buf * 16 * PT;
+        T *P = lds + buf * 16 * PS;
+        T *Tl = lds + 2 * 16 * PS + buf * 16 * PT;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const int r = my_row + RS * j;
-            P[r * PS + my_col] = (my_in && c * 16 + r < m) ? (double)ps[j] : 0.0;
+            P[r * PS + my_col] = (my_in && c * 16 + r < m) ? ps[j] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < NTT; ++q) {
@@ -1390,31 +1404,31 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
             if (idx < 16 * LT) Tl[(idx / LT) * PT + idx % LT] = tt[q];
         }
     };
-    mfma_d4 acc[NT][kApplyNE];
+    Acc acc[NT][kApplyNE];
 #pragma unroll
     for (int L = 0; L < NT; ++L)
 #pragma unroll
-        for (int E = 0; E < kApplyNE; ++E) acc[L][E] = mfma_d4{0, 0, 0, 0};
+        for (int E = 0; E < kApplyNE; ++E) acc[L][E] = Acc{0, 0, 0, 0};
     gload(0);
     lstore(0, 0);
     __syncthreads();
     for (int c = 0; c < nchunk; ++c) {
         const int buf = c & 1;
         if (c + 1 < nchunk) gload(c + 1);
-        const double *P = lds + buf * 16 * PS + wv * (16 * kApplyNE) + col;
-        const double *Tl = lds + 2 * 16 * PS + buf * 16 * PT + col;
+        const T *P = lds + buf * 16 * PS + wv * (16 * kApplyNE) + col;
+        const T *Tl = lds + 2 * 16 * PS + buf * 16 * PT + col;
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int row = 4 * kk + kr;
-            double bv[kApplyNE];
+            T bv[kApplyNE];
 #pragma unroll
             for (int E = 0; E < kApplyNE; ++E) bv[E] = P[row * PS + 16 * E];
 #pragma unroll
             for (int L = 0; L < NT; ++L) {
                 if (L < c) continue;
-                const double av = Tl[row * PT + 16 * L];
+                const T av = Tl[row * PT + 16 * L];
 #pragma unroll
-                for (int E = 0; E < kApplyNE; ++E) acc[L][E] = mfma16x16x4(av, bv[E], acc[L][E]);
+                for (int E = 0; E < kApplyNE; ++E) acc[L][E] = mma_apply(av, bv[E], acc[L][E]);
             }
         }
         if (c + 1 < nchunk) lstore(c + 1, buf ^ 1);
@@ -1424,7 +1438,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
     for (int L = 0; L < NT; ++L)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int l = 16 * L + kr + 4 * r;
+            const int l = sizeof(T) == 8 ? 16 * L + kr + 4 * r : 16 * L + 4 * kr + r;
             if (l < m) {
                 T *out = dst + (long)pb[l] * K4;
 #pragma unroll
@@ -1443,7 +1457,7 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
     // (the double-buffered LDS tiles exceed the default 64 KiB)
 #define MOCCA_APPLY_NT(K, W)                                                                \
     case K: {                                                                               \
-        constexpr size_t shm = gs_apply_mfma_lds<K, W>();                                   \
+        constexpr size_t shm = gs_apply_mfma_lds<T, K, W>();                                \
         static const bool lds_set = hipFuncSetAttribute(                                   \
             (const void *)k_gs_apply_mfma<T, K, W>,                                         \
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess;            \
