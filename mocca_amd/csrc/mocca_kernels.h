@@ -169,9 +169,10 @@ void launch_gram_reduce(const double *partial, const int *blk_m, int nblk,
                         int max_m, int mmax, int tile, int nsplit, double dv,
                         double *G, hipStream_t st);
 
-// Energy order, Cholesky and T = L^{-T} per block; in LDS when the largest
-// block fits in lds_limit bytes (cholesky_lds_limit()), else with its packed
-// lower triangle in LDS (up to 192 states), else in global memory.
+// Energy order, Cholesky and T = L^{-T} per block: in registers up to 63
+// states, in LDS when the largest block fits in lds_limit bytes
+// (cholesky_lds_limit()), else with its packed lower triangle in LDS (up to
+// 192 states), else in global memory.
 void launch_cholesky_ld(const double *G, const int *blk_states,
                         const int *blk_m, int nblk, int max_m, int mmax, const double *E,
                         double *Tm, int *perm, double *work, int *status,

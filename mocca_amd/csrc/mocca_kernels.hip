@@ -1149,6 +1149,136 @@ __global__ __launch_bounds__(1024) void k_cholesky_packed(const double *__restri
     }
 }
 
+// The same factorisation and inverse, operation for operation, for blocks
+// of up to kCholRegMax states with the lower triangle held in registers:
+// thread t owns the packed elements t and t + 1024 (EE slots).  Step k of
+// the factorisation reads column k (its raw values, published to LDS by
+// their owners at the end of step k-1) and updates the owned elements right
+// of it: one barrier per step, no LDS read-modify-write.  L then goes to
+// LDS (packed), and X = L^{-1} runs the same way on register-owned
+// R = I - ..., with row k of X published per step.  (Larger blocks keep
+// the in-LDS kernels: with more slots per thread this form is slower, C3
+// 0.165 -> 0.215 ms.)
+constexpr int kCholRegMax = 63;   // 63 * 64 / 2 <= 2 * 1024 elements
+
+template <int EE>
+__global__ __launch_bounds__(1024) void k_cholesky_reg(const double *__restrict__ G,
+                                                       const int *__restrict__ blk_states,
+                                                       const int *__restrict__ blk_m,
+                                                       int max_m, const double *__restrict__ E,
+                                                       double *__restrict__ Tm,
+                                                       int *__restrict__ perm,
+                                                       int *__restrict__ status) {
+    extern __shared__ double Lp[];          // packed L at the end
+    __shared__ int irank[kMaxBlock];   // sorted position -> block index
+    __shared__ double dinv[kMaxBlock];      // first the block's energies, then 1/L[i][i]
+    __shared__ double vec[2][kCholRegMax];
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int m = blk_m[b];
+    const int *st = blk_states + (long)b * max_m;
+    const double *g = G + (long)b * max_m * max_m;
+    double *Tb = Tm + (long)b * max_m * max_m;
+    int *pb = perm + (long)b * max_m;
+    for (int i = tid; i < m; i += nt) dinv[i] = E[st[i]];
+    __syncthreads();
+    for (int i = tid; i < m; i += nt) {
+        const double ei = dinv[i];
+        int r = 0;
+        for (int j = 0; j < m; ++j) {
+            const double ej = dinv[j];
+            if (ej < ei || (ej == ei && j < i)) ++r;
+        }
+        irank[r] = i;
+        pb[r] = st[i];
+    }
+    __syncthreads();
+    const int ne = m * (m + 1) / 2;
+    double a[EE];
+    int oi[EE], oj[EE];   // owned element (i, j), j <= i; oi = -1: none
+#pragma unroll
+    for (int r = 0; r < EE; ++r) {
+        const int e = tid + nt * r;
+        oi[r] = -1;
+        oj[r] = 0;
+        a[r] = 0.0;
+        if (e < ne) {
+            int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+            while (tri_row(i) > e) --i;
+            while (tri_row(i + 1) <= e) ++i;
+            const int j = e - tri_row(i);
+            oi[r] = i;
+            oj[r] = j;
+            a[r] = g[(long)irank[i] * max_m + irank[j]];
+        }
+    }
+    // column 0 for step 0
+#pragma unroll
+    for (int r = 0; r < EE; ++r)
+        if (oi[r] >= 0 && oj[r] == 0) vec[0][oi[r]] = a[r];
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        const double *col = vec[k & 1];
+        double *nxt = vec[(k + 1) & 1];
+        const double akk = col[k];
+        if (tid == 0 && !(akk > 0.0)) atomicOr(status, 1);
+        const double inv = 1.0 / (akk > 0.0 ? akk : 1.0);
+        const double d = sqrt(akk > 0 ? akk : 1.0);
+#pragma unroll
+        for (int r = 0; r < EE; ++r) {
+            const int i = oi[r], j = oj[r];
+            if (i < 0) continue;
+            if (j > k) {
+                const double aik = col[i] * inv;
+                a[r] -= aik * col[j];
+                if (j == k + 1) nxt[i] = a[r];
+            } else if (j == k) {
+                a[r] = i == k ? d : a[r] / d;   // column k of L
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < EE; ++r)
+        if (oi[r] >= 0) Lp[tri_row(oi[r]) + oj[r]] = a[r];
+    __syncthreads();
+    for (int i = tid; i < m; i += nt) dinv[i] = 1.0 / Lp[tri_row(i) + i];
+    __syncthreads();
+    // X = L^{-1}: R = I - (sum over finished rows); row k of X = R[k][:] / L[k][k]
+    // (published), then R[i][j] -= L[i][k] X[k][j] for i > k, j <= k.  Owned
+    // elements (i, j), j < i, hold R and end as X[i][j].
+#pragma unroll
+    for (int r = 0; r < EE; ++r) a[r] = 0.0;
+    for (int k = 0; k < m; ++k) {
+        double *xrow = vec[k & 1];
+        const double dk = dinv[k];
+#pragma unroll
+        for (int r = 0; r < EE; ++r)
+            if (oi[r] == k && oj[r] < k) {
+                a[r] *= dk;            // X[k][j]
+                xrow[oj[r]] = a[r];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < EE; ++r) {
+            const int i = oi[r], j = oj[r];
+            if (i > k && j <= k && j < i) {
+                const double xkj = j == k ? dk : xrow[j];
+                a[r] -= Lp[tri_row(i) + k] * xkj;
+            }
+        }
+    }
+    // T = L^{-T}: row j of T holds X[i][j] for i >= j
+#pragma unroll
+    for (int r = 0; r < EE; ++r) {
+        const int i = oi[r], j = oj[r];
+        if (i >= 0) Tb[(long)j * max_m + i] = i == j ? dinv[j] : a[r];
+    }
+    for (int q = tid; q < m * m; q += nt) {
+        const int j = q / m, i = q % m;
+        if (i < j) Tb[(long)j * max_m + i] = 0.0;
+    }
+}
+
 // Q = Psi_sorted T for each block, fused with the heavy-ball memory update
 // u = psi_new - u of GradDesc (ImaginaryTime.f90:497-499) when update_mode.
 // The T entries are the same for every lane: read from memory with
@@ -1572,10 +1702,45 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
 MOCCA_INST(double)
 MOCCA_INST(float)
 
+// Dynamic LDS available to k_cholesky_reg (the device limit less its static
+// arrays), set as the kernel's limit once.
+static size_t cholesky_reg_lds_limit() {
+    static const size_t lim = [] {
+        int dev = 0, l = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess)
+            return (size_t)0;
+        const size_t stat = kMaxBlock * (sizeof(int) + sizeof(double)) +
+                            2 * kCholRegMax * sizeof(double);
+        if ((size_t)l <= stat) return (size_t)0;
+        const size_t dyn = (size_t)l - stat;
+        const void *fns[] = {(const void *)k_cholesky_reg<1>, (const void *)k_cholesky_reg<2>};
+        for (const void *f : fns)
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
+                hipSuccess)
+                return (size_t)0;
+        return dyn;
+    }();
+    return lim;
+}
+
 void launch_cholesky_ld(const double *G, const int *blk_states, const int *blk_m,
                         int nblk, int max_m, int mmax, const double *E, double *Tm,
                         int *perm, double *work, int *status, size_t lds_limit,
                         hipStream_t st) {
+    // up to kCholRegMax states: the lower triangle in registers and L packed
+    // in LDS
+    const size_t lreg = sizeof(double) * (size_t)mmax * (mmax + 1) / 2;
+    if (mmax <= kCholRegMax && lreg <= cholesky_reg_lds_limit()) {
+        if (mmax * (mmax + 1) / 2 <= 1024)
+            hipLaunchKernelGGL(k_cholesky_reg<1>, dim3(nblk), dim3(1024), lreg, st, G,
+                               blk_states, blk_m, max_m, E, Tm, perm, status);
+        else
+            hipLaunchKernelGGL(k_cholesky_reg<2>, dim3(nblk), dim3(1024), lreg, st, G,
+                               blk_states, blk_m, max_m, E, Tm, perm, status);
+        return;
+    }
     // the whole block in LDS while it fits (up to ~137 states), else the
     // packed lower triangle (up to kCholPackedMax), else global memory
     const size_t lds = sizeof(double) * (size_t)mmax * (mmax + 1);

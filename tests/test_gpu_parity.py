@@ -128,6 +128,7 @@ def test_microbench_workload_vs_oracle():
 
 
 @pytest.mark.parametrize("m,prec", [(5, mocca_amd.FP64), (25, mocca_amd.FP64),
+                                    (60, mocca_amd.FP64),
                                     (100, mocca_amd.FP64), (140, mocca_amd.FP64),
                                     (150, mocca_amd.FP64), (150, mocca_amd.MIXED),
                                     (190, mocca_amd.FP64), (250, mocca_amd.FP64)])
@@ -136,8 +137,9 @@ def test_gram_schmidt_block_sizes_vs_householder_qr(m, prec):
     kernel family: MFMA Gram up to 128 states in one wave per K slice (one
     16-row tile from 1 to 16), 129-160 states as three jobs (140: nine row
     tiles, the second job half empty; 150: ten), MFMA apply up to 192, the
-    packed-LDS Cholesky up to 192, vector kernels and the global-memory
-    Cholesky beyond.  Modified Gram-Schmidt in a given order equals the QR
+    register Cholesky up to 63 states (one or two elements per thread: 25,
+    60), the LDS one up to ~137, the packed-LDS one up to 192, vector
+    kernels and the global-memory Cholesky beyond.  Modified Gram-Schmidt in a given order equals the QR
     factorisation with a positive-diagonal R, so the result is checked
     against numpy's Householder QR of the same energy-sorted states (an
     independent algorithm): 1e-10 in FP64; in mixed precision (FP32 states,
