@@ -863,7 +863,8 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     // Density tasks: (tile, slab, chunk); all tasks of one chunk on one XCD.
     std::vector<DTask> dtasks;
     {
-        const int nslab = (h->mesh.nz + kDensSlab - 1) / kDensSlab;
+        const int slab = density_slab(h->mesh.time_reversal, h->tsize);
+        const int nslab = (h->mesh.nz + slab - 1) / slab;
         std::vector<std::vector<DTask>> q(8);
         for (int c = 0; c < (int)dchunks.size(); ++c)
             for (int t = 0; t < h->ntile_y; ++t)

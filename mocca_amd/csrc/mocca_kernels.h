@@ -95,6 +95,12 @@ struct DTask {
     int tile, slab, chunk, pad;
 };
 constexpr int kDensSlab = 4;     // z planes per density task
+// FP32 states with the 6 time-odd fields: 2 planes per task (twice the
+// tasks, fewer accumulators per thread; in FP64 the longer slab is faster)
+constexpr int kDensSlabTodd32 = 2;
+inline int density_slab(int trc, int tsize) {
+    return !trc && tsize == 4 ? kDensSlabTodd32 : kDensSlab;
+}
 constexpr int kDensChunk = 16;   // states per density chunk
 
 // Rows per y-tile of the z-marching kernels (x spans the full line).

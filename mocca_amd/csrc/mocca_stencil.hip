@@ -1061,10 +1061,11 @@ void launch_density_fd(const MeshK<T> &m, const T *psi, const int *qn, const dou
 #define MOCCA_DENS_LAUNCH(NF, TZ, HM)                                                      \
     hipLaunchKernelGGL((k_density_fd<T, NF, TZ, HM>), dim3(ntasks), dim3(kBlock), shm, st, m, \
                        psi, qn, occ, order, dchunks, tasks, TY, part)
+    constexpr int TZ9 = sizeof(T) == 4 ? kDensSlabTodd32 : kDensSlab;
     if (m.nx <= 32) {
-        if (trc) MOCCA_DENS_LAUNCH(3, kDensSlab, 3); else MOCCA_DENS_LAUNCH(9, kDensSlab, 3);
+        if (trc) MOCCA_DENS_LAUNCH(3, kDensSlab, 3); else MOCCA_DENS_LAUNCH(9, TZ9, 3);
     } else {
-        if (trc) MOCCA_DENS_LAUNCH(3, kDensSlab, 6); else MOCCA_DENS_LAUNCH(9, kDensSlab, 6);
+        if (trc) MOCCA_DENS_LAUNCH(3, kDensSlab, 6); else MOCCA_DENS_LAUNCH(9, TZ9, 6);
     }
 #undef MOCCA_DENS_LAUNCH
 }
