@@ -276,10 +276,11 @@ def main():
     # roofline of the dominant kernel (hpsi_step): algorithmic bytes per launch
     # = read psi + read the states before the last step (the heavy-ball
     # memory) + write psi' for all states, plus the potentials of both
-    # isospins once (24 fields per isospin time-odd: U B NB(3) W(9) DW(3) S(3)
-    # A(3) DA; 17 with time reversal).
+    # isospins once (time-odd: U B NB(3) W DW(3) S(3) A(3) DA; the workload's
+    # W is antisymmetric with a zero diagonal, so 3 of its 9 components are
+    # independent: 18 fields per isospin, 11 with time reversal).
     s_bytes = 4 * N * 8
-    nfields = 17 if w["trc"] else 24
+    nfields = 11 if w["trc"] else 18
     alg_bytes = 3 * M * s_bytes + 2 * nfields * N * 8
     ms, cnt = prof["hpsi_step"]
     t_kernel = ms / cnt / 1e3

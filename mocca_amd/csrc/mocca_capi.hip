@@ -301,7 +301,7 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
                 launch_hpsi_fd_dma(m, dsrc, psi2, upd, (const T *)h->b.fields, h->b.qn,
                                    h->b.tasks_dma, h->ntasks_dma, h->TY, h->ntile_y, h->nzs,
                                    h->mesh.time_reversal, h->kin[0], h->kin[1], dth, mu,
-                                   h->b.eparts, h->st);
+                                   h->b.eparts, h->st, h->w_antisym);
             });
             goto stepped;
         }
@@ -1108,6 +1108,18 @@ int mocca_set_potentials(mocca_solve *h, const mocca_fields *f) {
         }
         if (int rc = up(h, F + q.off * N * ts, q.p, q.n * N)) return rc;
     }
+    // W[it][nu][mu]: antisymmetric with a zero diagonal, as without tensor
+    // terms, lets the step kernel move 3 of its 9 components
+    bool anti = true;
+    for (int it = 0; it < 2 && anti; ++it)
+        for (int nu = 0; nu < 3 && anti; ++nu)
+            for (int mu = nu; mu < 3 && anti; ++mu) {
+                const double *a = f->W + ((size_t)(it * 3 + nu) * 3 + mu) * N;
+                const double *b = f->W + ((size_t)(it * 3 + mu) * 3 + nu) * N;
+                for (size_t p = 0; p < N && anti; ++p)
+                    anti = mu == nu ? a[p] == 0.0 : a[p] == -b[p];
+            }
+    h->w_antisym = anti;
     h->kin[0] = f->kin[0];
     h->kin[1] = f->kin[1];
     h->fields_set = true;

@@ -167,6 +167,7 @@ struct mocca_solve {
     mocca::MeshK<double> md{};
     mocca::MeshK<float> mf{};
     bool meta_set = false, fields_set = false;
+    bool w_antisym = false;   // W[nu][mu] = -W[mu][nu], zero diagonal (no tensor terms)
     bool prev_valid = false;   // b.upd holds the states before the last GradDesc step
     // mocca_batch_run's captured iterations, valid while no buffer is
     // (re)allocated and no setting changes (graph_epoch counts both) and for

@@ -120,13 +120,14 @@ void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout
 // The FP64 GradDesc step (no stale derivatives) with the next plane's halo
 // rows and potentials moved to LDS by direct-to-LDS loads while the current
 // plane is computed; tasks carry tile + ntile_y * slab for nzs z slabs, and
-// eparts gets ntile_y * nzs slots per state.
+// eparts gets ntile_y * nzs slots per state.  w_antisym: the spin-orbit
+// field W is antisymmetric with a zero diagonal (3 components moved, not 9).
 bool hpsi_dma_ok(const MeshK<double> &m, int TY, int trc);
 void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_out,
                         const double *prev, const double *fields, const int *qn,
                         const Task *tasks, int ntasks, int TY, int ntile_y, int nzs, int trc,
                         double kin0, double kin1, double dth, double mu, double *eparts,
-                        hipStream_t st);
+                        hipStream_t st, bool w_antisym);
 
 template <typename T>
 void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,

@@ -257,6 +257,7 @@ int construct_impl(mocca_solve *h, int passes = 1) {
         if (rc) return rc;
     }
     h->fields_set = true;
+    h->w_antisym = true;   // W_{mu nu} = -eps_{k mu nu} w_k (k_potentials), mixed linearly
     return 0;
 }
 

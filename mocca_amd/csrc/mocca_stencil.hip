@@ -278,10 +278,19 @@ __device__ __forceinline__ void dma16(const T *g, T *lds_base) {
     __builtin_amdgcn_global_load_lds((const void *)g, (lds_raw_t *)lds_base, 16, 0, 0);
 }
 
+// Potential j of the pipelined step's LDS list: U, B, grad B (3), W (9:
+// W[nu][mu] at 5 + 3 nu + mu), div W (3), S (3), A (3), div A.  WA: W is
+// antisymmetric with a zero diagonal (no tensor terms: W_{mu nu} =
+// -eps_{k mu nu} w_k), so only W[0][1], W[0][2], W[1][2] are moved.
+template <bool WA>
 __device__ __forceinline__ int pot_slot(int j, int it) {
     if (j == 0) return kOffU + it;
     if (j == 1) return kOffB + it;
     if (j < 5) return kOffNB + 3 * it + (j - 2);
+    if constexpr (WA) {
+        if (j < 8) return kOffW + 9 * it + (j == 5 ? 1 : (j == 6 ? 2 : 5));
+        j += 6;
+    }
     if (j < 14) return kOffW + 9 * it + (j - 5);
     if (j < 17) return kOffDW + 3 * it + (j - 14);
     if (j < 20) return kOffS + 3 * it + (j - 17);
@@ -289,15 +298,15 @@ __device__ __forceinline__ int pot_slot(int j, int it) {
     return kOffDA + it;
 }
 
-template <typename T, bool TRC>
-constexpr int hpsi_dma_fields() { return TRC ? 17 : 24; }
+template <typename T, bool TRC, bool WA>
+constexpr int hpsi_dma_fields() { return (TRC ? 17 : 24) - (WA ? 6 : 0); }
 
-template <typename T, bool TRC>
+template <typename T, bool TRC, bool WA>
 size_t hpsi_dma_lds(int nx, int TY) {
     constexpr int EPI = 1024 / (int)sizeof(T);
     const int TNP = (TY * nx + EPI - 1) / EPI * EPI;
     return sizeof(T) * ((size_t)2 * 4 * TY * (nx + 8) + (size_t)2 * 4 * (TY + 8) * nx +
-                        (size_t)2 * hpsi_dma_fields<T, TRC>() * TNP);
+                        (size_t)2 * hpsi_dma_fields<T, TRC, WA>() * TNP);
 }
 
 // Two threads per point: lanes 0-31 of a wave own spinor components 0 and
@@ -308,7 +317,7 @@ size_t hpsi_dma_lds(int nx, int TY) {
 // halves the register queue per thread, so two waves share a SIMD.
 constexpr int kDmaThreads = 2 * kBlock;
 
-template <typename T, bool TRC>
+template <typename T, bool TRC, bool WA>
 __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     const T *__restrict__ prev, const T *__restrict__ F, const int *__restrict__ qn,
@@ -316,7 +325,8 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     T mu, double *__restrict__ eparts) {
     constexpr int G = 4;
     constexpr bool trc = TRC;
-    constexpr int NF = hpsi_dma_fields<T, TRC>();
+    constexpr int NF = hpsi_dma_fields<T, TRC, WA>();
+    constexpr int FD = WA ? 8 : 14;   // first div W slot; S, A, div A follow
     constexpr int NW = kDmaThreads / 64;       // waves
     constexpr int VPL = 16 / (int)sizeof(T);   // values per lane of one 16-byte load
     constexpr int EPI = 64 * VPL;              // values per wave load (1 KiB)
@@ -401,7 +411,7 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
             }
         }
         for (int j = wave; j < NF; j += NW) {
-            const T *src = F + (long)pot_slot(j, it) * N + (long)zz * plane + (long)y0 * nx +
+            const T *src = F + (long)pot_slot<WA>(j, it) * N + (long)zz * plane + (long)y0 * nx +
                            lane * VPL;
 #pragma unroll
             for (int ch = 0; ch < 2; ++ch) {
@@ -488,12 +498,20 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
                 const T lc = lx + ly + lz, vc = q[a][4];
                 kacc += (double)vc * (double)lc;
                 const T hown = -fb * lc - (nb0 * dxc + nb1 * dyc + nb2 * dzc) + fu * vc;
-                const T ex = pp[5 * TNP] * dxc + pp[6 * TNP] * dyc + pp[7 * TNP] * dzc +
-                             T(0.5) * pp[14 * TNP] * vc;
-                const T ey = pp[8 * TNP] * dxc + pp[9 * TNP] * dyc + pp[10 * TNP] * dzc +
-                             T(0.5) * pp[15 * TNP] * vc;
-                const T ez = pp[11 * TNP] * dxc + pp[12 * TNP] * dyc + pp[13 * TNP] * dzc +
-                             T(0.5) * pp[16 * TNP] * vc;
+                T ex, ey, ez;
+                if constexpr (WA) {
+                    const T w01 = pp[5 * TNP], w02 = pp[6 * TNP], w12 = pp[7 * TNP];
+                    ex = w01 * dyc + w02 * dzc + T(0.5) * pp[FD * TNP] * vc;
+                    ey = -w01 * dxc + w12 * dzc + T(0.5) * pp[(FD + 1) * TNP] * vc;
+                    ez = -w02 * dxc - w12 * dyc + T(0.5) * pp[(FD + 2) * TNP] * vc;
+                } else {
+                    ex = pp[5 * TNP] * dxc + pp[6 * TNP] * dyc + pp[7 * TNP] * dzc +
+                         T(0.5) * pp[FD * TNP] * vc;
+                    ey = pp[8 * TNP] * dxc + pp[9 * TNP] * dyc + pp[10 * TNP] * dzc +
+                         T(0.5) * pp[(FD + 1) * TNP] * vc;
+                    ez = pp[11 * TNP] * dxc + pp[12 * TNP] * dyc + pp[13 * TNP] * dzc +
+                         T(0.5) * pp[(FD + 2) * TNP] * vc;
+                }
                 if (a == 0) {
                     own0 += hown;
                     own1 -= sg * ez;
@@ -506,9 +524,10 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
                     send1 += sg * ey;
                 }
                 if (!trc) {
-                    const T s0 = pp[17 * TNP], s1 = pp[18 * TNP], s2 = pp[19 * TNP];
-                    const T aac = pp[20 * TNP] * dxc + pp[21 * TNP] * dyc + pp[22 * TNP] * dzc +
-                                  T(0.5) * pp[23 * TNP] * vc;
+                    const T s0 = pp[(FD + 3) * TNP], s1 = pp[(FD + 4) * TNP],
+                            s2 = pp[(FD + 5) * TNP];
+                    const T aac = pp[(FD + 6) * TNP] * dxc + pp[(FD + 7) * TNP] * dyc +
+                                  pp[(FD + 8) * TNP] * dzc + T(0.5) * pp[(FD + 9) * TNP] * vc;
                     if (a == 0) {
                         own0 += sg * s2 * vc;
                         send0 += s0 * vc;
@@ -612,8 +631,9 @@ static size_t hpsi_dma_lds_limit() {
 
 bool hpsi_dma_ok(const MeshK<double> &m, int TY, int trc) {
     if ((m.nx * sizeof(double)) % 16 != 0 || 4 * m.nx * sizeof(double) > 2048) return false;
-    const size_t dshm =
-        trc ? hpsi_dma_lds<double, true>(m.nx, TY) : hpsi_dma_lds<double, false>(m.nx, TY);
+    // (the general-W variant needs the most LDS)
+    const size_t dshm = trc ? hpsi_dma_lds<double, true, false>(m.nx, TY)
+                            : hpsi_dma_lds<double, false, false>(m.nx, TY);
     return dshm <= hpsi_dma_lds_limit();
 }
 
@@ -621,21 +641,24 @@ void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_o
                         const double *prev, const double *fields, const int *qn,
                         const Task *tasks, int ntasks, int TY, int ntile_y, int nzs, int trc,
                         double kin0, double kin1, double dth, double mu, double *eparts,
-                        hipStream_t st) {
-    const size_t dshm =
-        trc ? hpsi_dma_lds<double, true>(m.nx, TY) : hpsi_dma_lds<double, false>(m.nx, TY);
-#define MOCCA_HPSI_DMA(TR)                                                                  \
+                        hipStream_t st, bool w_antisym) {
+#define MOCCA_HPSI_DMA(TR, WA)                                                              \
     {                                                                                       \
+        const size_t dshm = hpsi_dma_lds<double, TR, WA>(m.nx, TY);                         \
         static const bool lds_set = hipFuncSetAttribute(                                   \
-            (const void *)k_hpsi_fd_dma<double, TR>,                                        \
+            (const void *)k_hpsi_fd_dma<double, TR, WA>,                                    \
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)hpsi_dma_lds_limit()) ==       \
             hipSuccess;                                                                     \
         (void)lds_set;                                                                      \
-        hipLaunchKernelGGL((k_hpsi_fd_dma<double, TR>), dim3(ntasks), dim3(kDmaThreads),    \
+        hipLaunchKernelGGL((k_hpsi_fd_dma<double, TR, WA>), dim3(ntasks), dim3(kDmaThreads), \
                            dshm, st, m, psi, psi_out, prev, fields, qn, tasks, TY, ntile_y, \
                            nzs, kin0, kin1, dth, mu, eparts);                               \
     }
-    if (trc) MOCCA_HPSI_DMA(true) else MOCCA_HPSI_DMA(false)
+    if (trc) {
+        if (w_antisym) MOCCA_HPSI_DMA(true, true) else MOCCA_HPSI_DMA(true, false)
+    } else {
+        if (w_antisym) MOCCA_HPSI_DMA(false, true) else MOCCA_HPSI_DMA(false, false)
+    }
 #undef MOCCA_HPSI_DMA
 }
 

@@ -224,3 +224,52 @@ def test_density_and_step_multitile_vs_oracle(nx, ny, nz, trc, prec, tol):
     assert relerr(e, ref_e) < tol
     assert relerr(out, ref_psi.reshape(M, 4, N)) < (1e-10 if prec == mocca_amd.FP64 else tol)
     s.close()
+
+
+@pytest.mark.parametrize("trc", [True, False])
+def test_step_general_w_vs_oracle(trc):
+    """The pipelined FP64 step moves only W[0][1], W[0][2], W[1][2] when the
+    uploaded spin-orbit field is antisymmetric with a zero diagonal (as
+    without tensor terms), all 9 components otherwise.  Both against the
+    oracle on a 32 x 20 x 13 mesh: the antisymmetric Woods-Saxon W, and the
+    same W plus a random symmetric part.  States after the step and
+    Gram-Schmidt 1e-10, energies 1e-11."""
+    nx, ny, nz, dx, M = 32, 20, 13, 0.8, 10
+    N = nx * ny * nz
+    dv = dx ** 3 * 8
+    rng = np.random.default_rng(77)
+    qn = workloads.balanced_qn(M, time_reversal=trc)
+    psi = workloads.plane_wave_states(rng, nx, ny, nz, dx, M)
+    psi /= np.sqrt((psi.reshape(M, -1) ** 2).sum(1) * dv)[:, None, None]
+    F0 = workloads.woods_saxon_fields(nx, ny, nz, dx, time_odd=not trc)
+    kin = np.array([workloads.HBM_HALF, workloads.HBM_HALF])
+    mesh = Mesh(nx, ny, nz, dx, False)
+    der = np.zeros((M, 3, 4, N))
+    lap = np.zeros((M, 4, N))
+    for i, q in enumerate(qn):
+        der[i], lap[i] = mesh.derive_spinor(psi[i], q[1], q[2], q[3])
+    dt, mu = 0.010 / 6.58211928, 0.5
+    for general in (False, True):
+        F = dict(F0)
+        if general:
+            sym = rng.standard_normal((2, 3, 3, N)) * 0.3
+            F["W"] = F0["W"] + sym + sym.transpose(0, 2, 1, 3)
+        s = mocca_amd.Solve(nx, ny, nz, dx, dv, M, time_reversal=trc)
+        s.set_states_meta(qn, np.ones(M))
+        s.upload_states(psi)
+        if trc:
+            s.set_potentials(F["U"], F["B"], F["NablaB"], F["W"], F["DivW"], kin)
+            F.update(S=np.zeros_like(F["NablaB"]), A=np.zeros_like(F["NablaB"]),
+                     DivA=np.zeros_like(F["U"]))
+        else:
+            s.set_potentials(F["U"], F["B"], F["NablaB"], F["W"], F["DivW"], kin,
+                             S=F["S"], A=F["A"], DivA=F["DivA"])
+        Ff = {k: np.ascontiguousarray(v).reshape(-1) for k, v in F.items()}
+        s.grad_desc(dt, mu)
+        out = s.download_states()
+        e, _ = s.energies()
+        ref_psi, _, ref_e, _ = graddesc(psi, der, lap, np.zeros_like(psi), qn, trc, kin, Ff,
+                                        dt, mu, dv)
+        assert relerr(e, ref_e) < 1e-11, general
+        assert relerr(out, ref_psi.reshape(M, 4, N)) < 1e-10, general
+        s.close()
