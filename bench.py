@@ -311,7 +311,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(
-                "mocca::k_hpsi_fd_dma<double, %s>" % ("true" if w["trc"] else "false")),
+                "mocca::k_hpsi_fd_dma<double, %s, true>" % ("true" if w["trc"] else "false")),
             "kernel": "hpsi_step", "kernel_ms": t_kernel * 1e3,
             "alg_bytes_per_launch": alg_bytes,
         },
