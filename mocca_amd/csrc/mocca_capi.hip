@@ -312,7 +312,7 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
             launch_hpsi_fd<T>(m, dsrc, psi2, upd, nullptr, (const T *)h->b.fields, h->b.qn,
                               h->b.tasks, h->ntasks, h->TY, h->ntile_y,
                               h->mesh.time_reversal, T(h->kin[0]), T(h->kin[1]), T(dth),
-                              T(mu), kStep, h->b.eparts, h->st, vsrc);
+                              T(mu), kStep, h->b.eparts, h->st, vsrc, h->w_antisym);
         });
     } else {
         slots = h->ntiles;

@@ -111,12 +111,14 @@ inline int plane_tile_rows(int nx, int ny) {
 
 // Finite-difference hPsi (+ heavy-ball step) marching along z: register
 // queue of 9 planes per component, LDS tile of the current plane with y
-// halos.  Requires nx <= kBlock.
+// halos.  Requires nx <= kBlock.  wa: W antisymmetric with a zero diagonal
+// (3 of its 9 components read).
 template <typename T>
 void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
                     const T *fields, const int *qn, const Task *tasks, int ntasks,
                     int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
-                    int mode, double *eparts, hipStream_t st, const T *vsrc = nullptr);
+                    int mode, double *eparts, hipStream_t st, const T *vsrc = nullptr,
+                    bool wa = false);
 // The FP64 GradDesc step (no stale derivatives) with the next plane's halo
 // rows and potentials moved to LDS by direct-to-LDS loads while the current
 // plane is computed; tasks carry tile + ntile_y * slab for nzs z slabs, and

@@ -53,7 +53,7 @@ __device__ __forceinline__ void block_sum3_fd(double &a, double &b, double &c, d
 // STALE: the values come from vsrc instead of psi, whose stencils give the
 // derivatives (the first GradDesc after an AlternateStep, which changes the
 // values but not the stored derivatives, ImaginaryTime.f90:306-437).
-template <typename T, int HMAX, bool TRC, bool STALE>
+template <typename T, int HMAX, bool TRC, bool STALE, bool WA>
 __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
@@ -124,11 +124,19 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
         for (int c = 0; c < 4; ++c) nxt[c] = ps[c * N + zn * plane + col];
         const T fu = Fu[p], fb = Fb[p] + kin;
         const T nb0 = Fnb[p], nb1 = Fnb[N + p], nb2 = Fnb[2 * N + p];
+        // W[nu][mu]; WA: antisymmetric with a zero diagonal (no tensor
+        // terms), only W[0][1], W[0][2], W[1][2] are read
         T fw[3][3];
+        if constexpr (WA) {
+            fw[0][1] = Fw[1 * N + p];
+            fw[0][2] = Fw[2 * N + p];
+            fw[1][2] = Fw[5 * N + p];
+        } else {
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int mu_ = 0; mu_ < 3; ++mu_) fw[a][mu_] = Fw[(a * 3 + mu_) * N + p];
+                for (int mu_ = 0; mu_ < 3; ++mu_) fw[a][mu_] = Fw[(a * 3 + mu_) * N + p];
+        }
         const T hx = T(0.5) * Fdw[p], hy = T(0.5) * Fdw[N + p], hz = T(0.5) * Fdw[2 * N + p];
         T s0 = 0, s1 = 0, s2 = 0, a0 = 0, a1 = 0, a2 = 0, hda = 0;
         if (!trc) {
@@ -195,9 +203,16 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
                 const T lc = lx + ly + lz, vc = val[c];
                 kacc += (double)vc * (double)lc;
                 h[c] += -fb * lc - (nb0 * dxc + nb1 * dyc + nb2 * dzc) + fu * vc;
-                const T ex = fw[0][0] * dxc + fw[0][1] * dyc + fw[0][2] * dzc + hx * vc;
-                const T ey = fw[1][0] * dxc + fw[1][1] * dyc + fw[1][2] * dzc + hy * vc;
-                const T ez = fw[2][0] * dxc + fw[2][1] * dyc + fw[2][2] * dzc + hz * vc;
+                T ex, ey, ez;
+                if constexpr (WA) {
+                    ex = fw[0][1] * dyc + fw[0][2] * dzc + hx * vc;
+                    ey = -fw[0][1] * dxc + fw[1][2] * dzc + hy * vc;
+                    ez = -fw[0][2] * dxc - fw[1][2] * dyc + hz * vc;
+                } else {
+                    ex = fw[0][0] * dxc + fw[0][1] * dyc + fw[0][2] * dzc + hx * vc;
+                    ey = fw[1][0] * dxc + fw[1][1] * dyc + fw[1][2] * dzc + hy * vc;
+                    ez = fw[2][0] * dxc + fw[2][1] * dyc + fw[2][2] * dzc + hz * vc;
+                }
                 if (c == 0) { aw[0] += ez; aw[2] += ex; aw[3] += ey; }
                 if (c == 1) { aw[1] += ez; aw[2] -= ey; aw[3] += ex; }
                 if (c == 2) { aw[0] += ex; aw[1] -= ey; aw[2] -= ez; }
@@ -593,17 +608,22 @@ template <typename T>
 void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout,
                     const T *fields, const int *qn, const Task *tasks, int ntasks,
                     int TY, int ntile_y, int trc, T kin0, T kin1, T dth, T mu,
-                    int mode, double *eparts, hipStream_t st, const T *vsrc) {
+                    int mode, double *eparts, hipStream_t st, const T *vsrc, bool wa) {
     // (whole 16-byte lane loads inside a row; the halo rows of one component
     // in at most two 1 KiB loads; the LDS within the device's per-workgroup
     // limit less the kernel's static reduction array)
     const size_t shm = sizeof(T) * 4 * (size_t)(TY + 8) * (m.nx + 8);
 #define MOCCA_HPSI_LAUNCH(HM, TR, ST)                                                         \
-    hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, ST>), dim3(ntasks), dim3(kBlock), shm, st, m,   \
-                       psi, psi_out, upd, hout, fields, qn, tasks, TY, ntile_y, kin0, kin1,  \
-                       dth, mu, mode, eparts, vsrc)
+    if (wa)                                                                                   \
+        hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, ST, true>), dim3(ntasks), dim3(kBlock), shm, \
+                           st, m, psi, psi_out, upd, hout, fields, qn, tasks, TY, ntile_y,    \
+                           kin0, kin1, dth, mu, mode, eparts, vsrc);                          \
+    else                                                                                      \
+        hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, ST, false>), dim3(ntasks), dim3(kBlock),     \
+                           shm, st, m, psi, psi_out, upd, hout, fields, qn, tasks, TY,        \
+                           ntile_y, kin0, kin1, dth, mu, mode, eparts, vsrc)
 #define MOCCA_HPSI_TRC(HM, ST)                                                                \
-    if (trc) MOCCA_HPSI_LAUNCH(HM, true, ST); else MOCCA_HPSI_LAUNCH(HM, false, ST)
+    if (trc) { MOCCA_HPSI_LAUNCH(HM, true, ST); } else { MOCCA_HPSI_LAUNCH(HM, false, ST); }
     if (vsrc) {
         if (m.nx <= 32) { MOCCA_HPSI_TRC(4, true); } else { MOCCA_HPSI_TRC(8, true); }
     } else {
@@ -666,11 +686,11 @@ template void launch_hpsi_fd<double>(const MeshK<double> &, const double *, doub
                                      double *, double *, const double *, const int *,
                                      const Task *, int, int, int, int, double, double,
                                      double, double, int, double *, hipStream_t,
-                                     const double *);
+                                     const double *, bool);
 template void launch_hpsi_fd<float>(const MeshK<float> &, const float *, float *, float *,
                                     float *, const float *, const int *, const Task *, int,
                                     int, int, int, float, float, float, float, int,
-                                    double *, hipStream_t, const float *);
+                                    double *, hipStream_t, const float *, bool);
 
 
 // ---------------------------------------------------------------------------
