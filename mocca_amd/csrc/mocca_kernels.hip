@@ -1157,8 +1157,8 @@ __global__ __launch_bounds__(1024) void k_cholesky_packed(const double *__restri
 // of it: one barrier per step, no LDS read-modify-write.  L then goes to
 // LDS (packed), and X = L^{-1} runs the same way on register-owned
 // R = I - ..., with row k of X published per step.  (Larger blocks keep
-// the in-LDS kernels: with more slots per thread this form is slower, C3
-// 0.165 -> 0.215 ms.)
+// the in-LDS kernels: with 5-6 slots per thread, blocks of 100 states, this
+// form took 0.215 ms against 0.165.)
 constexpr int kCholRegMax = 63;   // 63 * 64 / 2 <= 2 * 1024 elements
 
 template <int EE>
