@@ -333,7 +333,7 @@ def main():
         tf32, pf32, r32, cg32 = full_iteration_run(wf, MIXED, k3, args.warmup, True)
         out["full_iteration"] = {
             "workload": "40x40x40 octant mesh, dx=0.8 fm, 400 states (200 n + 200 p), "
-                        "HO seeds, SLy4 + Coulomb (CG), HF filling N=126 Z=82, "
+                        "HO seeds, SLy4 + Coulomb (direct solve), HF filling N=126 Z=82 (no pairing), "
                         "MixingScheme 0; step = the whole Evolve iteration on the device "
                         "(GradDesc + fill + densities + mixing + moments + Coulomb + "
                         "potentials + energy)",

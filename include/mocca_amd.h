@@ -206,7 +206,12 @@ int mocca_scf_start(mocca_solve *h);
 
 /* niter iterations of the Evolve loop (Main.f90:255-328) on the device.
  * energies (may be NULL): [niter+1][47] in the layout of mocca_comp_energy,
- * row 0 for the states on entry, row k after iteration k. */
+ * row 0 for the states on entry, row k after iteration k.
+ * Iterations are numbered as Evolve numbers them (the number that
+ * TurnOffConstraints compares with a constraint's switch-off iteration):
+ * the handle counts the iterations run since mocca_scf_start, so a run split
+ * over several calls (of mocca_iterate, mocca_iterate_estimated or
+ * mocca_batch_run) continues the count where the previous call ended. */
 int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
                   double *energies);
 

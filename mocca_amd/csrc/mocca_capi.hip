@@ -523,14 +523,17 @@ int estimate_any(mocca_solve *h, int iteration, double dt, double mu, double *ou
 // AlternateStep (ImaginaryTime.f90:306-437): psi <- psi - multipole psi,
 // then Gramschmidt; the states before the step stay in alt_old for the
 // derivatives of the next GradDesc.
+int alternate_prepare(mocca_solve *h) {
+    Scf &s = h->scf;
+    if (s.alt_old) return 0;
+    return dalloc(h, &s.alt_old, (size_t)h->M * 4 * h->N * h->tsize);
+}
+
 template <typename T>
 int alternate_impl(mocca_solve *h) {
     Scf &s = h->scf;
     const long N = h->N;
-    if (!s.alt_old) {
-        int rc = dalloc(h, &s.alt_old, (size_t)h->M * 4 * N * h->tsize);
-        if (rc) return rc;
-    }
+    if (int rc = alternate_prepare(h)) return rc;
     int rc = run(h, "alternate", [&] {
         launch_alternate<T>(s.con, s.ncon, s.Y, s.calt, s.cut, N, s.mult, (const T *)h->b.psi,
                             (T *)h->b.psi2, h->b.qn, h->M, h->st);

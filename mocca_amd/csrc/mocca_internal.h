@@ -102,6 +102,7 @@ struct Scf {
     void *alt_old = nullptr;       // states before the last AlternateStep (T [M][4][N])
     bool alt_pending = false;      // the next GradDesc takes its derivatives from alt_old
     bool alt_next = false;         // Evolve's AlternateCheck for the next iteration
+    int iteration = 0;             // Evolve iterations run since mocca_scf_start
 };
 
 // The extra state of IterativeEstimation (maxspwf) and its records.
@@ -242,6 +243,9 @@ int run(mocca_solve *h, const char *name, F &&f) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return fail(h, MOCCA_ERR_HIP, "launch of %s failed: %s", name, hipGetErrorString(e));
+    if (const char *k = take_launch_failure())
+        return fail(h, MOCCA_ERR_HIP, "launch of %s: the dynamic LDS limit of %s could not be raised",
+                    name, k);
     return 0;
 }
 
@@ -262,8 +266,10 @@ int hfb_solve_any(mocca_solve *h, bool first);
 int hfb_energy_impl(mocca_solve *h, double *row);
 int hfb_check_status(mocca_solve *h);
 constexpr int kRowLen = 47;   // energy row: 43 of CompEnergy + pairing energy(2) + Fermi(2)
-// AlternateStep (constraints of type 2) on the device.
+// AlternateStep (constraints of type 2) on the device; alternate_prepare
+// allocates the buffer of the states before the step ahead of time.
 int alternate_any(mocca_solve *h);
+int alternate_prepare(mocca_solve *h);
 // IterativeEstimation: out = dt, momentum, maxE, relE, con, iterations.
 int estimate_any(mocca_solve *h, int iteration, double dt, double mu, double *out);
 constexpr double kHbar = 6.58211928;   // Force.f90:103

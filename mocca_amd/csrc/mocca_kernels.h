@@ -17,6 +17,15 @@ constexpr int kApplyL = 32;      // output states per workgroup in the apply
 constexpr int kApplyChunks = 1;  // chunks of kBlock values per apply workgroup
 constexpr int kMaxBlock = 512;   // largest (isospin, parity, signature) block
 
+// Raises the dynamic-LDS limit of kernel f to bytes.  A failure is recorded
+// under the kernel's name and reported by the next take_launch_failure()
+// (the C-ABI layer checks it after every launch), so no kernel is launched
+// with a smaller LDS allocation than it was written for.
+bool require_lds(const void *f, size_t bytes, const char *kernel);
+// The kernel whose LDS limit could not be raised since the last call, or
+// nullptr; clears the record.
+const char *take_launch_failure();
+
 // Mesh description passed by value to kernels.
 template <typename T>
 struct MeshK {

@@ -9,6 +9,23 @@
 
 namespace mocca {
 
+namespace {
+thread_local const char *g_launch_failure = nullptr;
+}
+
+bool require_lds(const void *f, size_t bytes, const char *kernel) {
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+        hipSuccess)
+        return true;
+    g_launch_failure = kernel;
+    return false;
+}
+
+const char *take_launch_failure() {
+    const char *k = g_launch_failure;
+    g_launch_failure = nullptr;
+    return k;
+}
 
 // ---------------------------------------------------------------------------
 // Sign tables of the EV8 octant (OptimizedDerivatives.f90:24-44,115-132,
@@ -309,10 +326,8 @@ void launch_lag_derivs(const MeshK<double> &md, const T *psi, const int *qn, int
     {                                                                                        \
         constexpr size_t shm = sizeof(double) * (2 * NP * NP + 4 * NP * 16);               \
         static const bool lds_set =                                                          \
-            hipFuncSetAttribute((const void *)k_lag_axis<T, NP, AX>,                         \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) ==     \
-            hipSuccess;                                                                      \
-        (void)lds_set;                                                                       \
+            require_lds((const void *)k_lag_axis<T, NP, AX>, shm, "k_lag_axis");           \
+        if (!lds_set) require_lds((const void *)k_lag_axis<T, NP, AX>, shm, "k_lag_axis"); \
         hipLaunchKernelGGL((k_lag_axis<T, NP, AX>), dim3(4 * M), dim3(256), shm, st, md, psi, \
                            qn, M, dsc);                                                      \
     }
@@ -1588,10 +1603,10 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
 #define MOCCA_APPLY_NT(K, W)                                                                \
     case K: {                                                                               \
         constexpr size_t shm = gs_apply_mfma_lds<T, K, W>();                                \
-        static const bool lds_set = hipFuncSetAttribute(                                   \
-            (const void *)k_gs_apply_mfma<T, K, W>,                                         \
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess;            \
-        (void)lds_set;                                                                      \
+        static const bool lds_set =                                                         \
+            require_lds((const void *)k_gs_apply_mfma<T, K, W>, shm, "k_gs_apply_mfma");   \
+        if (!lds_set)                                                                       \
+            require_lds((const void *)k_gs_apply_mfma<T, K, W>, shm, "k_gs_apply_mfma");   \
         const dim3 g((K4 + apply_ew<W>() - 1) / apply_ew<W>(), nblk);                       \
         hipLaunchKernelGGL((k_gs_apply_mfma<T, K, W>), g, dim3(64 * W), shm, st, src, dst,  \
                            K4, perm, blk_m, max_m, Tm);                                     \

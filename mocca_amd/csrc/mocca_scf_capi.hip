@@ -444,11 +444,13 @@ int iterate_impl(mocca_solve *h, int niter, int first_iteration, double dt, doub
                  double momentum, bool estimate, double *energies, double *params) {
     int rc;
     if ((rc = iter_begin(h, niter))) return rc;
-    for (int k = 1; k <= niter; ++k)
+    for (int k = 1; k <= niter; ++k) {
         if ((rc = iter_step(h, k, first_iteration + k - 1, dt, dth, momentum, estimate, params,
                             h->scf.erows + (size_t)(k - 1) * kRowLen,
                             h->scf.erows + (size_t)k * kRowLen)))
             return rc;
+        h->scf.iteration = first_iteration + k - 1;
+    }
     return iter_end(h, niter, energies);
 }
 
@@ -491,13 +493,18 @@ namespace {
 // One replayed (or, for a new pointer configuration, captured) iteration k
 // of a batch; rows: two slots [2][kRowLen] (previous, current).
 int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, double dth,
-               double momentum, double *rows, bool &ok) {
+               double momentum, double *rows, bool &ok, bool capture) {
     Scf &s = h->scf;
     void *cur[4];
     ptr_state(h, cur);
     IterGraph *g = nullptr;
     for (IterGraph &c : cache)
         if (std::equal(cur, cur + 4, c.pre)) g = &c;
+    if (!g && !capture) {
+        // a configuration not seen in the captured iterations: run eagerly
+        ok = false;
+        return 0;
+    }
     if (!g) {
         IterGraph n{};
         std::copy(cur, cur + 4, n.pre);
@@ -507,8 +514,10 @@ int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, 
             ok = false;
             return 0;
         }
-        const int rc = iter_step(h, k, k, dt, dth, momentum, false, nullptr, rows,
-                                 rows + kRowLen);
+        // (the iteration number only reaches the graph through
+        // TurnOffConstraints, which graph_ok excludes)
+        const int rc = iter_step(h, k, h->scf.iteration + 1, dt, dth, momentum, false, nullptr,
+                                 rows, rows + kRowLen);
         const hipError_t ec = hipStreamEndCapture(h->st, &graph);
         if (rc || ec != hipSuccess || !graph ||
             hipGraphInstantiate(&n.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
@@ -530,6 +539,7 @@ int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, 
         s.alt_old = g->post[3];
     }
     HIPCHK(h, hipGraphLaunch(g->exec, h->st));
+    ++s.iteration;
     double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
     HIPCHK(h, hipMemcpyAsync(prev, rows, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
                              h->st));
@@ -868,13 +878,15 @@ int mocca_scf_start(mocca_solve *h) {
     }
     h->scf.alt_next = alternating(h);
     h->scf.started = true;
+    h->scf.iteration = 0;
     return 0;
 }
 
 int mocca_iterate(mocca_solve *h, int niter, double dt_over_hbar, double momentum,
                   double *energies) {
-    return iterate_impl(h, niter, 1, dt_over_hbar * kHbar, dt_over_hbar, momentum, false,
-                        energies, nullptr);
+    if (!h) return MOCCA_ERR_ARG;
+    return iterate_impl(h, niter, h->scf.iteration + 1, dt_over_hbar * kHbar, dt_over_hbar,
+                        momentum, false, energies, nullptr);
 }
 
 int mocca_iterate_estimated(mocca_solve *h, int first_iteration, int niter, double dt,
@@ -907,65 +919,92 @@ int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
         if (int rc = iter_begin(solves[i], niter)) return failed(i, rc);
     // Iteration 1 of every solve runs eagerly (it sets up what later
     // iterations reuse); later ones are graph replays where graph_ok holds.
-    // The captured iterations stay with the solve for later calls.  The
-    // solves are split over a few host threads, each issuing its share in
-    // turn: one thread cannot issue the kernels of many small solves as fast
-    // as the device runs them.
+    // The captured iterations stay with the solve for later calls.
+    //
+    // Every allocation and every stream capture happens on this thread,
+    // before any worker starts: the buffers an iteration may need (the
+    // row slots of the replays, the states before an AlternateStep) are
+    // allocated here, and the iterations in which a solve's graphs are
+    // captured (one per configuration of the rotating buffers, so the first
+    // kWarm) are issued here too.  The workers then only replay captured
+    // graphs or run iterations eagerly, while no capture is open anywhere.
+    constexpr int kWarm = 4;     // 1 eager + one capture per buffer configuration (period <= 3)
     std::vector<char> use_graph(nsolves, 0);
-    auto run_range = [&](int i0, int i1, int &rc, int &bad) {
-        for (int k = 1; k <= niter && !rc; ++k) {
-            for (int i = i0; i < i1 && !rc; ++i) {
-                mocca_solve *h = solves[i];
-                if (k == 2 && niter >= 3 && graph_ok(h)) {
-                    if (h->graphs_epoch != h->graph_epoch || h->graphs_dth != dth[i] ||
-                        h->graphs_mu != mu[i])
-                        graphs_clear(h);
-                    if (!h->graph_rows &&
-                        hipMalloc(&h->graph_rows, 2 * kRowLen * sizeof(double)) != hipSuccess) {
-                        h->graph_rows = nullptr;
-                        (void)hipGetLastError();
-                    }
-                    // both slots start as row 1 (entries an iteration does not
-                    // write, such as the HF pairing terms, keep their values)
-                    if (h->graph_rows &&
-                        hipMemcpyAsync(h->graph_rows, h->scf.erows + kRowLen,
-                                       kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
-                                       h->st) == hipSuccess &&
-                        hipMemcpyAsync(h->graph_rows + kRowLen, h->scf.erows + kRowLen,
-                                       kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
-                                       h->st) == hipSuccess) {
-                        use_graph[i] = 1;
-                        h->graphs_epoch = h->graph_epoch;
-                        h->graphs_dth = dth[i];
-                        h->graphs_mu = mu[i];
-                    } else {
-                        (void)hipGetLastError();
-                    }
+    for (int i = 0; i < nsolves; ++i) {
+        mocca_solve *h = solves[i];
+        if (alternating(h))
+            if (int rc = alternate_prepare(h)) return failed(i, rc);
+        if (niter >= 3 && graph_ok(h)) {
+            if (h->graphs_epoch != h->graph_epoch || h->graphs_dth != dth[i] ||
+                h->graphs_mu != mu[i])
+                graphs_clear(h);
+            if (!h->graph_rows) {
+                if (hipMalloc(&h->graph_rows, 2 * kRowLen * sizeof(double)) != hipSuccess) {
+                    h->graph_rows = nullptr;
+                    (void)hipGetLastError();
                 }
-                if (use_graph[i]) {
-                    bool ok = true;
-                    rc = graph_step(h, h->graphs, k, dt[i], dth[i], mu[i], h->graph_rows, ok);
-                    if (!rc && ok) continue;
-                    if (rc) { bad = i; break; }
-                    use_graph[i] = 0;   // capture refused: this solve runs eagerly
-                    graphs_clear(h);
-                }
-                rc = iter_step(h, k, k, dt[i], dth[i], mu[i], false, nullptr,
-                               h->scf.erows + (size_t)(k - 1) * kRowLen,
-                               h->scf.erows + (size_t)k * kRowLen);
-                if (rc) bad = i;
+                ++h->graph_epoch;
+            }
+            use_graph[i] = h->graph_rows != nullptr;
+            if (use_graph[i]) {
+                h->graphs_epoch = h->graph_epoch;
+                h->graphs_dth = dth[i];
+                h->graphs_mu = mu[i];
             }
         }
+    }
+    // one iteration k of solve i; capture = whether a graph may be captured
+    auto step = [&](int i, int k, bool capture) -> int {
+        mocca_solve *h = solves[i];
+        if (use_graph[i] && k == 2) {
+            // both slots start as row 1 (entries an iteration does not
+            // write, such as the HF pairing terms, keep their values)
+            if (hipMemcpyAsync(h->graph_rows, h->scf.erows + kRowLen, kRowLen * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->st) != hipSuccess ||
+                hipMemcpyAsync(h->graph_rows + kRowLen, h->scf.erows + kRowLen,
+                               kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
+                               h->st) != hipSuccess) {
+                (void)hipGetLastError();
+                use_graph[i] = 0;
+            }
+        }
+        if (use_graph[i] && k >= 2) {
+            bool ok = true;
+            const int rc = graph_step(h, h->graphs, k, dt[i], dth[i], mu[i], h->graph_rows, ok,
+                                      capture);
+            if (rc || ok) return rc;
+            use_graph[i] = 0;   // capture refused or not allowed here: run eagerly
+            if (capture) graphs_clear(h);
+        }
+        const int rc = iter_step(h, k, h->scf.iteration + 1, dt[i], dth[i], mu[i], false,
+                                 nullptr, h->scf.erows + (size_t)(k - 1) * kRowLen,
+                                 h->scf.erows + (size_t)k * kRowLen);
+        if (!rc) ++h->scf.iteration;
+        return rc;
+    };
+    const int kw = std::min(niter, kWarm);
+    for (int k = 1; k <= kw; ++k)
+        for (int i = 0; i < nsolves; ++i)
+            if (int rc = step(i, k, true)) return failed(i, rc);
+    // The remaining iterations, split over a few host threads, each issuing
+    // its share of the solves in turn: one thread cannot issue the kernels
+    // of many small solves as fast as the device runs them.
+    auto run_range = [&](int i0, int i1, int &rc, int &bad) {
+        for (int k = kw + 1; k <= niter && !rc; ++k)
+            for (int i = i0; i < i1 && !rc; ++i)
+                if ((rc = step(i, k, false))) bad = i;
     };
     const int nthr = std::max(1, std::min({nsolves, 8,
                                            (int)std::thread::hardware_concurrency() / 2}));
     std::vector<int> trc(nthr, 0), tbad(nthr, -1);
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthr; ++t)
-        pool.emplace_back(run_range, nsolves * t / nthr, nsolves * (t + 1) / nthr,
-                          std::ref(trc[t]), std::ref(tbad[t]));
-    run_range(0, nsolves / nthr, trc[0], tbad[0]);
-    for (std::thread &th : pool) th.join();
+    if (niter > kw) {
+        for (int t = 1; t < nthr; ++t)
+            pool.emplace_back(run_range, nsolves * t / nthr, nsolves * (t + 1) / nthr,
+                              std::ref(trc[t]), std::ref(tbad[t]));
+        run_range(0, nsolves / nthr, trc[0], tbad[0]);
+        for (std::thread &th : pool) th.join();
+    }
     int rc = 0, bad = -1;
     for (int t = 0; t < nthr && !rc; ++t)
         if (trc[t]) { rc = trc[t]; bad = tbad[t]; }

@@ -288,6 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
 // to be a multiple of 16 (whole 16-byte lane loads inside a row).
 typedef __attribute__((address_space(3))) void lds_raw_t;
 
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt = 0 only:
+// vmcnt bits [3:0] and [15:14] zero, expcnt [6:4] and lgkmcnt [11:8] at
+// their "no wait" maxima.
+constexpr unsigned kWaitVmcnt0 = 0x0F70;
+
 template <typename T>
 __device__ __forceinline__ void dma16(const T *g, T *lds_base) {
     __builtin_amdgcn_global_load_lds((const void *)g, (lds_raw_t *)lds_base, 16, 0, 0);
@@ -457,6 +462,10 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
                 Yc[(c * YR + tyl + 4) * nx + tx] = v;
             }
         }
+        // The direct-to-LDS loads of plane z (issued during plane z-1) must
+        // have landed before the barrier publishes buffer b to the other
+        // waves: a workgroup barrier does not wait on vmcnt by itself.
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __syncthreads();
         // ---- stores of plane z-1, loads of plane z+1 and of the queue ----
         if (pst_p >= 0) {
@@ -665,11 +674,11 @@ void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_o
 #define MOCCA_HPSI_DMA(TR, WA)                                                              \
     {                                                                                       \
         const size_t dshm = hpsi_dma_lds<double, TR, WA>(m.nx, TY);                         \
-        static const bool lds_set = hipFuncSetAttribute(                                   \
-            (const void *)k_hpsi_fd_dma<double, TR, WA>,                                    \
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)hpsi_dma_lds_limit()) ==       \
-            hipSuccess;                                                                     \
-        (void)lds_set;                                                                      \
+        static const bool lds_set = require_lds((const void *)k_hpsi_fd_dma<double, TR, WA>, \
+                                                hpsi_dma_lds_limit(), "k_hpsi_fd_dma");     \
+        if (!lds_set)                                                                       \
+            require_lds((const void *)k_hpsi_fd_dma<double, TR, WA>, hpsi_dma_lds_limit(),  \
+                        "k_hpsi_fd_dma");                                                   \
         hipLaunchKernelGGL((k_hpsi_fd_dma<double, TR, WA>), dim3(ntasks), dim3(kDmaThreads), \
                            dshm, st, m, psi, psi_out, prev, fields, qn, tasks, TY, ntile_y, \
                            nzs, kin0, kin1, dth, mu, eparts);                               \
@@ -999,6 +1008,9 @@ __global__ __launch_bounds__(kBlock) void k_density_fd_dma(
                     Yc[(c * YR + tyl + G) * nx + tx] = v;
                 }
             }
+            // this wave's direct-to-LDS loads of buffer b are complete before
+            // the barrier makes the buffer visible to the other waves
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
             __syncthreads();
             if (z + 1 < ze) issue(ps, z + 1, b ^ 1);
             else if (psn) issue(psn, zb, b ^ 1);
