@@ -685,6 +685,44 @@ void free_all(mocca_solve *h) {
 }  // namespace detail
 }  // namespace mocca
 
+namespace mocca {
+namespace detail {
+
+GsPlan gs_plan(int nblk, int mmax, long K4, int M, size_t tsize) {
+    GsPlan p;
+    p.nblk = nblk;
+    p.mmax = mmax;
+    p.max_m = (mmax + kGramTile - 1) / kGramTile * kGramTile;
+    p.gtile = mmax <= 32 ? 32 : 64;
+    // Tiles are counted from the largest block (the leading dimension max_m
+    // may be larger); each block gets enough K splits to fill the GPU.
+    const int nt = (mmax + p.gtile - 1) / p.gtile;
+    p.ntile_max = nt * (nt + 1) / 2;
+    int nsplit = std::max(1, 2048 / std::max(1, nblk * p.ntile_max));
+    nsplit = (int)std::min<long>(nsplit, std::max<long>(1, K4 / (kGramKC * 8)));
+    p.nsplit = nsplit;
+    // matrix-core Gram for blocks of up to kMfmaMaxRows states
+    p.mpad = (mmax + 15) / 16 * 16;
+    p.mfma_gram = p.mpad <= kMfmaMaxRows;
+    // (the apply keeps one output row of tiles per wave: up to 192 states)
+    p.mfma_apply = p.mpad <= kMfmaApplyMaxRows;
+    const int nt16 = p.mpad / 16;
+    p.ntile16 = nt16 * (nt16 + 1) / 2;
+    // one round of single-wave workgroups (1024 = 256 CUs x 4 SIMDs at one
+    // wave per SIMD; four per SIMD for blocks of up to 64 states, whose
+    // waves hold few registers), partial tiles under 15% of the states'
+    // bytes, at least 4 chunks of 16 values per wave
+    const double data = (double)M * K4 * tsize;
+    long ns = std::max(1, (nt16 <= 4 ? 4096 : 1024) / std::max(1, nblk));
+    ns = std::min<long>(ns, (long)(0.15 * data / ((double)std::max(1, nblk) * p.ntile16 * 256 * 8)));
+    ns = std::min<long>(ns, std::max<long>(1, K4 / 64));
+    p.nsplit_mfma = (int)std::max<long>(1, ns);
+    return p;
+}
+
+}  // namespace detail
+}  // namespace mocca
+
 using namespace mocca::detail;
 
 // ===========================================================================
@@ -966,41 +1004,19 @@ int mocca_set_states_meta(mocca_solve *h, const int *qn, const double *occ) {
     if ((rc = dalloc(h, &b.blk_states, bs.size() * sizeof(int)))) return rc;
     if ((rc = dalloc(h, &b.blk_m, nblk * sizeof(int)))) return rc;
     if ((rc = dalloc(h, &b.perm, bs.size() * sizeof(int)))) return rc;
+    const GsPlan gp = gs_plan(nblk, mmax, 4L * h->N, h->M, h->tsize);
     const size_t gsz = (size_t)nblk * max_m * max_m * sizeof(double);
     if ((rc = dalloc(h, &b.G, gsz))) return rc;
     if ((rc = dalloc(h, &b.Tm, gsz))) return rc;
     if ((rc = dalloc(h, &b.work, gsz))) return rc;
-    h->gtile = mmax <= 32 ? 32 : 64;
-    // Tiles are counted from the largest block (the leading dimension max_m
-    // may be larger); each block gets enough K splits to fill the GPU.
-    const int nt = (mmax + h->gtile - 1) / h->gtile, ntile_max = nt * (nt + 1) / 2;
+    h->gtile = gp.gtile;
     h->mmax = mmax;
-    const long K4 = 4L * h->N;
-    int nsplit = std::max(1, 2048 / std::max(1, nblk * ntile_max));
-    nsplit = (int)std::min<long>(nsplit, std::max<long>(1, K4 / (kGramKC * 8)));
-    h->nsplit = nsplit;
-    // matrix-core Gram for blocks of up to kMfmaMaxRows states: K splits for
-    // ~2048 workgroups, partial matrices kept under 5% of the states' bytes
-    h->mpad = (mmax + 15) / 16 * 16;
-    h->mfma_gram = h->mpad <= kMfmaMaxRows;
-    // (the apply keeps one output row of tiles per wave: up to 192 states)
-    h->mfma_apply = h->mpad <= kMfmaApplyMaxRows;
-    const int nt16 = h->mpad / 16, ntile16 = nt16 * (nt16 + 1) / 2;
-    {
-        // one round of single-wave workgroups (1024 = 256 CUs x 4 SIMDs at
-        // one wave per SIMD; four per SIMD for blocks of up to 64 states,
-        // whose waves hold few registers), partial tiles under 15% of the
-        // states' bytes, at least 4 chunks of 16 values per wave
-        const double data = (double)h->M * K4 * h->tsize;
-        long ns = std::max(1, (nt16 <= 4 ? 4096 : 1024) / nblk);
-        ns = std::min<long>(ns, (long)(0.15 * data / ((double)nblk * ntile16 * 256 * 8)));
-        ns = std::min<long>(ns, std::max<long>(1, K4 / 64));
-        h->nsplit_mfma = (int)std::max<long>(1, ns);
-    }
-    const size_t gpart_old = (size_t)nblk * ntile_max * nsplit * h->gtile * h->gtile;
-    const size_t gpart_new = (size_t)nblk * h->nsplit_mfma * ntile16 * 256;
-    if ((rc = dalloc(h, &b.gpart, std::max(gpart_old, gpart_new) * sizeof(double))))
-        return rc;
+    h->nsplit = gp.nsplit;
+    h->mpad = gp.mpad;
+    h->mfma_gram = gp.mfma_gram;
+    h->mfma_apply = gp.mfma_apply;
+    h->nsplit_mfma = gp.nsplit_mfma;
+    if ((rc = dalloc(h, &b.gpart, gp.gpart_doubles() * sizeof(double)))) return rc;
     const size_t npart = std::max(chunks.size(), dchunks.size());
     if ((rc = dalloc(h, &b.dpart, npart * 9 * (size_t)h->N * h->tsize))) return rc;
     if ((rc = dalloc(h, &b.dchunks, dchunks.size() * sizeof(Chunk)))) return rc;

@@ -206,6 +206,22 @@ namespace detail {
 
 int fail(mocca_solve *h, int code, const char *fmt, ...);
 
+// Launch plan of the Cholesky-QR Gram-Schmidt for nblk blocks of at most
+// mmax states over K4 = 4N values (M states of tsize bytes in all): tile
+// sizes, K splits and which kernels run.  The partial Gram matrices need
+// gpart_doubles() doubles.
+struct GsPlan {
+    int nblk = 0, mmax = 0, max_m = 0, gtile = 32, ntile_max = 0, nsplit = 1;
+    int mpad = 0, ntile16 = 0, nsplit_mfma = 1;
+    bool mfma_gram = false, mfma_apply = false;
+    size_t gpart_doubles() const {
+        const size_t a = (size_t)nblk * ntile_max * nsplit * gtile * gtile;
+        const size_t b = (size_t)nblk * nsplit_mfma * ntile16 * 256;
+        return a > b ? a : b;
+    }
+};
+GsPlan gs_plan(int nblk, int mmax, long K4, int M, size_t tsize);
+
 #define HIPCHK(h, call)                                                           \
     do {                                                                          \
         hipError_t e_ = (call);                                                   \
