@@ -124,6 +124,28 @@ int mocca_grad_desc(mocca_solve *h, double dt_over_hbar, double momentum);
  * per (isospin, parity, signature) block, done as Cholesky QR in FP64. */
 int mocca_gram_schmidt(mocca_solve *h);
 
+/* The orthonormalisation of mocca_gram_schmidt without any allocation, on
+ * FP64 states the caller keeps in device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).
+ *   psi, out   device, double [nwt][4][n]; out must not overlap psi
+ *   qn         host, int [nwt][4] as in mocca_set_states_meta (blocks by
+ *              isospin, parity, signature)
+ *   energy     device, double [nwt]: the ordering energies (mocca_set_energies)
+ *   dv         volume element of the inner products
+ *   status     device int or NULL: set to 0, then non-zero if a block had a
+ *              non-positive norm (the call does not synchronise, so it
+ *              cannot report that in its return code)
+ *   stream     hipStream_t to order the work on (NULL: the default stream)
+ * Its workspace is one static device array of 32 MiB per GPU, held until
+ * the process ends; nwt <= 512, blocks of at most 512 states, and sizes
+ * whose workspace exceeds the array fail with MOCCA_ERR_UNSUPPORTED (when
+ * only the Gram partials do not fit, fewer K splits are taken).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU race.
+ * Errors are described by mocca_last_error(NULL). */
+int mocca_gram_schmidt_noalloc(const double *psi, double *out, int nwt, long n, const int *qn,
+                               const double *energy, double dv, int *status, void *stream);
+
 /* ComputeDensity(Density, .false.) (Densities.f90:431-1187, SLy4-type
  * functional: no tensor or N2LO densities). */
 int mocca_compute_density(mocca_solve *h, mocca_density *out);

@@ -35,6 +35,7 @@ EXPORTS = (
     "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
     "mocca_alternate_step", "mocca_get_constraints", "mocca_set_pairing",
     "mocca_solve_pairing", "mocca_get_pairing", "mocca_batch_run",
+    "mocca_gram_schmidt_noalloc",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -109,12 +110,27 @@ def load():
     lib.mocca_last_error.argtypes = [ctypes.c_void_p]
     lib.mocca_solve_create.argtypes = [ctypes.POINTER(MeshC), ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_void_p)]
+    lib.mocca_gram_schmidt_noalloc.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_long, _ip, ctypes.c_void_p,
+        ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
     for name in EXPORTS:
         f = getattr(lib, name)
         if name not in ("mocca_last_error", "mocca_solve_create"):
             f.restype = ctypes.c_int
     _LIB = lib
     return lib
+
+
+def gram_schmidt_noalloc(psi, out, nwt, n, qn, energy, dv, status=None, stream=None):
+    """mocca_gram_schmidt_noalloc: psi, out, energy, status are device
+    addresses (ints), qn a host array [nwt][4], stream a hipStream_t handle
+    (None: the default stream)."""
+    lib = load()
+    qn = np.ascontiguousarray(qn, dtype=np.int32).reshape(nwt, 4)
+    rc = lib.mocca_gram_schmidt_noalloc(psi, out, nwt, n, _p(qn), energy, dv, status, stream)
+    if rc != 0:
+        msg = lib.mocca_last_error(None)
+        raise MoccaError(f"mocca error {rc}: {msg.decode() if msg else ''}")
 
 
 def _p(a):
