@@ -331,7 +331,18 @@ typedef struct mocca_pairing {
     int type;
     int guess_kappa;
     int iterations;       /* HFBIter (50 for the Broyden solver) */
-    int pad_;
+    int solver;           /* Fermi-energy solver: 0 = automatic (1 while every
+                             (isospin, parity) block holds at most 32 states,
+                             else 2); 1 = the reference's Broyden iteration
+                             (HFBFindFermiEnergyBroyden) in one workgroup per
+                             isospin with the matrices in LDS, blocks of at most
+                             32 states; 2 = the same iteration on the
+                             large-block path (blocks of up to 128 states);
+                             3 = Newton's method with the exact derivative
+                             dN/dlambda on the large-block path: fewer
+                             evaluations to the same FermiPrec, but where the
+                             reference's iteration stops unconverged it ends
+                             elsewhere */
     double strength[2];   /* PairingNeutron, PairingProton (MeV fm^3) */
     double alpha[2];      /* AlphaNeutron, AlphaProton */
     double cut[2];        /* CutNeutron, CutProton (5 MeV) */

@@ -84,7 +84,7 @@ ROW_LEN = len(ENERGY_FIELDS)
 
 class PairingC(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("guess_kappa", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("solver", ctypes.c_int),
                 ("strength", ctypes.c_double * 2), ("alpha", ctypes.c_double * 2),
                 ("cut", ctypes.c_double * 2), ("mu", ctypes.c_double * 2),
                 ("rhosat", ctypes.c_double), ("hfb_mix", ctypes.c_double),
@@ -336,10 +336,13 @@ class Solve:
 
     def set_pairing(self, type=2, strength=(1250.0, 1250.0), alpha=(1.0, 1.0), cut=(5.0, 5.0),
                     mu=(0.5, 0.5), rhosat=0.16, hfb_mix=0.1, iterations=50, fermi_prec=1e-12,
-                    fermi=(-10.0, -10.0), guess_kappa=True):
-        """Pairing of SolvePairing: type 0 HF, 2 HFB (namelist Pairing defaults)."""
+                    fermi=(-10.0, -10.0), guess_kappa=True, solver=0):
+        """Pairing of SolvePairing: type 0 HF, 2 HFB (namelist Pairing defaults);
+        solver 0 automatic, 1 in-LDS Broyden (blocks <= 32 states), 2 large-block
+        Broyden, 3 large-block Newton (blocks <= 128)."""
         d2 = lambda v: (ctypes.c_double * 2)(*[float(x) for x in v])
-        p = PairingC(int(type), int(bool(guess_kappa)), int(iterations), 0, d2(strength),
+        p = PairingC(int(type), int(bool(guess_kappa)), int(iterations), int(solver),
+                     d2(strength),
                      d2(alpha), d2(cut), d2(mu), float(rhosat), float(hfb_mix),
                      float(fermi_prec), d2(fermi))
         self._chk(self.lib.mocca_set_pairing(self.h, ctypes.byref(p)))

@@ -591,6 +591,7 @@ namespace detail {
 
 int hfb_free(mocca_solve *h) {
     Hfb &p = h->hfb;
+    hfb_large_free(h);
     void *ptrs[] = {p.kappa, p.rho, p.orho, p.okappa, p.delta, p.fermi, p.pc, p.dfac, p.field,
                     p.qpe, p.tcan, p.can_psi, p.can_occ, p.work, p.status, p.tables};
     for (void *q : ptrs)
@@ -646,8 +647,24 @@ int hfb_solve_impl(mocca_solve *h, bool first) {
 }
 
 int hfb_solve_any(mocca_solve *h, bool first) {
+    if (h->hfb.large) return hfb_large_solve_any(h);
     return h->prec == MOCCA_FP64 ? hfb_solve_impl<double>(h, first)
                                  : hfb_solve_impl<float>(h, first);
+}
+
+void hfb_launch_mix(mocca_solve *h, int first) {
+    Hfb &p = h->hfb;
+    hipLaunchKernelGGL(k_hfb_mix, dim3(1), dim3(256), 0, h->st, p.total, p.k.mix, first, p.rho,
+                       p.kappa, p.orho, p.okappa, p.delta);
+}
+
+void hfb_launch_cutoffs(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    const long N = h->N;
+    hipLaunchKernelGGL(k_hfb_cutoffs, dim3((h->M + 63) / 64), dim3(64), 0, h->st, p.k, h->b.E,
+                       h->b.qn, p.fermi, h->M, p.pc);
+    hipLaunchKernelGGL(k_hfb_dfac, dim3((N + 255) / 256), dim3(256), 0, h->st, p.k, h->b.dens,
+                       N, p.dfac);
 }
 
 int hfb_energy_impl(mocca_solve *h, double *row) {
@@ -662,10 +679,12 @@ int hfb_check_status(mocca_solve *h) {
     int st = 0;
     HIPCHK(h, hipMemcpyAsync(&st, h->hfb.status, sizeof(int), hipMemcpyDeviceToHost, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
-    if (st & 6) {
+    if (st & 14) {
         HIPCHK(h, hipMemsetAsync(h->hfb.status, 0, sizeof(int), h->st));
-        return fail(h, MOCCA_ERR_NUMERIC, (st & 2) ? "NaN in the calculation of Fermi energies"
-                                                    : "canonical occupations above one");
+        return fail(h, MOCCA_ERR_NUMERIC,
+                    (st & 2) ? "NaN in the calculation of Fermi energies"
+                             : (st & 4) ? "canonical occupations above one"
+                                        : "the HFB eigen-solver did not converge");
     }
     h->hfb.unconverged = (st & 1) != 0;
     return 0;
@@ -696,10 +715,6 @@ int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
                 if ((h->qn[4 * i] + 3) / 2 - 1 == it && (h->qn[4 * i + 1] + 3) / 2 - 1 == P)
                     st.push_back(i);
             if (st.empty()) continue;
-            if ((int)st.size() * 2 > kHfbMaxN)
-                return fail(h, MOCCA_ERR_UNSUPPORTED,
-                            "HFB block of %zu states (at most %d with partners)", st.size(),
-                            kHfbMaxN);
             HfbBlockK &bk = k.blk[nb];
             bk.it = it;
             bk.P = P;
@@ -747,8 +762,21 @@ int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
     }
     p.lds_broyden = lb * sizeof(double);
     p.lds_canonical = 2 * (size_t)p.hmax * p.hmax * sizeof(double);
-    if (p.lds_broyden > 60 * 1024)
-        return fail(h, MOCCA_ERR_UNSUPPORTED, "HFB blocks too large for the in-LDS solver");
+    // the in-LDS Broyden solver while its matrices fit, else the large-block
+    // Newton solver
+    const bool small_ok = 2 * p.hmax <= kHfbMaxN && p.lds_broyden <= 60 * 1024;
+    if (q->solver < 0 || q->solver > 3)
+        return fail(h, MOCCA_ERR_ARG,
+                    "pairing solver %d (0 automatic, 1 in-LDS Broyden, 2 large-block Broyden, "
+                    "3 large-block Newton)", q->solver);
+    if (q->solver == 1 && !small_ok)
+        return fail(h, MOCCA_ERR_UNSUPPORTED,
+                    "HFB block of %d states: the in-LDS solver takes at most %d", p.hmax,
+                    kHfbMaxN / 2);
+    p.large = q->solver >= 2 || (q->solver == 0 && !small_ok);
+    p.newton = q->solver == 3;
+    if (p.large && !hfb_large_fits(p.hmax))
+        return fail(h, MOCCA_ERR_UNSUPPORTED, "HFB block of %d states (at most 128)", p.hmax);
     const long N = h->N;
     int rc;
     if ((rc = dalloc(h, &p.kappa, off * sizeof(double))) ||
@@ -786,6 +814,8 @@ int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
     HIPCHK(h, hipMemset(p.status, 0, sizeof(int)));
     HIPCHK(h, hipMemset(p.qpe, 0, M * sizeof(double)));
     HIPCHK(h, hipMemset(p.can_occ, 0, M * sizeof(double)));
+    if (p.large)
+        if (int rc2 = hfb_large_setup(h)) return rc2;
     p.on = true;
     p.started = false;
     return 0;

@@ -1,0 +1,1070 @@
+// HFB pairing step for blocks too large for the in-LDS solvers of
+// mocca_hfb.hip (PairingType 2 of SolvePairing, Pairing.f90:743-890; time
+// reversal and signature +1 states, as there).  Everything is FP64 except
+// where the states are FP32 (mixed precision: they are widened).
+//
+//   gaps           Delta(a, h+a') = dv c_a c_a' sum_r [Re P Re F + Im P Im F]
+//                  (HFBGaps, HFB.f90:520-632) as a matrix product
+//                  D = Psi Ytilde^T over K = 4N, Ytilde a pointwise transform
+//                  of the states by the pairing field: FP64 matrix cores;
+//   pairing field  Delta_q(r) = -g(r) sum_a P(phi_a, psi_a),
+//                  phi_a = sum_b c_a c_b kappa(h+a, b) psi_b (HFBPairingField,
+//                  HFB.f90:350-518): the phi by the dense matrix-core apply,
+//                  then one pass over the points;
+//   Fermi energies N(lambda) = particle number (HFBNumberofParticles,
+//                  HFB.f90:810-1073) solved by the reference's Broyden
+//                  iteration (HFBFindFermiEnergyBroyden :1849-2161: the same
+//                  evaluation points and stopping rule, so also the same
+//                  result where it stops unconverged), or on request by
+//                  Newton's method with the exact derivative dN/dlambda =
+//                  2 sum_{k+, l-} C_kl^2 / (E_k - E_l), C = V^T Sigma_z V
+//                  (first-order perturbation of the eigenvectors by
+//                  dT = -Sigma_z dlambda), safeguarded by a bracket, which
+//                  reaches FermiPrec in fewer evaluations.  Each
+//                  evaluation diagonalises the signature-reduced HFB matrix
+//                  T = [[E - lambda, D], [D^T, -(E - lambda)]] of every block
+//                  (DiagonaliseHFBHamiltonian_Signature :2695-2801) by cyclic
+//                  Jacobi in the eigenbasis of the previous evaluation;
+//   canonical basis eigenvectors of rho_h (DiagonaliseRhoHFB :3156-3285,
+//                  HFBOccupations :3361-3561) by the same Jacobi, the
+//                  canonical states by the dense matrix-core apply.
+//
+// The Jacobi kernel keeps the packed upper triangle of the matrix in LDS (up
+// to 200 rows; in global memory above), runs whole sweeps of the round-robin
+// ordering (n/2 disjoint rotations per round, every 2x2 block of the matrix
+// updated from its old values in one pass) and logs each round's rotations;
+// the eigenvectors are then rotated by the log in a separate kernel whose
+// workgroups own disjoint rows of the basis, so the one-workgroup solver only
+// streams the matrix.
+#include "mocca_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace mocca {
+
+namespace {
+
+constexpr int kJacThreads = 512;
+constexpr int kJacLdsMaxN = 200;     // packed upper triangle + rotation tables in 160 KiB
+constexpr int kJacMaxSweeps = 24;
+constexpr int kApplyRows = 16;       // basis rows per workgroup of the rotation apply
+constexpr int kGramWaves = 1024;     // waves of one pair-Gram launch
+
+typedef double hl_d4 __attribute__((ext_vector_type(4)));   // FP64 MFMA accumulator
+
+__device__ __forceinline__ double hl_wave_sum(double a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    return a;
+}
+
+__device__ double hl_block_sum(double a, double *sh) {
+    a = hl_wave_sum(a);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64, nw = blockDim.x / 64;
+    __syncthreads();
+    if (l == 0) sh[w] = a;
+    __syncthreads();
+    double s = 0;
+    for (int k = 0; k < nw; ++k) s += sh[k];
+    __syncthreads();
+    return s;
+}
+
+// packed upper triangle, row-major: element (i, j), i <= j
+__device__ __forceinline__ int pk(int i, int j, int n) { return i * n - (i * (i - 1)) / 2 + (j - i); }
+
+// pair pp of round r of the round-robin ordering of m (even) players
+__device__ __forceinline__ void rr_pair(int pp, int r, int m, int &i, int &j) {
+    auto pos = [&](int x) { return x == 0 ? 0 : ((x - 1 + r) % (m - 1)) + 1; };
+    i = pos(pp);
+    j = pos(m - 1 - pp);
+    if (i > j) { const int t = i; i = j; j = t; }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi on the symmetric n x n matrices A[b] (dense, leading
+// dimension ld, stride mstride; symmetrised on load), one workgroup each:
+// eigenvalues into w[b][.], the rotations of every round into
+// log[b][round][pair] = (c, s), the number of rounds into rounds[b].  The
+// rotation of pair (i, j) is that of Golub & Van Loan 8.4.2, applied as
+// A <- J^T A J with J = [[c, s], [-s, c]] in rows/columns (i, j); sweeps stop
+// when the off-diagonal square sum is below 1e-32 of the diagonal one.
+// GA: the packed matrix in global memory (Ag[b]) instead of LDS.
+template <bool GA>
+__global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
+    const double *__restrict__ Ain, int ld, long mstride, const int *__restrict__ nb,
+    const int *__restrict__ active, double *__restrict__ Ag, double *__restrict__ w,
+    double2 *__restrict__ log, long logstride, int *__restrict__ rounds, int maxsweeps,
+    int *__restrict__ status) {
+    extern __shared__ __align__(16) double lds[];
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (active && !active[b]) return;
+    const int n = nb[b], m = n + (n & 1), M2 = m / 2, npk = n * (n + 1) / 2;
+    double2 *cs = reinterpret_cast<double2 *>(lds);                      // [M2]
+    int2 *pr = reinterpret_cast<int2 *>(cs + M2);                        // [M2]
+    double *sh = reinterpret_cast<double *>(pr + M2 + (M2 & 1));         // [16]
+    double *A = GA ? Ag + b * mstride : sh + 16;                         // packed upper
+    const double *Ab = Ain + b * mstride;
+    for (int q = tid; q < n * n; q += nt) {
+        const int i = q / n, j = q % n;
+        if (j >= i) A[pk(i, j, n)] = 0.5 * (Ab[(long)i * ld + j] + Ab[(long)j * ld + i]);
+    }
+    __syncthreads();
+    double2 *lb = log + b * logstride;
+    int done = 0;
+    bool conv = false;
+    for (int sweep = 0; sweep <= maxsweeps; ++sweep) {
+        double off = 0, dia = 0;
+        for (int q = tid; q < n * n; q += nt) {
+            const int i = q / n, j = q % n;
+            if (j < i) continue;
+            const double v = A[pk(i, j, n)];
+            if (i == j) dia += v * v; else off += 2.0 * v * v;
+        }
+        off = hl_block_sum(off, sh);
+        dia = hl_block_sum(dia, sh);
+        if (off <= 1e-32 * dia || off == 0.0) { conv = true; break; }
+        if (sweep == maxsweeps) break;
+        for (int r = 0; r < m - 1; ++r) {
+            for (int pp = tid; pp < M2; pp += nt) {
+                int i, j;
+                rr_pair(pp, r, m, i, j);
+                double c = 1.0, s = 0.0;
+                if (j < n) {
+                    const double apq = A[pk(i, j, n)];
+                    if (apq != 0.0) {
+                        const double tau = (A[pk(j, j, n)] - A[pk(i, i, n)]) / (2.0 * apq);
+                        const double t = tau >= 0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                                  : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                    }
+                }
+                cs[pp] = make_double2(c, s);
+                pr[pp] = make_int2(i, j);
+                lb[(long)done * M2 + pp] = make_double2(c, s);
+            }
+            __syncthreads();
+            // every 2x2 block (k, l), k <= l, from its old values: thread t
+            // takes column pair l = t % M2 and row pairs k = t / M2 + s G
+            const int G = (nt / M2) > 0 ? nt / M2 : 1;
+            for (int t = tid; t < G * M2; t += nt) {
+                const int l = t % M2;
+                const double2 cl = cs[l];
+                const int2 pl = pr[l];
+                const bool vl = pl.y < n;
+                for (int k = t / M2; k <= l; k += G) {
+                    const double2 ck = cs[k];
+                    if (ck.y == 0.0 && cl.y == 0.0) continue;
+                    const int2 pkk = pr[k];
+                    const bool vk = pkk.y < n;
+                    if (k == l) {
+                        if (!vk) continue;
+                        const int i = pkk.x, j = pkk.y;
+                        const double aii = A[pk(i, i, n)], ajj = A[pk(j, j, n)],
+                                     aij = A[pk(i, j, n)];
+                        const double c = ck.x, s = ck.y;
+                        // J^T [[aii, aij], [aij, ajj]] J, off-diagonal exactly 0
+                        A[pk(i, i, n)] = c * c * aii - 2.0 * c * s * aij + s * s * ajj;
+                        A[pk(j, j, n)] = s * s * aii + 2.0 * c * s * aij + c * c * ajj;
+                        A[pk(i, j, n)] = 0.0;
+                        continue;
+                    }
+                    // rows (ik, jk) x columns (il, jl); entries with an index
+                    // past n (the odd player) stay out
+                    const int r0 = pkk.x, r1 = pkk.y, c0 = pl.x, c1 = pl.y;
+                    auto at = [&](int a, int bb) { return a <= bb ? pk(a, bb, n) : pk(bb, a, n); };
+                    const double x00 = A[at(r0, c0)];
+                    const double x01 = vl ? A[at(r0, c1)] : 0.0;
+                    const double x10 = vk ? A[at(r1, c0)] : 0.0;
+                    const double x11 = (vk && vl) ? A[at(r1, c1)] : 0.0;
+                    // Y = Jk^T X Jl, Jk = [[ck, sk], [-sk, ck]]
+                    const double a0 = ck.x * x00 - ck.y * x10, a1 = ck.x * x01 - ck.y * x11;
+                    const double b0 = ck.y * x00 + ck.x * x10, b1 = ck.y * x01 + ck.x * x11;
+                    A[at(r0, c0)] = a0 * cl.x - a1 * cl.y;
+                    if (vl) A[at(r0, c1)] = a0 * cl.y + a1 * cl.x;
+                    if (vk) A[at(r1, c0)] = b0 * cl.x - b1 * cl.y;
+                    if (vk && vl) A[at(r1, c1)] = b0 * cl.y + b1 * cl.x;
+                }
+            }
+            __syncthreads();
+            ++done;
+        }
+    }
+    double *wb = w + (long)b * ld;
+    for (int i = tid; i < n; i += nt) wb[i] = A[pk(i, i, n)];
+    if (tid == 0) {
+        rounds[b] = done;
+        if (!conv) atomicOr(status, 8);
+    }
+}
+
+// The logged rotations applied to the basis: V[b] <- V[b] J_1 J_2 ... (in
+// place; V = identity first when ident); each workgroup owns kApplyRows rows
+// in LDS.  Round g of the log is round g mod (m-1) of the ordering.
+__global__ __launch_bounds__(256) void k_lj_apply(double *__restrict__ V, int ld, long mstride,
+                                                  const int *__restrict__ nb,
+                                                  const int *__restrict__ active,
+                                                  const double2 *__restrict__ log,
+                                                  long logstride, const int *__restrict__ rounds,
+                                                  int ident) {
+    extern __shared__ double rows[];   // [kApplyRows][n]
+    __shared__ int2 pr[128];
+    __shared__ double2 cs[128];
+    const int b = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
+    if (active && !active[b]) return;
+    const int n = nb[b], m = n + (n & 1), M2 = m / 2;
+    const int r0 = blockIdx.x * kApplyRows;
+    if (r0 >= n) return;
+    const int nr = min(kApplyRows, n - r0);
+    double *Vb = V + b * mstride;
+    for (int e = tid; e < nr * n; e += nt) {
+        const int r = e / n, c = e % n;
+        rows[e] = ident ? (r0 + r == c ? 1.0 : 0.0) : Vb[(long)(r0 + r) * ld + c];
+    }
+    const double2 *lb = log + b * logstride;
+    const int R = rounds[b];
+    for (int g = 0; g < R; ++g) {
+        __syncthreads();
+        for (int pp = tid; pp < M2; pp += nt) {
+            int i, j;
+            rr_pair(pp, g % (m - 1), m, i, j);
+            pr[pp] = make_int2(i, j);
+            cs[pp] = lb[(long)g * M2 + pp];
+        }
+        __syncthreads();
+        for (int e = tid; e < nr * M2; e += nt) {
+            const int r = e / M2, pp = e % M2;
+            const int2 p = pr[pp];
+            const double2 q = cs[pp];
+            if (p.y >= n || q.y == 0.0) continue;
+            double *row = rows + r * n;
+            const double vi = row[p.x], vj = row[p.y];
+            row[p.x] = q.x * vi - q.y * vj;
+            row[p.y] = q.y * vi + q.x * vj;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nr * n; e += nt) {
+        const int r = e / n, c = e % n;
+        Vb[(long)(r0 + r) * ld + c] = rows[e];
+    }
+}
+
+// C[b] = op(A[b]) B'[b] for n x n matrices (leading dimension ld), where
+// op = transpose when TA and B' = B with its rows k >= h[b] negated when SZ
+// (Sigma_z B); one 16 x 16 tile of C per workgroup.
+template <bool TA, bool SZ>
+__global__ __launch_bounds__(256) void k_lj_gemm(const double *__restrict__ A,
+                                                 const double *__restrict__ B,
+                                                 double *__restrict__ C, int ld, long mstride,
+                                                 const int *__restrict__ nb,
+                                                 const int *__restrict__ hb,
+                                                 const int *__restrict__ active) {
+    __shared__ double As[16][17], Bs[16][17];
+    const int b = blockIdx.z;
+    if (active && !active[b]) return;
+    const int n = nb[b], h = SZ ? hb[b] : 0;
+    const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+    if (i0 >= n || j0 >= n) return;
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    const double *Ab = A + b * mstride, *Bb = B + b * mstride;
+    double acc = 0;
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        const int ia = i0 + ty, ka = k0 + tx;
+        As[ty][tx] = (ia < n && ka < n) ? (TA ? Ab[(long)ka * ld + ia] : Ab[(long)ia * ld + ka])
+                                        : 0.0;
+        const int kb = k0 + ty, jb = j0 + tx;
+        double bv = (kb < n && jb < n) ? Bb[(long)kb * ld + jb] : 0.0;
+        if (SZ && kb >= h) bv = -bv;
+        Bs[ty][tx] = bv;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+        __syncthreads();
+    }
+    if (i0 + ty < n && j0 + tx < n) C[b * mstride + (long)(i0 + ty) * ld + j0 + tx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Gaps: partial sums S[a][a'] = sum_K psi_a[K] Ytilde_a'[K] over a K slice
+// per wave, all tiles of row tiles 0..NI-1 and column tiles
+// NJ blockIdx.z .. +NJ-1 of a block.  K is walked four points at a time:
+// lane group g = lane / 16 holds component g of points p0 .. p0+3, and the
+// transformed partner state is, with F = Fr + i Fi the pairing field,
+//   Ytilde_g = -Fr y_g + Fi y_{g+1} (g even),  -Fr y_g - Fi y_{g-1} (g odd),
+// so that sum_g x_g Ytilde_g = Re P(x, y) Fr + Im P(x, y) Fi for the
+// time-reversal pair amplitude P (GetPairDensity, PairingInteraction.f90:
+// 101-180).  Result lane L, register r: S[16 I + L/16 + 4 r][16 J + L%16].
+template <typename T, int NI, int NJ>
+__global__ __launch_bounds__(64) void k_hl_pair_gram(HfbK k, const T *__restrict__ psi, long N,
+                                                     const double *__restrict__ field,
+                                                     double *__restrict__ part, int nsplit,
+                                                     int nt) {
+    const int w = blockIdx.x, b = blockIdx.y, J0 = blockIdx.z * NJ, lane = threadIdx.x;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, g = lane / 16;
+    const double *Fr = field + (2 * bk.it) * N, *Fi = Fr + N;
+    const T *xr[NI];
+    const T *yr[NJ];
+    bool xok[NI], yok[NJ];
+#pragma unroll
+    for (int I = 0; I < NI; ++I) {
+        const int a = 16 * I + lane % 16;
+        xok[I] = a < h;
+        xr[I] = psi + (long)k.state[bk.first + (xok[I] ? a : 0)] * 4 * N + (long)g * N;
+    }
+#pragma unroll
+    for (int J = 0; J < NJ; ++J) {
+        const int a = 16 * (J0 + J) + lane % 16;
+        yok[J] = a < h && J0 + J < nt;
+        yr[J] = psi + (long)k.state[bk.first + (yok[J] ? a : 0)] * 4 * N;
+    }
+    const int gp = g ^ 1;
+    const double sg = (g & 1) ? -1.0 : 1.0;
+    hl_d4 acc[NI][NJ];
+#pragma unroll
+    for (int I = 0; I < NI; ++I)
+#pragma unroll
+        for (int J = 0; J < NJ; ++J) acc[I][J] = hl_d4{0, 0, 0, 0};
+    const long nch = (N + 3) / 4;
+    const long c0 = nch * w / nsplit, c1 = nch * (w + 1) / nsplit;
+    for (long c = c0; c < c1; ++c) {
+        const long p0 = 4 * c;
+        double fr[4], fi[4];
+        bool pin[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            pin[q] = p0 + q < N;
+            const long p = pin[q] ? p0 + q : 0;
+            fr[q] = Fr[p];
+            fi[q] = Fi[p];
+        }
+        double x[NI][4], yt[NJ][4];
+#pragma unroll
+        for (int I = 0; I < NI; ++I)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                x[I][q] = (xok[I] && pin[q]) ? (double)xr[I][pin[q] ? p0 + q : 0] : 0.0;
+#pragma unroll
+        for (int J = 0; J < NJ; ++J)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const long p = pin[q] ? p0 + q : 0;
+                const double y0 = (double)yr[J][(long)g * N + p];
+                const double y1 = (double)yr[J][(long)gp * N + p];
+                yt[J][q] = (yok[J] && pin[q]) ? (-fr[q] * y0 + sg * fi[q] * y1) : 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int I = 0; I < NI; ++I)
+#pragma unroll
+                for (int J = 0; J < NJ; ++J)
+                    acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[I][q], yt[J][q],
+                                                                     acc[I][J], 0, 0, 0);
+    }
+    double *P = part + ((long)b * nsplit + w) * nt * nt * 256 + lane * 4;
+#pragma unroll
+    for (int I = 0; I < NI; ++I)
+#pragma unroll
+        for (int J = 0; J < NJ; ++J) {
+            if (J0 + J >= nt) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) P[(I * nt + J0 + J) * 256 + r] = acc[I][J][r];
+        }
+}
+
+// Delta(a, h+a') = dv c_a c_a' sum_slices S, Delta(h+a', a) = -Delta(a, h+a'),
+// zero below the pair cut (HFBGaps :520-632): one thread per (a, a').
+__global__ void k_hl_gaps_reduce(HfbK k, const double *__restrict__ part, int nsplit, int nt,
+                                 const double *__restrict__ pc, double dv,
+                                 double *__restrict__ delta) {
+    const int b = blockIdx.y;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= h * h) return;
+    const int a = e / h, a2 = e % h;
+    const int s1 = k.state[bk.first + a], s2 = k.state[bk.first + a2];
+    const double cc = pc[s1] * pc[s2];
+    double *dl = delta + bk.off;
+    if (cc < kHfbNumCut) {
+        dl[a * n + h + a2] = 0.0;
+        dl[(h + a2) * n + a] = 0.0;
+        return;
+    }
+    const int I = a / 16, J = a2 / 16, ri = a % 16, cj = a2 % 16;
+    // lane L = 16 (ri % 4) + cj, register r = ri / 4
+    const int idx = (16 * (ri % 4) + cj) * 4 + ri / 4;
+    double s = 0;
+    for (int w = 0; w < nsplit; ++w)
+        s += part[(((long)b * nsplit + w) * nt * nt + I * nt + J) * 256 + idx];
+    const double d = dv * pc[s1] * pc[s2] * s;
+    dl[a * n + h + a2] = d;
+    dl[(h + a2) * n + a] = -d;
+}
+
+// The apply matrices of the pairing field: Tap[b][b1][a2] = c_b1 c_a2
+// kappa(h+a2, b1), zero for c_b1 or the product below the pair cut (the
+// pairs HFBPairingField drops); the block lists and sizes for the apply.
+__global__ void k_hl_field_w(HfbK k, const double *__restrict__ kappa,
+                             const double *__restrict__ pc, int hmax,
+                             double *__restrict__ Tap, int *__restrict__ list,
+                             int *__restrict__ hlist) {
+    const int b = blockIdx.x;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const double *kap = kappa + bk.off;
+    double *Tb = Tap + (long)b * hmax * hmax;
+    for (int e = threadIdx.x; e < hmax * hmax; e += blockDim.x) {
+        const int b1 = e / hmax, a2 = e % hmax;
+        double v = 0.0;
+        if (b1 < h && a2 < h) {
+            const double c1 = pc[k.state[bk.first + b1]], c2 = pc[k.state[bk.first + a2]];
+            const double kv = kap[(h + a2) * n + b1];
+            if (c1 >= kHfbNumCut && c1 * c2 * fabs(kv) >= kHfbNumCut) v = c1 * c2 * kv;
+        }
+        Tb[e] = v;
+    }
+    for (int i = threadIdx.x; i < hmax; i += blockDim.x)
+        list[b * hmax + i] = k.state[bk.first + (i < h ? i : 0)];
+    if (threadIdx.x == 0) hlist[b] = h;
+}
+
+// Pairing field of each isospin from phi (the apply output, in the states'
+// slots) and the states: field = -dfac sum_a P(phi_a, psi_a).
+template <typename T>
+__global__ __launch_bounds__(256) void k_hl_field_sum(HfbK k, const T *__restrict__ psi,
+                                                      const T *__restrict__ phi, long N,
+                                                      const double *__restrict__ dfac,
+                                                      double *__restrict__ field) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int it = blockIdx.y;
+    if (p >= N) return;
+    double fr = 0, fi = 0;
+    for (int b = 0; b < k.nblk; ++b) {
+        const HfbBlockK &bk = k.blk[b];
+        if (bk.it != it) continue;
+        for (int a = 0; a < bk.h; ++a) {
+            const long s = k.state[bk.first + a];
+            double x[4], v[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                x[c] = (double)phi[(s * 4 + c) * N + p];
+                v[c] = (double)psi[(s * 4 + c) * N + p];
+            }
+            fr -= -x[0] * v[0] - x[1] * v[1] - x[2] * v[2] - x[3] * v[3];
+            fi -= x[0] * v[1] - x[1] * v[0] + x[2] * v[3] - x[3] * v[2];
+        }
+    }
+    const double d = dfac[it * N + p];
+    field[(2 * it) * N + p] = fr * d;
+    field[(2 * it + 1) * N + p] = fi * d;
+}
+
+// T(lambda) of every block (dense, leading dimension ld): the HFB matrix of
+// hfb_block_eval (mocca_hfb.hip) without the warm-start products.
+__global__ void k_hl_build_T(HfbK k, const double *__restrict__ E,
+                             const double *__restrict__ delta, const double *__restrict__ lam,
+                             int ld, double *__restrict__ Tm) {
+    const int b = blockIdx.y;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int i = e / n, j = e % n;
+    const double *dl = delta + bk.off;
+    const double l = lam[bk.it];
+    double v = 0.0;
+    if (i < h && j < h) v = i == j ? E[k.state[bk.first + i]] - l : 0.0;
+    else if (i >= h && j >= h) v = i == j ? -(E[k.state[bk.first + i - h]] - l) : 0.0;
+    else if (i < h) v = dl[i * n + h + (j - h)];
+    else v = dl[j * n + h + (i - h)];
+    Tm[(long)b * ld * ld + (long)i * ld + j] = v;
+}
+
+// The next evaluation's matrix in the current eigenbasis:
+// A = diag(w) - delta_it C (T changes by -Sigma_z dlambda between
+// evaluations of one Fermi solve).
+__global__ void k_hl_shift_A(HfbK k, const double *__restrict__ w,
+                             const double *__restrict__ C, const double *__restrict__ nst,
+                             const int *__restrict__ active, int ld, double *__restrict__ A) {
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    const HfbBlockK &bk = k.blk[b];
+    const int n = 2 * bk.h;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int i = e / n, j = e % n;
+    const double dl = nst[bk.it * kHlNewton + 6];
+    const long o = (long)b * ld * ld + (long)i * ld + j;
+    A[o] = (i == j ? w[(long)b * ld + i] : 0.0) - dl * C[o];
+}
+
+// Particle number of each block and its derivative: ranks of the
+// eigenvalues (descending, ties by index; the h largest are the
+// quasiparticles), N_b = 2 sum_{k+} sum_a V[h+a][k]^2,
+// dN_b = 2 sum_{k+} sum_{l-} C_kl^2 / (w_k - w_l).
+__global__ __launch_bounds__(256) void k_hl_number(HfbK k, const double *__restrict__ V,
+                                                   const double *__restrict__ C,
+                                                   const double *__restrict__ w, int ld,
+                                                   const int *__restrict__ active,
+                                                   int *__restrict__ rank,
+                                                   double *__restrict__ num) {
+    __shared__ double sh[4];
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (!active[b]) return;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const double *wb = w + (long)b * ld;
+    const double *Vb = V + (long)b * ld * ld, *Cb = C + (long)b * ld * ld;
+    int *rb = rank + (long)b * ld;
+    for (int i = tid; i < n; i += nt) {
+        const double wi = wb[i];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+            const double wj = wb[j];
+            if (wj > wi || (wj == wi && j < i)) ++r;
+        }
+        rb[i] = r;
+    }
+    __syncthreads();
+    double nn = 0, dn = 0;
+    for (int e = tid; e < n * n; e += nt) {
+        const int a = e / n, q = e % n;
+        if (rb[q] >= h) continue;
+        if (a < h) {
+            const double v = Vb[(long)(h + a) * ld + q];
+            nn += v * v;
+        }
+        const int l = a;   // C[q][l], q positive, l negative
+        if (rb[l] < h) continue;
+        const double c = Cb[(long)q * ld + l];
+        dn += c * c / (wb[q] - wb[l]);
+    }
+    nn = hl_block_sum(nn, sh);
+    dn = hl_block_sum(dn, sh);
+    if (tid == 0) {
+        num[2 * b] = 2.0 * nn;
+        num[2 * b + 1] = 2.0 * dn;
+    }
+}
+
+// Newton step on the Fermi energy of each isospin (thread it).  Record nst
+// [it][kHlNewton] = lambda, lo, hi, has_lo, has_hi, converged, delta, iter.
+__global__ void k_hl_newton(HfbK k, const double *__restrict__ num, double *__restrict__ nst,
+                            int *__restrict__ active, double *__restrict__ fermi,
+                            int *__restrict__ status) {
+    const int it = threadIdx.x;
+    if (it >= 2) return;
+    double *s = nst + it * kHlNewton;
+    if (s[5] != 0.0) return;
+    double nn = 0, dn = 0;
+    bool any = false;
+    for (int b = 0; b < k.nblk; ++b)
+        if (k.blk[b].it == it) {
+            nn += num[2 * b];
+            dn += num[2 * b + 1];
+            any = true;
+        }
+    if (!any) { s[5] = 1.0; return; }
+    const double f = nn - k.particles[it], lam = s[0];
+    if (f != f || dn != dn) {
+        atomicOr(status, 2);
+        s[5] = 1.0;
+        return;
+    }
+    if (fabs(f) < k.prec) {
+        s[5] = 1.0;
+        fermi[it] = lam;
+        for (int b = 0; b < k.nblk; ++b)
+            if (k.blk[b].it == it) active[b] = 0;
+        return;
+    }
+    if (f < 0) { s[1] = lam; s[3] = 1.0; } else { s[2] = lam; s[4] = 1.0; }
+    double x = dn > 1e-14 ? lam - f / dn : lam + (f < 0 ? 1.0 : -1.0);
+    const bool blo = s[3] != 0.0, bhi = s[4] != 0.0;
+    if (blo && bhi) {
+        if (!(x > s[1] && x < s[2])) x = 0.5 * (s[1] + s[2]);
+    } else if (f < 0) {
+        if (!(x > lam)) x = lam + 1.0;
+        x = fmin(x, lam + 5.0);
+    } else {
+        if (!(x < lam)) x = lam - 1.0;
+        x = fmax(x, lam - 5.0);
+    }
+    s[6] = x - lam;
+    s[0] = x;
+    s[7] += 1.0;
+    fermi[it] = x;
+    if (s[7] >= k.iters) {
+        // HFBIter evaluations without reaching FermiPrec: keep the last
+        // evaluation's rho and kappa, as the reference does
+        s[5] = 1.0;
+        s[0] = lam;
+        fermi[it] = lam;
+        atomicOr(status, 1);
+        for (int b = 0; b < k.nblk; ++b)
+            if (k.blk[b].it == it) active[b] = 0;
+    }
+}
+
+// The reference's Broyden (secant) iteration on the Fermi energy of each
+// isospin (thread it), HFBFindFermiEnergyBroyden (HFB.f90:1849-2161) without
+// Lipkin-Nogami, as an event-driven machine: called after each evaluation
+// with the particle number there, it sets the next evaluation point.  The
+// evaluation points, their order and the stopping rule are the reference's:
+// F, F + 0.01, then F += 0.5 * (-N / J) with the Broyden update of J after
+// iteration 1 and the 0.99 ceiling of 1/J, until |N - Z| < FermiPrec or
+// HFBIter iterations; the last evaluation's rho and kappa are kept.
+// Record nst[it][kHlNewton]: 0 lambda of the current evaluation, 1 FH,
+// 2 J, 3 upd, 4 norm, 5 done, 6 next lambda - lambda, 7 iteration,
+// 8 N - Z at F, 9 phase (0: F evaluated, 1: FH evaluated, 2: loop),
+// 10 converged, 11 F.
+__global__ void k_hl_broyden(HfbK k, const double *__restrict__ num, double *__restrict__ nst,
+                             int *__restrict__ active, double *__restrict__ fermi,
+                             int *__restrict__ status) {
+    const int it = threadIdx.x;
+    if (it >= 2) return;
+    double *s = nst + it * kHlNewton;
+    if (s[5] != 0.0) return;
+    double nn = 0;
+    bool any = false;
+    for (int b = 0; b < k.nblk; ++b)
+        if (k.blk[b].it == it) {
+            nn += num[2 * b];
+            any = true;
+        }
+    auto finish = [&](bool unconverged) {
+        s[5] = 1.0;
+        fermi[it] = s[11];
+        if (unconverged) atomicOr(status, 1);
+        for (int b = 0; b < k.nblk; ++b)
+            if (k.blk[b].it == it) active[b] = 0;
+    };
+    if (!any) { s[5] = 1.0; return; }
+    const double Nv = nn - k.particles[it];
+    const double lam = s[0];
+    double F = s[11], FH = s[1], J = s[2], upd = s[3], norm = s[4];
+    int iter = (int)s[7];
+    bool conv = s[10] != 0.0;
+    const int phase = (int)s[9];
+    // the part of a loop iteration before its evaluation
+    auto pre_eval = [&]() {
+        const double det = J;
+        const double inv = fabs(det) < 0.001 ? copysign(0.99, det) : 1.0 / det;
+        if (!conv) {
+            upd = -inv * s[8];
+            norm = upd * upd;
+        }
+        upd = 0.5 * upd;
+        if (!conv) {
+            FH = F;
+            F = F + upd;
+        }
+    };
+    double next;
+    if (phase == 0) {
+        s[8] = Nv;
+        conv = fabs(Nv) < k.prec;
+        next = FH;                      // F + 0.01
+        s[9] = 1.0;
+    } else if (phase == 1) {
+        J = (s[8] - Nv) / (F - FH);
+        iter = 1;
+        pre_eval();
+        next = F;
+        s[9] = 2.0;
+    } else {
+        s[8] = Nv;
+        if (fabs(Nv) < k.prec) conv = true;
+        if (conv) {
+            s[10] = 1.0;
+            finish(false);
+            return;
+        }
+        if (iter > 1) J = J + Nv * upd / norm;
+        if (Nv == Nv + 1.0 || Nv != Nv) {
+            atomicOr(status, 2);
+            finish(false);
+            return;
+        }
+        if (iter >= k.iters) {
+            finish(true);
+            return;
+        }
+        ++iter;
+        pre_eval();
+        next = F;
+    }
+    s[1] = FH;
+    s[2] = J;
+    s[3] = upd;
+    s[4] = norm;
+    s[7] = iter;
+    s[10] = conv ? 1.0 : 0.0;
+    s[11] = F;
+    s[6] = next - lam;
+    s[0] = next;
+}
+
+// rho and kappa of the blocks from the eigenvectors of the h largest
+// eigenvalues (constructRhoHFB, ConstructKappaHFB :3128-3360), in the layout
+// of hfb_block_eval; the quasiparticle energies.
+__global__ void k_hl_rho_kappa(HfbK k, const double *__restrict__ V,
+                               const double *__restrict__ w, const int *__restrict__ rank,
+                               int ld, double *__restrict__ rho, double *__restrict__ kappa,
+                               double *__restrict__ qpe) {
+    const int b = blockIdx.y;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *Vb = V + (long)b * ld * ld, *wb = w + (long)b * ld;
+    const int *rb = rank + (long)b * ld;
+    if (e < n && rb[e] < h) qpe[bk.first + rb[e]] = wb[e];
+    if (e >= n * n) return;
+    const int i = e / n, j = e % n;
+    double v = 0.0, kv = 0.0;
+    if (i < h && j < h) {
+        for (int q = 0; q < n; ++q)
+            if (rb[q] < h) v += Vb[(long)(h + i) * ld + q] * Vb[(long)(h + j) * ld + q];
+    } else if (i >= h && j >= h) {
+        for (int q = 0; q < n; ++q)
+            if (rb[q] < h) v += Vb[(long)i * ld + q] * Vb[(long)j * ld + q];
+    } else if (i >= h) {
+        for (int q = 0; q < n; ++q)
+            if (rb[q] < h) kv += Vb[(long)i * ld + q] * Vb[(long)j * ld + q];
+    } else {
+        for (int q = 0; q < n; ++q)
+            if (rb[q] < h) kv -= Vb[(long)j * ld + q] * Vb[(long)i * ld + q];
+    }
+    rho[bk.off + e] = v;
+    kappa[bk.off + e] = kv;
+}
+
+// rho_h (the signature +1 part of rho) of every block into A (h x h,
+// leading dimension ld) for the canonical-basis diagonalisation.
+__global__ void k_hl_can_prep(HfbK k, const double *__restrict__ rho, int ld,
+                              double *__restrict__ A, int *__restrict__ hlist) {
+    const int b = blockIdx.y;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0 && e < k.nblk) hlist[e] = k.blk[e].h;
+    if (e >= h * h) return;
+    const int i = e / h, j = e % h;
+    A[(long)b * ld * ld + (long)i * ld + j] = rho[bk.off + i * n + j];
+}
+
+// Canonical transforms and occupations from the eigenvectors of rho_h
+// (k_hfb_canonical's rules): columns in ascending occupation, entries below
+// 1e-11 set to zero, occupations 2 x eigenvalue clamped to [0, 2]; the HF
+// states' occupations 2 rho_ii; the transforms also into the apply matrices
+// Tap (leading dimension hmax) with the block lists.
+__global__ __launch_bounds__(256) void k_hl_can_finish(
+    HfbK k, const double *__restrict__ V, const double *__restrict__ w, int ld,
+    const double *__restrict__ rho, int hmax, double *__restrict__ tcan,
+    double *__restrict__ can_occ, double *__restrict__ hf_occ, double *__restrict__ Tap,
+    int *__restrict__ list, int *__restrict__ status) {
+    __shared__ int rank[256];
+    const HfbBlockK &bk = k.blk[blockIdx.x];
+    const int h = bk.h, n = 2 * h, tid = threadIdx.x, nt = blockDim.x;
+    const double *Vb = V + (long)blockIdx.x * ld * ld, *wb = w + (long)blockIdx.x * ld;
+    for (int i = tid; i < h; i += nt) {
+        const double wi = wb[i];
+        int r = 0;
+        for (int j = 0; j < h; ++j) {
+            const double wj = wb[j];
+            if (wj < wi || (wj == wi && j < i)) ++r;
+        }
+        rank[i] = r;
+    }
+    __syncthreads();
+    double *tb = tcan + bk.toff;
+    double *Tb = Tap + (long)blockIdx.x * hmax * hmax;
+    for (int e = tid; e < hmax * hmax; e += nt) Tb[e] = 0.0;
+    __syncthreads();
+    for (int e = tid; e < h * h; e += nt) {
+        const int j = e / h, c = e % h;
+        const double v = Vb[(long)j * ld + c];
+        const double t = fabs(v) < 1e-11 ? 0.0 : v;
+        tb[j * h + rank[c]] = t;
+        Tb[j * hmax + rank[c]] = t;
+    }
+    for (int c = tid; c < h; c += nt) {
+        double x = wb[c];
+        if (x - 1.0 > 1e-5) atomicOr(status, 4);
+        if (x < 0.0) x = 0.0;
+        if (x > 1.0) x = 1.0;
+        can_occ[k.state[bk.first + rank[c]]] = 2.0 * x;
+        hf_occ[k.state[bk.first + c]] = 2.0 * rho[bk.off + c * n + c];
+    }
+    for (int i = tid; i < hmax; i += nt)
+        list[blockIdx.x * hmax + i] = k.state[bk.first + (i < h ? i : 0)];
+}
+
+__global__ void k_hl_newton_init(const double *__restrict__ fermi, double *__restrict__ nst,
+                                 int nblk, int *__restrict__ active) {
+    const int t = threadIdx.x;
+    if (t < 2) {
+        double *s = nst + t * kHlNewton;
+        for (int q = 0; q < kHlNewton; ++q) s[q] = 0.0;
+        s[0] = fermi[t];          // first evaluation at the last Fermi energy
+        s[11] = fermi[t];         // F of the Broyden iteration
+        s[1] = fermi[t] + 0.01;   // FH: FermiHistory = Fermi + 0.01
+    }
+    if (t < nblk) active[t] = 1;
+}
+
+// ===========================================================================
+// Host side
+namespace detail {
+
+namespace {
+
+template <typename T>
+void launch_pair_gram(const HfbK &k, const T *psi, long N, const double *field, double *part,
+                      int nsplit, int nt, int nblk, hipStream_t st) {
+#define MOCCA_PG(NI, NJ)                                                                    \
+    case NI:                                                                                \
+        hipLaunchKernelGGL((k_hl_pair_gram<T, NI, NJ>), dim3(nsplit, nblk, (nt + NJ - 1) / NJ), \
+                           dim3(64), 0, st, k, psi, N, field, part, nsplit, nt);           \
+        break;
+    switch (nt) {
+        MOCCA_PG(1, 1) MOCCA_PG(2, 2) MOCCA_PG(3, 3) MOCCA_PG(4, 4)
+        MOCCA_PG(5, 5) MOCCA_PG(6, 4) MOCCA_PG(7, 4) MOCCA_PG(8, 3)
+        default: break;
+    }
+#undef MOCCA_PG
+}
+
+int pair_gram_groups(int nt) {
+    static const int nj[9] = {1, 1, 2, 3, 4, 5, 4, 4, 3};
+    return (nt + nj[nt] - 1) / nj[nt];
+}
+
+}  // namespace
+
+bool hfb_large_fits(int hmax) { return hmax >= 1 && (hmax + 15) / 16 <= 8; }
+
+size_t hfb_large_jacobi_lds(int n) {
+    const int m = n + (n & 1), M2 = m / 2;
+    return sizeof(double) * ((size_t)n * (n + 1) / 2 + (size_t)M2 * 3 + (M2 & 1) + 16);
+}
+
+int hfb_large_setup(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    const HfbK &k = p.k;
+    int nmax = 0;
+    for (int b = 0; b < k.nblk; ++b) nmax = std::max(nmax, 2 * k.blk[b].h);
+    p.nmax = nmax;
+    const int ld = nmax, hmax = p.hmax;
+    const size_t mat = (size_t)k.nblk * ld * ld;
+    const int m = nmax + (nmax & 1);
+    p.lmax = (long)kJacMaxSweeps * (m - 1) * (m / 2);
+    const int ntile = (hmax + 15) / 16;
+    p.lsplit = std::max(1, kGramWaves / std::max(1, k.nblk * pair_gram_groups(ntile)));
+    p.lsplit = (int)std::min<long>(p.lsplit, std::max<long>(1, ((long)h->N + 3) / 4 / 8));
+    int rc;
+    if ((rc = dalloc(h, &p.lT, mat * sizeof(double))) ||
+        (rc = dalloc(h, &p.lV, mat * sizeof(double))) ||
+        (rc = dalloc(h, &p.lA, mat * sizeof(double))) ||
+        (rc = dalloc(h, &p.lW, mat * sizeof(double))) ||
+        (rc = dalloc(h, &p.lC, mat * sizeof(double))) ||
+        (rc = dalloc(h, &p.lVc, mat * sizeof(double))) ||
+        (rc = dalloc(h, &p.lw, (size_t)k.nblk * ld * sizeof(double))) ||
+        (rc = dalloc(h, &p.lwc, (size_t)k.nblk * ld * sizeof(double))) ||
+        (rc = dalloc(h, &p.lrank, (size_t)k.nblk * ld * sizeof(int))) ||
+        (rc = dalloc(h, &p.llog, (size_t)k.nblk * p.lmax * sizeof(double2))) ||
+        (rc = dalloc(h, &p.lrounds, 2 * (size_t)k.nblk * sizeof(int))) ||
+        (rc = dalloc(h, &p.lnum, 2 * (size_t)k.nblk * sizeof(double))) ||
+        (rc = dalloc(h, &p.lnst, 2 * kHlNewton * sizeof(double))) ||
+        (rc = dalloc(h, &p.lactive, (size_t)k.nblk * sizeof(int))) ||
+        (rc = dalloc(h, &p.lnb, 2 * (size_t)k.nblk * sizeof(int))) ||
+        (rc = dalloc(h, &p.lhb, (size_t)k.nblk * sizeof(int))) ||
+        (rc = dalloc(h, &p.llist, (size_t)k.nblk * hmax * sizeof(int))) ||
+        (rc = dalloc(h, &p.lTap, (size_t)k.nblk * hmax * hmax * sizeof(double))) ||
+        (rc = dalloc(h, &p.lgpart, (size_t)k.nblk * p.lsplit * ntile * ntile * 256 *
+                                       sizeof(double))))
+        return rc;
+    std::vector<int> nb(2 * k.nblk), hb(k.nblk);
+    for (int b = 0; b < k.nblk; ++b) {
+        nb[b] = 2 * k.blk[b].h;
+        nb[k.nblk + b] = k.blk[b].h;
+        hb[b] = k.blk[b].h;
+    }
+    HIPCHK(h, hipMemcpy(p.lnb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(p.lhb, hb.data(), hb.size() * sizeof(int), hipMemcpyHostToDevice));
+    p.lwarm = false;
+    // the in-LDS Jacobi wants the whole device limit
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+        p.lds_limit = (size_t)lim;
+    return 0;
+}
+
+int hfb_large_free(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    void *ptrs[] = {p.lT, p.lV, p.lA, p.lW, p.lC, p.lVc, p.lw, p.lwc, p.lrank, p.llog,
+                    p.lrounds, p.lnum, p.lnst, p.lactive, p.lnb, p.lhb, p.llist, p.lTap,
+                    p.lgpart};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    return 0;
+}
+
+namespace {
+
+// Jacobi + rotation apply on nblk matrices of sizes nb[] (A, leading
+// dimension ld) with the basis V (identity first when ident).
+int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int *nb, int nmax,
+               const int *active, int *rounds, bool ident) {
+    Hfb &p = h->hfb;
+    const int nblk = p.k.nblk, ld = p.nmax;
+    const long ms = (long)ld * ld;
+    int rc;
+    if (nmax <= kJacLdsMaxN && hfb_large_jacobi_lds(nmax) <= p.lds_limit) {
+        const size_t shm = hfb_large_jacobi_lds(nmax);
+        // (the limit for the largest in-LDS matrix, once)
+        const size_t top = std::min(hfb_large_jacobi_lds(kJacLdsMaxN), p.lds_limit);
+        rc = run(h, "hfb_jacobi", [&] {
+            static const bool ok = require_lds((const void *)k_lj_jacobi<false>, top, "k_lj_jacobi");
+            if (!ok) require_lds((const void *)k_lj_jacobi<false>, top, "k_lj_jacobi");
+            hipLaunchKernelGGL((k_lj_jacobi<false>), dim3(nblk), dim3(kJacThreads), shm, h->st, A,
+                               ld, ms, nb, active, nullptr, w, p.llog, p.lmax, rounds,
+                               kJacMaxSweeps, p.status);
+        });
+    } else {
+        const int m = nmax + (nmax & 1);
+        const size_t shm = sizeof(double) * (3 * (size_t)(m / 2) + 1 + 16);
+        rc = run(h, "hfb_jacobi", [&] {
+            hipLaunchKernelGGL((k_lj_jacobi<true>), dim3(nblk), dim3(kJacThreads), shm, h->st, A,
+                               ld, ms, nb, active, p.lW, w, p.llog, p.lmax, rounds,
+                               kJacMaxSweeps, p.status);
+        });
+    }
+    if (rc) return rc;
+    return run(h, "hfb_jacobi_apply", [&] {
+        const size_t shm = sizeof(double) * kApplyRows * nmax;
+        hipLaunchKernelGGL(k_lj_apply, dim3((nmax + kApplyRows - 1) / kApplyRows, nblk),
+                           dim3(256), shm, h->st, V, ld, ms, nb, active, p.llog, p.lmax,
+                           rounds, ident ? 1 : 0);
+    });
+}
+
+}  // namespace
+
+template <typename T>
+int hfb_large_solve_impl(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    const HfbK &k = p.k;
+    const long N = h->N;
+    const int M = h->M, nblk = k.nblk, ld = p.nmax, hmax = p.hmax;
+    const long ms = (long)ld * ld;
+    const int mpad = (hmax + 15) / 16 * 16, ntile = (hmax + 15) / 16;
+    int rc;
+    // SolvePairing :800-806: old rho, kappa take the current ones while the
+    // old rho is all zero (the first call)
+    if ((rc = run(h, "hfb_mix", [&] { hfb_launch_mix(h, 1); }))) return rc;
+    // cutoffs, density factor, pairing field (phi in the canonical states'
+    // buffer, which this step rewrites at its end)
+    rc = run(h, "hfb_field", [&] {
+        hfb_launch_cutoffs(h);
+        hipLaunchKernelGGL(k_hl_field_w, dim3(nblk), dim3(256), 0, h->st, p.k, p.kappa, p.pc,
+                           hmax, p.lTap, p.llist, p.lhb);
+        launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist, p.lhb,
+                                   nblk, hmax, mpad, p.lTap, h->st);
+        hipLaunchKernelGGL(k_hl_field_sum<T>, dim3((N + 255) / 256, 2), dim3(256), 0, h->st,
+                           p.k, (const T *)h->b.psi, (const T *)p.can_psi, N, p.dfac, p.field);
+    });
+    if (rc) return rc;
+    // gaps
+    rc = run(h, "hfb_gaps", [&] {
+        launch_pair_gram<T>(p.k, (const T *)h->b.psi, N, p.field, p.lgpart, p.lsplit, ntile,
+                            nblk, h->st);
+        hipLaunchKernelGGL(k_hl_gaps_reduce, dim3((hmax * hmax + 255) / 256, nblk), dim3(256),
+                           0, h->st, p.k, p.lgpart, p.lsplit, ntile, p.pc, h->mesh.dv, p.delta);
+    });
+    if (rc) return rc;
+    // Fermi energies: Newton evaluations until both isospins meet FermiPrec
+    const dim3 gmat((p.nmax + 15) / 16, (p.nmax + 15) / 16, nblk);
+    const int nel = p.nmax * p.nmax;
+    rc = run(h, "hfb_fermi", [&] {
+        hipLaunchKernelGGL(k_hl_newton_init, dim3(1), dim3(64), 0, h->st, p.fermi, p.lnst,
+                           nblk, p.lactive);
+        hipLaunchKernelGGL(k_hl_build_T, dim3((nel + 255) / 256, nblk), dim3(256), 0, h->st,
+                           p.k, h->b.E, p.delta, p.fermi, ld, p.lT);
+        if (p.lwarm) {
+            // A = V^T T V in the eigenbasis of the last call
+            hipLaunchKernelGGL((k_lj_gemm<false, false>), gmat, dim3(256), 0, h->st, p.lT, p.lV,
+                               p.lW, ld, ms, p.lnb, nullptr, nullptr);
+            hipLaunchKernelGGL((k_lj_gemm<true, false>), gmat, dim3(256), 0, h->st, p.lV, p.lW,
+                               p.lA, ld, ms, p.lnb, nullptr, nullptr);
+        }
+    });
+    if (rc) return rc;
+    const double *A0 = p.lwarm ? p.lA : p.lT;
+    int conv[2] = {0, 0};
+    // (Broyden: two evaluations before its HFBIter iterations)
+    for (int ev = 0; ev < std::max(1, k.iters) + 3; ++ev) {
+        if ((rc = jacobi_run(h, ev == 0 ? A0 : p.lA, p.lV, p.lw, p.lnb, p.nmax, p.lactive,
+                             p.lrounds, ev == 0 && !p.lwarm)))
+            return rc;
+        rc = run(h, "hfb_fermi", [&] {
+            hipLaunchKernelGGL((k_lj_gemm<true, true>), gmat, dim3(256), 0, h->st, p.lV, p.lV,
+                               p.lC, ld, ms, p.lnb, p.lhb, p.lactive);
+            hipLaunchKernelGGL(k_hl_number, dim3(nblk), dim3(256), 0, h->st, p.k, p.lV, p.lC,
+                               p.lw, ld, p.lactive, p.lrank, p.lnum);
+            if (p.newton)
+                hipLaunchKernelGGL(k_hl_newton, dim3(1), dim3(64), 0, h->st, p.k, p.lnum,
+                                   p.lnst, p.lactive, p.fermi, p.status);
+            else
+                hipLaunchKernelGGL(k_hl_broyden, dim3(1), dim3(64), 0, h->st, p.k, p.lnum,
+                                   p.lnst, p.lactive, p.fermi, p.status);
+            hipLaunchKernelGGL(k_hl_shift_A, dim3((nel + 255) / 256, nblk), dim3(256), 0,
+                               h->st, p.k, p.lw, p.lC, p.lnst, p.lactive, ld, p.lA);
+        });
+        if (rc) return rc;
+        double st[2 * kHlNewton];
+        HIPCHK(h, hipMemcpyAsync(st, p.lnst, sizeof st, hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        conv[0] = st[5] != 0.0;
+        conv[1] = st[kHlNewton + 5] != 0.0;
+        if (conv[0] && conv[1]) break;
+    }
+    p.lwarm = true;
+    // rho, kappa, quasiparticle energies; canonical basis and states; mixing
+    rc = run(h, "hfb_canonical", [&] {
+        hipLaunchKernelGGL(k_hl_rho_kappa, dim3((nel + 255) / 256, nblk), dim3(256), 0, h->st,
+                           p.k, p.lV, p.lw, p.lrank, ld, p.rho, p.kappa, p.qpe);
+        hipLaunchKernelGGL(k_hl_can_prep, dim3((hmax * hmax + 255) / 256, nblk), dim3(256), 0,
+                           h->st, p.k, p.rho, ld, p.lT, p.lhb);
+    });
+    if (rc) return rc;
+    // (the block sizes h are the second half of lnb)
+    if ((rc = jacobi_run(h, p.lT, p.lVc, p.lwc, p.lnb + nblk, hmax, nullptr,
+                         p.lrounds + nblk, true)))
+        return rc;
+    return run(h, "hfb_canonical", [&] {
+        hipLaunchKernelGGL(k_hl_can_finish, dim3(nblk), dim3(256), 0, h->st, p.k, p.lVc, p.lwc,
+                           ld, p.rho, hmax, p.tcan, p.can_occ, h->b.occ, p.lTap, p.llist,
+                           p.status);
+        launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist, p.lhb,
+                                   nblk, hmax, mpad, p.lTap, h->st);
+        hfb_launch_mix(h, 0);
+    });
+}
+
+int hfb_large_solve_any(mocca_solve *h) {
+    return h->prec == MOCCA_FP64 ? hfb_large_solve_impl<double>(h)
+                                 : hfb_large_solve_impl<float>(h);
+}
+
+}  // namespace detail
+}  // namespace mocca

@@ -140,6 +140,28 @@ struct Hfb {
     double *work = nullptr;
     int *status = nullptr;
     int *tables = nullptr;         // [3][M] state, slot_block, slot_index
+    // blocks above the in-LDS solver (mocca_hfb_large.hip): per block, n x n
+    // matrices with leading dimension nmax (the largest 2h)
+    bool large = false;
+    bool newton = false;           // Newton's method instead of the Broyden iteration
+    int nmax = 0, lsplit = 1;
+    long lmax = 0;                 // rotation-log entries per block
+    size_t lds_limit = 0;
+    bool lwarm = false;            // lV holds the eigenbasis of the last call
+    double *lT = nullptr, *lV = nullptr, *lA = nullptr, *lW = nullptr, *lC = nullptr;
+    double *lVc = nullptr;         // canonical-basis eigenvectors
+    double *lw = nullptr, *lwc = nullptr;   // eigenvalues [nblk][nmax]
+    int *lrank = nullptr;          // [nblk][nmax] eigenvalue ranks (descending)
+    double2 *llog = nullptr;       // [nblk][lmax] Jacobi rotations
+    int *lrounds = nullptr;        // [2][nblk] rounds logged (HFB, canonical)
+    double *lnum = nullptr;        // [nblk][2] particle number, derivative
+    double *lnst = nullptr;        // [2][kHlNewton] Newton records
+    int *lactive = nullptr;        // [nblk] block still evaluated
+    int *lnb = nullptr;            // [2][nblk] 2h, h
+    int *lhb = nullptr;            // [nblk] h (apply block sizes)
+    int *llist = nullptr;          // [nblk][hmax] the block's states (apply lists)
+    double *lTap = nullptr;        // [nblk][hmax][hmax] apply matrices
+    double *lgpart = nullptr;      // pair-Gram partial sums
 };
 
 }  // namespace detail
@@ -281,6 +303,17 @@ int hfb_free(mocca_solve *h);
 int hfb_solve_any(mocca_solve *h, bool first);
 int hfb_energy_impl(mocca_solve *h, double *row);
 int hfb_check_status(mocca_solve *h);
+// Kernels of mocca_hfb.hip launched by the large-block path: the rho/kappa
+// history step of SolvePairing (first = 1) or the HFBMix mixing (first = 0),
+// and the cutoff factors with the density factor.
+void hfb_launch_mix(mocca_solve *h, int first);
+void hfb_launch_cutoffs(mocca_solve *h);
+// The large-block path (mocca_hfb_large.hip): whether hmax states per block
+// fit it, its buffers, one SolvePairing step.
+bool hfb_large_fits(int hmax);
+int hfb_large_setup(mocca_solve *h);
+int hfb_large_free(mocca_solve *h);
+int hfb_large_solve_any(mocca_solve *h);
 constexpr int kRowLen = 47;   // energy row: 43 of CompEnergy + pairing energy(2) + Fermi(2)
 // AlternateStep (constraints of type 2) on the device; alternate_prepare
 // allocates the buffer of the states before the step ahead of time.

@@ -208,6 +208,13 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
                           int nblk, int max_m, int mpad_max, const double *Tm,
                           hipStream_t st);
 
+// out[list[b][l]] = sum_k src[list[b][k]] Tm[b][k][l] for a full (dense) Tm,
+// on the matrix cores as the apply above; blocks of up to 192 states.
+template <typename T>
+void launch_dense_apply_mfma(const T *src, T *dst, long K4, const int *list, const int *blk_m,
+                             int nblk, int max_m, int mpad_max, const double *Tm,
+                             hipStream_t st);
+
 // Gram matrices on the FP64 matrix cores (v_mfma_f64_16x16x4f64) for blocks
 // of up to kMfmaMaxRows states (one wave holds all upper 16x16 tiles of up
 // to 128 states; 129-160 states take three waves per K slice).
@@ -315,6 +322,7 @@ void launch_alternate(const ConstraintK *con, int nc, const double *Y, const dou
 // (isospin, parity) with h states and their h partners.
 constexpr double kHfbNumCut = 1e-12;   // HFBNumCut (HFB.f90)
 constexpr int kHfbMaxN = 64;           // block size 2h of the in-LDS solvers
+constexpr int kHlNewton = 12;          // Fermi-solver record of the large-block path
 struct HfbBlockK {
     int it, P, h;
     int first;      // index of the block's first state in the state table

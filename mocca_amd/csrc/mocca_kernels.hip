@@ -1491,7 +1491,9 @@ __device__ __forceinline__ mfma_d4 mma_apply(double a, double b, mfma_d4 c) {
 __device__ __forceinline__ mfma_f4 mma_apply(float a, float b, mfma_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
-template <typename T, int NT, int NW>
+// DENSE: T is a full matrix (the canonical transformation and the pairing
+// field's kappa combinations of the HFB step), no chunk is skipped.
+template <typename T, int NT, int NW, bool DENSE = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
     NW == 4 && NT <= 8 ? 2 : (NW == 8 ? 2 : 1)))) void k_gs_apply_mfma(const T *__restrict__ src,
                                                        T *__restrict__ dst, long K4,
@@ -1570,7 +1572,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
             for (int E = 0; E < kApplyNE; ++E) bv[E] = P[row * PS + 16 * E];
 #pragma unroll
             for (int L = 0; L < NT; ++L) {
-                if (L < c) continue;
+                if (!DENSE && L < c) continue;
                 const T av = Tl[row * PT + 16 * L];
 #pragma unroll
                 for (int E = 0; E < kApplyNE; ++E) acc[L][E] = mma_apply(av, bv[E], acc[L][E]);
@@ -1595,21 +1597,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
         }
 }
 
-template <typename T>
-void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
-                          int nblk, int max_m, int mpad_max, const double *Tm,
-                          hipStream_t st) {
+template <typename T, bool DENSE>
+static void apply_mfma_launch(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
+                              int nblk, int max_m, int mpad_max, const double *Tm,
+                              hipStream_t st) {
     // (the double-buffered LDS tiles exceed the default 64 KiB)
 #define MOCCA_APPLY_NT(K, W)                                                                \
     case K: {                                                                               \
         constexpr size_t shm = gs_apply_mfma_lds<T, K, W>();                                \
-        static const bool lds_set =                                                         \
-            require_lds((const void *)k_gs_apply_mfma<T, K, W>, shm, "k_gs_apply_mfma");   \
-        if (!lds_set)                                                                       \
-            require_lds((const void *)k_gs_apply_mfma<T, K, W>, shm, "k_gs_apply_mfma");   \
+        const void *fn = (const void *)k_gs_apply_mfma<T, K, W, DENSE>;                     \
+        static const bool lds_set = require_lds(fn, shm, "k_gs_apply_mfma");                \
+        if (!lds_set) require_lds(fn, shm, "k_gs_apply_mfma");                              \
         const dim3 g((K4 + apply_ew<W>() - 1) / apply_ew<W>(), nblk);                       \
-        hipLaunchKernelGGL((k_gs_apply_mfma<T, K, W>), g, dim3(64 * W), shm, st, src, dst,  \
-                           K4, perm, blk_m, max_m, Tm);                                     \
+        hipLaunchKernelGGL((k_gs_apply_mfma<T, K, W, DENSE>), g, dim3(64 * W), shm, st, src, \
+                           dst, K4, perm, blk_m, max_m, Tm);                                \
         break;                                                                              \
     }
     // eight waves from 5 row tiles on; 11-12 row tiles do not fit two waves
@@ -1621,6 +1622,20 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
         default: return;
     }
 #undef MOCCA_APPLY_NT
+}
+
+template <typename T>
+void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
+                          int nblk, int max_m, int mpad_max, const double *Tm,
+                          hipStream_t st) {
+    apply_mfma_launch<T, false>(src, dst, K4, perm, blk_m, nblk, max_m, mpad_max, Tm, st);
+}
+
+template <typename T>
+void launch_dense_apply_mfma(const T *src, T *dst, long K4, const int *list, const int *blk_m,
+                             int nblk, int max_m, int mpad_max, const double *Tm,
+                             hipStream_t st) {
+    apply_mfma_launch<T, true>(src, dst, K4, list, blk_m, nblk, max_m, mpad_max, Tm, st);
 }
 
 // G[b][i][j] = G[b][j][i] = dv sum_split of the tile element holding
@@ -1707,6 +1722,9 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
                                  int, int, double *, int, hipStream_t);                  \
     template void launch_gs_apply_mfma<T>(const T *, T *, long, const int *, const int *, \
                                           int, int, int, const double *, hipStream_t);    \
+    template void launch_dense_apply_mfma<T>(const T *, T *, long, const int *,           \
+                                             const int *, int, int, int, const double *,  \
+                                             hipStream_t);                                \
     template void launch_gs_apply<T>(const T *, T *, T *, long, const int *,             \
                                      const int *, int, int, const double *, int,         \
                                      hipStream_t);                                       \

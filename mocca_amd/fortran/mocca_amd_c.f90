@@ -51,7 +51,7 @@ module mocca_amd_c
   end type
 
   type, bind(C) :: mocca_pairing
-    integer(c_int) :: type, guess_kappa, iterations, pad_
+    integer(c_int) :: type, guess_kappa, iterations, solver
     real(c_double) :: strength(2), alpha(2), cut(2), mu(2)
     real(c_double) :: rhosat, hfb_mix, fermi_prec
     real(c_double) :: fermi(2)

@@ -73,6 +73,37 @@ def seeded_states(rng, nx, ny, nz, dx, qns, b=1.7, noise=1e-3):
     return out
 
 
+def ho_basis_states(rng, n, dx, qns, b=1.5, noise=1e-4):
+    """Harmonic-oscillator eigenfunctions H_a(x/b) H_b(y/b) H_c(z/b)
+    exp(-r^2/2b^2), shell by shell (a + b + c ascending, then
+    lexicographically), one per state: the k-th state of an (isospin, parity)
+    takes the k-th function of that parity, in the one spinor component whose
+    reflection signs match the function's per-axis parities (for P = S = TS =
+    +1 or P = -1: component 0 even-even, 1 odd-odd, 2 odd-even, 3 even-odd in
+    x, y), plus uniform noise."""
+    X, Y, Z = mesh(n, n, n, dx)
+    env = np.exp(-(X**2 + Y**2 + Z**2) / (2 * b * b))
+    funcs = {1: [], -1: []}
+    for N in range(16):
+        for a in range(N + 1):
+            for bb in range(N + 1 - a):
+                c = N - a - bb
+                funcs[1 if N % 2 == 0 else -1].append((a, bb, c))
+    comp = {(0, 0): 0, (1, 1): 1, (1, 0): 2, (0, 1): 3}
+    count = {}
+    out = np.zeros((len(qns), 4, n, n, n))
+    for i, (iso, P, S, TS) in enumerate(qns):
+        k = count.get((iso, P), 0)
+        count[(iso, P)] = k + 1
+        a, bb, c = funcs[P][k]
+        f = hermite(a, X / b) * hermite(bb, Y / b) * hermite(c, Z / b) * env
+        f /= np.sqrt(2.0 ** (a + bb + c) * np.prod(np.arange(1, a + 1)) *
+                     np.prod(np.arange(1, bb + 1)) * np.prod(np.arange(1, c + 1)))
+        out[i, comp[(a % 2, bb % 2)]] = f
+        out[i] += noise * rng.uniform(-1, 1, size=out[i].shape)
+    return out
+
+
 def synthetic_fields(rng, nx, ny, nz, dx, R=4.0, a=0.6):
     X, Y, Z = mesh(nx, ny, nz, dx)
     r = np.sqrt(X**2 + Y**2 + Z**2)
@@ -323,6 +354,14 @@ SCF_CASES = {
                     dict(mixing=2, constraints=["l=2, m=0, Constraint=50, iteration=10"],
                          pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
                                  'GuessKappa=.true.')),
+    # HFB blocks of 100 states (200 x 200 HFB matrices, the block size of the
+    # 40^3 / 400-state workload): 200 neutron and 40 proton states
+    # (harmonic-oscillator eigenfunctions, shells 0-9) for 50 neutrons and 14
+    # protons (open shells) on an 8^3 octant; the sample's pairing
+    "scfhfb_large": (8, 0.8, True, False, 50, 14, (200, 40), 4, 0,
+                     dict(pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
+                                  'GuessKappa=.true.', start="hf", seeds="ho_basis",
+                          keep_final=False)),
 }
 
 
@@ -335,12 +374,17 @@ def run_scf(name):
     e0 = f0 = None
     seed = sum(map(ord, name))
     rng = np.random.default_rng(seed)
-    pattern = [(1, 1), (-1, 1)] * (per_iso // 2) if trc else \
+    counts = per_iso if isinstance(per_iso, tuple) else (per_iso, per_iso)
+    pattern = [(1, 1), (-1, 1)] * (max(counts) // 2) if trc else \
         [(1, 1), (1, -1), (-1, 1), (-1, -1), (1, 1), (1, -1)]
-    qns = [(iso, P, S, 1) for iso in (-1, 1) for (P, S) in pattern[:per_iso]]
+    qns = [(iso, P, S, 1) for iso, cnt in ((-1, counts[0]), (1, counts[1]))
+           for (P, S) in pattern[:cnt]]
     nwt = len(qns)
     occ = [0.0] * nwt
-    psi = seeded_states(rng, n, n, n, dx, qns)
+    if opts.get("seeds") == "ho_basis":
+        psi = ho_basis_states(rng, n, dx, qns)
+    else:
+        psi = seeded_states(rng, n, n, n, dx, qns)
     # Orthonormalise the seeds per (isospin, parity, signature) block in the
     # octant inner product (dv = 8 dx^3), as a start-up would.
     dv8 = 8 * dx**3
@@ -440,6 +484,12 @@ def run_scf(name):
                 pairing_ddexp=list(pp[4:6]), pairing_cut=list(pp[6:8]),
                 pairing_mu=list(pp[8:10]), rhosat=float(pp[10]), hfb_mix=float(pp[11]),
                 source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")
+    if not opts.get("keep_final", True):
+        # (the large case keeps its inputs and energies only)
+        fin_psi = np.zeros(0)
+        extra = {k: v for k, v in extra.items() if not k.startswith("start_") or
+                 k in ("start_energies", "start_fermi")}
+        f = {k: v for k, v in f.items() if k in ("rho", "vcoul")}
     np.savez_compressed(os.path.join(OUTDIR, name + ".npz"), meta=json.dumps(meta),
                         qn=np.array(qns, np.int32), psi=psi, traj=traj,
                         final_psi=fin_psi, final_eo=fin_eo, coef=coef, consts=rest,
