@@ -46,10 +46,10 @@ namespace mocca {
 
 namespace {
 
-constexpr int kJacThreads = 512;
+constexpr int kJacThreads = 1024;
 constexpr int kJacLdsMaxN = 200;     // packed upper triangle + rotation tables in 160 KiB
 constexpr int kJacMaxSweeps = 24;
-constexpr int kApplyRows = 16;       // basis rows per workgroup of the rotation apply
+constexpr int kApplyRows = 16;       // basis rows per workgroup of the rotation apply (4 waves)
 constexpr int kGramWaves = 1024;     // waves of one pair-Gram launch
 
 typedef double hl_d4 __attribute__((ext_vector_type(4)));   // FP64 MFMA accumulator
@@ -103,41 +103,61 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
     extern __shared__ __align__(16) double lds[];
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (active && !active[b]) return;
-    const int n = nb[b], m = n + (n & 1), M2 = m / 2, npk = n * (n + 1) / 2;
+    const int n = nb[b], m = n + (n & 1), M2 = m / 2;
+    // per round: rotation (c, s) and pair (i, j, ro_i, ro_j) of each pair,
+    // ro_x = the packed offset of row x less x (element (x, y), x <= y, at
+    // ro_x + y)
     double2 *cs = reinterpret_cast<double2 *>(lds);                      // [M2]
-    int2 *pr = reinterpret_cast<int2 *>(cs + M2);                        // [M2]
-    double *sh = reinterpret_cast<double *>(pr + M2 + (M2 & 1));         // [16]
+    int *pij = reinterpret_cast<int *>(cs + M2);                         // [M2] i | j << 16
+    int *pri = pij + M2, *prj = pri + M2;                                // [M2] ro_i, ro_j
+    double *sh = reinterpret_cast<double *>(prj + M2 + (M2 & 1));        // [16]
     double *A = GA ? Ag + b * mstride : sh + 16;                         // packed upper
+    auto ro = [&](int x) { return x * n - (x * (x - 1)) / 2 - x; };
     const double *Ab = Ain + b * mstride;
     for (int q = tid; q < n * n; q += nt) {
         const int i = q / n, j = q % n;
-        if (j >= i) A[pk(i, j, n)] = 0.5 * (Ab[(long)i * ld + j] + Ab[(long)j * ld + i]);
+        if (j >= i) A[ro(i) + j] = 0.5 * (Ab[(long)i * ld + j] + Ab[(long)j * ld + i]);
     }
     __syncthreads();
     double2 *lb = log + b * logstride;
+    // 2x2 blocks (k, l), k <= l, over threads: column pairs u and M2-1-u are
+    // folded into one list of M2+1 blocks (M2/2+... lists of equal length),
+    // GT threads per list
+    const int L2 = (M2 + 1) / 2;
+    const int GT = nt / L2 > 0 ? nt / L2 : 1;
+    const int u = tid / GT, g = tid % GT;
+    const bool worker = u < L2;
+    const int l1 = u, l2 = M2 - 1 - u;
+    const int cnt = worker ? (l1 == l2 ? l1 + 1 : M2 + 1) : 0;
     int done = 0;
     bool conv = false;
     for (int sweep = 0; sweep <= maxsweeps; ++sweep) {
         double off = 0, dia = 0;
-        for (int q = tid; q < n * n; q += nt) {
-            const int i = q / n, j = q % n;
-            if (j < i) continue;
-            const double v = A[pk(i, j, n)];
-            if (i == j) dia += v * v; else off += 2.0 * v * v;
+        for (int i = tid; i < n; i += nt) {
+            const double *row = A + ro(i);
+            dia += row[i] * row[i];
+            for (int j = i + 1; j < n; ++j) off += 2.0 * row[j] * row[j];
         }
         off = hl_block_sum(off, sh);
         dia = hl_block_sum(dia, sh);
         if (off <= 1e-32 * dia || off == 0.0) { conv = true; break; }
         if (sweep == maxsweeps) break;
         for (int r = 0; r < m - 1; ++r) {
-            for (int pp = tid; pp < M2; pp += nt) {
+            if (tid < M2) {
+                const int pp = tid;
                 int i, j;
                 rr_pair(pp, r, m, i, j);
                 double c = 1.0, s = 0.0;
+                const int ri = ro(i), rj = j < n ? ro(j) : 0;
                 if (j < n) {
-                    const double apq = A[pk(i, j, n)];
-                    if (apq != 0.0) {
-                        const double tau = (A[pk(j, j, n)] - A[pk(i, i, n)]) / (2.0 * apq);
+                    const double apq = A[ri + j];
+                    // a rotation by an element below 1e-18 of the geometric
+                    // mean of its diagonal pair moves those diagonals by
+                    // ~1e-36 relative: it is skipped, so that the sweeps
+                    // that only confirm convergence cost little
+                    const double aii = A[ri + i], ajj = A[rj + j];
+                    if (apq != 0.0 && fabs(apq) > 1e-18 * sqrt(fabs(aii * ajj))) {
+                        const double tau = (ajj - aii) / (2.0 * apq);
                         const double t = tau >= 0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
                                                   : -1.0 / (-tau + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);
@@ -145,58 +165,64 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
                     }
                 }
                 cs[pp] = make_double2(c, s);
-                pr[pp] = make_int2(i, j);
+                pij[pp] = i | (j << 16);
+                pri[pp] = ri;
+                prj[pp] = rj;
                 lb[(long)done * M2 + pp] = make_double2(c, s);
             }
             __syncthreads();
-            // every 2x2 block (k, l), k <= l, from its old values: thread t
-            // takes column pair l = t % M2 and row pairs k = t / M2 + s G
-            const int G = (nt / M2) > 0 ? nt / M2 : 1;
-            for (int t = tid; t < G * M2; t += nt) {
-                const int l = t % M2;
-                const double2 cl = cs[l];
-                const int2 pl = pr[l];
-                const bool vl = pl.y < n;
-                for (int k = t / M2; k <= l; k += G) {
-                    const double2 ck = cs[k];
-                    if (ck.y == 0.0 && cl.y == 0.0) continue;
-                    const int2 pkk = pr[k];
-                    const bool vk = pkk.y < n;
-                    if (k == l) {
-                        if (!vk) continue;
-                        const int i = pkk.x, j = pkk.y;
-                        const double aii = A[pk(i, i, n)], ajj = A[pk(j, j, n)],
-                                     aij = A[pk(i, j, n)];
-                        const double c = ck.x, s = ck.y;
-                        // J^T [[aii, aij], [aij, ajj]] J, off-diagonal exactly 0
-                        A[pk(i, i, n)] = c * c * aii - 2.0 * c * s * aij + s * s * ajj;
-                        A[pk(j, j, n)] = s * s * aii + 2.0 * c * s * aij + c * c * ajj;
-                        A[pk(i, j, n)] = 0.0;
-                        continue;
-                    }
-                    // rows (ik, jk) x columns (il, jl); entries with an index
-                    // past n (the odd player) stay out
-                    const int r0 = pkk.x, r1 = pkk.y, c0 = pl.x, c1 = pl.y;
-                    auto at = [&](int a, int bb) { return a <= bb ? pk(a, bb, n) : pk(bb, a, n); };
-                    const double x00 = A[at(r0, c0)];
-                    const double x01 = vl ? A[at(r0, c1)] : 0.0;
-                    const double x10 = vk ? A[at(r1, c0)] : 0.0;
-                    const double x11 = (vk && vl) ? A[at(r1, c1)] : 0.0;
-                    // Y = Jk^T X Jl, Jk = [[ck, sk], [-sk, ck]]
-                    const double a0 = ck.x * x00 - ck.y * x10, a1 = ck.x * x01 - ck.y * x11;
-                    const double b0 = ck.y * x00 + ck.x * x10, b1 = ck.y * x01 + ck.x * x11;
-                    A[at(r0, c0)] = a0 * cl.x - a1 * cl.y;
-                    if (vl) A[at(r0, c1)] = a0 * cl.y + a1 * cl.x;
-                    if (vk) A[at(r1, c0)] = b0 * cl.x - b1 * cl.y;
-                    if (vk && vl) A[at(r1, c1)] = b0 * cl.y + b1 * cl.x;
+            int lcur = -1;
+            int4 Q = make_int4(0, 0, 0, 0);
+            double2 cl = make_double2(1.0, 0.0);
+            for (int q = g; q < cnt; q += GT) {
+                int k, l;
+                if (q <= l1) { k = q; l = l1; } else { k = q - l1 - 1; l = l2; }
+                if (l != lcur) {
+                    lcur = l;
+                    const int w2 = pij[l];
+                    Q = make_int4(w2 & 0xffff, w2 >> 16, pri[l], prj[l]);
+                    cl = cs[l];
                 }
+                const double2 ck = cs[k];
+                if (ck.y == 0.0 && cl.y == 0.0) continue;
+                const int w1 = pij[k];
+                const int4 P = make_int4(w1 & 0xffff, w1 >> 16, pri[k], prj[k]);
+                const bool vk = P.y < n, vl = Q.y < n;
+                if (k == l) {
+                    if (!vk) continue;
+                    const int ii = P.z + P.x, jj = P.w + P.y, ij = P.z + P.y;
+                    const double aii = A[ii], ajj = A[jj], aij = A[ij];
+                    const double c = ck.x, s = ck.y;
+                    // J^T [[aii, aij], [aij, ajj]] J, off-diagonal exactly 0
+                    A[ii] = c * c * aii - 2.0 * c * s * aij + s * s * ajj;
+                    A[jj] = s * s * aii + 2.0 * c * s * aij + c * c * ajj;
+                    A[ij] = 0.0;
+                    continue;
+                }
+                // rows (P.x, P.y) x columns (Q.x, Q.y); entries with an index
+                // past n (the odd player) stay out
+                const int e00 = P.x <= Q.x ? P.z + Q.x : Q.z + P.x;
+                const int e01 = P.x <= Q.y ? P.z + Q.y : Q.w + P.x;
+                const int e10 = P.y <= Q.x ? P.w + Q.x : Q.z + P.y;
+                const int e11 = P.y <= Q.y ? P.w + Q.y : Q.w + P.y;
+                const double x00 = A[e00];
+                const double x01 = vl ? A[e01] : 0.0;
+                const double x10 = vk ? A[e10] : 0.0;
+                const double x11 = (vk && vl) ? A[e11] : 0.0;
+                // Y = Jk^T X Jl, Jk = [[ck, sk], [-sk, ck]]
+                const double a0 = ck.x * x00 - ck.y * x10, a1 = ck.x * x01 - ck.y * x11;
+                const double b0 = ck.y * x00 + ck.x * x10, b1 = ck.y * x01 + ck.x * x11;
+                A[e00] = a0 * cl.x - a1 * cl.y;
+                if (vl) A[e01] = a0 * cl.y + a1 * cl.x;
+                if (vk) A[e10] = b0 * cl.x - b1 * cl.y;
+                if (vk && vl) A[e11] = b0 * cl.y + b1 * cl.x;
             }
             __syncthreads();
             ++done;
         }
     }
     double *wb = w + (long)b * ld;
-    for (int i = tid; i < n; i += nt) wb[i] = A[pk(i, i, n)];
+    for (int i = tid; i < n; i += nt) wb[i] = A[ro(i) + i];
     if (tid == 0) {
         rounds[b] = done;
         if (!conv) atomicOr(status, 8);
@@ -204,52 +230,88 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
 }
 
 // The logged rotations applied to the basis: V[b] <- V[b] J_1 J_2 ... (in
-// place; V = identity first when ident); each workgroup owns kApplyRows rows
-// in LDS.  Round g of the log is round g mod (m-1) of the ordering.
+// place; V = identity first when ident).  kApplyLanes lanes own one row of
+// the basis in LDS and take disjoint rotations of each round (pairs
+// q, q + kApplyLanes, ...), so a round touches disjoint columns of a row and
+// the rounds follow one another without a barrier: the lanes of a row are in
+// one wave, whose LDS accesses complete in order.  Round g of the log is
+// round g mod (m-1) of the ordering; a lane advances its pairs' positions by
+// one per round (position x > 0 moves to x mod (m-1) + 1).
+constexpr int kApplyLanes = 16;
+constexpr int kApplyMaxPairs = 128 / kApplyLanes;
+constexpr int kApplyChunk = 16;        // rounds of the log staged in LDS at a time
+size_t apply_lds(int n) {
+    const int M2 = (n + (n & 1)) / 2;
+    return sizeof(double) * kApplyRows * n + sizeof(double2) * kApplyChunk * M2;
+}
 __global__ __launch_bounds__(256) void k_lj_apply(double *__restrict__ V, int ld, long mstride,
                                                   const int *__restrict__ nb,
                                                   const int *__restrict__ active,
                                                   const double2 *__restrict__ log,
                                                   long logstride, const int *__restrict__ rounds,
                                                   int ident) {
-    extern __shared__ double rows[];   // [kApplyRows][n]
-    __shared__ int2 pr[128];
-    __shared__ double2 cs[128];
-    const int b = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
+    extern __shared__ __align__(16) double lds_apply[];
+    const int b = blockIdx.y, tid = threadIdx.x;
     if (active && !active[b]) return;
     const int n = nb[b], m = n + (n & 1), M2 = m / 2;
     const int r0 = blockIdx.x * kApplyRows;
     if (r0 >= n) return;
     const int nr = min(kApplyRows, n - r0);
+    double2 *lgc = reinterpret_cast<double2 *>(lds_apply);   // [kApplyChunk][M2]
+    double *rows = reinterpret_cast<double *>(lgc + kApplyChunk * M2);   // [kApplyRows][n]
     double *Vb = V + b * mstride;
-    for (int e = tid; e < nr * n; e += nt) {
+    for (int e = tid; e < nr * n; e += blockDim.x) {
         const int r = e / n, c = e % n;
         rows[e] = ident ? (r0 + r == c ? 1.0 : 0.0) : Vb[(long)(r0 + r) * ld + c];
     }
+    const int row = tid / kApplyLanes, q0 = tid % kApplyLanes;
+    double *rw = rows + row * n;
+    const bool mine = row < nr;
+    // positions of this lane's pairs (players pp and m-1-pp) in the current
+    // round; player x > 0 moves to x mod (m-1) + 1 from one round to the next
+    int pa[kApplyMaxPairs], pb[kApplyMaxPairs];
+#pragma unroll
+    for (int t = 0; t < kApplyMaxPairs; ++t) {
+        const int pp = q0 + kApplyLanes * t;
+        pa[t] = pp;
+        pb[t] = m - 1 - pp;
+    }
     const double2 *lb = log + b * logstride;
     const int R = rounds[b];
-    for (int g = 0; g < R; ++g) {
+    for (int g0 = 0; g0 < R; g0 += kApplyChunk) {
+        const int nc = min(kApplyChunk, R - g0);
         __syncthreads();
-        for (int pp = tid; pp < M2; pp += nt) {
-            int i, j;
-            rr_pair(pp, g % (m - 1), m, i, j);
-            pr[pp] = make_int2(i, j);
-            cs[pp] = lb[(long)g * M2 + pp];
-        }
+        for (int e = tid; e < nc * M2; e += blockDim.x) lgc[e] = lb[(long)g0 * M2 + e];
         __syncthreads();
-        for (int e = tid; e < nr * M2; e += nt) {
-            const int r = e / M2, pp = e % M2;
-            const int2 p = pr[pp];
-            const double2 q = cs[pp];
-            if (p.y >= n || q.y == 0.0) continue;
-            double *row = rows + r * n;
-            const double vi = row[p.x], vj = row[p.y];
-            row[p.x] = q.x * vi - q.y * vj;
-            row[p.y] = q.y * vi + q.x * vj;
+        if (!mine) continue;
+        for (int gc = 0; gc < nc; ++gc) {
+            const double2 *lg = lgc + gc * M2;
+#pragma unroll
+            for (int t = 0; t < kApplyMaxPairs; ++t) {
+                const int pp = q0 + kApplyLanes * t;
+                if (pp < M2) {
+                    const double2 cs = lg[pp];
+                    int i = pa[t], j = pb[t];
+                    if (i > j) { const int x = i; i = j; j = x; }
+                    if (j < n && cs.y != 0.0) {
+                        const double vi = rw[i], vj = rw[j];
+                        rw[i] = cs.x * vi - cs.y * vj;
+                        rw[j] = cs.y * vi + cs.x * vj;
+                    }
+                }
+            }
+            // the lanes of a row are in one wave, whose LDS accesses
+            // complete in order: the next round may read what this one wrote
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < kApplyMaxPairs; ++t) {
+                if (pa[t] != 0) pa[t] = pa[t] % (m - 1) + 1;
+                pb[t] = pb[t] % (m - 1) + 1;
+            }
         }
     }
     __syncthreads();
-    for (int e = tid; e < nr * n; e += nt) {
+    for (int e = tid; e < nr * n; e += blockDim.x) {
         const int r = e / n, c = e % n;
         Vb[(long)(r0 + r) * ld + c] = rows[e];
     }
@@ -854,7 +916,8 @@ bool hfb_large_fits(int hmax) { return hmax >= 1 && (hmax + 15) / 16 <= 8; }
 
 size_t hfb_large_jacobi_lds(int n) {
     const int m = n + (n & 1), M2 = m / 2;
-    return sizeof(double) * ((size_t)n * (n + 1) / 2 + (size_t)M2 * 3 + (M2 & 1) + 16);
+    return sizeof(double) * ((size_t)n * (n + 1) / 2 + 2 * (size_t)M2 + 16) +
+           sizeof(int) * (3 * (size_t)M2 + (M2 & 1));
 }
 
 int hfb_large_setup(mocca_solve *h) {
@@ -941,8 +1004,9 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
                                kJacMaxSweeps, p.status);
         });
     } else {
-        const int m = nmax + (nmax & 1);
-        const size_t shm = sizeof(double) * (3 * (size_t)(m / 2) + 1 + 16);
+        const int m = nmax + (nmax & 1), M2 = m / 2;
+        const size_t shm = sizeof(double) * (2 * (size_t)M2 + 16) +
+                           sizeof(int) * (3 * (size_t)M2 + (M2 & 1));
         rc = run(h, "hfb_jacobi", [&] {
             hipLaunchKernelGGL((k_lj_jacobi<true>), dim3(nblk), dim3(kJacThreads), shm, h->st, A,
                                ld, ms, nb, active, p.lW, w, p.llog, p.lmax, rounds,
@@ -951,7 +1015,7 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
     }
     if (rc) return rc;
     return run(h, "hfb_jacobi_apply", [&] {
-        const size_t shm = sizeof(double) * kApplyRows * nmax;
+        const size_t shm = apply_lds(nmax);
         hipLaunchKernelGGL(k_lj_apply, dim3((nmax + kApplyRows - 1) / kApplyRows, nblk),
                            dim3(256), shm, h->st, V, ld, ms, nb, active, p.llog, p.lmax,
                            rounds, ident ? 1 : 0);
