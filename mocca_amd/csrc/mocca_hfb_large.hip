@@ -49,6 +49,7 @@ namespace {
 constexpr int kJacThreads = 1024;
 constexpr int kJacLdsMaxN = 200;     // packed upper triangle + rotation tables in 160 KiB
 constexpr int kJacMaxSweeps = 24;
+constexpr int kJacBatch = 3;         // 2x2 blocks a thread reads before it updates them
 constexpr int kApplyRows = 16;       // basis rows per workgroup of the rotation apply (4 waves)
 constexpr int kGramWaves = 1024;     // waves of one pair-Gram launch
 
@@ -94,7 +95,7 @@ __device__ __forceinline__ void rr_pair(int pp, int r, int m, int &i, int &j) {
 // A <- J^T A J with J = [[c, s], [-s, c]] in rows/columns (i, j); sweeps stop
 // when the off-diagonal square sum is below 1e-32 of the diagonal one.
 // GA: the packed matrix in global memory (Ag[b]) instead of LDS.
-template <bool GA>
+template <bool GA, bool ODD>
 __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
     const double *__restrict__ Ain, int ld, long mstride, const int *__restrict__ nb,
     const int *__restrict__ active, double *__restrict__ Ag, double *__restrict__ w,
@@ -171,51 +172,90 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
                 lb[(long)done * M2 + pp] = make_double2(c, s);
             }
             __syncthreads();
-            int lcur = -1;
-            int4 Q = make_int4(0, 0, 0, 0);
-            double2 cl = make_double2(1.0, 0.0);
-            for (int q = g; q < cnt; q += GT) {
-                int k, l;
-                if (q <= l1) { k = q; l = l1; } else { k = q - l1 - 1; l = l2; }
-                if (l != lcur) {
-                    lcur = l;
-                    const int w2 = pij[l];
-                    Q = make_int4(w2 & 0xffff, w2 >> 16, pri[l], prj[l]);
-                    cl = cs[l];
+            // the column pair of each of this thread's two lists
+            const double2 cl1 = worker ? cs[l1] : make_double2(1.0, 0.0);
+            const double2 cl2 = worker ? cs[l2] : make_double2(1.0, 0.0);
+            const int4 Q1 = worker ? make_int4(pij[l1] & 0xffff, pij[l1] >> 16, pri[l1], prj[l1])
+                                   : make_int4(0, 0, 0, 0);
+            const int4 Q2 = worker ? make_int4(pij[l2] & 0xffff, pij[l2] >> 16, pri[l2], prj[l2])
+                                   : make_int4(0, 0, 0, 0);
+            // kJacBatch blocks at a time: their tables, then their elements,
+            // are read back to back so that the LDS latencies overlap; every
+            // element belongs to one block, so the reads of a batch see the
+            // old values and its writes touch nothing another block reads
+            for (int q0 = g; q0 < cnt; q0 += kJacBatch * GT) {
+                int e[kJacBatch][4];
+                double x[kJacBatch][4];
+                double2 ck[kJacBatch], cl[kJacBatch];
+                int kind[kJacBatch];   // 0 none, 1 off-diagonal block, 2 diagonal block
+#pragma unroll
+                for (int t = 0; t < kJacBatch; ++t) {
+                    const int q = q0 + t * GT;
+                    kind[t] = 0;
+                    if (q >= cnt) continue;
+                    const bool first = q <= l1;
+                    const int k = first ? q : q - l1 - 1, l = first ? l1 : l2;
+                    const int4 Q = first ? Q1 : Q2;
+                    cl[t] = first ? cl1 : cl2;
+                    ck[t] = cs[k];
+                    // a block whose two rotations are both the identity
+                    // keeps its values (the sweeps that only confirm
+                    // convergence read nothing more)
+                    if (ck[t].y == 0.0 && cl[t].y == 0.0) continue;
+                    const int w1 = pij[k];
+                    const int4 P = make_int4(w1 & 0xffff, w1 >> 16, pri[k], prj[k]);
+                    if (k == l) {
+                        kind[t] = (ODD && P.y >= n) || ck[t].y == 0.0 ? 0 : 2;
+                        e[t][0] = P.z + P.x;   // (i, i)
+                        e[t][1] = P.z + P.y;   // (i, j)
+                        e[t][3] = P.w + P.y;   // (j, j)
+                        e[t][2] = 0;
+                        continue;
+                    }
+                    kind[t] = 1;
+                    // rows (P.x, P.y) x columns (Q.x, Q.y); with an odd
+                    // size, entries with the index n (the dummy player) stay
+                    // out (their offset is set to -1)
+                    e[t][0] = P.x <= Q.x ? P.z + Q.x : Q.z + P.x;
+                    e[t][1] = P.x <= Q.y ? P.z + Q.y : Q.w + P.x;
+                    e[t][2] = P.y <= Q.x ? P.w + Q.x : Q.z + P.y;
+                    e[t][3] = P.y <= Q.y ? P.w + Q.y : Q.w + P.y;
+                    if (ODD) {
+                        if (Q.y >= n) e[t][1] = e[t][3] = -1;
+                        if (P.y >= n) e[t][2] = e[t][3] = -1;
+                    }
                 }
-                const double2 ck = cs[k];
-                if (ck.y == 0.0 && cl.y == 0.0) continue;
-                const int w1 = pij[k];
-                const int4 P = make_int4(w1 & 0xffff, w1 >> 16, pri[k], prj[k]);
-                const bool vk = P.y < n, vl = Q.y < n;
-                if (k == l) {
-                    if (!vk) continue;
-                    const int ii = P.z + P.x, jj = P.w + P.y, ij = P.z + P.y;
-                    const double aii = A[ii], ajj = A[jj], aij = A[ij];
-                    const double c = ck.x, s = ck.y;
-                    // J^T [[aii, aij], [aij, ajj]] J, off-diagonal exactly 0
-                    A[ii] = c * c * aii - 2.0 * c * s * aij + s * s * ajj;
-                    A[jj] = s * s * aii + 2.0 * c * s * aij + c * c * ajj;
-                    A[ij] = 0.0;
-                    continue;
+#pragma unroll
+                for (int t = 0; t < kJacBatch; ++t) {
+                    if (kind[t] == 0) continue;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        x[t][c] = (kind[t] == 2 && c == 2) || (ODD && e[t][c] < 0) ? 0.0
+                                                                                   : A[e[t][c]];
                 }
-                // rows (P.x, P.y) x columns (Q.x, Q.y); entries with an index
-                // past n (the odd player) stay out
-                const int e00 = P.x <= Q.x ? P.z + Q.x : Q.z + P.x;
-                const int e01 = P.x <= Q.y ? P.z + Q.y : Q.w + P.x;
-                const int e10 = P.y <= Q.x ? P.w + Q.x : Q.z + P.y;
-                const int e11 = P.y <= Q.y ? P.w + Q.y : Q.w + P.y;
-                const double x00 = A[e00];
-                const double x01 = vl ? A[e01] : 0.0;
-                const double x10 = vk ? A[e10] : 0.0;
-                const double x11 = (vk && vl) ? A[e11] : 0.0;
-                // Y = Jk^T X Jl, Jk = [[ck, sk], [-sk, ck]]
-                const double a0 = ck.x * x00 - ck.y * x10, a1 = ck.x * x01 - ck.y * x11;
-                const double b0 = ck.y * x00 + ck.x * x10, b1 = ck.y * x01 + ck.x * x11;
-                A[e00] = a0 * cl.x - a1 * cl.y;
-                if (vl) A[e01] = a0 * cl.y + a1 * cl.x;
-                if (vk) A[e10] = b0 * cl.x - b1 * cl.y;
-                if (vk && vl) A[e11] = b0 * cl.y + b1 * cl.x;
+#pragma unroll
+                for (int t = 0; t < kJacBatch; ++t) {
+                    if (kind[t] == 0) continue;
+                    const double2 a = ck[t], b = cl[t];
+                    if (kind[t] == 2) {
+                        // J^T [[aii, aij], [aij, ajj]] J, off-diagonal exactly 0
+                        const double aii = x[t][0], aij = x[t][1], ajj = x[t][3];
+                        const double c = a.x, s = a.y;
+                        A[e[t][0]] = c * c * aii - 2.0 * c * s * aij + s * s * ajj;
+                        A[e[t][3]] = s * s * aii + 2.0 * c * s * aij + c * c * ajj;
+                        A[e[t][1]] = 0.0;
+                        continue;
+                    }
+                    // Y = Jk^T X Jl, Jk = [[ck, sk], [-sk, ck]]
+                    const double a0 = a.x * x[t][0] - a.y * x[t][2];
+                    const double a1 = a.x * x[t][1] - a.y * x[t][3];
+                    const double b0 = a.y * x[t][0] + a.x * x[t][2];
+                    const double b1 = a.y * x[t][1] + a.x * x[t][3];
+                    A[e[t][0]] = a0 * b.x - a1 * b.y;
+                    if (!ODD || e[t][1] >= 0) A[e[t][1]] = a0 * b.y + a1 * b.x;
+                    if (!ODD || e[t][2] >= 0) A[e[t][2]] = b0 * b.x - b1 * b.y;
+                    if (!ODD || e[t][3] >= 0) A[e[t][3]] = b0 * b.y + b1 * b.x;
+                }
             }
             __syncthreads();
             ++done;
@@ -303,10 +343,11 @@ __global__ __launch_bounds__(256) void k_lj_apply(double *__restrict__ V, int ld
             // the lanes of a row are in one wave, whose LDS accesses
             // complete in order: the next round may read what this one wrote
             __builtin_amdgcn_wave_barrier();
+            // x in 1 .. m-1 moves to x mod (m-1) + 1: x + 1, or 1 after m-1
 #pragma unroll
             for (int t = 0; t < kApplyMaxPairs; ++t) {
-                if (pa[t] != 0) pa[t] = pa[t] % (m - 1) + 1;
-                pb[t] = pb[t] % (m - 1) + 1;
+                if (pa[t] != 0) pa[t] = pa[t] == m - 1 ? 1 : pa[t] + 1;
+                pb[t] = pb[t] == m - 1 ? 1 : pb[t] + 1;
             }
         }
     }
@@ -987,7 +1028,7 @@ namespace {
 // Jacobi + rotation apply on nblk matrices of sizes nb[] (A, leading
 // dimension ld) with the basis V (identity first when ident).
 int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int *nb, int nmax,
-               const int *active, int *rounds, bool ident) {
+               const int *active, int *rounds, bool ident, bool odd) {
     Hfb &p = h->hfb;
     const int nblk = p.k.nblk, ld = p.nmax;
     const long ms = (long)ld * ld;
@@ -997,20 +1038,21 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
         // (the limit for the largest in-LDS matrix, once)
         const size_t top = std::min(hfb_large_jacobi_lds(kJacLdsMaxN), p.lds_limit);
         rc = run(h, "hfb_jacobi", [&] {
-            static const bool ok = require_lds((const void *)k_lj_jacobi<false>, top, "k_lj_jacobi");
-            if (!ok) require_lds((const void *)k_lj_jacobi<false>, top, "k_lj_jacobi");
-            hipLaunchKernelGGL((k_lj_jacobi<false>), dim3(nblk), dim3(kJacThreads), shm, h->st, A,
-                               ld, ms, nb, active, nullptr, w, p.llog, p.lmax, rounds,
-                               kJacMaxSweeps, p.status);
+            auto kern = odd ? k_lj_jacobi<false, true> : k_lj_jacobi<false, false>;
+            require_lds((const void *)kern, top, "k_lj_jacobi");
+            hipLaunchKernelGGL(kern, dim3(nblk), dim3(kJacThreads), shm, h->st, A, ld, ms, nb,
+                               active, nullptr, w, p.llog, p.lmax, rounds, kJacMaxSweeps,
+                               p.status);
         });
     } else {
         const int m = nmax + (nmax & 1), M2 = m / 2;
         const size_t shm = sizeof(double) * (2 * (size_t)M2 + 16) +
                            sizeof(int) * (3 * (size_t)M2 + (M2 & 1));
         rc = run(h, "hfb_jacobi", [&] {
-            hipLaunchKernelGGL((k_lj_jacobi<true>), dim3(nblk), dim3(kJacThreads), shm, h->st, A,
-                               ld, ms, nb, active, p.lW, w, p.llog, p.lmax, rounds,
-                               kJacMaxSweeps, p.status);
+            auto kern = odd ? k_lj_jacobi<true, true> : k_lj_jacobi<true, false>;
+            hipLaunchKernelGGL(kern, dim3(nblk), dim3(kJacThreads), shm, h->st, A, ld, ms, nb,
+                               active, p.lW, w, p.llog, p.lmax, rounds, kJacMaxSweeps,
+                               p.status);
         });
     }
     if (rc) return rc;
@@ -1078,7 +1120,7 @@ int hfb_large_solve_impl(mocca_solve *h) {
     // (Broyden: two evaluations before its HFBIter iterations)
     for (int ev = 0; ev < std::max(1, k.iters) + 3; ++ev) {
         if ((rc = jacobi_run(h, ev == 0 ? A0 : p.lA, p.lV, p.lw, p.lnb, p.nmax, p.lactive,
-                             p.lrounds, ev == 0 && !p.lwarm)))
+                             p.lrounds, ev == 0 && !p.lwarm, false)))
             return rc;
         rc = run(h, "hfb_fermi", [&] {
             hipLaunchKernelGGL((k_lj_gemm<true, true>), gmat, dim3(256), 0, h->st, p.lV, p.lV,
@@ -1112,8 +1154,10 @@ int hfb_large_solve_impl(mocca_solve *h) {
     });
     if (rc) return rc;
     // (the block sizes h are the second half of lnb)
+    bool odd = false;
+    for (int b = 0; b < nblk; ++b) odd = odd || (k.blk[b].h & 1);
     if ((rc = jacobi_run(h, p.lT, p.lVc, p.lwc, p.lnb + nblk, hmax, nullptr,
-                         p.lrounds + nblk, true)))
+                         p.lrounds + nblk, true, odd)))
         return rc;
     return run(h, "hfb_canonical", [&] {
         hipLaunchKernelGGL(k_hl_can_finish, dim3(nblk), dim3(256), 0, h->st, p.k, p.lVc, p.lwc,

@@ -8,6 +8,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define IDX(x, y, z) (((size_t)(z) * ny + (y)) * nx + (x))
 
@@ -237,6 +240,36 @@ void or_derive_spinor(const double *psi, double *der, double *lap,
     }
 }
 
+void or_derive_states(const double *psi, double *der, double *lap, int M, const int *qn,
+                      int nx, int ny, int nz, double dx, int lag, const double *lagx,
+                      const double *lagy, const double *lagz)
+{
+    const size_t N = (size_t)nx * ny * nz;
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < M; ++i)
+        or_derive_spinor(psi + (size_t)i * 4 * N, der + (size_t)i * 12 * N,
+                         lap + (size_t)i * 4 * N, nx, ny, nz, dx, qn[4 * i + 1],
+                         qn[4 * i + 2], qn[4 * i + 3], lag, lagx, lagy, lagz);
+}
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int or_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* Spinor algebra (Spinor.f90:113-157,264-281,353-369).                     */
 static inline void pauli(const double in[4], int dir, double out[4])
@@ -342,13 +375,18 @@ void or_density(const double *psi, const double *der, int M, int N,
     static const int tri[6][3] = {{0, 1, 2}, {0, 2, 1}, {2, 1, 0},
                                   {2, 0, 1}, {1, 0, 2}, {1, 2, 0}};
     static const double lc[6] = {1, -1, -1, 1, -1, 1};
+    /* point blocks over threads; every point sums its states in index order */
+    const size_t pblk = 4096;
+#pragma omp parallel for schedule(static)
+    for (size_t p0 = 0; p0 < n; p0 += pblk)
     for (int s = 0; s < M; ++s) {
         const double o = occ[s];
         if (o == 0.0) continue;
         const int it = (qn[4 * s] + 3) / 2 - 1;
         const double *ps = psi + (size_t)s * 4 * n;
         const double *ds = der + (size_t)s * 12 * n;
-        for (size_t p = 0; p < n; ++p) {
+        const size_t p1 = p0 + pblk < n ? p0 + pblk : n;
+        for (size_t p = p0; p < p1; ++p) {
             double v[4], d[3][4], t[4];
             for (int c = 0; c < 4; ++c) {
                 v[c] = ps[c * n + p];
@@ -465,6 +503,7 @@ void or_gram_schmidt(double *psi, int M, int N, const int *qn,
             const double sc = 1.0 / sqrt(norm);
             for (size_t i = 0; i < n4; ++i) pk[i] = sc * pk[i];
             const int *q = qn + 4 * idx[k];
+#pragma omp parallel for schedule(dynamic)
             for (int l = k + 1; l < nw; ++l) {
                 const int *r = qn + 4 * idx[l];
                 if (r[0] != q[0] || r[1] != q[1] || r[2] != q[2]) continue;
@@ -486,7 +525,10 @@ void or_graddesc(double *psi, const double *der, const double *lap,
                  double *energy, double *disp)
 {
     const size_t n4 = (size_t)4 * N;
+#pragma omp parallel
+    {
     double *h = malloc(sizeof(double) * n4);
+#pragma omp for schedule(dynamic)
     for (int i = 0; i < M; ++i) {
         double *p = psi + i * n4, *u = upd + i * n4;
         const int it = (qn[4 * i] + 3) / 2 - 1;
@@ -501,7 +543,8 @@ void or_graddesc(double *psi, const double *der, const double *lap,
             p[k] = p[k] + step;
         }
     }
+    free(h);
+    }
     or_gram_schmidt(psi, M, N, qn, energy, dv);
     for (size_t k = 0; k < (size_t)M * n4; ++k) upd[k] = psi[k] - upd[k];
-    free(h);
 }

@@ -58,6 +58,17 @@ void or_derive_spinor(const double *psi, double *der, double *lap,
                       int lag, const double *lagx, const double *lagy,
                       const double *lagz);
 
+/* CompDer for states 0..M-1 (DeriveAll, SpwfStorage.f90:586-604), over
+ * OpenMP threads; qn[4*i+1..3] = parity, signature, time simplex. */
+void or_derive_states(const double *psi, double *der, double *lap, int M, const int *qn,
+                      int nx, int ny, int nz, double dx, int lag, const double *lagx,
+                      const double *lagy, const double *lagz);
+
+/* OpenMP threads of the loops over states and points (results do not depend
+ * on the count: every point and every state keeps its operation order). */
+void or_set_threads(int n);
+int or_max_threads(void);
+
 /* hPsi for one state (ImaginaryTime.f90:213-304): ActionOfBNew + ActionOfU
  * + ActionOfW (+ ActionOfS + ActionOfA when trc == 0)
  * (MeanFields.f90:792-851,1029-1065,1107-1147).

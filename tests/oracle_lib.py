@@ -73,6 +73,17 @@ class Mesh:
                                int(TS_), self.lag, *self.lagmats())
         return der, lap
 
+    def derive_states(self, psi, qn):
+        """CompDer of every state: der[M][3][4][N], lap[M][4][N]."""
+        M = psi.shape[0]
+        psi = np.ascontiguousarray(psi.reshape(M, 4, self.N), dtype=np.float64)
+        der = np.zeros((M, 3, 4, self.N))
+        lap = np.zeros((M, 4, self.N))
+        lib().or_derive_states(P(psi), P(der), P(lap), int(M),
+                               P(np.ascontiguousarray(qn, dtype=np.int32)), self.nx, self.ny,
+                               self.nz, ctypes.c_double(self.dx), self.lag, *self.lagmats())
+        return der, lap
+
     def derive_field(self, f, axis, sign):
         f = np.ascontiguousarray(f.reshape(self.N), dtype=np.float64)
         out = np.zeros(self.N)
@@ -488,10 +499,7 @@ class Constraints:
 def all_densities(mesh, psi, qn, occ, trc):
     """ComputeDensity + the derived fields, and each state's derivatives."""
     M, N = psi.shape[0], mesh.N
-    der = np.zeros((M, 3, 4, N))
-    lap = np.zeros((M, 4, N))
-    for j, q in enumerate(qn):
-        der[j], lap[j] = mesh.derive_spinor(psi[j], q[1], q[2], q[3])
+    der, lap = mesh.derive_states(psi, qn)
     rho, tau, nj, vj, vs = density(psi, der, qn, occ, trc, N)
     derrho, laprho, rotj, rots, _ = density_derived(mesh, rho, vj, vs, trc)
     d = dict(rho=rho, tau=tau, nablaJ=nj, derrho=derrho, laprho=laprho)
@@ -500,11 +508,13 @@ def all_densities(mesh, psi, qn, occ, trc):
     return d, der, lap
 
 
-def scf_run(g, niter):
+def scf_run(g, niter, on_iteration=None):
     """The Evolve loop of the reference (Main.f90:172-225, 255-328) for HF with
     fixed dt/momentum and linear density mixing, restated on the oracle.
     Returns the per-iteration [E, sum kin, E_C, E_Cx, sum com, terms(36)]
-    rows (start-up row first), the final states and potentials."""
+    rows (start-up row first), the final states and potentials.
+    on_iteration(k) is called after the start-up (k = 0) and after each
+    iteration k."""
     m = g["meta"]
     mesh = Mesh(m["nx"], m["ny"], m["nz"], m["dx"], m["lagrange"])
     coef, consts = g["coef"], g["consts"]
@@ -562,11 +572,10 @@ def scf_run(g, niter):
         ep = hfb.energy()
         E[0] += ep.sum()
         pairs.append(np.concatenate([hfb.fermi, ep]))
-        der = np.zeros((M, 3, 4, N))
-        lap = np.zeros((M, 4, N))
-        for i in range(M):
-            der[i], lap[i] = mesh.derive_spinor(psi[i], qn[i][1], qn[i][2], qn[i][3])
+        der, lap = mesh.derive_states(psi, qn)
     traj = [rows(E)]
+    if on_iteration is not None:
+        on_iteration(0)
     altcheck = cons is not None and cons.alternating()
     upd = np.zeros_like(psi)
     hbar = consts[5]
@@ -629,9 +638,10 @@ def scf_run(g, niter):
             ep = hfb.energy()
             E[0] += ep.sum()
             pairs.append(np.concatenate([hfb.fermi, ep]))
-            for i in range(M):
-                der[i], lap[i] = mesh.derive_spinor(psi[i], qn[i][1], qn[i][2], qn[i][3])
+            der, lap = mesh.derive_states(psi, qn)
         traj.append(rows(E))
+        if on_iteration is not None:
+            on_iteration(k + 1)
     return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
                 vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params),
                 pairing=np.array(pairs), hfb=hfb,
@@ -741,57 +751,94 @@ class HFB:
         return np.stack([-self.G[it] * (1.0 - self.alpha[it] / self.rhosat * rho0)
                          for it in range(2)])
 
-    def _pairs(self, k, pc, need_kappa):
-        idx = self.blocks[k]
-        N = len(idx)
-        for j in range(N):
-            jj = idx[j]
-            jjj = jj % self.nwt
-            if pc[jjj] < HFB_NUM_CUT:
-                continue
-            for i in range(j + 1, N):
-                ii = idx[i]
-                iii = ii % self.nwt
-                if self.sig(jj) != -self.sig(ii):
-                    continue
-                if need_kappa and pc[jjj] * pc[iii] * abs(self.kappa[k][i, j]) < HFB_NUM_CUT:
-                    continue
-                tr = (ii != iii or jj != jjj) and not (ii != iii and jj != jjj)
-                yield i, j, iii, jjj, tr
+    def _block(self, k):
+        """Positions of block k: original state of each (idx mod nwt), the
+        reversed flag, and the signature of each position."""
+        idx = np.array(self.blocks[k])
+        orig = idx % self.nwt
+        rev = idx >= self.nwt
+        sig = np.where(rev, -1, 1) * self.qn[orig, 2]
+        return idx, orig, rev, sig
+
+    @staticmethod
+    def _partner(y, F, tr):
+        """Y with sum_g x_g Y_g = Re P(x, y) Re F + Im P(x, y) Im F for the pair
+        amplitude P of pair_density (y: [..., 4, N], F complex [N])."""
+        fr, fi = F.real, F.imag
+        if tr:
+            return np.stack([-fr * y[..., 0, :] + fi * y[..., 1, :],
+                             -fr * y[..., 1, :] - fi * y[..., 0, :],
+                             -fr * y[..., 2, :] + fi * y[..., 3, :],
+                             -fr * y[..., 3, :] - fi * y[..., 2, :]], axis=-2)
+        return np.stack([fr * y[..., 2, :] + fi * y[..., 3, :],
+                         -fr * y[..., 3, :] + fi * y[..., 2, :],
+                         -fr * y[..., 0, :] - fi * y[..., 1, :],
+                         fr * y[..., 1, :] - fi * y[..., 0, :]], axis=-2)
 
     def pairing_field(self, psi, pc, df):
+        """HFBPairingField: field_it = -sum over pairs (i > j, opposite
+        signatures, pc_i pc_j |kappa_ij| above the cut) of pc_i pc_j kappa_ij
+        P(psi_j, psi_i); P is bilinear, so the sum runs as
+        sum_o P(psi_o, phi_o) with phi = W psi over the block's states."""
         N = psi.shape[-1]
         field = np.zeros((2, N), complex)
         for k in self.keys:
             it = k[1] - 1
-            for i, j, iii, jjj, tr in self._pairs(k, pc, True):
-                a = pair_density(psi[jjj], psi[iii], tr)
-                f = pc[jjj] * pc[iii] * self.kappa[k][i, j]
-                field[it] = field[it] - f * a
+            idx, orig, rev, sig = self._block(k)
+            n = len(idx)
+            pcs = pc[orig]
+            kap = self.kappa[k]
+            lower = np.tril(np.ones((n, n), bool), -1)             # i > j
+            mask = lower & (pcs[None, :] >= HFB_NUM_CUT) & (sig[:, None] == -sig[None, :])
+            mask &= pcs[None, :] * pcs[:, None] * np.abs(kap) >= HFB_NUM_CUT
+            uo, inv = np.unique(orig, return_inverse=True)
+            base = psi[uo]
+            for tr in (False, True):
+                sel = mask & ((rev[:, None] ^ rev[None, :]) == tr)
+                if not sel.any():
+                    continue
+                W = np.where(sel, pcs[:, None] * pcs[None, :] * kap, 0.0)   # W[i, j]
+                # Wm[o_j, o_i] = sum of W[i, j] over positions of those states
+                Wm = np.zeros((len(uo), len(uo)))
+                np.add.at(Wm, (inv[None, :].repeat(n, 0), inv[:, None].repeat(n, 1)), W)
+                phi = np.tensordot(Wm, base, axes=(1, 0))                # [o][4][N]
+                x, y = base, phi
+                if tr:
+                    re = -(x[:, 0] * y[:, 0] + x[:, 1] * y[:, 1] + x[:, 2] * y[:, 2] +
+                           x[:, 3] * y[:, 3])
+                    im = x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0] + x[:, 2] * y[:, 3] - \
+                        x[:, 3] * y[:, 2]
+                else:
+                    re = x[:, 0] * y[:, 2] - x[:, 1] * y[:, 3] - x[:, 2] * y[:, 0] + \
+                        x[:, 3] * y[:, 1]
+                    im = x[:, 0] * y[:, 3] + x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1] - \
+                        x[:, 3] * y[:, 0]
+                field[it] = field[it] - (re.sum(0) + 1j * im.sum(0))
         return field * df
 
     def gaps(self, psi, field, pc, dv):
+        """HFBGaps: Delta_ij = dv pc_i pc_j sum [Re P(psi_i, psi_j) Re F + Im P
+        Im F] for i < j of opposite signatures above the cut, Delta_ji =
+        -Delta_ij; the sums over points as products psi . Y(psi)^T."""
         for k in self.keys:
             it = k[1] - 1
-            idx = self.blocks[k]
-            N = len(idx)
-            d = np.zeros((N, N))
-            for i in range(N):
-                ii = idx[i]
-                iii = ii % self.nwt
-                for j in range(i + 1, N):
-                    jj = idx[j]
-                    jjj = jj % self.nwt
-                    if self.sig(ii) != -self.sig(jj):
-                        continue
-                    if pc[iii] * pc[jjj] < HFB_NUM_CUT:
-                        continue
-                    tr = (ii != iii or jj != jjj) and not (ii != iii and jj != jjj)
-                    t = pair_density(psi[iii], psi[jjj], tr)
-                    d[i, j] = dv * pc[iii] * pc[jjj] * (np.sum(t.real * field[it].real) +
-                                                         np.sum(t.imag * field[it].imag))
-                    d[j, i] = -d[i, j]
-            self.delta[k] = d
+            idx, orig, rev, sig = self._block(k)
+            n = len(idx)
+            pcs = pc[orig]
+            uo, inv = np.unique(orig, return_inverse=True)
+            base = psi[uo].reshape(len(uo), -1)
+            upper = np.triu(np.ones((n, n), bool), 1)
+            mask = upper & (sig[:, None] == -sig[None, :])
+            mask &= pcs[:, None] * pcs[None, :] >= HFB_NUM_CUT
+            d = np.zeros((n, n))
+            for tr in (False, True):
+                sel = mask & ((rev[:, None] ^ rev[None, :]) == tr)
+                if not sel.any():
+                    continue
+                Y = self._partner(psi[uo], field[it], tr).reshape(len(uo), -1)
+                Q = base @ Y.T                                           # Q[o, o']
+                d = np.where(sel, dv * pcs[:, None] * pcs[None, :] * Q[inv][:, inv], d)
+            self.delta[k] = d - d.T
 
     def number(self, lam, energy):
         """HFBNumberofParticles: rho, kappa, quasiparticle energies at lam."""
@@ -799,25 +846,18 @@ class HFB:
         self.qpe = {}
         for k in self.keys:
             it = k[1] - 1
-            idx = self.blocks[k]
+            idx = np.array(self.blocks[k])
             N = len(idx)
-            H = np.zeros((2 * N, 2 * N))
-            for i in range(N):
-                H[i, i] = energy[idx[i] % self.nwt] - lam[it]
-                for j in range(N):
-                    H[i + N, j + N] = -H[i, j]
-                for j in range(N):
-                    H[i, j + N] = self.delta[k][i, j]
-            for i in range(2 * N):
-                for j in range(i + 1, 2 * N):
-                    H[j, i] = H[i, j]
             h = N // 2
+            e = energy[idx % self.nwt] - lam[it]
+            # the signature-reduced matrix of ConstructHFBHamiltonian:
+            # [[e_top - lam, Delta_top,bottom], [Delta^T, -(e_bottom - lam)]]
             T = np.zeros((N, N))
-            for j in range(h):
-                for i in range(h):
-                    T[i, j] = T[j, i] = H[i, j]
-                    T[i + h, j + h] = T[j + h, i + h] = H[i + 3 * h, j + 3 * h]
-                    T[i, j + h] = T[j + h, i] = H[i, j + 3 * h]
+            ar = np.arange(h)
+            T[ar, ar] = e[:h]
+            T[h + ar, h + ar] = -e[h:]
+            T[:h, h:] = self.delta[k][:h, h:]
+            T[h:, :h] = self.delta[k][:h, h:].T
             w, vec = np.linalg.eigh(T)
             U = np.zeros((N, 2 * N))
             V = np.zeros((N, 2 * N))
@@ -829,12 +869,8 @@ class HFB:
             cols = [2 * N - i + 1 for i in range(1, h + 1)] + list(range(h + 1, N + 1))
             cols = np.array(cols) - 1
             rho = V[:, cols] @ V[:, cols].T
-            kap = np.zeros((N, N))
-            for j in range(N):
-                for i in range(j + 1, N):
-                    kap[i, j] = np.sum(V[i, cols] * U[j, cols])
-                    kap[j, i] = -kap[i, j]
-            self.rho[k], self.kappa[k] = rho, kap
+            K = np.tril(V[:, cols] @ U[:, cols].T, -1)
+            self.rho[k], self.kappa[k] = rho, K - K.T
             self.qpe[k] = qpe[cols]
             n[it] += np.trace(rho)
         return n
@@ -898,16 +934,18 @@ class HFB:
             cols = [c for c in range(N) if not np.any(np.abs(vec[h:, c]) > HFB_NUM_CUT)]
             slots = [i for i in idx if i < self.nwt]
             assert len(cols) == len(slots)
-            for slot, c in zip(slots, cols):
-                v = np.zeros_like(psi[0])
-                for j in range(N):
-                    if abs(vec[j, c]) < HFB_NUM_CUT:
-                        continue
-                    jjj = idx[j] % self.nwt
-                    src = psi[jjj] if idx[j] < self.nwt else time_reverse(psi[jjj])
-                    v = v + vec[j, c] * src
-                can[slot] = v
-                occ[slot] = w[c]
+            # canonical state c = sum_j vec[j, c] (psi_j, or its time reverse
+            # for a reversed position); time reversal is linear
+            ia = np.array(idx)
+            cv = np.where(np.abs(vec[:, cols]) < HFB_NUM_CUT, 0.0, vec[:, cols])   # [j][c]
+            fw, bw = ia < self.nwt, ia >= self.nwt
+            v = np.tensordot(cv[fw].T, psi[ia[fw] % self.nwt], axes=(1, 0))
+            if bw.any():
+                v = v + time_reverse(np.moveaxis(
+                    np.tensordot(cv[bw].T, psi[ia[bw] % self.nwt], axes=(1, 0)), 1, 0)
+                ).swapaxes(0, 1)
+            can[slots] = v
+            occ[slots] = w[cols]
         if any(np.any(self.old_rho[k] != 0.0) for k in self.keys):
             for k in self.keys:
                 self.rho[k] = self.mix * self.rho[k] + (1.0 - self.mix) * self.old_rho[k]

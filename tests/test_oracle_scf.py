@@ -372,3 +372,16 @@ def test_mix_linear():
     want = 0.25 * d + 0.75 * h
     ol.lib().or_mix(ol.P(d), ol.P(h), ol.ctypes.c_size_t(100), ol.ctypes.c_double(0.75))
     assert np.max(np.abs(d - want)) < 1e-15
+
+
+def test_scf_hfb_blocks_of_100_states_vs_reference():
+    """HFB blocks of 100 states (200 x 200 signature-reduced matrices): the
+    reference's run of 200 neutron and 40 proton harmonic-oscillator states,
+    4 iterations with pairing on throughout."""
+    g = golden("scfhfb_large")
+    r = ol.scf_run(g, g["meta"]["niter"])
+    rel = np.abs(r["traj"][:, 0] - g["traj"][:, 0]) / np.abs(g["traj"][:, 0])
+    assert rel.max() < 1e-9
+    assert relerr(r["traj"], g["traj"]) < 1e-10
+    assert np.max(np.abs(r["pairing"] - g["pairing"])) < 1e-8       # Fermi, E_pair
+    assert np.all(np.abs(g["pairing"][:, 2:]) > 0.1)                 # pairing stays on
