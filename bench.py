@@ -2,7 +2,7 @@
 
 Headline (`value`): the 40^3 / 400-state self-consistent iteration with the
 HFB pairing update (40x40x40 octant mesh, dx = 0.8 fm, 200 neutron + 200
-proton harmonic-oscillator seeds, SLy4 + Coulomb, N = 126, Z = 82, HFB with
+proton harmonic-oscillator seeds, SLy4 + Coulomb, N = 146, Z = 94, HFB with
 a 1250 MeV fm^3 contact pairing and GuessKappa), in FP64.  One step is one
 whole Evolve iteration on the device:
     GradDesc  -- hPsi of every state, heavy-ball step, energy-ordered
@@ -54,6 +54,11 @@ STAGES = ("lag_derivs", "derive_all", "hpsi_step", "hpsi", "reduce_ed", "gram", 
           "coulomb_bc", "coulomb_cg", "coulomb_direct", "potentials", "precondition",
           "pot_jobs", "pot_convert", "energy", "energy_kinetic", "kinetic", "cutoff",
           "est_start", "est_reset", "est_hpsi", "est_step", "est_finalize")
+
+# The headline nucleus: open shells in both isospins (146 neutrons, 94
+# protons), so that the pairing stays on through the run (a doubly magic
+# filling such as 126 / 82 loses its pairing and the HFB step turns trivial)
+NUCLEUS = dict(neutrons=146, protons=94)
 
 # HFB pairing of the headline workload (the Pairing namelist of the README
 # sample: contact pairing 1250 MeV fm^3, alpha 1, GuessKappa)
@@ -152,7 +157,7 @@ def measured_traffic(profile, kernel):
 
 def headline(args):
     from mocca_amd import FP64, MIXED, workloads
-    w = workloads.full_iteration()
+    w = workloads.full_iteration(**NUCLEUS)
     M, N = w["M"], w["nx"] ** 3
     r64 = full_iteration_run(w, FP64, args.steps, args.warmup, True, keep_start=True)
     r32 = full_iteration_run(w, MIXED, args.steps, args.warmup, True)
@@ -204,11 +209,11 @@ def headline(args):
 
 def hf_no_pairing(steps, warmup):
     from mocca_amd import FP64, MIXED, workloads
-    w = workloads.full_iteration()
+    w = workloads.full_iteration(**NUCLEUS)
     r64 = full_iteration_run(w, FP64, steps, warmup, False)
     r32 = full_iteration_run(w, MIXED, steps, warmup, False)
     return {
-        "workload": "HF, no pairing: the headline workload with HF filling of N=126, Z=82 in "
+        "workload": "HF, no pairing: the headline workload with HF filling of N=146, Z=94 in "
                     "place of the HFB step; SLy4 + Coulomb (direct solve); step = the whole "
                     "Evolve iteration on the device",
         "steps": steps, "warmup": warmup,
@@ -529,7 +534,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "full_iteration_hfb: 40x40x40 octant mesh (EV8), dx=0.8 fm, 400 states "
-                        "(200 n + 200 p), HO seeds, SLy4 + Coulomb (direct solve), N=126 Z=82, "
+                        "(200 n + 200 p), HO seeds, SLy4 + Coulomb (direct solve), N=146 Z=94, "
                         "HFB pairing 1250 MeV fm^3 (100-state blocks, 200 x 200 HFB matrices, "
                         "the reference's Broyden Fermi solve), MixingScheme 0; step = the whole "
                         "Evolve iteration on the device",
@@ -539,6 +544,7 @@ def main():
         "dominant_stage": h["dominant_stage"],
         "stage_shares": h["stage_shares"],
         "kernel_ms_per_step": {k: v["ms_per_step"] for k, v in r64["prof"].items()},
+        "launches_per_step": {k: v["launches_per_step"] for k, v in r64["prof"].items()},
         "mixed": h["mixed"],
         "energy_fp64_MeV": h["energy_fp64_MeV"],
         "energy_mixed_MeV": h["energy_mixed_MeV"],

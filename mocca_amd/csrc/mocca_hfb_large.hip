@@ -86,33 +86,44 @@ __device__ __forceinline__ void rr_pair(int pp, int r, int m, int &i, int &j) {
 
 }  // namespace
 
+// LDS of the Jacobi kernel in doubles before its reduction slots: (c, s),
+// pair and the two packed row offsets of each of the M2 = ceil(n / 2) pairs
+__host__ __device__ inline int jac_sh_offset(int n) {
+    const int M2 = (n + (n & 1)) / 2;
+    return (16 * M2 + 12 * M2 + 7) / 8;
+}
+
 // ---------------------------------------------------------------------------
 // Cyclic Jacobi on the symmetric n x n matrices A[b] (dense, leading
 // dimension ld, stride mstride; symmetrised on load), one workgroup each:
-// eigenvalues into w[b][.], the rotations of every round into
-// log[b][round][pair] = (c, s), the number of rounds into rounds[b].  The
-// rotation of pair (i, j) is that of Golub & Van Loan 8.4.2, applied as
-// A <- J^T A J with J = [[c, s], [-s, c]] in rows/columns (i, j); sweeps stop
-// when the off-diagonal square sum is below 1e-32 of the diagonal one.
+// diagonal into w[b][.], the final matrix into Afin[b] (dense, when given),
+// the rotations of every round into log[b][round][pair] = (c, s) with their
+// pairs in logij[b][round][pair] = i | j << 16, the number of rounds into
+// rounds[b].  The rotation of pair (i, j) is that of Golub & Van Loan 8.4.2,
+// applied as A <- J^T A J with J = [[c, s], [-s, c]] in rows/columns (i, j).
+//
+// The round-robin ordering pairs all n (+1) indices; sweeps stop when the
+// off-diagonal square sum is below 1e-32 of the diagonal one.
 // GA: the packed matrix in global memory (Ag[b]) instead of LDS.
 template <bool GA, bool ODD>
 __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
     const double *__restrict__ Ain, int ld, long mstride, const int *__restrict__ nb,
-    const int *__restrict__ active, double *__restrict__ Ag, double *__restrict__ w,
-    double2 *__restrict__ log, long logstride, int *__restrict__ rounds, int maxsweeps,
+    const int *__restrict__ active, double *__restrict__ Ag,
+    double *__restrict__ w, double *__restrict__ Afin, double2 *__restrict__ log,
+    int *__restrict__ logij, long logstride, int *__restrict__ rounds, int maxsweeps,
     int *__restrict__ status) {
     extern __shared__ __align__(16) double lds[];
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (active && !active[b]) return;
     const int n = nb[b], m = n + (n & 1), M2 = m / 2;
-    // per round: rotation (c, s) and pair (i, j, ro_i, ro_j) of each pair,
-    // ro_x = the packed offset of row x less x (element (x, y), x <= y, at
-    // ro_x + y)
+    // per round: rotation (c, s), pair i | j << 16 and the packed offsets
+    // ro(i), ro(j) of each pair
     double2 *cs = reinterpret_cast<double2 *>(lds);                      // [M2]
-    int *pij = reinterpret_cast<int *>(cs + M2);                         // [M2] i | j << 16
-    int *pri = pij + M2, *prj = pri + M2;                                // [M2] ro_i, ro_j
-    double *sh = reinterpret_cast<double *>(prj + M2 + (M2 & 1));        // [16]
+    int *pij = reinterpret_cast<int *>(cs + M2);                         // [M2]
+    int *pri = pij + M2, *prj = pri + M2;                                // [M2]
+    double *sh = reinterpret_cast<double *>(lds) + jac_sh_offset(n);     // [16]
     double *A = GA ? Ag + b * mstride : sh + 16;                         // packed upper
+    // packed offset of row x less x: element (x, y), x <= y, at ro(x) + y
     auto ro = [&](int x) { return x * n - (x * (x - 1)) / 2 - x; };
     const double *Ab = Ain + b * mstride;
     for (int q = tid; q < n * n; q += nt) {
@@ -121,6 +132,7 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
     }
     __syncthreads();
     double2 *lb = log + b * logstride;
+    int *li = logij + b * logstride;
     // 2x2 blocks (k, l), k <= l, over threads: column pairs u and M2-1-u are
     // folded into one list of M2+1 blocks (M2/2+... lists of equal length),
     // GT threads per list
@@ -170,14 +182,15 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
                 pri[pp] = ri;
                 prj[pp] = rj;
                 lb[(long)done * M2 + pp] = make_double2(c, s);
+                li[(long)done * M2 + pp] = i | (j << 16);
             }
             __syncthreads();
             // the column pair of each of this thread's two lists
             const double2 cl1 = worker ? cs[l1] : make_double2(1.0, 0.0);
             const double2 cl2 = worker ? cs[l2] : make_double2(1.0, 0.0);
-            const int4 Q1 = worker ? make_int4(pij[l1] & 0xffff, pij[l1] >> 16, pri[l1], prj[l1])
+            const int4 R1 = worker ? make_int4(pij[l1] & 0xffff, pij[l1] >> 16, pri[l1], prj[l1])
                                    : make_int4(0, 0, 0, 0);
-            const int4 Q2 = worker ? make_int4(pij[l2] & 0xffff, pij[l2] >> 16, pri[l2], prj[l2])
+            const int4 R2 = worker ? make_int4(pij[l2] & 0xffff, pij[l2] >> 16, pri[l2], prj[l2])
                                    : make_int4(0, 0, 0, 0);
             // kJacBatch blocks at a time: their tables, then their elements,
             // are read back to back so that the LDS latencies overlap; every
@@ -195,7 +208,7 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
                     if (q >= cnt) continue;
                     const bool first = q <= l1;
                     const int k = first ? q : q - l1 - 1, l = first ? l1 : l2;
-                    const int4 Q = first ? Q1 : Q2;
+                    const int4 Q = first ? R1 : R2;
                     cl[t] = first ? cl1 : cl2;
                     ck[t] = cs[k];
                     // a block whose two rotations are both the identity
@@ -236,7 +249,7 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
 #pragma unroll
                 for (int t = 0; t < kJacBatch; ++t) {
                     if (kind[t] == 0) continue;
-                    const double2 a = ck[t], b = cl[t];
+                    const double2 a = ck[t], bq = cl[t];
                     if (kind[t] == 2) {
                         // J^T [[aii, aij], [aij, ajj]] J, off-diagonal exactly 0
                         const double aii = x[t][0], aij = x[t][1], ajj = x[t][3];
@@ -251,10 +264,10 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
                     const double a1 = a.x * x[t][1] - a.y * x[t][3];
                     const double b0 = a.y * x[t][0] + a.x * x[t][2];
                     const double b1 = a.y * x[t][1] + a.x * x[t][3];
-                    A[e[t][0]] = a0 * b.x - a1 * b.y;
-                    if (!ODD || e[t][1] >= 0) A[e[t][1]] = a0 * b.y + a1 * b.x;
-                    if (!ODD || e[t][2] >= 0) A[e[t][2]] = b0 * b.x - b1 * b.y;
-                    if (!ODD || e[t][3] >= 0) A[e[t][3]] = b0 * b.y + b1 * b.x;
+                    A[e[t][0]] = a0 * bq.x - a1 * bq.y;
+                    if (!ODD || e[t][1] >= 0) A[e[t][1]] = a0 * bq.y + a1 * bq.x;
+                    if (!ODD || e[t][2] >= 0) A[e[t][2]] = b0 * bq.x - b1 * bq.y;
+                    if (!ODD || e[t][3] >= 0) A[e[t][3]] = b0 * bq.y + b1 * bq.x;
                 }
             }
             __syncthreads();
@@ -263,6 +276,13 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
     }
     double *wb = w + (long)b * ld;
     for (int i = tid; i < n; i += nt) wb[i] = A[ro(i) + i];
+    if (Afin) {
+        double *Fb = Afin + b * mstride;
+        for (int q = tid; q < n * n; q += nt) {
+            const int i = q / n, j = q % n;
+            Fb[(long)i * ld + j] = i <= j ? A[ro(i) + j] : A[ro(j) + i];
+        }
+    }
     if (tid == 0) {
         rounds[b] = done;
         if (!conv) atomicOr(status, 8);
@@ -270,26 +290,26 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
 }
 
 // The logged rotations applied to the basis: V[b] <- V[b] J_1 J_2 ... (in
-// place; V = identity first when ident).  kApplyLanes lanes own one row of
-// the basis in LDS and take disjoint rotations of each round (pairs
-// q, q + kApplyLanes, ...), so a round touches disjoint columns of a row and
-// the rounds follow one another without a barrier: the lanes of a row are in
-// one wave, whose LDS accesses complete in order.  Round g of the log is
-// round g mod (m-1) of the ordering; a lane advances its pairs' positions by
-// one per round (position x > 0 moves to x mod (m-1) + 1).
+// place; V = identity first when ident or cold[b]).  kApplyLanes lanes own
+// one row of the basis in LDS and take disjoint rotations of each round
+// (pairs q, q + kApplyLanes, ...), so a round touches disjoint columns of a
+// row and the rounds follow one another without a barrier: the lanes of a
+// row are in one wave, whose LDS accesses complete in order.  The pairs of
+// each round come from the pair log.
 constexpr int kApplyLanes = 16;
 constexpr int kApplyMaxPairs = 128 / kApplyLanes;
 constexpr int kApplyChunk = 16;        // rounds of the log staged in LDS at a time
 size_t apply_lds(int n) {
     const int M2 = (n + (n & 1)) / 2;
-    return sizeof(double) * kApplyRows * n + sizeof(double2) * kApplyChunk * M2;
+    return sizeof(double) * kApplyRows * n + (sizeof(double2) + sizeof(int)) * kApplyChunk * M2;
 }
 __global__ __launch_bounds__(256) void k_lj_apply(double *__restrict__ V, int ld, long mstride,
                                                   const int *__restrict__ nb,
                                                   const int *__restrict__ active,
                                                   const double2 *__restrict__ log,
+                                                  const int *__restrict__ logij,
                                                   long logstride, const int *__restrict__ rounds,
-                                                  int ident) {
+                                                  int ident, const int *__restrict__ cold) {
     extern __shared__ __align__(16) double lds_apply[];
     const int b = blockIdx.y, tid = threadIdx.x;
     if (active && !active[b]) return;
@@ -299,40 +319,38 @@ __global__ __launch_bounds__(256) void k_lj_apply(double *__restrict__ V, int ld
     const int nr = min(kApplyRows, n - r0);
     double2 *lgc = reinterpret_cast<double2 *>(lds_apply);   // [kApplyChunk][M2]
     double *rows = reinterpret_cast<double *>(lgc + kApplyChunk * M2);   // [kApplyRows][n]
+    int *lij = reinterpret_cast<int *>(rows + kApplyRows * n);          // [kApplyChunk][M2]
     double *Vb = V + b * mstride;
+    const bool id = ident || (cold && cold[b]);
     for (int e = tid; e < nr * n; e += blockDim.x) {
         const int r = e / n, c = e % n;
-        rows[e] = ident ? (r0 + r == c ? 1.0 : 0.0) : Vb[(long)(r0 + r) * ld + c];
+        rows[e] = id ? (r0 + r == c ? 1.0 : 0.0) : Vb[(long)(r0 + r) * ld + c];
     }
     const int row = tid / kApplyLanes, q0 = tid % kApplyLanes;
     double *rw = rows + row * n;
     const bool mine = row < nr;
-    // positions of this lane's pairs (players pp and m-1-pp) in the current
-    // round; player x > 0 moves to x mod (m-1) + 1 from one round to the next
-    int pa[kApplyMaxPairs], pb[kApplyMaxPairs];
-#pragma unroll
-    for (int t = 0; t < kApplyMaxPairs; ++t) {
-        const int pp = q0 + kApplyLanes * t;
-        pa[t] = pp;
-        pb[t] = m - 1 - pp;
-    }
     const double2 *lb = log + b * logstride;
+    const int *lb2 = logij + b * logstride;
     const int R = rounds[b];
     for (int g0 = 0; g0 < R; g0 += kApplyChunk) {
         const int nc = min(kApplyChunk, R - g0);
         __syncthreads();
-        for (int e = tid; e < nc * M2; e += blockDim.x) lgc[e] = lb[(long)g0 * M2 + e];
+        for (int e = tid; e < nc * M2; e += blockDim.x) {
+            lgc[e] = lb[(long)g0 * M2 + e];
+            lij[e] = lb2[(long)g0 * M2 + e];
+        }
         __syncthreads();
         if (!mine) continue;
         for (int gc = 0; gc < nc; ++gc) {
             const double2 *lg = lgc + gc * M2;
+            const int *lp = lij + gc * M2;
 #pragma unroll
             for (int t = 0; t < kApplyMaxPairs; ++t) {
                 const int pp = q0 + kApplyLanes * t;
                 if (pp < M2) {
                     const double2 cs = lg[pp];
-                    int i = pa[t], j = pb[t];
-                    if (i > j) { const int x = i; i = j; j = x; }
+                    const int ij = lp[pp];
+                    const int i = ij & 0xffff, j = ij >> 16;
                     if (j < n && cs.y != 0.0) {
                         const double vi = rw[i], vj = rw[j];
                         rw[i] = cs.x * vi - cs.y * vj;
@@ -343,12 +361,6 @@ __global__ __launch_bounds__(256) void k_lj_apply(double *__restrict__ V, int ld
             // the lanes of a row are in one wave, whose LDS accesses
             // complete in order: the next round may read what this one wrote
             __builtin_amdgcn_wave_barrier();
-            // x in 1 .. m-1 moves to x mod (m-1) + 1: x + 1, or 1 after m-1
-#pragma unroll
-            for (int t = 0; t < kApplyMaxPairs; ++t) {
-                if (pa[t] != 0) pa[t] = pa[t] == m - 1 ? 1 : pa[t] + 1;
-                pb[t] = pb[t] == m - 1 ? 1 : pb[t] + 1;
-            }
         }
     }
     __syncthreads();
@@ -591,22 +603,41 @@ __global__ void k_hl_build_T(HfbK k, const double *__restrict__ E,
     Tm[(long)b * ld * ld + (long)i * ld + j] = v;
 }
 
-// The next evaluation's matrix in the current eigenbasis:
-// A = diag(w) - delta_it C (T changes by -Sigma_z dlambda between
-// evaluations of one Fermi solve).
-__global__ void k_hl_shift_A(HfbK k, const double *__restrict__ w,
+// The next evaluation's start matrix.  In the current basis it is
+// A = Afin - dlambda C (Afin the last Jacobi's matrix, diagonal to its
+// convergence threshold; T changes by -Sigma_z dlambda between evaluations
+// of one Fermi solve); when dlambda is so large that its
+// off-diagonal part exceeds that of T(lambda') itself (the gaps), the
+// evaluation starts from T(lambda') in the original basis instead (cold[b]
+// = 1: the rotations are then applied to the identity).
+__global__ void k_hl_shift_A(HfbK k, const double *__restrict__ Af,
                              const double *__restrict__ C, const double *__restrict__ nst,
-                             const int *__restrict__ active, int ld, double *__restrict__ A) {
+                             const int *__restrict__ active, int ld, double *__restrict__ A,
+                             const double *__restrict__ E, const double *__restrict__ delta,
+                             const double *__restrict__ off, int *__restrict__ cold) {
     const int b = blockIdx.y;
     if (!active[b]) return;
     const HfbBlockK &bk = k.blk[b];
-    const int n = 2 * bk.h;
+    const int h = bk.h, n = 2 * h;
+    const double dl = nst[bk.it * kHlNewton + 6];
+    const bool c0 = dl * dl * off[2 * b + 1] > off[2 * b];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) cold[b] = c0 ? 1 : 0;
     if (e >= n * n) return;
     const int i = e / n, j = e % n;
-    const double dl = nst[bk.it * kHlNewton + 6];
     const long o = (long)b * ld * ld + (long)i * ld + j;
-    A[o] = (i == j ? w[(long)b * ld + i] : 0.0) - dl * C[o];
+    if (!c0) {
+        A[o] = Af[o] - dl * C[o];
+        return;
+    }
+    const double *dd = delta + bk.off;
+    const double l = nst[bk.it * kHlNewton + 0];   // the next evaluation's lambda
+    double v;
+    if (i < h && j < h) v = i == j ? E[k.state[bk.first + i]] - l : 0.0;
+    else if (i >= h && j >= h) v = i == j ? -(E[k.state[bk.first + i - h]] - l) : 0.0;
+    else if (i < h) v = dd[i * n + h + (j - h)];
+    else v = dd[j * n + h + (i - h)];
+    A[o] = v;
 }
 
 // Particle number of each block and its derivative: ranks of the
@@ -618,7 +649,9 @@ __global__ __launch_bounds__(256) void k_hl_number(HfbK k, const double *__restr
                                                    const double *__restrict__ w, int ld,
                                                    const int *__restrict__ active,
                                                    int *__restrict__ rank,
-                                                   double *__restrict__ num) {
+                                                   double *__restrict__ num,
+                                                   const double *__restrict__ delta,
+                                                   double *__restrict__ off) {
     __shared__ double sh[4];
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (!active[b]) return;
@@ -652,9 +685,30 @@ __global__ __launch_bounds__(256) void k_hl_number(HfbK k, const double *__restr
     }
     nn = hl_block_sum(nn, sh);
     dn = hl_block_sum(dn, sh);
+    // off-diagonal square sums of the two possible start matrices of the
+    // next evaluation: T(lambda') in the original basis (the gaps, both
+    // triangles; independent of lambda) and C (times dlambda^2 for
+    // diag(w) - dlambda C in the current eigenbasis)
+    double od = 0, oc = 0;
+    const double *dl = delta + bk.off;
+    for (int e = tid; e < h * h; e += nt) {
+        const double d = dl[(e / h) * n + h + e % h];
+        od += 2.0 * d * d;
+    }
+    for (int e = tid; e < n * n; e += nt) {
+        const int a = e / n, q = e % n;
+        if (a != q) {
+            const double c = Cb[(long)a * ld + q];
+            oc += c * c;
+        }
+    }
+    od = hl_block_sum(od, sh);
+    oc = hl_block_sum(oc, sh);
     if (tid == 0) {
         num[2 * b] = 2.0 * nn;
         num[2 * b + 1] = 2.0 * dn;
+        off[2 * b] = od;
+        off[2 * b + 1] = oc;
     }
 }
 
@@ -956,9 +1010,7 @@ int pair_gram_groups(int nt) {
 bool hfb_large_fits(int hmax) { return hmax >= 1 && (hmax + 15) / 16 <= 8; }
 
 size_t hfb_large_jacobi_lds(int n) {
-    const int m = n + (n & 1), M2 = m / 2;
-    return sizeof(double) * ((size_t)n * (n + 1) / 2 + 2 * (size_t)M2 + 16) +
-           sizeof(int) * (3 * (size_t)M2 + (M2 & 1));
+    return sizeof(double) * ((size_t)jac_sh_offset(n) + 16 + (size_t)n * (n + 1) / 2);
 }
 
 int hfb_large_setup(mocca_solve *h) {
@@ -985,8 +1037,12 @@ int hfb_large_setup(mocca_solve *h) {
         (rc = dalloc(h, &p.lwc, (size_t)k.nblk * ld * sizeof(double))) ||
         (rc = dalloc(h, &p.lrank, (size_t)k.nblk * ld * sizeof(int))) ||
         (rc = dalloc(h, &p.llog, (size_t)k.nblk * p.lmax * sizeof(double2))) ||
+        (rc = dalloc(h, &p.llogij, (size_t)k.nblk * p.lmax * sizeof(int))) ||
+        (rc = dalloc(h, &p.lAf, mat * sizeof(double))) ||
         (rc = dalloc(h, &p.lrounds, 2 * (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.lnum, 2 * (size_t)k.nblk * sizeof(double))) ||
+        (rc = dalloc(h, &p.loff, 2 * (size_t)k.nblk * sizeof(double))) ||
+        (rc = dalloc(h, &p.lcold, (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.lnst, 2 * kHlNewton * sizeof(double))) ||
         (rc = dalloc(h, &p.lactive, (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.lnb, 2 * (size_t)k.nblk * sizeof(int))) ||
@@ -1017,7 +1073,7 @@ int hfb_large_free(mocca_solve *h) {
     Hfb &p = h->hfb;
     void *ptrs[] = {p.lT, p.lV, p.lA, p.lW, p.lC, p.lVc, p.lw, p.lwc, p.lrank, p.llog,
                     p.lrounds, p.lnum, p.lnst, p.lactive, p.lnb, p.lhb, p.llist, p.lTap,
-                    p.lgpart};
+                    p.lgpart, p.loff, p.lcold, p.llogij, p.lAf};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     return 0;
@@ -1028,7 +1084,8 @@ namespace {
 // Jacobi + rotation apply on nblk matrices of sizes nb[] (A, leading
 // dimension ld) with the basis V (identity first when ident).
 int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int *nb, int nmax,
-               const int *active, int *rounds, bool ident, bool odd) {
+               const int *active, int *rounds, bool ident, bool odd, const int *cold,
+               double *Afin) {
     Hfb &p = h->hfb;
     const int nblk = p.k.nblk, ld = p.nmax;
     const long ms = (long)ld * ld;
@@ -1041,26 +1098,25 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
             auto kern = odd ? k_lj_jacobi<false, true> : k_lj_jacobi<false, false>;
             require_lds((const void *)kern, top, "k_lj_jacobi");
             hipLaunchKernelGGL(kern, dim3(nblk), dim3(kJacThreads), shm, h->st, A, ld, ms, nb,
-                               active, nullptr, w, p.llog, p.lmax, rounds, kJacMaxSweeps,
-                               p.status);
+                               active, nullptr, w, Afin, p.llog, p.llogij, p.lmax, rounds,
+                               kJacMaxSweeps, p.status);
         });
     } else {
-        const int m = nmax + (nmax & 1), M2 = m / 2;
-        const size_t shm = sizeof(double) * (2 * (size_t)M2 + 16) +
-                           sizeof(int) * (3 * (size_t)M2 + (M2 & 1));
+        const size_t shm = sizeof(double) * ((size_t)jac_sh_offset(nmax) + 16);
         rc = run(h, "hfb_jacobi", [&] {
             auto kern = odd ? k_lj_jacobi<true, true> : k_lj_jacobi<true, false>;
             hipLaunchKernelGGL(kern, dim3(nblk), dim3(kJacThreads), shm, h->st, A, ld, ms, nb,
-                               active, p.lW, w, p.llog, p.lmax, rounds, kJacMaxSweeps,
-                               p.status);
+                               active, p.lW, w, Afin, p.llog, p.llogij, p.lmax, rounds,
+                               kJacMaxSweeps, p.status);
         });
     }
     if (rc) return rc;
     return run(h, "hfb_jacobi_apply", [&] {
         const size_t shm = apply_lds(nmax);
+        require_lds((const void *)k_lj_apply, shm, "k_lj_apply");
         hipLaunchKernelGGL(k_lj_apply, dim3((nmax + kApplyRows - 1) / kApplyRows, nblk),
-                           dim3(256), shm, h->st, V, ld, ms, nb, active, p.llog, p.lmax,
-                           rounds, ident ? 1 : 0);
+                           dim3(256), shm, h->st, V, ld, ms, nb, active, p.llog, p.llogij,
+                           p.lmax, rounds, ident ? 1 : 0, cold);
     });
 }
 
@@ -1120,13 +1176,14 @@ int hfb_large_solve_impl(mocca_solve *h) {
     // (Broyden: two evaluations before its HFBIter iterations)
     for (int ev = 0; ev < std::max(1, k.iters) + 3; ++ev) {
         if ((rc = jacobi_run(h, ev == 0 ? A0 : p.lA, p.lV, p.lw, p.lnb, p.nmax, p.lactive,
-                             p.lrounds, ev == 0 && !p.lwarm, false)))
+                             p.lrounds, ev == 0 && !p.lwarm, false,
+                             ev == 0 ? nullptr : p.lcold, p.lAf)))
             return rc;
         rc = run(h, "hfb_fermi", [&] {
             hipLaunchKernelGGL((k_lj_gemm<true, true>), gmat, dim3(256), 0, h->st, p.lV, p.lV,
                                p.lC, ld, ms, p.lnb, p.lhb, p.lactive);
             hipLaunchKernelGGL(k_hl_number, dim3(nblk), dim3(256), 0, h->st, p.k, p.lV, p.lC,
-                               p.lw, ld, p.lactive, p.lrank, p.lnum);
+                               p.lw, ld, p.lactive, p.lrank, p.lnum, p.delta, p.loff);
             if (p.newton)
                 hipLaunchKernelGGL(k_hl_newton, dim3(1), dim3(64), 0, h->st, p.k, p.lnum,
                                    p.lnst, p.lactive, p.fermi, p.status);
@@ -1134,7 +1191,8 @@ int hfb_large_solve_impl(mocca_solve *h) {
                 hipLaunchKernelGGL(k_hl_broyden, dim3(1), dim3(64), 0, h->st, p.k, p.lnum,
                                    p.lnst, p.lactive, p.fermi, p.status);
             hipLaunchKernelGGL(k_hl_shift_A, dim3((nel + 255) / 256, nblk), dim3(256), 0,
-                               h->st, p.k, p.lw, p.lC, p.lnst, p.lactive, ld, p.lA);
+                               h->st, p.k, p.lAf, p.lC, p.lnst, p.lactive, ld, p.lA, h->b.E,
+                               p.delta, p.loff, p.lcold);
         });
         if (rc) return rc;
         double st[2 * kHlNewton];
@@ -1157,7 +1215,7 @@ int hfb_large_solve_impl(mocca_solve *h) {
     bool odd = false;
     for (int b = 0; b < nblk; ++b) odd = odd || (k.blk[b].h & 1);
     if ((rc = jacobi_run(h, p.lT, p.lVc, p.lwc, p.lnb + nblk, hmax, nullptr,
-                         p.lrounds + nblk, true, odd)))
+                         p.lrounds + nblk, true, odd, nullptr, nullptr)))
         return rc;
     return run(h, "hfb_canonical", [&] {
         hipLaunchKernelGGL(k_hl_can_finish, dim3(nblk), dim3(256), 0, h->st, p.k, p.lVc, p.lwc,

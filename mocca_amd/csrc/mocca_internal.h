@@ -153,8 +153,12 @@ struct Hfb {
     double *lw = nullptr, *lwc = nullptr;   // eigenvalues [nblk][nmax]
     int *lrank = nullptr;          // [nblk][nmax] eigenvalue ranks (descending)
     double2 *llog = nullptr;       // [nblk][lmax] Jacobi rotations
+    int *llogij = nullptr;         // [nblk][lmax] their pairs, i | j << 16
+    double *lAf = nullptr;         // [nblk][ld][ld] the matrix after the last Jacobi
     int *lrounds = nullptr;        // [2][nblk] rounds logged (HFB, canonical)
     double *lnum = nullptr;        // [nblk][2] particle number, derivative
+    double *loff = nullptr;        // [nblk][2] off-diagonal square sums of T and C
+    int *lcold = nullptr;          // [nblk] evaluation started in the original basis
     double *lnst = nullptr;        // [2][kHlNewton] Newton records
     int *lactive = nullptr;        // [nblk] block still evaluated
     int *lnb = nullptr;            // [2][nblk] 2h, h

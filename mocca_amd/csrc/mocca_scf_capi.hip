@@ -926,8 +926,10 @@ int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
     // row slots of the replays, the states before an AlternateStep) are
     // allocated here, and the iterations in which a solve's graphs are
     // captured (one per configuration of the rotating buffers, so the first
-    // kWarm) are issued here too.  The workers then only replay captured
-    // graphs or run iterations eagerly, while no capture is open anywhere.
+    // kWarm) are issued here too, as is the first iteration of a solve that
+    // has not run in a batch before (whatever it sets up lazily).  The
+    // workers then only replay captured graphs or run iterations eagerly,
+    // while no capture is open anywhere.
     constexpr int kWarm = 4;     // 1 eager + one capture per buffer configuration (period <= 3)
     std::vector<char> use_graph(nsolves, 0);
     for (int i = 0; i < nsolves; ++i) {
@@ -982,23 +984,54 @@ int mocca_batch_run(mocca_solve *const *solves, int nsolves, int niter,
         if (!rc) ++h->scf.iteration;
         return rc;
     };
+    // whether the graphs a solve captured in earlier calls cover every
+    // pointer configuration from its current one on (a closed cycle), so
+    // that its later iterations need no capture
+    auto covered = [&](int i) {
+        const mocca_solve *h = solves[i];
+        void *cur[4], *c[4];
+        ptr_state(const_cast<mocca_solve *>(h), cur);
+        std::copy(cur, cur + 4, c);
+        for (size_t n = 0; n < h->graphs.size(); ++n) {
+            const IterGraph *g = nullptr;
+            for (const IterGraph &x : h->graphs)
+                if (std::equal(c, c + 4, x.pre)) g = &x;
+            if (!g) return false;
+            std::copy(g->post, g->post + 4, c);
+            if (std::equal(c, c + 4, cur)) return true;
+        }
+        return false;
+    };
     const int kw = std::min(niter, kWarm);
-    for (int k = 1; k <= kw; ++k)
+    // first iteration a worker issues: 1 for solves whose earlier calls
+    // captured every graph they need (nothing left to capture or
+    // allocate); otherwise this thread runs iteration 1 and, for graph
+    // solves, the capturing iterations up to kw
+    std::vector<int> k0(nsolves, kw + 1);
+    for (int i = 0; i < nsolves; ++i)
+        if (use_graph[i] && covered(i)) k0[i] = 1;
+    for (int i = 0; i < nsolves; ++i)
+        if (k0[i] > 1)
+            if (int rc = step(i, 1, true)) return failed(i, rc);
+    for (int i = 0; i < nsolves; ++i)
+        if (k0[i] > 1 && (!use_graph[i] || covered(i))) k0[i] = 2;
+    for (int k = 2; k <= kw; ++k)
         for (int i = 0; i < nsolves; ++i)
-            if (int rc = step(i, k, true)) return failed(i, rc);
+            if (k0[i] > kw)
+                if (int rc = step(i, k, true)) return failed(i, rc);
     // The remaining iterations, split over a few host threads, each issuing
     // its share of the solves in turn: one thread cannot issue the kernels
     // of many small solves as fast as the device runs them.
     auto run_range = [&](int i0, int i1, int &rc, int &bad) {
-        for (int k = kw + 1; k <= niter && !rc; ++k)
+        for (int k = 1; k <= niter && !rc; ++k)
             for (int i = i0; i < i1 && !rc; ++i)
-                if ((rc = step(i, k, false))) bad = i;
+                if (k >= k0[i] && (rc = step(i, k, false))) bad = i;
     };
     const int nthr = std::max(1, std::min({nsolves, 8,
                                            (int)std::thread::hardware_concurrency() / 2}));
     std::vector<int> trc(nthr, 0), tbad(nthr, -1);
     std::vector<std::thread> pool;
-    if (niter > kw) {
+    if (niter >= 1) {
         for (int t = 1; t < nthr; ++t)
             pool.emplace_back(run_range, nsolves * t / nthr, nsolves * (t + 1) / nthr,
                               std::ref(trc[t]), std::ref(tbad[t]));

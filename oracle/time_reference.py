@@ -8,7 +8,7 @@ the two wall-clock times over K, so start-up and file I/O cancel.  The
 reference is serial (MKL only inside off-path LAPACK calls): one core, with
 MKL and OpenMP pinned to one thread.
 
-    python oracle/time_reference.py [K] > profiles/<summary>.md
+    python oracle/time_reference.py [K [neutrons protons]] > profiles/<summary>.md
 
 Needs the reference sources (to have built the driver) and forces.param;
 the GPU side never runs this.
@@ -34,7 +34,8 @@ PAIRING = 'Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, GuessKappa=.true.'
 
 def main():
     K = int(sys.argv[1]) if len(sys.argv) > 1 else 2
-    w = workloads.full_iteration()
+    nz = (int(sys.argv[2]), int(sys.argv[3])) if len(sys.argv) > 3 else (146, 94)
+    w = workloads.full_iteration(neutrons=nz[0], protons=nz[1])
     n, dx, M = w["nx"], w["dx"], w["M"]
     f = w["functional"]
     qns = [tuple(int(x) for x in q) for q in w["qn"]]
@@ -88,14 +89,14 @@ def main():
     except OSError:
         pass
     print("# Reference build, 40^3 mesh / 400 states with HFB pairing, one core\n")
-    print("Command: `python oracle/time_reference.py %d` (build container; the reference\n"
-          "sources are not on the GPU host)\n" % K)
+    print("Command: `python oracle/time_reference.py %d %d %d` (build container; the "
+          "reference\nsources are not on the GPU host)\n" % (K, nz[0], nz[1]))
     print("| item | value |\n|---|---|")
     print(f"| binary | oracle/_ref/mocca_ref_driver (amdflang, oracle/ref.mk flags) |")
     print(f"| host | {cpu}, {os.cpu_count()} logical CPUs, {platform.machine()} |")
     print("| threads | 1 (OMP_NUM_THREADS=MKL_NUM_THREADS=1; the Evolve loop is serial) |")
     print("| workload | 40x40x40 octant mesh (EV8), dx=0.8 fm, 400 states (200 n + 200 p), "
-          "HO seeds (workloads.full_iteration), SLy4, N=126 Z=82, HFB 1250 MeV fm^3 "
+          f"HO seeds (workloads.full_iteration), SLy4, N={nz[0]} Z={nz[1]}, HFB 1250 MeV fm^3 "
           "(GuessKappa), reference Coulomb default, fixed dt/momentum, MixingScheme 0 |")
     print(f"| wall time, 1 iteration (start-up included) | {times[1]:.2f} s |")
     print(f"| wall time, {1 + K} iterations | {times[1 + K]:.2f} s |")
