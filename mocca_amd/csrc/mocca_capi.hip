@@ -359,7 +359,7 @@ int kinetic_impl(mocca_solve *h) {
             launch_hpsi_fd<T>(m, psi, nullptr, nullptr, nullptr,
                               (const T *)h->b.fields, h->b.qn, h->b.tasks, h->ntasks, h->TY,
                               h->ntile_y, h->mesh.time_reversal, T(h->kin[0]), T(h->kin[1]),
-                              T(0), T(0), 0, h->b.eparts, h->st);
+                              T(0), T(0), kKinetic, h->b.eparts, h->st);
         });
     } else {
         slots = h->ntiles;

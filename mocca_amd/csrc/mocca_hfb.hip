@@ -512,60 +512,81 @@ __global__ __launch_bounds__(kHfbThreads) void k_hfb_can_states(HfbK k, const T 
 // test is made on the gaps as well: Jacobi leaves couplings below its
 // stopping threshold untouched.  first = 1 instead sets rho_old, kappa_old
 // to the current ones while rho_old is all zero (SolvePairing :800-806).
+// HFBMix (SolvePairing :800-806, :820-840) over every element of the packed
+// blocks, in three launches (the flags are reduced over the whole grid):
+// zero the flags; flags[0] = any old rho != 0, flags[1] = any kappa or
+// delta != 0; then with first = 1 the old matrices take the current ones
+// while the old rho is all zero, and with first = 0 rho and kappa are mixed
+// with the old ones (each where its flag is set) and become the old ones.
+__global__ void k_hfb_mix_zero(int *__restrict__ flags) {
+    if (threadIdx.x < 2) flags[threadIdx.x] = 0;
+}
+
+__global__ void k_hfb_mix_flags(int total, const double *__restrict__ kappa,
+                                const double *__restrict__ orho,
+                                const double *__restrict__ delta, int *__restrict__ flags) {
+    int f0 = 0, f1 = 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        f0 |= orho[e] != 0.0;
+        f1 |= kappa[e] != 0.0 || delta[e] != 0.0;
+    }
+    f0 = __syncthreads_or(f0);
+    f1 = __syncthreads_or(f1);
+    if (threadIdx.x == 0) {
+        if (f0) atomicOr(flags, 1);
+        if (f1) atomicOr(flags + 1, 1);
+    }
+}
+
 __global__ void k_hfb_mix(int total, double mix, int first, double *__restrict__ rho,
                           double *__restrict__ kappa, double *__restrict__ orho,
-                          double *__restrict__ okappa, const double *__restrict__ delta) {
-    __shared__ int flags[2];
-    const int tid = threadIdx.x;
-    if (first) {
-        if (tid == 0) flags[0] = 0;
-        __syncthreads();
-        for (int e = tid; e < total; e += blockDim.x)
-            if (orho[e] != 0.0) flags[0] = 1;
-        __syncthreads();
-        if (!flags[0])
-            for (int e = tid; e < total; e += blockDim.x) {
+                          double *__restrict__ okappa, const int *__restrict__ flags) {
+    const bool f0 = flags[0] != 0, f1 = flags[1] != 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        if (first) {
+            if (!f0) {
                 orho[e] = rho[e];
                 okappa[e] = kappa[e];
             }
-        return;
-    }
-    if (tid == 0) { flags[0] = 0; flags[1] = 0; }
-    __syncthreads();
-    for (int e = tid; e < total; e += blockDim.x) {
-        if (orho[e] != 0.0) flags[0] = 1;
-        if (kappa[e] != 0.0 || delta[e] != 0.0) flags[1] = 1;
-    }
-    __syncthreads();
-    for (int e = tid; e < total; e += blockDim.x) {
-        if (flags[0]) rho[e] = mix * rho[e] + (1.0 - mix) * orho[e];
-        if (flags[1]) kappa[e] = mix * kappa[e] + (1.0 - mix) * okappa[e];
+            continue;
+        }
+        if (f0) rho[e] = mix * rho[e] + (1.0 - mix) * orho[e];
+        if (f1) kappa[e] = mix * kappa[e] + (1.0 - mix) * okappa[e];
         orho[e] = rho[e];
         okappa[e] = kappa[e];
     }
 }
 
 // Pairing energy E_q = 1/2 sum_blocks sum_ij Delta_ij kappa_ij (HFBEnergy
-// :296-348) into the energy row: row[43..44] = E_pair, row[45..46] = Fermi.
-__global__ void k_hfb_energy(HfbK k, const double *__restrict__ delta,
-                             const double *__restrict__ kappa, const double *__restrict__ fermi,
-                             double *__restrict__ row) {
+// :296-348): partial sums of kHfbEnergyParts workgroups per block, then one
+// workgroup adds them in a fixed order into the energy row: row[43..44] =
+// E_pair, row[45..46] = Fermi.
+__global__ void k_hfb_energy_part(HfbK k, const double *__restrict__ delta,
+                                  const double *__restrict__ kappa, double *__restrict__ part) {
     __shared__ double sh[kHfbThreads / 64];
+    const int b = blockIdx.y;
+    const HfbBlockK &bk = k.blk[b];
+    const int n = 2 * bk.h;
+    double s = 0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n * n; q += gridDim.x * blockDim.x)
+        s += delta[bk.off + q] * kappa[bk.off + q];
+    s = hfb_block_sum(s, sh);
+    if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_hfb_energy(HfbK k, const double *__restrict__ part, int nparts,
+                             const double *__restrict__ fermi, double *__restrict__ row) {
+    if (threadIdx.x != 0) return;
     double e[2] = {0, 0};
     for (int b = 0; b < k.nblk; ++b) {
-        const HfbBlockK &bk = k.blk[b];
-        const int n = 2 * bk.h;
         double s = 0;
-        for (int q = threadIdx.x; q < n * n; q += blockDim.x)
-            s += delta[bk.off + q] * kappa[bk.off + q];
-        e[bk.it] += hfb_block_sum(s, sh);
+        for (int x = 0; x < nparts; ++x) s += part[b * nparts + x];
+        e[k.blk[b].it] += s;
     }
-    if (threadIdx.x == 0) {
-        row[43] = -0.5 * (-e[0]);
-        row[44] = -0.5 * (-e[1]);
-        row[45] = fermi[0];
-        row[46] = fermi[1];
-    }
+    row[43] = -0.5 * (-e[0]);
+    row[44] = -0.5 * (-e[1]);
+    row[45] = fermi[0];
+    row[46] = fermi[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -593,7 +614,8 @@ int hfb_free(mocca_solve *h) {
     Hfb &p = h->hfb;
     hfb_large_free(h);
     void *ptrs[] = {p.kappa, p.rho, p.orho, p.okappa, p.delta, p.fermi, p.pc, p.dfac, p.field,
-                    p.qpe, p.tcan, p.can_psi, p.can_occ, p.work, p.status, p.tables};
+                    p.qpe, p.tcan, p.can_psi, p.can_occ, p.work, p.status, p.tables,
+                    p.mflags, p.epart};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p = Hfb{};
@@ -608,10 +630,7 @@ int hfb_solve_impl(mocca_solve *h, bool first) {
     (void)first;
     // SolvePairing :800-806: old rho, kappa take the current ones while the
     // old rho is all zero (the first call)
-    int rc = run(h, "hfb_mix", [&] {
-        hipLaunchKernelGGL(k_hfb_mix, dim3(1), dim3(256), 0, h->st, p.total, p.k.mix, 1, p.rho,
-                           p.kappa, p.orho, p.okappa, p.delta);
-    });
+    int rc = run(h, "hfb_mix", [&] { hfb_launch_mix(h, 1); });
     if (rc) return rc;
     rc = run(h, "hfb_field", [&] {
         hipLaunchKernelGGL(k_hfb_cutoffs, dim3((M + 63) / 64), dim3(64), 0, h->st, p.k, h->b.E,
@@ -640,8 +659,7 @@ int hfb_solve_impl(mocca_solve *h, bool first) {
         hipLaunchKernelGGL(k_hfb_can_states<T>, dim3((N + kHfbThreads - 1) / kHfbThreads, M),
                            dim3(kHfbThreads), 0, h->st, p.k, (const T *)h->b.psi, N, p.tcan,
                            (T *)p.can_psi);
-        hipLaunchKernelGGL(k_hfb_mix, dim3(1), dim3(256), 0, h->st, p.total, p.k.mix, 0, p.rho,
-                           p.kappa, p.orho, p.okappa, p.delta);
+        hfb_launch_mix(h, 0);
     });
     return rc;
 }
@@ -654,8 +672,12 @@ int hfb_solve_any(mocca_solve *h, bool first) {
 
 void hfb_launch_mix(mocca_solve *h, int first) {
     Hfb &p = h->hfb;
-    hipLaunchKernelGGL(k_hfb_mix, dim3(1), dim3(256), 0, h->st, p.total, p.k.mix, first, p.rho,
-                       p.kappa, p.orho, p.okappa, p.delta);
+    const int grid = std::max(1, std::min(1024, (p.total + 255) / 256));
+    hipLaunchKernelGGL(k_hfb_mix_zero, dim3(1), dim3(64), 0, h->st, p.mflags);
+    hipLaunchKernelGGL(k_hfb_mix_flags, dim3(grid), dim3(256), 0, h->st, p.total, p.kappa,
+                       p.orho, p.delta, p.mflags);
+    hipLaunchKernelGGL(k_hfb_mix, dim3(grid), dim3(256), 0, h->st, p.total, p.k.mix, first, p.rho,
+                       p.kappa, p.orho, p.okappa, p.mflags);
 }
 
 void hfb_launch_cutoffs(mocca_solve *h) {
@@ -670,8 +692,10 @@ void hfb_launch_cutoffs(mocca_solve *h) {
 int hfb_energy_impl(mocca_solve *h, double *row) {
     Hfb &p = h->hfb;
     return run(h, "hfb_energy", [&] {
-        hipLaunchKernelGGL(k_hfb_energy, dim3(1), dim3(kHfbThreads), 0, h->st, p.k, p.delta,
-                           p.kappa, p.fermi, row);
+        hipLaunchKernelGGL(k_hfb_energy_part, dim3(kHfbEnergyParts, p.k.nblk),
+                           dim3(kHfbThreads), 0, h->st, p.k, p.delta, p.kappa, p.epart);
+        hipLaunchKernelGGL(k_hfb_energy, dim3(1), dim3(64), 0, h->st, p.k, p.epart,
+                           kHfbEnergyParts, p.fermi, row);
     });
 }
 
@@ -793,7 +817,10 @@ int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
         (rc = dalloc(h, &p.can_psi, (size_t)M * 4 * N * h->tsize)) ||
         (rc = dalloc(h, &p.can_occ, M * sizeof(double))) ||
         (rc = dalloc(h, &p.work, 8)) || (rc = dalloc(h, &p.status, sizeof(int))) ||
-        (rc = dalloc(h, &p.tables, 3 * (size_t)M * sizeof(int))))
+        (rc = dalloc(h, &p.tables, 3 * (size_t)M * sizeof(int))) ||
+        (rc = dalloc(h, &p.mflags, 2 * sizeof(int))) ||
+        (rc = dalloc(h, &p.epart, (size_t)std::max(1, p.k.nblk) * kHfbEnergyParts *
+                                      sizeof(double))))
         return rc;
     std::vector<int> tab(3 * (size_t)M);
     for (int i = 0; i < M; ++i) {

@@ -640,19 +640,21 @@ __global__ void k_hl_shift_A(HfbK k, const double *__restrict__ Af,
     A[o] = v;
 }
 
+constexpr int kNumThreads = 1024;
+constexpr int kHfbLargeMaxH = 128;   // states per (isospin, parity) block
+
 // Particle number of each block and its derivative: ranks of the
 // eigenvalues (descending, ties by index; the h largest are the
 // quasiparticles), N_b = 2 sum_{k+} sum_a V[h+a][k]^2,
 // dN_b = 2 sum_{k+} sum_{l-} C_kl^2 / (w_k - w_l).
-__global__ __launch_bounds__(256) void k_hl_number(HfbK k, const double *__restrict__ V,
-                                                   const double *__restrict__ C,
-                                                   const double *__restrict__ w, int ld,
-                                                   const int *__restrict__ active,
-                                                   int *__restrict__ rank,
-                                                   double *__restrict__ num,
-                                                   const double *__restrict__ delta,
-                                                   double *__restrict__ off) {
-    __shared__ double sh[4];
+__global__ __launch_bounds__(kNumThreads) void k_hl_number(
+    HfbK k, const double *__restrict__ V, const double *__restrict__ C,
+    const double *__restrict__ w, int ld, const int *__restrict__ active,
+    int *__restrict__ rank, double *__restrict__ num, const double *__restrict__ delta,
+    double *__restrict__ off) {
+    __shared__ double sh[kNumThreads / 64];
+    __shared__ int rk[2 * kHfbLargeMaxH];
+    __shared__ double ws[2 * kHfbLargeMaxH];
     const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (!active[b]) return;
     const HfbBlockK &bk = k.blk[b];
@@ -660,48 +662,47 @@ __global__ __launch_bounds__(256) void k_hl_number(HfbK k, const double *__restr
     const double *wb = w + (long)b * ld;
     const double *Vb = V + (long)b * ld * ld, *Cb = C + (long)b * ld * ld;
     int *rb = rank + (long)b * ld;
+    for (int i = tid; i < n; i += nt) ws[i] = wb[i];
+    __syncthreads();
     for (int i = tid; i < n; i += nt) {
-        const double wi = wb[i];
+        const double wi = ws[i];
         int r = 0;
         for (int j = 0; j < n; ++j) {
-            const double wj = wb[j];
+            const double wj = ws[j];
             if (wj > wi || (wj == wi && j < i)) ++r;
         }
+        rk[i] = r;
         rb[i] = r;
     }
     __syncthreads();
-    double nn = 0, dn = 0;
+    // particle number, its derivative and the off-diagonal square sum of C
+    // in one pass over the n x n elements
+    double nn = 0, dn = 0, oc = 0;
     for (int e = tid; e < n * n; e += nt) {
         const int a = e / n, q = e % n;
-        if (rb[q] >= h) continue;
+        const double caq = Cb[(long)a * ld + q];
+        if (a != q) oc += caq * caq;
+        if (rk[q] >= h) continue;
         if (a < h) {
             const double v = Vb[(long)(h + a) * ld + q];
             nn += v * v;
         }
         const int l = a;   // C[q][l], q positive, l negative
-        if (rb[l] < h) continue;
+        if (rk[l] < h) continue;
         const double c = Cb[(long)q * ld + l];
-        dn += c * c / (wb[q] - wb[l]);
+        dn += c * c / (ws[q] - ws[l]);
     }
-    nn = hl_block_sum(nn, sh);
-    dn = hl_block_sum(dn, sh);
-    // off-diagonal square sums of the two possible start matrices of the
-    // next evaluation: T(lambda') in the original basis (the gaps, both
-    // triangles; independent of lambda) and C (times dlambda^2 for
-    // diag(w) - dlambda C in the current eigenbasis)
-    double od = 0, oc = 0;
+    // off-diagonal square sum of T(lambda') in the original basis (the gaps,
+    // both triangles; independent of lambda), against dlambda^2 times that
+    // of C for Afin - dlambda C in the current basis
+    double od = 0;
     const double *dl = delta + bk.off;
     for (int e = tid; e < h * h; e += nt) {
         const double d = dl[(e / h) * n + h + e % h];
         od += 2.0 * d * d;
     }
-    for (int e = tid; e < n * n; e += nt) {
-        const int a = e / n, q = e % n;
-        if (a != q) {
-            const double c = Cb[(long)a * ld + q];
-            oc += c * c;
-        }
-    }
+    nn = hl_block_sum(nn, sh);
+    dn = hl_block_sum(dn, sh);
     od = hl_block_sum(od, sh);
     oc = hl_block_sum(oc, sh);
     if (tid == 0) {
@@ -1182,8 +1183,8 @@ int hfb_large_solve_impl(mocca_solve *h) {
         rc = run(h, "hfb_fermi", [&] {
             hipLaunchKernelGGL((k_lj_gemm<true, true>), gmat, dim3(256), 0, h->st, p.lV, p.lV,
                                p.lC, ld, ms, p.lnb, p.lhb, p.lactive);
-            hipLaunchKernelGGL(k_hl_number, dim3(nblk), dim3(256), 0, h->st, p.k, p.lV, p.lC,
-                               p.lw, ld, p.lactive, p.lrank, p.lnum, p.delta, p.loff);
+            hipLaunchKernelGGL(k_hl_number, dim3(nblk), dim3(kNumThreads), 0, h->st, p.k, p.lV,
+                               p.lC, p.lw, ld, p.lactive, p.lrank, p.lnum, p.delta, p.loff);
             if (p.newton)
                 hipLaunchKernelGGL(k_hl_newton, dim3(1), dim3(64), 0, h->st, p.k, p.lnum,
                                    p.lnst, p.lactive, p.fermi, p.status);

@@ -140,6 +140,8 @@ struct Hfb {
     double *work = nullptr;
     int *status = nullptr;
     int *tables = nullptr;         // [3][M] state, slot_block, slot_index
+    int *mflags = nullptr;         // [2] whether old rho / kappa or delta are non-zero
+    double *epart = nullptr;       // [nblk][kHfbEnergyParts] pairing-energy partials
     // blocks above the in-LDS solver (mocca_hfb_large.hip): per block, n x n
     // matrices with leading dimension nmax (the largest 2h)
     bool large = false;

@@ -86,7 +86,7 @@ struct PrecJob {
     int pad;
 };
 
-enum HpsiMode { kWriteH = 1, kStep = 2 };
+enum HpsiMode { kWriteH = 1, kStep = 2, kKinetic = 4 };   // kKinetic: only <psi|Lap psi> partials
 
 // Field buffer offsets, in units of N (one contiguous buffer per handle):
 // U[2] B[2] NablaB[2][3] W[2][nu][mu] DivW[2][3] S[2][3] A[2][3] DivA[2]
@@ -322,7 +322,8 @@ void launch_alternate(const ConstraintK *con, int nc, const double *Y, const dou
 // (isospin, parity) with h states and their h partners.
 constexpr double kHfbNumCut = 1e-12;   // HFBNumCut (HFB.f90)
 constexpr int kHfbMaxN = 64;           // block size 2h of the in-LDS solvers
-constexpr int kHlNewton = 12;          // Fermi-solver record of the large-block path
+constexpr int kHlNewton = 12;
+constexpr int kHfbEnergyParts = 16;   // workgroups per block of the pairing-energy sum          // Fermi-solver record of the large-block path
 struct HfbBlockK {
     int it, P, h;
     int first;      // index of the block's first state in the state table

@@ -53,7 +53,10 @@ __device__ __forceinline__ void block_sum3_fd(double &a, double &b, double &c, d
 // STALE: the values come from vsrc instead of psi, whose stencils give the
 // derivatives (the first GradDesc after an AlternateStep, which changes the
 // values but not the stored derivatives, ImaginaryTime.f90:306-437).
-template <typename T, int HMAX, bool TRC, bool STALE, bool WA>
+// KIN: only the partial sums of <psi|Lap psi> (the kinetic sums of a basis
+// that no GradDesc produced, such as the canonical states under HFB); the
+// potentials and the first derivatives are not touched.
+template <typename T, int HMAX, bool TRC, bool STALE, bool WA, bool KIN = false>
 __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     T *__restrict__ upd, T *__restrict__ hout, const T *__restrict__ F,
@@ -189,6 +192,21 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
                 if constexpr (STALE) val[c] = vsrc[(long)s * 4 * N + c * N + p];
                 else val[c] = q[c][4];
             }
+            if constexpr (KIN) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const T *row = sp + (c * rows + tyl + 4) * W + tx + G;
+                    T w[9];
+#pragma unroll
+                    for (int k = -4; k <= 4; ++k) w[k + 4] = row[k];
+                    const T lx = lap9(m, w);
+#pragma unroll
+                    for (int k = -4; k <= 4; ++k) w[k + 4] = row[k * W];
+                    const T ly = lap9(m, w);
+                    const T lc = lx + ly + lap9(m, q[c]);
+                    kacc += (double)val[c] * (double)lc;
+                }
+            } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const T *row = sp + (c * rows + tyl + 4) * W + tx + G;
@@ -234,6 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_hpsi_fd(
                 eacc += (double)vc * (double)h[c];
                 dacc += (double)h[c] * (double)h[c];
             }
+            }   // !KIN
             if (mode & kWriteH) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) hout[c * N + p] = h[c];
@@ -633,11 +652,22 @@ void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout
                            ntile_y, kin0, kin1, dth, mu, mode, eparts, vsrc)
 #define MOCCA_HPSI_TRC(HM, ST)                                                                \
     if (trc) { MOCCA_HPSI_LAUNCH(HM, true, ST); } else { MOCCA_HPSI_LAUNCH(HM, false, ST); }
-    if (vsrc) {
+#define MOCCA_HPSI_KIN(HM, TR)                                                                \
+    hipLaunchKernelGGL((k_hpsi_fd<T, HM, TR, false, false, true>), dim3(ntasks), dim3(kBlock), \
+                       shm, st, m, psi, psi_out, upd, hout, fields, qn, tasks, TY, ntile_y,   \
+                       kin0, kin1, dth, mu, 0, eparts, vsrc)
+    if (mode & kKinetic) {
+        if (m.nx <= 32) {
+            if (trc) MOCCA_HPSI_KIN(4, true); else MOCCA_HPSI_KIN(4, false);
+        } else {
+            if (trc) MOCCA_HPSI_KIN(8, true); else MOCCA_HPSI_KIN(8, false);
+        }
+    } else if (vsrc) {
         if (m.nx <= 32) { MOCCA_HPSI_TRC(4, true); } else { MOCCA_HPSI_TRC(8, true); }
     } else {
         if (m.nx <= 32) { MOCCA_HPSI_TRC(4, false); } else { MOCCA_HPSI_TRC(8, false); }
     }
+#undef MOCCA_HPSI_KIN
 #undef MOCCA_HPSI_TRC
 #undef MOCCA_HPSI_LAUNCH
 }
