@@ -50,7 +50,7 @@ constexpr int kJacThreads = 1024;
 constexpr int kJacLdsMaxN = 200;     // packed upper triangle + rotation tables in 160 KiB
 constexpr int kJacMaxSweeps = 24;
 constexpr int kJacBatch = 3;         // 2x2 blocks a thread reads before it updates them
-constexpr int kApplyRows = 16;       // basis rows per workgroup of the rotation apply (4 waves)
+constexpr int kApplyRows = 4;        // basis rows per workgroup of the rotation apply (a wave each)
 constexpr int kGramWaves = 1024;     // waves of one pair-Gram launch
 
 typedef double hl_d4 __attribute__((ext_vector_type(4)));   // FP64 MFMA accumulator
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
 // row and the rounds follow one another without a barrier: the lanes of a
 // row are in one wave, whose LDS accesses complete in order.  The pairs of
 // each round come from the pair log.
-constexpr int kApplyLanes = 16;
+constexpr int kApplyLanes = 64;
 constexpr int kApplyMaxPairs = 128 / kApplyLanes;
 constexpr int kApplyChunk = 16;        // rounds of the log staged in LDS at a time
 size_t apply_lds(int n) {
