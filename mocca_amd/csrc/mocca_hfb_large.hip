@@ -289,6 +289,218 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same cyclic Jacobi for even n with the rows and columns kept in slot
+// order: the round-robin ordering of the circle method seats the players
+// (matrix indices) in slots, pair k of a round being slots (2k, 2k+1); after
+// each round every player but the one in slot 0 moves one seat along the
+// circle 2 -> 4 -> ... -> m-2 -> m-1 -> m-3 -> ... -> 1 -> 2.  The matrix is
+// stored by 2x2 blocks of slot pairs (K, L), K <= L (packed upper triangle
+// of blocks; a diagonal block holds all four elements), element (a, c) of
+// block (K, L) in plane 2a + c, so the lanes of a wave read consecutive
+// blocks of a plane.  A round reads every block into registers, then writes
+// the rotated elements to the seats their players move to: every element is
+// read and written once per round, as in place, with the moves for free and
+// no index tables.  slot_player[s] is the player in seat s; the log holds
+// each rotation with its players (i < j), the rotation being that of
+// Golub & Van Loan 8.4.2 for (i, j) as in k_lj_jacobi.
+__device__ __forceinline__ int seat_next(int u, int m) {
+    if (u == 0) return 0;
+    if (!(u & 1)) return u == m - 2 ? m - 1 : u + 2;
+    return u == 1 ? 2 : u - 2;
+}
+__device__ __forceinline__ int blk_index(int K, int L, int M2) {
+    return K * M2 - (K * (K - 1)) / 2 + (L - K);
+}
+size_t slots_jacobi_lds(int n) {
+    const int M2 = n / 2, nb = M2 * (M2 + 1) / 2;
+    return sizeof(double) * (4 * (size_t)nb + 2 * (size_t)M2 + 16) +
+           sizeof(short) * (size_t)((n + 3) / 4 * 4);
+}
+__global__ __launch_bounds__(kJacThreads) void k_lj_jacobi_slots(
+    const double *__restrict__ Ain, int ld, long mstride, const int *__restrict__ nb,
+    const int *__restrict__ active, double *__restrict__ w, double *__restrict__ Afin,
+    double2 *__restrict__ log, int *__restrict__ logij, long logstride,
+    int *__restrict__ rounds, int maxsweeps, int *__restrict__ status) {
+    extern __shared__ __align__(16) double lds_s[];
+    const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    if (active && !active[b]) return;
+    const int n = nb[b], m = n, M2 = n / 2, NB = M2 * (M2 + 1) / 2;
+    double *E = lds_s;                                              // [4][NB]
+    double2 *cs = reinterpret_cast<double2 *>(E + 4 * NB);          // [M2] slot rotations
+    double *sh = reinterpret_cast<double *>(cs + M2);               // [16]
+    short *sp = reinterpret_cast<short *>(sh + 16);                 // [n] slot -> player
+    // location of element (seat u, seat v): plane and block index, with
+    // *mir set when it sits in a diagonal block off its diagonal (the
+    // mirror position holds the same value)
+    auto loc = [&](int u, int v, int &mir) {
+        int K = u >> 1, a = u & 1, L = v >> 1, c = v & 1;
+        mir = 0;
+        if (K > L) {
+            const int t = K; K = L; L = t;
+            const int q = a; a = c; c = q;
+        } else if (K == L && a != c) {
+            mir = 1;
+        }
+        return (2 * a + c) * NB + blk_index(K, L, M2);
+    };
+    const double *Ab = Ain + b * mstride;
+    for (int q = tid; q < n * n; q += nt) {
+        const int i = q / n, j = q % n;
+        if ((i >> 1) <= (j >> 1)) {
+            int mir;
+            E[loc(i, j, mir)] = 0.5 * (Ab[(long)i * ld + j] + Ab[(long)j * ld + i]);
+        }
+    }
+    for (int i = tid; i < n; i += nt) sp[i] = (short)i;
+    __syncthreads();
+    double2 *lb = log + b * logstride;
+    int *li = logij + b * logstride;
+    constexpr int kMaxBlk = (kJacLdsMaxN / 2 * (kJacLdsMaxN / 2 + 1) / 2 + kJacThreads - 1) /
+                            kJacThreads;
+    // this thread's blocks t = tid + q nt: block (K, L) and, for each of its
+    // elements, where the seat move sends it (the same every round): 16-bit
+    // offsets, two per word, 0xffff for none (element (1, 0) of a diagonal
+    // block, which is its (0, 1)); bit 4q + e of mirb set when the target
+    // needs its mirror too
+    int kl[kMaxBlk], tg[kMaxBlk][2], mirb = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxBlk; ++q) {
+        const int t = tid + q * nt;
+        kl[q] = -1;
+        tg[q][0] = tg[q][1] = -1;
+        if (t >= NB) continue;
+        int K = (int)((2 * M2 + 1 - sqrtf((float)((2 * M2 + 1) * (2 * M2 + 1) - 8 * t))) / 2);
+        while (K > 0 && blk_index(K, K, M2) > t) --K;
+        while (K + 1 < M2 && blk_index(K + 1, K + 1, M2) <= t) ++K;
+        const int L = K + (t - blk_index(K, K, M2));
+        kl[q] = K | (L << 16);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int o = 0xffff, mir = 0;
+            if (!(K == L && e == 2))
+                o = loc(seat_next(2 * K + (e >> 1), m), seat_next(2 * L + (e & 1), m), mir);
+            const int sh16 = 16 * (e & 1);
+            tg[q][e >> 1] = (tg[q][e >> 1] & ~(0xffff << sh16)) | (o << sh16);
+            if (mir) mirb |= 1 << (4 * q + e);
+        }
+    }
+    int done = 0;
+    bool conv = false;
+    for (int sweep = 0; sweep <= maxsweeps; ++sweep) {
+        double off = 0, dia = 0;
+        for (int t = tid; t < NB; t += nt) {
+            // block (K, L) of index t
+            int K = (int)((2 * M2 + 1 - sqrtf((float)((2 * M2 + 1) * (2 * M2 + 1) - 8 * t))) / 2);
+            while (K > 0 && blk_index(K, K, M2) > t) --K;
+            while (K + 1 < M2 && blk_index(K + 1, K + 1, M2) <= t) ++K;
+            const int L = K + (t - blk_index(K, K, M2));
+            const double x0 = E[t], x1 = E[NB + t], x2 = E[2 * NB + t], x3 = E[3 * NB + t];
+            if (K == L) {
+                dia += x0 * x0 + x3 * x3;
+                off += 2.0 * x1 * x1;
+            } else {
+                off += 2.0 * (x0 * x0 + x1 * x1 + x2 * x2 + x3 * x3);
+            }
+        }
+        off = hl_block_sum(off, sh);
+        dia = hl_block_sum(dia, sh);
+        if (off <= 1e-32 * dia || off == 0.0) { conv = true; break; }
+        if (sweep == maxsweeps) break;
+        for (int r = 0; r < m - 1; ++r) {
+            // rotations of the pairs (players of seats 2k, 2k+1)
+            if (tid < M2) {
+                const int k = tid, dk = blk_index(k, k, M2);
+                const int pu = sp[2 * k], pv = sp[2 * k + 1];
+                const bool sw = pu > pv;          // player i of (i < j) in seat 2k+1
+                const double xuu = E[dk], xuv = E[NB + dk], xvv = E[3 * NB + dk];
+                const double aii = sw ? xvv : xuu, ajj = sw ? xuu : xvv, apq = xuv;
+                double c = 1.0, s = 0.0;
+                // (a rotation by an element below 1e-18 of the geometric mean
+                // of its diagonal pair is skipped, as in k_lj_jacobi)
+                if (apq != 0.0 && fabs(apq) > 1e-18 * sqrt(fabs(aii * ajj))) {
+                    const double tau = (ajj - aii) / (2.0 * apq);
+                    const double t = tau >= 0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                              : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    s = t * c;
+                }
+                cs[k] = make_double2(c, sw ? -s : s);
+                lb[(long)done * M2 + k] = make_double2(c, s);
+                li[(long)done * M2 + k] = sw ? (pv | (pu << 16)) : (pu | (pv << 16));
+            }
+            __syncthreads();
+            // every block into registers (and the players of the seats)
+            double x[kMaxBlk][4];
+#pragma unroll
+            for (int q = 0; q < kMaxBlk; ++q) {
+                if (kl[q] < 0) continue;
+                const int t = tid + q * nt;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[q][e] = E[e * NB + t];
+            }
+            short pl = 0;
+            if (tid < n) pl = sp[tid];
+            __syncthreads();
+            if (tid < n) sp[seat_next(tid, m)] = pl;
+#pragma unroll
+            for (int q = 0; q < kMaxBlk; ++q) {
+                if (kl[q] < 0) continue;
+                const int K = kl[q] & 0xffff, L = kl[q] >> 16;
+                const double2 a = cs[K], bq = cs[L];
+                double y[4];
+                if (K == L) {
+                    // J^T [[x00, x01], [x01, x11]] J, off-diagonal exactly 0
+                    const double c = a.x, sn = a.y;
+                    y[0] = c * c * x[q][0] - 2.0 * c * sn * x[q][1] + sn * sn * x[q][3];
+                    y[3] = sn * sn * x[q][0] + 2.0 * c * sn * x[q][1] + c * c * x[q][3];
+                    y[1] = 0.0;
+                    y[2] = 0.0;
+                } else {
+                    // Y = JK^T X JL
+                    const double a0 = a.x * x[q][0] - a.y * x[q][2];
+                    const double a1 = a.x * x[q][1] - a.y * x[q][3];
+                    const double b0 = a.y * x[q][0] + a.x * x[q][2];
+                    const double b1 = a.y * x[q][1] + a.x * x[q][3];
+                    y[0] = a0 * bq.x - a1 * bq.y;
+                    y[1] = a0 * bq.y + a1 * bq.x;
+                    y[2] = b0 * bq.x - b1 * bq.y;
+                    y[3] = b0 * bq.y + b1 * bq.x;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int o = (tg[q][e >> 1] >> (16 * (e & 1))) & 0xffff;
+                    if (o == 0xffff) continue;
+                    E[o] = y[e];
+                    // the mirror of an element (0, 1) of a diagonal block is
+                    // its (1, 0), one plane further, and back
+                    if (mirb & (1 << (4 * q + e))) E[o < 2 * NB ? o + NB : o - NB] = y[e];
+                }
+            }
+            __syncthreads();
+            ++done;
+        }
+    }
+    // diagonal and the final matrix in player order
+    double *wb = w + (long)b * ld;
+    for (int u = tid; u < n; u += nt) {
+        int mir;
+        wb[sp[u]] = E[loc(u, u, mir)];
+    }
+    if (Afin) {
+        double *Fb = Afin + b * mstride;
+        for (int q = tid; q < n * n; q += nt) {
+            const int u = q / n, v = q % n;
+            int mir;
+            Fb[(long)sp[u] * ld + sp[v]] = E[loc(u, v, mir)];
+        }
+    }
+    if (tid == 0) {
+        rounds[b] = done;
+        if (!conv) atomicOr(status, 8);
+    }
+}
+
 // The logged rotations applied to the basis: V[b] <- V[b] J_1 J_2 ... (in
 // place; V = identity first when ident or cold[b]).  kApplyLanes lanes own
 // one row of the basis in LDS and take disjoint rotations of each round
@@ -1091,7 +1303,16 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
     const int nblk = p.k.nblk, ld = p.nmax;
     const long ms = (long)ld * ld;
     int rc;
-    if (nmax <= kJacLdsMaxN && hfb_large_jacobi_lds(nmax) <= p.lds_limit) {
+    if (!odd && nmax <= kJacLdsMaxN && slots_jacobi_lds(nmax) <= p.lds_limit) {
+        const size_t shm = slots_jacobi_lds(nmax);
+        const size_t top = std::min(slots_jacobi_lds(kJacLdsMaxN), p.lds_limit);
+        rc = run(h, "hfb_jacobi", [&] {
+            require_lds((const void *)k_lj_jacobi_slots, top, "k_lj_jacobi_slots");
+            hipLaunchKernelGGL(k_lj_jacobi_slots, dim3(nblk), dim3(kJacThreads), shm, h->st, A,
+                               ld, ms, nb, active, w, Afin, p.llog, p.llogij, p.lmax, rounds,
+                               kJacMaxSweeps, p.status);
+        });
+    } else if (nmax <= kJacLdsMaxN && hfb_large_jacobi_lds(nmax) <= p.lds_limit) {
         const size_t shm = hfb_large_jacobi_lds(nmax);
         // (the limit for the largest in-LDS matrix, once)
         const size_t top = std::min(hfb_large_jacobi_lds(kJacLdsMaxN), p.lds_limit);
