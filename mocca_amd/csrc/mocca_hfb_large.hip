@@ -312,12 +312,21 @@ __device__ __forceinline__ int seat_next(int u, int m) {
 __device__ __forceinline__ int blk_index(int K, int L, int M2) {
     return K * M2 - (K * (K - 1)) / 2 + (L - K);
 }
+// block row K of packed block index t: the largest K with
+// blk_index(K, K) <= t, K = floor((2 M2 + 1 - sqrt((2 M2 + 1)^2 - 8 t)) / 2)
+// (the radicand is an integer below 2^24, its square root exact enough in
+// FP64 that the floor is right, exact where it is a perfect square)
+__device__ __forceinline__ int blk_row(int t, int M2) {
+    const int a = 2 * M2 + 1;
+    return (int)((a - sqrt((double)(a * a - 8 * t))) * 0.5);
+}
 size_t slots_jacobi_lds(int n) {
     const int M2 = n / 2, nb = M2 * (M2 + 1) / 2;
     return sizeof(double) * (4 * (size_t)nb + 2 * (size_t)M2 + 16) +
            sizeof(short) * (size_t)((n + 3) / 4 * 4);
 }
-__global__ __launch_bounds__(kJacThreads) void k_lj_jacobi_slots(
+constexpr int kSlotThreads = 1024;   // the rotated blocks of a round stay in registers
+__global__ __launch_bounds__(kSlotThreads) void k_lj_jacobi_slots(
     const double *__restrict__ Ain, int ld, long mstride, const int *__restrict__ nb,
     const int *__restrict__ active, double *__restrict__ w, double *__restrict__ Afin,
     double2 *__restrict__ log, int *__restrict__ logij, long logstride,
@@ -356,8 +365,8 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi_slots(
     __syncthreads();
     double2 *lb = log + b * logstride;
     int *li = logij + b * logstride;
-    constexpr int kMaxBlk = (kJacLdsMaxN / 2 * (kJacLdsMaxN / 2 + 1) / 2 + kJacThreads - 1) /
-                            kJacThreads;
+    constexpr int kMaxBlk = (kJacLdsMaxN / 2 * (kJacLdsMaxN / 2 + 1) / 2 + kSlotThreads - 1) /
+                            kSlotThreads;
     // this thread's blocks t = tid + q nt: block (K, L) and, for each of its
     // elements, where the seat move sends it (the same every round): 16-bit
     // offsets, two per word, 0xffff for none (element (1, 0) of a diagonal
@@ -370,10 +379,7 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi_slots(
         kl[q] = -1;
         tg[q][0] = tg[q][1] = -1;
         if (t >= NB) continue;
-        int K = (int)((2 * M2 + 1 - sqrtf((float)((2 * M2 + 1) * (2 * M2 + 1) - 8 * t))) / 2);
-        while (K > 0 && blk_index(K, K, M2) > t) --K;
-        while (K + 1 < M2 && blk_index(K + 1, K + 1, M2) <= t) ++K;
-        const int L = K + (t - blk_index(K, K, M2));
+        const int K = blk_row(t, M2), L = K + (t - blk_index(K, K, M2));
         kl[q] = K | (L << 16);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -391,10 +397,7 @@ __global__ __launch_bounds__(kJacThreads) void k_lj_jacobi_slots(
         double off = 0, dia = 0;
         for (int t = tid; t < NB; t += nt) {
             // block (K, L) of index t
-            int K = (int)((2 * M2 + 1 - sqrtf((float)((2 * M2 + 1) * (2 * M2 + 1) - 8 * t))) / 2);
-            while (K > 0 && blk_index(K, K, M2) > t) --K;
-            while (K + 1 < M2 && blk_index(K + 1, K + 1, M2) <= t) ++K;
-            const int L = K + (t - blk_index(K, K, M2));
+            const int K = blk_row(t, M2), L = K + (t - blk_index(K, K, M2));
             const double x0 = E[t], x1 = E[NB + t], x2 = E[2 * NB + t], x3 = E[3 * NB + t];
             if (K == L) {
                 dia += x0 * x0 + x3 * x3;
@@ -1308,7 +1311,7 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
         const size_t top = std::min(slots_jacobi_lds(kJacLdsMaxN), p.lds_limit);
         rc = run(h, "hfb_jacobi", [&] {
             require_lds((const void *)k_lj_jacobi_slots, top, "k_lj_jacobi_slots");
-            hipLaunchKernelGGL(k_lj_jacobi_slots, dim3(nblk), dim3(kJacThreads), shm, h->st, A,
+            hipLaunchKernelGGL(k_lj_jacobi_slots, dim3(nblk), dim3(kSlotThreads), shm, h->st, A,
                                ld, ms, nb, active, w, Afin, p.llog, p.llogij, p.lmax, rounds,
                                kJacMaxSweeps, p.status);
         });
