@@ -185,7 +185,7 @@ def headline(args):
         },
         dominant_stage={
             "stage": shares[0][0], "share": shares[0][1],
-            "kernel": "mocca::k_lj_jacobi<false, false>" if shares[0][0] == "hfb_jacobi"
+            "kernel": "mocca::k_lj_jacobi_slots" if shares[0][0] == "hfb_jacobi"
                       else None,
             "ms_per_step": prof[shares[0][0]]["ms_per_step"],
             "launches_per_step": prof[shares[0][0]]["launches_per_step"],
@@ -376,9 +376,12 @@ def batch_scan(steps, warmup):
         mocca_amd.batch_run(solves, max(warmup, 4), dts, mus, fetch=False)
         for s in solves:
             s.synchronize()
+        rep0 = sum(s.profile_get("graph_replay")[1] for s in solves)
         t0 = time.perf_counter()
         rows = mocca_amd.batch_run(solves, steps, dts, mus)
         t1 = time.perf_counter()
+        replayed = sum(s.profile_get("graph_replay")[1] for s in solves) - rep0
+        failed = sum(s.profile_get("graph_capture_failed")[1] for s in solves)
         last[name] = np.array([r[-1, 0] for r in rows])
         t2 = time.perf_counter()
         for s, w in zip(solves, ws):
@@ -389,7 +392,9 @@ def batch_scan(steps, warmup):
         res[name] = {"value": len(ws) * steps / (t1 - t0), "unit": "solve-iters/s",
                      "ms_per_batch_step": (t1 - t0) / steps * 1e3,
                      "sequential_value": len(ws) * steps / (t3 - t2),
-                     "speedup_vs_sequential": (t3 - t2) / (t1 - t0)}
+                     "speedup_vs_sequential": (t3 - t2) / (t1 - t0),
+                     "replayed_fraction": replayed / (len(ws) * steps),
+                     "captures_failed": failed}
     return {
         "workload": "64 solves of 12 n + 12 p, cubic meshes n ~ U{16..32} (dx=0.8 fm), "
                     "nwt ~ U{20..160}, HO seeds, SLy4 + Coulomb, HF, alternating Q20 "

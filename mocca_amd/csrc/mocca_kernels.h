@@ -266,6 +266,8 @@ int launch_coulomb_cg(const CoulombK &c, double *x, const double *rho_p, double 
                       double *pb, double *ap, double *part, int nwg, double *vcoul,
                       double *vcoulx, int *iters, hipStream_t st);
 int coulomb_cg_max_groups();
+// interior points of the Coulomb box up to which the CG runs as one workgroup
+constexpr long kCgSingleMax = 4096;
 // Direct solve: Q[a] (n_a x n_a, row-major, columns = eigenvectors) and
 // lam[a] of the 1-D operators; w0, w1 scratch of nx*ny*nz doubles.
 void launch_coulomb_direct(const CoulombK &c, double *x, const double *rho_p,

@@ -182,7 +182,12 @@ void launch_coulomb_bc(const double *Q, int nmom, const double *lfac, const int 
 // forms p = r + c p_old on the fly (double-buffered p), Ap and the partial
 // r.Ap; grid barrier; phase B updates V and r and the partial r.r; grid
 // barrier.  Every workgroup sums the partials in the same order, so all
-// take the same decisions and leave the loop together.
+// take the same decisions and leave the loop together.  Small boxes run the
+// same code as one ordinary workgroup (COOP = false, workgroup barriers):
+// a cooperative launch in a process costs the later launches of other
+// streams their concurrency on this runtime (measured: 64 graph-replayed
+// solves on their own streams ran 1.6x slower after one), which the
+// single-workgroup form avoids where it is fast enough.
 namespace {
 
 __device__ __forceinline__ int mirror(int a) { return a < 0 ? -1 - a : a; }
@@ -218,6 +223,7 @@ __device__ __forceinline__ double coul_lap(const CoulombK &c, int i, int j, int 
     return (dx * c.f + dy * c.f) + dz * c.f;
 }
 
+template <bool COOP>
 __global__ __launch_bounds__(kBlock) void k_coulomb_cg(CoulombK c, double *__restrict__ x,
                                                         const double *__restrict__ rho_p,
                                                         double *__restrict__ r,
@@ -229,7 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_coulomb_cg(CoulombK c, double *__res
                                                         double *__restrict__ vcoulx,
                                                         int *__restrict__ iters) {
     namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
+    auto gsync = [&] {
+        if constexpr (COOP) cg::this_grid().sync();
+        else __syncthreads();
+    };
     __shared__ double sh[kBlock / 64 + 1];
     const int nwg = gridDim.x;
     const long stride = (long)nwg * kBlock;
@@ -258,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_coulomb_cg(CoulombK c, double *__res
     // every other workgroup's read of it)
     double s = block_sum(loc, sh);
     if (threadIdx.x == 0) part[nwg + blockIdx.x] = s;
-    grid.sync();
+    gsync();
     double norm = grid_total(part + nwg, nwg, sh);
     double cc = 0;
     double *pcur = pa, *pnext = pb;
@@ -282,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_coulomb_cg(CoulombK c, double *__res
         }
         s = block_sum(loc, sh);
         if (threadIdx.x == 0) part[blockIdx.x] = s;
-        grid.sync();
+        gsync();
         if (!first) { double *t = pcur; pcur = pnext; pnext = t; }
         const double integral = grid_total(part, nwg, sh);
         const double a = norm / integral;
@@ -299,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_coulomb_cg(CoulombK c, double *__res
         }
         s = block_sum(loc, sh);
         if (threadIdx.x == 0) part[nwg + blockIdx.x] = s;
-        grid.sync();
+        gsync();
         const double nn = grid_total(part + nwg, nwg, sh);
         if (nn <= c.prec) break;
         cc = nn / norm;
@@ -321,9 +330,14 @@ int launch_coulomb_cg(const CoulombK &c, double *x, const double *rho_p, double 
                       double *pb, double *ap, double *part, int nwg, double *vcoul,
                       double *vcoulx, int *iters, hipStream_t st) {
     CoulombK cc = c;
+    if (nwg == 1) {
+        hipLaunchKernelGGL(k_coulomb_cg<false>, dim3(1), dim3(kBlock), 0, st, cc, x, rho_p, r, pa,
+                           pb, ap, part, vcoul, vcoulx, iters);
+        return (int)hipGetLastError();
+    }
     void *args[] = {&cc, &x, &rho_p, &r, &pa, &pb, &ap, &part, &vcoul, &vcoulx, &iters};
-    return (int)hipLaunchCooperativeKernel((const void *)k_coulomb_cg, dim3(nwg), dim3(kBlock),
-                                           args, 0, st);
+    return (int)hipLaunchCooperativeKernel((const void *)k_coulomb_cg<true>, dim3(nwg),
+                                           dim3(kBlock), args, 0, st);
 }
 
 int coulomb_cg_max_groups() {
@@ -331,8 +345,8 @@ int coulomb_cg_max_groups() {
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_coulomb_cg, kBlock,
-                                                     0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_coulomb_cg<true>,
+                                                     kBlock, 0) != hipSuccess)
         return 0;
     return per_cu * ncu;
 }

@@ -511,6 +511,7 @@ int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, 
         hipGraph_t graph = nullptr;
         if (hipStreamBeginCapture(h->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
             (void)hipGetLastError();
+            ++h->acc["graph_capture_failed"].second;
             ok = false;
             return 0;
         }
@@ -527,10 +528,12 @@ int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, 
             // the host state moved on as if the iteration had run: undo the
             // pointer rotation and run it eagerly
             h->b.psi = cur[0]; h->b.psi2 = cur[1]; h->b.upd = cur[2]; s.alt_old = cur[3];
+            ++h->acc["graph_capture_failed"].second;
             ok = false;
             return 0;
         }
         (void)hipGraphDestroy(graph);
+        ++h->acc["graph_capture"].second;
         ptr_state(h, n.post);
         cache.push_back(n);
         g = &cache.back();
@@ -539,6 +542,7 @@ int graph_step(mocca_solve *h, std::vector<IterGraph> &cache, int k, double dt, 
         s.alt_old = g->post[3];
     }
     HIPCHK(h, hipGraphLaunch(g->exec, h->st));
+    ++h->acc["graph_replay"].second;
     ++s.iteration;
     double *prev = s.erows + (size_t)(k - 1) * kRowLen, *row = s.erows + (size_t)k * kRowLen;
     HIPCHK(h, hipMemcpyAsync(prev, rows, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
@@ -729,13 +733,20 @@ int mocca_scf_setup(mocca_solve *h, const mocca_functional *f) {
     HIPCHK(h, hipMemset(s.cpb, 0, M3 * sizeof(double)));
     HIPCHK(h, hipMemset(s.cap, 0, M3 * sizeof(double)));
     {
+        // boxes up to kCgSingleMax interior points: one ordinary workgroup;
+        // larger ones: one cooperative launch over up to 256 workgroups
         const long nint = (long)nx * ny * nz;
-        const int cap = coulomb_cg_max_groups();
-        if (cap < 1) {
-            scf_free(h);
-            return fail(h, MOCCA_ERR_HIP, "cooperative launch not available for the Coulomb solver");
+        if (nint <= kCgSingleMax) {
+            s.cg_nwg = 1;
+        } else {
+            const int cap = coulomb_cg_max_groups();
+            if (cap < 1) {
+                scf_free(h);
+                return fail(h, MOCCA_ERR_HIP,
+                            "cooperative launch not available for the Coulomb solver");
+            }
+            s.cg_nwg = (int)std::min<long>(std::min(cap, 256), (nint + kBlock - 1) / kBlock);
         }
-        s.cg_nwg = (int)std::min<long>(std::min(cap, 256), (nint + kBlock - 1) / kBlock);
     }
     SCF_ALLOC(s.cpart, 2 * (size_t)s.cg_nwg * sizeof(double));
     // direct solve: eigen-decomposition of the 1-D operators
