@@ -226,6 +226,29 @@ int mocca_comp_energy(mocca_solve *h, double *e);
  * CalculateAllMoments, ConstructPotentials, CompEnergy. */
 int mocca_scf_start(mocca_solve *h);
 
+/* ConvergenceCheck (Main.f90:368-397) after the last CompEnergy of the
+ * handle (mocca_scf_start, each iteration of mocca_iterate and its variants,
+ * mocca_comp_energy, mocca_final_iteration).  prec[4] = MomentPrec,
+ * EnergyPrec, PairingPrec, CrankPrec (the reference's defaults: 1e-4,
+ * 1e-9, 1e-4, 1e-4).  out[6] = converged (all four tests), the
+ * "Positive total energy!" condition on which the reference stops (total
+ * energy >= 0), then the tests one by one: ConverMultipoleAll
+ * (Moments.f90:2628-2701: each constrained moment's last 8 values within
+ * prec[0] of its target, relative unless the target is 0), ConverEnergy
+ * (Energy.f90:1458-1481: the total energy within prec[1] relative of each of
+ * the 7 before it), ConverFermi (Pairing.f90:1048-1070: with HFB, |Fermi|
+ * <= prec[2] for both isospins, the reference's FermiHistory staying zero;
+ * always true for HF) and ConverCranking (Cranking.f90:602-619: true, no
+ * cranking constraints are offered). */
+int mocca_convergence_check(mocca_solve *h, const double *prec, int *out);
+
+/* FinalIteration (Main.f90:482-529): the derivatives of the density basis
+ * with the Lagrange scheme, UpdateDensities(0), CompEnergy into row[47]
+ * (the layout of mocca_comp_energy; may be NULL), then the densities again
+ * with the handle's own scheme (finite differences), unmixed.  The Coulomb
+ * potential is the last one solved, as in the reference. */
+int mocca_final_iteration(mocca_solve *h, double *row);
+
 /* niter iterations of the Evolve loop (Main.f90:255-328) on the device.
  * energies (may be NULL): [niter+1][47] in the layout of mocca_comp_energy,
  * row 0 for the states on entry, row k after iteration k.

@@ -30,6 +30,7 @@ EXPORTS = (
     "mocca_profile_enable", "mocca_profile_get",
     "mocca_scf_setup", "mocca_hf_fill", "mocca_get_occupations", "mocca_update_densities",
     "mocca_mix_densities", "mocca_construct_potentials", "mocca_comp_energy",
+    "mocca_convergence_check", "mocca_final_iteration",
     "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
     "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
     "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
@@ -293,6 +294,23 @@ class Solve:
 
     def scf_start(self):
         self._chk(self.lib.mocca_scf_start(self.h))
+
+    def convergence_check(self, moment_prec=1e-4, energy_prec=1e-9, pairing_prec=1e-4,
+                          crank_prec=1e-4):
+        """ConvergenceCheck after the last CompEnergy: dict with converged,
+        positive_energy (the reference's stop condition) and the four tests."""
+        prec = np.array([moment_prec, energy_prec, pairing_prec, crank_prec], np.float64)
+        out = (ctypes.c_int * 6)()
+        self._chk(self.lib.mocca_convergence_check(self.h, _p(prec), out))
+        keys = ("converged", "positive_energy", "moments", "energy", "fermi", "cranking")
+        return {k: bool(v) for k, v in zip(keys, out)}
+
+    def final_iteration(self):
+        """FinalIteration: the energy row with Lagrange derivatives; the
+        handle's densities are then the unmixed ones of its own scheme."""
+        e = np.zeros(ROW_LEN)
+        self._chk(self.lib.mocca_final_iteration(self.h, _p(e)))
+        return e
 
     def iterate(self, niter, dt_over_hbar, momentum, fetch=True):
         e = np.zeros((niter + 1, ROW_LEN)) if fetch else None

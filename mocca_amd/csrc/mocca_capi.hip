@@ -661,6 +661,31 @@ int down(mocca_solve *h, double *dst, const void *src, size_t n) {
                                  : download_convert(h, dst, (const float *)src, n);
 }
 
+int set_scheme(mocca_solve *h, int lagrange) {
+    lagrange = lagrange ? 1 : 0;
+    if (h->mesh.lagrange == lagrange) return 0;
+    if (!lagrange && (h->mesh.nx < 6 || h->mesh.ny < 6 || h->mesh.nz < 6))
+        return fail(h, MOCCA_ERR_ARG, "EV8 finite differences need at least 6 points per axis");
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->mesh.lagrange = lagrange;
+    Buffers &b = h->b;
+    fill_mesh<double>(h, h->md, b.lagD);
+    fill_mesh<float>(h, h->mf, (const float *)b.lagT);
+    if (h->prec == MOCCA_FP64) fill_mesh<double>(h, h->md, (const double *)b.lagT);
+    // the z-marching kernels are finite-difference only; a mesh created with
+    // Lagrange derivatives has no z-marching task lists
+    h->march = !lagrange && h->mesh.nx <= 64 && h->ntasks > 0;
+    h->dma_hpsi = h->march && h->prec == MOCCA_FP64 && h->ntasks_dma > 0 &&
+                  hpsi_dma_ok(h->md, h->TY, h->mesh.time_reversal);
+    std::vector<FieldJob> jobs;
+    build_density_jobs(h, jobs);
+    if ((int)jobs.size() != h->njobs)
+        return fail(h, MOCCA_ERR_STATE, "derived-density jobs changed in number");
+    HIPCHK(h, hipMemcpy(b.jobs, jobs.data(), jobs.size() * sizeof(FieldJob),
+                        hipMemcpyHostToDevice));
+    return 0;
+}
+
 void free_all(mocca_solve *h) {
     Buffers &b = h->b;
     void *ptrs[] = {b.psi, b.psi2, b.upd, b.hbuf, b.fields, b.dpart, b.lagT, b.lagD,

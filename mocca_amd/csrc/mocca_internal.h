@@ -76,9 +76,18 @@ struct Scf {
     double *pot = nullptr;         // FP64 potentials [48][N] (aliases fields in FP64)
     FieldJob *pjobs = nullptr;
     double *epart = nullptr;       // [e_nblk][kEnergySlots]
-    double *ecur = nullptr;        // [43] energy of the current state
-    double *erows = nullptr;       // [erow_cap][43]
+    double *ecur = nullptr;        // [kRowLen] energy of the current state
+    double *erows = nullptr;       // [erow_cap][kRowLen]
     int erow_cap = 0;
+    // ConvergenceCheck (Main.f90:368-397) state: on the host, TotalEnergy of
+    // the last CompEnergy and OldEnergy(1:7) before it (Energy.f90:93-98),
+    // and the Fermi energies of the last energy row; on the device, the last
+    // 8 values of sum(Value) of each constrained moment (Value, then
+    // OldValue(:,1:7), Moments.f90:1920-1932)
+    double ehist[8] = {};
+    double efermi[2] = {};
+    bool start_pending = false;    // the start-up row is not yet in ehist (HF)
+    double *chist = nullptr;       // [ncon][8]
     // MixingScheme 2 (potential preconditioning)
     int mixing_scheme = 0;
     double preconu = 1.0;
@@ -332,6 +341,14 @@ constexpr double kHbar = 6.58211928;   // Force.f90:103
 void graphs_clear(mocca_solve *h);
 // Releases the buffers of the self-consistent part.
 void scf_free(mocca_solve *h);
+// Switches the derivative scheme of the handle between the EV8 finite
+// differences and the Lagrange mesh (AssignFDCoefs, Derivatives.f90:206-346):
+// the mesh descriptors, the kernel paths and the derived-density jobs.
+int set_scheme(mocca_solve *h, int lagrange);
+// Densities of the density basis without mixing (UpdateDensities(0)).
+int density_any(mocca_solve *h);
+// dv <psi|Lap psi> of every state of the density basis.
+int kinetic_any(mocca_solve *h);
 
 }  // namespace detail
 }  // namespace mocca

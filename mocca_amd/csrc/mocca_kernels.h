@@ -312,6 +312,11 @@ void launch_con_squared(const ConstraintK *con, int nc, const double *Y, const d
 void launch_con_update(ConstraintK *con, int nc, ConstraintCtl *ctl, const double *Q,
                        const double *sq, const double *cstat, int mode, int iteration,
                        double momentum, double *ce_coef, double *alt, hipStream_t st);
+// OldValue bookkeeping of CalculateAllMoments for ConvergenceCheck: passes
+// times, hist[c][1:8] <- hist[c][0:7] and hist[c][0] = sum over isospins of
+// the constraint's moment.
+void launch_con_history(const ConstraintK *con, int nc, const double *Q, double *hist,
+                        int passes, hipStream_t st);
 void launch_con_field(const ConstraintK *con, int nc, const double *Y, const double *coef,
                       const ConstraintCtl *ctl, const double *cut, long N, double *ce,
                       hipStream_t st);

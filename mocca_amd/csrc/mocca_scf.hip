@@ -1088,6 +1088,28 @@ void launch_con_update(ConstraintK *con, int nc, ConstraintCtl *ctl, const doubl
                        iteration, momentum, ce_coef, alt);
 }
 
+// The value history of each constrained moment (CalculateAllMoments shifts
+// OldValue(:,1:7) and stores the new Value at every call, Moments.f90:
+// 1920-1932; ConverMultipole compares sum(Value) and sum(OldValue(:,1:7))
+// with the target, :2628-2682).  One thread per constraint.
+__global__ void k_con_history(const ConstraintK *__restrict__ con, int nc,
+                              const double *__restrict__ Q, double *__restrict__ hist,
+                              int passes) {
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        const double v = Q[2 * con[c].idx] + Q[2 * con[c].idx + 1];
+        double *hc = hist + 8 * c;
+        for (int p = 0; p < passes; ++p) {
+            for (int i = 7; i > 0; --i) hc[i] = hc[i - 1];
+            hc[0] = v;
+        }
+    }
+}
+
+void launch_con_history(const ConstraintK *con, int nc, const double *Q, double *hist,
+                        int passes, hipStream_t st) {
+    hipLaunchKernelGGL(k_con_history, dim3(1), dim3(64), 0, st, con, nc, Q, hist, passes);
+}
+
 void launch_con_field(const ConstraintK *con, int nc, const double *Y, const double *coef,
                       const ConstraintCtl *ctl, const double *cut, long N, double *ce,
                       hipStream_t st) {

@@ -192,6 +192,7 @@ int construct_impl(mocca_solve *h, int passes = 1) {
                                   s.ccoef, s.calt, h->st);
                 launch_con_field(s.con, s.ncon, s.Y, s.ccoef, s.cctl, s.cut, N, s.ce, h->st);
             }
+            launch_con_history(s.con, s.ncon, s.Q, s.chist, passes, h->st);
         });
         if (rc) return rc;
     }
@@ -339,6 +340,16 @@ int check_gs_status(mocca_solve *h) {
     return 0;
 }
 
+// CompEnergy's OldEnergy shift (Energy.f90:93-98) with the total energy of
+// a completed energy row, and the Fermi energies of that row.
+void push_energy(mocca_solve *h, const double *row) {
+    Scf &s = h->scf;
+    for (int i = 7; i > 0; --i) s.ehist[i] = s.ehist[i - 1];
+    s.ehist[0] = row[0];
+    s.efermi[0] = row[45];
+    s.efermi[1] = row[46];
+}
+
 // The Evolve loop (Main.f90:255-328) for niter iterations.  With estimate,
 // IterativeEstimation runs before each step (ParameterEstimation = 1,
 // ImaginaryTime.f90:464) and dt, momentum carry over from one iteration to
@@ -431,11 +442,15 @@ int iter_end(mocca_solve *h, int niter, double *energies) {
     }
     HIPCHK(h, hipMemcpyAsync(s.ecur, last, kRowLen * sizeof(double), hipMemcpyDeviceToDevice,
                              h->st));
-    if (energies) {
-        HIPCHK(h, hipStreamSynchronize(h->st));
-        HIPCHK(h, hipMemcpy(energies, s.erows, (size_t)(niter + 1) * kRowLen * sizeof(double),
-                            hipMemcpyDeviceToHost));
-    }
+    // (the rows also feed the energy history of the convergence check)
+    std::vector<double> rows((size_t)(niter + 1) * kRowLen);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(rows.data(), s.erows, rows.size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    if (energies) std::copy(rows.begin(), rows.end(), energies);
+    if (s.start_pending && niter > 0) push_energy(h, rows.data());
+    if (niter > 0) s.start_pending = false;
+    for (int k = 1; k <= niter; ++k) push_energy(h, rows.data() + (size_t)k * kRowLen);
     if ((rc = check_gs_status(h))) return rc;
     return h->hfb.on ? hfb_check_status(h) : 0;
 }
@@ -567,7 +582,7 @@ void scf_free(mocca_solve *h) {
                     s.ecur, s.erows, s.cq[0], s.cq[1], s.cq[2], s.clam[0], s.clam[1],
                     s.clam[2], s.cw0, s.cw1, s.pold, s.pcwork, s.pcjobs, s.pc_iters, s.con,
                     s.cctl, s.csq, s.ccoef, s.calt, s.cut, s.ce, s.mult, s.cstat, s.ccnt,
-                    s.csurf, s.alt_old};
+                    s.csurf, s.alt_old, s.chist};
     for (void *p : ptrs) free_ptr(p);
     if (s.pot && s.pot != (double *)h->b.fields) (void)hipFree(s.pot);
     s = Scf{};
@@ -867,11 +882,98 @@ int mocca_comp_energy(mocca_solve *h, double *e) {
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
     if (h->hfb.on && (rc = hfb_energy_impl(h, h->scf.ecur))) return rc;
     if ((rc = energy_kinetic_impl(h, h->scf.ecur))) return rc;
-    if (e) {
-        HIPCHK(h, hipStreamSynchronize(h->st));
-        HIPCHK(h, hipMemcpy(e, h->scf.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToHost));
-    }
+    double row[kRowLen];
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(row, h->scf.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToHost));
+    push_energy(h, row);
+    if (e) std::copy(row, row + kRowLen, e);
     return 0;
+}
+
+int mocca_convergence_check(mocca_solve *h, const double *prec, int *out) {
+    if (int rc = check_scf(h)) return rc;
+    if (!prec || !out) return fail(h, MOCCA_ERR_ARG, "null argument");
+    Scf &s = h->scf;
+    // ConverMultipoleAll / ConverMultipole (Moments.f90:2628-2701): each
+    // constrained moment's Value and OldValue(:,1:7), summed over isospins,
+    // against sum(Constraint), relative unless sum(TrueConstraint) is 0;
+    // moments without a constraint (or switched off) pass
+    bool mom = true;
+    if (s.ncon) {
+        std::vector<ConstraintK> ck(s.ncon);
+        std::vector<double> hist(8 * (size_t)s.ncon);
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(ck.data(), s.con, ck.size() * sizeof(ConstraintK),
+                            hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(hist.data(), s.chist, hist.size() * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        for (int c = 0; c < s.ncon && mom; ++c) {
+            const ConstraintK &k = ck[c];
+            if (k.type == 0) continue;
+            // Isoswitch 1 keeps one target (the sum over isospins), 2 one each
+            const double con = k.iso == 1 ? k.con[0] : k.con[0] + k.con[1];
+            const double tru = k.iso == 1 ? k.truec[0] : k.truec[0] + k.truec[1];
+            for (int i = 0; i < 8 && mom; ++i) {
+                const double q = hist[8 * (size_t)c + i];
+                const double d = tru != 0.0 ? std::fabs((con - q) / con) : std::fabs(con - q);
+                if (!(d < prec[0])) mom = false;
+            }
+        }
+    }
+    // ConverEnergy (Energy.f90:1458-1481)
+    bool en = true;
+    for (int i = 1; i <= 7 && en; ++i)
+        en = std::fabs((s.ehist[i] - s.ehist[0]) / s.ehist[0]) <= prec[1];
+    // ConverFermi (Pairing.f90:1048-1070): HF passes; with pairing each
+    // Fermi energy is compared with FermiHistory(:,1:7), which the reference
+    // declares zero and never updates
+    bool fe = true;
+    if (h->hfb.on)
+        for (int it = 0; it < 2; ++it)
+            if (std::fabs(s.efermi[it]) > prec[2]) fe = false;
+    // ConverCranking (Cranking.f90:602-619): no cranking constraints
+    const bool cr = true;
+    out[0] = mom && en && fe && cr;
+    out[1] = s.ehist[0] >= 0.0;   // Main.f90:389-396, no trap potential
+    out[2] = mom;
+    out[3] = en;
+    out[4] = fe;
+    out[5] = cr;
+    return 0;
+}
+
+int mocca_final_iteration(mocca_solve *h, double *row) {
+    if (int rc = check_scf(h)) return rc;
+    if (!h->scf.started) return fail(h, MOCCA_ERR_STATE, "mocca_scf_start not called");
+    if (!h->fields_set) return fail(h, MOCCA_ERR_STATE, "no potentials yet");
+    const bool fd = !h->mesh.lagrange;
+    // AssignFDCoefs(-1, -1, CoulombLapOrder): Lagrange first and second
+    // derivatives for the states and the density fields (the Coulomb solve
+    // is not repeated)
+    int rc = set_scheme(h, 1);
+    if (rc) return rc;
+    auto lagrange_pass = [&]() -> int {
+        int r;
+        if ((r = density_any(h))) return r;
+        if ((r = kinetic_any(h))) return r;
+        if ((r = energy_terms_impl(h, h->scf.ecur))) return r;
+        if (h->hfb.on && (r = hfb_energy_impl(h, h->scf.ecur))) return r;
+        return energy_kinetic_impl(h, h->scf.ecur);
+    };
+    rc = lagrange_pass();
+    if (fd) {
+        const int r2 = set_scheme(h, 0);
+        if (!rc) rc = r2;
+    }
+    if (rc) return rc;
+    double e[kRowLen];
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(e, h->scf.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToHost));
+    push_energy(h, e);
+    if (row) std::copy(e, e + kRowLen, row);
+    // the densities of the final states with the run's own derivatives,
+    // unmixed (UpdateDensities(0) after AssignFDCoefs(MaxFDOrder, ...))
+    return fd ? density_any(h) : 0;
 }
 
 int mocca_scf_start(mocca_solve *h) {
@@ -890,6 +992,16 @@ int mocca_scf_start(mocca_solve *h) {
     h->scf.alt_next = alternating(h);
     h->scf.started = true;
     h->scf.iteration = 0;
+    // the start-up row enters the energy history once it is complete: at
+    // once with HFB, after the first iteration's hPsi pass with HF (the
+    // kinetic part of an HF row comes from the next GradDesc)
+    h->scf.start_pending = !h->hfb.on;
+    if (h->hfb.on) {
+        double row[kRowLen];
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(row, h->scf.ecur, kRowLen * sizeof(double), hipMemcpyDeviceToHost));
+        push_energy(h, row);
+    }
     return 0;
 }
 
@@ -1156,9 +1268,9 @@ int mocca_set_constraints(mocca_solve *h, const mocca_constraint *c, int count, 
               [](const ConstraintK &a, const ConstraintK &b) { return a.idx < b.idx; });
     HIPCHK(h, hipStreamSynchronize(h->st));
     void *old[] = {s.con, s.cctl, s.csq, s.ccoef, s.calt, s.cut, s.ce, s.mult, s.cstat, s.ccnt,
-                   s.csurf};
+                   s.csurf, s.chist};
     for (void *p : old) free_ptr(p);
-    s.con = nullptr; s.cctl = nullptr; s.csq = s.ccoef = s.calt = nullptr;
+    s.con = nullptr; s.cctl = nullptr; s.csq = s.ccoef = s.calt = s.chist = nullptr;
     s.cut = s.ce = s.mult = s.cstat = s.csurf = nullptr; s.ccnt = nullptr;
     s.ncon = (int)ck.size();
     s.htype.clear();
@@ -1182,8 +1294,10 @@ int mocca_set_constraints(mocca_solve *h, const mocca_constraint *c, int count, 
         (rc = dalloc(h, &s.mult, 2 * N * sizeof(double))) ||
         (rc = dalloc(h, &s.cstat, 4 * sizeof(double))) ||
         (rc = dalloc(h, &s.ccnt, 2 * sizeof(int))) ||
-        (rc = dalloc(h, &s.csurf, 2 * 3 * (size_t)N * 3 * sizeof(double))))
+        (rc = dalloc(h, &s.csurf, 2 * 3 * (size_t)N * 3 * sizeof(double))) ||
+        (rc = dalloc(h, &s.chist, 8 * ck.size() * sizeof(double))))
         return rc;
+    HIPCHK(h, hipMemset(s.chist, 0, 8 * ck.size() * sizeof(double)));
     HIPCHK(h, hipMemcpy(s.con, ck.data(), ck.size() * sizeof(ConstraintK), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(s.cctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemset(s.ce, 0, 2 * N * sizeof(double)));

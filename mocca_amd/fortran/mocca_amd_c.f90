@@ -183,6 +183,17 @@ module mocca_amd_c
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function
+    integer(c_int) function mocca_convergence_check(h, prec, out) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: prec(*)
+      integer(c_int), intent(out) :: out(*)
+    end function
+    integer(c_int) function mocca_final_iteration(h, row) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: row(*)
+    end function
     integer(c_int) function mocca_iterate(h, niter, dt_over_hbar, momentum, energies) bind(C)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

@@ -27,6 +27,16 @@
 ! <prefix>_params.bin.  An optional energies.bin (real8 e(nwt)) gives the
 ! single-particle energies of the start states, fermi.bin (real8 (2)) the
 ! start Fermi energies, as a wavefunction file would provide them.
+! With 'conv' in the tasks the scf task also writes, after the start-up's and
+! every iteration's CompEnergy, the convergence tests of ConvergenceCheck
+! (Main.f90:368-397) at several precisions (<prefix>_conv.bin: int32
+! flags(13) = ConverEnergy at 1e-9, 1e-6, 1e-4, 1e-3, 1e-2, ConverMultipoleAll
+! at 1e-4, 1e-2, 1e-1, 0.5, ConverFermi at 1e-4, 1, 10, 100).  With 'final'
+! it runs the steps of FinalIteration (Main.f90:482-529) after the loop:
+! Lagrange derivatives of the density basis, UpdateDensities(0),
+! CompEnergy (<prefix>_final.bin: the energy row as in _scf.bin), then the
+! finite-difference derivatives and UpdateDensities(0) again (the density
+! rho(n,2) follows the row).
 !-------------------------------------------------------------------------------
 program ref_driver
   use CompilationInfo
@@ -44,7 +54,7 @@ program ref_driver
   use Coulomb
   use Energy
   use PotentialMixing
-  use Pairing, only : SolvePairing, Fermi, PairingEnergy, PairingType, Delta
+  use Pairing, only : SolvePairing, Fermi, PairingEnergy, PairingType, Delta, ConverFermi
   use PairingInteraction, only : PairingStrength, Alpha, DDexp, PairingCut, PairingMu,   &
   &                              RhoSat, CutType, DensityFactor, PCutoffs
   use HFB, only : HFBMix, HFBIter, RhoHFB, KappaHFB, blocksizes, QuasiEnergies
@@ -52,7 +62,7 @@ program ref_driver
   implicit none
 
   character(len=512) :: infile, prefix, tasks, arg4
-  integer            :: nwtf, i, m, it, uu, n, niter, up, upp, uh
+  integer            :: nwtf, i, m, it, uu, n, niter, up, upp, uh, uc
   logical            :: altcheck, havee
   real(KIND=dp), allocatable :: estart(:)
   integer, allocatable :: qn(:,:)
@@ -242,6 +252,11 @@ program ref_driver
     open(newunit=upp, file=trim(prefix)//'_pairing.bin', access='stream',       &
     &    form='unformatted', status='replace')
     write(upp) Fermi, PairingEnergy
+    if(index(tasks,'conv').gt.0) then
+      open(newunit=uc, file=trim(prefix)//'_conv.bin', access='stream',         &
+      &    form='unformatted', status='replace')
+      call dump_conv(uc)
+    endif
     altcheck = CheckForAlternateConstraints()
     do it=1,niter
       if(altcheck) call AlternateStep()
@@ -260,9 +275,11 @@ program ref_driver
       write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,      &
       &        sum(CoMCorrection), Skyrmeterms
       write(upp) Fermi, PairingEnergy
+      if(index(tasks,'conv').gt.0) call dump_conv(uc)
     enddo
     close(uu)
     close(upp)
+    if(index(tasks,'conv').gt.0) close(uc)
     if(ParameterEstimation.eq.1) close(up)
     open(newunit=uu, file=trim(prefix)//'_scfstates.bin', access='stream',      &
     &    form='unformatted', status='replace')
@@ -274,9 +291,48 @@ program ref_driver
     enddo
     close(uu)
     call dump_fields(trim(prefix)//'_scfpot.bin')
+    if(index(tasks,'final').gt.0) then
+      ! FinalIteration (Main.f90:482-529), without the printing
+      call AssignFDCoefs(-1, -1, CoulombLapOrder)
+      do i=1,nwt
+        call DensityBasis(i)%CompDer()
+      enddo
+      call UpdateDensities(0)
+      call CompEnergy
+      open(newunit=uu, file=trim(prefix)//'_final.bin', access='stream',        &
+      &    form='unformatted', status='replace')
+      write(uu) TotalEnergy, sum(Kinetic), CoulombEnergy, CoulombExchange,      &
+      &        sum(CoMCorrection), Skyrmeterms
+      call AssignFDCoefs(MaxFDOrder, MaxFDLapOrder, CoulombLapOrder)
+      do i=1,nwt
+        call DensityBasis(i)%CompDer()
+      enddo
+      call UpdateDensities(0)
+      write(uu) Density%Rho
+      close(uu)
+    endif
   endif
 
 contains
+
+  subroutine dump_conv(u)
+    ! the tests of ConvergenceCheck (Main.f90:368-397) at several precisions
+    integer, intent(in) :: u
+    real(KIND=dp), parameter :: ep(5) = (/1d-9, 1d-6, 1d-4, 1d-3, 1d-2/)
+    real(KIND=dp), parameter :: mp(4) = (/1d-4, 1d-2, 1d-1, 5d-1/)
+    real(KIND=dp), parameter :: pp(4) = (/1d-4, 1d0, 1d1, 1d2/)
+    integer :: fl(13), k
+    do k=1,5
+      fl(k) = merge(1, 0, ConverEnergy(ep(k)))
+    enddo
+    do k=1,4
+      fl(5+k) = merge(1, 0, ConverMultipoleAll(mp(k)))
+    enddo
+    do k=1,4
+      fl(9+k) = merge(1, 0, ConverFermi(pp(k)))
+    enddo
+    write(u) fl
+  end subroutine dump_conv
 
   subroutine dump_moments(fn)
     character(len=*), intent(in) :: fn

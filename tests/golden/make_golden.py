@@ -443,12 +443,18 @@ def run_scf(name):
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
     with open(os.path.join(wd, "input.nml")) as fin:
-        subprocess.run([DRIVER, "input.bin", "out", "force,scf", str(niter)], cwd=wd,
-                       stdin=fin, check=True, stdout=subprocess.DEVNULL)
+        subprocess.run([DRIVER, "input.bin", "out", "force,scf,conv,final", str(niter)],
+                       cwd=wd, stdin=fin, check=True, stdout=subprocess.DEVNULL)
     if pe:
         extra["params"] = np.fromfile(os.path.join(wd, "out_params.bin"),
                                       "<f8").reshape(niter, 2)
     traj = np.fromfile(os.path.join(wd, "out_scf.bin"), "<f8").reshape(niter + 1, 41)
+    # ConvergenceCheck's tests after every CompEnergy (start-up first) and
+    # FinalIteration's Lagrange energy row, then the finite-difference rho
+    extra["conv"] = np.fromfile(os.path.join(wd, "out_conv.bin"), "<i4").reshape(niter + 1, 13)
+    fraw = np.fromfile(os.path.join(wd, "out_final.bin"), "<f8")
+    extra["final_row"] = fraw[:41]
+    extra["final_iteration_rho"] = fraw[41:].reshape(2, n ** 3)
     st = np.fromfile(os.path.join(wd, "out_scfstates.bin"), "<f8")
     N = n ** 3
     fin_psi = st[:nwt * 4 * N].reshape(nwt, 4, n, n, n)

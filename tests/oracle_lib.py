@@ -418,7 +418,7 @@ class Constraints:
             self.items.append(dict(idx=idx, type=p.get("constrainttype", 2), iso=iso,
                                    iteration=p.get("iteration", -1), con=c.copy(),
                                    true=c.copy(), inten=np.array([p.get("intensity", 0.0)] * 2),
-                                   Y=ylm_field(mesh, l, m)))
+                                   Y=ylm_field(mesh, l, m), hist=np.zeros(8)))
         self.items.sort(key=lambda c: c["idx"])     # the moment list's order
         self.ce = np.zeros((2, mesh.N))
         self.first = True
@@ -428,6 +428,8 @@ class Constraints:
         self.cut = rutz_cutoff(self.mesh, rho, self.dv)
         for c in self.items:
             c["value"] = np.array(val[c["idx"]])
+            # Value and OldValue(:,1:7) summed over isospins (Moments.f90:1920-1932)
+            c["hist"] = np.concatenate([[c["value"].sum()], c["hist"][:7]])
             c["sq"] = np.array([np.sum(c["Y"] ** 2 * rho[it]) * self.dv for it in range(2)])
             if self.first and c["type"] != 0:
                 c["con"] = (np.array([c["value"].sum()] * 2) if c["iso"] == 1
@@ -508,6 +510,44 @@ def all_densities(mesh, psi, qn, occ, trc):
     return d, der, lap
 
 
+# The precisions at which the reference driver records ConvergenceCheck's
+# tests (oracle/ref_driver.f90 dump_conv): energy, moments, Fermi.
+CONV_EPREC = (1e-9, 1e-6, 1e-4, 1e-3, 1e-2)
+CONV_MPREC = (1e-4, 1e-2, 1e-1, 5e-1)
+CONV_PPREC = (1e-4, 1.0, 10.0, 100.0)
+
+
+def convergence_tests(ehist, fermi, pairing, cons, mprec, eprec, pprec):
+    """The tests of ConvergenceCheck (Main.f90:368-397): ConverMultipoleAll /
+    ConverMultipole (Moments.f90:2628-2701: the last 8 values of each
+    constrained moment, summed over isospins, against sum(Constraint),
+    relative unless sum(TrueConstraint) is 0, failing at >= prec),
+    ConverEnergy (Energy.f90:1458-1481: the total energy against the 7 before
+    it, relative, passing at <= prec) and ConverFermi (Pairing.f90:1048-1070:
+    HF passes; with pairing each Fermi energy against FermiHistory, which the
+    reference never updates from its initial zeros).  ehist = [E, OldEnergy(1:7)]."""
+    mom = True
+    for c in (cons.items if cons is not None else []):
+        if c["type"] == 0:
+            continue
+        con = c["con"][0] if c["iso"] == 1 else c["con"].sum()
+        tru = c["true"][0] if c["iso"] == 1 else c["true"].sum()
+        d = np.abs((con - c["hist"]) / con) if tru != 0.0 else np.abs(con - c["hist"])
+        if np.any(d >= mprec):
+            mom = False
+    en = all(abs((ehist[i] - ehist[0]) / ehist[0]) <= eprec for i in range(1, 8))
+    fe = (not pairing) or all(abs(f) <= pprec for f in fermi)
+    return mom, en, fe
+
+
+def conv_flags(ehist, fermi, pairing, cons):
+    """convergence_tests at the driver's precisions: 13 flags."""
+    out = [convergence_tests(ehist, fermi, pairing, cons, 1.0, e, 1.0)[1] for e in CONV_EPREC]
+    out += [convergence_tests(ehist, fermi, pairing, cons, mp, 1.0, 1.0)[0] for mp in CONV_MPREC]
+    out += [convergence_tests(ehist, fermi, pairing, cons, 1.0, 1.0, pp)[2] for pp in CONV_PPREC]
+    return [int(x) for x in out]
+
+
 def scf_run(g, niter, on_iteration=None):
     """The Evolve loop of the reference (Main.f90:172-225, 255-328) for HF with
     fixed dt/momentum and linear density mixing, restated on the oracle.
@@ -574,6 +614,16 @@ def scf_run(g, niter, on_iteration=None):
         pairs.append(np.concatenate([hfb.fermi, ep]))
         der, lap = mesh.derive_states(psi, qn)
     traj = [rows(E)]
+    ehist = np.zeros(8)
+    conv = []
+
+    def check(E):
+        nonlocal ehist
+        ehist = np.concatenate([[E[0]], ehist[:7]])    # CompEnergy's OldEnergy shift
+        fermi = hfb.fermi if hfb is not None else np.zeros(2)
+        conv.append(conv_flags(ehist, fermi, hfb is not None, cons))
+
+    check(E)
     if on_iteration is not None:
         on_iteration(0)
     altcheck = cons is not None and cons.alternating()
@@ -640,14 +690,36 @@ def scf_run(g, niter, on_iteration=None):
             pairs.append(np.concatenate([hfb.fermi, ep]))
             der, lap = mesh.derive_states(psi, qn)
         traj.append(rows(E))
+        check(E)
         if on_iteration is not None:
             on_iteration(k + 1)
     return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
+                conv=np.array(conv, np.int32), dbasis=dbasis, mesh=mesh,
                 vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params),
                 pairing=np.array(pairs), hfb=hfb,
                 constraints=[] if cons is None else
                 [dict(type=c["type"], con=c["con"].copy(), inten=c["inten"].copy(),
                       sq=c["sq"].copy()) for c in cons.items])
+
+
+def final_iteration(g, r):
+    """FinalIteration (Main.f90:482-529) after scf_run's loop: the density
+    basis differentiated with the Lagrange scheme, UpdateDensities(0) and
+    CompEnergy with the last Coulomb potential; the energy row in the layout
+    of scf_run's rows, and the unmixed densities of the run's own scheme."""
+    m = g["meta"]
+    coef, consts = g["coef"], g["consts"]
+    dv, trc, qn = consts[6], m["trc"], g["qn"]
+    mesh = r["mesh"]
+    lag = Mesh(mesh.nx, mesh.ny, mesh.nz, m["dx"], True)
+    d, der, lap = all_densities(lag, r["dbasis"], qn, r["occ"], trc)
+    E = energy(coef, consts, m["com1body"], trc, lag, d, r["vcoul"],
+               kinetic_sums(r["dbasis"], lap, qn, r["occ"], dv))
+    if r["hfb"] is not None:
+        E[0] += r["hfb"].energy().sum()
+    row = np.concatenate([[E[0], E[1] + E[2], E[3], E[4], E[5] + E[6]], E[7:]])
+    d_own, _, _ = all_densities(mesh, r["dbasis"], qn, r["occ"], trc)
+    return row, d_own
 
 
 # ---- HFB pairing (HFB.f90, Pairing.f90, PairingInteraction.f90) -----------

@@ -196,6 +196,48 @@ def test_sample_input_vs_reference():
     assert g["traj"][-1, 0] < -190.0             # bound 24Mg after 100 iterations
 
 
+@pytest.mark.parametrize("case", ["scf_trc_fd", "scf_trc_lag", "scfcon_trc_fd",
+                                  "scfcon_todd_lag", "scfhfb_trc_fd", "sample_mg24"])
+def test_convergence_tests_and_final_iteration_vs_reference(case):
+    """ConvergenceCheck's tests after the start-up and every iteration, at
+    the precisions the reference driver records, flag for flag; the energy
+    row of FinalIteration (Lagrange derivatives, unmixed densities) to 1e-12
+    of the row's scale and the finite-difference densities after it."""
+    g = golden(case)
+    r = ol.scf_run(g, g["meta"]["niter"])
+    assert np.array_equal(r["conv"], g["conv"])
+    row, d = ol.final_iteration(g, r)
+    assert np.max(np.abs(row - g["final_row"])) / np.max(np.abs(g["final_row"])) < 1e-12
+    assert relerr(d["rho"].reshape(2, -1), g["final_iteration_rho"]) < 1e-12
+
+
+def test_convergence_tests_pick_out_planted_mistakes():
+    """The oracle's tests fail on each of the three mistakes a restatement
+    could make: ConverEnergy's OldEnergy shift off by one row, and
+    ConverFermi reading the last Fermi energies instead of the reference's
+    zero history."""
+    g = golden("sample_mg24")
+    E = g["traj"][:, 0]
+    n = len(E) - 1
+    differs = False
+    for k in range(9, n + 1):
+        good = np.concatenate([[E[k]], E[k - 7:k][::-1]])       # E_k, then E_{k-1} .. E_{k-7}
+        late = np.concatenate([[E[k]], E[k - 8:k - 1][::-1]])   # the shift one row late
+        f1 = [ol.convergence_tests(good, [0, 0], False, None, 1, e, 1)[1]
+              for e in ol.CONV_EPREC]
+        f2 = [ol.convergence_tests(late, [0, 0], False, None, 1, e, 1)[1]
+              for e in ol.CONV_EPREC]
+        assert f1 == [bool(x) for x in g["conv"][k][:5]]
+        differs = differs or f1 != f2
+    assert differs
+    good = np.concatenate([[E[n]], E[n - 7:n][::-1]])
+    fermi = g["pairing"][-1, 0:2]
+    ok = [ol.convergence_tests(good, fermi, True, None, 1, 1, p)[2] for p in ol.CONV_PPREC]
+    assert ok == [bool(x) for x in g["conv"][-1][9:]]
+    # a history that remembered the last Fermi energies would always pass
+    assert not all(ok)
+
+
 # ---- checks fixed by the mathematics ---------------------------------------
 
 def test_hfb_generalized_density_is_a_projector():
