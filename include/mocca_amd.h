@@ -226,6 +226,21 @@ int mocca_comp_energy(mocca_solve *h, double *e);
  * CalculateAllMoments, ConstructPotentials, CompEnergy. */
 int mocca_scf_start(mocca_solve *h);
 
+/* UpdateAM (Cranking.f90:135-232) with DiagSpin and DiagAng
+ * (Spwf.f90:977-1489) of every state, derivatives formed on the fly from the
+ * current states.  states[M][21] (may be NULL) for the HF basis and
+ * canonical[M][21] (may be NULL; HFB only) for the canonical basis: J(3)
+ * JTR(3) JTI(3) J2(3) Sp(3) SpTR(3) SpTI(3) in units of hbar (hbar^2 for
+ * J2), with the components that signature and time-simplex symmetry
+ * remove from the EV8 octant at zero, as the reference sets them.
+ * totals[39] (may be NULL) over the density basis with its occupations,
+ * zero with time reversal as in the reference: TotalAngMom(3), JTR(3),
+ * JTI(3), J2Total(3), TotalAngMom_dens(3) (the integrals of the spin and
+ * current densities of the handle's last densities), AMIsoblock(2,2,2,3)
+ * in Fortran order (signature, parity, isospin, direction). */
+int mocca_angular_momentum(mocca_solve *h, double *states, double *canonical,
+                           double *totals);
+
 /* ConvergenceCheck (Main.f90:368-397) after the last CompEnergy of the
  * handle (mocca_scf_start, each iteration of mocca_iterate and its variants,
  * mocca_comp_energy, mocca_final_iteration).  prec[4] = MomentPrec,

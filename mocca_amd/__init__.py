@@ -30,7 +30,7 @@ EXPORTS = (
     "mocca_profile_enable", "mocca_profile_get",
     "mocca_scf_setup", "mocca_hf_fill", "mocca_get_occupations", "mocca_update_densities",
     "mocca_mix_densities", "mocca_construct_potentials", "mocca_comp_energy",
-    "mocca_convergence_check", "mocca_final_iteration",
+    "mocca_convergence_check", "mocca_final_iteration", "mocca_angular_momentum",
     "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
     "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
     "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
@@ -304,6 +304,23 @@ class Solve:
         self._chk(self.lib.mocca_convergence_check(self.h, _p(prec), out))
         keys = ("converged", "positive_energy", "moments", "energy", "fermi", "cranking")
         return {k: bool(v) for k, v in zip(keys, out)}
+
+    def angular_momentum(self, canonical=False):
+        """UpdateAM: per-state [M][7][3] (J, JTR, JTI, J2, Sp, SpTR, SpTI) of
+        the HF basis (and of the canonical basis when asked), and the totals
+        over the density basis: [5][3] (TotalAngMom, JTR, JTI, J2Total,
+        TotalAngMom_dens) and AMIsoblock [3][2][2][2] (direction, isospin,
+        parity, signature)."""
+        st = np.zeros((self.M, 21))
+        cn = np.zeros((self.M, 21)) if canonical else None
+        tot = np.zeros(39)
+        self._chk(self.lib.mocca_angular_momentum(self.h, _p(st), _p(cn) if canonical else None,
+                                                  _p(tot)))
+        out = dict(states=st.reshape(self.M, 7, 3), totals=tot[:15].reshape(5, 3),
+                   isoblock=tot[15:].reshape(3, 2, 2, 2))
+        if canonical:
+            out["canonical"] = cn.reshape(self.M, 7, 3)
+        return out
 
     def final_iteration(self):
         """FinalIteration: the energy row with Lagrange derivatives; the

@@ -1232,6 +1232,90 @@ int mocca_gram_schmidt(mocca_solve *h) {
     return 0;
 }
 
+int mocca_angular_momentum(mocca_solve *h, double *states, double *canonical,
+                           double *totals) {
+    if (int rc = check_ready(h)) return rc;
+    if (!states && !canonical && !totals) return fail(h, MOCCA_ERR_ARG, "null arguments");
+    if (canonical && !h->hfb.on)
+        return fail(h, MOCCA_ERR_STATE, "no canonical basis: HFB pairing is not active");
+    const int M = h->M;
+    const long N = h->N;
+    const int nblk = (int)((N + kBlock - 1) / kBlock);
+    const bool can = h->hfb.on;
+    const int nb = can ? 2 : 1;
+    double *part = nullptr, *out = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &part, std::max<size_t>((size_t)M * nblk * kAmSums, nblk) *
+                                   sizeof(double))) ||
+        (rc = dalloc(h, &out, (size_t)nb * M * 21 * sizeof(double)))) {
+        if (part) (void)hipFree(part);
+        return rc;
+    }
+    std::vector<double> res((size_t)nb * M * 21), dpart(nblk);
+    auto body = [&]() -> int {
+        HIPCHK(h, hipMemsetAsync(out, 0, res.size() * sizeof(double), h->st));
+        int r = run(h, "angmom", [&] {
+            for (int b = 0; b < nb; ++b) {
+                const void *src = b == 0 ? h->b.psi : h->hfb.can_psi;
+                if (h->prec == MOCCA_FP64)
+                    launch_angmom<double>(h->md, (const double *)src, h->b.qn, M, h->mesh.dx,
+                                          h->mesh.dv, part, out + (size_t)b * M * 21, h->st);
+                else
+                    launch_angmom<float>(h->mf, (const float *)src, h->b.qn, M, h->mesh.dx,
+                                         h->mesh.dv, part, out + (size_t)b * M * 21, h->st);
+            }
+        });
+        if (r) return r;
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(res.data(), out, res.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (totals && !h->mesh.time_reversal) {
+            r = run(h, "angmom", [&] {
+                launch_angmom_dens(h->b.dens, h->mesh.nx, h->mesh.ny, N, h->mesh.dx, part, h->st);
+            });
+            if (r) return r;
+            HIPCHK(h, hipStreamSynchronize(h->st));
+            HIPCHK(h, hipMemcpy(dpart.data(), part, nblk * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return 0;
+    };
+    rc = body();
+    (void)hipStreamSynchronize(h->st);
+    (void)hipFree(part);
+    (void)hipFree(out);
+    if (rc) return rc;
+    if (states) std::copy(res.begin(), res.begin() + (size_t)M * 21, states);
+    if (canonical) std::copy(res.begin() + (size_t)M * 21, res.end(), canonical);
+    if (!totals) return 0;
+    // UpdateAM's sums (Cranking.f90:135-232) over the density basis, only
+    // without time reversal: TotalAngMom, JTR, JTI, J2Total,
+    // TotalAngMom_dens, AMIsoblock(S, P, it, i)
+    std::fill(totals, totals + 39, 0.0);
+    if (h->mesh.time_reversal) return 0;
+    std::vector<double> occ(M);
+    std::vector<int> qn(4 * (size_t)M);
+    HIPCHK(h, hipMemcpy(occ.data(), can ? h->hfb.can_occ : h->b.occ, M * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(qn.data(), h->b.qn, qn.size() * sizeof(int), hipMemcpyDeviceToHost));
+    const double *db = res.data() + (can ? (size_t)M * 21 : 0);
+    for (int w = 0; w < M; ++w) {
+        const double o = occ[w];
+        const double *r = db + (size_t)w * 21;
+        const int it = (qn[4 * w] + 3) / 2 - 1, P = (qn[4 * w + 1] + 3) / 2 - 1,
+                  S = (qn[4 * w + 2] + 3) / 2 - 1;
+        for (int i = 0; i < 3; ++i) {
+            totals[i] += o * r[i];
+            totals[3 + i] += o * r[3 + i];
+            totals[6 + i] += o * r[6 + i];
+            totals[9 + i] += o * r[9 + i];
+            totals[15 + S + 2 * P + 4 * it + 8 * i] += o * r[i];
+        }
+    }
+    double dz = 0.0;
+    for (int b = 0; b < nblk; ++b) dz += dpart[b];
+    totals[14] = dz * h->mesh.dv;
+    return 0;
+}
+
 int mocca_compute_density(mocca_solve *h, mocca_density *out) {
     if (int rc = check_ready(h)) return rc;
     int rc = density_any(h);

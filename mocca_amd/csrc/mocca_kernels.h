@@ -140,6 +140,18 @@ void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_o
                         double kin0, double kin1, double dth, double mu, double *eparts,
                         hipStream_t st, bool w_antisym);
 
+// DiagSpin / DiagAng of M states (Spwf.f90:977-1489): out[M][21] = J(3)
+// JTR(3) JTI(3) J2(3) Sp(3) SpTR(3) SpTI(3) (the entries the EV8 symmetries
+// zero are left untouched: clear out first); part: [nblk][M][kAmSums] with
+// nblk = ceil(N / kBlock).
+constexpr int kAmSums = 9;
+template <typename T>
+void launch_angmom(const MeshK<T> &m, const T *psi, const int *qn, int M, double dx, double dv,
+                   double *part, double *out, hipStream_t st);
+// Per-workgroup sums (ceil(N / kBlock) of them) of the z components of the
+// spin and current density integrals of UpdateAM.
+void launch_angmom_dens(const double *dens, int nx, int ny, long N, double dx, double *part,
+                        hipStream_t st);
 template <typename T>
 void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
                        double *der, double *lap, hipStream_t st);

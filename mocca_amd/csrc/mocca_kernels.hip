@@ -157,6 +157,155 @@ void launch_derive_all(const MeshK<T> &m, const T *psi, const int *qn, int M,
 }
 
 // ---------------------------------------------------------------------------
+// DiagSpin and DiagAng (Spwf.f90:977-1489) of every state: <S>, <S T>,
+// <J>, <J T> and <J_i^2> from the values, first derivatives and coordinates
+// (x, y, z) = ((i + 1/2) dx, ...) of the EV8 octant.  Signature and time
+// simplex are conserved there, which zeroes J_x, J_y, (J T)_x imaginary,
+// (J T)_y real and the z parts of J T; the nonzero sums are, per state,
+// kAmSums partial sums per workgroup: J_z, Re (J T)_x, Im (J T)_y,
+// <J_x^2>, <J_y^2>, <J_z^2>, S_z, Re (S T)_x, Im (S T)_y.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_angmom(MeshK<T> m, const T *__restrict__ psi,
+                                                   const int *__restrict__ qn, double dx,
+                                                   double *__restrict__ part) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int s = blockIdx.y;
+    double acc[kAmSums];
+#pragma unroll
+    for (int k = 0; k < kAmSums; ++k) acc[k] = 0.0;
+    if (p < m.N) {
+        const int x = p % m.nx, y = (p / m.nx) % m.ny, z = p / ((long)m.nx * m.ny);
+        T vt[4], dt[3][4], lt[4];
+        state_derivs(m, psi + (long)s * 4 * m.N, qn[4 * s + 1], qn[4 * s + 2], qn[4 * s + 3], x,
+                     y, z, p, vt, dt, lt, false);
+        double v[4], dX[4], dY[4], dZ[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            v[c] = (double)vt[c];
+            dX[c] = (double)dt[0][c];
+            dY[c] = (double)dt[1][c];
+            dZ[c] = (double)dt[2][c];
+        }
+        const double X = (x + 0.5) * dx, Y = (y + 0.5) * dx, Z = (z + 0.5) * dx;
+        // spin parts
+        const double sz = 0.5 * (v[0] * v[0] + v[1] * v[1] - v[2] * v[2] - v[3] * v[3]);
+        const double stx = 0.5 * (-v[0] * v[0] + v[1] * v[1] + v[2] * v[2] - v[3] * v[3]);
+        const double sty = 0.5 * (-v[0] * v[0] + v[1] * v[1] - v[2] * v[2] + v[3] * v[3]);
+        // orbital parts: l = -i r x grad acting on the spinor, with the
+        // component pairing of the reference's expressions
+        const double lz = v[1] * Y * dX[0] - v[1] * X * dY[0] - v[0] * Y * dX[1] +
+                          v[0] * X * dY[1] + v[3] * Y * dX[2] - v[3] * X * dY[2] -
+                          v[2] * Y * dX[3] + v[2] * X * dY[3];
+        const double ltx = v[1] * Z * dY[2] - v[1] * Y * dZ[2] + v[0] * Z * dY[3] -
+                           v[0] * Y * dZ[3] - v[3] * Z * dY[0] + v[3] * Y * dZ[0] -
+                           v[2] * Z * dY[1] + v[2] * Y * dZ[1];
+        const double lty = v[0] * Z * dX[2] - v[0] * X * dZ[2] - v[1] * Z * dX[3] +
+                           v[1] * X * dZ[3] - v[2] * Z * dX[0] + v[2] * X * dZ[0] +
+                           v[3] * Z * dX[1] - v[3] * X * dZ[1];
+        // <J_i^2> = |J_i psi|^2, J_i psi component by component
+        const double a1 = -Z * dY[1] + Y * dZ[1] + 0.5 * v[2];
+        const double a2 = Z * dY[0] - Y * dZ[0] + 0.5 * v[3];
+        const double a3 = -Z * dY[3] + Y * dZ[3] + 0.5 * v[0];
+        const double a4 = Z * dY[2] - Y * dZ[2] + 0.5 * v[1];
+        const double b1 = -X * dZ[1] + Z * dX[1] + 0.5 * v[3];
+        const double b2 = X * dZ[0] - Z * dX[0] - 0.5 * v[2];
+        const double b3 = -X * dZ[3] + Z * dX[3] - 0.5 * v[1];
+        const double b4 = X * dZ[2] - Z * dX[2] + 0.5 * v[0];
+        const double c1 = -Y * dX[1] + X * dY[1] + 0.5 * v[0];
+        const double c2 = Y * dX[0] - X * dY[0] + 0.5 * v[1];
+        const double c3 = -Y * dX[3] + X * dY[3] - 0.5 * v[2];
+        const double c4 = Y * dX[2] - X * dY[2] - 0.5 * v[3];
+        acc[0] = sz + lz;
+        acc[1] = stx + ltx;
+        acc[2] = sty + lty;
+        acc[3] = a1 * a1 + a2 * a2 + a3 * a3 + a4 * a4;
+        acc[4] = b1 * b1 + b2 * b2 + b3 * b3 + b4 * b4;
+        acc[5] = c1 * c1 + c2 * c2 + c3 * c3 + c4 * c4;
+        acc[6] = sz;
+        acc[7] = stx;
+        acc[8] = sty;
+    }
+    __shared__ double sh[kBlock / 64][kAmSums];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int k = 0; k < kAmSums; ++k) {
+        double a = acc[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if (l == 0) sh[w][k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kAmSums) {
+        double t = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
+        part[((long)s * gridDim.x + blockIdx.x) * kAmSums + threadIdx.x] = t;
+    }
+}
+
+// The sums over the workgroups, in their order, times dv, into the layout
+// J(3) JTR(3) JTI(3) J2(3) Sp(3) SpTR(3) SpTI(3) of each state.
+__global__ void k_angmom_reduce(const double *__restrict__ part, int nblk, int M, double dv,
+                                double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M * kAmSums) return;
+    const int s = t / kAmSums, k = t % kAmSums;
+    double a = 0.0;
+    for (int b = 0; b < nblk; ++b) a += part[((long)s * nblk + b) * kAmSums + k];
+    // slot of each sum in the 21-value record
+    const int slot[kAmSums] = {2, 3, 7, 9, 10, 11, 14, 15, 19};
+    out[(long)s * 21 + slot[k]] = a * dv;
+}
+
+// Integrals of the spin and current densities (UpdateAM, Cranking.f90:
+// 186-212), z component: sum over isospins of 1/2 s_z - y j_x + x j_y, per
+// workgroup; dens = [2][9][N] with j at fields 3..5 and s at 6..8.
+__global__ __launch_bounds__(kBlock) void k_angmom_dens(const double *__restrict__ dens,
+                                                        int nx, int ny, long N, double dx,
+                                                        double *__restrict__ part) {
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    double a = 0.0;
+    if (p < N) {
+        const int x = p % nx, y = (p / nx) % ny;
+        const double X = (x + 0.5) * dx, Y = (y + 0.5) * dx;
+        for (int it = 0; it < 2; ++it) {
+            const double *d = dens + (long)it * 9 * N;
+            a += 0.5 * d[8 * N + p];
+            a += -Y * d[3 * N + p] + X * d[4 * N + p];
+        }
+    }
+    __shared__ double sh[kBlock / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) t += sh[q];
+        part[blockIdx.x] = t;
+    }
+}
+
+template <typename T>
+void launch_angmom(const MeshK<T> &m, const T *psi, const int *qn, int M, double dx, double dv,
+                   double *part, double *out, hipStream_t st) {
+    const int nblk = (int)((m.N + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_angmom<T>, dim3(nblk, M), dim3(kBlock), 0, st, m, psi, qn, dx, part);
+    const int nt = M * kAmSums;
+    hipLaunchKernelGGL(k_angmom_reduce, dim3((nt + 255) / 256), dim3(256), 0, st, part, nblk, M,
+                       dv, out);
+}
+template void launch_angmom<double>(const MeshK<double> &, const double *, const int *, int,
+                                    double, double, double *, double *, hipStream_t);
+template void launch_angmom<float>(const MeshK<float> &, const float *, const int *, int,
+                                   double, double, double *, double *, hipStream_t);
+
+void launch_angmom_dens(const double *dens, int nx, int ny, long N, double dx, double *part,
+                        hipStream_t st) {
+    const int nblk = (int)((N + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_angmom_dens, dim3(nblk), dim3(kBlock), 0, st, dens, nx, ny, N, dx, part);
+}
+
+// ---------------------------------------------------------------------------
 // Block-wide sum of two values (wave64 shuffles, then LDS over the waves).
 __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *sh) {
 #pragma unroll

@@ -189,6 +189,10 @@ module mocca_amd_c
       real(c_double), intent(in) :: prec(*)
       integer(c_int), intent(out) :: out(*)
     end function
+    integer(c_int) function mocca_angular_momentum(h, states, canonical, totals) bind(C)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, states, canonical, totals
+    end function
     integer(c_int) function mocca_final_iteration(h, row) bind(C)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

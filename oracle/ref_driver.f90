@@ -16,7 +16,10 @@
 !   real8 grid(nx,ny,nz,4,nwt)
 !   real8 U(n,2) B(n,2) NablaB(n,3,2) W(n,3,3,2) DivW(n,3,2)
 !   real8 S(n,3,2) A(n,3,2) DivA(n,2)
-! Tasks (comma separated): derive, hpsi, dens, step, force (synthetic mean
+! Tasks (comma separated): derive, hpsi, dens, am (UpdateAM after dens: the
+! angular-momentum and spin expectation values, <prefix>_am.bin = per state
+! J(3) JTR(3) JTI(3) J2(3) Sp(3) SpTR(3) SpTI(3), then TotalAngMom(3) JTR(3)
+! JTI(3) J2Total(3) TotalAngMom_dens(3) AMIsoblock(2,2,2,3)), step, force (synthetic mean
 ! fields from the input file), pot (densities -> moments -> Coulomb ->
 ! potentials -> energy, reference routines), scf (the Evolve loop of
 ! Main.f90:255-328 for HF, with AlternateStep while alternating constraints
@@ -59,6 +62,7 @@ program ref_driver
   &                              RhoSat, CutType, DensityFactor, PCutoffs
   use HFB, only : HFBMix, HFBIter, RhoHFB, KappaHFB, blocksizes, QuasiEnergies
   use DensityMixing, only : MixDensities
+  use Cranking, only : UpdateAM, TotalAngMom, TotalAngMom_dens
   implicit none
 
   character(len=512) :: infile, prefix, tasks, arg4
@@ -155,6 +159,21 @@ program ref_driver
       write(uu) Density%Vecj, Density%Vecs, Density%RotVecJ, Density%RotS,      &
       &        Density%DerS
     endif
+
+  if(index(tasks,'am').gt.0) then
+    ! UpdateAM (Cranking.f90:135-232) after the densities of 'dens': DiagSpin
+    ! and DiagAng of every state (Spwf.f90:977-1489), the sums over the
+    ! density basis and the integrals of the spin and current densities
+    call UpdateAM()
+    open(newunit=uu, file=trim(prefix)//'_am.bin', access='stream',            &
+    &    form='unformatted', status='replace')
+    do i=1,nwt
+      write(uu) HFBasis(i)%J, HFBasis(i)%JTR, HFBasis(i)%JTI, HFBasis(i)%J2,     &
+      &         HFBasis(i)%Sp, HFBasis(i)%SpTR, HFBasis(i)%SpTI
+    enddo
+    write(uu) TotalAngMom, JTR, JTI, J2Total, TotalAngMom_dens, AMIsoblock
+    close(uu)
+  endif
     close(uu)
   endif
 

@@ -214,7 +214,7 @@ def run_case(name):
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
     with open(os.path.join(wd, "input.nml")) as fin:
-        subprocess.run([DRIVER, "input.bin", "out", "force,derive,hpsi,dens,step"],
+        subprocess.run([DRIVER, "input.bin", "out", "force,derive,hpsi,dens,am,step"],
                        cwd=wd, stdin=fin, check=True, stdout=subprocess.DEVNULL)
     with open(os.path.join(wd, "input.nml")) as fin:
         subprocess.run([DRIVER, "input.bin", "out", "pot"],
@@ -244,6 +244,12 @@ def run_case(name):
                    rotvecj=take((2, 3, nz, ny, nx)), rots=take((2, 3, nz, ny, nx)),
                    ders=take((2, 3, 3, nz, ny, nx)))
     assert k == dens.size
+    # UpdateAM: per state J, JTR, JTI, J2, Sp, SpTR, SpTI (3 each), then
+    # TotalAngMom, JTR, JTI, J2Total, TotalAngMom_dens (3 each), AMIsoblock
+    am = rd("out_am.bin")
+    res["am_states"] = am[:nwt * 21].reshape(nwt, 7, 3)
+    res["am_totals"] = am[nwt * 21:nwt * 21 + 15].reshape(5, 3)
+    res["am_isoblock"] = am[nwt * 21 + 15:].reshape(3, 2, 2, 2)   # [i][it][P][S]
     st = rd("out_step.bin")
     per = nwt * 4 * n + 2 * nwt
     steps = []

@@ -498,6 +498,64 @@ class Constraints:
         return mult
 
 
+def angular_momentum(mesh, psi, qn, occ, dv, trc, dens=None):
+    """UpdateAM (Cranking.f90:135-232) with DiagSpin and DiagAng
+    (Spwf.f90:977-1489) of every state under the EV8 symmetries (signature
+    and time simplex conserved: J_x, J_y, Im (J T)_x, Re (J T)_y, the z parts
+    of J T and the matching spin parts stay zero).  Returns per state
+    [M][7][3] = J, JTR, JTI, J2, Sp, SpTR, SpTI, and with time reversal
+    broken the totals [5][3] = TotalAngMom, JTR, JTI, J2Total,
+    TotalAngMom_dens and AMIsoblock [3][2][2][2] (direction, isospin, parity,
+    signature) over the states with their occupations."""
+    M = psi.shape[0]
+    der, _ = mesh.derive_states(psi, qn)
+    nx, ny, nz, dx = mesh.nx, mesh.ny, mesh.nz, mesh.dx
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    X, Y, Z = ((a.reshape(-1) + 0.5) * dx for a in (x, y, z))
+    out = np.zeros((M, 7, 3))
+    for w in range(M):
+        p1, p2, p3, p4 = psi[w].reshape(4, -1)
+        (x1, x2, x3, x4), (y1, y2, y3, y4), (z1, z2, z3, z4) = der[w]
+        sz = 0.5 * (p1 * p1 + p2 * p2 - p3 * p3 - p4 * p4)
+        stx = 0.5 * (-p1 * p1 + p2 * p2 + p3 * p3 - p4 * p4)
+        sty = 0.5 * (-p1 * p1 + p2 * p2 - p3 * p3 + p4 * p4)
+        J3 = sz + p2 * Y * x1 - p2 * X * y1 - p1 * Y * x2 + p1 * X * y2 \
+            + p4 * Y * x3 - p4 * X * y3 - p3 * Y * x4 + p3 * X * y4
+        JTR1 = stx + p2 * Z * y3 - p2 * Y * z3 + p1 * Z * y4 - p1 * Y * z4 \
+            - p4 * Z * y1 + p4 * Y * z1 - p3 * Z * y2 + p3 * Y * z2
+        JTI2 = sty + p1 * Z * x3 - p1 * X * z3 - p2 * Z * x4 + p2 * X * z4 \
+            - p3 * Z * x1 + p3 * X * z1 + p4 * Z * x2 - p4 * X * z2
+        r = (-Z * y2 + Y * z2 + 0.5 * p3, Z * y1 - Y * z1 + 0.5 * p4,
+             -Z * y4 + Y * z4 + 0.5 * p1, Z * y3 - Y * z3 + 0.5 * p2)
+        J2x = sum(a * a for a in r)
+        r = (-X * z2 + Z * x2 + 0.5 * p4, X * z1 - Z * x1 - 0.5 * p3,
+             -X * z4 + Z * x4 - 0.5 * p2, X * z3 - Z * x3 + 0.5 * p1)
+        J2y = sum(a * a for a in r)
+        r = (-Y * x2 + X * y2 + 0.5 * p1, Y * x1 - X * y1 + 0.5 * p2,
+             -Y * x4 + X * y4 - 0.5 * p3, Y * x3 - X * y3 - 0.5 * p4)
+        J2z = sum(a * a for a in r)
+        out[w, 0, 2] = J3.sum() * dv
+        out[w, 1, 0] = JTR1.sum() * dv
+        out[w, 2, 1] = JTI2.sum() * dv
+        out[w, 3] = [J2x.sum() * dv, J2y.sum() * dv, J2z.sum() * dv]
+        out[w, 4, 2] = sz.sum() * dv
+        out[w, 5, 0] = stx.sum() * dv
+        out[w, 6, 1] = sty.sum() * dv
+    tot = np.zeros((5, 3))
+    iso = np.zeros((3, 2, 2, 2))
+    if not trc:
+        for w in range(M):
+            tot[0:4] += occ[w] * out[w, 0:4]
+            it, P_, S_ = ((qn[w][k] + 3) // 2 - 1 for k in (0, 1, 2))
+            iso[:, it, P_, S_] += occ[w] * out[w, 0]
+        if dens is not None:
+            jx, jy, szd = dens["vecj"][:, 0], dens["vecj"][:, 1], dens["vecs"][:, 2]
+            jx, jy, szd = (a.reshape(2, -1) for a in (jx, jy, szd))
+            tot[4, 2] = sum(0.5 * szd[it].sum() + (-Y * jx[it] + X * jy[it]).sum()
+                            for it in range(2)) * dv
+    return out, tot, iso
+
+
 def all_densities(mesh, psi, qn, occ, trc):
     """ComputeDensity + the derived fields, and each state's derivatives."""
     M, N = psi.shape[0], mesh.N

@@ -55,6 +55,27 @@ def test_derivatives_hpsi_density_vs_reference(case, prec, tol):
 
 
 @pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("prec,tol", [(mocca_amd.FP64, 1e-11), (mocca_amd.MIXED, 2e-5)])
+def test_angular_momentum_vs_reference(case, prec, tol):
+    """UpdateAM (DiagSpin, DiagAng of every state; totals over the occupied
+    states with the density integrals when time reversal is broken) against
+    the reference's values for the same states."""
+    g = golden(case)
+    m = g["meta"]
+    print(f"angular momentum: case {case} precision {prec}", flush=True)
+    s = _solve(g, prec)
+    s.compute_density()
+    am = s.angular_momentum()
+    s.close()
+    assert relerr(am["states"], g["am_states"]) < tol
+    if m["trc"]:
+        assert not np.any(am["totals"]) and not np.any(am["isoblock"])
+    else:
+        assert relerr(am["totals"], g["am_totals"]) < tol
+        assert relerr(am["isoblock"], g["am_isoblock"]) < tol
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
 def test_grad_desc_gram_schmidt_vs_reference(case):
     g = golden(case)
     m = g["meta"]

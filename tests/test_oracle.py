@@ -7,6 +7,7 @@ is at rounding level; 1e-12 relative (max-norm) is asserted throughout.
 import numpy as np
 import pytest
 
+import oracle_lib as ol
 from oracle_lib import (GOLDEN_CASES, Mesh, density, density_derived,
                         fields_of, golden, gram_schmidt, graddesc, hpsi,
                         kin_coefs, relerr)
@@ -93,6 +94,28 @@ def test_graddesc_gram_schmidt_vs_reference(case):
 
 
 # ---------------------------------------------------------------- closed forms
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_angular_momentum_vs_reference(case):
+    """UpdateAM with DiagSpin/DiagAng of every state (and the totals over the
+    occupied states when time reversal is broken) against the reference's
+    own values for the same states and densities."""
+    g = golden(case)
+    m = g["meta"]
+    mesh = Mesh(m["nx"], m["ny"], m["nz"], m["dx"], m["lagrange"])
+    dens = None if m["trc"] else dict(vecj=g["vecj"], vecs=g["vecs"])
+    st, tot, iso = ol.angular_momentum(mesh, g["psi"].reshape(m["nwt"], 4, -1), g["qn"],
+                                       g["occ"], g["consts"][6], m["trc"], dens)
+    assert relerr(st, g["am_states"]) < 1e-13
+    if m["trc"]:
+        assert not np.any(g["am_totals"]) and not np.any(tot)
+    else:
+        assert relerr(tot, g["am_totals"]) < 1e-13
+        assert relerr(iso, g["am_isoblock"]) < 1e-13
+        # the two routes to J_z agree: sum of the states and the integral of
+        # the spin and current densities
+        assert abs(tot[0, 2] - tot[4, 2]) < 1e-10 * abs(tot[0, 2])
+
+
 def test_lagrange_exact_on_band_limited_functions():
     """Lagrange-mesh derivatives are exact for the trigonometric functions of
     their basis (Baye & Heenen 1986): with the csc-type coefficients the basis
