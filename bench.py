@@ -57,8 +57,10 @@ STAGES = ("lag_derivs", "derive_all", "hpsi_step", "hpsi", "reduce_ed", "gram", 
 
 # The headline nucleus: open shells in both isospins (146 neutrons, 94
 # protons), so that the pairing stays on through the run (a doubly magic
-# filling such as 126 / 82 loses its pairing and the HFB step turns trivial)
-NUCLEUS = dict(neutrons=146, protons=94)
+# filling such as 126 / 82 loses its pairing and the HFB step turns trivial),
+# started from filled harmonic-oscillator shells (b = 2.4 fm), bound from
+# the first iteration on
+NUCLEUS = dict(neutrons=146, protons=94, seeds="eigen")
 
 # HFB pairing of the headline workload (the Pairing namelist of the README
 # sample: contact pairing 1250 MeV fm^3, alpha 1, GuessKappa)
@@ -81,6 +83,8 @@ def scf_solve(w, precision):
     import mocca_amd
     s = mocca_amd.Solve(w["nx"], w["ny"], w["nz"], w["dx"], w["dv"], w["M"],
                         time_reversal=w["trc"], lagrange=w["lagrange"], precision=precision)
+    if w.get("lagrange_split"):
+        s.set_lagrange_precision(1)
     s.set_states_meta(w["qn"], w["occ"])
     s.upload_states(w["psi"])
     s.scf_setup(**w["functional"])
@@ -90,13 +94,36 @@ def scf_solve(w, precision):
     return s
 
 
-def full_iteration_run(w, precision, steps, warmup, pairing, keep_start=False):
+def start_data(w):
+    """Single-particle energies and Fermi energies of the start states, as a
+    wavefunction file gives them to the reference: the energies <h> of one HF
+    iteration from the same seeds (FP64), and each Fermi energy halfway
+    between the last filled and the first empty level of the HF filling."""
+    from mocca_amd import FP64
+    s = scf_solve(w, FP64)
+    s.scf_start()
+    s.iterate(1, w["dt_over_hbar"], w["momentum"], fetch=False)
+    e, _ = s.energies()
+    s.close()
+    f = w["functional"]
+    fermi = []
+    for iso, npart in ((-1, f["neutrons"]), (1, f["protons"])):
+        es = np.sort(e[w["qn"][:, 0] == iso])
+        k = int(npart // 2)          # time reversal: two nucleons per state
+        fermi.append(0.5 * (es[k - 1] + es[k]))
+    return e, np.array(fermi)
+
+
+def full_iteration_run(w, precision, steps, warmup, pairing, keep_start=False, start=None):
     """Start-up, `warmup` iterations (the first one fetched, for the oracle's
-    check), then `steps` timed iterations with the stage events on."""
+    check), then `steps` timed iterations with the stage events on.  With
+    pairing, start = (energies, Fermi energies) of start_data."""
     s = scf_solve(w, precision)
-    start = s.download_states() if keep_start else None
+    state0 = s.download_states() if keep_start else None
     if pairing:
-        s.set_pairing(**PAIRING)
+        e0, f0 = start
+        s.set_energies(e0)
+        s.set_pairing(**dict(PAIRING, fermi=tuple(f0)))
     s.scf_start()
     first = s.iterate(1, w["dt_over_hbar"], w["momentum"])
     if warmup > 1:
@@ -109,7 +136,7 @@ def full_iteration_run(w, precision, steps, warmup, pairing, keep_start=False):
     prof = profile_table(s, steps)
     s.profile_enable(False)
     s.close()
-    return dict(t=(t1 - t0) / steps, prof=prof, rows=rows, first=first, start=start)
+    return dict(t=(t1 - t0) / steps, prof=prof, rows=rows, first=first, start=state0)
 
 
 def hpsi_roofline(M, N, trc, stage):
@@ -159,8 +186,10 @@ def headline(args):
     from mocca_amd import FP64, MIXED, workloads
     w = workloads.full_iteration(**NUCLEUS)
     M, N = w["M"], w["nx"] ** 3
-    r64 = full_iteration_run(w, FP64, args.steps, args.warmup, True, keep_start=True)
-    r32 = full_iteration_run(w, MIXED, args.steps, args.warmup, True)
+    start = start_data(w)
+    r64 = full_iteration_run(w, FP64, args.steps, args.warmup, True, keep_start=True,
+                             start=start)
+    r32 = full_iteration_run(w, MIXED, args.steps, args.warmup, True, start=start)
     prof = r64["prof"]
     tot = sum(v["ms_per_step"] for v in prof.values())
     shares = sorted(((k, v["ms_per_step"] / tot) for k, v in prof.items()),
@@ -170,7 +199,7 @@ def headline(args):
     jac = prof.get("hfb_jacobi", {})
     e64, e32 = r64["rows"][:, 0], r32["rows"][:, 0]
     out = dict(
-        w=w, r64=r64, r32=r32,
+        w=w, r64=r64, r32=r32, start=start,
         roofline={
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
@@ -230,6 +259,8 @@ def micro_run(w, precision, steps, warmup):
     import mocca_amd
     s = mocca_amd.Solve(w["nx"], w["ny"], w["nz"], w["dx"], w["dv"], w["M"],
                         time_reversal=w["trc"], lagrange=w["lagrange"], precision=precision)
+    if w.get("lagrange_split"):
+        s.set_lagrange_precision(1)
     s.set_states_meta(w["qn"], w["occ"])
     s.upload_states(w["psi"])
     F = w["fields"]
@@ -338,11 +369,13 @@ def lagrange_48(steps, warmup):
     w = workloads.lagrange_48()
     r64 = full_iteration_run(w, FP64, steps, warmup, False)
     r32 = full_iteration_run(w, MIXED, steps, warmup, False)
+    rsp = full_iteration_run(dict(w, lagrange_split=True), MIXED, steps, warmup, False)
     return {
         "workload": "48x48x48 Lagrange octant mesh (48-point derivative matrices on the FP64 "
-                    "matrix cores), dx=0.7 fm, 600 states (300 n + 300 p), HO seeds, SLy4 + "
-                    "Coulomb (direct solve on the 50^3 box), HF filling N=Z=12; step = the "
-                    "whole Evolve iteration on the device",
+                    "matrix cores; mixed_fp32_split: on the FP32 matrix cores with the "
+                    "matrices split into high and low FP32 parts), dx=0.7 fm, 600 states "
+                    "(300 n + 300 p), HO seeds, SLy4 + Coulomb (direct solve on the 50^3 box), "
+                    "HF filling N=Z=12; step = the whole Evolve iteration on the device",
         "steps": steps, "warmup": warmup,
         "fp64": {"value": 1.0 / r64["t"], "unit": "iters/s", "ms_per_step": r64["t"] * 1e3,
                  "kernel_ms_per_step": {k: v["ms_per_step"] for k, v in r64["prof"].items()}},
@@ -350,6 +383,13 @@ def lagrange_48(steps, warmup):
                   "speedup_vs_fp64": r64["t"] / r32["t"],
                   "kernel_ms_per_step": {k: v["ms_per_step"] for k, v in r32["prof"].items()}},
         "total_energy_err_keV": float(abs(r32["rows"][-1, 0] - r64["rows"][-1, 0]) * 1e3),
+        "mixed_fp32_split": {
+            "value": 1.0 / rsp["t"], "unit": "iters/s", "ms_per_step": rsp["t"] * 1e3,
+            "speedup_vs_fp64": r64["t"] / rsp["t"],
+            "lag_derivs_ms_per_step": rsp["prof"].get("lag_derivs", {}).get("ms_per_step"),
+            "total_energy_err_keV": float(abs(rsp["rows"][-1, 0] - r64["rows"][-1, 0]) * 1e3),
+            "max_total_energy_err_keV_along_trajectory":
+                float(np.max(np.abs(rsp["rows"][:, 0] - r64["rows"][:, 0])) * 1e3)},
     }
 
 
@@ -408,7 +448,7 @@ def batch_scan(steps, warmup):
     }
 
 
-def cpu_baseline(w, start, gpu_first, threads_all):
+def cpu_baseline(w, start, gpu_first, threads_all, start_ef=None):
     """The oracle's restatement (C restatement for derivatives, hPsi, step,
     Gram-Schmidt, densities, potentials, Coulomb, energy; numpy/LAPACK for the
     HFB matrices) of the same Evolve iteration with HFB pairing, from the same
@@ -431,6 +471,8 @@ def cpu_baseline(w, start, gpu_first, threads_all):
                 rhosat=PAIRING["rhosat"], hfb_mix=PAIRING["hfb_mix"],
                 hfb_iter=PAIRING["iterations"])
     g = dict(meta=meta, coef=np.asarray(f["B"]), consts=consts, qn=w["qn"], psi=start)
+    if start_ef is not None:
+        g["start_energies"], g["start_fermi"] = start_ef
     out = {}
     traj = None
     for nthr in (1, threads_all):
@@ -539,7 +581,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "full_iteration_hfb: 40x40x40 octant mesh (EV8), dx=0.8 fm, 400 states "
-                        "(200 n + 200 p), HO seeds, SLy4 + Coulomb (direct solve), N=146 Z=94, "
+                        "(200 n + 200 p), harmonic-oscillator eigenfunction seeds (b=2.4 fm) "
+                        "with the single-particle and Fermi energies of one HF iteration as "
+                        "start data, SLy4 + Coulomb (direct solve), "
+                        "N=146 Z=94, "
                         "HFB pairing 1250 MeV fm^3 (100-state blocks, 200 x 200 HFB matrices, "
                         "the reference's Broyden Fermi solve), MixingScheme 0; step = the whole "
                         "Evolve iteration on the device",
@@ -572,7 +617,8 @@ def main():
         print("# batch leg done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
-        out["cpu_baseline"] = cpu_baseline(w, r64["start"], r64["first"], threads)
+        out["cpu_baseline"] = cpu_baseline(w, r64["start"], r64["first"], threads,
+                                           start_ef=h["start"])
         print("# cpu baseline done", file=sys.stderr, flush=True)
     out["reference_container"] = reference_container()
     print(json.dumps(out), flush=True)

@@ -226,6 +226,15 @@ int mocca_comp_energy(mocca_solve *h, double *e);
  * CalculateAllMoments, ConstructPotentials, CompEnergy. */
 int mocca_scf_start(mocca_solve *h);
 
+/* Arithmetic of the Lagrange derivatives (Opt_LAG{X,Y,Z}_EV8,
+ * Lapla_LAG_EV8, OptimizedDerivatives.f90:115-148,306-338,432-464,551-615)
+ * on lines of 16-64 points, which run as small GEMMs on the matrix cores:
+ * mode 0 (default) the FP64 matrix cores (mixed precision: FP32 states
+ * widened, FP64 sums); mode 1, mixed precision only, the FP32 matrix cores
+ * with each FP64 Lagrange matrix split into two FP32 parts (high and low)
+ * and FP32 sums. */
+int mocca_set_lagrange_precision(mocca_solve *h, int mode);
+
 /* UpdateAM (Cranking.f90:135-232) with DiagSpin and DiagAng
  * (Spwf.f90:977-1489) of every state, derivatives formed on the fly from the
  * current states.  states[M][21] (may be NULL) for the HF basis and

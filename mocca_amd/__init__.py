@@ -31,6 +31,7 @@ EXPORTS = (
     "mocca_scf_setup", "mocca_hf_fill", "mocca_get_occupations", "mocca_update_densities",
     "mocca_mix_densities", "mocca_construct_potentials", "mocca_comp_energy",
     "mocca_convergence_check", "mocca_final_iteration", "mocca_angular_momentum",
+    "mocca_set_lagrange_precision",
     "mocca_scf_start", "mocca_iterate", "mocca_get_moments", "mocca_get_coulomb",
     "mocca_get_potentials", "mocca_estimate_parameters", "mocca_estimation_seed",
     "mocca_iterate_estimated", "mocca_set_constraints", "mocca_readjust_constraints",
@@ -304,6 +305,11 @@ class Solve:
         self._chk(self.lib.mocca_convergence_check(self.h, _p(prec), out))
         keys = ("converged", "positive_energy", "moments", "energy", "fermi", "cranking")
         return {k: bool(v) for k, v in zip(keys, out)}
+
+    def set_lagrange_precision(self, mode):
+        """Lagrange derivatives on the matrix cores: 0 FP64, 1 FP32 with split
+        matrices (mixed precision only)."""
+        self._chk(self.lib.mocca_set_lagrange_precision(self.h, int(mode)))
 
     def angular_momentum(self, canonical=False):
         """UpdateAM: per-state [M][7][3] (J, JTR, JTI, J2, Sp, SpTR, SpTI) of

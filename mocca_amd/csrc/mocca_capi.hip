@@ -276,6 +276,19 @@ int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_m
 }
 
 
+// The matrix-core Lagrange derivatives of every state into the scratch:
+// FP64 matrix cores, or for FP32 states on request the split FP32 ones.
+template <typename T>
+void lag_derivs(mocca_solve *h, const T *psi, T *dsc) {
+    if constexpr (sizeof(T) == 4) {
+        if (h->lag_split) {
+            launch_lag_derivs_split(h->md, psi, h->b.qn, h->M, dsc, h->st);
+            return;
+        }
+    }
+    launch_lag_derivs<T>(h->md, psi, h->b.qn, h->M, dsc, h->st);
+}
+
 template <typename T>
 int grad_desc_impl(mocca_solve *h, double dth, double mu) {
     const MeshK<T> &m = mesh_of<T>(h);
@@ -318,9 +331,7 @@ int grad_desc_impl(mocca_solve *h, double dth, double mu) {
         slots = h->ntiles;
         T *dsc = (T *)h->b.lagscr;
         if (dsc) {
-            rc = run(h, "lag_derivs", [&] {
-                launch_lag_derivs<T>(h->md, dsrc, h->b.qn, h->M, dsc, h->st);
-            });
+            rc = run(h, "lag_derivs", [&] { lag_derivs<T>(h, dsrc, dsc); });
             if (rc) return rc;
         }
         rc = run(h, "hpsi_step", [&] {
@@ -365,9 +376,7 @@ int kinetic_impl(mocca_solve *h) {
         slots = h->ntiles;
         T *dsc = (T *)h->b.lagscr;
         if (dsc) {
-            rc = run(h, "lag_derivs", [&] {
-                launch_lag_derivs<T>(h->md, psi, h->b.qn, h->M, dsc, h->st);
-            });
+            rc = run(h, "lag_derivs", [&] { lag_derivs<T>(h, psi, dsc); });
             if (rc) return rc;
         }
         rc = run(h, "kinetic", [&] {
@@ -1229,6 +1238,17 @@ int mocca_gram_schmidt(mocca_solve *h) {
                                             (float *)h->b.psi2, nullptr, 0);
     if (rc) return rc;
     std::swap(h->b.psi, h->b.psi2);
+    return 0;
+}
+
+int mocca_set_lagrange_precision(mocca_solve *h, int mode) {
+    if (!h) return fail(nullptr, MOCCA_ERR_ARG, "null solve handle");
+    if (mode != 0 && mode != 1) return fail(h, MOCCA_ERR_ARG, "unknown mode %d (0 or 1)", mode);
+    if (mode == 1 && h->prec != MOCCA_MIXED)
+        return fail(h, MOCCA_ERR_UNSUPPORTED,
+                    "FP32 matrix-core Lagrange derivatives need the mixed-precision states");
+    ++h->graph_epoch;
+    h->lag_split = mode == 1;
     return 0;
 }
 

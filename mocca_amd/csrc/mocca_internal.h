@@ -222,6 +222,9 @@ struct mocca_solve {
     int njobs = 0;
     // z-marching FD kernels: y-tile rows, tiles per plane, task count
     bool march = false;
+    // mixed precision: Lagrange derivatives on the FP32 matrix cores with
+    // split matrices (mocca_set_lagrange_precision) instead of the FP64 ones
+    bool lag_split = false;
     int TY = 0, ntile_y = 0, ntasks = 0;
     // the pipelined FP64 step kernel: usable, z slabs per state and tile, tasks
     bool dma_hpsi = false;

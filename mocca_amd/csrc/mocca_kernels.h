@@ -169,6 +169,10 @@ constexpr int kLagMaxN = 64;   // longest line of the matrix-core Lagrange deriv
 template <typename T>
 void launch_lag_derivs(const MeshK<double> &md, const T *psi, const int *qn, int M, T *dsc,
                        hipStream_t st);
+// The same for FP32 states on the FP32 matrix cores, the Lagrange matrices
+// split into a high and a low FP32 part (k_lag_axis_split).
+void launch_lag_derivs_split(const MeshK<double> &md, const float *psi, const int *qn, int M,
+                             float *dsc, hipStream_t st);
 
 // E = dv sum psi.h, D = dv sum h.h - E^2, K = dv sum psi.Lap psi per state
 // (any of E, D, K may be null).

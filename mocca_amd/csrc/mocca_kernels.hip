@@ -467,6 +467,166 @@ __global__ __launch_bounds__(256) void k_lag_axis(MeshK<double> md, const T *__r
     }
 }
 
+// The same derivatives for FP32 states on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32: operands as above, result lane l register r =
+// D[4 (l/16) + r][l%16]) with the FP64 Lagrange matrices split into two FP32
+// parts, L = L_hi + L_lo (L_hi = fl32(L), L_lo = fl32(L - L_hi)), each
+// product accumulated in its own FP32 accumulator and the two added at the
+// end: the matrices keep ~48 bits, the sums are FP32.
+typedef float lag_f4 __attribute__((ext_vector_type(4)));
+template <int NPAD, int AXIS>
+__global__ __launch_bounds__(256) void k_lag_axis_split(MeshK<double> md,
+                                                        const float *__restrict__ psi,
+                                                        const int *__restrict__ qn, int M,
+                                                        float *__restrict__ dsc) {
+    constexpr int NT = NPAD / 16, NKS = NPAD / 4, NEL = NPAD / 4;
+    const int s = blockIdx.x / 4, c = blockIdx.x % 4;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int nx = md.nx, ny = md.ny, nz = md.nz;
+    const long N = md.N;
+    const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+    extern __shared__ float lags_lds[];
+    float *Lt = lags_lds;                                  // [4][NPAD][NPAD]: L1 hi, lo, L2 hi, lo
+    float *Lam = lags_lds + 4 * NPAD * NPAD + wave * NPAD * 16;   // [NPAD][16]
+    const int P = qn[4 * s + 1], S = qn[4 * s + 2], TS = qn[4 * s + 3];
+    const int s1 = sign_first(AXIS, P, S, TS, c, 1), s2 = sign_lap(AXIS, P, S, TS, c);
+    const double *L1 = md.lag1[AXIS][s1 > 0 ? 1 : 0], *L2 = md.lag2[AXIS][s2 > 0 ? 1 : 0];
+    for (int q = threadIdx.x; q < 2 * NPAD * NPAD; q += blockDim.x) {
+        const int mt = q / (NPAD * NPAD), r = q % (NPAD * NPAD);
+        const int k = r / NPAD, i = r % NPAD;
+        const double v = (i < n && k < n) ? (mt ? L2 : L1)[i * n + k] : 0.0;
+        const float hi = (float)v;
+        Lt[(2 * mt) * NPAD * NPAD + r] = hi;
+        Lt[(2 * mt + 1) * NPAD * NPAD + r] = (float)(v - (double)hi);
+    }
+    __syncthreads();
+    const float *ps = psi + ((long)s * 4 + c) * N;
+    float *d1 = dsc + (((long)s * 4 + c) * 6 + AXIS) * N;
+    float *d2 = dsc + (((long)s * 4 + c) * 6 + 3 + AXIS) * N;
+    const int nxb = (nx + 15) / 16;
+    const int nblk = AXIS == 0 ? (ny * nz + 15) / 16 : nxb * (AXIS == 1 ? nz : ny);
+    const int kr = lane / 16, col = lane % 16;
+    auto geom = [&](int b, int &outer, int &x0, int &npen) {
+        outer = 0; x0 = 0;
+        if (AXIS == 0) {
+            npen = min(16, ny * nz - 16 * b);
+        } else {
+            outer = b / nxb;
+            x0 = (b % nxb) * 16;
+            npen = min(16, nx - x0);
+        }
+    };
+    auto gather = [&](int b, float (&pf)[NEL]) {
+        int outer, x0, npen;
+        geom(b, outer, x0, npen);
+#pragma unroll
+        for (int j = 0; j < NEL; ++j) {
+            const int e = lane + 64 * j;
+            int k, p;
+            long a;
+            if (AXIS == 0) { p = e / NPAD; k = e % NPAD; a = (long)(16 * b + p) * nx + k; }
+            else {
+                k = e / 16; p = e % 16;
+                a = AXIS == 1 ? ((long)outer * ny + k) * nx + x0 + p
+                              : ((long)k * ny + outer) * nx + x0 + p;
+            }
+            const bool ok = k < n && p < npen;
+            const float v = ps[ok ? a : 0];
+            pf[j] = ok ? v : 0.0f;
+        }
+    };
+    auto put = [&](const float (&pf)[NEL]) {
+#pragma unroll
+        for (int j = 0; j < NEL; ++j) {
+            const int e = lane + 64 * j;
+            const int k = AXIS == 0 ? e % NPAD : e / 16, p = AXIS == 0 ? e / NPAD : e % 16;
+            Lam[k * 16 + p] = pf[j];
+        }
+    };
+    float pf[NEL];
+    if (wave < nblk) {
+        gather(wave, pf);
+        put(pf);
+    }
+    for (int blk = wave; blk < nblk; blk += 4) {
+        const int nb = blk + 4;
+        if (nb < nblk) gather(nb, pf);
+        int outer, x0, npen;
+        geom(blk, outer, x0, npen);
+        // (the wave reads only its own slice: its LDS writes complete in order)
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        lag_f4 acc[2][2][NT];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[mt][h][t] = lag_f4{0, 0, 0, 0};
+#pragma unroll 4
+        for (int ks = 0; ks < NKS; ++ks) {
+            const int k = 4 * ks + kr;
+            const float lam = Lam[k * 16 + col];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float ell = Lt[((2 * mt + h) * NPAD + k) * NPAD + 16 * t + col];
+                        acc[mt][h][t] =
+                            AXIS == 0
+                                ? __builtin_amdgcn_mfma_f32_16x16x4f32(lam, ell, acc[mt][h][t], 0, 0, 0)
+                                : __builtin_amdgcn_mfma_f32_16x16x4f32(ell, lam, acc[mt][h][t], 0, 0, 0);
+                    }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i, p;
+                if (AXIS == 0) { p = 4 * kr + r; i = 16 * t + col; }
+                else { i = 16 * t + 4 * kr + r; p = col; }
+                if (i < n && p < npen) {
+                    long a;
+                    if (AXIS == 0) a = (long)(16 * blk + p) * nx + i;
+                    else if (AXIS == 1) a = ((long)outer * ny + i) * nx + x0 + p;
+                    else a = ((long)i * ny + outer) * nx + x0 + p;
+                    d1[a] = acc[0][0][t][r] + acc[0][1][t][r];
+                    d2[a] = acc[1][0][t][r] + acc[1][1][t][r];
+                }
+            }
+        if (nb < nblk) put(pf);
+    }
+}
+
+void launch_lag_derivs_split(const MeshK<double> &md, const float *psi, const int *qn, int M,
+                             float *dsc, hipStream_t st) {
+    const int ns[3] = {md.nx, md.ny, md.nz};
+#define MOCCA_LAG_AXIS(NP, AX)                                                                  \
+    {                                                                                           \
+        constexpr size_t shm = sizeof(float) * (4 * NP * NP + 4 * NP * 16);                    \
+        static const bool lds_set =                                                             \
+            require_lds((const void *)k_lag_axis_split<NP, AX>, shm, "k_lag_axis_split");     \
+        if (!lds_set)                                                                           \
+            require_lds((const void *)k_lag_axis_split<NP, AX>, shm, "k_lag_axis_split");     \
+        hipLaunchKernelGGL((k_lag_axis_split<NP, AX>), dim3(4 * M), dim3(256), shm, st, md,     \
+                           psi, qn, M, dsc);                                                    \
+    }
+#define MOCCA_LAG_NP(AX)                                                                        \
+    switch ((ns[AX] + 15) / 16) {                                                               \
+        case 1: MOCCA_LAG_AXIS(16, AX) break;                                                   \
+        case 2: MOCCA_LAG_AXIS(32, AX) break;                                                   \
+        case 3: MOCCA_LAG_AXIS(48, AX) break;                                                   \
+        case 4: MOCCA_LAG_AXIS(64, AX) break;                                                   \
+        default: break;                                                                         \
+    }
+    MOCCA_LAG_NP(0)
+    MOCCA_LAG_NP(1)
+    MOCCA_LAG_NP(2)
+#undef MOCCA_LAG_NP
+#undef MOCCA_LAG_AXIS
+}
+
 template <typename T>
 void launch_lag_derivs(const MeshK<double> &md, const T *psi, const int *qn, int M, T *dsc,
                        hipStream_t st) {

@@ -189,6 +189,11 @@ module mocca_amd_c
       real(c_double), intent(in) :: prec(*)
       integer(c_int), intent(out) :: out(*)
     end function
+    integer(c_int) function mocca_set_lagrange_precision(h, mode) bind(C)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+    end function
     integer(c_int) function mocca_angular_momentum(h, states, canonical, totals) bind(C)
       import :: c_int, c_ptr
       type(c_ptr), value :: h, states, canonical, totals

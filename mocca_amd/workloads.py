@@ -113,6 +113,36 @@ def ho_states(rng, nx, ny, nz, dx, qns, b=1.7, noise=1e-3, nmax=10, terms=3):
     return out
 
 
+def ho_eigen_states(rng, nx, ny, nz, dx, qns, b=2.4, noise=1e-4):
+    """Harmonic-oscillator eigenfunctions H_a(x/b) H_b(y/b) H_c(z/b)
+    exp(-r^2/2b^2), shell by shell (a + b + c ascending, then
+    lexicographically), one per state: the k-th state of an (isospin, parity)
+    takes the k-th function of that parity, in the one spinor component whose
+    reflection signs in x, y match the function's parities (component 0
+    even-even, 1 odd-odd, 2 odd-even, 3 even-odd), plus uniform noise: the
+    filled oscillator shells a mesh solver starts a nucleus from."""
+    X, Y, Z = mesh(nx, ny, nz, dx)
+    env = np.exp(-(X**2 + Y**2 + Z**2) / (2 * b * b))
+    funcs = {1: [], -1: []}
+    for N in range(24):
+        for a in range(N + 1):
+            for bb in range(N + 1 - a):
+                funcs[1 if N % 2 == 0 else -1].append((a, bb, N - a - bb))
+    comp = {(0, 0): 0, (1, 1): 1, (1, 0): 2, (0, 1): 3}
+    count = {}
+    out = np.zeros((len(qns), 4, nz * ny * nx))
+    for i, (iso, P, S, TS) in enumerate(qns):
+        k = count.get((iso, P), 0)
+        count[(iso, P)] = k + 1
+        a, bb, c = funcs[P][k]
+        f = hermite(a, X / b) * hermite(bb, Y / b) * hermite(c, Z / b) * env
+        f /= np.sqrt(2.0 ** (a + bb + c) * np.prod(np.arange(1, a + 1)) *
+                     np.prod(np.arange(1, bb + 1)) * np.prod(np.arange(1, c + 1)))
+        out[i, comp[(a % 2, bb % 2)]] = f.reshape(-1)
+        out[i] += noise * rng.uniform(-1, 1, size=out[i].shape)
+    return out
+
+
 def balanced_qn(M, time_reversal):
     """Quantum numbers spread evenly over isospin, parity (and signature
     when time reversal is broken); time simplex +1 (EV8)."""
@@ -142,7 +172,8 @@ def hpsi_microbench(seed=2, n=32, dx=0.8, M=200):
                 dt_over_hbar=0.010 / 6.58211928, momentum=0.5)
 
 
-def full_iteration(seed=3, n=40, dx=0.8, M=400, neutrons=126, protons=82, b=2.4):
+def full_iteration(seed=3, n=40, dx=0.8, M=400, neutrons=126, protons=82, b=2.4,
+                   seeds="random"):
     """The 40^3 / 400-state self-consistent workload: 200 neutron and 200
     proton states (time reversal, parity, signature and time simplex
     conserved; parity alternating), harmonic-oscillator seeds, SLy4 with
@@ -152,7 +183,10 @@ def full_iteration(seed=3, n=40, dx=0.8, M=400, neutrons=126, protons=82, b=2.4)
     rng = np.random.default_rng(seed)
     qn = np.array([(iso, 1 if j % 2 == 0 else -1, 1, 1)
                    for iso in (-1, 1) for j in range(M // 2)], np.int32)
-    psi = ho_states(rng, n, n, n, dx, [tuple(q) for q in qn], b=b)
+    if seeds == "eigen":
+        psi = ho_eigen_states(rng, n, n, n, dx, [tuple(q) for q in qn], b=b)
+    else:
+        psi = ho_states(rng, n, n, n, dx, [tuple(q) for q in qn], b=b)
     return dict(nx=n, ny=n, nz=n, dx=dx, dv=dx**3 * 8, M=M, trc=True, lagrange=False,
                 qn=qn, occ=np.zeros(M), psi=psi,
                 functional=functional.scf_arguments("SLy4", neutrons, protons, True),

@@ -101,6 +101,41 @@ def test_lagrange_48_and_64_point_lines_vs_oracle(shape, prec, tol_e, tol_psi):
     s.close()
 
 
+@pytest.mark.parametrize("shape", [(48, 64, 16), (40, 56, 20), (48, 48, 48)])
+def test_lagrange_fp32_split_matrix_cores_vs_oracle_and_fp64(shape):
+    """Mixed precision with the Lagrange derivatives on the FP32 matrix cores
+    (matrices split into high and low FP32 parts, FP32 sums), against the
+    oracle's FP64 derivatives: hPsi within 2e-5 of the largest value, the step
+    + Gram-Schmidt within 2e-5 (the tolerance of the mixed path).  Its error
+    is printed beside that of the FP64 matrix cores on the same FP32 states."""
+    nx, ny, nz = shape
+    print(f"lagrange lines, FP32 split matrix cores: mesh {nx}x{ny}x{nz}", flush=True)
+    c = _fields_case(nx, ny, nz, 0.7, 10, seed=23, time_odd=True, lagrange=True)
+    M, N = c["M"], nx * ny * nz
+    der, lap = _oracle_derivatives(c)
+    Ff = {k: np.ascontiguousarray(v).reshape(-1) for k, v in c["F"].items()}
+    ref_psi, _, ref_e, _ = graddesc(c["psi"], der, lap, np.zeros_like(c["psi"]), c["qn"],
+                                    False, c["kin"], Ff, c["dt"], c["mu"], c["dv"])
+    errs = {}
+    for mode in (0, 1):
+        s = _solve_of(c, mocca_amd.MIXED)
+        s.set_lagrange_precision(mode)
+        hp = []
+        for i in (0, M - 1):
+            it = (c["qn"][i][0] + 3) // 2 - 1
+            ref = or_hpsi(c["psi"][i], der[i], lap[i], it, False, c["kin"][it], Ff, N)
+            hp.append(relerr(s.hpsi(i), ref))
+        s.grad_desc(c["dt"], c["mu"])
+        psi = s.download_states()
+        e, _ = s.energies()
+        errs[mode] = (max(hp), relerr(e, ref_e), relerr(psi, ref_psi.reshape(M, 4, N)))
+        s.close()
+    print("  FP64 matrix cores: hpsi %.2e energies %.2e states %.2e" % errs[0], flush=True)
+    print("  FP32 split:        hpsi %.2e energies %.2e states %.2e" % errs[1], flush=True)
+    assert max(errs[1]) < 2e-5
+    assert max(errs[0]) < 2e-5
+
+
 def _woods_saxon_proton_case(n=48, dx=0.7, Z=12.0, A=24.0, a=0.6):
     """One proton state carrying a Woods-Saxon density of Z protons (and one
     neutron state so that both isospins exist): rho_p = occ psi^2 with psi in
