@@ -18,14 +18,15 @@ difference from FP64 after the same iterations is reported beside it.
 Further legs: the same workload with HF filling (`hf_no_pairing`), the
 32^3 / 200-state hPsi microbench (`hpsi_microbench`), the README sample
 input (`readme_sample`, 10^3 octant / 40 states, HFB, estimation, Q20), the
-48^3 / 600-state Lagrange workload (`lagrange_48`) and 64 constrained solves
-through mocca_batch_run (`batch_scan`).  `cpu_baseline` is the oracle's
+48^3 / 600-state Lagrange workload (`lagrange_48`), 64 constrained solves
+through mocca_batch_run (`batch_scan`) and a MOCCa v1 checkpoint and restart
+of the headline solve (`checkpoint_v1`).  `cpu_baseline` is the oracle's
 restatement of the same iteration on the host (1 thread and all threads);
 `reference_container` is the reference build timed in the build container.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
                     [--skip-hf] [--skip-micro] [--skip-sample]
-                    [--skip-lagrange] [--skip-batch]
+                    [--skip-lagrange] [--skip-batch] [--skip-checkpoint]
 """
 import argparse
 import json
@@ -234,6 +235,51 @@ def headline(args):
         hfb_jacobi_ms_per_launch=jac.get("ms_per_launch"),
     )
     return out
+
+
+def checkpoint(w, start):
+    """The headline solve (FP64, HFB) after the start-up and 2 iterations
+    written to a MOCCa v1 file (mocca_v1_save: states, densities, HFB
+    matrices of the 800-row blocks, energies) and a second solve restarted
+    from it (mocca_v1_load); wall-clock seconds of each call, the file size,
+    and whether the restarted states equal the saved ones bit for bit.  The
+    file goes to the host's temporary directory, so the rates include the
+    device-host copies and the page cache, not a disk."""
+    import shutil
+    import tempfile
+    from mocca_amd import FP64
+    e0, f0 = start
+    s = scf_solve(w, FP64)
+    s.set_energies(e0)
+    s.set_pairing(**dict(PAIRING, fermi=tuple(f0)))
+    s.scf_start()
+    s.iterate(2, w["dt_over_hbar"], w["momentum"], fetch=False)
+    s.synchronize()
+    d = tempfile.mkdtemp()
+    path = os.path.join(d, "checkpoint.v1")
+    try:
+        t0 = time.perf_counter()
+        s.save_v1(path)
+        t1 = time.perf_counter()
+        size = os.path.getsize(path)
+        psi = s.download_states()
+        s.close()
+        s2 = scf_solve(w, FP64)
+        s2.set_pairing(**dict(PAIRING, fermi=tuple(f0), guess_kappa=False))
+        s2.synchronize()
+        t2 = time.perf_counter()
+        s2.load_v1(path)
+        s2.synchronize()
+        t3 = time.perf_counter()
+        same = bool(np.array_equal(s2.download_states(), psi))
+        s2.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"file_bytes": size, "save_s": t1 - t0, "load_s": t3 - t2,
+            "save_GB_per_s": size / (t1 - t0) / 1e9, "load_GB_per_s": size / (t3 - t2) / 1e9,
+            "restart_states_bit_exact": same,
+            "note": "mocca_v1_save / mocca_v1_load of the 40^3 / 400-state HFB solve, host "
+                    "temporary directory (page cache), wall clock"}
 
 
 def hf_no_pairing(steps, warmup):
@@ -526,10 +572,12 @@ def main():
     ap.add_argument("--skip-lagrange", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-checkpoint", action="store_true")
     ap.add_argument("--legs", default="all",
                     help="comma list of the legs to run (headline, hf, micro, sample, "
-                         "lagrange, batch) or 'all'; without 'headline' only the chosen "
-                         "legs run and print their JSON (for profiling them alone)")
+                         "lagrange, batch, checkpoint) or 'all'; without 'headline' only "
+                         "the chosen legs run and print their JSON (for profiling them "
+                         "alone)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the "
                          "reference kernel for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -547,7 +595,7 @@ def main():
         print("bench.py measures one GPU only: the hot path runs on a single MI355X "
               "(sharding over several GPUs is a design note, not built)", file=sys.stderr)
         sys.exit(2)
-    legs = {"headline", "hf", "micro", "sample", "lagrange", "batch"}
+    legs = {"headline", "hf", "micro", "sample", "lagrange", "batch", "checkpoint"}
     if args.legs != "all":
         legs = set(args.legs.split(","))
     if "headline" not in legs:
@@ -562,6 +610,10 @@ def main():
             part["lagrange_48"] = lagrange_48(args.lagrange_steps, args.warmup)
         if "batch" in legs:
             part["batch_scan"] = batch_scan(args.batch_steps, args.warmup)
+        if "checkpoint" in legs:
+            from mocca_amd import workloads
+            w = workloads.full_iteration(**NUCLEUS)
+            part["checkpoint_v1"] = checkpoint(w, start_data(w))
         print(json.dumps(part), flush=True)
         return
     h = headline(args)
@@ -615,6 +667,9 @@ def main():
     if not args.skip_batch and "batch" in legs:
         out["batch_scan"] = batch_scan(args.batch_steps, args.warmup)
         print("# batch leg done", file=sys.stderr, flush=True)
+    if not args.skip_checkpoint and "checkpoint" in legs:
+        out["checkpoint_v1"] = checkpoint(w, h["start"])
+        print("# checkpoint leg done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
         out["cpu_baseline"] = cpu_baseline(w, r64["start"], r64["first"], threads,

@@ -427,6 +427,118 @@ int mocca_get_coulomb(mocca_solve *h, double *vcoul, double *vcoulx, double *cpo
 int mocca_get_potentials(mocca_solve *h, double *U, double *B, double *NablaB,
                          double *W, double *DivW, double *S, double *A, double *DivA);
 
+/* ---- MOCCa v1 wavefunction files ------------------------------------------
+ * The file WriteMOCCa_v1 writes and ReadMOCCa_v1 reads (InOut.f90:905-1416):
+ * Fortran sequential unformatted records, each framed by its byte count
+ * (int32, little endian) before and after; int32 integers and logicals
+ * (1 = true), FP64 reals, complex as (re, im) FP64 pairs.  Records in order:
+ * version (1); TotalEnergy, TotalEnergy - OldEnergy(1:7); nx, ny, nz, dx;
+ * TRC, PC, SC, TSC, IC; nwt, neutrons, protons (reals); per state the grid
+ * (nx,ny,nz,4) and Occupation, Energy, Dispersion, Norm, J(3), Isospin,
+ * TimeSimplex, Parity, Signature, TimeReversal (WriteSpwf, Spwf.f90:2246-
+ * 2262); the densities (WriteDensity, Densities.f90:1189-1240: Rho DerRho
+ * LapRho | Tau | NablaJ | JMuNu (empty) | 5 time-odd records, empty with
+ * time reversal); MaxFDOrder, MaxFDLapOrder, CoulombLapOrder; the force name;
+ * t0 .. wsoq; name, hbar, hbm(2), COM1Body, COM2Body; B1 .. B21 (28);
+ * PairingType, PairingStrength(2), alpha(2), PairingCut(2); PCutoffs (empty
+ * without pairing); Fermi(2), LNLambda(2); blocksizes; U, V; HFBColumns;
+ * RhoHFB, KappaHFB; CanTransfo; Omega(3), CrankValues(3); per constrained
+ * moment two records (WriteMoment, Moments.f90:3330-3363).  Records of 2 GiB
+ * or more are not supported. */
+#define MOCCA_V1_MAX_MOMENTS 32
+
+/* One constrained moment (WriteMoment / ReadMoment, Moments.f90:3330-3420).
+ * With Isoswitch 1 the file repeats Intensity, Constraint and
+ * TrueConstraint; the values here are the file's pairs. */
+typedef struct mocca_v1_moment {
+    int constraint_type, isoswitch, l, m, impart;
+    double intensity[2], constraint[2], true_constraint[2];
+    double value[2];          /* Value(2): neutrons, protons */
+    double old_value[14];     /* OldValue(2,7), Fortran order */
+} mocca_v1_moment;
+
+/* Every record of a v1 file except the per-state and array ones. */
+typedef struct mocca_v1_info {
+    double energy;            /* TotalEnergy */
+    double denergy[7];        /* TotalEnergy - OldEnergy(1:7) */
+    int nx, ny, nz;
+    double dx;
+    int trc, pc, sc, tsc, ic; /* symmetries conserved (logicals) */
+    int nwt;
+    double neutrons, protons;
+    int fd_order[3];          /* MaxFDOrder, MaxFDLapOrder, CoulombLapOrder */
+    char force[200];          /* afor, blank padded, no terminator */
+    double skyrme[14];        /* t0 x0 t1 x1 t2 x2 t3a x3a yt3a t3b x3b yt3b wso wsoq */
+    double hbar, hbm[2];
+    int com1body, com2body;
+    double B[28];             /* B1 .. B21 in the order of mocca_functional.B */
+    int pairing_type;
+    double pairing_strength[2], pairing_alpha[2], pairing_cut[2];
+    int has_cutoffs;          /* the PCutoffs record holds nwt values */
+    double fermi[2], lnlambda[2];
+    int hfb_size;             /* S = HFBSize: 2 nwt with time reversal, else nwt */
+    int npar, niso;           /* Pindex, Iindex: 2 with parity / isospin conserved */
+    int blocksizes[4];        /* blocksizes(npar, niso), Fortran order */
+    double omega[3], crank_values[3];
+    int nmoments;
+    mocca_v1_moment moments[MOCCA_V1_MAX_MOMENTS];
+} mocca_v1_info;
+
+/* The per-state and array records, host memory (N = nx ny nz, S =
+ * hfb_size).  On read any pointer may be NULL (that part is skipped); on
+ * write grid, state and qn are required, NULL density members write zeros,
+ * NULL cutoffs write an empty record, NULL HFB matrices write zeros. */
+typedef struct mocca_v1_arrays {
+    double *grid;             /* [nwt][4][N] HFBasis(i)%Value%Grid */
+    double *state;            /* [nwt][7] Occupation, Energy, Dispersion, Norm, J(3) */
+    int *qn;                  /* [nwt][5] Isospin, TimeSimplex, Parity, Signature,
+                                 TimeReversal */
+    double *cutoffs;          /* [nwt] PCutoffs */
+    mocca_density dens;       /* rho, derrho, laprho, tau, nablaJ; without time
+                                 reversal also vecj, rotvecj, vecs, rots, ders */
+    double *u, *v;            /* complex U(S, 2S, 2, 2), V(S, 2S, 2, 2) */
+    int *hfb_columns;         /* HFBColumns(S, npar, niso) */
+    double *rho_hfb, *kappa_hfb;   /* complex (S, S, 2, 2) */
+    double *can_transfo;      /* complex (S, S, npar, niso) */
+} mocca_v1_arrays;
+
+/* Read a v1 file (ReadMOCCa_v1, InOut.f90:1040-1416): every header record
+ * into info, and the arrays whose pointers are set.  Host only (no device
+ * use).  Returns MOCCA_ERR_ARG for a file that is not a v1 file or does not
+ * hold the records its header announces. */
+int mocca_v1_read(const char *path, mocca_v1_info *info, const mocca_v1_arrays *arrays);
+
+/* Write a v1 file from info and host arrays (WriteMOCCa_v1,
+ * InOut.f90:905-1038).  Host only.  A file read by mocca_v1_read and
+ * written back is identical byte for byte. */
+int mocca_v1_write(const char *path, const mocca_v1_info *info,
+                   const mocca_v1_arrays *arrays);
+
+/* Checkpoint of a solve as Evolve writes one (Main.f90:318-325 calling
+ * WriteMOCCa_v1): the states with their occupations, energies, dispersions,
+ * Gram-Schmidt norms and angular momenta (UpdateAM), the current (mixed)
+ * densities, the energy history, the functional, the pairing (HFB: the
+ * cutoff factors, Fermi energies, RhoHFB and KappaHFB; HF: KappaHFB as
+ * GuessHFBMatrices(0) fills it, HFB.f90:3686-3715) and the constrained
+ * moments.  U, V and CanTransfo are written as zeros (a restart rebuilds
+ * them in its first SolvePairing).  extra gives what the handle does not
+ * hold: fd_order[2], force, skyrme, hbar, com2body, omega, crank_values,
+ * and for HF the Fermi energies; NULL takes CoulombLapOrder 2, a blank
+ * name, zeros, hbar = 6.58211928, Fermi = -10. */
+int mocca_v1_save(mocca_solve *h, const char *path, const mocca_v1_info *extra);
+
+/* Restart a solve from a v1 file as Input/ReadMOCCa_v1 and Evolve's
+ * start-up take one (InOut.f90:122-187,1040-1416; Main.f90:206-215): the
+ * mesh and nwt must equal the handle's, and the file may not break a
+ * symmetry the handle conserves.  Loads the states, quantum numbers,
+ * occupations, single-particle energies and dispersions; with HFB selected
+ * (mocca_set_pairing before this call) the Fermi energies and RhoHFB /
+ * KappaHFB; unless recalc (the reference's Recalc) the file's densities,
+ * which the next mocca_scf_start then uses instead of recomputing them.
+ * info (may be NULL) receives the header, including the constrained
+ * moments (ContinueMoment is left to the caller: mocca_set_constraints). */
+int mocca_v1_load(mocca_solve *h, const char *path, int recalc, mocca_v1_info *info);
+
 /* Kernel timing over the handle's stream with HIP events (for benchmarks):
  * enable = 1 starts recording each launch; mocca_profile_get returns the
  * summed milliseconds and launch count for a kernel family name

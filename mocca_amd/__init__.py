@@ -38,6 +38,7 @@ EXPORTS = (
     "mocca_alternate_step", "mocca_get_constraints", "mocca_set_pairing",
     "mocca_solve_pairing", "mocca_get_pairing", "mocca_batch_run",
     "mocca_gram_schmidt_noalloc",
+    "mocca_v1_read", "mocca_v1_write", "mocca_v1_save", "mocca_v1_load",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -93,6 +94,34 @@ class PairingC(ctypes.Structure):
                 ("fermi_prec", ctypes.c_double), ("fermi", ctypes.c_double * 2)]
 
 
+V1_MAX_MOMENTS = 32
+
+
+class V1MomentC(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int) for k in ("constraint_type", "isoswitch", "l", "m", "impart")] + \
+        [(k, ctypes.c_double * 2) for k in ("intensity", "constraint", "true_constraint",
+                                            "value")] + [("old_value", ctypes.c_double * 14)]
+
+
+class V1InfoC(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("denergy", ctypes.c_double * 7),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double)] + \
+        [(k, ctypes.c_int) for k in ("trc", "pc", "sc", "tsc", "ic", "nwt")] + \
+        [("neutrons", ctypes.c_double), ("protons", ctypes.c_double),
+         ("fd_order", ctypes.c_int * 3), ("force", ctypes.c_char * 200),
+         ("skyrme", ctypes.c_double * 14), ("hbar", ctypes.c_double),
+         ("hbm", ctypes.c_double * 2), ("com1body", ctypes.c_int), ("com2body", ctypes.c_int),
+         ("B", ctypes.c_double * 28), ("pairing_type", ctypes.c_int),
+         ("pairing_strength", ctypes.c_double * 2), ("pairing_alpha", ctypes.c_double * 2),
+         ("pairing_cut", ctypes.c_double * 2), ("has_cutoffs", ctypes.c_int),
+         ("fermi", ctypes.c_double * 2), ("lnlambda", ctypes.c_double * 2),
+         ("hfb_size", ctypes.c_int), ("npar", ctypes.c_int), ("niso", ctypes.c_int),
+         ("blocksizes", ctypes.c_int * 4), ("omega", ctypes.c_double * 3),
+         ("crank_values", ctypes.c_double * 3), ("nmoments", ctypes.c_int),
+         ("moments", V1MomentC * V1_MAX_MOMENTS)]
+
+
 class MoccaError(RuntimeError):
     pass
 
@@ -133,6 +162,131 @@ def gram_schmidt_noalloc(psi, out, nwt, n, qn, energy, dv, status=None, stream=N
     if rc != 0:
         msg = lib.mocca_last_error(None)
         raise MoccaError(f"mocca error {rc}: {msg.decode() if msg else ''}")
+
+
+class V1ArraysC(ctypes.Structure):
+    pass
+
+
+def _v1_arrays_type():
+    if not hasattr(V1ArraysC, "_fields_"):
+        V1ArraysC._fields_ = [("grid", _dp), ("state", _dp), ("qn", _ip), ("cutoffs", _dp),
+                              ("dens", DensityC), ("u", _dp), ("v", _dp),
+                              ("hfb_columns", _ip), ("rho_hfb", _dp), ("kappa_hfb", _dp),
+                              ("can_transfo", _dp)]
+    return V1ArraysC
+
+
+def _v1_chk(lib, rc):
+    if rc != 0:
+        msg = lib.mocca_last_error(None)
+        raise MoccaError(f"mocca error {rc}: {msg.decode() if msg else ''}")
+
+
+def _info_dict(c):
+    """V1InfoC -> dict of plain values (the constrained moments as a list)."""
+    d = {}
+    for name, _ in V1InfoC._fields_:
+        if name == "moments":
+            continue
+        v = getattr(c, name)
+        d[name] = bytes(v) if name == "force" else (list(v) if hasattr(v, "__len__") else v)
+    d["force"] = bytes(c.force).ljust(200, b" ")
+    d["moments"] = [{k: (list(getattr(m, k)) if hasattr(getattr(m, k), "__len__") else
+                         getattr(m, k)) for k, _ in V1MomentC._fields_}
+                    for m in c.moments[:c.nmoments]]
+    return d
+
+
+def _info_c(d):
+    c = V1InfoC()
+    for name, typ in V1InfoC._fields_:
+        if name == "moments" or name not in d:
+            continue
+        v = d[name]
+        if name == "force":
+            c.force = bytes(v)[:200].ljust(200, b" ")
+        elif hasattr(typ, "_length_"):
+            getattr(c, name)[:] = list(v)
+        else:
+            setattr(c, name, v)
+    ms = d.get("moments", [])
+    c.nmoments = len(ms)
+    for k, m in enumerate(ms):
+        for name, typ in V1MomentC._fields_:
+            if hasattr(typ, "_length_"):
+                getattr(c.moments[k], name)[:] = list(m[name])
+            else:
+                setattr(c.moments[k], name, int(m[name]))
+    return c
+
+
+def _v1_buffers(info, N):
+    """Host arrays of a v1 file's per-state and array records."""
+    M, S = info["nwt"], info["hfb_size"]
+    nb = info["npar"] * info["niso"]
+    a = dict(grid=np.zeros((M, 4, N)), state=np.zeros((M, 7)), qn=np.zeros((M, 5), np.int32),
+             cutoffs=np.zeros(M), rho=np.zeros((2, N)), derrho=np.zeros((2, 3, N)),
+             laprho=np.zeros((2, N)), tau=np.zeros((2, N)), nablaJ=np.zeros((2, N)),
+             u=np.zeros((4, 2 * S, S), np.complex128), v=np.zeros((4, 2 * S, S), np.complex128),
+             hfb_columns=np.zeros((nb, S), np.int32),
+             rho_hfb=np.zeros((4, S, S), np.complex128),
+             kappa_hfb=np.zeros((4, S, S), np.complex128),
+             can_transfo=np.zeros((nb, S, S), np.complex128))
+    if not info["trc"]:
+        a.update(vecj=np.zeros((2, 3, N)), rotvecj=np.zeros((2, 3, N)), vecs=np.zeros((2, 3, N)),
+                 rots=np.zeros((2, 3, N)), ders=np.zeros((2, 3, 3, N)))
+    return a
+
+
+def _v1_arrays_c(a):
+    T = _v1_arrays_type()
+    c = T()
+
+    def dp(x):
+        if x is None:
+            return None
+        return (x.view(np.float64) if x.dtype == np.complex128 else x).ctypes.data_as(_dp)
+
+    c.grid, c.state, c.cutoffs = dp(a.get("grid")), dp(a.get("state")), dp(a.get("cutoffs"))
+    c.qn = a["qn"].ctypes.data_as(_ip) if a.get("qn") is not None else None
+    for k in ("rho", "tau", "nablaJ", "derrho", "laprho", "vecj", "vecs", "rotvecj", "rots",
+              "ders"):
+        setattr(c.dens, k, dp(a.get(k)))
+    c.u, c.v = dp(a.get("u")), dp(a.get("v"))
+    c.hfb_columns = (a["hfb_columns"].ctypes.data_as(_ip)
+                     if a.get("hfb_columns") is not None else None)
+    c.rho_hfb, c.kappa_hfb = dp(a.get("rho_hfb")), dp(a.get("kappa_hfb"))
+    c.can_transfo = dp(a.get("can_transfo"))
+    return c
+
+
+def v1_read(path, arrays=True):
+    """mocca_v1_read: (info dict, dict of host arrays or None).  Array
+    shapes: grid [nwt][4][N], state [nwt][7] (Occupation, Energy,
+    Dispersion, Norm, J), qn [nwt][5], densities as mocca_density, the HFB
+    matrices complex [block][column][row] (Fortran order)."""
+    lib = load()
+    c = V1InfoC()
+    _v1_chk(lib, lib.mocca_v1_read(os.fsencode(path), ctypes.byref(c), None))
+    info = _info_dict(c)
+    if not arrays:
+        return info, None
+    a = _v1_buffers(info, info["nx"] * info["ny"] * info["nz"])
+    ac = _v1_arrays_c(a)
+    _v1_chk(lib, lib.mocca_v1_read(os.fsencode(path), ctypes.byref(c), ctypes.byref(ac)))
+    if not info["has_cutoffs"]:
+        a["cutoffs"] = None
+    return info, a
+
+
+def v1_write(path, info, arrays):
+    """mocca_v1_write from v1_read's pair (info dict, arrays dict)."""
+    lib = load()
+    c = _info_c(info)
+    a = {k: (np.ascontiguousarray(v) if v is not None else None) for k, v in arrays.items()}
+    _v1_chk(lib, lib.mocca_v1_write(os.fsencode(path), ctypes.byref(c),
+                                    ctypes.byref(_v1_arrays_c(a))))
 
 
 def _p(a):
@@ -334,6 +488,25 @@ class Solve:
         e = np.zeros(ROW_LEN)
         self._chk(self.lib.mocca_final_iteration(self.h, _p(e)))
         return e
+
+    def save_v1(self, path, **extra):
+        """mocca_v1_save: a checkpoint in the MOCCa v1 format.  extra: any
+        mocca_v1_info member the handle does not hold (fd_order, force,
+        skyrme, hbar, com2body, omega, crank_values, fermi for HF)."""
+        if extra:
+            base = dict(fd_order=[0, 0, 2], force=b"", hbar=6.58211928, fermi=[-10.0, -10.0])
+            base.update(extra)
+            c = _info_c(base)
+            self._chk(self.lib.mocca_v1_save(self.h, os.fsencode(path), ctypes.byref(c)))
+        else:
+            self._chk(self.lib.mocca_v1_save(self.h, os.fsencode(path), None))
+
+    def load_v1(self, path, recalc=False):
+        """mocca_v1_load: restart from a v1 file; returns its header dict."""
+        c = V1InfoC()
+        self._chk(self.lib.mocca_v1_load(self.h, os.fsencode(path), int(bool(recalc)),
+                                         ctypes.byref(c)))
+        return _info_dict(c)
 
     def iterate(self, niter, dt_over_hbar, momentum, fetch=True):
         e = np.zeros((niter + 1, ROW_LEN)) if fetch else None

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = [os.path.join(HERE, "csrc", f) for f in ("mocca_kernels.hip", "mocca_stencil.hip", "mocca_scf.hip",
                                               "mocca_capi.hip", "mocca_scf_capi.hip",
-                                              "mocca_hfb.hip", "mocca_hfb_large.hip", "mocca_noalloc.hip")]
+                                              "mocca_hfb.hip", "mocca_hfb_large.hip", "mocca_noalloc.hip",
+                                              "mocca_v1.hip")]
 OUT = os.path.join(HERE, "libmocca_amd.so")
 OBJ = os.path.join(HERE, "_build")
 

@@ -262,6 +262,7 @@ int gram_schmidt_impl(mocca_solve *h, const T *src, T *dst, T *upd, int update_m
         launch_cholesky_ld(h->b.G, h->b.blk_states, h->b.blk_m, h->nblk, h->max_m, h->mmax,
                            h->b.E, h->b.Tm, h->b.perm, h->b.work, h->b.status,
                            h->chol_lds, h->st);
+        launch_gs_norms(h->b.Tm, h->b.perm, h->b.blk_m, h->nblk, h->max_m, h->b.nrm, h->st);
     });
     if (rc) return rc;
     if (h->mfma_apply && !update_mode)
@@ -698,7 +699,7 @@ int set_scheme(mocca_solve *h, int lagrange) {
 void free_all(mocca_solve *h) {
     Buffers &b = h->b;
     void *ptrs[] = {b.psi, b.psi2, b.upd, b.hbuf, b.fields, b.dpart, b.lagT, b.lagD,
-                    b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.K, b.eparts,
+                    b.der, b.lap, b.dens, b.derived, b.qn, b.occ, b.E, b.D, b.nrm, b.K, b.eparts,
                     b.order, b.chunks, b.all_states, b.blk_states, b.blk_m, b.perm,
                     b.G, b.Tm, b.work, b.gpart, b.status, b.jobs, b.one_chunk,
                     b.one_order, b.tasks, b.one_tasks, b.dchunks, b.dtasks, b.tasks_dma,
@@ -819,6 +820,7 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     TRY(dalloc(h, &b.occ, M * sizeof(double)));
     TRY(dalloc(h, &b.E, M * sizeof(double)));
     TRY(dalloc(h, &b.D, M * sizeof(double)));
+    TRY(dalloc(h, &b.nrm, M * sizeof(double)));
     TRY(dalloc(h, &b.status, sizeof(int)));
     TRY(dalloc(h, &b.one_chunk, sizeof(Chunk)));
     TRY(dalloc(h, &b.one_order, sizeof(int)));
@@ -850,6 +852,7 @@ int mocca_solve_create(const mocca_mesh *mesh, int nwt, int precision,
     HIPCHK(h, hipMemset(b.derived, 0, 38 * N * sizeof(double)));
     HIPCHK(h, hipMemset(b.E, 0, M * sizeof(double)));
     HIPCHK(h, hipMemset(b.D, 0, M * sizeof(double)));
+    HIPCHK(h, hipMemset(b.nrm, 0, M * sizeof(double)));
     HIPCHK(h, hipMemset(b.status, 0, sizeof(int)));
     // Lagrange matrices (EV8: the full line doubles every stored axis).
     const int ns[3] = {mesh->nx, mesh->ny, mesh->nz};

@@ -717,6 +717,10 @@ int hfb_check_status(mocca_solve *h) {
 int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
     hfb_free(h);
     Hfb &p = h->hfb;
+    if (q) {
+        p.in = *q;
+        p.in_set = true;
+    }
     if (!q || q->type == 0) return 0;
     if (q->type != 2)
         return fail(h, MOCCA_ERR_UNSUPPORTED, "pairing type %d (0 HF, 2 HFB)", q->type);

@@ -31,6 +31,8 @@ struct Buffers {
     double *derived = nullptr;     // derrho[2][3] laprho[2] rotvecj[2][3] rots[2][3] ders[2][3][3]
     int *qn = nullptr;
     double *occ = nullptr, *E = nullptr, *D = nullptr, *K = nullptr;
+    double *nrm = nullptr;         // [M] norm of each state in the last Gram-Schmidt
+                                   // before its normalisation (the Spwf's Norm)
     double *eparts = nullptr;      // [M][tiles][3]: <psi|h psi>, <h psi|h psi>, <psi|Lap psi>
     int *order = nullptr;
     Chunk *chunks = nullptr;
@@ -83,11 +85,13 @@ struct Scf {
     // the last CompEnergy and OldEnergy(1:7) before it (Energy.f90:93-98),
     // and the Fermi energies of the last energy row; on the device, the last
     // 8 values of sum(Value) of each constrained moment (Value, then
-    // OldValue(:,1:7), Moments.f90:1920-1932)
+    // OldValue(:,1:7), Moments.f90:1920-1932, per isospin)
     double ehist[8] = {};
     double efermi[2] = {};
     bool start_pending = false;    // the start-up row is not yet in ehist (HF)
-    double *chist = nullptr;       // [ncon][8]
+    bool dens_loaded = false;      // dens/derived hold a wavefunction file's densities,
+                                   // which the next mocca_scf_start keeps
+    double *chist = nullptr;       // [ncon][8][2]
     // MixingScheme 2 (potential preconditioning)
     int mixing_scheme = 0;
     double preconu = 1.0;
@@ -133,6 +137,8 @@ struct Est {
 // HFB pairing state (PairingType 2).
 struct Hfb {
     bool on = false, started = false, unconverged = false;
+    mocca_pairing in{};            // the parameters of the last mocca_set_pairing
+    bool in_set = false;
     HfbK k{};
     int total = 0, hmax = 0;
     size_t lds_broyden = 0, lds_canonical = 0;

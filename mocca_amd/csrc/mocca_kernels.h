@@ -212,6 +212,12 @@ void launch_cholesky_ld(const double *G, const int *blk_states,
                         double *Tm, int *perm, double *work, int *status,
                         size_t lds_limit, hipStream_t st);
 size_t cholesky_lds_limit();
+// Norm of each state before its normalisation in the Gram-Schmidt whose
+// factor Tm = L^-T, perm holds: nrm[perm[b][r]] = L[r][r]^2 (the squared
+// norm of the state less its projections on the lower-energy states of its
+// block, Gramschmidt's CompNorm, SpwfStorage.f90:388-389).
+void launch_gs_norms(const double *Tm, const int *perm, const int *blk_m, int nblk,
+                     int max_m, double *nrm, hipStream_t st);
 
 template <typename T>
 void launch_gs_apply(const T *src, T *dst, T *upd, long K4, const int *perm,
@@ -328,9 +334,9 @@ void launch_con_squared(const ConstraintK *con, int nc, const double *Y, const d
 void launch_con_update(ConstraintK *con, int nc, ConstraintCtl *ctl, const double *Q,
                        const double *sq, const double *cstat, int mode, int iteration,
                        double momentum, double *ce_coef, double *alt, hipStream_t st);
-// OldValue bookkeeping of CalculateAllMoments for ConvergenceCheck: passes
-// times, hist[c][1:8] <- hist[c][0:7] and hist[c][0] = sum over isospins of
-// the constraint's moment.
+// Value / OldValue bookkeeping of CalculateAllMoments (ConvergenceCheck, the
+// wavefunction file): passes times, hist[c][1:8][:] <- hist[c][0:7][:] and
+// hist[c][0][it] = the constraint's moment of isospin it.
 void launch_con_history(const ConstraintK *con, int nc, const double *Q, double *hist,
                         int passes, hipStream_t st);
 void launch_con_field(const ConstraintK *con, int nc, const double *Y, const double *coef,

@@ -2101,6 +2101,21 @@ void launch_cholesky_ld(const double *G, const int *blk_states, const int *blk_m
                        blk_m, max_m, E, Tm, perm, work, status);
 }
 
+__global__ void k_gs_norms(const double *__restrict__ Tm, const int *__restrict__ perm,
+                           const int *__restrict__ blk_m, int max_m, double *__restrict__ nrm) {
+    const int b = blockIdx.x;
+    const double *Tb = Tm + (long)b * max_m * max_m;
+    for (int r = threadIdx.x; r < blk_m[b]; r += blockDim.x) {
+        const double d = Tb[(long)r * max_m + r];   // 1 / L[r][r]
+        nrm[perm[(long)b * max_m + r]] = 1.0 / (d * d);
+    }
+}
+
+void launch_gs_norms(const double *Tm, const int *perm, const int *blk_m, int nblk,
+                     int max_m, double *nrm, hipStream_t st) {
+    hipLaunchKernelGGL(k_gs_norms, dim3(nblk), dim3(64), 0, st, Tm, perm, blk_m, max_m, nrm);
+}
+
 size_t cholesky_lds_limit() {
     int dev = 0, lim = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;

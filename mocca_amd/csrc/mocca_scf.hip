@@ -1096,11 +1096,15 @@ __global__ void k_con_history(const ConstraintK *__restrict__ con, int nc,
                               const double *__restrict__ Q, double *__restrict__ hist,
                               int passes) {
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-        const double v = Q[2 * con[c].idx] + Q[2 * con[c].idx + 1];
-        double *hc = hist + 8 * c;
+        const double v0 = Q[2 * con[c].idx], v1 = Q[2 * con[c].idx + 1];
+        double *hc = hist + 16 * c;
         for (int p = 0; p < passes; ++p) {
-            for (int i = 7; i > 0; --i) hc[i] = hc[i - 1];
-            hc[0] = v;
+            for (int i = 7; i > 0; --i) {
+                hc[2 * i] = hc[2 * i - 2];
+                hc[2 * i + 1] = hc[2 * i - 1];
+            }
+            hc[0] = v0;
+            hc[1] = v1;
         }
     }
 }

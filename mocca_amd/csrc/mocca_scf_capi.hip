@@ -901,7 +901,7 @@ int mocca_convergence_check(mocca_solve *h, const double *prec, int *out) {
     bool mom = true;
     if (s.ncon) {
         std::vector<ConstraintK> ck(s.ncon);
-        std::vector<double> hist(8 * (size_t)s.ncon);
+        std::vector<double> hist(16 * (size_t)s.ncon);
         HIPCHK(h, hipStreamSynchronize(h->st));
         HIPCHK(h, hipMemcpy(ck.data(), s.con, ck.size() * sizeof(ConstraintK),
                             hipMemcpyDeviceToHost));
@@ -914,7 +914,7 @@ int mocca_convergence_check(mocca_solve *h, const double *prec, int *out) {
             const double con = k.iso == 1 ? k.con[0] : k.con[0] + k.con[1];
             const double tru = k.iso == 1 ? k.truec[0] : k.truec[0] + k.truec[1];
             for (int i = 0; i < 8 && mom; ++i) {
-                const double q = hist[8 * (size_t)c + i];
+                const double q = hist[16 * (size_t)c + 2 * i] + hist[16 * (size_t)c + 2 * i + 1];
                 const double d = tru != 0.0 ? std::fabs((con - q) / con) : std::fabs(con - q);
                 if (!(d < prec[0])) mom = false;
             }
@@ -980,7 +980,10 @@ int mocca_scf_start(mocca_solve *h) {
     if (int rc = check_scf(h)) return rc;
     int rc;
     if ((rc = fill_impl(h))) return rc;
-    if ((rc = density_any(h))) return rc;
+    // densities of the states, or those of a wavefunction file read without
+    // Recalc (Main.f90:206-215)
+    if (!h->scf.dens_loaded && (rc = density_any(h))) return rc;
+    h->scf.dens_loaded = false;
     if ((rc = save_history(h))) return rc;
     if ((rc = construct_impl(h, 3))) return rc;
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
@@ -1295,9 +1298,9 @@ int mocca_set_constraints(mocca_solve *h, const mocca_constraint *c, int count, 
         (rc = dalloc(h, &s.cstat, 4 * sizeof(double))) ||
         (rc = dalloc(h, &s.ccnt, 2 * sizeof(int))) ||
         (rc = dalloc(h, &s.csurf, 2 * 3 * (size_t)N * 3 * sizeof(double))) ||
-        (rc = dalloc(h, &s.chist, 8 * ck.size() * sizeof(double))))
+        (rc = dalloc(h, &s.chist, 16 * ck.size() * sizeof(double))))
         return rc;
-    HIPCHK(h, hipMemset(s.chist, 0, 8 * ck.size() * sizeof(double)));
+    HIPCHK(h, hipMemset(s.chist, 0, 16 * ck.size() * sizeof(double)));
     HIPCHK(h, hipMemcpy(s.con, ck.data(), ck.size() * sizeof(ConstraintK), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(s.cctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemset(s.ce, 0, 2 * N * sizeof(double)));

@@ -198,6 +198,21 @@ module mocca_amd_c
       import :: c_int, c_ptr
       type(c_ptr), value :: h, states, canonical, totals
     end function
+    ! Checkpoint / restart in the MOCCa v1 format (WriteMOCCa_v1 /
+    ! ReadMOCCa_v1, InOut.f90:905-1416); path: a NUL-terminated name
+    ! (trim(name)//c_null_char); extra / info: c_null_ptr or the address of
+    ! a mocca_v1_info record
+    integer(c_int) function mocca_v1_save(h, path, extra) bind(C)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, extra
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function mocca_v1_load(h, path, recalc, info) bind(C)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, info
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: recalc
+    end function
     integer(c_int) function mocca_final_iteration(h, row) bind(C)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

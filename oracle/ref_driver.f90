@@ -40,6 +40,14 @@
 ! CompEnergy (<prefix>_final.bin: the energy row as in _scf.bin), then the
 ! finite-difference derivatives and UpdateDensities(0) again (the density
 ! rho(n,2) follows the row).
+! With 'wv1' the scf task ends by writing a wavefunction file as Evolve does
+! at a checkpoint (Main.f90:318-325), after the iteration's UpdateAM (:301):
+! WriteMOCCa_v1 to <prefix>.v1.  With 'rv1' the states, densities, energies,
+! occupations, Fermi energies (and, with HFB, the HFB matrices) come from the
+! wavefunction file restart.v1 (ReadMOCCa_v1, then the HFB matrices as Input
+! takes them, InOut.f90:171-183) instead of the input file's seeds, and the
+! scf start-up recomputes the densities only when Recalc is set
+! (Main.f90:206-215).
 !-------------------------------------------------------------------------------
 program ref_driver
   use CompilationInfo
@@ -52,7 +60,10 @@ program ref_driver
   use Densities
   use MeanFields
   use ImaginaryTime
-  use InOutput, only : ReadRunInfo
+  use InOutput, only : ReadRunInfo, WriteMOCCa_v1, ReadMOCCa_v1, OutputFileName,        &
+  &                    InputFileName, InputKappa, InputRho, InputU, InputV, inPC, inTRC,   &
+  &                    inIC, filenwt, FileHFBColumns, FileBlocksizes
+  use Transform, only : TransformHFBMatrices
   use Moments
   use Coulomb
   use Energy
@@ -99,6 +110,16 @@ program ref_driver
     call HFBasis(i)%SetOcc(occ(i))
   enddo
   DensityBasis => HFBasis
+
+  if(index(tasks,'rv1').gt.0) then
+    InputFileName = 'restart.v1'
+    call ReadMOCCa_v1(62)
+    if(allocated(InputKappa)) then
+      call TransformHFBMatrices(InputU, InputV, InputRho, InputKappa, inPC, inTRC, inIC, &
+      &                         filenwt, FileHFBColumns, FileBlocksizes)
+    endif
+    DensityBasis => HFBasis
+  endif
 
   if(index(tasks,'hpsi').gt.0 .or. index(tasks,'step').gt.0) then
     ! Mean-field arrays (allocated the way ConstructPotentials does,
@@ -258,7 +279,7 @@ program ref_driver
       close(uh)
     endif
     call DeriveAll
-    call UpdateDensities(0)
+    if(index(tasks,'rv1').eq.0 .or. Recalc) call UpdateDensities(0)
     call CalculateAllMoments(1)
     call CalculateAllMoments(1)
     call CalculateAllMoments(1)
@@ -310,6 +331,11 @@ program ref_driver
     enddo
     close(uu)
     call dump_fields(trim(prefix)//'_scfpot.bin')
+    if(index(tasks,'wv1').gt.0) then
+      call UpdateAM()
+      OutputFileName = trim(prefix)//'.v1'
+      call WriteMOCCa_v1(61)
+    endif
     if(index(tasks,'final').gt.0) then
       ! FinalIteration (Main.f90:482-529), without the printing
       call AssignFDCoefs(-1, -1, CoulombLapOrder)

@@ -16,6 +16,7 @@ Meshes are deliberately non-cubic so that axis mix-ups show.
 """
 import json
 import os
+import shutil
 import subprocess
 import sys
 
@@ -326,7 +327,7 @@ def read_pot(wd, prefix, nx, ny, nz, trc):
 SCF_CASES = {
     # name: (n, dx, trc, lag, neutrons, protons, nwt_per_isospin, niter,
     #        ParameterEstimation)
-    "scf_trc_fd": (10, 0.8, True, False, 8, 8, 6, 12, 0),
+    "scf_trc_fd": (10, 0.8, True, False, 8, 8, 6, 12, 0, dict(v1=True)),
     "scf_trc_lag": (10, 0.8, True, True, 8, 8, 6, 12, 0),
     "scf_todd_fd": (8, 0.9, False, False, 4, 4, 6, 8, 0),
     # IterativeEstimation before every step; maxspwf starts from seed.bin
@@ -345,12 +346,13 @@ SCF_CASES = {
     "scfcon_todd_lag": (8, 0.9, False, True, 4, 4, 6, 8, 0,
                         dict(constraints=["l=2, m=0, Isoswitch=2, ConstraintNeutrons=9.0, "
                                           "ConstraintProtons=7.0",
-                                          "l=2, m=2, ConstraintType=1, Constraint=2.0"])),
+                                          "l=2, m=2, ConstraintType=1, Constraint=2.0"],
+                             v1=True)),
     # HFB pairing (the sample input's pairing namelist): 4 + 4 particles in
     # 6 + 6 states and their time-reversed partners
     "scfhfb_trc_fd": (10, 0.8, True, False, 6, 4, 6, 10, 0,
                       dict(pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
-                                   'GuessKappa=.true.', start="hf")),
+                                   'GuessKappa=.true.', start="hf", v1=True)),
     # The sample input of the reference's README (24Mg): 12 + 12 nucleons,
     # 20 + 20 states on the 10^3 octant of a 20^3 mesh at dx = 0.8 fm,
     # Lagrange derivatives, ParameterEstimation 1, MixingScheme 2, HFB with
@@ -448,9 +450,36 @@ def run_scf(name):
                     mixing=mixing, preconw=preconw, constraints=constraints, pairing=pairing)
     with open(os.path.join(wd, "input.nml"), "w") as fh:
         fh.write(nml)
+    tasks = "force,scf,conv,final" + (",wv1" if opts.get("v1") else "")
     with open(os.path.join(wd, "input.nml")) as fin:
-        subprocess.run([DRIVER, "input.bin", "out", "force,scf,conv,final", str(niter)],
+        subprocess.run([DRIVER, "input.bin", "out", tasks, str(niter)],
                        cwd=wd, stdin=fin, check=True, stdout=subprocess.DEVNULL)
+    if opts.get("v1"):
+        # The wavefunction file the reference writes at the end of the run
+        # (WriteMOCCa_v1), and a restart from it: ReadMOCCa_v1, the start-up
+        # with the file's densities (Recalc false) and 3 iterations, with
+        # the same namelists except GuessKappa (a restart keeps the file's
+        # kappa)
+        raw = open(os.path.join(wd, "out.v1"), "rb").read()
+        extra["v1_file"] = np.frombuffer(raw, np.uint8)
+        wd2 = wd + "_rv1"
+        os.makedirs(wd2, exist_ok=True)
+        for fn in ("energies.bin", "fermi.bin"):
+            if os.path.exists(os.path.join(wd2, fn)):
+                os.remove(os.path.join(wd2, fn))
+        if not os.path.exists(os.path.join(wd2, "forces.param")):
+            os.symlink(FORCES, os.path.join(wd2, "forces.param"))
+        shutil.copy(os.path.join(wd, "input.bin"), os.path.join(wd2, "input.bin"))
+        open(os.path.join(wd2, "restart.v1"), "wb").write(raw)
+        with open(os.path.join(wd2, "input.nml"), "w") as fh:
+            fh.write(nml.replace("GuessKappa=.true.", "GuessKappa=.false."))
+        with open(os.path.join(wd2, "input.nml")) as fin:
+            subprocess.run([DRIVER, "input.bin", "rs", "force,scf,rv1", "3"], cwd=wd2,
+                           stdin=fin, check=True, stdout=subprocess.DEVNULL)
+        extra["v1_restart_traj"] = np.fromfile(os.path.join(wd2, "rs_scf.bin"),
+                                               "<f8").reshape(4, 41)
+        extra["v1_restart_pairing"] = np.fromfile(os.path.join(wd2, "rs_pairing.bin"),
+                                                  "<f8").reshape(4, 4)
     if pe:
         extra["params"] = np.fromfile(os.path.join(wd, "out_params.bin"),
                                       "<f8").reshape(niter, 2)

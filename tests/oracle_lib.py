@@ -1105,3 +1105,185 @@ class HFB:
         self.gaps(psi, field, pc, dv)
         self.broyden(energy)
         return self.occupations(psi)
+
+
+# ---------------------------------------------------------------------------
+# MOCCa v1 wavefunction files (WriteMOCCa_v1 / ReadMOCCa_v1,
+# src/InOut.f90:905-1416): Fortran sequential unformatted records, each
+# framed by its byte count (int32 little endian) before and after.  Parsed
+# here record by record into named numpy arrays, and rebuilt from them.
+
+def v1_records(buf):
+    out, p = [], 0
+    while p < len(buf):
+        n = int(np.frombuffer(buf[p:p + 4], "<i4")[0])
+        assert int(np.frombuffer(buf[p + 4 + n:p + 8 + n], "<i4")[0]) == n
+        out.append(bytes(buf[p + 4:p + 4 + n]))
+        p += 8 + n
+    assert p == len(buf)
+    return out
+
+
+def _fields(rec, spec):
+    """Split one record by spec = [(name, dtype, count)]."""
+    d, p = {}, 0
+    for name, dt, cnt in spec:
+        size = np.dtype(dt).itemsize * cnt
+        d[name] = np.frombuffer(rec[p:p + size], dt).copy()
+        p += size
+    assert p == len(rec), (p, len(rec))
+    return d
+
+
+V1_STATE = [("occupation", "<f8", 1), ("energy", "<f8", 1), ("dispersion", "<f8", 1),
+            ("norm", "<f8", 1), ("J", "<f8", 3), ("qn", "<i4", 5)]   # Spwf.f90:2257-2260
+V1_MOMENT = [("intensity", "<f8", 2), ("constraint", "<f8", 2), ("true_constraint", "<f8", 2),
+             ("value", "<f8", 2), ("old_value", "<f8", 14)]           # Moments.f90:3341-3359
+
+
+def v1_parse(buf):
+    """The contents of a v1 file in the order WriteMOCCa_v1 writes them
+    (InOut.f90:957-1036).  Density arrays keep Fortran order flattened."""
+    r = v1_records(buf)
+    f = {}
+    f["version"] = int(np.frombuffer(r[0], "<i4")[0])
+    e = np.frombuffer(r[1], "<f8")
+    f["energy"], f["denergy"] = float(e[0]), e[1:].copy()               # :960
+    f.update(_fields(r[2], [("n", "<i4", 3), ("dx", "<f8", 1)]))       # :962
+    f["sym"] = np.frombuffer(r[3], "<i4").copy()                        # TRC PC SC TSC IC
+    f.update(_fields(r[4], [("nwt", "<i4", 1), ("particles", "<f8", 2)]))
+    nwt = int(f["nwt"][0])
+    nx, ny, nz = (int(v) for v in f["n"])
+    N = nx * ny * nz
+    k = 5
+    grid, st = [], []
+    for _ in range(nwt):                                                # Spwf.f90:2255-2260
+        grid.append(np.frombuffer(r[k], "<f8").reshape(4, N))
+        st.append(_fields(r[k + 1], V1_STATE))
+        k += 2
+    f["grid"] = np.array(grid)
+    for name in ("occupation", "energy", "dispersion", "norm", "J", "qn"):
+        f["state_" + name] = np.array([s[name] for s in st]).reshape(nwt, -1).squeeze()
+    # WriteDensity (Densities.f90:1189-1240)
+    f["dens_rec"] = [np.frombuffer(x, "<f8").copy() for x in r[k:k + 9]]
+    k += 9
+    f["fd_order"] = np.frombuffer(r[k], "<i4").copy()
+    f["force"] = r[k + 1]
+    f["skyrme"] = np.frombuffer(r[k + 2], "<f8").copy()
+    f.update(_fields(r[k + 3], [("force2", "S200", 1), ("hbar", "<f8", 1), ("hbm", "<f8", 2),
+                                ("com", "<i4", 2)]))
+    f["B"] = np.frombuffer(r[k + 4], "<f8").copy()
+    f.update(_fields(r[k + 5], [("pairing_type", "<i4", 1), ("strength", "<f8", 2),
+                                ("alpha", "<f8", 2), ("cut", "<f8", 2)]))
+    f["cutoffs"] = np.frombuffer(r[k + 6], "<f8").copy()
+    fl = np.frombuffer(r[k + 7], "<f8")
+    f["fermi"], f["lnlambda"] = fl[:2].copy(), fl[2:].copy()
+    k += 8
+    S = 2 * nwt if f["sym"][0] else nwt
+    pi = 2 if f["sym"][1] else 1
+    ii = 2 if f["sym"][4] else 1
+    f["hfb_size"] = S
+    f["blocksizes"] = np.frombuffer(r[k], "<i4").reshape(ii, pi).copy()   # (P, it) Fortran
+    uv = np.frombuffer(r[k + 1], "<c16")
+    f["U"] = uv[:uv.size // 2].reshape(-1, 2 * S, S).copy()              # U(S, 2S, :, :)
+    f["V"] = uv[uv.size // 2:].reshape(-1, 2 * S, S).copy()
+    f["hfb_columns"] = np.frombuffer(r[k + 2], "<i4").reshape(-1, S).copy()
+    rk = np.frombuffer(r[k + 3], "<c16")
+    f["rho_hfb"] = rk[:rk.size // 2].reshape(-1, S, S).copy()            # (S, S, 2, 2)
+    f["kappa_hfb"] = rk[rk.size // 2:].reshape(-1, S, S).copy()
+    f["can_transfo"] = np.frombuffer(r[k + 4], "<c16").reshape(-1, S, S).copy()
+    oc = np.frombuffer(r[k + 5], "<f8")
+    f["omega"], f["crank_values"] = oc[:3].copy(), oc[3:].copy()
+    k += 6
+    f["moments"] = []
+    while k < len(r):                                                     # WriteMoment
+        h = np.frombuffer(r[k], "<i4")
+        m = dict(constraint_type=int(h[0]), isoswitch=int(h[1]), l=int(h[2]), m=int(h[3]),
+                 impart=int(h[4]))
+        m.update(_fields(r[k + 1], V1_MOMENT))
+        f["moments"].append(m)
+        k += 2
+    return f
+
+
+def v1_build(f):
+    """Bytes of a v1 file from v1_parse's dictionary."""
+    recs = []
+    put = recs.append
+    put(np.array([f["version"]], "<i4").tobytes())
+    put(np.concatenate([[f["energy"]], f["denergy"]]).astype("<f8").tobytes())
+    put(np.asarray(f["n"], "<i4").tobytes() + np.asarray(f["dx"], "<f8").tobytes())
+    put(np.asarray(f["sym"], "<i4").tobytes())
+    put(np.asarray(f["nwt"], "<i4").tobytes() + np.asarray(f["particles"], "<f8").tobytes())
+    nwt = int(np.asarray(f["nwt"]).reshape(-1)[0])
+    for i in range(nwt):
+        put(np.asarray(f["grid"][i], "<f8").tobytes())
+        put(np.array([f["state_occupation"][i], f["state_energy"][i],
+                      f["state_dispersion"][i], f["state_norm"][i]], "<f8").tobytes() +
+            np.asarray(f["state_J"][i], "<f8").tobytes() +
+            np.asarray(f["state_qn"][i], "<i4").tobytes())
+    for d in f["dens_rec"]:
+        put(np.asarray(d, "<f8").tobytes())
+    put(np.asarray(f["fd_order"], "<i4").tobytes())
+    put(f["force"])
+    put(np.asarray(f["skyrme"], "<f8").tobytes())
+    put(np.asarray(f["force2"], "S200").tobytes() + np.asarray(f["hbar"], "<f8").tobytes() +
+        np.asarray(f["hbm"], "<f8").tobytes() + np.asarray(f["com"], "<i4").tobytes())
+    put(np.asarray(f["B"], "<f8").tobytes())
+    put(np.asarray(f["pairing_type"], "<i4").tobytes() +
+        np.concatenate([f["strength"], f["alpha"], f["cut"]]).astype("<f8").tobytes())
+    put(np.asarray(f["cutoffs"], "<f8").tobytes())
+    put(np.concatenate([f["fermi"], f["lnlambda"]]).astype("<f8").tobytes())
+    put(np.asarray(f["blocksizes"], "<i4").tobytes())
+    put(np.asarray(f["U"], "<c16").tobytes() + np.asarray(f["V"], "<c16").tobytes())
+    put(np.asarray(f["hfb_columns"], "<i4").tobytes())
+    put(np.asarray(f["rho_hfb"], "<c16").tobytes() + np.asarray(f["kappa_hfb"], "<c16").tobytes())
+    put(np.asarray(f["can_transfo"], "<c16").tobytes())
+    put(np.concatenate([f["omega"], f["crank_values"]]).astype("<f8").tobytes())
+    for m in f["moments"]:
+        put(np.array([m["constraint_type"], m["isoswitch"], m["l"], m["m"], m["impart"]],
+                     "<i4").tobytes())
+        put(b"".join(np.asarray(m[k], "<f8").tobytes() for k, _, _ in V1_MOMENT))
+    out = bytearray()
+    for x in recs:
+        n = np.array([len(x)], "<i4").tobytes()
+        out += n + x + n
+    return bytes(out)
+
+
+def v1_blocks(qn, trc):
+    """Rows of the reference's HFB blocks (PrepareHFBModule,
+    HFB.f90:248-294): rows i = 0..S-1 hold state i mod nwt, the
+    time-reversed copy for i >= nwt; block (P, it) lists its rows in order.
+    qn: [nwt][4] (isospin, parity, signature, time simplex)."""
+    nwt = len(qn)
+    S = 2 * nwt if trc else nwt
+    rows = {}
+    for i in range(S):
+        iso, P = qn[i % nwt][0], qn[i % nwt][1]
+        rows.setdefault(((P + 3) // 2, (iso + 3) // 2), []).append(i)
+    return S, rows
+
+
+def v1_hf_kappa_guess(qn, trc):
+    """KappaHFB of GuessHFBMatrices for PairingType 0 (HFB.f90:3686-3715):
+    in each block, +0.1 at (i, j) and -0.1 at (j, i) for rows i < j of
+    opposite signature (a time-reversed copy has the opposite one).
+    Returned as complex [it][P][j][i] (the file's Fortran order)."""
+    nwt = len(qn)
+    S, rows = v1_blocks(qn, trc)
+    K = np.zeros((2, 2, S, S), np.complex128)
+
+    def sig(i):
+        s = qn[i % nwt][2]
+        return -s if i >= nwt else s
+
+    for (P, it), r in rows.items():
+        # the reference's loop order (j outer, i inner): the later write wins
+        for jj in range(len(r)):
+            for ii in range(len(r)):
+                if sig(r[jj]) * sig(r[ii]) > 0:
+                    continue
+                K[it - 1, P - 1, jj, ii] = 0.1      # KappaHFB(i, j) = Occ
+                K[it - 1, P - 1, ii, jj] = -0.1     # KappaHFB(j, i) = -Occ
+    return K

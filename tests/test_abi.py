@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "mocca_amd.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(mocca_[a-z_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(mocca_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_and_bindings_agree():
