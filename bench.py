@@ -50,7 +50,7 @@ REFERENCE_TIMING = "cpu_reference_full_iteration_hfb_40cubed_400states.md"
 # stages timed by the library's own HIP events (mocca_profile_get)
 STAGES = ("lag_derivs", "derive_all", "hpsi_step", "hpsi", "reduce_ed", "gram", "gram_reduce",
           "cholesky", "gs_apply", "hf_fill", "hfb_mix", "hfb_field", "hfb_gaps", "hfb_fermi",
-          "hfb_jacobi", "hfb_jacobi_apply", "hfb_canonical", "hfb_energy", "density",
+          "hfb_jacobi", "hfb_jacobi_apply", "hfb_sign", "hfb_canonical", "hfb_energy", "density",
           "density_reduce", "field_jobs", "mix", "moments", "constraints", "alternate",
           "coulomb_bc", "coulomb_cg", "coulomb_direct", "potentials", "precondition",
           "pot_jobs", "pot_convert", "energy", "energy_kinetic", "kinetic", "cutoff",

@@ -380,16 +380,21 @@ typedef struct mocca_pairing {
     int iterations;       /* HFBIter (50 for the Broyden solver) */
     int solver;           /* Fermi-energy solver: 0 = automatic (1 while every
                              (isospin, parity) block holds at most 32 states,
-                             else 2); 1 = the reference's Broyden iteration
+                             else 4); 1 = the reference's Broyden iteration
                              (HFBFindFermiEnergyBroyden) in one workgroup per
                              isospin with the matrices in LDS, blocks of at most
                              32 states; 2 = the same iteration on the
-                             large-block path (blocks of up to 128 states);
+                             large-block path (blocks of up to 128 states),
+                             each evaluation a Jacobi diagonalisation;
                              3 = Newton's method with the exact derivative
                              dN/dlambda on the large-block path: fewer
                              evaluations to the same FermiPrec, but where the
                              reference's iteration stops unconverged it ends
-                             elsewhere */
+                             elsewhere; 4 = the iteration of 2 with rho, kappa
+                             and N(lambda) from the spectral projector
+                             (1 + sign T) / 2 of each evaluation (matrix-core
+                             products, no diagonalisation; the quasiparticle
+                             energies are formed when asked for) */
     double strength[2];   /* PairingNeutron, PairingProton (MeV fm^3) */
     double alpha[2];      /* AlphaNeutron, AlphaProton */
     double cut[2];        /* CutNeutron, CutProton (5 MeV) */

@@ -553,7 +553,9 @@ class Solve:
                     fermi=(-10.0, -10.0), guess_kappa=True, solver=0):
         """Pairing of SolvePairing: type 0 HF, 2 HFB (namelist Pairing defaults);
         solver 0 automatic, 1 in-LDS Broyden (blocks <= 32 states), 2 large-block
-        Broyden, 3 large-block Newton (blocks <= 128)."""
+        Broyden with Jacobi diagonalisations, 3 large-block Newton, 4 large-block
+        Broyden with the sign function (blocks <= 128; the automatic choice above
+        32 states)."""
         d2 = lambda v: (ctypes.c_double * 2)(*[float(x) for x in v])
         p = PairingC(int(type), int(bool(guess_kappa)), int(iterations), int(solver),
                      d2(strength),

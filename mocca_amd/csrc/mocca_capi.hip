@@ -506,6 +506,7 @@ int estimate_impl(mocca_solve *h, int iteration, double dt, double mu, double *o
         if (rec[3] != 0.0) break;
         batch = std::min(2 * batch, 64);
     }
+    if (h->hfb.on && (rc = hfb_ensure_qpe(h))) return rc;
     rc = run(h, "est_finalize", [&] {
         launch_est_finalize(h->b.E, h->b.occ, h->b.qn, h->M, h->hfb.on ? 2 : 0, h->hfb.qpe,
                             h->hfb.on ? h->M : 0, kHbar, e.rec, h->st);

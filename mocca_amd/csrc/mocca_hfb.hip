@@ -793,16 +793,18 @@ int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
     // the in-LDS Broyden solver while its matrices fit, else the large-block
     // Newton solver
     const bool small_ok = 2 * p.hmax <= kHfbMaxN && p.lds_broyden <= 60 * 1024;
-    if (q->solver < 0 || q->solver > 3)
+    if (q->solver < 0 || q->solver > 4)
         return fail(h, MOCCA_ERR_ARG,
                     "pairing solver %d (0 automatic, 1 in-LDS Broyden, 2 large-block Broyden, "
-                    "3 large-block Newton)", q->solver);
+                    "3 large-block Newton, 4 large-block Broyden with the sign function)",
+                    q->solver);
     if (q->solver == 1 && !small_ok)
         return fail(h, MOCCA_ERR_UNSUPPORTED,
                     "HFB block of %d states: the in-LDS solver takes at most %d", p.hmax,
                     kHfbMaxN / 2);
     p.large = q->solver >= 2 || (q->solver == 0 && !small_ok);
     p.newton = q->solver == 3;
+    p.sign = q->solver == 4 || (q->solver == 0 && p.large);
     if (p.large && !hfb_large_fits(p.hmax))
         return fail(h, MOCCA_ERR_UNSUPPORTED, "HFB block of %d states (at most 128)", p.hmax);
     const long N = h->N;

@@ -1001,9 +1001,9 @@ __global__ void k_hl_newton(HfbK k, const double *__restrict__ num, double *__re
 // 10 converged, 11 F.
 __global__ void k_hl_broyden(HfbK k, const double *__restrict__ num, double *__restrict__ nst,
                              int *__restrict__ active, double *__restrict__ fermi,
-                             int *__restrict__ status) {
+                             int *__restrict__ status, const double *__restrict__ skip) {
     const int it = threadIdx.x;
-    if (it >= 2) return;
+    if (it >= 2 || (skip && *skip != 0.0)) return;
     double *s = nst + it * kHlNewton;
     if (s[5] != 0.0) return;
     double nn = 0;
@@ -1084,6 +1084,245 @@ __global__ void k_hl_broyden(HfbK k, const double *__restrict__ num, double *__r
     s[11] = F;
     s[6] = next - lam;
     s[0] = next;
+}
+
+// ---------------------------------------------------------------------------
+// Particle numbers without an eigen-decomposition.  The quasiparticles that
+// HFBColumns selects are the eigenvectors of T(lambda) with positive
+// eigenvalue (the spectrum of the signature-reduced T comes in +-pairs), so
+// rho, kappa and N(lambda) are blocks of the spectral projector
+// P+ = (1 + sign T) / 2:
+//   rho = [[R, 0], [0, R]],  R = P+[h:, h:];
+//   kappa = [[0, -Q^T], [Q, 0]],  Q = P+[h:, :h];   N = 2 tr R.
+// sign T is the limit of the scaled Newton-Schulz iteration
+//   X0 = T / s,  X <- (a/2) X (3 - a^2 X^2),
+// s the largest absolute row sum of T (a bound of its spectral radius), a
+// the scaling for the eigenvalue interval [l, 1] of X (l from a bound of the
+// smallest |eigenvalue| relative to s, raised along the iteration), two
+// symmetric matrix products per step on the FP64 matrix cores.  Every X is
+// a polynomial in T, so the products commute and stay symmetric: each is
+// formed for the upper 32 x 32 tiles and mirrored.
+constexpr int kSignTile = 16;
+constexpr int kSignSteps = 16;           // k steps of 4 per wave (blocks up to 256 rows)
+// residual slots per block: four waves of each of the at most 16 x 17 / 2
+// upper tiles
+constexpr int kSignParts = 4 * 16 * 17 / 2;
+constexpr double kSignResidual = 1e-8;   // max |X^2 - 1| before the last step
+
+__device__ __forceinline__ double hl_wave_max(double a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    return a;
+}
+
+// Off-diagonal absolute row sums of T (the gaps: independent of lambda),
+// once per Fermi solve: rs[b][i] = sum_j |T_ij|, j != i.
+__global__ __launch_bounds__(256) void k_hs_rowsum(HfbK k, const double *__restrict__ delta,
+                                                   int ld, double *__restrict__ rs) {
+    const int b = blockIdx.x;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const double *dl = delta + bk.off;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double r = 0.0;
+        if (i < h)
+            for (int j = 0; j < h; ++j) r += fabs(dl[i * n + h + j]);
+        else
+            for (int j = 0; j < h; ++j) r += fabs(dl[j * n + i]);
+        rs[(long)b * ld + i] = r;
+    }
+}
+
+// X = T(lambda) / s for the active blocks (lambda of the evaluation from the
+// record nst[it][0]), s = the largest absolute row sum of T (a bound of its
+// spectral radius; every workgroup of a block forms it from rs and the
+// diagonal); the redo flag nst[2][0] cleared.  T as k_hl_build_T forms it.
+__global__ __launch_bounds__(256) void k_hs_build(HfbK k, const double *__restrict__ E,
+                                                  const double *__restrict__ delta,
+                                                  double *__restrict__ nst,
+                                                  const double *__restrict__ rs,
+                                                  const int *__restrict__ active, int ld,
+                                                  double *__restrict__ X) {
+    __shared__ double sinv;
+    const int b = blockIdx.y, tid = threadIdx.x;
+    if (b == 0 && blockIdx.x == 0 && tid == 0) nst[2 * kHlNewton] = 0.0;   // redo flag
+    if (!active[b]) return;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const double l = nst[bk.it * kHlNewton];   // lambda of this evaluation
+    if (tid < 64) {
+        double mx = 0.0;
+        for (int i = tid; i < n; i += 64) {
+            const double d = E[k.state[bk.first + (i < h ? i : i - h)]] - l;
+            mx = fmax(mx, fabs(d) + rs[(long)b * ld + i]);
+        }
+        mx = hl_wave_max(mx);
+        if (tid == 0) sinv = mx > 0.0 ? 1.0 / mx : 1.0;
+    }
+    __syncthreads();
+    const double inv = sinv;
+    const int e = blockIdx.x * blockDim.x + tid;
+    if (e >= n * n) return;
+    const int i = e / n, j = e % n;
+    const double *dl = delta + bk.off;
+    double v = 0.0;
+    if (i < h && j < h) v = i == j ? E[k.state[bk.first + i]] - l : 0.0;
+    else if (i >= h && j >= h) v = i == j ? -(E[k.state[bk.first + i - h]] - l) : 0.0;
+    else if (i < h) v = dl[i * n + h + (j - h)];
+    else v = dl[j * n + h + (i - h)];
+    X[(long)b * ld * ld + (long)i * ld + j] = v * inv;
+}
+
+// C = c1 D + c2 A B for symmetric A, B whose product is symmetric; one
+// workgroup of four waves per upper 16 x 16 tile (I <= J) of a block,
+// mirrored into (J, I).  The waves split the k range in four (every operand
+// of a wave loaded before its first product) and add their partial tiles
+// through LDS.  With RESID each wave's largest |C - 1| goes into its slot
+// of sig[b][.] (reduced by k_hs_number).
+template <bool RESID>
+__global__ __launch_bounds__(256) void k_hs_gemm(const double *__restrict__ A,
+                                                 const double *__restrict__ B,
+                                                 const double *__restrict__ D, double c1,
+                                                 double c2, double *__restrict__ C, int ld,
+                                                 long mstride, const int *__restrict__ nb,
+                                                 const int *__restrict__ active,
+                                                 double *__restrict__ sig) {
+    __shared__ double part[4][4][64];   // [wave][register][lane]
+    const int b = blockIdx.y, tid = threadIdx.x, wv = tid / 64, lane = tid % 64;
+    if (!active[b]) return;
+    const int n = nb[b], nt = (n + kSignTile - 1) / kSignTile;
+    int t = blockIdx.x, I = 0;
+    while (I < nt && t >= nt - I) {
+        t -= nt - I;
+        ++I;
+    }
+    if (I >= nt) return;
+    const int J = I + t;
+    const int i0 = I * kSignTile, j0 = J * kSignTile;
+    const double *Ab = A + b * mstride, *Bb = B + b * mstride;
+    const int lr = lane % 16, lk = lane / 16;
+    hl_d4 acc = hl_d4{0, 0, 0, 0};
+    // this wave's quarter of k, in steps of 4 (at most kSignSteps: n <= 256)
+    const int kq = ((n + 15) / 16) * 4;
+    const int kb = wv * kq, ke = min(n, kb + kq);
+    // A is symmetric: its operand A[i][k] is read as A[k][i] (rows, coalesced)
+    const bool ia = i0 + lr < n, jb = j0 + lr < n;
+    double av[kSignSteps], bv[kSignSteps];
+#pragma unroll
+    for (int q = 0; q < kSignSteps; ++q) {
+        const int kk = kb + 4 * q + lk;
+        const bool kin = kk < ke;
+        const long row = (long)(kin ? kk : 0) * ld;
+        av[q] = kin && ia ? Ab[row + i0 + lr] : 0.0;
+        bv[q] = kin && jb ? Bb[row + j0 + lr] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kSignSteps; ++q) {
+        if (kb + 4 * q >= ke) break;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part[wv][r][lane] = acc[r];
+    __syncthreads();
+    // wave wv finishes register wv of the tile
+    const int r = wv;
+    const double sum = part[0][r][lane] + part[1][r][lane] + part[2][r][lane] + part[3][r][lane];
+    // accumulator lane L, register r: row L/16 + 4 r, column L%16
+    const int row = i0 + lk + 4 * r, col = j0 + lr;
+    double res = 0.0;
+    if (row < n && col < n && !(I == J && row > col)) {
+        double v = c2 * sum;
+        if (D) v += c1 * D[b * mstride + (long)row * ld + col];
+        double *Cb = C + b * mstride;
+        Cb[(long)row * ld + col] = v;
+        Cb[(long)col * ld + row] = v;
+        if (RESID) res = fabs(v - (row == col ? 1.0 : 0.0));
+    }
+    if (RESID) {
+        res = hl_wave_max(res);
+        if (lane == 0) sig[(long)b * kSignParts + 4 * blockIdx.x + wv] = res;
+    }
+}
+
+// N of each active block from the converged X: num[2b] = 2 tr R =
+// sum_{i >= h} (1 + X_ii) (one wave per block); *redo = 1 when the
+// block's residual max |X^2 - 1| before the last step (the slots sig[b][.])
+// is above kSignResidual (the evaluation then continues before the Fermi
+// step).
+__global__ __launch_bounds__(64) void k_hs_number(HfbK k, const double *__restrict__ X, int ld,
+                                                  const int *__restrict__ active,
+                                                  const double *__restrict__ sig,
+                                                  double *__restrict__ num,
+                                                  double *__restrict__ redo) {
+    const int b = blockIdx.x, lane = threadIdx.x;
+    if (!active[b]) return;
+    const int h = k.blk[b].h, n = 2 * h;
+    const int nt = (n + kSignTile - 1) / kSignTile, nparts = 4 * nt * (nt + 1) / 2;
+    double res = 0.0;
+    for (int q = lane; q < nparts; q += 64) res = fmax(res, sig[(long)b * kSignParts + q]);
+    res = hl_wave_max(res);
+    if (!(res <= kSignResidual)) {
+        if (lane == 0) *redo = 1.0;
+        return;
+    }
+    const double *Xb = X + (long)b * ld * ld;
+    double s = 0.0;
+    for (int i = h + lane; i < n; i += 64) s += 1.0 + Xb[(long)i * ld + i];
+    s = hl_wave_sum(s);
+    if (lane == 0) {
+        num[2 * b] = s;
+        num[2 * b + 1] = 0.0;
+    }
+}
+
+__global__ void k_hs_clear(double *__restrict__ flag) { *flag = 0.0; }
+
+
+__global__ void k_hs_activate(int *__restrict__ active, int nblk) {
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) active[b] = 1;
+}
+
+
+// rho and kappa from X (the layout of k_hl_rho_kappa).
+__global__ void k_hs_rho_kappa(HfbK k, const double *__restrict__ X, int ld,
+                               double *__restrict__ rho, double *__restrict__ kappa) {
+    const int b = blockIdx.y;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int i = e / n, j = e % n;
+    const double *Xb = X + (long)b * ld * ld;
+    double v = 0.0, kv = 0.0;
+    if (i < h && j < h)
+        v = 0.5 * ((i == j ? 1.0 : 0.0) + Xb[(long)(h + i) * ld + h + j]);
+    else if (i >= h && j >= h)
+        v = 0.5 * ((i == j ? 1.0 : 0.0) + Xb[(long)i * ld + j]);
+    else if (i >= h)
+        kv = 0.5 * Xb[(long)i * ld + j];
+    else
+        kv = -0.5 * Xb[(long)j * ld + i];
+    rho[bk.off + e] = v;
+    kappa[bk.off + e] = kv;
+}
+
+// The h largest eigenvalues of each block, in descending order, as
+// k_hl_rho_kappa stores them.
+__global__ void k_hl_qpe(HfbK k, const double *__restrict__ w, int ld,
+                         double *__restrict__ qpe) {
+    const int b = blockIdx.x;
+    const HfbBlockK &bk = k.blk[b];
+    const int h = bk.h, n = 2 * h;
+    const double *wb = w + (long)b * ld;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double wi = wb[i];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+            const double wj = wb[j];
+            if (wj > wi || (wj == wi && j < i)) ++r;
+        }
+        if (r < h) qpe[bk.first + r] = wi;
+    }
 }
 
 // rho and kappa of the blocks from the eigenvectors of the h largest
@@ -1259,20 +1498,25 @@ int hfb_large_setup(mocca_solve *h) {
         (rc = dalloc(h, &p.lnum, 2 * (size_t)k.nblk * sizeof(double))) ||
         (rc = dalloc(h, &p.loff, 2 * (size_t)k.nblk * sizeof(double))) ||
         (rc = dalloc(h, &p.lcold, (size_t)k.nblk * sizeof(int))) ||
-        (rc = dalloc(h, &p.lnst, 2 * kHlNewton * sizeof(double))) ||
+        (rc = dalloc(h, &p.lnst, (2 * kHlNewton + 2) * sizeof(double))) ||
         (rc = dalloc(h, &p.lactive, (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.lnb, 2 * (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.lhb, (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.llist, (size_t)k.nblk * hmax * sizeof(int))) ||
         (rc = dalloc(h, &p.lTap, (size_t)k.nblk * hmax * hmax * sizeof(double))) ||
         (rc = dalloc(h, &p.lgpart, (size_t)k.nblk * p.lsplit * ntile * ntile * 256 *
-                                       sizeof(double))))
+                                       sizeof(double))) ||
+        (rc = dalloc(h, &p.lsig, (size_t)k.nblk * kSignParts * sizeof(double))))
         return rc;
     std::vector<int> nb(2 * k.nblk), hb(k.nblk);
     for (int b = 0; b < k.nblk; ++b) {
         nb[b] = 2 * k.blk[b].h;
         nb[k.nblk + b] = k.blk[b].h;
         hb[b] = k.blk[b].h;
+    }
+    if (hipHostMalloc((void **)&p.lhost, (2 * kHlNewton + 2) * sizeof(double)) != hipSuccess) {
+        p.lhost = nullptr;
+        return fail(h, MOCCA_ERR_HIP, "hipHostMalloc of the Fermi-solve records failed");
     }
     HIPCHK(h, hipMemcpy(p.lnb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(p.lhb, hb.data(), hb.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1289,9 +1533,11 @@ int hfb_large_free(mocca_solve *h) {
     Hfb &p = h->hfb;
     void *ptrs[] = {p.lT, p.lV, p.lA, p.lW, p.lC, p.lVc, p.lw, p.lwc, p.lrank, p.llog,
                     p.lrounds, p.lnum, p.lnst, p.lactive, p.lnb, p.lhb, p.llist, p.lTap,
-                    p.lgpart, p.loff, p.lcold, p.llogij, p.lAf};
+                    p.lgpart, p.loff, p.lcold, p.llogij, p.lAf, p.lsig};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    if (p.lhost) (void)hipHostFree(p.lhost);
+    p.lhost = nullptr;
     return 0;
 }
 
@@ -1347,6 +1593,149 @@ int jacobi_run(mocca_solve *h, const double *A, double *V, double *w, const int 
 
 }  // namespace
 
+// The canonical basis from rho (in lT: rho_h of every block): eigenvectors
+// by the Jacobi kernel, the canonical states by the dense matrix-core apply,
+// then the rho/kappa mixing.
+template <typename T>
+int hfb_large_canonical(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    const HfbK &k = p.k;
+    const long N = h->N;
+    const int nblk = k.nblk, ld = p.nmax, hmax = p.hmax;
+    const int mpad = (hmax + 15) / 16 * 16;
+    // (the block sizes h are the second half of lnb)
+    bool odd = false;
+    for (int b = 0; b < nblk; ++b) odd = odd || (k.blk[b].h & 1);
+    // after the first step the Jacobi starts in the last step's canonical
+    // basis: A = Vc^T rho_h Vc, the rotations applied to Vc
+    const bool warm = p.can_warm;
+    if (warm) {
+        const long ms = (long)ld * ld;
+        const dim3 gm((hmax + 15) / 16, (hmax + 15) / 16, nblk);
+        if (int rc = run(h, "hfb_canonical", [&] {
+                hipLaunchKernelGGL((k_lj_gemm<false, false>), gm, dim3(256), 0, h->st, p.lT,
+                                   p.lVc, p.lW, ld, ms, p.lnb + nblk, nullptr, nullptr);
+                hipLaunchKernelGGL((k_lj_gemm<true, false>), gm, dim3(256), 0, h->st, p.lVc,
+                                   p.lW, p.lC, ld, ms, p.lnb + nblk, nullptr, nullptr);
+            }))
+            return rc;
+    }
+    if (int rc = jacobi_run(h, warm ? p.lC : p.lT, p.lVc, p.lwc, p.lnb + nblk, hmax, nullptr,
+                            p.lrounds + nblk, !warm, odd, nullptr, nullptr))
+        return rc;
+    p.can_warm = true;
+    return run(h, "hfb_canonical", [&] {
+        hipLaunchKernelGGL(k_hl_can_finish, dim3(nblk), dim3(256), 0, h->st, p.k, p.lVc, p.lwc,
+                           ld, p.rho, hmax, p.tcan, p.can_occ, h->b.occ, p.lTap, p.llist,
+                           p.status);
+        launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist, p.lhb,
+                                   nblk, hmax, mpad, p.lTap, h->st);
+        hfb_launch_mix(h, 0);
+    });
+}
+
+// The scaled Newton-Schulz schedule: the scalings a_k for an interval
+// [l0, 1] of |eigenvalues| / s, padded with a plain step (a = 1) to an even
+// count (the iterate then ends in the buffer it starts in).
+const std::vector<double> &sign_schedule() {
+    static const std::vector<double> sched = [] {
+        std::vector<double> a;
+        double l = 1e-3;
+        while (1.0 - l > 1e-15 && a.size() < 40) {
+            const double al = std::sqrt(3.0 / (1.0 + l + l * l));
+            a.push_back(al);
+            l = 0.5 * al * l * (3.0 - al * al * l * l);
+        }
+        if (a.size() & 1) a.push_back(1.0);
+        return a;
+    }();
+    return sched;
+}
+
+// Newton-Schulz steps on X (in X0, ending in X0 for an even count; X1 and Y
+// scratch); the first products of the last step record the residual.
+void sign_steps(mocca_solve *h, const double *alphas, int nsteps) {
+    Hfb &p = h->hfb;
+    const int nblk = p.k.nblk, ld = p.nmax;
+    const long ms = (long)ld * ld;
+    const int nt = (p.nmax + kSignTile - 1) / kSignTile;
+    const dim3 grid(nt * (nt + 1) / 2, nblk);
+    double *X = p.lA, *Xn = p.lC, *Y = p.lW;
+    for (int q = 0; q < nsteps; ++q) {
+        const double a = alphas[q];
+        if (q == nsteps - 1)
+            hipLaunchKernelGGL(k_hs_gemm<true>, grid, dim3(256), 0, h->st, X, X, nullptr, 0.0,
+                               1.0, Y, ld, ms, p.lnb, p.lactive, p.lsig);
+        else
+            hipLaunchKernelGGL(k_hs_gemm<false>, grid, dim3(256), 0, h->st, X, X, nullptr, 0.0,
+                               1.0, Y, ld, ms, p.lnb, p.lactive, p.lsig);
+        hipLaunchKernelGGL(k_hs_gemm<false>, grid, dim3(256), 0, h->st, X, Y, X, 1.5 * a,
+                           -0.5 * a * a * a, Xn, ld, ms, p.lnb, p.lactive, p.lsig);
+        std::swap(X, Xn);
+    }
+}
+
+// The Fermi solve of the sign path: the reference's Broyden iteration with
+// N(lambda) from sign T(lambda) at every evaluation (lambda taken from the
+// iteration's record), until both isospins meet FermiPrec or HFBIter
+// evaluations; X of every block's last evaluation stays in lA.
+int sign_fermi_solve(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    const HfbK &k = p.k;
+    const int nblk = k.nblk, ld = p.nmax;
+    const int nel = p.nmax * p.nmax;
+    double *redo = p.lnst + 2 * kHlNewton;
+    const std::vector<double> &sched = sign_schedule();
+    const double polish[2] = {1.0, 1.0};
+    double *rs = p.lw;   // (the Jacobi's eigenvalue buffer, unused on this path)
+    int rc = run(h, "hfb_fermi", [&] {
+        hipLaunchKernelGGL(k_hl_newton_init, dim3(1), dim3(64), 0, h->st, p.fermi, p.lnst,
+                           nblk, p.lactive);
+        hipLaunchKernelGGL(k_hs_rowsum, dim3(nblk), dim3(256), 0, h->st, k, p.delta, ld, rs);
+    });
+    if (rc) return rc;
+    auto number = [&]() {
+        return run(h, "hfb_fermi", [&] {
+            hipLaunchKernelGGL(k_hs_number, dim3(nblk), dim3(64), 0, h->st, k, p.lA, ld,
+                               p.lactive, p.lsig, p.lnum, redo);
+            hipLaunchKernelGGL(k_hl_broyden, dim3(1), dim3(64), 0, h->st, k, p.lnum, p.lnst,
+                               p.lactive, p.fermi, p.status, redo);
+        });
+    };
+    const size_t nrec = (2 * kHlNewton + 1) * sizeof(double);
+    for (int ev = 0; ev < std::max(1, k.iters) + 3; ++ev) {
+        rc = run(h, "hfb_sign", [&] {
+            hipLaunchKernelGGL(k_hs_build, dim3((nel + 255) / 256, nblk), dim3(256), 0, h->st,
+                               k, h->b.E, p.delta, p.lnst, rs, p.lactive, ld, p.lA);
+            sign_steps(h, sched.data(), (int)sched.size());
+        });
+        if (rc || (rc = number())) return rc;
+        int extra = 0;
+        for (;;) {
+            const double *st = p.lhost;
+            HIPCHK(h, hipMemcpyAsync(p.lhost, p.lnst, nrec, hipMemcpyDeviceToHost, h->st));
+            HIPCHK(h, hipStreamSynchronize(h->st));
+            if (st[2 * kHlNewton] == 0.0) {
+                if (st[5] != 0.0 && st[kHlNewton + 5] != 0.0) return 0;
+                break;
+            }
+            // an eigenvalue closer to 0 than the schedule assumed: more
+            // plain steps (pairs, so that X stays in lA)
+            if ((extra += 2) > 64)
+                return fail(h, MOCCA_ERR_NUMERIC,
+                            "HFB: the sign iteration of a Fermi-solve evaluation did not "
+                            "converge (an eigenvalue of the HFB matrix at 0?); pairing "
+                            "solver 2 diagonalises instead");
+            rc = run(h, "hfb_sign", [&] {
+                hipLaunchKernelGGL(k_hs_clear, dim3(1), dim3(1), 0, h->st, redo);
+                sign_steps(h, polish, 2);
+            });
+            if (rc || (rc = number())) return rc;
+        }
+    }
+    return 0;
+}
+
 template <typename T>
 int hfb_large_solve_impl(mocca_solve *h) {
     Hfb &p = h->hfb;
@@ -1382,6 +1771,20 @@ int hfb_large_solve_impl(mocca_solve *h) {
     // Fermi energies: Newton evaluations until both isospins meet FermiPrec
     const dim3 gmat((p.nmax + 15) / 16, (p.nmax + 15) / 16, nblk);
     const int nel = p.nmax * p.nmax;
+    if (p.sign) {
+        if ((rc = sign_fermi_solve(h))) return rc;
+        p.lwarm = false;
+        p.qpe_stale = true;
+        rc = run(h, "hfb_canonical", [&] {
+            hipLaunchKernelGGL(k_hs_rho_kappa, dim3((nel + 255) / 256, nblk), dim3(256), 0, h->st,
+                               p.k, p.lA, ld, p.rho, p.kappa);
+            hipLaunchKernelGGL(k_hl_can_prep, dim3((hmax * hmax + 255) / 256, nblk), dim3(256),
+                               0, h->st, p.k, p.rho, ld, p.lT, p.lhb);
+        });
+        if (rc) return rc;
+        return hfb_large_canonical<T>(h);
+    }
+    p.qpe_stale = false;
     rc = run(h, "hfb_fermi", [&] {
         hipLaunchKernelGGL(k_hl_newton_init, dim3(1), dim3(64), 0, h->st, p.fermi, p.lnst,
                            nblk, p.lactive);
@@ -1414,7 +1817,7 @@ int hfb_large_solve_impl(mocca_solve *h) {
                                    p.lnst, p.lactive, p.fermi, p.status);
             else
                 hipLaunchKernelGGL(k_hl_broyden, dim3(1), dim3(64), 0, h->st, p.k, p.lnum,
-                                   p.lnst, p.lactive, p.fermi, p.status);
+                                   p.lnst, p.lactive, p.fermi, p.status, nullptr);
             hipLaunchKernelGGL(k_hl_shift_A, dim3((nel + 255) / 256, nblk), dim3(256), 0,
                                h->st, p.k, p.lAf, p.lC, p.lnst, p.lactive, ld, p.lA, h->b.E,
                                p.delta, p.loff, p.lcold);
@@ -1436,20 +1839,27 @@ int hfb_large_solve_impl(mocca_solve *h) {
                            h->st, p.k, p.rho, ld, p.lT, p.lhb);
     });
     if (rc) return rc;
-    // (the block sizes h are the second half of lnb)
-    bool odd = false;
-    for (int b = 0; b < nblk; ++b) odd = odd || (k.blk[b].h & 1);
-    if ((rc = jacobi_run(h, p.lT, p.lVc, p.lwc, p.lnb + nblk, hmax, nullptr,
-                         p.lrounds + nblk, true, odd, nullptr, nullptr)))
-        return rc;
-    return run(h, "hfb_canonical", [&] {
-        hipLaunchKernelGGL(k_hl_can_finish, dim3(nblk), dim3(256), 0, h->st, p.k, p.lVc, p.lwc,
-                           ld, p.rho, hmax, p.tcan, p.can_occ, h->b.occ, p.lTap, p.llist,
-                           p.status);
-        launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist, p.lhb,
-                                   nblk, hmax, mpad, p.lTap, h->st);
-        hfb_launch_mix(h, 0);
+    return hfb_large_canonical<T>(h);
+}
+
+int hfb_ensure_qpe(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    if (!p.on || !p.large || !p.qpe_stale) return 0;
+    const HfbK &k = p.k;
+    const int nblk = k.nblk, ld = p.nmax, nel = p.nmax * p.nmax;
+    int rc = run(h, "hfb_fermi", [&] {
+        hipLaunchKernelGGL(k_hs_activate, dim3(1), dim3(64), 0, h->st, p.lactive, nblk);
+        hipLaunchKernelGGL(k_hl_build_T, dim3((nel + 255) / 256, nblk), dim3(256), 0, h->st, k,
+                           h->b.E, p.delta, p.fermi, ld, p.lT);
     });
+    if (rc || (rc = jacobi_run(h, p.lT, p.lV, p.lw, p.lnb, p.nmax, p.lactive, p.lrounds, true,
+                               false, nullptr, nullptr)))
+        return rc;
+    rc = run(h, "hfb_fermi", [&] {
+        hipLaunchKernelGGL(k_hl_qpe, dim3(nblk), dim3(256), 0, h->st, k, p.lw, ld, p.qpe);
+    });
+    if (!rc) p.qpe_stale = false;
+    return rc;
 }
 
 int hfb_large_solve_any(mocca_solve *h) {

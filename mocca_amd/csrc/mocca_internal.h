@@ -161,6 +161,11 @@ struct Hfb {
     // matrices with leading dimension nmax (the largest 2h)
     bool large = false;
     bool newton = false;           // Newton's method instead of the Broyden iteration
+    bool sign = false;             // particle numbers from the matrix sign function
+    bool qpe_stale = false;        // qpe not formed by the last pairing step (sign path)
+    bool can_warm = false;         // lVc holds the canonical basis of the last step
+    double *lsig = nullptr;        // [nblk][parts] sign iteration: residual of each wave
+    double *lhost = nullptr;       // pinned host copy of the Fermi-solve records
     int nmax = 0, lsplit = 1;
     long lmax = 0;                 // rotation-log entries per block
     size_t lds_limit = 0;
@@ -327,6 +332,10 @@ int hfb_free(mocca_solve *h);
 int hfb_solve_any(mocca_solve *h, bool first);
 int hfb_energy_impl(mocca_solve *h, double *row);
 int hfb_check_status(mocca_solve *h);
+// The quasiparticle energies of the last pairing step where that step did
+// not form them (the sign-function path): one diagonalisation at its Fermi
+// energies.
+int hfb_ensure_qpe(mocca_solve *h);
 // Kernels of mocca_hfb.hip launched by the large-block path: the rho/kappa
 // history step of SolvePairing (first = 1) or the HFBMix mixing (first = 0),
 // and the cutoff factors with the density factor.

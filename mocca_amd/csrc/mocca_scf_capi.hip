@@ -1379,7 +1379,7 @@ int mocca_get_pairing(mocca_solve *h, double *out, double *canonical_occ) {
                 (void)hipStreamSynchronize(h->st);
             }
             (void)hipFree(tmp);
-            if (rc) return rc;
+            if (rc || (rc = hfb_ensure_qpe(h))) return rc;
             HIPCHK(h, hipMemcpy(out + 4, p.qpe, h->M * sizeof(double), hipMemcpyDeviceToHost));
         } else {
             for (int i = 0; i < h->M; ++i) out[4 + i] = 0.0;

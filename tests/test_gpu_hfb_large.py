@@ -4,8 +4,9 @@ matrix-core applies for the pairing field and the canonical states) through
 the C ABI.
 
 - Forced onto the reference's own HFB fixtures, whose blocks the in-LDS
-  solver also takes: the large-block Broyden iteration (solver 2) and
-  Newton's method (solver 3) give every iteration's total energy within 1e-9
+  solver also takes: the large-block Broyden iteration with Jacobi
+  diagonalisations (solver 2) or with the matrix sign function (solver 4)
+  and Newton's method (solver 3) give every iteration's total energy within 1e-9
   relative of the reference run, pairing and Fermi energies within 1e-8 MeV
   (both stop at the reference's FermiPrec, 1e-12 in the particle number);
   the README sample (24Mg, 100 iterations with IterativeEstimation,
@@ -32,7 +33,7 @@ def _traj_cols(rows):
                            rows[:, 5:6] + rows[:, 6:7], rows[:, 7:43]], axis=1)
 
 
-@pytest.mark.parametrize("solver", [2, 3])
+@pytest.mark.parametrize("solver", [2, 3, 4])
 def test_large_block_solvers_on_the_reference_hfb_fixture(solver):
     from test_gpu_scf import _solve
     g = golden("scfhfb_trc_fd")
@@ -62,19 +63,23 @@ def test_large_block_solvers_on_the_reference_hfb_fixture(solver):
     s.close()
 
 
-def test_large_block_broyden_on_the_sample_input():
+@pytest.mark.parametrize("solver", [2, 4])
+def test_large_block_broyden_on_the_sample_input(solver):
     """The sample's start-up step has every single-particle energy at 0, where
     the reference's Broyden iteration stops unconverged after HFBIter
-    iterations; the large-block path replays that iteration exactly."""
+    iterations; the large-block path replays that iteration exactly (with
+    the sign function, solver 4, the quasiparticle energies that
+    IterativeEstimation reads come from one diagonalisation per step)."""
     from test_gpu_scf import _solve
     g = golden("sample_mg24")
     m = g["meta"]
     n = m["niter"]
-    print("sample input, large-block Broyden solver", flush=True)
+    print(f"sample input, large-block Broyden solver {solver}", flush=True)
     s = _solve(g, np.zeros(m["nwt"]))
     s.set_pairing(type=2, strength=m["pairing_strength"], alpha=m["pairing_alpha"],
                   cut=m["pairing_cut"], mu=m["pairing_mu"], rhosat=m["rhosat"],
-                  hfb_mix=m["hfb_mix"], iterations=m["hfb_iter"], guess_kappa=True, solver=2)
+                  hfb_mix=m["hfb_mix"], iterations=m["hfb_iter"], guess_kappa=True,
+                  solver=solver)
     s.estimation_seed(g["est_seed"])
     s.scf_start()
     rows, prm = s.iterate_estimated(n, m["dt"], m["momentum"])
@@ -96,22 +101,25 @@ def test_large_block_broyden_on_the_sample_input():
     s.close()
 
 
-@pytest.mark.parametrize("prec", [mocca_amd.FP64, mocca_amd.MIXED])
-def test_blocks_of_100_states_vs_reference(prec):
+@pytest.mark.parametrize("prec,solver", [(mocca_amd.FP64, 0), (mocca_amd.FP64, 2),
+                                         (mocca_amd.MIXED, 0)])
+def test_blocks_of_100_states_vs_reference(prec, solver):
     """The reference's run with HFB blocks of 100 states (200 x 200 HFB
     matrices): 200 neutron and 40 proton states, 50 + 14 nucleons, 4
-    iterations.  The automatic solver takes the large-block path here."""
+    iterations.  The automatic solver takes the large-block path with the
+    sign function here; solver 2 diagonalises every evaluation."""
     from test_gpu_scf import _solve
     g = golden("scfhfb_large")
     m = g["meta"]
     n = m["niter"]
-    print(f"HFB blocks of 100 states vs the reference: precision {prec}", flush=True)
+    print(f"HFB blocks of 100 states vs the reference: precision {prec}, solver {solver}",
+          flush=True)
     s = _solve(g, np.zeros(m["nwt"]), precision=prec, coulomb=1)
     s.set_energies(g["start_energies"])
     s.set_pairing(type=2, strength=m["pairing_strength"], alpha=m["pairing_alpha"],
                   cut=m["pairing_cut"], mu=m["pairing_mu"], rhosat=m["rhosat"],
                   hfb_mix=m["hfb_mix"], iterations=m["hfb_iter"], fermi=g["start_fermi"],
-                  guess_kappa=True)
+                  guess_kappa=True, solver=solver)
     s.scf_start()
     rows = s.iterate(n, m["dt"] / g["consts"][5], m["momentum"])
     traj = _traj_cols(rows)
