@@ -1640,7 +1640,7 @@ int hfb_large_canonical(mocca_solve *h) {
 const std::vector<double> &sign_schedule() {
     static const std::vector<double> sched = [] {
         std::vector<double> a;
-        double l = 1e-3;
+        double l = 1e-2;
         while (1.0 - l > 1e-15 && a.size() < 40) {
             const double al = std::sqrt(3.0 / (1.0 + l + l * l));
             a.push_back(al);
