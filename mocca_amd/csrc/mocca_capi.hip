@@ -365,6 +365,21 @@ int kinetic_impl(mocca_solve *h) {
     // the density basis: the canonical states under HFB
     const T *psi = (const T *)(h->hfb.on ? h->hfb.can_psi : h->b.psi);
     int rc, slots;
+    if constexpr (sizeof(T) == 8) {
+        if (h->march && h->dma_hpsi) {
+            slots = h->ntile_y * h->nzs;
+            rc = run(h, "kinetic", [&] {
+                launch_hpsi_kinetic_dma(m, psi, h->b.qn, h->b.tasks_dma, h->ntasks_dma, h->TY,
+                                        h->ntile_y, h->nzs, h->mesh.time_reversal,
+                                        h->b.eparts, h->st);
+            });
+            if (rc) return rc;
+            return run(h, "reduce_ed", [&] {
+                launch_reduce_ed(h->b.eparts, slots, h->b.all_states, h->M, h->mesh.dv,
+                                 nullptr, nullptr, h->b.K, h->st);
+            });
+        }
+    }
     if (h->march) {
         slots = h->ntile_y;
         rc = run(h, "kinetic", [&] {

@@ -134,6 +134,11 @@ void launch_hpsi_fd(const MeshK<T> &m, const T *psi, T *psi_out, T *upd, T *hout
 // eparts gets ntile_y * nzs slots per state.  w_antisym: the spin-orbit
 // field W is antisymmetric with a zero diagonal (3 components moved, not 9).
 bool hpsi_dma_ok(const MeshK<double> &m, int TY, int trc);
+// <psi|Lap psi> partial sums of every state by the pipelined FP64 kernel
+// (tasks, eparts as launch_hpsi_fd_dma; element 2 of each slot).
+void launch_hpsi_kinetic_dma(const MeshK<double> &m, const double *psi, const int *qn,
+                             const Task *tasks, int ntasks, int TY, int ntile_y, int nzs,
+                             int trc, double *eparts, hipStream_t st);
 void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_out,
                         const double *prev, const double *fields, const int *qn,
                         const Task *tasks, int ntasks, int TY, int ntile_y, int nzs, int trc,

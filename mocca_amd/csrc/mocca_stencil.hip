@@ -356,7 +356,9 @@ size_t hpsi_dma_lds(int nx, int TY) {
 // halves the register queue per thread, so two waves share a SIMD.
 constexpr int kDmaThreads = 2 * kBlock;
 
-template <typename T, bool TRC, bool WA>
+// KIN: only the sums <psi|Lap psi> of every state (CompKinetic's inner
+// products), no potentials, no step, no stores of states.
+template <typename T, bool TRC, bool WA, bool KIN = false>
 __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     MeshK<T> m, const T *__restrict__ psi, T *__restrict__ psi_out,
     const T *__restrict__ prev, const T *__restrict__ F, const int *__restrict__ qn,
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
     T mu, double *__restrict__ eparts) {
     constexpr int G = 4;
     constexpr bool trc = TRC;
-    constexpr int NF = hpsi_dma_fields<T, TRC, WA>();
+    constexpr int NF = KIN ? 0 : hpsi_dma_fields<T, TRC, WA>();
     constexpr int FD = WA ? 8 : 14;   // first div W slot; S, A, div A follow
     constexpr int NW = kDmaThreads / 64;       // waves
     constexpr int VPL = 16 / (int)sizeof(T);   // values per lane of one 16-byte load
@@ -487,18 +489,18 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __syncthreads();
         // ---- stores of plane z-1, loads of plane z+1 and of the queue ----
-        if (pst_p >= 0) {
+        if (!KIN && pst_p >= 0) {
             po[pst_p] = pst[0];
             po[N + pst_p] = pst[1];
         }
         if (z + 1 < ze) issue(z + 1, b ^ 1);
         const long p = (long)z * plane + col;
         const int zn = z + 5 < nz ? z + 5 : nz - 1;
-        T nxt[2], uo[2];
+        T nxt[2], uo[2] = {T(0), T(0)};
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             nxt[a] = ps[cN + a * N + zn * plane + col];
-            uo[a] = pv[a * N + p];
+            if constexpr (!KIN) uo[a] = pv[a * N + p];
         }
         __builtin_amdgcn_sched_barrier(0);
         // Contributions of this thread's two components (a = 0, 1: c = 2 half
@@ -513,7 +515,31 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
         // Lanes outside the tile compute on zeros.
         const T sg = half ? T(-1) : T(1);
         T own0 = 0, own1 = 0, send0 = 0, send1 = 0;
-        {
+        if constexpr (KIN) {
+            const int rowi = inplane ? tyl : 0;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int c = 2 * half + a;
+                const T *row = Xc + (c * TY + rowi) * W + tx + G;
+                T w[9], dxc, dyc, dzc, lx, ly, lz;
+#pragma unroll
+                for (int k = -4; k <= 4; ++k) w[k + 4] = row[k];
+                fd_pair(m, w, dxc, lx);
+                const T *colp = Yc + (c * YR + rowi + 4) * nx + tx;
+#pragma unroll
+                for (int k = -4; k <= 4; ++k) w[k + 4] = colp[k * nx];
+                if (edge) {
+#pragma unroll
+                    for (int k = -4; k <= 4; ++k) {
+                        const int yy = y + k;
+                        w[k + 4] *= yy < 0 ? sgy[a] : (yy < ny ? T(1) : T(0));
+                    }
+                }
+                fd_pair(m, w, dyc, ly);
+                fd_pair(m, q[a], dzc, lz);
+                kacc += (double)q[a][4] * (double)(lx + ly + lz);
+            }
+        } else {
             const T *pp = Pc + (inplane ? pt : 0);
             const T fu = pp[0], fb = pp[TNP] + kin;
             const T nb0 = pp[2 * TNP], nb1 = pp[3 * TNP], nb2 = pp[4 * TNP];
@@ -588,7 +614,7 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
         // the partner's partial sums of this thread's components
         const T h0 = own0 + __shfl_xor(send0, 32, 64);
         const T h1 = own1 + __shfl_xor(send1, 32, 64);
-        if (active) {
+        if (!KIN && active) {
             const T hh[2] = {h0, h1};
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
@@ -608,7 +634,7 @@ __global__ __launch_bounds__(kDmaThreads) void k_hpsi_fd_dma(
             q[a][8] = (active && z + 5 < nz) ? nxt[a] : T(0);
         }
     }
-    if (pst_p >= 0) {
+    if (!KIN && pst_p >= 0) {
         po[pst_p] = pst[0];
         po[N + pst_p] = pst[1];
     }
@@ -719,6 +745,28 @@ void launch_hpsi_fd_dma(const MeshK<double> &m, const double *psi, double *psi_o
         if (w_antisym) MOCCA_HPSI_DMA(false, true) else MOCCA_HPSI_DMA(false, false)
     }
 #undef MOCCA_HPSI_DMA
+}
+
+void launch_hpsi_kinetic_dma(const MeshK<double> &m, const double *psi, const int *qn,
+                             const Task *tasks, int ntasks, int TY, int ntile_y, int nzs,
+                             int trc, double *eparts, hipStream_t st) {
+    // the LDS of the state buffers only (no potentials)
+    const size_t dshm = sizeof(double) * ((size_t)2 * 4 * TY * (m.nx + 8) +
+                                          (size_t)2 * 4 * (TY + 8) * m.nx);
+#define MOCCA_KIN_DMA(TR)                                                                    \
+    {                                                                                        \
+        static const bool lds_set = require_lds(                                             \
+            (const void *)k_hpsi_fd_dma<double, TR, false, true>, hpsi_dma_lds_limit(),      \
+            "k_hpsi_fd_dma");                                                                \
+        if (!lds_set)                                                                        \
+            require_lds((const void *)k_hpsi_fd_dma<double, TR, false, true>,                \
+                        hpsi_dma_lds_limit(), "k_hpsi_fd_dma");                              \
+        hipLaunchKernelGGL((k_hpsi_fd_dma<double, TR, false, true>), dim3(ntasks),           \
+                           dim3(kDmaThreads), dshm, st, m, psi, nullptr, nullptr, nullptr, qn, \
+                           tasks, TY, ntile_y, nzs, 0.0, 0.0, 0.0, 0.0, eparts);             \
+    }
+    if (trc) MOCCA_KIN_DMA(true) else MOCCA_KIN_DMA(false)
+#undef MOCCA_KIN_DMA
 }
 
 template void launch_hpsi_fd<double>(const MeshK<double> &, const double *, double *,
