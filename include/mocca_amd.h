@@ -368,7 +368,10 @@ int mocca_alternate_step(mocca_solve *h);
 int mocca_get_constraints(mocca_solve *h, double *out);
 
 /* Pairing (namelist Pairing, Pairing.f90:143-560).  type 0 = HF (NaiveFill,
- * the default), 2 = HFB with a density-dependent contact interaction
+ * the default), 1 = BCS (BCS.f90: gaps, Fermi energies and occupations of
+ * the HF states, the loop of SolvePairing over at most `iterations`
+ * (PairingIter, 50 for BCS) steps to FermiPrec; no Lipkin-Nogami), 2 = HFB
+ * with a density-dependent contact interaction
  * -G (1 - alpha rho / rho_sat) (CompDensityFactor,
  * PairingInteraction.f90:265-354), SymmetricFermi cutoff factors, the
  * Broyden Fermi solver and HFBMix mixing of rho and kappa.  HFB here needs
@@ -377,7 +380,8 @@ int mocca_get_constraints(mocca_solve *h, double *out);
 typedef struct mocca_pairing {
     int type;
     int guess_kappa;
-    int iterations;       /* HFBIter (50 for the Broyden solver) */
+    int iterations;       /* HFBIter (50 for the Broyden solver); with BCS
+                             PairingIter (50) */
     int solver;           /* Fermi-energy solver: 0 = automatic (1 while every
                              (isospin, parity) block holds at most 32 states,
                              else 4); 1 = the reference's Broyden iteration

@@ -551,7 +551,8 @@ class Solve:
     def set_pairing(self, type=2, strength=(1250.0, 1250.0), alpha=(1.0, 1.0), cut=(5.0, 5.0),
                     mu=(0.5, 0.5), rhosat=0.16, hfb_mix=0.1, iterations=50, fermi_prec=1e-12,
                     fermi=(-10.0, -10.0), guess_kappa=True, solver=0):
-        """Pairing of SolvePairing: type 0 HF, 2 HFB (namelist Pairing defaults);
+        """Pairing of SolvePairing: type 0 HF, 1 BCS (iterations = PairingIter),
+        2 HFB (namelist Pairing defaults);
         solver 0 automatic, 1 in-LDS Broyden (blocks <= 32 states), 2 large-block
         Broyden with Jacobi diagonalisations, 3 large-block Newton, 4 large-block
         Broyden with the sign function (blocks <= 128; the automatic choice above

@@ -523,8 +523,9 @@ int estimate_impl(mocca_solve *h, int iteration, double dt, double mu, double *o
     }
     if (h->hfb.on && (rc = hfb_ensure_qpe(h))) return rc;
     rc = run(h, "est_finalize", [&] {
-        launch_est_finalize(h->b.E, h->b.occ, h->b.qn, h->M, h->hfb.on ? 2 : 0, h->hfb.qpe,
-                            h->hfb.on ? h->M : 0, kHbar, e.rec, h->st);
+        const bool pair = h->hfb.on || h->hfb.bcs;   // (BCS: qpe holds Eqp of every state)
+        launch_est_finalize(h->b.E, h->b.occ, h->b.qn, h->M, pair ? 2 : 0, h->hfb.qpe,
+                            pair ? h->M : 0, kHbar, e.rec, h->st);
     });
     if (rc) return rc;
     HIPCHK(h, hipMemcpyAsync(rec, e.rec, sizeof rec, hipMemcpyDeviceToHost, h->st));

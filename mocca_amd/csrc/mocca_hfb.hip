@@ -605,6 +605,146 @@ void hfb_guess(const HfbK &k, std::vector<double> &kappa) {
 
 }  // namespace mocca
 
+namespace mocca {
+
+// ---------------------------------------------------------------------------
+// BCS pairing (PairingType 1: BCS.f90, the loop of SolvePairing,
+// Pairing.f90:826-858).  Per state i of isospin it: cutoff factor c_i,
+// gap Delta_i, quasiparticle energy E_i = sqrt((e_i - lambda)^2 + Delta_i^2)
+// (BCSQPEnergies :151-188), occupation 1 - (e_i - lambda) / E_i clamped at
+// 0 (BCSOccupations :190-226); everything real (the pair density of a state
+// with its own time reverse is real: -rho_i).
+
+// Pairing field F_it(r) = dfac_it(r) sum_{i in it} (1/2) c_i^2 Delta_i / E_i
+// (-rho_i(r)), summed state by state as BCSPairingField (:65-101) does.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bcs_field(const T *__restrict__ psi, long N, int M,
+                                                   const int *__restrict__ qn,
+                                                   const double *__restrict__ pc,
+                                                   const double *__restrict__ delta,
+                                                   const double *__restrict__ eqp,
+                                                   const double *__restrict__ dfac,
+                                                   double *__restrict__ field) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int it = blockIdx.y;
+    if (p >= N) return;
+    double f = 0.0;
+    for (int i = 0; i < M; ++i) {
+        if ((qn[4 * i] + 3) / 2 - 1 != it) continue;
+        const double w = 0.5 * pc[i] * pc[i] * delta[i] / eqp[i];
+        const T *ps = psi + (long)i * 4 * N + p;
+        double r = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double v = (double)ps[c * N];
+            r += v * v;
+        }
+        f += w * (-r);
+    }
+    field[it * N + p] = dfac[it * N + p] * f;
+}
+
+// Gaps Delta_i = c_i^2 dv sum_r rho_i(r) F_it(r) (BCSGaps :16-63), one
+// workgroup per state.
+template <typename T>
+__global__ __launch_bounds__(kHfbThreads) void k_bcs_gaps(const T *__restrict__ psi, long N,
+                                                         const int *__restrict__ qn,
+                                                         const double *__restrict__ pc,
+                                                         const double *__restrict__ field,
+                                                         double dv, double *__restrict__ delta) {
+    __shared__ double sh[kHfbThreads / 64];
+    const int i = blockIdx.x;
+    const int it = (qn[4 * i] + 3) / 2 - 1;
+    const T *ps = psi + (long)i * 4 * N;
+    const double *F = field + it * N;
+    double s = 0.0;
+    for (long p = threadIdx.x; p < N; p += blockDim.x) {
+        double r = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double v = (double)ps[c * N + p];
+            r += v * v;
+        }
+        s += r * F[p];
+    }
+    s = hfb_block_sum(s, sh);
+    if (threadIdx.x == 0) delta[i] = pc[i] * pc[i] * dv * s;
+}
+
+// BCSFindFermiEnergy (:103-149) for isospin threadIdx.x: the quasiparticle
+// energies at the current lambda, then lambda = (N - sum (1 - e/E)) /
+// sum 1/E; rec[it] keeps the old lambda.  With first set, gaps all below
+// 1e-5 are first set to 1 (SolvePairing :812) -- that happens in
+// k_bcs_start instead.
+__global__ void k_bcs_fermi(int M, const int *__restrict__ qn, const double *__restrict__ E,
+                            const double *__restrict__ delta, double n0, double n1,
+                            double *__restrict__ fermi, double *__restrict__ eqp,
+                            double *__restrict__ rec) {
+    const int it = threadIdx.x;
+    if (it >= 2) return;
+    const double lam = fermi[it];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < M; ++i) {
+        if ((qn[4 * i] + 3) / 2 - 1 != it) continue;
+        const double d = E[i] - lam;
+        const double e = sqrt(d * d + delta[i] * delta[i]);
+        eqp[i] = e;
+        s1 += 1.0 - E[i] / e;
+        s2 += 1.0 / e;
+    }
+    rec[it] = lam;
+    fermi[it] = ((it == 0 ? n0 : n1) - s1) / s2;
+}
+
+// BCSOccupations (:190-226) at the new lambda, and the convergence test of
+// the loop (|lambda - old| < FermiPrec for both isospins, :854) into rec[2].
+__global__ void k_bcs_occ(int M, const int *__restrict__ qn, const double *__restrict__ E,
+                          const double *__restrict__ delta, const double *__restrict__ fermi,
+                          double *__restrict__ eqp, double *__restrict__ occ, double prec,
+                          double *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+        rec[2] = (fabs(fermi[0] - rec[0]) < prec && fabs(fermi[1] - rec[1]) < prec) ? 1.0 : 0.0;
+    if (i >= M) return;
+    const int it = (qn[4 * i] + 3) / 2 - 1;
+    const double d = E[i] - fermi[it];
+    const double e = sqrt(d * d + delta[i] * delta[i]);
+    eqp[i] = e;
+    const double o = 1.0 - d / e;
+    occ[i] = o < 0.0 ? 0.0 : o;
+}
+
+// Gaps below 1e-5 in every state are set to 1 (SolvePairing :812).
+__global__ void k_bcs_start(int M, double *__restrict__ delta) {
+    __shared__ int big;
+    if (threadIdx.x == 0) big = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < M; i += blockDim.x)
+        if (fabs(delta[i]) >= 1e-5) big = 1;
+    __syncthreads();
+    if (!big)
+        for (int i = threadIdx.x; i < M; i += blockDim.x) delta[i] = 1.0;
+}
+
+// BCSEnergy (:228-247): E_pair(it) = -1/2 sum |Delta_i|^2 / E_i, and the
+// Fermi energies, into the row (elements 43..46).
+__global__ void k_bcs_energy(int M, const int *__restrict__ qn, const double *__restrict__ delta,
+                             const double *__restrict__ eqp, const double *__restrict__ fermi,
+                             double *__restrict__ row) {
+    if (threadIdx.x != 0) return;
+    double e[2] = {0.0, 0.0};
+    for (int i = 0; i < M; ++i) {
+        const int it = (qn[4 * i] + 3) / 2 - 1;
+        e[it] += delta[i] * delta[i] / eqp[i];
+    }
+    row[43] = -0.5 * e[0];
+    row[44] = -0.5 * e[1];
+    row[45] = fermi[0];
+    row[46] = fermi[1];
+}
+
+}  // namespace mocca
+
 // ===========================================================================
 // Host side
 namespace mocca {
@@ -714,6 +854,103 @@ int hfb_check_status(mocca_solve *h) {
     return 0;
 }
 
+// BCS: the parameters in p.k (cutoffs, density factor), Fermi energies,
+// gaps 0 (set to 1 by the first SolvePairing), Eqp 1 (the Spwf default).
+int bcs_setup(mocca_solve *h, const mocca_pairing *q) {
+    Hfb &p = h->hfb;
+    if (!h->scf.set) return fail(h, MOCCA_ERR_STATE, "mocca_scf_setup not called");
+    if (q->iterations < 1) return fail(h, MOCCA_ERR_ARG, "BCS: iterations (PairingIter) < 1");
+    const int M = h->M;
+    const long N = h->N;
+    HfbK &k = p.k;
+    k = HfbK{};
+    for (int it = 0; it < 2; ++it) {
+        k.G[it] = q->strength[it];
+        k.alpha[it] = q->alpha[it];
+        k.cut[it] = q->cut[it];
+        k.mu[it] = q->mu[it];
+    }
+    k.rhosat = q->rhosat;
+    k.prec = q->fermi_prec;
+    k.iters = q->iterations;
+    k.particles[0] = h->scf.sk.neutrons;
+    k.particles[1] = h->scf.sk.protons;
+    int rc;
+    if ((rc = dalloc(h, &p.delta, M * sizeof(double))) ||
+        (rc = dalloc(h, &p.qpe, M * sizeof(double))) ||
+        (rc = dalloc(h, &p.pc, M * sizeof(double))) ||
+        (rc = dalloc(h, &p.fermi, 2 * sizeof(double))) ||
+        (rc = dalloc(h, &p.dfac, 2 * N * sizeof(double))) ||
+        (rc = dalloc(h, &p.field, 2 * N * sizeof(double))) ||
+        (rc = dalloc(h, &p.work, 4 * sizeof(double))))
+        return rc;
+    std::vector<double> ones(M, 1.0);
+    HIPCHK(h, hipMemset(p.delta, 0, M * sizeof(double)));
+    HIPCHK(h, hipMemcpy(p.qpe, ones.data(), M * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(p.pc, 0, M * sizeof(double)));
+    HIPCHK(h, hipMemcpy(p.fermi, q->fermi, 2 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(p.work, 0, 4 * sizeof(double)));
+    p.bcs = true;
+    return 0;
+}
+
+template <typename T>
+int bcs_solve_impl(mocca_solve *h) {
+    Hfb &p = h->hfb;
+    const int M = h->M;
+    const long N = h->N;
+    const T *psi = (const T *)h->b.psi;
+    double *rec = p.work;   // old lambda (2), converged flag
+    int rc = run(h, "bcs", [&] {
+        hipLaunchKernelGGL(k_bcs_start, dim3(1), dim3(256), 0, h->st, M, p.delta);
+        hipLaunchKernelGGL(k_hfb_dfac, dim3((N + 255) / 256), dim3(256), 0, h->st, p.k,
+                           h->b.dens, N, p.dfac);
+    });
+    if (rc) return rc;
+    for (int outer = 0; outer < p.k.iters; ++outer) {
+        rc = run(h, "bcs", [&] {
+            hipLaunchKernelGGL(k_hfb_cutoffs, dim3((M + 63) / 64), dim3(64), 0, h->st, p.k,
+                               h->b.E, h->b.qn, p.fermi, M, p.pc);
+            hipLaunchKernelGGL(k_bcs_field<T>, dim3((N + 255) / 256, 2), dim3(256), 0, h->st,
+                               psi, N, M, h->b.qn, p.pc, p.delta, p.qpe, p.dfac, p.field);
+            hipLaunchKernelGGL(k_bcs_gaps<T>, dim3(M), dim3(kHfbThreads), 0, h->st, psi, N,
+                               h->b.qn, p.pc, p.field, h->mesh.dv, p.delta);
+            hipLaunchKernelGGL(k_bcs_fermi, dim3(1), dim3(64), 0, h->st, M, h->b.qn, h->b.E,
+                               p.delta, p.k.particles[0], p.k.particles[1], p.fermi, p.qpe,
+                               rec);
+            hipLaunchKernelGGL(k_hfb_cutoffs, dim3((M + 63) / 64), dim3(64), 0, h->st, p.k,
+                               h->b.E, h->b.qn, p.fermi, M, p.pc);
+            hipLaunchKernelGGL(k_bcs_occ, dim3((M + 255) / 256), dim3(256), 0, h->st, M,
+                               h->b.qn, h->b.E, p.delta, p.fermi, p.qpe, h->b.occ, p.k.prec,
+                               rec);
+        });
+        if (rc) return rc;
+        double conv = 0.0;
+        HIPCHK(h, hipMemcpyAsync(&conv, rec + 2, sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        if (conv != 0.0) break;
+    }
+    return 0;
+}
+
+int bcs_solve(mocca_solve *h) {
+    return h->prec == MOCCA_FP64 ? bcs_solve_impl<double>(h) : bcs_solve_impl<float>(h);
+}
+
+int bcs_energy_impl(mocca_solve *h, double *row) {
+    Hfb &p = h->hfb;
+    return run(h, "hfb_energy", [&] {
+        hipLaunchKernelGGL(k_bcs_energy, dim3(1), dim3(64), 0, h->st, h->M, h->b.qn, p.delta,
+                           p.qpe, p.fermi, row);
+    });
+}
+
+int pairing_energy_impl(mocca_solve *h, double *row) {
+    if (h->hfb.on) return hfb_energy_impl(h, row);
+    if (h->hfb.bcs) return bcs_energy_impl(h, row);
+    return 0;
+}
+
 int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
     hfb_free(h);
     Hfb &p = h->hfb;
@@ -722,8 +959,9 @@ int hfb_setup(mocca_solve *h, const mocca_pairing *q) {
         p.in_set = true;
     }
     if (!q || q->type == 0) return 0;
+    if (q->type == 1) return bcs_setup(h, q);
     if (q->type != 2)
-        return fail(h, MOCCA_ERR_UNSUPPORTED, "pairing type %d (0 HF, 2 HFB)", q->type);
+        return fail(h, MOCCA_ERR_UNSUPPORTED, "pairing type %d (0 HF, 1 BCS, 2 HFB)", q->type);
     if (!h->mesh.time_reversal)
         return fail(h, MOCCA_ERR_UNSUPPORTED, "HFB needs time-reversal symmetry here");
     if (!h->scf.set) return fail(h, MOCCA_ERR_STATE, "mocca_scf_setup not called");

@@ -137,6 +137,10 @@ struct Est {
 // HFB pairing state (PairingType 2).
 struct Hfb {
     bool on = false, started = false, unconverged = false;
+    // BCS pairing (PairingType 1): per state the gap delta[i], the
+    // quasiparticle energy qpe[i] (Eqp), the cutoff factor pc[i]; the
+    // pairing field in field[2][N] (real); no HFB blocks
+    bool bcs = false;
     mocca_pairing in{};            // the parameters of the last mocca_set_pairing
     bool in_set = false;
     HfbK k{};
@@ -327,6 +331,12 @@ int density_any(mocca_solve *h);
 int kinetic_any(mocca_solve *h);
 // HFB pairing: setup from the C-ABI parameters, one SolvePairing step, the
 // pairing energy and Fermi energies into an energy row, the status check.
+// BCS pairing (PairingType 1): setup, one SolvePairing step, the energy
+// row's pairing part.  pairing_energy_impl: the pairing part of a row for
+// either (nothing for HF).
+int bcs_solve(mocca_solve *h);
+int bcs_energy_impl(mocca_solve *h, double *row);
+int pairing_energy_impl(mocca_solve *h, double *row);
 int hfb_setup(mocca_solve *h, const mocca_pairing *q);
 int hfb_free(mocca_solve *h);
 int hfb_solve_any(mocca_solve *h, bool first);

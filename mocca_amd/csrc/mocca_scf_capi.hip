@@ -279,6 +279,7 @@ int energy_kinetic_impl(mocca_solve *h, double *row) {
 
 int fill_impl(mocca_solve *h) {
     Scf &s = h->scf;
+    if (h->hfb.bcs) return bcs_solve(h);
     if (h->hfb.on) {
         const bool first = !h->hfb.started;
         h->hfb.started = true;
@@ -421,6 +422,7 @@ int iter_step(mocca_solve *h, int k, int iter, double &dt, double &dth, double &
     if ((rc = construct_impl(h))) return rc;
     if ((rc = readjust_impl(h, iter, momentum))) return rc;
     if ((rc = energy_terms_impl(h, row))) return rc;
+    if (h->hfb.bcs && (rc = bcs_energy_impl(h, row))) return rc;
     if (!deferred) {
         // HFB: the density basis is the canonical one, whose kinetic sums
         // the GradDesc pass does not give
@@ -485,7 +487,7 @@ void ptr_state(mocca_solve *h, void *(&k)[4]) {
 // that are never switched off (TurnOffConstraints reads the iteration
 // number).
 bool graph_ok(const mocca_solve *h) {
-    if (h->hfb.on || h->prof) return false;
+    if (h->hfb.on || h->hfb.bcs || h->prof) return false;
     for (size_t c = 0; c < h->scf.htype.size(); ++c)
         if (h->scf.htype[c] != 0 && h->scf.hiter[c] > 0) return false;
     return true;
@@ -880,7 +882,7 @@ int mocca_comp_energy(mocca_solve *h, double *e) {
     int rc;
     if ((rc = kinetic_any(h))) return rc;
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
-    if (h->hfb.on && (rc = hfb_energy_impl(h, h->scf.ecur))) return rc;
+    if ((rc = pairing_energy_impl(h, h->scf.ecur))) return rc;
     if ((rc = energy_kinetic_impl(h, h->scf.ecur))) return rc;
     double row[kRowLen];
     HIPCHK(h, hipStreamSynchronize(h->st));
@@ -928,7 +930,7 @@ int mocca_convergence_check(mocca_solve *h, const double *prec, int *out) {
     // Fermi energy is compared with FermiHistory(:,1:7), which the reference
     // declares zero and never updates
     bool fe = true;
-    if (h->hfb.on)
+    if (h->hfb.on || h->hfb.bcs)
         for (int it = 0; it < 2; ++it)
             if (std::fabs(s.efermi[it]) > prec[2]) fe = false;
     // ConverCranking (Cranking.f90:602-619): no cranking constraints
@@ -957,7 +959,7 @@ int mocca_final_iteration(mocca_solve *h, double *row) {
         if ((r = density_any(h))) return r;
         if ((r = kinetic_any(h))) return r;
         if ((r = energy_terms_impl(h, h->scf.ecur))) return r;
-        if (h->hfb.on && (r = hfb_energy_impl(h, h->scf.ecur))) return r;
+        if ((r = pairing_energy_impl(h, h->scf.ecur))) return r;
         return energy_kinetic_impl(h, h->scf.ecur);
     };
     rc = lagrange_pass();
@@ -987,6 +989,7 @@ int mocca_scf_start(mocca_solve *h) {
     if ((rc = save_history(h))) return rc;
     if ((rc = construct_impl(h, 3))) return rc;
     if ((rc = energy_terms_impl(h, h->scf.ecur))) return rc;
+    if (h->hfb.bcs && (rc = bcs_energy_impl(h, h->scf.ecur))) return rc;
     if (h->hfb.on) {
         if ((rc = hfb_energy_impl(h, h->scf.ecur))) return rc;
         if ((rc = kinetic_any(h))) return rc;
@@ -1369,10 +1372,10 @@ int mocca_get_pairing(mocca_solve *h, double *out, double *canonical_occ) {
     HIPCHK(h, hipStreamSynchronize(h->st));
     if (out) {
         std::vector<double> row(kRowLen, 0.0);
-        if (p.on) {
+        if (p.on || p.bcs) {
             double *tmp = nullptr;
             if (int rc = dalloc(h, &tmp, kRowLen * sizeof(double))) return rc;
-            int rc = hfb_energy_impl(h, tmp);
+            int rc = pairing_energy_impl(h, tmp);
             if (!rc) {
                 (void)hipMemcpyAsync(row.data(), tmp, kRowLen * sizeof(double),
                                      hipMemcpyDeviceToHost, h->st);

@@ -450,7 +450,7 @@ int mocca_v1_save(mocca_solve *h, const char *path, const mocca_v1_info *extra) 
                            s.sk.B10, s.sk.B11, s.sk.B12a, s.sk.B12b, s.sk.B13a, s.sk.B13b};
     std::copy(Bs, Bs + 20, I.B);   // B14 .. B21: zero (tensor terms are rejected)
     // pairing (:990-1002)
-    I.pairing_type = p.on ? 2 : 0;
+    I.pairing_type = p.on ? 2 : (p.bcs ? 1 : 0);
     if (p.in_set) {
         for (int it = 0; it < 2; ++it) {
             I.pairing_strength[it] = p.in.strength[it];
@@ -461,7 +461,7 @@ int mocca_v1_save(mocca_solve *h, const char *path, const mocca_v1_info *extra) 
         I.pairing_cut[0] = I.pairing_cut[1] = 5.0;   // the namelist default (CutNeutron)
     }
     std::vector<double> cut;
-    if (p.on) {
+    if (p.on || p.bcs) {
         // PCutoffs as SolvePairing leaves them: ComputePairingCutoffs with
         // the Fermi energies it found (Pairing.f90:849,
         // PairingInteraction.f90:182-199, SymmetricFermi)
@@ -517,6 +517,22 @@ int mocca_v1_save(mocca_solve *h, const char *path, const mocca_v1_info *extra) 
             for (int a = 0; a < bk.h; ++a) c[a] = (int)(2 * n) - a;
             for (size_t a = bk.h; a < n; ++a) c[a] = (int)a + 1;
         }
+        A.hfb_columns = cols.data();
+    } else if (p.bcs) {
+        // GuessHFBMatrices for PairingType 1 (HFB.f90:3719-3744): row i of
+        // the first half of a block paired with row i + size/2,
+        // kappa = +-sqrt(o (1 - o)), o = half the state's occupation
+        for (int P = 0; P < 2; ++P)
+            for (int it = 0; it < 2; ++it) {
+                const std::vector<int> &r = rb.rows[P][it];
+                const size_t half = r.size() / 2;
+                for (size_t a = 0; a < half; ++a) {
+                    const double o = 0.5 * occ[r[a] % M];
+                    const double kv = std::sqrt(o * (1.0 - o));
+                    kap[2 * rb.at(P, it, a, a + half)] = kv;
+                    kap[2 * rb.at(P, it, a + half, a)] = -kv;
+                }
+            }
         A.hfb_columns = cols.data();
     } else {
         // GuessHFBMatrices for PairingType 0 (HFB.f90:3686-3715): kappa =
@@ -662,6 +678,7 @@ int mocca_v1_load(mocca_solve *h, const char *path, int recalc, mocca_v1_info *i
         (rc = ul(h, h->b.D, D.data(), M * sizeof(double))) ||
         (rc = ul(h, h->b.nrm, nrm.data(), M * sizeof(double))))
         return rc;
+    if (p.bcs && (rc = ul(h, p.fermi, I.fermi, 2 * sizeof(double)))) return rc;
     if (p.on) {
         // Fermi energies and the HFB density matrices (Input :175-183 with
         // TransformHFBMatrices' copy for unchanged symmetries)

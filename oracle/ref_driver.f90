@@ -68,7 +68,8 @@ program ref_driver
   use Coulomb
   use Energy
   use PotentialMixing
-  use Pairing, only : SolvePairing, Fermi, PairingEnergy, PairingType, Delta, ConverFermi
+  use Pairing, only : SolvePairing, Fermi, PairingEnergy, PairingType, Delta, ConverFermi,   &
+  &                   PairingIter, FermiPrec
   use PairingInteraction, only : PairingStrength, Alpha, DDexp, PairingCut, PairingMu,   &
   &                              RhoSat, CutType, DensityFactor, PCutoffs
   use HFB, only : HFBMix, HFBIter, RhoHFB, KappaHFB, blocksizes, QuasiEnergies
@@ -143,6 +144,7 @@ program ref_driver
     write(uu) COM1Body
     write(uu) PairingType, CutType, HFBIter
     write(uu) PairingStrength, Alpha, DDexp, PairingCut, PairingMu, RhoSat, HFBMix
+    write(uu) dble(PairingIter), FermiPrec
     close(uu)
   endif
 
@@ -276,6 +278,15 @@ program ref_driver
       open(newunit=uh, file=trim(prefix)//'_hfbstart.bin', access='stream',    &
       &    form='unformatted', status='replace')
       write(uh) dble(Delta), dble(KappaHFB), dble(RhoHFB), DensityFactor, PCutoffs
+      close(uh)
+    endif
+    if(PairingType.eq.1) then
+      ! the start-up BCS step: gaps, cutoffs, quasiparticle energies,
+      ! occupations, Fermi energies
+      open(newunit=uh, file=trim(prefix)//'_bcsstart.bin', access='stream',    &
+      &    form='unformatted', status='replace')
+      write(uh) dble(Delta(:,1,1,1)), PCutoffs, (HFBasis(i)%GetEqp(), i=1,nwt),    &
+      &         (HFBasis(i)%GetOcc(), i=1,nwt), Fermi
       close(uh)
     endif
     call DeriveAll

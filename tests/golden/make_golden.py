@@ -353,6 +353,11 @@ SCF_CASES = {
     "scfhfb_trc_fd": (10, 0.8, True, False, 6, 4, 6, 10, 0,
                       dict(pairing='Type="HFB", PairingNeutron=1250, AlphaNeutron=1.0, '
                                    'GuessKappa=.true.', start="hf", v1=True)),
+    # BCS pairing (Type="BCS", PairingIter 50 by default, the sample's
+    # strength): the same nucleus and start data as scfhfb_trc_fd
+    "scfbcs_trc_fd": (10, 0.8, True, False, 6, 4, 6, 10, 0,
+                      dict(pairing='Type="BCS", PairingNeutron=1250, AlphaNeutron=1.0',
+                           start="hf", v1=True)),
     # The sample input of the reference's README (24Mg): 12 + 12 nucleons,
     # 20 + 20 states on the 10^3 octant of a 20^3 mesh at dx = 0.8 fm,
     # Lagrange derivatives, ParameterEstimation 1, MixingScheme 2, HFB with
@@ -501,9 +506,14 @@ def run_scf(name):
     com1 = int(np.frombuffer(fb[37 * 8:37 * 8 + 4], "<i4")[0])
     ptype, cuttype, hfbiter = (int(x) for x in np.frombuffer(fb[37 * 8 + 4:37 * 8 + 16], "<i4"))
     pp = np.frombuffer(fb[37 * 8 + 16:], "<f8")
+    if ptype == 1:
+        raw = np.fromfile(os.path.join(wd, "out_bcsstart.bin"), "<f8")
+        extra["bcs_start"] = raw[:4 * nwt].reshape(4, nwt)   # Delta, PCutoffs, Eqp, Occ
+        extra["bcs_start_fermi"] = raw[4 * nwt:]
     if ptype:
         extra["pairing"] = np.fromfile(os.path.join(wd, "out_pairing.bin"),
                                        "<f8").reshape(niter + 1, 4)
+    if ptype == 2:
         raw = np.fromfile(os.path.join(wd, "out_hfbstart.bin"), "<f8")
         S = 2 * nwt if trc else nwt
         k = 0
@@ -521,7 +531,8 @@ def run_scf(name):
                         f"scheme {mixing}, PreConW {preconw}"),
                 mixing_scheme=mixing, preconw=preconw, parameter_estimation=pe,
                 constraints=list(constraints), pairing_type=ptype, cut_type=cuttype,
-                hfb_iter=hfbiter, pairing_strength=list(pp[0:2]), pairing_alpha=list(pp[2:4]),
+                hfb_iter=hfbiter, pairing_iter=int(pp[12]), fermi_prec=float(pp[13]),
+                pairing_strength=list(pp[0:2]), pairing_alpha=list(pp[2:4]),
                 pairing_ddexp=list(pp[4:6]), pairing_cut=list(pp[6:8]),
                 pairing_mu=list(pp[8:10]), rhosat=float(pp[10]), hfb_mix=float(pp[11]),
                 source="reference MOCCa modules via oracle/ref_driver.f90 (amdflang -O2)")

@@ -646,12 +646,16 @@ def scf_run(g, niter, on_iteration=None):
     scheme = m.get("mixing_scheme", 0)
     preconw = m.get("preconw", 0.0)
     hfb = HFB(qn, trc, m) if m.get("pairing_type", 0) == 2 else None
+    bcs = BCS(qn, m) if m.get("pairing_type", 0) == 1 else None
+    pair = hfb if hfb is not None else bcs
     e_start = np.asarray(g["start_energies"]) if "start_energies" in g else np.zeros(M)
     pairs = []
 
     def pairing_step(psi, e_sp, rho):
-        """SolvePairing: HF filling, or HFB with the canonical basis as the
-        density basis (Pairing.f90:812)."""
+        """SolvePairing: HF filling, BCS in the HF basis, or HFB with the
+        canonical basis as the density basis (Pairing.f90:812)."""
+        if bcs is not None:
+            return psi, bcs.solve(psi.reshape(M, 4, N), e_sp, rho, dv)
         if hfb is None:
             return psi, hf_fill(e_sp, qn, m["neutrons"], m["protons"], trc)
         can, occ_c = hfb.solve(psi.reshape(M, 4, N), e_sp, rho, dv)
@@ -659,17 +663,18 @@ def scf_run(g, niter, on_iteration=None):
 
     if hfb is not None:
         hfb.guess()
-        if "start_fermi" in g:
-            hfb.fermi = np.array(g["start_fermi"], dtype=float)
+    if pair is not None and "start_fermi" in g:
+        pair.fermi = np.array(g["start_fermi"], dtype=float)
     dbasis, occ = pairing_step(psi, e_start, np.zeros((2, N)))
     d, der, lap = all_densities(mesh, dbasis, qn, occ, trc)
     F, vc, vx = construct(d, calc=3)      # CalculateAllMoments three times (Main.f90:213-215)
     E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
                kinetic_sums(dbasis, lap, qn, occ, dv))
-    if hfb is not None:
-        ep = hfb.energy()
+    if pair is not None:
+        ep = pair.energy()
         E[0] += ep.sum()
-        pairs.append(np.concatenate([hfb.fermi, ep]))
+        pairs.append(np.concatenate([pair.fermi, ep]))
+    if hfb is not None:
         der, lap = mesh.derive_states(psi, qn)
     traj = [rows(E)]
     ehist = np.zeros(8)
@@ -678,8 +683,8 @@ def scf_run(g, niter, on_iteration=None):
     def check(E):
         nonlocal ehist
         ehist = np.concatenate([[E[0]], ehist[:7]])    # CompEnergy's OldEnergy shift
-        fermi = hfb.fermi if hfb is not None else np.zeros(2)
-        conv.append(conv_flags(ehist, fermi, hfb is not None, cons))
+        fermi = pair.fermi if pair is not None else np.zeros(2)
+        conv.append(conv_flags(ehist, fermi, pair is not None, cons))
 
     check(E)
     if on_iteration is not None:
@@ -713,6 +718,8 @@ def scf_run(g, niter, on_iteration=None):
             if hfb is not None:
                 pos = np.concatenate([q[q > 0.0] for q in hfb.qpe.values()])
                 rel = min(10000.0, pos.min()) if pos.size else 10000.0
+            elif bcs is not None:
+                rel = min(10000.0, bcs.eqp.min())     # ImaginaryTime.f90 case(1)
             r = estimate(mesh, est_psi, k == 0, qn_est, trc, kin2[it_est], Ff, dt, mom, dv,
                          hbar, qn, occ, e_sp, rel)
             dt, mom = r[0], r[1]
@@ -742,10 +749,11 @@ def scf_run(g, niter, on_iteration=None):
             precondition_potentials(mesh, F, Fold, trc, dv, preconw)
         E = energy(coef, consts, m["com1body"], trc, mesh, d, vc,
                    kinetic_sums(dbasis, lap, qn, occ, dv))
-        if hfb is not None:
-            ep = hfb.energy()
+        if pair is not None:
+            ep = pair.energy()
             E[0] += ep.sum()
-            pairs.append(np.concatenate([hfb.fermi, ep]))
+            pairs.append(np.concatenate([pair.fermi, ep]))
+        if hfb is not None:
             der, lap = mesh.derive_states(psi, qn)
         traj.append(rows(E))
         check(E)
@@ -754,7 +762,7 @@ def scf_run(g, niter, on_iteration=None):
     return dict(traj=np.array(traj), psi=psi, energies=e_sp, occ=occ, fields=F, dens=d,
                 conv=np.array(conv, np.int32), dbasis=dbasis, mesh=mesh,
                 vcoul=vc, vcoulx=vx, cpot=coul.cpot, params=np.array(params),
-                pairing=np.array(pairs), hfb=hfb,
+                pairing=np.array(pairs), hfb=hfb, bcs=bcs,
                 constraints=[] if cons is None else
                 [dict(type=c["type"], con=c["con"].copy(), inten=c["inten"].copy(),
                       sq=c["sq"].copy()) for c in cons.items])
@@ -1287,3 +1295,71 @@ def v1_hf_kappa_guess(qn, trc):
                 K[it - 1, P - 1, jj, ii] = 0.1      # KappaHFB(i, j) = Occ
                 K[it - 1, P - 1, ii, jj] = -0.1     # KappaHFB(j, i) = -Occ
     return K
+
+
+class BCS:
+    """BCS pairing (PairingType 1): the loop of SolvePairing (Pairing.f90:
+    826-858) over BCSPairingField, BCSGaps, BCSFindFermiEnergy,
+    BCSQPEnergies and BCSOccupations (BCS.f90:16-247), per state i with
+    cutoff factor c_i, gap Delta_i, quasiparticle energy E_i.  The pair
+    density of a state with its own time reverse is -rho_i (GetPairDensity,
+    PairingInteraction.f90:150-178), so everything is real."""
+
+    def __init__(self, qn, m):
+        self.qn = np.asarray(qn)
+        self.nwt = len(qn)
+        self.it = (self.qn[:, 0] + 3) // 2 - 1
+        self.G = np.asarray(m["pairing_strength"], float)
+        self.alpha = np.asarray(m["pairing_alpha"], float)
+        self.cut = np.asarray(m["pairing_cut"], float)
+        self.mu = np.asarray(m["pairing_mu"], float)
+        self.rhosat = float(m["rhosat"])
+        self.prec = float(m.get("fermi_prec", 1e-12))
+        self.iters = int(m.get("pairing_iter", 50))
+        self.particles = np.array([m["neutrons"], m["protons"]], float)
+        self.delta = np.zeros(self.nwt)
+        self.eqp = np.ones(self.nwt)           # Spwf's Eqp default
+        self.fermi = np.array([-10.0, -10.0])
+        self.pc = np.zeros(self.nwt)
+        self.occ = np.zeros(self.nwt)
+
+    cutoffs = HFB.cutoffs
+    density_factor = HFB.density_factor
+
+    def solve(self, psi, energy, rho, dv):
+        """SolvePairing for PairingType 1; the occupations of the HF basis."""
+        energy = np.asarray(energy, float)
+        if np.all(np.abs(self.delta) < 1e-5):   # SolvePairing :812
+            self.delta[:] = 1.0
+        df = self.density_factor(rho)
+        r = np.einsum("icp,icp->ip", psi, psi)   # GetDensity of each state
+        N = r.shape[1]
+        for _ in range(self.iters):
+            pc = self.cutoffs(energy, self.fermi)
+            field = np.zeros((2, N))
+            for i in range(self.nwt):
+                field[self.it[i]] += 0.5 * pc[i] ** 2 * self.delta[i] / self.eqp[i] * (-r[i])
+            field *= df
+            for i in range(self.nwt):
+                self.delta[i] = pc[i] ** 2 * dv * np.sum(r[i] * field[self.it[i]])
+            old = self.fermi.copy()
+            lam = old[self.it]
+            self.eqp = np.sqrt((energy - lam) ** 2 + self.delta ** 2)
+            for t in range(2):
+                sel = self.it == t
+                s1 = np.sum(1.0 - energy[sel] / self.eqp[sel])
+                s2 = np.sum(1.0 / self.eqp[sel])
+                self.fermi[t] = (self.particles[t] - s1) / s2
+            self.pc = self.cutoffs(energy, self.fermi)
+            lam = self.fermi[self.it]
+            self.eqp = np.sqrt((energy - lam) ** 2 + self.delta ** 2)
+            self.occ = np.maximum(0.0, 1.0 - (energy - lam) / self.eqp)
+            if np.all(np.abs(self.fermi - old) < self.prec):
+                break
+        return self.occ.copy()
+
+    def energy(self):
+        e = np.zeros(2)
+        for i in range(self.nwt):
+            e[self.it[i]] += self.delta[i] ** 2 / self.eqp[i]
+        return -0.5 * e

@@ -21,10 +21,11 @@ import pytest
 import mocca_amd
 import oracle_lib as ol
 from oracle_lib import golden, relerr
+from test_gpu_bcs import _bcs
 from test_gpu_scf import _constraints, _functional, _pairing, _solve, _traj_cols
 
 pytestmark = pytest.mark.gpu
-V1_CASES = ["scf_trc_fd", "scfcon_todd_lag", "scfhfb_trc_fd"]
+V1_CASES = ["scf_trc_fd", "scfcon_todd_lag", "scfhfb_trc_fd", "scfbcs_trc_fd"]
 
 
 def _run(g):
@@ -32,6 +33,8 @@ def _run(g):
     s = _solve(g, np.zeros(m["nwt"]))
     if m["pairing_type"] == 2:
         _pairing(s, g)
+    elif m["pairing_type"] == 1:
+        _bcs(s, g)
     s.scf_start()
     s.iterate(m["niter"], m["dt"] / g["consts"][5], m["momentum"], fetch=False)
     return s
@@ -46,7 +49,7 @@ def _save(s, ref, path):
 def test_save_matches_the_reference_checkpoint(case, tmp_path):
     g = golden(case)
     m = g["meta"]
-    hfb = m["pairing_type"] == 2
+    hfb = m["pairing_type"] != 0      # HFB or BCS
     print(f"v1 save: case {case}", flush=True)
     ref = ol.v1_parse(g["v1_file"].tobytes())
     s = _run(g)
@@ -99,7 +102,6 @@ def test_save_matches_the_reference_checkpoint(case, tmp_path):
 def test_restart_from_a_checkpoint_vs_reference(case, source, tmp_path):
     g = golden(case)
     m = g["meta"]
-    hfb = m["pairing_type"] == 2
     print(f"v1 restart: case {case} from the {source}'s file", flush=True)
     path = tmp_path / "restart.v1"
     if source == "reference":
@@ -114,10 +116,15 @@ def test_restart_from_a_checkpoint_vs_reference(case, source, tmp_path):
     _functional(s, g)
     if m.get("constraints"):
         s.set_constraints(_constraints(m))
-    if hfb:
+    if m["pairing_type"] == 2:
         s.set_pairing(type=2, strength=m["pairing_strength"], alpha=m["pairing_alpha"],
                       cut=m["pairing_cut"], mu=m["pairing_mu"], rhosat=m["rhosat"],
                       hfb_mix=m["hfb_mix"], iterations=m["hfb_iter"], guess_kappa=False)
+    elif m["pairing_type"] == 1:
+        s.set_pairing(type=1, strength=m["pairing_strength"], alpha=m["pairing_alpha"],
+                      cut=m["pairing_cut"], mu=m["pairing_mu"], rhosat=m["rhosat"],
+                      iterations=m["pairing_iter"], fermi_prec=m["fermi_prec"],
+                      guess_kappa=False)
     info = s.load_v1(str(path))
     assert info["nwt"] == m["nwt"] and info["pairing_type"] == m["pairing_type"]
     # what the handle took from the file, written back before any step
@@ -127,7 +134,7 @@ def test_restart_from_a_checkpoint_vs_reference(case, source, tmp_path):
     for k in ("grid", "state_qn", "state_energy", "state_dispersion", "state_occupation",
               "state_norm", "fermi"):
         assert np.array_equal(back[k], src[k]), k
-    if hfb:
+    if m["pairing_type"] == 2:
         assert np.array_equal(back["rho_hfb"], src["rho_hfb"])
         assert np.array_equal(back["kappa_hfb"], src["kappa_hfb"])
     for a, b in zip(back["dens_rec"][:3], src["dens_rec"][:3]):
@@ -140,7 +147,7 @@ def test_restart_from_a_checkpoint_vs_reference(case, source, tmp_path):
     print(f"  energies {traj[:, 0]} (reference {ref[:, 0]})", flush=True)
     assert np.max(np.abs(traj[:, 0] - ref[:, 0])) < 1e-9
     assert relerr(traj, ref) < 1e-10
-    if hfb:
+    if m["pairing_type"] != 0:
         pr = g["v1_restart_pairing"]                   # Fermi(2), E_pair(2)
         assert np.max(np.abs(rows[:, 45:47] - pr[:, :2])) < 1e-9
         assert np.max(np.abs(rows[:, 43:45] - pr[:, 2:])) < 1e-9

@@ -427,3 +427,28 @@ def test_scf_hfb_blocks_of_100_states_vs_reference():
     assert relerr(r["traj"], g["traj"]) < 1e-10
     assert np.max(np.abs(r["pairing"] - g["pairing"])) < 1e-8       # Fermi, E_pair
     assert np.all(np.abs(g["pairing"][:, 2:]) > 0.1)                 # pairing stays on
+
+
+def test_scf_bcs_vs_reference():
+    """BCS pairing (PairingType 1, PairingIter 50, FermiPrec 1e-12): the
+    start-up step's gaps, cutoff factors, quasiparticle energies,
+    occupations and Fermi energies, then 10 iterations against the
+    reference run."""
+    g = golden("scfbcs_trc_fd")
+    m = g["meta"]
+    M, N = m["nwt"], m["nx"] ** 3
+    b = ol.BCS(g["qn"], m)
+    b.fermi = np.array(g["start_fermi"], float)
+    occ = b.solve(g["psi"].reshape(M, 4, N), g["start_energies"], np.zeros((2, N)),
+                  g["consts"][6])
+    ref = g["bcs_start"]
+    for mine, theirs in ((b.delta, ref[0]), (b.pc, ref[1]), (b.eqp, ref[2]), (occ, ref[3])):
+        assert np.max(np.abs(mine - theirs)) < 1e-12 * max(1.0, np.abs(theirs).max())
+    assert np.max(np.abs(b.fermi - g["bcs_start_fermi"])) < 1e-12
+    r = ol.scf_run(g, m["niter"])
+    assert np.max(np.abs(r["traj"][:, 0] - g["traj"][:, 0])) < 1e-10
+    assert relerr(r["traj"], g["traj"]) < 1e-12
+    assert np.max(np.abs(r["pairing"] - g["pairing"])) < 1e-11       # Fermi, E_pair
+    assert relerr(r["psi"].reshape(-1), g["final_psi"].reshape(-1)) < 1e-12
+    assert np.max(np.abs(r["occ"] - g["final_eo"][:, 1])) < 1e-12
+    assert np.all(g["pairing"][:, 2:] < 0.0)                         # pairing on

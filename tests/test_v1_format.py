@@ -3,7 +3,8 @@ src/InOut.f90:905-1416) on the host.
 
 The fixtures hold files the reference itself wrote (oracle/ref_driver.f90,
 task wv1, after an HF run with time reversal, an HF run with time-odd
-fields, a Lagrange mesh and two constrained moments, and an HFB run).
+fields, a Lagrange mesh and two constrained moments, an HFB run and a BCS
+run).
 
 - The oracle's record-by-record restatement parses each file and rebuilds
   it byte for byte, and its contents equal what the same reference run
@@ -23,7 +24,7 @@ import mocca_amd
 import oracle_lib as ol
 from oracle_lib import golden
 
-V1_CASES = ["scf_trc_fd", "scfcon_todd_lag", "scfhfb_trc_fd"]
+V1_CASES = ["scf_trc_fd", "scfcon_todd_lag", "scfhfb_trc_fd", "scfbcs_trc_fd"]
 
 
 @pytest.mark.parametrize("case", V1_CASES)
@@ -83,6 +84,21 @@ def test_oracle_parses_reference_files_and_rebuilds_them(case):
             assert np.abs(rb - rb.T).max() < 1e-14 and np.abs(kb + kb.T).max() < 1e-14
             tr[it - 1] += np.trace(rb)
         assert np.allclose(tr, [m["neutrons"], m["protons"]], rtol=0, atol=1e-6)
+    elif m["pairing_type"] == 1:
+        # GuessHFBMatrices(1) (HFB.f90:3719-3744): row i of the first half of
+        # a block with row i + size/2, kappa = +-sqrt(o (1 - o)), o = half
+        # the occupation; the rest zero
+        assert int(f["pairing_type"][0]) == 1 and f["cutoffs"].size == M
+        assert np.array_equal(f["fermi"], g["pairing"][-1, :2])
+        K = np.zeros((4, S, S), np.complex128)
+        for (P, it), r in rows.items():
+            half = len(r) // 2
+            for a in range(half):
+                o = 0.5 * f["state_occupation"][r[a] % M]
+                K[(it - 1) * 2 + P - 1, a + half, a] = np.sqrt(o * (1.0 - o))
+                K[(it - 1) * 2 + P - 1, a, a + half] = -np.sqrt(o * (1.0 - o))
+        assert np.array_equal(f["kappa_hfb"], K)
+        assert not f["rho_hfb"].any() and not f["U"].any() and not f["hfb_columns"].any()
     else:
         assert int(f["pairing_type"][0]) == 0 and f["cutoffs"].size == 0
         assert list(f["fermi"]) == [-10.0, -10.0]
