@@ -620,28 +620,52 @@ __global__ __launch_bounds__(256) void k_lj_gemm(const double *__restrict__ A,
     if (i0 + ty < n && j0 + tx < n) C[b * mstride + (long)(i0 + ty) * ld + j0 + tx] = acc;
 }
 
+// Four consecutive values (16- or 32-byte aligned) by vector loads.
+__device__ __forceinline__ void ld4(const double *p, double (&v)[4]) {
+    const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void ld4(const float *p, float (&v)[4]) {
+    const float4 a = *(const float4 *)p;
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+
 // ---------------------------------------------------------------------------
 // Gaps: partial sums S[a][a'] = sum_K psi_a[K] Ytilde_a'[K] over a K slice
-// per wave, all tiles of row tiles 0..NI-1 and column tiles
-// NJ blockIdx.z .. +NJ-1 of a block.  K is walked four points at a time:
-// lane group g = lane / 16 holds component g of points p0 .. p0+3, and the
-// transformed partner state is, with F = Fr + i Fi the pairing field,
+// per wave, all tiles of row tiles 0..NI-1 (NI = the block's tile count)
+// against the column tiles J0 = Z NJ .. J0+NJ-1 (group Z = blockIdx.z).  K
+// is walked four points at a time: lane group g = lane / 16 holds component
+// g of points p0 .. p0+3 of the state 16 I + lane % 16, and the transformed
+// partner state is, with F = Fr + i Fi the pairing field,
 //   Ytilde_g = -Fr y_g + Fi y_{g+1} (g even),  -Fr y_g - Fi y_{g-1} (g odd),
 // so that sum_g x_g Ytilde_g = Re P(x, y) Fr + Im P(x, y) Fi for the
 // time-reversal pair amplitude P (GetPairDensity, PairingInteraction.f90:
-// 101-180).  Result lane L, register r: S[16 I + L/16 + 4 r][16 J + L%16].
-template <typename T, int NI, int NJ>
-__global__ __launch_bounds__(64) void k_hl_pair_gram(HfbK k, const T *__restrict__ psi, long N,
-                                                     const double *__restrict__ field,
-                                                     double *__restrict__ part, int nsplit,
-                                                     int nt) {
-    const int w = blockIdx.x, b = blockIdx.y, J0 = blockIdx.z * NJ, lane = threadIdx.x;
+// 101-180).  The partner tiles are row tiles of the same block: component g
+// comes from the rows already in registers, component g^1 from a second
+// read of the partner rows.  Chunk c+1 is loaded in the storage type,
+// without branches (points past N and states past h read point / state 0),
+// while the MFMAs of chunk c run, and is widened to FP64 with those entries
+// zeroed afterwards; with N % 4 == 0 (V4) a lane's four points are one
+// aligned vector load.  The field is read by vector loads of one address per
+// wave: scalar loads count on the wait counter of LDS traffic, which the
+// compiler then waits on before the chunk's first MFMA.  Result lane L,
+// register r: S[16 I + L/16 + 4 r][16 J + L%16].
+template <typename T, int NI, int NJ, int Z, bool V4>
+__device__ __forceinline__ void pair_gram_group(const HfbK &k, const T *__restrict__ psi, long N,
+                                                const double *__restrict__ field,
+                                                double *__restrict__ part, int nsplit) {
+    constexpr int J0 = Z * NJ;
+    constexpr int NJE = NI - J0 < NJ ? NI - J0 : NJ;   // column tiles of this group
+    const int w = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
     const HfbBlockK &bk = k.blk[b];
     const int h = bk.h, g = lane / 16;
-    const double *Fr = field + (2 * bk.it) * N, *Fi = Fr + N;
+    // a zero the compiler cannot prove uniform keeps the field reads vector
+    int vz;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+    const double *Fr = field + (2 * bk.it) * N + vz, *Fi = Fr + N;
     const T *xr[NI];
-    const T *yr[NJ];
-    bool xok[NI], yok[NJ];
+    const T *yr[NJE];
+    bool xok[NI];
 #pragma unroll
     for (int I = 0; I < NI; ++I) {
         const int a = 16 * I + lane % 16;
@@ -649,66 +673,111 @@ __global__ __launch_bounds__(64) void k_hl_pair_gram(HfbK k, const T *__restrict
         xr[I] = psi + (long)k.state[bk.first + (xok[I] ? a : 0)] * 4 * N + (long)g * N;
     }
 #pragma unroll
-    for (int J = 0; J < NJ; ++J) {
+    for (int J = 0; J < NJE; ++J) {
         const int a = 16 * (J0 + J) + lane % 16;
-        yok[J] = a < h && J0 + J < nt;
-        yr[J] = psi + (long)k.state[bk.first + (yok[J] ? a : 0)] * 4 * N;
+        yr[J] = psi + (long)k.state[bk.first + (a < h ? a : 0)] * 4 * N + (long)(g ^ 1) * N;
     }
-    const int gp = g ^ 1;
     const double sg = (g & 1) ? -1.0 : 1.0;
-    hl_d4 acc[NI][NJ];
+    hl_d4 acc[NI][NJE];
 #pragma unroll
     for (int I = 0; I < NI; ++I)
 #pragma unroll
-        for (int J = 0; J < NJ; ++J) acc[I][J] = hl_d4{0, 0, 0, 0};
+        for (int J = 0; J < NJE; ++J) acc[I][J] = hl_d4{0, 0, 0, 0};
     const long nch = (N + 3) / 4;
     const long c0 = nch * w / nsplit, c1 = nch * (w + 1) / nsplit;
-    for (long c = c0; c < c1; ++c) {
-        const long p0 = 4 * c;
-        double fr[4], fi[4];
-        bool pin[4];
+    T raw[NI][4], rawy[NJE][4];
+    double fraw[2][4];
+    auto load = [&](long c) {
+        if constexpr (V4) {
+            // N % 4 == 0: a chunk is in range as a whole, 4 aligned values
+            const long p = 4 * c < N ? 4 * c : 0;
+            ld4(Fr + p, fraw[0]);
+            ld4(Fi + p, fraw[1]);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            pin[q] = p0 + q < N;
-            const long p = pin[q] ? p0 + q : 0;
-            fr[q] = Fr[p];
-            fi[q] = Fi[p];
-        }
-        double x[NI][4], yt[NJ][4];
+            for (int I = 0; I < NI; ++I) ld4(xr[I] + p, raw[I]);
 #pragma unroll
-        for (int I = 0; I < NI; ++I)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                x[I][q] = (xok[I] && pin[q]) ? (double)xr[I][pin[q] ? p0 + q : 0] : 0.0;
-#pragma unroll
-        for (int J = 0; J < NJ; ++J)
+            for (int J = 0; J < NJE; ++J) ld4(yr[J] + p, rawy[J]);
+        } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const long p = pin[q] ? p0 + q : 0;
-                const double y0 = (double)yr[J][(long)g * N + p];
-                const double y1 = (double)yr[J][(long)gp * N + p];
-                yt[J][q] = (yok[J] && pin[q]) ? (-fr[q] * y0 + sg * fi[q] * y1) : 0.0;
+                const long p = 4 * c + q < N ? 4 * c + q : 0;
+                fraw[0][q] = Fr[p];
+                fraw[1][q] = Fi[p];
+#pragma unroll
+                for (int I = 0; I < NI; ++I) raw[I][q] = xr[I][p];
+#pragma unroll
+                for (int J = 0; J < NJE; ++J) rawy[J][q] = yr[J][p];
             }
+        }
+    };
+    double x[NI][4], y1[NJE][4], f[2][4];
+    auto widen = [&](long c) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool pin = 4 * c + q < N;
+            f[0][q] = fraw[0][q];
+            f[1][q] = fraw[1][q];
+#pragma unroll
+            for (int I = 0; I < NI; ++I) x[I][q] = (pin && xok[I]) ? (double)raw[I][q] : 0.0;
+#pragma unroll
+            for (int J = 0; J < NJE; ++J)
+                y1[J][q] = (pin && xok[J0 + J]) ? (double)rawy[J][q] : 0.0;
+        }
+    };
+    if (c0 < c1) {
+        load(c0);
+        widen(c0);
+    }
+    for (long c = c0; c < c1; ++c) {
+        if (c + 1 < c1) load(c + 1);
+        // keep the widening of chunk c+1 behind the MFMAs of chunk c (the
+        // scheduler would otherwise pull it up and wait for the loads there)
+        __builtin_amdgcn_sched_barrier(0);
+        double yt[NJE][4];
+#pragma unroll
+        for (int J = 0; J < NJE; ++J)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                yt[J][q] = -f[0][q] * x[J0 + J][q] + sg * f[1][q] * y1[J][q];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int I = 0; I < NI; ++I)
 #pragma unroll
-                for (int J = 0; J < NJ; ++J)
+                for (int J = 0; J < NJE; ++J)
                     acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[I][q], yt[J][q],
                                                                      acc[I][J], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        widen(c + 1);   // after the last chunk: unused
     }
-    double *P = part + ((long)b * nsplit + w) * nt * nt * 256 + lane * 4;
+    double *P = part + ((long)b * nsplit + w) * NI * NI * 256 + lane * 4;
 #pragma unroll
     for (int I = 0; I < NI; ++I)
 #pragma unroll
-        for (int J = 0; J < NJ; ++J) {
-            if (J0 + J >= nt) continue;
+        for (int J = 0; J < NJE; ++J)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) P[(I * nt + J0 + J) * 256 + r] = acc[I][J][r];
-        }
+            for (int r = 0; r < 4; ++r) P[(I * NI + J0 + J) * 256 + r] = acc[I][J][r];
 }
 
+// One wave per (K slice, block, column-tile group); the group index selects
+// a compile-time tile range, so every register array is indexed statically.
+template <typename T, int NI, int NJ, bool V4>
+__global__ __launch_bounds__(64) void k_hl_pair_gram(HfbK k, const T *__restrict__ psi, long N,
+                                                     const double *__restrict__ field,
+                                                     double *__restrict__ part, int nsplit) {
+    constexpr int NZ = (NI + NJ - 1) / NJ;
+    static_assert(NZ <= 3, "at most three column-tile groups");
+    const int z = blockIdx.z;
+    if (z == 0) {
+        pair_gram_group<T, NI, NJ, 0, V4>(k, psi, N, field, part, nsplit);
+    } else if constexpr (NZ > 1) {
+        if (z == 1) {
+            pair_gram_group<T, NI, NJ, 1, V4>(k, psi, N, field, part, nsplit);
+        } else if constexpr (NZ > 2) {
+            if (z == 2) pair_gram_group<T, NI, NJ, 2, V4>(k, psi, N, field, part, nsplit);
+        }
+    }
+}
 // Delta(a, h+a') = dv c_a c_a' sum_slices S, Delta(h+a', a) = -Delta(a, h+a'),
 // zero below the pair cut (HFBGaps :520-632): one thread per (a, a').
 __global__ void k_hl_gaps_reduce(HfbK k, const double *__restrict__ part, int nsplit, int nt,
@@ -1442,10 +1511,15 @@ namespace {
 template <typename T>
 void launch_pair_gram(const HfbK &k, const T *psi, long N, const double *field, double *part,
                       int nsplit, int nt, int nblk, hipStream_t st) {
+    auto grid = [&](int nj) { return dim3(nsplit, nblk, (nt + nj - 1) / nj); };
 #define MOCCA_PG(NI, NJ)                                                                    \
     case NI:                                                                                \
-        hipLaunchKernelGGL((k_hl_pair_gram<T, NI, NJ>), dim3(nsplit, nblk, (nt + NJ - 1) / NJ), \
-                           dim3(64), 0, st, k, psi, N, field, part, nsplit, nt);           \
+        if (N % 4 == 0)                                                                     \
+            hipLaunchKernelGGL((k_hl_pair_gram<T, NI, NJ, true>), grid(NJ), dim3(64), 0, st, k, \
+                               psi, N, field, part, nsplit);                                \
+        else                                                                                \
+            hipLaunchKernelGGL((k_hl_pair_gram<T, NI, NJ, false>), grid(NJ), dim3(64), 0, st, \
+                               k, psi, N, field, part, nsplit);                             \
         break;
     switch (nt) {
         MOCCA_PG(1, 1) MOCCA_PG(2, 2) MOCCA_PG(3, 3) MOCCA_PG(4, 4)

@@ -140,3 +140,41 @@ def test_blocks_of_100_states_vs_reference(prec, solver):
         assert de < 1e-8
     else:
         assert np.max(np.abs(traj[:, 0] - g["traj"][:, 0])) < 1e-3
+
+
+def test_large_block_path_on_a_mesh_of_odd_size():
+    """A mesh whose point count is not a multiple of 4 (10 x 9 x 9 = 810,
+    the reference's HFB fixture cut to it): the large-block path (its pair
+    Gram then reads the states point by point) against the in-LDS path,
+    whose gaps and pairing field are separate kernels pinned to the
+    reference, three iterations each: total energies 1e-10 relative,
+    Fermi and pairing energies 1e-9 MeV."""
+    from test_gpu_scf import _solve
+    g0 = golden("scfhfb_trc_fd")
+    m = dict(g0["meta"])
+    m["ny"], m["nz"] = 9, 9
+    g = dict(g0)
+    g["meta"] = m
+    g["psi"] = np.ascontiguousarray(g0["psi"][:, :, :9, :9, :])
+    print("HFB on a 10 x 9 x 9 mesh: in-LDS path vs large-block solvers 2 and 4", flush=True)
+    out = {}
+    for solver in (1, 2, 4):
+        s = _solve(g, np.zeros(m["nwt"]), coulomb=1)
+        s.set_energies(g0["start_energies"])
+        s.set_pairing(type=2, strength=m["pairing_strength"], alpha=m["pairing_alpha"],
+                      cut=m["pairing_cut"], mu=m["pairing_mu"], rhosat=m["rhosat"],
+                      hfb_mix=m["hfb_mix"], iterations=m["hfb_iter"],
+                      fermi=g0["start_fermi"], guess_kappa=True, solver=solver)
+        s.scf_start()
+        out[solver] = s.iterate(3, m["dt"] / g0["consts"][5], m["momentum"])
+        s.close()
+    ref = out[1]
+    assert np.all(np.isfinite(ref))
+    for solver in (2, 4):
+        rows = out[solver]
+        rel = np.max(np.abs(rows[:, 0] - ref[:, 0]) / np.abs(ref[:, 0]))
+        dp = np.max(np.abs(rows[:, 43:47] - ref[:, 43:47]))
+        print(f"  solver {solver}: relative energy difference {rel:.2e}, "
+              f"pairing/Fermi {dp:.2e} MeV", flush=True)
+        assert rel < 1e-10
+        assert dp < 1e-9
