@@ -7,9 +7,10 @@ a 1250 MeV fm^3 contact pairing and GuessKappa), in FP64.  One step is one
 whole Evolve iteration on the device:
     GradDesc  -- hPsi of every state, heavy-ball step, energy-ordered
                  Gram-Schmidt (Cholesky QR), heavy-ball memory update
-    pairing   -- pairing field, gaps, the Broyden Fermi solve over the
-                 diagonalised 200 x 200 HFB matrices, canonical basis and
-                 states, rho/kappa mixing
+    pairing   -- pairing field, gaps, the Broyden Fermi solve (each
+                 evaluation's rho, kappa and particle number from the sign
+                 function of the 200 x 200 HFB matrices), canonical basis
+                 and states, rho/kappa mixing
     densities, mixing, moments, Coulomb, potentials, energy.
 The mixed run (FP32 stencils/hPsi/densities, FP64 Gram-Schmidt, energy,
 Coulomb and pairing) starts from the same states; its total-energy
@@ -55,6 +56,21 @@ STAGES = ("lag_derivs", "derive_all", "hpsi_step", "hpsi", "reduce_ed", "gram", 
           "coulomb_bc", "coulomb_cg", "coulomb_direct", "potentials", "precondition",
           "pot_jobs", "pot_convert", "energy", "energy_kinetic", "kinetic", "cutoff",
           "est_start", "est_reset", "est_hpsi", "est_step", "est_finalize")
+
+# the main kernel of a stage and what bounds it (for the dominant stage)
+STAGE_KERNELS = {
+    "hfb_sign": ("mocca::k_hs_gemm<false>",
+                 "launch and memory latency: 20 dependent 200 x 200 x 200 FP64 matrix "
+                 "products per Broyden evaluation (about 6 us each, a few % of the FP64 "
+                 "matrix-core peak)"),
+    "hfb_jacobi": ("mocca::k_lj_jacobi_slots",
+                   "LDS bandwidth and barrier latency of one workgroup per 100 x 100 "
+                   "density block (4 blocks: 4 of 256 CUs)"),
+    "hfb_gaps": ("mocca::k_hl_pair_gram<double, 7, 4, true>", "FP64 matrix cores"),
+    "hfb_field": ("mocca::k_gs_apply_mfma<double, 7, 8, true>", "FP64 matrix cores"),
+    "hpsi_step": ("mocca::k_hpsi_fd_dma<double, true, true, false>",
+                  "on-chip issue (VALU, LDS, one barrier per plane)"),
+}
 
 # The headline nucleus: open shells in both isospins (146 neutrons, 94
 # protons), so that the pairing stays on through the run (a doubly magic
@@ -196,7 +212,7 @@ def headline(args):
     shares = sorted(((k, v["ms_per_step"] / tot) for k, v in prof.items()),
                     key=lambda kv: -kv[1])
     alg, achieved, t_k = hpsi_roofline(M, N, w["trc"], prof["hpsi_step"])
-    kname = "mocca::k_hpsi_fd_dma<double, true, true>"
+    kname = "mocca::k_hpsi_fd_dma<double, true, true, false>"
     jac = prof.get("hfb_jacobi", {})
     e64, e32 = r64["rows"][:, 0], r32["rows"][:, 0]
     out = dict(
@@ -210,18 +226,15 @@ def headline(args):
             "traffic_source": "profiles/" + HEADLINE_PROFILE + " (rocprofv3 --pmc FETCH_SIZE, "
                               "WRITE_SIZE of the same workload, copy-calibrated)",
             "note": "the largest HBM-bound kernel of the hPsi + densities + Gram-Schmidt "
-                    "path; the iteration's largest time share is the HFB eigen-solver "
-                    "(dominant_stage), which is bound by LDS on one CU per HFB matrix",
+                    "path; the iteration's largest time share is the HFB pairing update "
+                    "(dominant_stage), whose kernels are latency- or matrix-core-bound",
         },
         dominant_stage={
             "stage": shares[0][0], "share": shares[0][1],
-            "kernel": "mocca::k_lj_jacobi_slots" if shares[0][0] == "hfb_jacobi"
-                      else None,
+            "kernel": STAGE_KERNELS.get(shares[0][0], (None, None))[0],
             "ms_per_step": prof[shares[0][0]]["ms_per_step"],
             "launches_per_step": prof[shares[0][0]]["launches_per_step"],
-            "bound": "LDS bandwidth and barrier latency of one workgroup per 200 x 200 HFB "
-                     "matrix (4 matrices: 4 of 256 CUs)" if shares[0][0] == "hfb_jacobi"
-                     else None,
+            "bound": STAGE_KERNELS.get(shares[0][0], (None, None))[1],
         },
         stage_shares={k: round(v, 4) for k, v in shares},
         mixed={

@@ -633,22 +633,22 @@ __device__ __forceinline__ void ld4(const float *p, float (&v)[4]) {
 // ---------------------------------------------------------------------------
 // Gaps: partial sums S[a][a'] = sum_K psi_a[K] Ytilde_a'[K] over a K slice
 // per wave, all tiles of row tiles 0..NI-1 (NI = the block's tile count)
-// against the column tiles J0 = Z NJ .. J0+NJ-1 (group Z = blockIdx.z).  K
-// is walked four points at a time: lane group g = lane / 16 holds component
-// g of points p0 .. p0+3 of the state 16 I + lane % 16, and the transformed
-// partner state is, with F = Fr + i Fi the pairing field,
-//   Ytilde_g = -Fr y_g + Fi y_{g+1} (g even),  -Fr y_g - Fi y_{g-1} (g odd),
-// so that sum_g x_g Ytilde_g = Re P(x, y) Fr + Im P(x, y) Fi for the
+// against the column tiles J0 = Z NJ .. J0+NJ-1 (group Z = blockIdx.z).
+// With F = Fr + i Fi the pairing field, the transformed partner state is
+//   Ytilde_c = -Fr y_c + Fi y_{c+1} (c even),  -Fr y_c - Fi y_{c-1} (c odd),
+// so that sum_c x_c Ytilde_c = Re P(x, y) Fr + Im P(x, y) Fi for the
 // time-reversal pair amplitude P (GetPairDensity, PairingInteraction.f90:
-// 101-180).  The partner tiles are row tiles of the same block: component g
-// comes from the rows already in registers, component g^1 from a second
-// read of the partner rows.  Chunk c+1 is loaded in the storage type,
-// without branches (points past N and states past h read point / state 0),
-// while the MFMAs of chunk c run, and is widened to FP64 with those entries
-// zeroed afterwards; with N % 4 == 0 (V4) a lane's four points are one
-// aligned vector load.  The field is read by vector loads of one address per
-// wave: scalar loads count on the wait counter of LDS traffic, which the
-// compiler then waits on before the chunk's first MFMA.  Result lane L,
+// 101-180).  K is walked in sub-chunks: component c of 16 consecutive
+// points, lane group g = lane / 16 holding points 4g .. 4g+3 of the state
+// 16 I + lane % 16, so that one load instruction reads whole 128-byte lines
+// of a row (with one component per lane group, the rows' lines were
+// fetched from HBM several times over).  The partner tiles are row tiles
+// of the same block: component c comes from the rows already in registers,
+// component c^1 from a second read of the partner rows.  Sub-chunk s+1 is
+// loaded in the storage type, without branches (points past N and states
+// past h read point / state 0), while the MFMAs of sub-chunk s run, and is
+// widened to FP64 with those entries zeroed afterwards; with N % 4 == 0
+// (V4) a lane's four points are one aligned vector load.  Result lane L,
 // register r: S[16 I + L/16 + 4 r][16 J + L%16].
 template <typename T, int NI, int NJ, int Z, bool V4>
 __device__ __forceinline__ void pair_gram_group(const HfbK &k, const T *__restrict__ psi, long N,
@@ -659,10 +659,7 @@ __device__ __forceinline__ void pair_gram_group(const HfbK &k, const T *__restri
     const int w = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
     const HfbBlockK &bk = k.blk[b];
     const int h = bk.h, g = lane / 16;
-    // a zero the compiler cannot prove uniform keeps the field reads vector
-    int vz;
-    asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
-    const double *Fr = field + (2 * bk.it) * N + vz, *Fi = Fr + N;
+    const double *Fr = field + (2 * bk.it) * N, *Fi = Fr + N;
     const T *xr[NI];
     const T *yr[NJE];
     bool xok[NI];
@@ -670,51 +667,54 @@ __device__ __forceinline__ void pair_gram_group(const HfbK &k, const T *__restri
     for (int I = 0; I < NI; ++I) {
         const int a = 16 * I + lane % 16;
         xok[I] = a < h;
-        xr[I] = psi + (long)k.state[bk.first + (xok[I] ? a : 0)] * 4 * N + (long)g * N;
+        xr[I] = psi + (long)k.state[bk.first + (xok[I] ? a : 0)] * 4 * N;
     }
 #pragma unroll
     for (int J = 0; J < NJE; ++J) {
         const int a = 16 * (J0 + J) + lane % 16;
-        yr[J] = psi + (long)k.state[bk.first + (a < h ? a : 0)] * 4 * N + (long)(g ^ 1) * N;
+        yr[J] = psi + (long)k.state[bk.first + (a < h ? a : 0)] * 4 * N;
     }
-    const double sg = (g & 1) ? -1.0 : 1.0;
     hl_d4 acc[NI][NJE];
 #pragma unroll
     for (int I = 0; I < NI; ++I)
 #pragma unroll
         for (int J = 0; J < NJE; ++J) acc[I][J] = hl_d4{0, 0, 0, 0};
-    const long nch = (N + 3) / 4;
-    const long c0 = nch * w / nsplit, c1 = nch * (w + 1) / nsplit;
+    // sub-chunk s: component s % 4 of the 16 points of chunk s / 4
+    const long nch = (N + 15) / 16;
+    const long s0 = 4 * (nch * w / nsplit), s1 = 4 * (nch * (w + 1) / nsplit);
     T raw[NI][4], rawy[NJE][4];
     double fraw[2][4];
-    auto load = [&](long c) {
+    auto load = [&](long sc) {
+        const long pb = 16 * (sc >> 2) + 4 * g;
+        const long cx = (long)(sc & 3) * N, cy = (long)((sc & 3) ^ 1) * N;
         if constexpr (V4) {
-            // N % 4 == 0: a chunk is in range as a whole, 4 aligned values
-            const long p = 4 * c < N ? 4 * c : 0;
+            // N % 4 == 0: a lane's 4 points are in range together, aligned
+            const long p = pb < N ? pb : 0;
             ld4(Fr + p, fraw[0]);
             ld4(Fi + p, fraw[1]);
 #pragma unroll
-            for (int I = 0; I < NI; ++I) ld4(xr[I] + p, raw[I]);
+            for (int I = 0; I < NI; ++I) ld4(xr[I] + cx + p, raw[I]);
 #pragma unroll
-            for (int J = 0; J < NJE; ++J) ld4(yr[J] + p, rawy[J]);
+            for (int J = 0; J < NJE; ++J) ld4(yr[J] + cy + p, rawy[J]);
         } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const long p = 4 * c + q < N ? 4 * c + q : 0;
+                const long p = pb + q < N ? pb + q : 0;
                 fraw[0][q] = Fr[p];
                 fraw[1][q] = Fi[p];
 #pragma unroll
-                for (int I = 0; I < NI; ++I) raw[I][q] = xr[I][p];
+                for (int I = 0; I < NI; ++I) raw[I][q] = xr[I][cx + p];
 #pragma unroll
-                for (int J = 0; J < NJE; ++J) rawy[J][q] = yr[J][p];
+                for (int J = 0; J < NJE; ++J) rawy[J][q] = yr[J][cy + p];
             }
         }
     };
     double x[NI][4], y1[NJE][4], f[2][4];
-    auto widen = [&](long c) {
+    auto widen = [&](long sc) {
+        const long pb = 16 * (sc >> 2) + 4 * g;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const bool pin = 4 * c + q < N;
+            const bool pin = pb + q < N;
             f[0][q] = fraw[0][q];
             f[1][q] = fraw[1][q];
 #pragma unroll
@@ -724,15 +724,16 @@ __device__ __forceinline__ void pair_gram_group(const HfbK &k, const T *__restri
                 y1[J][q] = (pin && xok[J0 + J]) ? (double)rawy[J][q] : 0.0;
         }
     };
-    if (c0 < c1) {
-        load(c0);
-        widen(c0);
+    if (s0 < s1) {
+        load(s0);
+        widen(s0);
     }
-    for (long c = c0; c < c1; ++c) {
-        if (c + 1 < c1) load(c + 1);
-        // keep the widening of chunk c+1 behind the MFMAs of chunk c (the
+    for (long sc = s0; sc < s1; ++sc) {
+        if (sc + 1 < s1) load(sc + 1);
+        // keep the widening of sub-chunk sc+1 behind the MFMAs of sc (the
         // scheduler would otherwise pull it up and wait for the loads there)
         __builtin_amdgcn_sched_barrier(0);
+        const double sg = (sc & 1) ? -1.0 : 1.0;   // odd components take -Fi
         double yt[NJE][4];
 #pragma unroll
         for (int J = 0; J < NJE; ++J)
@@ -748,7 +749,7 @@ __device__ __forceinline__ void pair_gram_group(const HfbK &k, const T *__restri
                     acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[I][q], yt[J][q],
                                                                      acc[I][J], 0, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
-        widen(c + 1);   // after the last chunk: unused
+        widen(sc + 1);   // after the last sub-chunk: unused
     }
     double *P = part + ((long)b * nsplit + w) * NI * NI * 256 + lane * 4;
 #pragma unroll
@@ -1242,24 +1243,30 @@ __global__ __launch_bounds__(256) void k_hs_build(HfbK k, const double *__restri
     X[(long)b * ld * ld + (long)i * ld + j] = v * inv;
 }
 
+// Matrix sizes of the blocks, passed by value so that a kernel's first
+// loads do not wait for a table read.
+struct SignDims {
+    int n[4];
+};
+
 // C = c1 D + c2 A B for symmetric A, B whose product is symmetric; one
 // workgroup of four waves per upper 16 x 16 tile (I <= J) of a block,
 // mirrored into (J, I).  The waves split the k range in four (every operand
 // of a wave loaded before its first product) and add their partial tiles
-// through LDS.  With RESID each wave's largest |C - 1| goes into its slot
-// of sig[b][.] (reduced by k_hs_number).
+// through LDS.  The block's active flag is read while the operands load
+// (an inactive block's tiles are read and dropped).  With RESID each wave's
+// largest |C - 1| goes into its slot of sig[b][.] (reduced by k_hs_number).
 template <bool RESID>
 __global__ __launch_bounds__(256) void k_hs_gemm(const double *__restrict__ A,
                                                  const double *__restrict__ B,
                                                  const double *__restrict__ D, double c1,
                                                  double c2, double *__restrict__ C, int ld,
-                                                 long mstride, const int *__restrict__ nb,
+                                                 long mstride, SignDims dims,
                                                  const int *__restrict__ active,
                                                  double *__restrict__ sig) {
     __shared__ double part[4][4][64];   // [wave][register][lane]
     const int b = blockIdx.y, tid = threadIdx.x, wv = tid / 64, lane = tid % 64;
-    if (!active[b]) return;
-    const int n = nb[b], nt = (n + kSignTile - 1) / kSignTile;
+    const int n = dims.n[b], nt = (n + kSignTile - 1) / kSignTile;
     int t = blockIdx.x, I = 0;
     while (I < nt && t >= nt - I) {
         t -= nt - I;
@@ -1285,6 +1292,7 @@ __global__ __launch_bounds__(256) void k_hs_gemm(const double *__restrict__ A,
         av[q] = kin && ia ? Ab[row + i0 + lr] : 0.0;
         bv[q] = kin && jb ? Bb[row + j0 + lr] : 0.0;
     }
+    if (!active[b]) return;
 #pragma unroll
     for (int q = 0; q < kSignSteps; ++q) {
         if (kb + 4 * q >= ke) break;
@@ -1735,16 +1743,18 @@ void sign_steps(mocca_solve *h, const double *alphas, int nsteps) {
     const int nt = (p.nmax + kSignTile - 1) / kSignTile;
     const dim3 grid(nt * (nt + 1) / 2, nblk);
     double *X = p.lA, *Xn = p.lC, *Y = p.lW;
+    SignDims dims{};
+    for (int b = 0; b < nblk; ++b) dims.n[b] = 2 * p.k.blk[b].h;
     for (int q = 0; q < nsteps; ++q) {
         const double a = alphas[q];
         if (q == nsteps - 1)
             hipLaunchKernelGGL(k_hs_gemm<true>, grid, dim3(256), 0, h->st, X, X, nullptr, 0.0,
-                               1.0, Y, ld, ms, p.lnb, p.lactive, p.lsig);
+                               1.0, Y, ld, ms, dims, p.lactive, p.lsig);
         else
             hipLaunchKernelGGL(k_hs_gemm<false>, grid, dim3(256), 0, h->st, X, X, nullptr, 0.0,
-                               1.0, Y, ld, ms, p.lnb, p.lactive, p.lsig);
+                               1.0, Y, ld, ms, dims, p.lactive, p.lsig);
         hipLaunchKernelGGL(k_hs_gemm<false>, grid, dim3(256), 0, h->st, X, Y, X, 1.5 * a,
-                           -0.5 * a * a * a, Xn, ld, ms, p.lnb, p.lactive, p.lsig);
+                           -0.5 * a * a * a, Xn, ld, ms, dims, p.lactive, p.lsig);
         std::swap(X, Xn);
     }
 }
