@@ -1906,6 +1906,111 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
         }
 }
 
+// The same product for blocks of up to 128 states with the whole T (rounded
+// to the storage type) in LDS for the launch: after the one barrier of its
+// staging, every wave streams the states of its own 16 NE K values from HBM
+// (16 lanes = 16 consecutive K values of one state, the four lane groups
+// four states) and reads its A operands (T rows) from LDS; the next chunk of
+// 16 states is loaded while the MFMAs of this one run.  No barrier in the
+// loop, and no LDS traffic for the states.
+template <typename T>
+constexpr int apply_t_pitch(int nt) {
+    return nt * 16 + (sizeof(T) == 8 ? 2 : 4);
+}
+template <typename T, int NT>
+constexpr size_t apply_direct_lds() {
+    return sizeof(T) * (size_t)(NT * 16) * apply_t_pitch<T>(NT);
+}
+template <typename T, int NT, int NE, int NW, bool DENSE>
+__global__ __launch_bounds__(64 * NW) void k_apply_direct(const T *__restrict__ src,
+                                                          T *__restrict__ dst, long K4,
+                                                          const int *__restrict__ perm,
+                                                          const int *__restrict__ blk_m,
+                                                          int max_m,
+                                                          const double *__restrict__ Tm) {
+    constexpr int LT = NT * 16, PT = apply_t_pitch<T>(NT);
+    typedef std::conditional_t<sizeof(T) == 8, mfma_d4, mfma_f4> Acc;
+    extern __shared__ double lds_raw[];
+    T *const Tl = reinterpret_cast<T *>(lds_raw);   // T[k][l], k, l < LT (zero past m)
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    const int m = blk_m[b];
+    const double *Tb = Tm + (long)b * max_m * max_m;
+    const int *pb = perm + (long)b * max_m;
+    for (int idx = tid; idx < LT * LT; idx += 64 * NW) {
+        const int k = idx / LT, l = idx % LT;
+        const bool in = k < m && l < m;
+        const double v = Tb[in ? (long)k * max_m + l : 0];
+        Tl[k * PT + l] = in ? (T)v : T(0);
+    }
+    const int kr = lane / 16, col = lane % 16;
+    const long e0 = ((long)blockIdx.x * NW + wv) * 16 * NE + col;
+    bool ein[NE];
+    long es[NE];
+#pragma unroll
+    for (int E = 0; E < NE; ++E) {
+        ein[E] = e0 + 16 * E < K4;
+        es[E] = ein[E] ? e0 + 16 * E : 0;
+    }
+    const int nchunk = (m + 15) / 16;
+    // raw[kk][E]: state 16 c + 4 kk + kr at K value es[E] (state 0 past m,
+    // zeroed when widened; T's row is zero there anyway)
+    T raw[4][NE], bv[4][NE];
+    auto gload = [&](int c) {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int k = 16 * c + 4 * kk + kr;
+            const T *row = src + (long)pb[k < m ? k : 0] * K4;
+#pragma unroll
+            for (int E = 0; E < NE; ++E) raw[kk][E] = row[es[E]];
+        }
+    };
+    auto take = [&](int c) {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const bool kin = 16 * c + 4 * kk + kr < m;
+#pragma unroll
+            for (int E = 0; E < NE; ++E) bv[kk][E] = (kin && ein[E]) ? raw[kk][E] : T(0);
+        }
+    };
+    Acc acc[NT][NE];
+#pragma unroll
+    for (int L = 0; L < NT; ++L)
+#pragma unroll
+        for (int E = 0; E < NE; ++E) acc[L][E] = Acc{0, 0, 0, 0};
+    gload(0);
+    __syncthreads();   // T staged
+    take(0);
+    for (int c = 0; c < nchunk; ++c) {
+        if (c + 1 < nchunk) gload(c + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        const T *Tr = Tl + (16 * c + kr) * PT + col;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+            for (int L = 0; L < NT; ++L) {
+                if (!DENSE && L < c) continue;
+                const T av = Tr[4 * kk * PT + 16 * L];
+#pragma unroll
+                for (int E = 0; E < NE; ++E) acc[L][E] = mma_apply(av, bv[kk][E], acc[L][E]);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        take(c + 1);   // after the last chunk: unused
+    }
+#pragma unroll
+    for (int L = 0; L < NT; ++L)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int l = sizeof(T) == 8 ? 16 * L + kr + 4 * r : 16 * L + 4 * kr + r;
+            if (l < m) {
+                T *out = dst + (long)pb[l] * K4;
+#pragma unroll
+                for (int E = 0; E < NE; ++E)
+                    if (ein[E]) out[e0 + 16 * E] = (T)acc[L][E][r];
+            }
+        }
+}
+
 template <typename T, bool DENSE>
 static void apply_mfma_launch(const T *src, T *dst, long K4, const int *perm, const int *blk_m,
                               int nblk, int max_m, int mpad_max, const double *Tm,
@@ -1922,6 +2027,25 @@ static void apply_mfma_launch(const T *src, T *dst, long K4, const int *perm, co
                            dst, K4, perm, blk_m, max_m, Tm);                                \
         break;                                                                              \
     }
+    // up to 8 row tiles: T resident in LDS (k_apply_direct)
+#define MOCCA_APPLY_DIRECT(K, NE, W)                                                        \
+    case K: {                                                                               \
+        constexpr size_t shm = apply_direct_lds<T, K>();                                    \
+        const void *fn = (const void *)k_apply_direct<T, K, NE, W, DENSE>;                  \
+        static const bool lds_set = require_lds(fn, shm, "k_apply_direct");                 \
+        if (!lds_set) require_lds(fn, shm, "k_apply_direct");                               \
+        const dim3 g((K4 + 16 * NE * W - 1) / (16 * NE * W), nblk);                         \
+        hipLaunchKernelGGL((k_apply_direct<T, K, NE, W, DENSE>), g, dim3(64 * W), shm, st,   \
+                           src, dst, K4, perm, blk_m, max_m, Tm);                           \
+        return;                                                                             \
+    }
+    switch (mpad_max / 16) {
+        MOCCA_APPLY_DIRECT(1, 2, 8) MOCCA_APPLY_DIRECT(2, 2, 8) MOCCA_APPLY_DIRECT(3, 2, 8)
+        MOCCA_APPLY_DIRECT(4, 2, 8) MOCCA_APPLY_DIRECT(5, 2, 8) MOCCA_APPLY_DIRECT(6, 2, 8)
+        MOCCA_APPLY_DIRECT(7, 2, 8) MOCCA_APPLY_DIRECT(8, 2, 8)
+        default: break;
+    }
+#undef MOCCA_APPLY_DIRECT
     // eight waves from 5 row tiles on; 11-12 row tiles do not fit two waves
     // per SIMD, and keep four
     switch (mpad_max / 16) {
