@@ -2039,10 +2039,12 @@ static void apply_mfma_launch(const T *src, T *dst, long K4, const int *perm, co
                            src, dst, K4, perm, blk_m, max_m, Tm);                           \
         return;                                                                             \
     }
+    constexpr int DNE = sizeof(T) == 8 ? 1 : 2, DNW = sizeof(T) == 8 ? 16 : 8;
     switch (mpad_max / 16) {
-        MOCCA_APPLY_DIRECT(1, 2, 8) MOCCA_APPLY_DIRECT(2, 2, 8) MOCCA_APPLY_DIRECT(3, 2, 8)
-        MOCCA_APPLY_DIRECT(4, 2, 8) MOCCA_APPLY_DIRECT(5, 2, 8) MOCCA_APPLY_DIRECT(6, 2, 8)
-        MOCCA_APPLY_DIRECT(7, 2, 8) MOCCA_APPLY_DIRECT(8, 2, 8)
+        MOCCA_APPLY_DIRECT(1, DNE, DNW) MOCCA_APPLY_DIRECT(2, DNE, DNW)
+        MOCCA_APPLY_DIRECT(3, DNE, DNW) MOCCA_APPLY_DIRECT(4, DNE, DNW)
+        MOCCA_APPLY_DIRECT(5, DNE, DNW) MOCCA_APPLY_DIRECT(6, DNE, DNW)
+        MOCCA_APPLY_DIRECT(7, DNE, DNW) MOCCA_APPLY_DIRECT(8, DNE, DNW)
         default: break;
     }
 #undef MOCCA_APPLY_DIRECT
