@@ -850,6 +850,8 @@ __global__ __launch_bounds__(256) void k_hl_field_sum(HfbK k, const T *__restric
     for (int b = 0; b < k.nblk; ++b) {
         const HfbBlockK &bk = k.blk[b];
         if (bk.it != it) continue;
+        // (unrolled: several states' loads in flight per thread)
+#pragma unroll 8
         for (int a = 0; a < bk.h; ++a) {
             const long s = k.state[bk.first + a];
             double x[4], v[4];
