@@ -1588,8 +1588,9 @@ int hfb_large_setup(mocca_solve *h) {
         (rc = dalloc(h, &p.lhb, (size_t)k.nblk * sizeof(int))) ||
         (rc = dalloc(h, &p.llist, (size_t)k.nblk * hmax * sizeof(int))) ||
         (rc = dalloc(h, &p.lTap, (size_t)k.nblk * hmax * hmax * sizeof(double))) ||
-        (rc = dalloc(h, &p.lgpart, (size_t)k.nblk * p.lsplit * ntile * ntile * 256 *
-                                       sizeof(double))) ||
+        (rc = dalloc(h, &p.lgpart, sizeof(double) *
+                                       std::max((size_t)k.nblk * p.lsplit * ntile * ntile * 256,
+                                                (size_t)k.nblk * 8 * h->N))) ||
         (rc = dalloc(h, &p.lsig, (size_t)k.nblk * kSignParts * sizeof(double))))
         return rc;
     std::vector<int> nb(2 * k.nblk), hb(k.nblk);
@@ -1834,16 +1835,21 @@ int hfb_large_solve_impl(mocca_solve *h) {
     // SolvePairing :800-806: old rho, kappa take the current ones while the
     // old rho is all zero (the first call)
     if ((rc = run(h, "hfb_mix", [&] { hfb_launch_mix(h, 1); }))) return rc;
-    // cutoffs, density factor, pairing field (phi in the canonical states'
-    // buffer, which this step rewrites at its end)
+    // cutoffs, density factor, pairing field
     rc = run(h, "hfb_field", [&] {
         hfb_launch_cutoffs(h);
         hipLaunchKernelGGL(k_hl_field_w, dim3(nblk), dim3(256), 0, h->st, p.k, p.kappa, p.pc,
                            hmax, p.lTap, p.llist, p.lhb);
-        launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist, p.lhb,
-                                   nblk, hmax, mpad, p.lTap, h->st);
-        hipLaunchKernelGGL(k_hl_field_sum<T>, dim3((N + 255) / 256, 2), dim3(256), 0, h->st,
-                           p.k, (const T *)h->b.psi, (const T *)p.can_psi, N, p.dfac, p.field);
+        if (!launch_field_apply<T>(p.k, (const T *)h->b.psi, N, p.llist, p.lhb, hmax, mpad,
+                                   p.lTap, p.dfac, p.lgpart, p.field, h->st)) {
+            // (blocks above 128 states: phi through the canonical states'
+            // buffer, which this step rewrites at its end)
+            launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist,
+                                       p.lhb, nblk, hmax, mpad, p.lTap, h->st);
+            hipLaunchKernelGGL(k_hl_field_sum<T>, dim3((N + 255) / 256, 2), dim3(256), 0,
+                               h->st, p.k, (const T *)h->b.psi, (const T *)p.can_psi, N, p.dfac,
+                               p.field);
+        }
     });
     if (rc) return rc;
     // gaps

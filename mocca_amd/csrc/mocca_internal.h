@@ -191,7 +191,7 @@ struct Hfb {
     int *lhb = nullptr;            // [nblk] h (apply block sizes)
     int *llist = nullptr;          // [nblk][hmax] the block's states (apply lists)
     double *lTap = nullptr;        // [nblk][hmax][hmax] apply matrices
-    double *lgpart = nullptr;      // pair-Gram partial sums
+    double *lgpart = nullptr;      // pair-Gram partial sums; pairing-field partials before
 };
 
 }  // namespace detail

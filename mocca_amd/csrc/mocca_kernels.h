@@ -375,6 +375,16 @@ struct HfbK {
     const int *slot_index;   // [M] position of each state in its block
 };
 
+// Pairing field of the large-block HFB step without phi in HBM: the dense
+// apply phi_l = sum_k psi_list[k] Tm[k][l] of every block contracted with
+// the states in the same kernel (per-block partials in fpart, nblk x 2 x 4N
+// doubles), then field[2][2][N] = dfac x their sums over the blocks of each
+// isospin.  False (nothing launched) for blocks above 128 states.
+template <typename T>
+bool launch_field_apply(const HfbK &k, const T *psi, long N, const int *list,
+                        const int *blk_m, int max_m, int mpad_max, const double *Tm,
+                        const double *dfac, double *fpart, double *field, hipStream_t st);
+
 // IterativeEstimation (power iteration for dt and momentum).
 constexpr int kEstSlots = 8;
 template <typename T>

@@ -1921,13 +1921,20 @@ template <typename T, int NT>
 constexpr size_t apply_direct_lds() {
     return sizeof(T) * (size_t)(NT * 16) * apply_t_pitch<T>(NT);
 }
-template <typename T, int NT, int NE, int NW, bool DENSE>
+// FIELD (the pairing field of the HFB step, src = the states, K4 = 4N):
+// instead of storing phi_l = (Q)_l, the wave contracts it with the states
+// at its K values e = c N + p: fpart[b][0][e] = sum_l phi_l[e] psi_l[e],
+// fpart[b][1][e] = s_c sum_l phi_l[e] psi_l[(c^1) N + p] (s_c = -1 for odd
+// c), summed over the lanes of the four output classes; phi never reaches
+// HBM.
+template <typename T, int NT, int NE, int NW, bool DENSE, bool FIELD = false>
 __global__ __launch_bounds__(64 * NW) void k_apply_direct(const T *__restrict__ src,
                                                           T *__restrict__ dst, long K4,
                                                           const int *__restrict__ perm,
                                                           const int *__restrict__ blk_m,
                                                           int max_m,
-                                                          const double *__restrict__ Tm) {
+                                                          const double *__restrict__ Tm,
+                                                          double *__restrict__ fpart) {
     constexpr int LT = NT * 16, PT = apply_t_pitch<T>(NT);
     typedef std::conditional_t<sizeof(T) == 8, mfma_d4, mfma_f4> Acc;
     extern __shared__ double lds_raw[];
@@ -1997,6 +2004,44 @@ __global__ __launch_bounds__(64 * NW) void k_apply_direct(const T *__restrict__ 
         __builtin_amdgcn_sched_barrier(0);
         take(c + 1);   // after the last chunk: unused
     }
+    if constexpr (FIELD) {
+        const long N = K4 / 4;
+        double re[NE], im[NE];
+        long ex[NE];
+#pragma unroll
+        for (int E = 0; E < NE; ++E) {
+            re[E] = im[E] = 0.0;
+            const long e = es[E], cc = e / N;
+            ex[E] = (cc ^ 1) * N + (e - cc * N);
+        }
+#pragma unroll
+        for (int L = 0; L < NT; ++L)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int l = sizeof(T) == 8 ? 16 * L + kr + 4 * r : 16 * L + 4 * kr + r;
+                // (past m: phi is 0, T's columns being 0 there; state 0 read)
+                const T *row = src + (long)pb[l < m ? l : 0] * K4;
+#pragma unroll
+                for (int E = 0; E < NE; ++E) {
+                    const double ph = (double)acc[L][E][r];
+                    re[E] += ph * (double)row[es[E]];
+                    im[E] += ph * (double)row[ex[E]];
+                }
+            }
+#pragma unroll
+        for (int E = 0; E < NE; ++E) {
+            re[E] += __shfl_xor(re[E], 16, 64);
+            im[E] += __shfl_xor(im[E], 16, 64);
+            re[E] += __shfl_xor(re[E], 32, 64);
+            im[E] += __shfl_xor(im[E], 32, 64);
+            if (kr == 0 && ein[E]) {
+                const long e = es[E];
+                fpart[(2L * b) * K4 + e] = re[E];
+                fpart[(2L * b + 1) * K4 + e] = ((e / N) & 1) ? -im[E] : im[E];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int L = 0; L < NT; ++L)
 #pragma unroll
@@ -2009,6 +2054,28 @@ __global__ __launch_bounds__(64 * NW) void k_apply_direct(const T *__restrict__ 
                     if (ein[E]) out[e0 + 16 * E] = (T)acc[L][E][r];
             }
         }
+}
+
+// Pairing field from the partials of k_apply_direct<FIELD>: field[2 it][p] =
+// dfac sum_{blocks of it} sum_c fpart[b][0][c N + p], field[2 it + 1][p] =
+// -dfac sum_b sum_c fpart[b][1][c N + p] (block order, then components).
+__global__ void k_field_from_parts(HfbK k, const double *__restrict__ fpart, long N,
+                                   const double *__restrict__ dfac,
+                                   double *__restrict__ field) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int it = blockIdx.y;
+    if (p >= N) return;
+    double fr = 0.0, fi = 0.0;
+    for (int b = 0; b < k.nblk; ++b) {
+        if (k.blk[b].it != it) continue;
+        const double *f = fpart + 2 * b * 4 * N;
+        fr += f[p] + f[N + p] + f[2 * N + p] + f[3 * N + p];
+        f += 4 * N;
+        fi += f[p] + f[N + p] + f[2 * N + p] + f[3 * N + p];
+    }
+    const double d = dfac[it * N + p];
+    field[(2 * it) * N + p] = fr * d;
+    field[(2 * it + 1) * N + p] = -fi * d;
 }
 
 template <typename T, bool DENSE>
@@ -2036,7 +2103,7 @@ static void apply_mfma_launch(const T *src, T *dst, long K4, const int *perm, co
         if (!lds_set) require_lds(fn, shm, "k_apply_direct");                               \
         const dim3 g((K4 + 16 * NE * W - 1) / (16 * NE * W), nblk);                         \
         hipLaunchKernelGGL((k_apply_direct<T, K, NE, W, DENSE>), g, dim3(64 * W), shm, st,   \
-                           src, dst, K4, perm, blk_m, max_m, Tm);                           \
+                           src, dst, K4, perm, blk_m, max_m, Tm, nullptr);                  \
         return;                                                                             \
     }
     constexpr int DNE = sizeof(T) == 8 ? 1 : 2, DNW = sizeof(T) == 8 ? 16 : 8;
@@ -2064,6 +2131,35 @@ void launch_gs_apply_mfma(const T *src, T *dst, long K4, const int *perm, const 
                           int nblk, int max_m, int mpad_max, const double *Tm,
                           hipStream_t st) {
     apply_mfma_launch<T, false>(src, dst, K4, perm, blk_m, nblk, max_m, mpad_max, Tm, st);
+}
+
+template <typename T>
+bool launch_field_apply(const HfbK &k, const T *psi, long N, const int *list,
+                        const int *blk_m, int max_m, int mpad_max, const double *Tm,
+                        const double *dfac, double *fpart, double *field, hipStream_t st) {
+    const long K4 = 4 * N;
+    const int nblk = k.nblk;
+    constexpr int NE = sizeof(T) == 8 ? 1 : 2, W = sizeof(T) == 8 ? 16 : 8;
+#define MOCCA_FIELD_APPLY(K)                                                                \
+    case K: {                                                                               \
+        constexpr size_t shm = apply_direct_lds<T, K>();                                    \
+        const void *fn = (const void *)k_apply_direct<T, K, NE, W, true, true>;             \
+        static const bool lds_set = require_lds(fn, shm, "k_apply_direct");                 \
+        if (!lds_set) require_lds(fn, shm, "k_apply_direct");                               \
+        const dim3 g((K4 + 16 * NE * W - 1) / (16 * NE * W), nblk);                         \
+        hipLaunchKernelGGL((k_apply_direct<T, K, NE, W, true, true>), g, dim3(64 * W), shm,  \
+                           st, psi, nullptr, K4, list, blk_m, max_m, Tm, fpart);            \
+        break;                                                                              \
+    }
+    switch (mpad_max / 16) {
+        MOCCA_FIELD_APPLY(1) MOCCA_FIELD_APPLY(2) MOCCA_FIELD_APPLY(3) MOCCA_FIELD_APPLY(4)
+        MOCCA_FIELD_APPLY(5) MOCCA_FIELD_APPLY(6) MOCCA_FIELD_APPLY(7) MOCCA_FIELD_APPLY(8)
+        default: return false;
+    }
+#undef MOCCA_FIELD_APPLY
+    hipLaunchKernelGGL(k_field_from_parts, dim3((N + 255) / 256, 2), dim3(256), 0, st, k,
+                       fpart, N, dfac, field);
+    return true;
 }
 
 template <typename T>
@@ -2160,6 +2256,9 @@ void launch_gram_mfma(const T *psi, long K4, const int *blk_states, const int *b
     template void launch_dense_apply_mfma<T>(const T *, T *, long, const int *,           \
                                              const int *, int, int, int, const double *,  \
                                              hipStream_t);                                \
+    template bool launch_field_apply<T>(const HfbK &, const T *, long, const int *,       \
+                                        const int *, int, int, const double *,           \
+                                        const double *, double *, double *, hipStream_t); \
     template void launch_gs_apply<T>(const T *, T *, T *, long, const int *,             \
                                      const int *, int, int, const double *, int,         \
                                      hipStream_t);                                       \
