@@ -67,7 +67,7 @@ STAGE_KERNELS = {
                    "LDS bandwidth and barrier latency of one workgroup per 100 x 100 "
                    "density block (4 blocks: 4 of 256 CUs)"),
     "hfb_gaps": ("mocca::k_hl_pair_gram<double, 7, 4, true>", "FP64 matrix cores"),
-    "hfb_field": ("mocca::k_gs_apply_mfma<double, 7, 8, true>", "FP64 matrix cores"),
+    "hfb_field": ("mocca::k_apply_direct<double, 7, 1, 16, true, true>", "FP64 matrix cores"),
     "hpsi_step": ("mocca::k_hpsi_fd_dma<double, true, true, false>",
                   "on-chip issue (VALU, LDS, one barrier per plane)"),
 }
