@@ -802,6 +802,7 @@ __global__ void k_hl_gaps_reduce(HfbK k, const double *__restrict__ part, int ns
     // lane L = 16 (ri % 4) + cj, register r = ri / 4
     const int idx = (16 * (ri % 4) + cj) * 4 + ri / 4;
     double s = 0;
+#pragma unroll 8
     for (int w = 0; w < nsplit; ++w)
         s += part[(((long)b * nsplit + w) * nt * nt + I * nt + J) * 256 + idx];
     const double d = dv * pc[s1] * pc[s2] * s;
@@ -1188,21 +1189,19 @@ __device__ __forceinline__ double hl_wave_max(double a) {
 }
 
 // Off-diagonal absolute row sums of T (the gaps: independent of lambda),
-// once per Fermi solve: rs[b][i] = sum_j |T_ij|, j != i.
+// once per Fermi solve: rs[b][i] = sum_j |T_ij|, j != i (one wave per row).
 __global__ __launch_bounds__(256) void k_hs_rowsum(HfbK k, const double *__restrict__ delta,
                                                    int ld, double *__restrict__ rs) {
-    const int b = blockIdx.x;
+    const int b = blockIdx.y, lane = threadIdx.x % 64;
+    const int i = blockIdx.x * 4 + threadIdx.x / 64;
     const HfbBlockK &bk = k.blk[b];
     const int h = bk.h, n = 2 * h;
+    if (i >= n) return;
     const double *dl = delta + bk.off;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double r = 0.0;
-        if (i < h)
-            for (int j = 0; j < h; ++j) r += fabs(dl[i * n + h + j]);
-        else
-            for (int j = 0; j < h; ++j) r += fabs(dl[j * n + i]);
-        rs[(long)b * ld + i] = r;
-    }
+    double r = 0.0;
+    for (int j = lane; j < h; j += 64) r += fabs(i < h ? dl[i * n + h + j] : dl[j * n + i]);
+    r = hl_wave_sum(r);
+    if (lane == 0) rs[(long)b * ld + i] = r;
 }
 
 // X = T(lambda) / s for the active blocks (lambda of the evaluation from the
@@ -1778,7 +1777,8 @@ int sign_fermi_solve(mocca_solve *h) {
     int rc = run(h, "hfb_fermi", [&] {
         hipLaunchKernelGGL(k_hl_newton_init, dim3(1), dim3(64), 0, h->st, p.fermi, p.lnst,
                            nblk, p.lactive);
-        hipLaunchKernelGGL(k_hs_rowsum, dim3(nblk), dim3(256), 0, h->st, k, p.delta, ld, rs);
+        hipLaunchKernelGGL(k_hs_rowsum, dim3((ld + 3) / 4, nblk), dim3(256), 0, h->st, k,
+                           p.delta, ld, rs);
     });
     if (rc) return rc;
     auto number = [&]() {

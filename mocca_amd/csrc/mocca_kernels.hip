@@ -1769,7 +1769,8 @@ __global__ __launch_bounds__(64) void k_gram_mfma(const T *__restrict__ psi, lon
         }
 }
 
-// Q = Psi_sorted T on the FP64 matrix cores, for blocks of up to 192 states,
+// Q = Psi_sorted T on the FP64 matrix cores, for blocks of 129-192 states
+// (k_apply_direct below takes the smaller ones, whose T fits LDS whole),
 // as a GEMM [K4 x m] = [K4 x m] [m x m] tiled for LDS: a workgroup of NW
 // waves owns 32 NW consecutive values of K and all m outputs; the sorted
 // states and T pass through LDS in chunks of 16 states, double-buffered
@@ -2115,11 +2116,9 @@ static void apply_mfma_launch(const T *src, T *dst, long K4, const int *perm, co
         default: break;
     }
 #undef MOCCA_APPLY_DIRECT
-    // eight waves from 5 row tiles on; 11-12 row tiles do not fit two waves
-    // per SIMD, and keep four
+    // 9-12 row tiles: T staged in LDS chunk by chunk; 11-12 row tiles do
+    // not fit two waves per SIMD, and keep four
     switch (mpad_max / 16) {
-        MOCCA_APPLY_NT(1, 4) MOCCA_APPLY_NT(2, 4) MOCCA_APPLY_NT(3, 4) MOCCA_APPLY_NT(4, 4)
-        MOCCA_APPLY_NT(5, 8) MOCCA_APPLY_NT(6, 8) MOCCA_APPLY_NT(7, 8) MOCCA_APPLY_NT(8, 8)
         MOCCA_APPLY_NT(9, 8) MOCCA_APPLY_NT(10, 8) MOCCA_APPLY_NT(11, 4) MOCCA_APPLY_NT(12, 4)
         default: return;
     }
