@@ -9,8 +9,9 @@
 //                  of the states by the pairing field: FP64 matrix cores;
 //   pairing field  Delta_q(r) = -g(r) sum_a P(phi_a, psi_a),
 //                  phi_a = sum_b c_a c_b kappa(h+a, b) psi_b (HFBPairingField,
-//                  HFB.f90:350-518): the phi by the dense matrix-core apply,
-//                  then one pass over the points;
+//                  HFB.f90:350-518): the phi by the dense matrix-core apply
+//                  with T in LDS, contracted with the states in its epilogue
+//                  (launch_field_apply: phi never in HBM);
 //   Fermi energies N(lambda) = particle number (HFBNumberofParticles,
 //                  HFB.f90:810-1073) solved by the reference's Broyden
 //                  iteration (HFBFindFermiEnergyBroyden :1849-2161: the same
@@ -835,39 +836,6 @@ __global__ void k_hl_field_w(HfbK k, const double *__restrict__ kappa,
     for (int i = threadIdx.x; i < hmax; i += blockDim.x)
         list[b * hmax + i] = k.state[bk.first + (i < h ? i : 0)];
     if (threadIdx.x == 0) hlist[b] = h;
-}
-
-// Pairing field of each isospin from phi (the apply output, in the states'
-// slots) and the states: field = -dfac sum_a P(phi_a, psi_a).
-template <typename T>
-__global__ __launch_bounds__(256) void k_hl_field_sum(HfbK k, const T *__restrict__ psi,
-                                                      const T *__restrict__ phi, long N,
-                                                      const double *__restrict__ dfac,
-                                                      double *__restrict__ field) {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int it = blockIdx.y;
-    if (p >= N) return;
-    double fr = 0, fi = 0;
-    for (int b = 0; b < k.nblk; ++b) {
-        const HfbBlockK &bk = k.blk[b];
-        if (bk.it != it) continue;
-        // (unrolled: several states' loads in flight per thread)
-#pragma unroll 8
-        for (int a = 0; a < bk.h; ++a) {
-            const long s = k.state[bk.first + a];
-            double x[4], v[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                x[c] = (double)phi[(s * 4 + c) * N + p];
-                v[c] = (double)psi[(s * 4 + c) * N + p];
-            }
-            fr -= -x[0] * v[0] - x[1] * v[1] - x[2] * v[2] - x[3] * v[3];
-            fi -= x[0] * v[1] - x[1] * v[0] + x[2] * v[3] - x[3] * v[2];
-        }
-    }
-    const double d = dfac[it * N + p];
-    field[(2 * it) * N + p] = fr * d;
-    field[(2 * it + 1) * N + p] = fi * d;
 }
 
 // T(lambda) of every block (dense, leading dimension ld): the HFB matrix of
@@ -1840,16 +1808,9 @@ int hfb_large_solve_impl(mocca_solve *h) {
         hfb_launch_cutoffs(h);
         hipLaunchKernelGGL(k_hl_field_w, dim3(nblk), dim3(256), 0, h->st, p.k, p.kappa, p.pc,
                            hmax, p.lTap, p.llist, p.lhb);
-        if (!launch_field_apply<T>(p.k, (const T *)h->b.psi, N, p.llist, p.lhb, hmax, mpad,
-                                   p.lTap, p.dfac, p.lgpart, p.field, h->st)) {
-            // (blocks above 128 states: phi through the canonical states'
-            // buffer, which this step rewrites at its end)
-            launch_dense_apply_mfma<T>((const T *)h->b.psi, (T *)p.can_psi, 4 * N, p.llist,
-                                       p.lhb, nblk, hmax, mpad, p.lTap, h->st);
-            hipLaunchKernelGGL(k_hl_field_sum<T>, dim3((N + 255) / 256, 2), dim3(256), 0,
-                               h->st, p.k, (const T *)h->b.psi, (const T *)p.can_psi, N, p.dfac,
-                               p.field);
-        }
+        // (this path's blocks have at most 128 states: hfb_large_fits)
+        launch_field_apply<T>(p.k, (const T *)h->b.psi, N, p.llist, p.lhb, hmax, mpad, p.lTap,
+                              p.dfac, p.lgpart, p.field, h->st);
     });
     if (rc) return rc;
     // gaps
